@@ -1,0 +1,317 @@
+"""Pin the oracle's pointwise functions and interface fluxes.
+
+Each check compares the oracle with something it does not share code or
+formula arrangement with: published known answers (Toro's five tests, the
+reference's own `test` kernel input), closed forms, independent textbook
+solvers (tests/textbook.py) and exact invariances.
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import textbook as TB
+
+NORMALS = [(1.0, 0.0), (0.0, -1.0), (-1.0, 0.0), (0.0, 1.0)]
+
+
+def rel(a, b, floor=1e-300):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), floor))
+
+
+# ------------------------------------------------------------ minmod family
+
+def test_minmod_is_median_of_zero_and_args():
+    vals = [0.0, -0.0, 1e-300, -1e-300, 0.5, -0.5, 2.0, -2.0, 3.0, -7.25, 1e300, -1e300]
+    for x in vals:
+        for y in vals:
+            med = sorted([0.0, x, y])[1]
+            assert O.minmod(x, y) == med, (x, y)
+
+
+def test_linear2_closed_form_on_power_of_two_grid():
+    # On a 2^-k grid the absolute-coordinate arithmetic of linear2 is exact,
+    # so the face values equal t1 -/+ minmod(t1 - t0, t2 - t1) / 2.
+    rng = random.Random(3)
+    dx = 1.0 / 4096
+    for _ in range(2000):
+        n = rng.randrange(4096)
+        x = dx / 2 + n * dx
+        t0, t1, t2 = (rng.uniform(-2, 2) for _ in range(3))
+        A, B = O.linear2(x - dx, t0, x, t1, x + dx, t2, x - dx / 2, x + dx / 2)
+        d = sorted([0.0, t1 - t0, t2 - t1])[1]
+        assert A == t1 - 0.5 * d and B == t1 + 0.5 * d
+
+
+def test_linear_matches_neighbour_side_of_linear2():
+    # linear(...) of the east neighbour seen from cell n equals that
+    # neighbour's own west-face value from linear2 (exactly on a 2^-k grid).
+    rng = random.Random(4)
+    dx = 1.0 / 1024
+    for _ in range(2000):
+        n = rng.randrange(1, 1000)
+        x = dx / 2 + n * dx
+        t1, t2, t3 = (rng.uniform(0.1, 2) for _ in range(3))
+        v = O.linear(x, t1, x + dx, t2, x + 2 * dx, t3, x + dx / 2)
+        A, _ = O.linear2(x, t1, x + dx, t2, x + 2 * dx, t3, x + dx / 2, x + 3 * dx / 2)
+        assert v == A
+
+
+# ------------------------------------------------------------ exact Riemann
+
+TORO = [  # Toro table 4.2/4.3 (gamma = 1.4): L, R = (rho, p, u); p*, u*
+    ((1.0, 1.0, 0.0), (0.125, 0.1, 0.0), 0.30313, 0.92745),
+    ((1.0, 0.4, -2.0), (1.0, 0.4, 2.0), 0.00189, 0.0),
+    ((1.0, 1000.0, 0.0), (1.0, 0.01, 0.0), 460.894, 19.5975),
+    ((1.0, 0.01, 0.0), (1.0, 100.0, 0.0), 46.0950, -6.19633),
+    ((5.99924, 460.894, 19.5975), (5.99242, 46.0950, -6.19633), 1691.64, 8.68975),
+]
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_exact_solver_toro_known_answers(case):
+    L, R, ps, us = TORO[case]
+    for n in NORMALS:
+        # put the 1-D data along the normal n; the tangential velocity is 0
+        Lp = (L[0], L[1], L[2] * n[0], L[2] * n[1])
+        Rp = (R[0], R[1], R[2] * n[0], R[2] * n[1])
+        p, u, _ = O.godunov_star(1.4, Lp, Rp, n)
+        assert abs(p - ps) <= 2e-5 * ps + 1e-5
+        assert abs(u - us) <= 2e-5 * max(abs(us), 1.0)
+
+
+def _flux_vs_textbook(gamma, L, R, n, tol):
+    """L, R = (rho, p, u_n, u_t) in the face frame of n; compare lab fluxes."""
+    t = (-n[1], n[0])
+    lab = lambda q: (q[0], q[1], q[2] * n[0] + q[3] * t[0], q[2] * n[1] + q[3] * t[1])
+    F, _ = O.euler_flux("exact", gamma, 0.25, lab(L), lab(R), n)
+    r, p, u = TB.toro_sample0(gamma, (L[0], L[1], L[2]), (R[0], R[1], R[2]))
+    ut = L[3] if u >= 0 else R[3]
+    Fn = TB.euler_phys_flux(gamma, r, p, u, ut)  # (rho, n-mom, t-mom, E)
+    exp = (Fn[0], Fn[3], Fn[1] * n[0] + Fn[2] * t[0], Fn[1] * n[1] + Fn[2] * t[1])
+    scale = max(abs(v) for v in exp) + 1e-12
+    assert max(abs(a - b) for a, b in zip(F, exp)) <= tol * scale, (L, R, n, F, exp)
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_exact_flux_matches_toro_sampling(case):
+    L, R, _, _ = TORO[case]
+    for n in NORMALS:
+        _flux_vs_textbook(1.4, (L[0], L[1], L[2], 0.3), (R[0], R[1], R[2], -0.7), n, 2e-8)
+
+
+def test_exact_flux_random_states_match_toro():
+    rng = random.Random(11)
+    for gamma in (1.4, 5.0 / 3.0):
+        for _ in range(400):
+            L = (rng.uniform(0.1, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(-1, 1))
+            R = (rng.uniform(0.1, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(-1, 1))
+            # no vacuum for these magnitudes
+            n = NORMALS[rng.randrange(4)]
+            _flux_vs_textbook(gamma, L, R, n, 2e-8)
+
+
+def test_exact_reference_test_kernel_fixture():
+    # kernel.cu:3668-3680: uniform rho=1, p=0.0666666, u=(-1,0) on both sides
+    # (gamma = 5/3 as in Header.h). No fan: the flux is the physical flux.
+    s, u = (1.0, 0.0666666, -1.0, 0.0), (1.0, 0.0666666, -1.0, 0.0)
+    F, _ = O.euler_flux("exact", 5.0 / 3.0, 0.25, s, u, (1.0, 0.0))
+    assert F[0] == -1.0
+    F, _ = O.euler_flux("exact", 5.0 / 3.0, 0.25, s, u, (-1.0, 0.0))
+    assert F[0] == 1.0
+
+
+def test_exact_vacuum_generation_symmetric():
+    L = (1.0, 0.4, -6.0, 0.0)
+    R = (1.0, 0.4, 6.0, 0.0)
+    p, u, pat = O.godunov_star(1.4, L, R)
+    assert pat == 3
+    F, _ = O.euler_flux("exact", 1.4, 0.25, L, R, (1.0, 0.0))
+    assert F[0] == 0.0 and F[1] == 0.0 and F[3] == 0.0  # vacuum at x/t = 0
+    assert F[2] == 0.0
+
+
+# ------------------------------------------------------------ HLLC
+
+def _rand_state(rng):
+    return (rng.uniform(0.1, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(-1, 1))
+
+
+def test_hllc_matches_toro_hllc():
+    rng = random.Random(5)
+    for gamma in (1.4, 5.0 / 3.0):
+        for _ in range(3000):
+            L, R = _rand_state(rng), _rand_state(rng)
+            n = NORMALS[rng.randrange(4)]
+            t = (-n[1], n[0])
+            F, _ = O.euler_flux("hllc", gamma, 0.25, L, R, n)
+            un = lambda q: q[2] * n[0] + q[3] * n[1]
+            ut = lambda q: q[2] * t[0] + q[3] * t[1]
+            Fb = TB.toro_hllc(gamma, (L[0], L[1], un(L), ut(L)), (R[0], R[1], un(R), ut(R)))
+            exp = (Fb[0], Fb[3], Fb[1] * n[0] + Fb[2] * t[0], Fb[1] * n[1] + Fb[2] * t[1])
+            scale = max(abs(v) for v in exp) + 1e-12
+            assert max(abs(a - b) for a, b in zip(F, exp)) <= 1e-12 * scale
+
+
+def test_hllc_consistency_and_supersonic_upwinding():
+    rng = random.Random(6)
+    for _ in range(500):
+        q = _rand_state(rng)
+        for n in NORMALS:
+            F, _ = O.euler_flux("hllc", 1.4, 0.25, q, q, n)
+            un = q[2] * n[0] + q[3] * n[1]
+            E = q[1] / 0.4 + 0.5 * q[0] * (q[2] ** 2 + q[3] ** 2)
+            exp = (q[0] * un, un * (E + q[1]), q[0] * un * q[2] + q[1] * n[0], q[0] * un * q[3] + q[1] * n[1])
+            assert rel(F, exp) < 1e-13
+    # supersonic to the right: F = F(L)
+    L, R = (1.0, 1.0, 5.0, 0.2), (0.5, 0.3, 4.0, -0.1)
+    F, _ = O.euler_flux("hllc", 1.4, 0.25, L, R, (1.0, 0.0))
+    E = 1.0 / 0.4 + 0.5 * (25 + 0.04)
+    assert rel(F, (5.0, 5.0 * (E + 1.0), 25.0 + 1.0, 5.0 * 0.2)) < 1e-14
+
+
+def test_hllc_resolves_stationary_contact_exactly():
+    for n in NORMALS:
+        t = (-n[1], n[0])  # zero normal velocity, tangential shear across the contact
+        L = (1.0, 0.7, 0.3 * t[0], 0.3 * t[1])
+        R = (0.2, 0.7, -0.5 * t[0], -0.5 * t[1])
+        F, _ = O.euler_flux("hllc", 1.4, 0.25, L, R, n)
+        assert abs(F[0]) < 1e-15 and abs(F[1]) < 1e-15
+        assert abs(F[2] - 0.7 * n[0]) < 1e-15 and abs(F[3] - 0.7 * n[1]) < 1e-15
+
+
+def test_hllc_rotation_and_mirror_invariance():
+    rng = random.Random(7)
+    for _ in range(500):
+        L, R = _rand_state(rng), _rand_state(rng)
+        Fx, tx = O.euler_flux("hllc", 1.4, 0.25, L, R, (1.0, 0.0))
+        # rotate by +90 degrees: (u, v) -> (-v, u), normal (1,0) -> (0,1)
+        rot = lambda q: (q[0], q[1], -q[3], q[2])
+        Fy, ty = O.euler_flux("hllc", 1.4, 0.25, rot(L), rot(R), (0.0, 1.0))
+        assert rel((Fy[0], Fy[1], Fy[2], Fy[3]), (Fx[0], Fx[1], -Fx[3], Fx[2])) < 1e-14
+        assert tx == ty
+        # mirror: F(L, R, n) = -F(R, L, -n)
+        Fm, _ = O.euler_flux("hllc", 1.4, 0.25, R, L, (-1.0, 0.0))
+        assert rel(Fm, tuple(-v for v in Fx)) < 1e-12
+
+
+def test_hllc_face_timestep_closed_form():
+    L, R = (1.0, 1.0, 0.3, 0.0), (0.5, 0.4, -0.2, 0.0)
+    _, t = O.euler_flux("hllc", 1.4, 0.4, L, R, (1.0, 0.0), rad=0.01)
+    cL, cR = math.sqrt(1.4), math.sqrt(1.4 * 0.4 / 0.5)
+    S = max(abs(min(0.3 - cL, -0.2 - cR)), abs(max(0.3 + cL, -0.2 + cR)))
+    assert abs(t - 0.4 * 0.01 / S) < 1e-16
+
+
+# ------------------------------------------------------------ HLLD / HLLC-MHD
+
+SPI4 = math.sqrt(4.0 * math.pi)
+
+
+def _rand_mhd(rng, bn=None):
+    r, p = rng.uniform(0.2, 2), rng.uniform(0.2, 2)
+    v = [rng.uniform(-1, 1), rng.uniform(-1, 1)]
+    B = [rng.uniform(-2, 2) for _ in range(3)]
+    return r, p, v, B
+
+
+def _frame(n):
+    # face frame of Gas_Dinamic.cu:986-1015 for the four in-plane normals
+    return {(1.0, 0.0): ((0, 1, 0), (0, 0, 1)), (0.0, 1.0): ((0, 0, 1), (1, 0, 0)),
+            (-1.0, 0.0): ((0, 0, -1), (0, -1, 0)), (0.0, -1.0): ((-1, 0, 0), (0, 0, -1))}[n]
+
+
+def _hlld_vs_mk(rng, gamma, same_bn=True):
+    n2 = NORMALS[rng.randrange(4)]
+    n = (n2[0], n2[1], 0.0)
+    t, m = _frame(n2)
+    rL, pL, vL, BL = _rand_mhd(rng)
+    rR, pR, vR, BR = _rand_mhd(rng)
+    if same_bn:  # equal normal B on both sides (Gaussian)
+        if n2[0] != 0:
+            BR[0] = BL[0]
+        else:
+            BR[1] = BL[1]
+    L = (rL, pL, vL[0], vL[1], 0.0, *BL)
+    R = (rR, pR, vR[0], vR[1], 0.0, *BR)
+    P, _ = O.mhd_flux(gamma, 0.25, L, R, n, 3)
+    dot = lambda a, b: a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+    vl3, vr3 = (vL[0], vL[1], 0.0), (vR[0], vR[1], 0.0)
+    bl3, br3 = [b / SPI4 for b in BL], [b / SPI4 for b in BR]
+    ql = (rL, pL, dot(vl3, n), dot(vl3, t), dot(vl3, m), dot(bl3, n), dot(bl3, t), dot(bl3, m))
+    qr = (rR, pR, dot(vr3, n), dot(vr3, t), dot(vr3, m), dot(br3, n), dot(br3, t), dot(br3, m))
+    F, _ = TB.mk_hlld(gamma, ql, qr)
+    # back to lab frame; B fluxes back to Gaussian
+    mom = [F[1] * n[i] + F[2] * t[i] + F[3] * m[i] for i in range(3)]
+    bfl = [SPI4 * (0.0 * n[i] + F[4] * t[i] + F[5] * m[i]) for i in range(3)]
+    exp = [F[0], mom[0], mom[1], mom[2], bfl[0], bfl[1], bfl[2], F[6]]
+    scale = max(abs(v) for v in exp) + 1e-12
+    return float(np.max(np.abs(P - np.array(exp)))) / scale
+
+
+def test_hlld_matches_miyoshi_kusano_constant_bn():
+    rng = random.Random(8)
+    worst = 0.0
+    for gamma in (5.0 / 3.0, 1.4):
+        for _ in range(2000):
+            worst = max(worst, _hlld_vs_mk(rng, gamma))
+    assert worst < 1e-11, worst
+
+
+def test_hlld_consistency():
+    rng = random.Random(9)
+    for metod in (2, 3):
+        for _ in range(300):
+            r, p, v, B = _rand_mhd(rng)
+            q = (r, p, v[0], v[1], 0.0, *B)
+            for n2 in NORMALS:
+                n = (n2[0], n2[1], 0.0)
+                t, m = _frame(n2)
+                P, _ = O.mhd_flux(5 / 3, 0.25, q, q, n, metod)
+                dot = lambda a, b: a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+                v3, b3 = (v[0], v[1], 0.0), [x / SPI4 for x in B]
+                F = TB.mhd_phys_flux_hl(5 / 3, r, p, dot(v3, n), dot(v3, t), dot(v3, m),
+                                        dot(b3, n), dot(b3, t), dot(b3, m))
+                mom = [F[1] * n[i] + F[2] * t[i] + F[3] * m[i] for i in range(3)]
+                bfl = [SPI4 * (F[4] * t[i] + F[5] * m[i]) for i in range(3)]
+                exp = np.array([F[0], *mom, *bfl, F[6]])
+                assert float(np.max(np.abs(P - exp))) <= 1e-13 * (np.max(np.abs(exp)) + 1)
+
+
+def test_hlld_zero_field_matches_davis_hllc_and_metod2():
+    # B = 0: HLLD is HLLC with S_L = min(uL,uR) - max(cL,cR); HLLC-MHD (metod 2)
+    # agrees on mass, normal momentum and energy and uses the HLL-averaged
+    # tangential velocity (Gas_Dinamic.cu:1160-1163).
+    rng = random.Random(10)
+    for _ in range(1000):
+        L = (rng.uniform(0.1, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), 0.0, 0, 0, 0)
+        R = (rng.uniform(0.1, 2), rng.uniform(0.1, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), 0.0, 0, 0, 0)
+        P3, _ = O.mhd_flux(1.4, 0.25, L, R, (1.0, 0.0, 0.0), 3)
+        P2, _ = O.mhd_flux(1.4, 0.25, L, R, (1.0, 0.0, 0.0), 2)
+        F, _ = TB.mk_hlld(1.4, (L[0], L[1], L[2], L[3], 0.0, 0.0, 0.0, 0.0),
+                          (R[0], R[1], R[2], R[3], 0.0, 0.0, 0.0, 0.0))
+        exp = np.array([F[0], F[1], F[2], F[3], 0, 0, 0, F[6]])
+        assert float(np.max(np.abs(P3 - exp))) < 1e-12 * (np.max(np.abs(exp)) + 1)
+        for k in (0, 1, 7):
+            assert abs(P2[k] - P3[k]) < 1e-12 * (abs(P3[k]) + 1)
+        assert np.all(P2[4:7] == 0) and np.all(P3[4:7] == 0)
+
+
+def test_hlld_normal_field_dissipation():
+    # normal-B flux replaced by -max(|SL|,|SR|)(bn_R - bn_L)*sqrt(4 pi)
+    # (Gas_Dinamic.cu:1488-1490, 1495)
+    L = (1.0, 1.0, 0.1, 0.0, 0.0, 0.5, 1.0, 0.0)
+    R = (0.5, 0.5, 0.0, 0.1, 0.0, 0.9, -1.0, 0.2)
+    P, t = O.mhd_flux(5 / 3, 0.25, L, R, (1.0, 0.0, 0.0), 3, rad=1.0)
+    SN = 0.25 / t
+    assert abs(P[4] - (-SN * (0.9 - 0.5) / SPI4) * SPI4) < 1e-14
+
+
+def test_hlld_rejects_unknown_metod():
+    O.diag_reset()
+    P, t = O.mhd_flux(5 / 3, 0.25, (1, 1, 0, 0, 0, 0, 0, 0), (1, 1, 0, 0, 0, 0, 0, 0), (1.0, 0, 0), 1)
+    assert t == -1.0 and np.all(P == 0)
+    assert O.diag()["bad_branch"] == 1

@@ -1,0 +1,235 @@
+"""Independent textbook restatements used to pin the oracle.
+
+These follow published methods, not the reference source:
+  * exact Riemann solver: Toro, "Riemann Solvers and Numerical Methods for
+    Fluid Dynamics", 3rd ed., ch. 4 (pressure function, Newton iteration,
+    sampling);
+  * HLLC: Toro ch. 10.4, star state eq. (10.39), wave speeds from the Davis
+    estimate S_L = min(uL - cL, uR - cR), S_R = max(uL + cL, uR + cR);
+  * HLLD: Miyoshi & Kusano, J. Comput. Phys. 208 (2005) 315, eqs. (38)-(63),
+    Heaviside-Lorentz units, constant normal field.
+All in pure Python floats (small inputs only).
+"""
+from __future__ import annotations
+
+import math
+
+
+# ------------------------------------------------------------------ Euler
+
+def euler_phys_flux(gamma, rho, p, un, ut):
+    E = p / (gamma - 1.0) + 0.5 * rho * (un * un + ut * ut)
+    return (rho * un, rho * un * un + p, rho * un * ut, un * (E + p))
+
+
+def toro_exact(gamma, L, R, tol=1e-15):
+    """L, R = (rho, p, u) along the normal. Returns (p*, u*)."""
+    rL, pL, uL = L
+    rR, pR, uR = R
+    cL, cR = math.sqrt(gamma * pL / rL), math.sqrt(gamma * pR / rR)
+    AL, AR = 2.0 / ((gamma + 1) * rL), 2.0 / ((gamma + 1) * rR)
+    BL, BR = (gamma - 1) / (gamma + 1) * pL, (gamma - 1) / (gamma + 1) * pR
+
+    def f(p, pk, rk, ck, Ak, Bk):
+        if p > pk:
+            q = math.sqrt(Ak / (p + Bk))
+            return (p - pk) * q, q * (1 - (p - pk) / (2 * (Bk + p)))
+        pr = p / pk
+        return (2 * ck / (gamma - 1)) * (pr ** ((gamma - 1) / (2 * gamma)) - 1), \
+            1 / (rk * ck) * pr ** (-(gamma + 1) / (2 * gamma))
+
+    # two-rarefaction initial guess (Toro eq. 4.46)
+    z = (gamma - 1) / (2 * gamma)
+    p = ((cL + cR - 0.5 * (gamma - 1) * (uR - uL)) / (cL / pL ** z + cR / pR ** z)) ** (1 / z)
+    p = max(p, 1e-14)
+    for _ in range(200):
+        fl, dl = f(p, pL, rL, cL, AL, BL)
+        fr, dr = f(p, pR, rR, cR, AR, BR)
+        pn = p - (fl + fr + uR - uL) / (dl + dr)
+        if pn < 0:
+            pn = tol
+        if abs(pn - p) <= tol * 0.5 * (pn + p):
+            p = pn
+            break
+        p = pn
+    fl, _ = f(p, pL, rL, cL, AL, BL)
+    fr, _ = f(p, pR, rR, cR, AR, BR)
+    return p, 0.5 * (uL + uR) + 0.5 * (fr - fl)
+
+
+def toro_sample0(gamma, L, R):
+    """State (rho, p, u) of the exact solution on x/t = 0 (Toro sec. 4.5)."""
+    rL, pL, uL = L
+    rR, pR, uR = R
+    ps, us = toro_exact(gamma, L, R)
+    g = gamma
+    cL, cR = math.sqrt(g * pL / rL), math.sqrt(g * pR / rR)
+    S = 0.0
+    if S <= us:  # left of contact
+        if ps > pL:
+            SL = uL - cL * math.sqrt((g + 1) / (2 * g) * ps / pL + (g - 1) / (2 * g))
+            if S <= SL:
+                return rL, pL, uL
+            r = rL * (ps / pL + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * ps / pL + 1)
+            return r, ps, us
+        SHL = uL - cL
+        if S <= SHL:
+            return rL, pL, uL
+        csL = cL * (ps / pL) ** ((g - 1) / (2 * g))
+        STL = us - csL
+        if S > STL:
+            return rL * (ps / pL) ** (1 / g), ps, us
+        c = 2 / (g + 1) * (cL + (g - 1) / 2 * (uL - S))
+        u = 2 / (g + 1) * (cL + (g - 1) / 2 * uL + S)
+        r = rL * (c / cL) ** (2 / (g - 1))
+        return r, pL * (c / cL) ** (2 * g / (g - 1)), u
+    if ps > pR:
+        SR = uR + cR * math.sqrt((g + 1) / (2 * g) * ps / pR + (g - 1) / (2 * g))
+        if S >= SR:
+            return rR, pR, uR
+        r = rR * (ps / pR + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * ps / pR + 1)
+        return r, ps, us
+    SHR = uR + cR
+    if S >= SHR:
+        return rR, pR, uR
+    csR = cR * (ps / pR) ** ((g - 1) / (2 * g))
+    STR = us + csR
+    if S < STR:
+        return rR * (ps / pR) ** (1 / g), ps, us
+    c = 2 / (g + 1) * (cR - (g - 1) / 2 * (uR - S))
+    u = 2 / (g + 1) * (-cR + (g - 1) / 2 * uR + S)
+    r = rR * (c / cR) ** (2 / (g - 1))
+    return r, pR * (c / cR) ** (2 * g / (g - 1)), u
+
+
+def toro_hllc(gamma, L, R):
+    """L, R = (rho, p, un, ut). Returns normal-frame flux (F_rho, F_n, F_t, F_E)."""
+    rL, pL, uL, vL = L
+    rR, pR, uR, vR = R
+    cL, cR = math.sqrt(gamma * pL / rL), math.sqrt(gamma * pR / rR)
+    SL, SR = min(uL - cL, uR - cR), max(uL + cL, uR + cR)
+    Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
+    FL = euler_phys_flux(gamma, rL, pL, uL, vL)
+    FR = euler_phys_flux(gamma, rR, pR, uR, vR)
+
+    def ustar(r, p, u, v, S):
+        E = p / (gamma - 1) + 0.5 * r * (u * u + v * v)
+        f = r * (S - u) / (S - Ss)
+        return (f, f * Ss, f * v, f * (E / r + (Ss - u) * (Ss + p / (r * (S - u)))))
+
+    def cons(r, p, u, v):
+        return (r, r * u, r * v, p / (gamma - 1) + 0.5 * r * (u * u + v * v))
+
+    if SL >= 0:
+        F = FL
+    elif Ss >= 0:
+        Us, U = ustar(rL, pL, uL, vL, SL), cons(rL, pL, uL, vL)
+        F = tuple(FL[i] + SL * (Us[i] - U[i]) for i in range(4))
+    elif SR >= 0:
+        Us, U = ustar(rR, pR, uR, vR, SR), cons(rR, pR, uR, vR)
+        F = tuple(FR[i] + SR * (Us[i] - U[i]) for i in range(4))
+    else:
+        F = FR
+    return (F[0], F[1], F[2], F[3])
+
+
+# ------------------------------------------------------------------ MHD
+
+def mk_hlld(gamma, L, R):
+    """Miyoshi-Kusano HLLD, Heaviside-Lorentz B, constant normal field.
+
+    L, R = (rho, p, un, vt, wm, Bn, Bt, Bm) in the face frame with the same Bn.
+    Returns the 7 fluxes (rho, mn, mt, mm, Bt, Bm, E) and (SL, SR).
+    """
+    g = gamma
+
+    def unpack(Q):
+        r, p, u, v, w, bn, bt, bm = Q
+        b2 = bn * bn + bt * bt + bm * bm
+        pt = p + 0.5 * b2
+        E = p / (g - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
+        cf = math.sqrt((g * p + b2 + math.sqrt((g * p + b2) ** 2 - 4 * g * p * bn * bn)) / (2 * r))
+        vb = u * bn + v * bt + w * bm
+        F = (r * u, r * u * u + pt - bn * bn, r * u * v - bn * bt, r * u * w - bn * bm,
+             u * bt - v * bn, u * bm - w * bn, (E + pt) * u - bn * vb)
+        U = (r, r * u, r * v, r * w, bt, bm, E)
+        return r, p, u, v, w, bn, bt, bm, pt, E, cf, F, U
+
+    rL, pL, uL, vL, wL, Bx, btL, bmL, ptL, EL, cfL, FL, UL = unpack(L)
+    rR, pR, uR, vR, wR, _, btR, bmR, ptR, ER, cfR, FR, UR = unpack(R)
+    SL = min(uL, uR) - max(cfL, cfR)
+    SR = max(uL, uR) + max(cfL, cfR)
+    SM = ((SR - uR) * rR * uR - (SL - uL) * rL * uL - ptR + ptL) / ((SR - uR) * rR - (SL - uL) * rL)
+    pts = ((SR - uR) * rR * ptL - (SL - uL) * rL * ptR + rL * rR * (SR - uR) * (SL - uL) * (uR - uL)) / \
+          ((SR - uR) * rR - (SL - uL) * rL)
+
+    def star(r, u, v, w, bt, bm, pt, E, S):
+        rs = r * (S - u) / (S - SM)
+        den = r * (S - u) * (S - SM) - Bx * Bx
+        if abs(den) < 1e-12 * max(1.0, abs(r * (S - u) * (S - SM))):
+            vs, ws, bts, bms = v, w, bt, bm
+        else:
+            vs = v - Bx * bt * (SM - u) / den
+            ws = w - Bx * bm * (SM - u) / den
+            bts = bt * (r * (S - u) ** 2 - Bx * Bx) / den
+            bms = bm * (r * (S - u) ** 2 - Bx * Bx) / den
+        vbs = SM * Bx + vs * bts + ws * bms
+        Es = ((S - u) * E - pt * u + pts * SM + Bx * (u * Bx + v * bt + w * bm - vbs)) / (S - SM)
+        return rs, vs, ws, bts, bms, Es, (rs, rs * SM, rs * vs, rs * ws, bts, bms, Es)
+
+    rsL, vsL, wsL, btsL, bmsL, EsL, UsL = star(rL, uL, vL, wL, btL, bmL, ptL, EL, SL)
+    rsR, vsR, wsR, btsR, bmsR, EsR, UsR = star(rR, uR, vR, wR, btR, bmR, ptR, ER, SR)
+    sL, sR = math.sqrt(rsL), math.sqrt(rsR)
+    SsL = SM - abs(Bx) / sL
+    SsR = SM + abs(Bx) / sR
+    if Bx == 0.0:
+        SsL = SsR = SM
+        sgn = 0.0
+    else:
+        sgn = math.copysign(1.0, Bx)
+    vss = (sL * vsL + sR * vsR + (btsR - btsL) * sgn) / (sL + sR)
+    wss = (sL * wsL + sR * wsR + (bmsR - bmsL) * sgn) / (sL + sR)
+    btss = (sL * btsR + sR * btsL + sL * sR * (vsR - vsL) * sgn) / (sL + sR)
+    bmss = (sL * bmsR + sR * bmsL + sL * sR * (wsR - wsL) * sgn) / (sL + sR)
+    vbss = SM * Bx + vss * btss + wss * bmss
+    EssL = EsL - sL * (SM * Bx + vsL * btsL + wsL * bmsL - vbss) * sgn
+    EssR = EsR + sR * (SM * Bx + vsR * btsR + wsR * bmsR - vbss) * sgn
+    UssL = (rsL, rsL * SM, rsL * vss, rsL * wss, btss, bmss, EssL)
+    UssR = (rsR, rsR * SM, rsR * vss, rsR * wss, btss, bmss, EssR)
+    if SL >= 0:
+        F = FL
+    elif SsL >= 0:
+        F = tuple(FL[i] + SL * (UsL[i] - UL[i]) for i in range(7))
+    elif SM >= 0:
+        F = tuple(FL[i] + SsL * UssL[i] - (SsL - SL) * UsL[i] - SL * UL[i] for i in range(7))
+    elif SsR > 0:
+        F = tuple(FR[i] + SsR * UssR[i] - (SsR - SR) * UsR[i] - SR * UR[i] for i in range(7))
+    elif SR > 0:
+        F = tuple(FR[i] + SR * (UsR[i] - UR[i]) for i in range(7))
+    else:
+        F = FR
+    return F, (SL, SR)
+
+
+def mhd_phys_flux_hl(gamma, r, p, u, v, w, bn, bt, bm):
+    b2 = bn * bn + bt * bt + bm * bm
+    pt = p + 0.5 * b2
+    E = p / (gamma - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
+    vb = u * bn + v * bt + w * bm
+    return (r * u, r * u * u + pt - bn * bn, r * u * v - bn * bt, r * u * w - bn * bm,
+            u * bt - v * bn, u * bm - w * bn, (E + pt) * u - bn * vb)
+
+
+# ------------------------------------------------------------------ Sod
+
+def sod_exact(gamma, x, t, x0=0.5, L=(1.0, 1.0, 0.0), R=(0.125, 0.1, 0.0)):
+    """Exact Sod profile (rho, p, u) at points x, time t (Toro sampling)."""
+    out = []
+    for xi in x:
+        S = (xi - x0) / t
+        # shift the frame: sample at speed S by Galilean transform
+        Ls = (L[0], L[1], L[2] - S)
+        Rs = (R[0], R[1], R[2] - S)
+        r, p, u = toro_sample0(gamma, Ls, Rs)
+        out.append((r, p, u + S))
+    return out
