@@ -1,0 +1,256 @@
+"""Host-side Python mirror of the libgd2.so C ABI (include/gd2.h).
+
+Device buffers are torch tensors on a ROCm device (torch is the allocator and
+stream provider only); every compute call goes through libgd2.so. If the
+library is not built the first call raises: there is no CPU fallback.
+
+Reference-layout functions keep the reference kernels' names and argument
+order (kernel.cu:1736 Ker_Dekard, 3286 Kernel_TVD_old, 1877 add2, 2198
+Kernel_TVD, 1722 funk_time); tensors s, u have shape (ny, nx, 2) (double2 per
+cell) and b has shape (ny, nx, 3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO_PATH = os.path.join(HERE, "libgd2.so")
+
+BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
+EULER, MHD = 0, 1
+FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD = 1, 2, 3, 4
+FORWARD_EULER, SSPRK2 = 1, 2
+
+_lib = None
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double), ("gamma", C.c_double),
+                ("krit", C.c_double), ("bc", C.c_int * 4)]
+
+
+class Scheme(C.Structure):
+    _fields_ = [("physics", C.c_int), ("flux", C.c_int), ("order", C.c_int), ("integrator", C.c_int),
+                ("cfl", C.c_double)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("clamp_rho", C.c_longlong), ("clamp_p", C.c_longlong),
+                ("newton_cap_hits", C.c_longlong), ("bisect_cap_hits", C.c_longlong)]
+
+
+class GD2Error(RuntimeError):
+    pass
+
+
+def make_grid(nx, ny, *, dx=None, dy=None, x0=None, y0=None, gamma=1.4, krit=0.25,
+              bc=(BC_TRANSMISSIVE,) * 4) -> Grid:
+    """Uniform grid on [0, nx*dx] x [0, ny*dy]; defaults to the unit square."""
+    dx = 1.0 / nx if dx is None else dx
+    dy = 1.0 / ny if dy is None else dy
+    g = Grid(nx, ny, dx, dy, dx / 2.0 if x0 is None else x0, dy / 2.0 if y0 is None else y0, gamma, krit)
+    for i in range(4):
+        g.bc[i] = bc[i]
+    return g
+
+
+def make_scheme(physics=EULER, flux=FLUX_HLLC, order=2, integrator=SSPRK2, cfl=0.4) -> Scheme:
+    return Scheme(physics, flux, order, integrator, cfl)
+
+
+def lib():
+    """Load libgd2.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO_PATH):
+            raise GD2Error(f"{SO_PATH} is missing: build it with golikov_2d_amd/build.py "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(SO_PATH)
+        vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+        gp, sp = C.POINTER(Grid), C.POINTER(Scheme)
+        sig = {
+            "gd2_nfields": (ip, [ip]),
+            "gd2_last_hip_error": (ip, []),
+            "gd2_version": (C.c_char_p, []),
+            "gd2_stage": (ip, [gp, sp, vp, vp, vp, dp, dp, dp, vp, vp, vp]),
+            "gd2_wavespeed_max": (ip, [gp, sp, vp, vp, vp]),
+            "gd2_ker_dekard": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
+            "gd2_kernel_tvd_old": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
+            "gd2_add2": (ip, [gp, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip, vp]),
+            "gd2_kernel_tvd": (ip, [gp, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip, vp]),
+            "gd2_funk_time": (ip, [vp, vp, vp, vp, vp]),
+            "gd2_prim_to_cons": (ip, [gp, ip, vp, vp, vp, vp, vp]),
+            "gd2_cons_to_prim": (ip, [gp, ip, vp, vp, vp, vp, vp]),
+            "gd2_ctx_create": (ip, [gp, sp, C.POINTER(vp)]),
+            "gd2_ctx_destroy": (ip, [vp]),
+            "gd2_ctx_start": (ip, [vp, vp, dp, vp]),
+            "gd2_ctx_advance": (ip, [vp, vp, ip, dp, vp]),
+            "gd2_ctx_time": (ip, [vp, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
+            "gd2_get_diagnostics": (ip, [C.POINTER(Diag), ip]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd2_wavespeed_max",
+            "gd2_ker_dekard", "gd2_kernel_tvd_old", "gd2_add2", "gd2_kernel_tvd", "gd2_funk_time",
+            "gd2_prim_to_cons", "gd2_cons_to_prim", "gd2_ctx_create", "gd2_ctx_destroy", "gd2_ctx_start",
+            "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics"]
+
+
+def _check(rc, name):
+    if rc != 0:
+        raise GD2Error(f"{name} returned {rc} (hip error {lib().gd2_last_hip_error()})")
+
+
+def _p(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise GD2Error("device tensor expected")
+    if not t.is_contiguous():
+        raise GD2Error("contiguous tensor expected")
+    return C.c_void_p(t.data_ptr())
+
+
+def _f64(t):
+    if t is not None and t.dtype != torch.float64:
+        raise GD2Error("float64 tensor expected")
+    return _p(t)
+
+
+def _stream(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def nfields(physics):
+    return lib().gd2_nfields(physics)
+
+
+# ------------------------------------------------ reference-layout entry points
+
+def _check_aos(g, *ts):
+    for t in ts:
+        if t is not None and (t.shape[0] != g.ny or t.shape[1] != g.nx):
+            raise GD2Error(f"state tensor shape {tuple(t.shape)} does not match grid {g.ny}x{g.nx}")
+
+
+def ker_dekard(g, s, u, s2, u2, T, T_do, method, stream=None):
+    _check_aos(g, s, u, s2, u2)
+    _check(lib().gd2_ker_dekard(C.byref(g), _f64(s), _f64(u), _f64(s2), _f64(u2), _f64(T), _f64(T_do),
+                                method, _stream(stream)), "gd2_ker_dekard")
+
+
+def kernel_tvd_old(g, s, u, s2, u2, T, T_do, method, stream=None):
+    _check_aos(g, s, u, s2, u2)
+    _check(lib().gd2_kernel_tvd_old(C.byref(g), _f64(s), _f64(u), _f64(s2), _f64(u2), _f64(T), _f64(T_do),
+                                    method, _stream(stream)), "gd2_kernel_tvd_old")
+
+
+def add2(g, s, u, b, s2, u2, b2, T, T_do, i, method, stream=None):
+    _check_aos(g, s, u, b, s2, u2, b2)
+    _check(lib().gd2_add2(C.byref(g), _f64(s), _f64(u), _f64(b), _f64(s2), _f64(u2), _f64(b2), _f64(T),
+                          _f64(T_do), i, method, _stream(stream)), "gd2_add2")
+
+
+def kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, i, method, stream=None):
+    _check_aos(g, s, u, b, s2, u2, b2)
+    _check(lib().gd2_kernel_tvd(C.byref(g), _f64(s), _f64(u), _f64(b), _f64(s2), _f64(u2), _f64(b2),
+                                _f64(T), _f64(T_do), i, method, _stream(stream)), "gd2_kernel_tvd")
+
+
+def funk_time(T, T_do, TT, i, stream=None):
+    if i.dtype != torch.int32:
+        raise GD2Error("step counter must be int32")
+    _check(lib().gd2_funk_time(_f64(T), _f64(T_do), _f64(TT), _p(i), _stream(stream)), "gd2_funk_time")
+
+
+# ------------------------------------------------ native path
+
+def stage(g, sc, Uin, Ubase, Uout, a=0.0, b=1.0, dt=0.0, dt_dev=None, wmax_bits=None, stream=None):
+    _check(lib().gd2_stage(C.byref(g), C.byref(sc), _f64(Uin), _f64(Ubase), _f64(Uout), float(a), float(b),
+                           float(dt), _f64(dt_dev), _p(wmax_bits), _stream(stream)), "gd2_stage")
+
+
+def bits_to_double(t):
+    """View an int64 device scalar holding double bits as a Python float."""
+    return t.view(torch.float64).item()
+
+
+def wavespeed_max(g, sc, U, stream=None) -> float:
+    w = torch.zeros(1, dtype=torch.int64, device=U.device)
+    _check(lib().gd2_wavespeed_max(C.byref(g), C.byref(sc), _f64(U), _p(w), _stream(stream)), "gd2_wavespeed_max")
+    return bits_to_double(w)
+
+
+def prim_to_cons(g, physics, s, u, b=None, stream=None):
+    U = torch.empty((nfields(physics), g.ny, g.nx), dtype=torch.float64, device=s.device)
+    _check(lib().gd2_prim_to_cons(C.byref(g), physics, _f64(s), _f64(u), _f64(b), _f64(U), _stream(stream)),
+           "gd2_prim_to_cons")
+    return U
+
+
+def cons_to_prim(g, physics, U, stream=None):
+    dev = U.device
+    s = torch.empty((g.ny, g.nx, 2), dtype=torch.float64, device=dev)
+    u = torch.empty((g.ny, g.nx, 2), dtype=torch.float64, device=dev)
+    b = torch.empty((g.ny, g.nx, 3), dtype=torch.float64, device=dev) if physics == MHD else None
+    _check(lib().gd2_cons_to_prim(C.byref(g), physics, _f64(U), _f64(s), _f64(u), _f64(b), _stream(stream)),
+           "gd2_cons_to_prim")
+    return s, u, b
+
+
+def diagnostics(reset=False):
+    d = Diag()
+    _check(lib().gd2_get_diagnostics(C.byref(d), 1 if reset else 0), "gd2_get_diagnostics")
+    return {f: getattr(d, f) for f, _ in Diag._fields_}
+
+
+class Solver:
+    """Native solver context (gd2_ctx): scratch state + device clock."""
+
+    def __init__(self, g: Grid, sc: Scheme):
+        self.g, self.sc = g, sc
+        h = C.c_void_p()
+        _check(lib().gd2_ctx_create(C.byref(g), C.byref(sc), C.byref(h)), "gd2_ctx_create")
+        self._h = h
+
+    def start(self, U, t0=0.0, stream=None):
+        _check(lib().gd2_ctx_start(self._h, _f64(U), float(t0), _stream(stream)), "gd2_ctx_start")
+
+    def advance(self, U, nsteps, t_end=0.0, stream=None):
+        _check(lib().gd2_ctx_advance(self._h, _f64(U), int(nsteps), float(t_end), _stream(stream)),
+               "gd2_ctx_advance")
+
+    def time(self, stream=None):
+        t, dt, n = C.c_double(), C.c_double(), C.c_longlong()
+        _check(lib().gd2_ctx_time(self._h, C.byref(t), C.byref(n), C.byref(dt), _stream(stream)), "gd2_ctx_time")
+        return t.value, n.value, dt.value
+
+    def run_to(self, U, t_end, chunk=32, max_steps=10 ** 7):
+        """Advance until t_end, checking the device clock every `chunk` steps."""
+        while True:
+            t, n, _ = self.time()
+            if not (t < t_end) or n >= max_steps:
+                return t, n
+            self.advance(U, chunk, t_end)
+
+    def close(self):
+        if self._h:
+            lib().gd2_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

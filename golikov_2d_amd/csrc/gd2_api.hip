@@ -1,0 +1,451 @@
+// gd2_api.hip -- the C ABI of libgd2.so (declared in include/gd2.h):
+// argument checks, launch configuration, the device clock of the native
+// solver context and the device diagnostics. All compute is in the kernels of
+// gd2_euler.hpp / gd2_mhd.hpp.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "../../include/gd2.h"
+#include "gd2_euler.hpp"
+#include "gd2_mhd.hpp"
+
+namespace gd2 {
+
+__device__ DeviceDiag g_diag;
+
+static thread_local int g_last_hip = 0;
+
+static int hip_fail(hipError_t e) {
+    g_last_hip = (int)e;
+    return GD2_ERR_HIP;
+}
+
+#define GD2_HIP(call)                                 \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(e_);    \
+    } while (0)
+
+static DeviceDiag* diag_ptr() {
+    static DeviceDiag* p = nullptr;
+    if (!p) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(g_diag)) != hipSuccess) return nullptr;
+        p = static_cast<DeviceDiag*>(q);
+    }
+    return p;
+}
+
+static bool grid_ok(const gd2_grid* g, bool allow_frozen) {
+    if (!g || g->nx < 1 || g->ny < 1) return false;
+    if (!(g->dx > 0.0) || !(g->dy > 0.0) || !(g->gamma > 1.0)) return false;
+    if ((long long)g->nx * g->ny > (1ll << 40)) return false;
+    for (int s = 0; s < 4; ++s) {
+        if (g->bc[s] < GD2_BC_TRANSMISSIVE || g->bc[s] > GD2_BC_FROZEN) return false;
+        if (g->bc[s] == GD2_BC_FROZEN && !allow_frozen) return false;
+    }
+    // a periodic side needs a periodic opposite side
+    if ((g->bc[0] == GD2_BC_PERIODIC) != (g->bc[1] == GD2_BC_PERIODIC)) return false;
+    if ((g->bc[2] == GD2_BC_PERIODIC) != (g->bc[3] == GD2_BC_PERIODIC)) return false;
+    return true;
+}
+
+// Rows per wave strip. Default: long enough to amortise the 4-row y-halo,
+// short enough to give every CU several waves; GD2_TILE_ROWS overrides it.
+static int tile_rows(int nx, int ny) {
+    static int env = -1;
+    if (env < 0) {
+        const char* e = std::getenv("GD2_TILE_ROWS");
+        env = e ? std::atoi(e) : 0;
+    }
+    if (env > 0) return env;
+    const long long tiles_x = (nx + kTileW - 1) / kTileW;
+    int rows = 64;
+    while (rows > 8 && tiles_x * ((ny + rows - 1) / rows) < 4096) rows /= 2;
+    return rows;
+}
+
+static void fill_common(EulerArgs& A, const gd2_grid* g) {
+    std::memset(&A, 0, sizeof A);
+    A.nx = g->nx;
+    A.ny = g->ny;
+    A.plane = (long long)g->nx * g->ny;
+    A.dx = g->dx;
+    A.dy = g->dy;
+    A.idx = 1.0 / g->dx;
+    A.idy = 1.0 / g->dy;
+    A.gamma = g->gamma;
+    A.g1 = g->gamma - 1.0;
+    A.inv_g1 = 1.0 / (g->gamma - 1.0);
+    A.krit = g->krit;
+    for (int s = 0; s < 4; ++s) A.bc[s] = g->bc[s];
+    A.diag = diag_ptr();
+    A.rows = tile_rows(g->nx, g->ny);
+    A.tiles_x = (g->nx + kTileW - 1) / kTileW;
+}
+
+static unsigned euler_blocks(const EulerArgs& A) {
+    return (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
+}
+
+template <int ORDER, int FLUX, int LAYOUT, bool W>
+static int launch_euler(const EulerArgs& A, hipStream_t st) {
+    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W>), dim3(euler_blocks(A)), dim3(kWave), 0, st, A);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+template <int LAYOUT, bool W>
+static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t st) {
+    if (order == 2) {
+        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, LAYOUT, W>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, LAYOUT, W>(A, st);
+    } else if (order == 1) {
+        if (flux == GD2_FLUX_HLLC) return launch_euler<1, GD2_FLUX_HLLC, LAYOUT, W>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W>(A, st);
+    }
+    return GD2_ERR_UNSUPPORTED;
+}
+
+static bool scheme_ok(const gd2_scheme* sc) {
+    if (!sc || (sc->order != 1 && sc->order != 2)) return false;
+    if (sc->physics == GD2_EULER) return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT;
+    if (sc->physics == GD2_MHD) return sc->flux == GD2_FLUX_HLLC_MHD || sc->flux == GD2_FLUX_HLLD;
+    return false;
+}
+
+// ---------------------------------------------------------------- clock
+struct Clock {
+    double t, dt, cfl, pad;
+    unsigned long long w[2];
+    long long step;
+};
+
+__global__ void clock_begin_kernel(Clock* c, int cur, double t_end) {
+    const double w = __longlong_as_double((long long)c->w[cur]);
+    double dt = c->cfl / w;
+    bool live = true;
+    if (t_end > 0.0) {
+        if (!(c->t < t_end)) {
+            dt = 0.0;
+            live = false;
+        } else if (dt > t_end - c->t) {
+            dt = t_end - c->t;
+        }
+    }
+    c->dt = dt;
+    c->w[cur ^ 1] = 0ull;
+    if (live) {
+        c->t = c->t + dt;
+        c->step += 1;
+    }
+}
+
+__global__ void clock_start_kernel(Clock* c, double t0, double cfl) {
+    c->t = t0;
+    c->dt = 0.0;
+    c->cfl = cfl;
+    c->w[0] = 0ull;
+    c->w[1] = 0ull;
+    c->step = 0;
+}
+
+__global__ void funk_time_kernel(double* T, double* T_do, double* TT, int* i) {
+    // kernel.cu:1722-1733 (its every-5000-steps printf is left out)
+    *T_do = *T;
+    *TT = *TT + *T_do;
+    *T = 10000000;
+    *i = *i + 1;
+}
+
+__global__ void prim_to_cons_kernel(const double2* s, const double2* u, const double3* b, double* U, long long n,
+                                    double gamma, int mhd) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const double r = s[k].x, p = s[k].y, vx = u[k].x, vy = u[k].y;
+        U[k] = r;
+        U[n + k] = r * vx;
+        U[2 * n + k] = r * vy;
+        if (mhd) {
+            const double3 B = b[k];
+            U[3 * n + k] = B.x;
+            U[4 * n + k] = B.y;
+            U[5 * n + k] = B.z;
+            U[6 * n + k] = p / (gamma - 1.0) + 0.5 * r * (vx * vx + vy * vy) + (B.x * B.x + B.y * B.y + B.z * B.z) / kCpi8;
+        } else {
+            U[3 * n + k] = p / (gamma - 1.0) + 0.5 * r * (vx * vx + vy * vy);
+        }
+    }
+}
+
+__global__ void cons_to_prim_kernel(const double* U, double2* s, double2* u, double3* b, long long n, double gamma,
+                                    int mhd) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const double r = U[k], mx = U[n + k], my = U[2 * n + k];
+        const double inv = 1.0 / r;
+        const double vx = mx * inv, vy = my * inv;
+        const double ke = 0.5 * (mx * vx + my * vy);
+        double p;
+        if (mhd) {
+            const double bx = U[3 * n + k], by = U[4 * n + k], bz = U[5 * n + k];
+            p = (gamma - 1.0) * (U[6 * n + k] - ke - (bx * bx + by * by + bz * bz) / kCpi8);
+            if (b) b[k] = make_double3(bx, by, bz);
+        } else {
+            p = (gamma - 1.0) * (U[3 * n + k] - ke);
+        }
+        s[k] = make_double2(r, p);
+        u[k] = make_double2(vx, vy);
+    }
+}
+
+static unsigned ew_blocks(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace gd2
+
+using namespace gd2;
+
+struct gd2_ctx {
+    gd2_grid g;
+    gd2_scheme sc;
+    double* scratch;  // one state (nfields planes)
+    Clock* clock;
+    long long host_steps;  // steps enqueued (parity of the wave-speed slots)
+};
+
+extern "C" {
+
+int gd2_nfields(int physics) { return physics == GD2_MHD ? 7 : (physics == GD2_EULER ? 4 : GD2_ERR_ARG); }
+
+int gd2_last_hip_error(void) { return g_last_hip; }
+
+const char* gd2_version(void) { return "gd2 0.1.0 (gfx950)"; }
+
+int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const double* Ubase, double* Uout,
+              double a, double b, double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
+    if (!grid_ok(g, false) || !scheme_ok(sc) || !Uin || !Uout || Uin == Uout) return GD2_ERR_ARG;
+    const bool plain = (a == 0.0 && b == 1.0);
+    if (!plain && !Ubase) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    if (sc->physics == GD2_MHD) return mhd_stage(g, sc, Uin, Ubase, Uout, a, b, dt, dt_dev, wmax_bits, diag_ptr(), st);
+    EulerArgs A;
+    fill_common(A, g);
+    A.Uin = Uin;
+    A.Ubase = Ubase;
+    A.Uout = Uout;
+    A.a = a;
+    A.b = b;
+    A.plain = plain ? 1 : 0;
+    A.dt = dt;
+    A.dt_dev = dt_dev;
+    A.wmax_bits = wmax_bits;
+    if (!A.diag) return GD2_ERR_HIP;
+    if (wmax_bits) return dispatch_euler<kSoA, true>(sc->order, sc->flux, A, st);
+    return dispatch_euler<kSoA, false>(sc->order, sc->flux, A, st);
+}
+
+int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, unsigned long long* wmax_bits,
+                      void* stream) {
+    if (!grid_ok(g, true) || !scheme_ok(sc) || !U || !wmax_bits) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = (long long)g->nx * g->ny;
+    if (sc->physics == GD2_MHD) return mhd_wavespeed(g, U, wmax_bits, st);
+    hipLaunchKernelGGL(euler_wavespeed_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, U, n, n, g->gamma,
+                       g->gamma - 1.0, 1.0 / g->dx, 1.0 / g->dy, wmax_bits);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2,
+                     gd2_double2* u2, double* T, const double* T_do, int method, void* stream) {
+    if (!grid_ok(g, true) || !s || !u || !s2 || !u2 || !T || !T_do) return GD2_ERR_ARG;
+    if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2) return GD2_ERR_ARG;
+    if (method != 1 && method != 2) return GD2_ERR_UNSUPPORTED;
+    EulerArgs A;
+    fill_common(A, g);
+    A.s = reinterpret_cast<const double2*>(s);
+    A.u = reinterpret_cast<const double2*>(u);
+    A.s2 = reinterpret_cast<double2*>(s2);
+    A.u2 = reinterpret_cast<double2*>(u2);
+    A.dt_dev = T_do;
+    A.tmin_bits = reinterpret_cast<unsigned long long*>(T);
+    A.plain = 1;
+    A.a = 0.0;
+    A.b = 1.0;
+    if (!A.diag) return GD2_ERR_HIP;
+    return dispatch_euler<kAoS, false>(order, method == 2 ? GD2_FLUX_EXACT : GD2_FLUX_HLLC, A, (hipStream_t)stream);
+}
+
+int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2, gd2_double2* u2,
+                   double* T, const double* T_do, int method, void* stream) {
+    return ref_euler(g, 1, s, u, s2, u2, T, T_do, method, stream);
+}
+
+int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2,
+                       gd2_double2* u2, double* T, const double* T_do, int method, void* stream) {
+    return ref_euler(g, 2, s, u, s2, u2, T, T_do, method, stream);
+}
+
+int gd2_add2(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b, gd2_double2* s2,
+             gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i, int method, void* stream) {
+    (void)i;
+    if (!grid_ok(g, true)) return GD2_ERR_ARG;
+    return mhd_ref_step(g, 1, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
+}
+
+int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                   gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i,
+                   int method, void* stream) {
+    (void)i;
+    if (!grid_ok(g, true)) return GD2_ERR_ARG;
+    return mhd_ref_step(g, 2, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
+}
+
+int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream) {
+    if (!T || !T_do || !TT || !i) return GD2_ERR_ARG;
+    hipLaunchKernelGGL(funk_time_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, T, T_do, TT, i);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
+                     const gd2_double3* b, double* U, void* stream) {
+    if (!grid_ok(g, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
+    const long long n = (long long)g->nx * g->ny;
+    hipLaunchKernelGGL(prim_to_cons_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const double2*>(s), reinterpret_cast<const double2*>(u),
+                       reinterpret_cast<const double3*>(b), U, n, g->gamma, physics == GD2_MHD ? 1 : 0);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+int gd2_cons_to_prim(const gd2_grid* g, int physics, const double* U, gd2_double2* s, gd2_double2* u,
+                     gd2_double3* b, void* stream) {
+    if (!grid_ok(g, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
+    const long long n = (long long)g->nx * g->ny;
+    hipLaunchKernelGGL(cons_to_prim_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, U,
+                       reinterpret_cast<double2*>(s), reinterpret_cast<double2*>(u), reinterpret_cast<double3*>(b), n,
+                       g->gamma, physics == GD2_MHD ? 1 : 0);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+int gd2_ctx_create(const gd2_grid* g, const gd2_scheme* sc, gd2_ctx** out) {
+    if (!out) return GD2_ERR_ARG;
+    *out = nullptr;
+    if (!grid_ok(g, false) || !scheme_ok(sc)) return GD2_ERR_ARG;
+    if (sc->integrator != GD2_FORWARD_EULER && sc->integrator != GD2_SSPRK2) return GD2_ERR_ARG;
+    if (!(sc->cfl > 0.0)) return GD2_ERR_ARG;
+    if (!diag_ptr()) return GD2_ERR_HIP;
+    gd2_ctx* c = new (std::nothrow) gd2_ctx();
+    if (!c) return GD2_ERR_NOMEM;
+    c->g = *g;
+    c->sc = *sc;
+    const size_t bytes = sizeof(double) * (size_t)gd2_nfields(sc->physics) * (size_t)g->nx * (size_t)g->ny;
+    hipError_t e = hipMalloc(&c->scratch, bytes);
+    if (e != hipSuccess) {
+        delete c;
+        g_last_hip = (int)e;
+        return GD2_ERR_NOMEM;
+    }
+    e = hipMalloc(&c->clock, sizeof(Clock));
+    if (e != hipSuccess) {
+        (void)hipFree(c->scratch);
+        delete c;
+        g_last_hip = (int)e;
+        return GD2_ERR_NOMEM;
+    }
+    c->host_steps = 0;
+    *out = c;
+    return GD2_OK;
+}
+
+int gd2_ctx_destroy(gd2_ctx* c) {
+    if (!c) return GD2_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(c->scratch);
+    (void)hipFree(c->clock);
+    delete c;
+    return GD2_OK;
+}
+
+int gd2_ctx_start(gd2_ctx* c, const double* U, double t0, void* stream) {
+    if (!c || !U) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(clock_start_kernel, dim3(1), dim3(1), 0, st, c->clock, t0, c->sc.cfl);
+    GD2_HIP(hipGetLastError());
+    c->host_steps = 0;
+    return gd2_wavespeed_max(&c->g, &c->sc, U, &c->clock->w[0], stream);
+}
+
+int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* stream) {
+    if (!c || !U || nsteps < 0) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    double* S = c->scratch;
+    const double* dtp = &c->clock->dt;
+    double* cur = U;  // forward Euler ping-pongs between U and S
+    for (int i = 0; i < nsteps; ++i) {
+        const int slot = (int)(c->host_steps & 1);
+        hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end);
+        GD2_HIP(hipGetLastError());
+        unsigned long long* wn = &c->clock->w[slot ^ 1];
+        int rc;
+        if (c->sc.integrator == GD2_SSPRK2) {
+            rc = gd2_stage(&c->g, &c->sc, U, nullptr, S, 0.0, 1.0, 0.0, dtp, nullptr, stream);
+            if (rc) return rc;
+            rc = gd2_stage(&c->g, &c->sc, S, U, U, 0.5, 0.5, 0.0, dtp, wn, stream);
+            if (rc) return rc;
+        } else {
+            double* nxt = (cur == U) ? S : U;
+            rc = gd2_stage(&c->g, &c->sc, cur, nullptr, nxt, 0.0, 1.0, 0.0, dtp, wn, stream);
+            if (rc) return rc;
+            cur = nxt;
+        }
+        c->host_steps += 1;
+    }
+    if (cur != U) {
+        const size_t bytes = sizeof(double) * (size_t)gd2_nfields(c->sc.physics) * (size_t)c->g.nx * (size_t)c->g.ny;
+        GD2_HIP(hipMemcpyAsync(U, cur, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    return GD2_OK;
+}
+
+int gd2_ctx_time(gd2_ctx* c, double* t, long long* steps, double* dt, void* stream) {
+    if (!c) return GD2_ERR_ARG;
+    Clock h;
+    GD2_HIP(hipMemcpyAsync(&h, c->clock, sizeof(Clock), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    GD2_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (t) *t = h.t;
+    if (steps) *steps = h.step;
+    if (dt) *dt = h.dt;
+    return GD2_OK;
+}
+
+int gd2_get_diagnostics(gd2_diag* out, int reset) {
+    DeviceDiag* p = diag_ptr();
+    if (!p || !out) return p ? GD2_ERR_ARG : GD2_ERR_HIP;
+    GD2_HIP(hipDeviceSynchronize());
+    DeviceDiag h;
+    GD2_HIP(hipMemcpy(&h, p, sizeof h, hipMemcpyDeviceToHost));
+    out->clamp_rho = (long long)h.clamp_rho;
+    out->clamp_p = (long long)h.clamp_p;
+    out->newton_cap_hits = (long long)h.newton_cap;
+    out->bisect_cap_hits = (long long)h.bisect_cap;
+    if (reset) {
+        std::memset(&h, 0, sizeof h);
+        GD2_HIP(hipMemcpy(p, &h, sizeof h, hipMemcpyHostToDevice));
+    }
+    return GD2_OK;
+}
+
+}  // extern "C"

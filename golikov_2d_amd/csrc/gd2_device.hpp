@@ -1,0 +1,371 @@
+// gd2_device.hpp -- device-side building blocks of the gfx950 step kernels:
+// boundary index mapping, minmod, primitive/conserved conversions, the HLLC
+// and exact-Riemann Euler interface fluxes, and wave-level reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gd2.h"
+
+namespace gd2 {
+
+constexpr int kWave = 64;
+constexpr double kCpi4 = 12.56637061435917295384;  // 4 pi    (Header.h:20)
+constexpr double kCpi8 = 25.13274122871834590768;  // 8 pi    (Header.h:21)
+constexpr double kSpi4 = 3.544907701811032;        // sqrt(4 pi) literal (Header.h:22)
+
+// Device counters (clamps, iteration caps) read by gd2_get_diagnostics.
+struct DeviceDiag {
+    unsigned long long clamp_rho, clamp_p, newton_cap, bisect_cap;
+};
+
+// Logical index k (may be -2..n+1, or further on padding lanes) -> stored
+// index; flip is set where the reflective rule negates the normal vector
+// components. Transmissive and frozen sides clamp (kernel.cu:1762-1793,
+// 3374-3417), reflective sides mirror (kernel.cu:2263-2277, 2356-2405),
+// periodic sides wrap.
+__device__ __forceinline__ int map_index(int k, int n, int lo, int hi, bool& flip) {
+    flip = false;
+    if (k < 0) {
+        if (lo == GD2_BC_PERIODIC) return ((k % n) + n) % n;
+        if (lo == GD2_BC_REFLECTIVE) {
+            flip = true;
+            int m = -1 - k;
+            return m < n ? m : n - 1;
+        }
+        return 0;
+    }
+    if (k >= n) {
+        if (hi == GD2_BC_PERIODIC) return k % n;
+        if (hi == GD2_BC_REFLECTIVE) {
+            flip = true;
+            int m = 2 * n - 1 - k;
+            return m >= 0 ? m : 0;
+        }
+        return n - 1;
+    }
+    return k;
+}
+
+// minmod(a, b) of kernel.cu:51-62 written as the median of (0, a, b): equal
+// for every input including signed zeros' magnitudes (sign-sum zero -> 0,
+// same signs -> the argument of smaller magnitude).
+__device__ __forceinline__ double minmod(double a, double b) {
+    return fmax(fmin(a, b), fmin(fmax(a, b), 0.0));
+}
+
+// Primitive Euler state.
+struct PrimE {
+    double r, u, v, p;
+};
+
+__device__ __forceinline__ PrimE cons_to_prim_e(double r, double mx, double my, double E, double g1) {
+    PrimE q;
+    double inv = 1.0 / r;
+    q.r = r;
+    q.u = mx * inv;
+    q.v = my * inv;
+    q.p = g1 * (E - 0.5 * (mx * q.u + my * q.v));
+    return q;
+}
+
+// Face flux in the face frame: (mass, normal momentum, tangential momentum,
+// energy) along the face normal.
+struct FluxE {
+    double fr, fn, ft, fe;
+};
+
+// HLLC flux of HLLC_Aleksashov (kernel.cu:853-1165) with B = 0, in its
+// face frame: normal velocity un, tangential ut. Only the star state of the
+// side chosen by sign(SM) is formed (the reference forms both); the branch
+// choice SL > 0 / SL <= 0 <= SM / SM <= 0 <= SR / SR < 0 (kernel.cu:1119-1158)
+// becomes a select. smax returns max(|SL|, |SR|).
+__device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, double pL,
+                                            double rR, double unR, double utR, double pR,
+                                            double gamma, double inv_g1, double& smax) {
+    const double cL = sqrt(gamma * pL / rL);
+    const double cR = sqrt(gamma * pR / rR);
+    const double SL = fmin(unL - cL, unR - cR);
+    const double SR = fmax(unL + cL, unR + cR);
+    smax = fmax(fabs(SL), fabs(SR));
+    const double suL = SL - unL, suR = SR - unR;
+    const double mL = suL * rL, mR = suR * rR;
+    const double SM = (mR * unR - pR + pL - mL * unL) / (mR - mL);
+    // averaged star total pressure (kernel.cu:1069-1071)
+    const double ptz = 0.5 * ((pR + mR * (SM - unR)) + (pL + mL * (SM - unL)));
+    const bool left = SM >= 0.0;
+    const double S = left ? SL : SR;
+    const double r = left ? rL : rR;
+    const double un = left ? unL : unR;
+    const double ut = left ? utL : utR;
+    const double p = left ? pL : pR;
+    const double su = left ? suL : suR;
+    const bool super = (SL > 0.0) || (SR < 0.0);
+    const double e = p * inv_g1 + r * (0.5 * (un * un + ut * ut));
+    const double den = super ? 1.0 : (S - SM);
+    const double inv = 1.0 / den;
+    const double sum = su * inv;
+    const double rz = r * sum;
+    const double ez = e * sum + (ptz * SM - p * un) * inv;
+    const double Se = super ? 0.0 : S;
+    const double run = r * un;
+    FluxE F;
+    F.fr = run + Se * (rz - r);
+    F.fn = run * un + p + Se * (SM * rz - run);
+    F.ft = run * ut + Se * (ut * rz - r * ut);
+    F.fe = (e + p) * un + Se * (ez - e);
+    return F;
+}
+
+// ---------------------------------------------------------------------------
+// Exact Riemann solver (Godunov_Solver_Alexashov, kernel.cu:214-656; newton
+// 721-766; devtwo 768-835; lev 837-851) with gamma at run time. The
+// reference loops without a bound; the caps below are never reached on
+// finite inputs and each hit is counted.
+// ---------------------------------------------------------------------------
+constexpr int kNewtonCap = 200;
+constexpr int kBisectCap = 400;
+
+struct GConst {
+    double ga, g1, g2, gp, gm;
+};
+
+__device__ __forceinline__ GConst make_gconst(double gamma) {
+    GConst k;
+    k.ga = gamma;
+    k.g1 = gamma - 1.0;
+    k.g2 = gamma + 1.0;
+    k.gp = (k.g2 / k.ga) / 2.0;
+    k.gm = (k.g1 / k.ga) / 2.0;
+    return k;
+}
+
+__device__ inline double lev_f(const GConst& k, double enI, double pI, double rI, double enII, double pII,
+                               double rII, double uuu) {
+    double cI = sqrt(k.ga * pI / rI);
+    double cII = sqrt(k.ga * pII / rII);
+    double fI = (uuu - pI) / (rI * cI * sqrt(k.gp * uuu / pI + k.gm));
+    double fII = 2.0 / k.g1 * cII * (pow((uuu / pII), k.gm) - 1.0);
+    return (fI + fII) - (enI - enII);
+}
+
+__device__ inline double devtwo(const GConst& k, double enI, double pI, double rI, double enII, double pII,
+                                double rII, DeviceDiag* diag) {
+    const double epsil = 10e-10;
+    double kl = pI, kp = pII;
+    double ksi = lev_f(k, enI, pI, rI, enII, pII, rII, kl);
+    double ksir = lev_f(k, enI, pI, rI, enII, pII, rII, kp);
+    if (fabs(ksi) <= epsil) return kl;
+    if (fabs(ksir) <= epsil) return kp;
+    for (int it = 1;; ++it) {
+        double kc = (kl + kp) / 2.0;
+        double ksit = lev_f(k, enI, pI, rI, enII, pII, rII, kc);
+        if (fabs(ksit) <= epsil) return kc;
+        if (it >= kBisectCap) {
+            if (diag) atomicAdd(&diag->bisect_cap, 1ull);
+            return kc;
+        }
+        if ((ksi * ksit) <= 0.0) {
+            kp = kc;
+        } else {
+            kl = kc;
+            ksi = ksit;
+        }
+    }
+}
+
+__device__ inline double newton(const GConst& k, double enI, double pI, double rI, double enII, double pII,
+                                double rII, DeviceDiag* diag) {
+    double cI = sqrt(k.ga * pI / rI);
+    double cII = sqrt(k.ga * pII / rII);
+    double pn = pI * rII * cII + pII * rI * cI + (enI - enII) * rI * cI * rII * cII;
+    pn = pn / (rI * cI + rII * cII);
+    const double pee = pn;
+    for (int kiter = 1;; ++kiter) {
+        double p = pn;
+        double fI = (p - pI) / (rI * cI * sqrt(k.gp * p / pI + k.gm));
+        double fIs = (k.ga + 1.0) * p / pI + (3.0 * k.ga - 1.0);
+        fIs = fIs / (4.0 * k.ga * rI * cI * pow((k.gp * p / pI + k.gm), (3.0 / 2.0)));
+        double fII = (p - pII) / (rII * cII * sqrt(k.gp * p / pII + k.gm));
+        double fIIs = (k.ga + 1.0) * p / pII + (3.0 * k.ga - 1.0);
+        fIIs = fIIs / (4.0 * k.ga * rII * cII * pow((k.gp * p / pII + k.gm), (3.0 / 2.0)));
+        pn = p - (fI + fII - (enI - enII)) / (fIs + fIIs);
+        if (!(fabs(pn / pee - p / pee) >= 1e-10)) break;
+        if (kiter >= kNewtonCap) {
+            if (diag) atomicAdd(&diag->newton_cap, 1ull);
+            break;
+        }
+    }
+    return pn;
+}
+
+__device__ __forceinline__ double fan_a(const GConst& k, double p, double pK, double rK, double cK) {
+    if (fabs(p - pK) < 1e-10) return rK * cK;
+    return k.gm * rK * cK * (1.0 - p / pK) / (1.0 - pow((p / pK), k.gm));
+}
+
+// Exact-Riemann flux in the face frame (normal along +n, tangential ut).
+// Returns the face time step numerator factor: UU (the largest wave-speed
+// magnitude, clamped below by 1e-8 as in kernel.cu:331-339) through smax.
+__device__ inline FluxE exact_euler(double rL, double unL, double utL, double pL,
+                                    double rR, double unR, double utR, double pR,
+                                    double gamma, double& smax, DeviceDiag* diag) {
+    const GConst k = make_gconst(gamma);
+    double enI = unL, teI = utL, pI = pL, rI = rL;
+    double enII = unR, teII = utR, pII = pR, rII = rR;
+    bool ipiz = false;
+    if (pI > pII) {  // kernel.cu:243-270
+        enI = -unR; teI = utR; pI = pR; rI = rR;
+        enII = -unL; teII = utL; pII = pL; rII = rL;
+        ipiz = true;
+    }
+    double cI = 0.0, cII = 0.0;
+    if (rI != 0.0) cI = sqrt(k.ga * pI / rI);
+    if (rII != 0.0) cII = sqrt(k.ga * pII / rII);
+    const double a = sqrt(rI * (k.g2 * pII + k.g1 * pI) / 2.0);
+    const double Uud = (pII - pI) / a;
+    const double Urz = -2.0 * cII / k.g1 * (1.0 - pow((pI / pII), k.gm));
+    const double Uvk = -2.0 * (cII + cI) / k.g1;
+    const double Udf = enI - enII;
+    int il = -2, ip = -2;
+    double p = 0.0;
+    if (Udf < Uvk) {
+        il = -1; ip = -1;
+    } else if ((Udf >= Uvk) && (Udf <= Urz)) {
+        p = pI * pow(((Udf - Uvk) / (Urz - Uvk)), (1.0 / k.gm));
+        il = 0; ip = 0;
+    } else if ((Udf > Urz) && (Udf <= Uud)) {
+        p = devtwo(k, enI, pI, rI, enII, pII, rII, diag);
+        il = 1; ip = 0;
+    } else if (Udf > Uud) {
+        p = newton(k, enI, pI, rI, enII, pII, rII, diag);
+        il = 1; ip = 1;
+    }
+    const double w = 0.0;
+    double r = 0.0, te = 0.0, en = 0.0, UU = 0.0;
+    if (il == 1 && ip == 1) {  // two shocks, kernel.cu:320-374
+        double aI = sqrt(rI * (k.g2 / 2.0 * p + k.g1 / 2.0 * pI));
+        double aII = sqrt(rII * (k.g2 / 2.0 * p + k.g1 / 2.0 * pII));
+        double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        double dI = enI - aI / rI;
+        double dII = enII + aII / rII;
+        UU = fmax(fabs(dI), fabs(dII));
+        if (w <= dI) { en = enI; p = pI; r = rI; te = teI; }
+        else if ((w > dI) && (w <= u)) { en = u; r = rI * aI / (aI - rI * (enI - u)); te = teI; }
+        else if ((w > u) && (w < dII)) { en = u; r = rII * aII / (aII + rII * (enII - u)); te = teII; }
+        else if (w >= dII) { en = enII; p = pII; r = rII; te = teII; }
+    } else if (il == 1 && ip == 0) {  // shock + fan, kernel.cu:378-450
+        double aI = sqrt(rI * (k.g2 / 2.0 * p + k.g1 / 2.0 * pI));
+        double aII = fan_a(k, p, pII, rII, cII);
+        double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        double dI = enI - aI / rI;
+        double dII = enII + cII;
+        double ddII = u + cII - k.g1 * (enII - u) / 2.0;
+        UU = fmax(fmax(fabs(dI), fabs(dII)), fabs(ddII));
+        if (w <= dI) { en = enI; p = pI; r = rI; te = teI; }
+        if ((w > dI) && (w <= u)) { en = u; r = rI * aI / (aI - rI * (enI - u)); te = teI; }
+        if ((w > u) && (w <= ddII)) {
+            double ce = cII - k.g1 / 2.0 * (enII - u);
+            en = u; r = k.ga * p / ce / ce; te = teII;
+        }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -k.g1 / k.g2 * (enII - w) + 2.0 / k.g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / k.gm)); r = k.ga * p / ce / ce; te = teII;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te = teII; }
+    } else if (il == 0 && ip == 0) {  // two fans, kernel.cu:452-563
+        double aI = fan_a(k, p, pI, rI, cI);
+        double aII = fan_a(k, p, pII, rII, cII);
+        double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        double dI = enI - cI;
+        double ddI = u - cI - k.g1 * (enI - u) / 2.0;
+        double dII = enII + cII;
+        double ddII = u + cII - k.g1 * (enII - u) / 2.0;
+        UU = fmax(fmax(fmax(fabs(dI), fabs(dII)), fabs(ddII)), fabs(ddI));
+        if (w <= dI) { en = enI; p = pI; r = rI; te = teI; }
+        if ((w > dI) && (w < ddI)) {
+            double ce = k.g1 / k.g2 * (enI - w) + 2.0 / k.g2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / k.gm)); r = k.ga * p / ce / ce; te = teI;
+        }
+        if ((w >= ddI) && (w <= u)) {
+            double ce = cI + k.g1 / 2.0 * (enI - u);
+            en = u; r = k.ga * p / ce / ce; te = teI;
+        }
+        if ((w > u) && (w <= ddII)) {
+            double ce = cII - k.g1 / 2.0 * (enII - u);
+            en = u; r = k.ga * p / ce / ce; te = teII;
+        }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -k.g1 / k.g2 * (enII - w) + 2.0 / k.g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / k.gm)); r = k.ga * p / ce / ce; te = teII;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te = teII; }
+    } else if (il == -1 && ip == -1) {  // vacuum, kernel.cu:566-630
+        double dI = enI - cI;
+        double ddI = enI + 2.0 / k.g1 * cI;
+        double dII = enII + cII;
+        double ddII = enII - 2.0 / k.g1 * cII;
+        UU = fmax(fmax(fmax(fabs(dI), fabs(dII)), fabs(ddII)), fabs(ddI));
+        if (w <= dI) { en = enI; p = pI; r = rI; te = teI; }
+        if ((w > dI) && (w < ddI)) {
+            double ce = k.g1 / k.g2 * (enI - w) + 2.0 / k.g2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / k.gm)); r = k.ga * p / ce / ce; te = teI;
+        }
+        if ((w >= ddI) && (w <= ddII)) { en = w; p = 0.0; r = 0.0; te = 0.0; }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -k.g1 / k.g2 * (enII - w) + 2.0 / k.g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / k.gm)); r = k.ga * p / ce / ce; te = teII;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te = teII; }
+    } else {
+        smax = __builtin_nan("");
+        FluxE bad = {smax, smax, smax, smax};
+        return bad;
+    }
+    smax = UU > 1e-8 ? UU : 1e-8;
+    if (ipiz) en = -en;
+    // kernel.cu:639-651 in the face frame (the third frame vector is zero)
+    const double eo = p / k.g1 + 0.5 * r * (en * en + te * te);
+    FluxE F;
+    F.fr = r * en;
+    F.fn = r * en * en + p;
+    F.ft = r * en * te;
+    F.fe = en * eo + en * p;
+    return F;
+}
+
+// Wave-wide max of a non-negative double (DPP/permute butterfly).
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Bit patterns of non-negative doubles order like the doubles, so a device
+// max / min over them is an exact, order-independent reduction.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v) {
+    atomicMax(dst, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ void atomic_min_nonneg(unsigned long long* dst, double v) {
+    atomicMin(dst, (unsigned long long)__double_as_longlong(v));
+}
+
+// XCD-aware tile order: the hardware hands consecutive workgroups to the 8
+// XCDs in turn; remap so that each XCD walks a contiguous range of tiles (its
+// neighbours' halo rows and columns then sit in the same L2).
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    constexpr int kXcd = 8;
+    const int xcd = b % kXcd, local = b / kXcd;
+    const int per = nblocks / kXcd, rem = nblocks % kXcd;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+}  // namespace gd2

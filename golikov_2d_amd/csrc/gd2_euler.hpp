@@ -1,0 +1,364 @@
+// gd2_euler.hpp -- fused Euler stage kernel for gfx950.
+//
+// One 64-lane wave owns a strip of 63 interior columns and marches up a run
+// of rows. Lane l holds column cbase + l:
+//   * y-direction: the lane keeps rows j, j+1, j+2 of its own column in
+//     registers, so every y-face flux G(j+1/2) is computed once and carried
+//     to the next row;
+//   * x-direction: each loaded row (cbase-2 .. cbase+64, converted to
+//     primitives once) goes to a 3-row LDS ring; lane l computes the x-face
+//     F(c-1/2) from LDS and receives F(c+1/2) from lane l+1 (lane 63 only
+//     supplies that face), so every x-face is also computed once.
+// A stage reads each state value of the strip once from HBM (plus a 2-cell
+// halo) and writes each output once: reconstruct -> flux -> update fused.
+//
+// Layouts: kSoA = native SoA conserved planes with an optional RK
+// combination and wave-speed epilogue; kAoS = the reference AoS primitive
+// layout with its primitive-form update (kernel.cu:1859-1874), its lagged
+// face time step and frozen edge cells.
+#pragma once
+
+#include "gd2_device.hpp"
+
+namespace gd2 {
+
+constexpr int kTileW = 63;   // interior columns per wave
+constexpr int kRowBuf = 68;  // LDS row: columns cbase-2 .. cbase+64 (67 used)
+
+enum { kSoA = 0, kAoS = 1 };
+
+struct EulerArgs {
+    // state
+    const double* Uin;     // kSoA: 4 planes
+    const double* Ubase;   // kSoA: 4 planes or null (a == 0)
+    double* Uout;          // kSoA: 4 planes
+    const double2* s;      // kAoS: (rho, p)
+    const double2* u;      // kAoS: (vx, vy)
+    double2* s2;
+    double2* u2;
+    // grid
+    int nx, ny;
+    long long plane;
+    double dx, dy, idx, idy;
+    double gamma, g1, inv_g1, krit;
+    int bc[4];
+    // time stepping
+    double a, b;
+    int plain;                     // a == 0 && b == 1
+    double dt;                     // used when dt_dev == null
+    const double* dt_dev;
+    unsigned long long* wmax_bits; // kSoA: wave-speed max of the output (may be null)
+    unsigned long long* tmin_bits; // kAoS: face time-step min (the reference's *T)
+    DeviceDiag* diag;
+    // tiling
+    int tiles_x, rows;
+};
+
+struct Slope {
+    double r, u, v, p;
+};
+
+template <int LAYOUT>
+struct RowVal {
+    PrimE q;
+    double U[4];  // conserved (kSoA) or unused
+};
+
+template <int LAYOUT>
+__device__ __forceinline__ RowVal<LAYOUT> load_cell(const EulerArgs& A, long long k, bool fx, bool fy) {
+    RowVal<LAYOUT> v;
+    if constexpr (LAYOUT == kSoA) {
+        const long long P = A.plane;
+        v.U[0] = A.Uin[k];
+        v.U[1] = A.Uin[P + k];
+        v.U[2] = A.Uin[2 * P + k];
+        v.U[3] = A.Uin[3 * P + k];
+        v.q = cons_to_prim_e(v.U[0], v.U[1], v.U[2], v.U[3], A.g1);
+    } else {
+        const double2 sv = A.s[k];
+        const double2 uv = A.u[k];
+        v.q.r = sv.x;
+        v.q.p = sv.y;
+        v.q.u = uv.x;
+        v.q.v = uv.y;
+        v.U[0] = v.U[1] = v.U[2] = v.U[3] = 0.0;
+    }
+    if (fx) v.q.u = -v.q.u;
+    if (fy) v.q.v = -v.q.v;
+    return v;
+}
+
+__device__ __forceinline__ void lds_put(double (*ring)[4][kRowBuf], int slot, int e, const PrimE& q) {
+    ring[slot][0][e] = q.r;
+    ring[slot][1][e] = q.u;
+    ring[slot][2][e] = q.v;
+    ring[slot][3][e] = q.p;
+}
+
+template <int FLUX>
+__device__ __forceinline__ FluxE face_flux(const EulerArgs& A, double rL, double unL, double utL, double pL,
+                                           double rR, double unR, double utR, double pR, double& smax) {
+    if constexpr (FLUX == GD2_FLUX_EXACT)
+        return exact_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, smax, A.diag);
+    else
+        return hllc_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
+}
+
+// minmod slope of the middle value; half-slope face values with the
+// reference's rho/p <= 0 fall-back to the cell value (grad_p, kernel.cu:28,
+// 3436-3544).
+template <int ORDER>
+__device__ __forceinline__ Slope slope3(const PrimE& a, const PrimE& b, const PrimE& c) {
+    Slope s;
+    if constexpr (ORDER == 2) {
+        s.r = minmod(b.r - a.r, c.r - b.r);
+        s.u = minmod(b.u - a.u, c.u - b.u);
+        s.v = minmod(b.v - a.v, c.v - b.v);
+        s.p = minmod(b.p - a.p, c.p - b.p);
+    } else {
+        s.r = s.u = s.v = s.p = 0.0;
+    }
+    return s;
+}
+
+__device__ __forceinline__ PrimE face_plus(const PrimE& q, const Slope& s) {
+    PrimE f;
+    f.r = q.r + 0.5 * s.r;
+    f.u = q.u + 0.5 * s.u;
+    f.v = q.v + 0.5 * s.v;
+    f.p = q.p + 0.5 * s.p;
+    if (f.r <= 0.0) f.r = q.r;
+    if (f.p <= 0.0) f.p = q.p;
+    return f;
+}
+
+__device__ __forceinline__ PrimE face_minus(const PrimE& q, const Slope& s) {
+    PrimE f;
+    f.r = q.r - 0.5 * s.r;
+    f.u = q.u - 0.5 * s.u;
+    f.v = q.v - 0.5 * s.v;
+    f.p = q.p - 0.5 * s.p;
+    if (f.r <= 0.0) f.r = q.r;
+    if (f.p <= 0.0) f.p = q.p;
+    return f;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
+    if constexpr (LAYOUT == kAoS) {
+        return (A.bc[0] == GD2_BC_FROZEN && c == 0) || (A.bc[1] == GD2_BC_FROZEN && c == A.nx - 1) ||
+               (A.bc[2] == GD2_BC_FROZEN && j == 0) || (A.bc[3] == GD2_BC_FROZEN && j == A.ny - 1);
+    } else {
+        return false;
+    }
+}
+
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
+__global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
+    __shared__ double ring[3][4][kRowBuf];
+
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    const int cbase = tx * kTileW;
+    const int j0 = ty * A.rows;
+    const int j1 = min(j0 + A.rows, A.ny);
+    const int c = cbase + lane;
+
+    bool fx, hfx;
+    const int ci = map_index(c, A.nx, A.bc[0], A.bc[1], fx);
+    const int hcol = lane == 0 ? cbase - 2 : (lane == 1 ? cbase - 1 : cbase + 64);
+    const int hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
+    const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
+    const bool has_halo = lane < 3;
+    const bool owns = (lane < kTileW) && (c < A.nx);
+
+    const double dt = A.dt_dev ? *A.dt_dev : A.dt;
+    const double dx = A.dx, dy = A.dy;
+    const double dtdV = dt / (dx * dy);
+    // lagged face time step radius: dy on x-faces and dx on y-faces as in
+    // Ker_Dekard / Kernel_TVD_old (kernel.cu:1798-1801, 3549-3552)
+    const double krx = A.krit * dy, kry = A.krit * dx;
+
+    // load one logical row jl into registers (own column) and the LDS ring
+    auto load_row = [&](int jl, int slot, bool to_lds) -> RowVal<LAYOUT> {
+        bool fy;
+        const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
+        const long long rowk = (long long)jj * A.nx;
+        RowVal<LAYOUT> v = load_cell<LAYOUT>(A, rowk + ci, fx, fy);
+        if (to_lds) {
+            lds_put(ring, slot, lane + 2, v.q);
+            if (has_halo) {
+                RowVal<LAYOUT> h = load_cell<LAYOUT>(A, rowk + hci, hfx, fy);
+                lds_put(ring, slot, hslot, h.q);
+            }
+        }
+        return v;
+    };
+
+    // prologue: rows j0-2, j0-1 (y-stencil only), j0, j0+1 (also in LDS)
+    RowVal<LAYOUT> vm2 = load_row(j0 - 2, 0, false);
+    RowVal<LAYOUT> vm1 = load_row(j0 - 1, 0, false);
+    RowVal<LAYOUT> v0 = load_row(j0, j0 % 3, true);
+    RowVal<LAYOUT> v1 = load_row(j0 + 1, (j0 + 1) % 3, true);
+
+    Slope sm1 = slope3<ORDER>(vm2.q, vm1.q, v0.q);
+    Slope s0 = slope3<ORDER>(vm1.q, v0.q, v1.q);
+    double smax;
+    FluxE Gm;  // y-face flux G(j-1/2), frame (n = +y, t = +x)
+    {
+        PrimE L = face_plus(vm1.q, sm1), R = face_minus(v0.q, s0);
+        Gm = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
+    }
+    double tGm = kry / smax;
+
+    double wmax = 0.0;
+    double tmin = 1e300;
+
+    for (int j = j0; j < j1; ++j) {
+        __syncthreads();  // slot (j+2)%3 = slot of row j-1, whose x-faces are done
+        RowVal<LAYOUT> v2 = load_row(j + 2, (j + 2) % 3, true);
+        // y-face G(j+1/2)
+        Slope s1 = slope3<ORDER>(v0.q, v1.q, v2.q);
+        FluxE Gp;
+        double tGp;
+        {
+            PrimE L = face_plus(v0.q, s0), R = face_minus(v1.q, s1);
+            Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
+            tGp = kry / smax;
+        }
+        // row j reached LDS two iterations ago (before two barriers)
+        // x-face F(c-1/2) of row j from cells c-2, c-1, c, c+1
+        FluxE Fl;
+        double tFl;
+        {
+            const int sl = j % 3;
+            PrimE qa, qb, qd;
+            qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
+            qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
+            qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
+            const PrimE& qc = v0.q;
+            Slope sb = slope3<ORDER>(qa, qb, qc);
+            Slope sc = slope3<ORDER>(qb, qc, qd);
+            PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
+            Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+            tFl = krx / smax;
+        }
+        // F(c+1/2) from the next lane
+        FluxE Fr;
+        Fr.fr = __shfl_down(Fl.fr, 1, kWave);
+        Fr.fn = __shfl_down(Fl.fn, 1, kWave);
+        Fr.ft = __shfl_down(Fl.ft, 1, kWave);
+        Fr.fe = __shfl_down(Fl.fe, 1, kWave);
+        const double tFr = __shfl_down(tFl, 1, kWave);
+
+        if (owns && !frozen_cell<LAYOUT>(A, c, j)) {
+            // outward-normal sum with face lengths, in the order of
+            // kernel.cu:1854-1857 (east, south, west, north)
+            const double PSr = ((Fr.fr * dy + (-Gm.fr) * dx) + (-Fl.fr) * dy) + Gp.fr * dx;
+            const double PSe = ((Fr.fe * dy + (-Gm.fe) * dx) + (-Fl.fe) * dy) + Gp.fe * dx;
+            const double PUx = ((Fr.fn * dy + (-Gm.ft) * dx) + (-Fl.fn) * dy) + Gp.ft * dx;
+            const double PUy = ((Fr.ft * dy + (-Gm.fn) * dx) + (-Fl.ft) * dy) + Gp.fn * dx;
+            const long long k = (long long)j * A.nx + c;
+            if constexpr (LAYOUT == kSoA) {
+                double o0 = v0.U[0] - dtdV * PSr;
+                double o1 = v0.U[1] - dtdV * PUx;
+                double o2 = v0.U[2] - dtdV * PUy;
+                double o3 = v0.U[3] - dtdV * PSe;
+                if (!A.plain) {
+                    const long long P = A.plane;
+                    o0 = A.a * A.Ubase[k] + A.b * o0;
+                    o1 = A.a * A.Ubase[P + k] + A.b * o1;
+                    o2 = A.a * A.Ubase[2 * P + k] + A.b * o2;
+                    o3 = A.a * A.Ubase[3 * P + k] + A.b * o3;
+                }
+                // clamps (kernel.cu:1862-1866, 1871-1874) in conserved form
+                if (o0 <= 0.0) {
+                    atomicAdd(&A.diag->clamp_rho, 1ull);
+                    o0 = 0.0001;
+                }
+                const double inv = 1.0 / o0;
+                const double ux = o1 * inv, uy = o2 * inv;
+                const double ke = 0.5 * (o1 * ux + o2 * uy);
+                double p = A.g1 * (o3 - ke);
+                if (p <= 0.0) {
+                    atomicAdd(&A.diag->clamp_p, 1ull);
+                    o3 = 0.000001 / A.g1 + ke;
+                    p = A.g1 * (o3 - ke);
+                }
+                const long long P = A.plane;
+                A.Uout[k] = o0;
+                A.Uout[P + k] = o1;
+                A.Uout[2 * P + k] = o2;
+                A.Uout[3 * P + k] = o3;
+                if constexpr (WANT_W) {
+                    const double cs = sqrt(A.gamma * p / o0);
+                    const double w = (fabs(ux) + cs) * A.idx + (fabs(uy) + cs) * A.idy;
+                    wmax = fmax(wmax, w);
+                }
+            } else {
+                // primitive-form update of Ker_Dekard (kernel.cu:1859-1874)
+                const PrimE& q = v0.q;
+                const double gm1 = A.gamma - 1.0;
+                const double Tdv = dt / (dx * dy);
+                double r2 = q.r - Tdv * PSr;
+                if (r2 <= 0.0) {
+                    atomicAdd(&A.diag->clamp_rho, 1ull);
+                    r2 = 0.0001;
+                }
+                const double ux = (q.r * q.u - Tdv * PUx) / r2;
+                const double uy = (q.r * q.v - Tdv * PUy) / r2;
+                double p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe) -
+                             0.5 * r2 * (ux * ux + uy * uy)) * gm1;
+                if (p2 <= 0.0) {
+                    atomicAdd(&A.diag->clamp_p, 1ull);
+                    p2 = 0.000001;
+                }
+                A.s2[k] = make_double2(r2, p2);
+                A.u2[k] = make_double2(ux, uy);
+                // per-cell tmin starts at 1000 (kernel.cu:1748)
+                tmin = fmin(tmin, fmin(1000.0, fmin(fmin(tFr, tGm), fmin(tFl, tGp))));
+            }
+        } else if constexpr (LAYOUT == kAoS) {
+            if (owns) {  // frozen edge cell: carried over unchanged
+                const long long k = (long long)j * A.nx + c;
+                A.s2[k] = A.s[k];
+                A.u2[k] = A.u[k];
+            }
+        }
+
+        // advance the y-window
+        v0 = v1;
+        v1 = v2;
+        s0 = s1;
+        Gm = Gp;
+        tGm = tGp;
+    }
+
+    if constexpr (LAYOUT == kSoA && WANT_W) {
+        const double w = wave_max(wmax);
+        if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
+    }
+    if constexpr (LAYOUT == kAoS) {
+        const double t = wave_min(tmin);
+        if (lane == 0 && t < 1e300) atomic_min_nonneg(A.tmin_bits, t);
+    }
+}
+
+// Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA
+// Euler state, folded into *wmax_bits.
+__global__ __launch_bounds__(256) void euler_wavespeed_kernel(const double* U, long long n, long long plane,
+                                                              double gamma, double g1, double idx, double idy,
+                                                              unsigned long long* wmax_bits) {
+    double w = 0.0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        PrimE q = cons_to_prim_e(U[k], U[plane + k], U[2 * plane + k], U[3 * plane + k], g1);
+        const double cs = sqrt(gamma * q.p / q.r);
+        w = fmax(w, (fabs(q.u) + cs) * idx + (fabs(q.v) + cs) * idy);
+    }
+    w = wave_max(w);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomic_max_nonneg(wmax_bits, w);
+}
+
+}  // namespace gd2

@@ -1,0 +1,175 @@
+/*
+ * gd2.h -- C ABI of libgd2.so, the MI355X (gfx950) explicit finite-volume
+ * step for 2-D Euler and ideal MHD on a structured grid.
+ *
+ * Plain C: pointers, sizes and an opaque stream handle (a hipStream_t passed as
+ * void*; NULL = the default stream). Every device pointer is caller-owned
+ * device memory; no entry point allocates except gd2_ctx_create. Entry points
+ * are stream-ordered and never synchronise the host, except gd2_ctx_time and
+ * gd2_get_diagnostics, which say so.
+ *
+ * Return value: GD2_OK (0) or a negative gd2_status.
+ *
+ * State layouts
+ *   reference layout (drop-in for the reference kernels): row-major AoS
+ *     primitives, index k = m*nx + n (n along x, fastest):
+ *       s[k] = (rho, p), u[k] = (vx, vy), b[k] = (Bx, By, Bz) Gaussian B
+ *     (kernel.cu:1738-1740, 3812-3847 of the reference).
+ *   native layout: SoA conserved planes, nfields x ny x nx doubles:
+ *       Euler: rho, rho*vx, rho*vy, E
+ *       MHD:   rho, rho*vx, rho*vy, Bx, By, Bz, E   (Gaussian B,
+ *              E = p/(gamma-1) + rho|v|^2/2 + |B|^2/(8 pi))
+ */
+#ifndef GD2_H
+#define GD2_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double x, y; } gd2_double2;     /* layout of double2 */
+typedef struct { double x, y, z; } gd2_double3;  /* layout of double3 */
+
+enum gd2_status {
+    GD2_OK = 0,
+    GD2_ERR_ARG = -1,          /* bad pointer, size or enum value */
+    GD2_ERR_UNSUPPORTED = -2,  /* method / flux / boundary kind not offered by this entry point */
+    GD2_ERR_HIP = -3,          /* a HIP runtime call failed (see gd2_last_hip_error) */
+    GD2_ERR_NOMEM = -4
+};
+
+/* Boundary kind of one side. The reference's transmissive sides copy the edge
+ * cell (kernel.cu:1762-1793); its "hard" sides keep the edge cells frozen
+ * (kernel.cu:1752-1758); its symmetry sides mirror with the normal velocity
+ * (and normal B) negated (kernel.cu:2263-2277). Periodic has no reference
+ * counterpart. FROZEN is accepted by the reference-layout entry points only. */
+enum gd2_bc { GD2_BC_TRANSMISSIVE = 0, GD2_BC_PERIODIC = 1, GD2_BC_REFLECTIVE = 2, GD2_BC_FROZEN = 3 };
+
+/* Grid and physics descriptor. The reference has all of these as
+ * compile-time macros: N, M, dx, dy, x_min, y_min (kernel.cu:12-20), gamma
+ * (Header.h:4-5) and the CFL factor krit (Header.h:3). */
+typedef struct gd2_grid {
+    int nx, ny;          /* cells along x (N) and y (M) */
+    double dx, dy;       /* cell sizes */
+    double x0, y0;       /* centre of cell (0,0): x_min, y_min */
+    double gamma;        /* ratio of specific heats */
+    double krit;         /* CFL factor of the reference-layout steps' face time step */
+    int bc[4];           /* west, east, south, north: enum gd2_bc */
+} gd2_grid;
+
+/* ----------------------------------------------------------------------
+ * Reference-layout entry points: the reference's __global__ step kernels as
+ * host-callable functions with the same argument order, plus a leading grid
+ * descriptor and a trailing stream. Each launches one step:
+ * (s,u[,b]) -> (s2,u2[,b2]) using the time step *T_do, and folds the smallest
+ * face time step krit*rad/max(|S_L|,|S_R|) of this step into *T with an exact
+ * (deterministic) device-wide minimum. As in the reference, the step used is
+ * the one computed in the previous step (lagged); gd2_funk_time rotates it.
+ * Cartesian forms: the reference's axisymmetric 1/y source terms and its
+ * problem-specific inner-sphere and inflow cells are not part of these.
+ * ---------------------------------------------------------------------- */
+
+/* Euler, first order. Replaces Ker_Dekard (kernel.cu:1736).
+ * method 1 = HLLC (HLLC_Aleksashov, kernel.cu:853), 2 = exact Riemann
+ * (Godunov_Solver_Alexashov, kernel.cu:214). Face time step radius as in the
+ * reference: dy on x-faces, dx on y-faces (kernel.cu:1798-1801). */
+int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                   gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
+                   int method, void* stream);
+
+/* Euler, minmod MUSCL on primitives. Replaces the Cartesian part of
+ * Kernel_TVD_old (kernel.cu:3286); method as for gd2_ker_dekard. */
+int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                       gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
+                       int method, void* stream);
+
+/* MHD (vz = 0), first order, Powell 8-wave source. Replaces the Cartesian part
+ * of add2 (kernel.cu:1877). method 2 = HLLC-MHD, 3 = HLLD (HLLDQ_Korolkov
+ * metod 2 / 3, Gas_Dinamic.cu:964). i is the step counter (unused, as in the
+ * reference). */
+int gd2_add2(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+             gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
+             int i, int method, void* stream);
+
+/* MHD, minmod MUSCL on the 7 primitives + Powell source. Replaces the
+ * Cartesian part of Kernel_TVD (kernel.cu:2198). */
+int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                   gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
+                   int i, int method, void* stream);
+
+/* Time-step rotation, replaces funk_time (kernel.cu:1722):
+ * *T_do = *T; *TT += *T_do; *T = 1e7; ++*i. All four are device pointers. */
+int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);
+
+/* ----------------------------------------------------------------------
+ * Native path: SoA conserved state, fused reconstruct->flux->update stages,
+ * cell-centred CFL step computed on the device.
+ * ---------------------------------------------------------------------- */
+enum gd2_physics { GD2_EULER = 0, GD2_MHD = 1 };
+enum gd2_flux { GD2_FLUX_HLLC = 1, GD2_FLUX_EXACT = 2, GD2_FLUX_HLLC_MHD = 3, GD2_FLUX_HLLD = 4 };
+enum gd2_integrator { GD2_FORWARD_EULER = 1, GD2_SSPRK2 = 2 };
+
+typedef struct gd2_scheme {
+    int physics;      /* enum gd2_physics */
+    int flux;         /* enum gd2_flux (HLLC/EXACT for Euler, HLLC_MHD/HLLD for MHD) */
+    int order;        /* 1 = piecewise constant, 2 = minmod MUSCL on primitives */
+    int integrator;   /* enum gd2_integrator */
+    double cfl;       /* dt = cfl / max_cells((|u|+c)/dx + (|v|+c)/dy) */
+} gd2_scheme;
+
+/* Number of conserved fields of a physics kind (4 or 7). */
+int gd2_nfields(int physics);
+
+/* One stage on caller-owned SoA device buffers:
+ *   Uout = a*Ubase + b*(Uin - dt/(dx dy) * sum of face fluxes(Uin))
+ * (a = 0, b = 1 is a forward-Euler step; Ubase may then be NULL; Uout may
+ * alias Ubase but not Uin). dt is taken from *dt_dev when dt_dev != NULL,
+ * else from dt. When wmax_bits != NULL (device, 8 bytes) the cell-centred
+ * wave-speed bound of Uout is max-folded into it as the bit pattern of a
+ * non-negative double. Clamp hits (rho <= 0, p <= 0) are counted in the
+ * library's device diagnostics. */
+int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const double* Ubase,
+              double* Uout, double a, double b, double dt, const double* dt_dev,
+              unsigned long long* wmax_bits, void* stream);
+
+/* max over cells of (|u|+c)/dx + (|v|+c)/dy of U, folded into *wmax_bits
+ * (device) as above; the caller zeroes *wmax_bits first. */
+int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U,
+                      unsigned long long* wmax_bits, void* stream);
+
+/* Solver context: owns one scratch state and a device clock. */
+typedef struct gd2_ctx gd2_ctx;
+
+int gd2_ctx_create(const gd2_grid* g, const gd2_scheme* sc, gd2_ctx** out);
+int gd2_ctx_destroy(gd2_ctx* ctx);
+/* Set the clock to time t0 and compute the first dt from U (device). */
+int gd2_ctx_start(gd2_ctx* ctx, const double* U, double t0, void* stream);
+/* Advance the device state U in place by nsteps steps. Each step takes
+ * dt = min(cfl / wmax(U^n), t_end - t) when t_end > 0, else cfl / wmax(U^n);
+ * steps requested after t reached t_end are exact no-ops. No host sync. */
+int gd2_ctx_advance(gd2_ctx* ctx, double* U, int nsteps, double t_end, void* stream);
+/* Synchronises the stream and reads the clock: time, steps taken, last dt. */
+int gd2_ctx_time(gd2_ctx* ctx, double* t, long long* steps, double* dt, void* stream);
+
+/* Layout conversion between the reference AoS primitives and SoA conserved. */
+int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
+                     const gd2_double3* b, double* U, void* stream);
+int gd2_cons_to_prim(const gd2_grid* g, int physics, const double* U, gd2_double2* s,
+                     gd2_double2* u, gd2_double3* b, void* stream);
+
+/* Device diagnostics: clamp hits (rho, p) and exact-solver iteration-cap hits
+ * (newton, bisection). Synchronises the device. */
+typedef struct gd2_diag {
+    long long clamp_rho, clamp_p, newton_cap_hits, bisect_cap_hits;
+} gd2_diag;
+int gd2_get_diagnostics(gd2_diag* out, int reset);
+
+/* Last HIP error code seen by the library (hipError_t value). */
+int gd2_last_hip_error(void);
+/* Library version string. */
+const char* gd2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GD2_H */

@@ -1,0 +1,206 @@
+"""GPU parity of the Euler path through the C ABI against the CPU oracle.
+
+Bar (per conserved field, relative L1 = sum|gpu - oracle| / sum|oracle|):
+one stage or step from identical inputs <= 1e-12; 100 SSP-RK2 steps
+<= 1e-10; density and pressure positive; no clamp hits. The face time step
+of the reference-layout entry points and the CFL wave-speed maximum agree
+to 1e-14 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from golikov_2d_amd import api
+from golikov_2d_amd import problems as PB
+from oracle import oracle as O
+from tests.gpu_util import dev, grids, rel_l1, say, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-12
+RUN_TOL = 1e-10
+
+BCS = {
+    "transmissive": (O.BC_TRANSMISSIVE,) * 4,
+    "periodic": (O.BC_PERIODIC,) * 4,
+    "mixed": (O.BC_REFLECTIVE, O.BC_TRANSMISSIVE, O.BC_PERIODIC, O.BC_PERIODIC),
+}
+
+
+def oscheme(flux, order, cfl=0.4):
+    return O.Scheme(O.PHYS_EULER, flux, order, cfl)
+
+
+def pscheme(flux, order, integ=api.SSPRK2, cfl=0.4):
+    return api.make_scheme(api.EULER, flux, order, integ, cfl)
+
+
+@pytest.mark.parametrize("shape", [(130, 97), (63, 64), (64, 63), (1, 5), (200, 3)])
+@pytest.mark.parametrize("bc", list(BCS))
+@pytest.mark.parametrize("flux,order", [(api.FLUX_HLLC, 2), (api.FLUX_HLLC, 1), (api.FLUX_EXACT, 1),
+                                        (api.FLUX_EXACT, 2)])
+def test_stage_matches_oracle(shape, bc, flux, order):
+    nx, ny = shape
+    say(f"stage {nx}x{ny} bc={bc} flux={flux} order={order}")
+    g, og = grids(nx, ny, BCS[bc])
+    U = PB.four_quadrant(og, seed=nx * 1000 + ny)
+    dt = 0.2 / (nx + ny)
+    ref, wref = O.stage(og, oscheme(flux, order), U, None, 0.0, 1.0, dt, want_w=True)
+    Ud = to_dev(U)
+    out = torch.empty_like(Ud)
+    w = torch.zeros(1, dtype=torch.int64, device=dev())
+    api.stage(g, pscheme(flux, order), Ud, None, out, 0.0, 1.0, dt, wmax_bits=w)
+    got = to_host(out)
+    assert np.all(rel_l1(got, ref) <= STEP_TOL), rel_l1(got, ref)
+    assert abs(api.bits_to_double(w) - wref) <= 1e-14 * wref
+
+
+@pytest.mark.parametrize("bc", list(BCS))
+def test_ssprk2_second_stage_matches_oracle(bc):
+    say(f"ssprk2 stage 2 bc={bc}")
+    g, og = grids(150, 70, BCS[bc])
+    U0 = PB.four_quadrant(og, seed=21)
+    sc = oscheme(O.FLUX_HLLC, 2)
+    dt = 1e-3
+    U1 = O.stage(og, sc, U0, None, 0.0, 1.0, dt)
+    ref = O.stage(og, sc, U1, U0, 0.5, 0.5, dt)
+    U0d, U1d = to_dev(U0), to_dev(U1)
+    dtd = torch.tensor([dt], dtype=torch.float64, device=dev())
+    api.stage(g, pscheme(api.FLUX_HLLC, 2), U1d, U0d, U0d, 0.5, 0.5, dt_dev=dtd)  # in place on the base
+    assert np.all(rel_l1(to_host(U0d), ref) <= STEP_TOL)
+
+
+@pytest.mark.parametrize("n", [128, 160])
+def test_100_step_ssprk2_run_matches_oracle(n):
+    say(f"100-step run {n}^2")
+    g, og = grids(n, n, BCS["transmissive"])
+    U = PB.four_quadrant(og, seed=5)
+    Ud = to_dev(U)
+    O.diag_reset()
+    steps, t = O.run(og, oscheme(O.FLUX_HLLC, 2), 2, U, nsteps=100)
+    api.diagnostics(reset=True)
+    s = api.Solver(g, pscheme(api.FLUX_HLLC, 2))
+    s.start(Ud)
+    s.advance(Ud, 100)
+    tg, ng, _ = s.time()
+    got = to_host(Ud)
+    assert ng == steps == 100
+    assert abs(tg - t) <= 1e-12 * t
+    assert np.all(rel_l1(got, U) <= RUN_TOL), rel_l1(got, U)
+    rho = got[0]
+    p = 0.4 * (got[3] - 0.5 * (got[1] ** 2 + got[2] ** 2) / rho)
+    assert rho.min() > 0 and p.min() > 0
+    d = api.diagnostics()
+    assert d["clamp_rho"] == 0 and d["clamp_p"] == 0
+    s.close()
+
+
+def test_sod_godunov_to_t02_matches_oracle():
+    say("C0 Sod 200x200 exact Godunov to t=0.2")
+    g, og = grids(200, 200, BCS["transmissive"])
+    U = PB.sod_x(og)
+    Ud = to_dev(U)
+    steps, t = O.run(og, oscheme(O.FLUX_EXACT, 1), 1, U, t_end=0.2)
+    s = api.Solver(g, pscheme(api.FLUX_EXACT, 1, api.FORWARD_EULER))
+    s.start(Ud)
+    tg, ng = s.run_to(Ud, 0.2, chunk=64)
+    got = to_host(Ud)
+    assert ng == steps and tg == t == 0.2
+    assert np.all(rel_l1(got, U) <= RUN_TOL), rel_l1(got, U)
+    assert np.all(got == got[:, :1, :])  # y-invariant
+    d = api.diagnostics()
+    assert d["newton_cap_hits"] == 0 and d["bisect_cap_hits"] == 0
+    s.close()
+
+
+@pytest.mark.parametrize("kind,order", [("ker_dekard", 1), ("kernel_tvd_old", 2)])
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("bc", [(O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_TRANSMISSIVE, O.BC_FROZEN),
+                                (O.BC_REFLECTIVE, O.BC_PERIODIC, O.BC_TRANSMISSIVE, O.BC_TRANSMISSIVE)])
+def test_reference_layout_step_matches_oracle(kind, order, method, bc):
+    if bc[0] == O.BC_REFLECTIVE and bc[1] == O.BC_PERIODIC:
+        bc = (O.BC_REFLECTIVE, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE, O.BC_TRANSMISSIVE)
+    say(f"reference layout {kind} method={method} bc={bc}")
+    g, og = grids(131, 77, bc, gamma=5.0 / 3.0, krit=0.25)
+    U = PB.four_quadrant(og, seed=8)
+    s, u, _ = O.cons_to_prim(og, O.PHYS_EULER, U)
+    T0, Tdo = 1e7, 3e-4
+    s2r, u2r, Tr = O.ref_euler_step(kind, og, s, u, T0, Tdo, method)
+    sd, ud = to_dev(s), to_dev(u)
+    s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+    T = torch.tensor([T0], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([Tdo], dtype=torch.float64, device=dev())
+    fn = api.ker_dekard if kind == "ker_dekard" else api.kernel_tvd_old
+    fn(g, sd, ud, s2, u2, T, T_do, method)
+    s2g, u2g = to_host(s2), to_host(u2)
+    assert np.max(np.abs(s2g - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(u2g - u2r)) <= 1e-12
+    assert abs(T.item() - Tr) <= 1e-14 * Tr
+
+
+def test_funk_time_rotation():
+    say("funk_time")
+    T = torch.tensor([0.01], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([1e-8], dtype=torch.float64, device=dev())
+    TT = torch.tensor([0.5], dtype=torch.float64, device=dev())
+    i = torch.tensor([4], dtype=torch.int32, device=dev())
+    api.funk_time(T, T_do, TT, i)
+    torch.cuda.synchronize()
+    assert (T.item(), T_do.item(), TT.item(), i.item()) == O.funk_time(0.01, 1e-8, 0.5, 4)
+
+
+def test_layout_conversion_roundtrip():
+    say("prim<->cons")
+    g, og = grids(70, 40, BCS["periodic"])
+    U = PB.four_quadrant(og, seed=3)
+    s, u, _ = O.cons_to_prim(og, O.PHYS_EULER, U)
+    sd, ud = to_dev(s), to_dev(u)
+    Ud = api.prim_to_cons(g, api.EULER, sd, ud)
+    assert np.max(np.abs(to_host(Ud) - O.prim_to_cons(og, O.PHYS_EULER, s, u))) <= 1e-15 * 4
+    s2, u2, _ = api.cons_to_prim(g, api.EULER, to_dev(U))
+    assert np.array_equal(to_host(s2), s) and np.array_equal(to_host(u2), u)
+
+
+def test_full_size_stage_parity_4096():
+    """One HLLC-MUSCL stage at the 4096^2 benchmark size against the oracle."""
+    say("4096^2 stage parity")
+    n = 4096
+    g, og = grids(n, n, BCS["transmissive"])
+    U = PB.four_quadrant(og, seed=0)
+    dt = 0.4 / O.wavespeed_max(og, oscheme(O.FLUX_HLLC, 2), U)
+    ref = O.stage(og, oscheme(O.FLUX_HLLC, 2), U, None, 0.0, 1.0, dt)
+    Ud = to_dev(U)
+    out = torch.empty_like(Ud)
+    api.stage(g, pscheme(api.FLUX_HLLC, 2), Ud, None, out, 0.0, 1.0, dt)
+    got = to_host(out)
+    assert np.all(rel_l1(got, ref) <= STEP_TOL), rel_l1(got, ref)
+
+
+def test_full_size_invariants_4096():
+    """Size-independent properties at 4096^2: free stream is preserved exactly,
+    periodic totals are conserved, x-directed data stays y-invariant."""
+    n = 4096
+    say("4096^2 invariants")
+    g = api.make_grid(n, n, bc=(api.BC_PERIODIC,) * 4)
+    og = O.make_grid(n, n, bc=(O.BC_PERIODIC,) * 4)
+    U = to_dev(PB.uniform_euler(og, rho=0.8, p=1.1, u=0.3, v=-0.6))
+    s = api.Solver(g, pscheme(api.FLUX_HLLC, 2))
+    s.start(U)
+    U0 = U.clone()
+    s.advance(U, 3)
+    assert torch.equal(U, U0)
+    U = to_dev(PB.four_quadrant(og, seed=1))
+    tot0 = U.sum(dim=(1, 2))
+    s.start(U)
+    s.advance(U, 5)
+    tot1 = U.sum(dim=(1, 2))
+    assert torch.all((tot1 - tot0).abs() <= 1e-12 * U.abs().sum(dim=(1, 2)))
+    s.close()
+    g2 = api.make_grid(n, 64)
+    og2 = O.make_grid(n, 64)
+    U = to_dev(PB.sod_x(og2))
+    s2 = api.Solver(g2, pscheme(api.FLUX_HLLC, 2))
+    s2.start(U)
+    s2.advance(U, 10)
+    assert torch.equal(U, U[:, :1, :].expand_as(U))
+    s2.close()
