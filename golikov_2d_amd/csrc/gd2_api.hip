@@ -53,9 +53,11 @@ static bool grid_ok(const gd2_grid* g, bool allow_frozen) {
     return true;
 }
 
+int hip_error_code(hipError_t e) { return e == hipSuccess ? GD2_OK : hip_fail(e); }
+
 // Rows per wave strip. Default: long enough to amortise the 4-row y-halo,
 // short enough to give every CU several waves; GD2_TILE_ROWS overrides it.
-static int tile_rows(int nx, int ny) {
+int tile_rows_for(int nx, int ny) {
     static int env = -1;
     if (env < 0) {
         const char* e = std::getenv("GD2_TILE_ROWS");
@@ -83,7 +85,7 @@ static void fill_common(EulerArgs& A, const gd2_grid* g) {
     A.krit = g->krit;
     for (int s = 0; s < 4; ++s) A.bc[s] = g->bc[s];
     A.diag = diag_ptr();
-    A.rows = tile_rows(g->nx, g->ny);
+    A.rows = tile_rows_for(g->nx, g->ny);
     A.tiles_x = (g->nx + kTileW - 1) / kTileW;
 }
 
@@ -204,7 +206,7 @@ __global__ void cons_to_prim_kernel(const double* U, double2* s, double2* u, dou
 
 static unsigned ew_blocks(long long n) {
     long long b = (n + 255) / 256;
-    if (b > 65536) b = 65536;
+    if (b > 2048) b = 2048;
     if (b < 1) b = 1;
     return (unsigned)b;
 }

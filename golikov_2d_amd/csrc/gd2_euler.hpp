@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
         PrimE L = face_plus(vm1.q, sm1), R = face_minus(v0.q, s0);
         Gm = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
     }
-    double tGm = kry / smax;
+    double tGm = (LAYOUT == kAoS) ? kry / smax : 0.0;
 
     double wmax = 0.0;
     double tmin = 1e300;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
         {
             PrimE L = face_plus(v0.q, s0), R = face_minus(v1.q, s1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
-            tGp = kry / smax;
+            tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         }
         // row j reached LDS two iterations ago (before two barriers)
         // x-face F(c-1/2) of row j from cells c-2, c-1, c, c+1
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
             Slope sc = slope3<ORDER>(qb, qc, qd);
             PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
             Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
-            tFl = krx / smax;
+            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
         }
         // F(c+1/2) from the next lane
         FluxE Fr;
@@ -250,15 +250,16 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
         Fr.fn = __shfl_down(Fl.fn, 1, kWave);
         Fr.ft = __shfl_down(Fl.ft, 1, kWave);
         Fr.fe = __shfl_down(Fl.fe, 1, kWave);
-        const double tFr = __shfl_down(tFl, 1, kWave);
+        const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
 
         if (owns && !frozen_cell<LAYOUT>(A, c, j)) {
-            // outward-normal sum with face lengths, in the order of
-            // kernel.cu:1854-1857 (east, south, west, north)
-            const double PSr = ((Fr.fr * dy + (-Gm.fr) * dx) + (-Fl.fr) * dy) + Gp.fr * dx;
-            const double PSe = ((Fr.fe * dy + (-Gm.fe) * dx) + (-Fl.fe) * dy) + Gp.fe * dx;
-            const double PUx = ((Fr.fn * dy + (-Gm.ft) * dx) + (-Fl.fn) * dy) + Gp.ft * dx;
-            const double PUy = ((Fr.ft * dy + (-Gm.fn) * dx) + (-Fl.ft) * dy) + Gp.fn * dx;
+            // sum of outward fluxes times face lengths (kernel.cu:1854-1857),
+            // differenced per direction so that equal opposite fluxes cancel
+            // exactly whatever the multiply-add contraction
+            const double PSr = (Fr.fr - Fl.fr) * dy + (Gp.fr - Gm.fr) * dx;
+            const double PSe = (Fr.fe - Fl.fe) * dy + (Gp.fe - Gm.fe) * dx;
+            const double PUx = (Fr.fn - Fl.fn) * dy + (Gp.ft - Gm.ft) * dx;
+            const double PUy = (Fr.ft - Fl.ft) * dy + (Gp.fn - Gm.fn) * dx;
             const long long k = (long long)j * A.nx + c;
             if constexpr (LAYOUT == kSoA) {
                 double o0 = v0.U[0] - dtdV * PSr;
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
                     atomicAdd(&A.diag->clamp_rho, 1ull);
                     o0 = 0.0001;
                 }
-                const double inv = 1.0 / o0;
+                const double inv = fast_rcp(o0);
                 const double ux = o1 * inv, uy = o2 * inv;
                 const double ke = 0.5 * (o1 * ux + o2 * uy);
                 double p = A.g1 * (o3 - ke);
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
                 A.Uout[2 * P + k] = o2;
                 A.Uout[3 * P + k] = o3;
                 if constexpr (WANT_W) {
-                    const double cs = sqrt(A.gamma * p / o0);
+                    const double cs = fast_sqrt(A.gamma * p * inv);
                     const double w = (fabs(ux) + cs) * A.idx + (fabs(uy) + cs) * A.idy;
                     wmax = fmax(wmax, w);
                 }
@@ -357,8 +358,16 @@ __global__ __launch_bounds__(256) void euler_wavespeed_kernel(const double* U, l
         const double cs = sqrt(gamma * q.p / q.r);
         w = fmax(w, (fabs(q.u) + cs) * idx + (fabs(q.v) + cs) * idy);
     }
+    // wave max, then block max through LDS, then one atomic per block
+    __shared__ double part[256 / kWave];
     w = wave_max(w);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomic_max_nonneg(wmax_bits, w);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int i = 1; i < (int)(blockDim.x / kWave); ++i) m = fmax(m, part[i]);
+        atomic_max_nonneg(wmax_bits, m);
+    }
 }
 
 }  // namespace gd2

@@ -1,23 +1,667 @@
-// gd2_mhd.hpp -- ideal-MHD stage kernels (HLLC-MHD / HLLD + Powell source).
+// gd2_mhd.hpp -- fused ideal-MHD stage kernel for gfx950 (vz = 0, 7 fields,
+// Gaussian B as in the reference), HLLC-MHD or HLLD face flux, minmod MUSCL on
+// the 7 primitives, Powell 8-wave source, conservative update.
+//
+// Same tiling as the Euler kernel (gd2_euler.hpp): one wave per 63-column
+// strip marching up its rows, y-faces carried in registers, x-faces from a
+// 3-row LDS ring, each face flux computed once.
 #pragma once
 
-#include "gd2_device.hpp"
+#include "gd2_euler.hpp"
 
 namespace gd2 {
 
-inline int mhd_stage(const gd2_grid*, const gd2_scheme*, const double*, const double*, double*, double, double,
-                     double, const double*, unsigned long long*, DeviceDiag*, hipStream_t) {
+// Primitive MHD state, Gaussian B.
+struct PrimM {
+    double r, u, v, p, bx, by, bz;
+};
+
+// HLLDQ_Korolkov (Gas_Dinamic.cu:964-1510) in its face frame, metod 2
+// (HLLC-MHD) or 3 (HLLD). Inputs: density, pressure, velocity (n, t, m) and
+// Heaviside-Lorentz field (n, t, m) on both sides. Output PO in the face
+// frame: mass, momentum (n, t, m), energy, field flux (n, t, m) with the
+// normal-field flux -max(|SL|,|SR|)(bn_R - bn_L) (Gas_Dinamic.cu:1488-1490).
+struct FluxM {
+    double f[8];  // 0 mass, 1..3 momentum n/t/m, 4 energy, 5..7 field n/t/m
+};
+
+template <int METOD>
+__device__ __forceinline__ FluxM hlld_face(double ro_L, double p_L, double u1, double v1, double w1, double bn1,
+                                           double bt1, double bm1, double ro_R, double p_R, double u2, double v2,
+                                           double w2, double bn2, double bt2, double bm2, double gamma,
+                                           double inv_g1, double& smax) {
+    // Gas_Dinamic.cu:1047-1078
+    const double bb_L = bn1 * bn1 + bt1 * bt1 + bm1 * bm1;
+    const double bb_R = bn2 * bn2 + bt2 * bt2 + bm2 * bm2;
+    const double uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
+    const double uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
+    const double gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R;
+    const double iroL = fast_rcp(ro_L), iroR = fast_rcp(ro_R);
+    const double cfL = fast_sqrt((gpL + fast_sqrt(gpL * gpL - 4.0 * gamma * p_L * (bn1 * bn1))) * (0.5 * iroL));
+    const double cfR = fast_sqrt((gpR + fast_sqrt(gpR * gpR - 4.0 * gamma * p_R * (bn2 * bn2))) * (0.5 * iroR));
+    const double cmax = fmax(cfL, cfR);
+    const double SL = fmin(u1, u2) - cmax;
+    const double SR = fmax(u1, u2) + cmax;
+    smax = fmax(fabs(SL), fabs(SR));
+    const double pTL = p_L + 0.5 * bb_L, pTR = p_R + 0.5 * bb_R;
+    const double suR = SR - u2, suL = SL - u1;
+    const double mR = suR * ro_R, mL = suL * ro_L;
+    const double idm = fast_rcp(mR - mL);
+    const double SM = (mR * u2 - mL * u1 - pTR + pTL) * idm;
+    const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
+    const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
+
+    double FL[8], FR[8], UL[8], UR[8];
+    FL[0] = ro_L * u1;
+    FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1;
+    FL[2] = ro_L * u1 * v1 - bn1 * bt1;
+    FL[3] = ro_L * u1 * w1 - bn1 * bm1;
+    FL[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
+    FL[5] = 0.0;
+    FL[6] = u1 * bt1 - v1 * bn1;
+    FL[7] = u1 * bm1 - w1 * bn1;
+    FR[0] = ro_R * u2;
+    FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2;
+    FR[2] = ro_R * u2 * v2 - bn2 * bt2;
+    FR[3] = ro_R * u2 * w2 - bn2 * bm2;
+    FR[4] = (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2);
+    FR[5] = 0.0;
+    FR[6] = u2 * bt2 - v2 * bn2;
+    FR[7] = u2 * bm2 - w2 * bn2;
+    UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1;
+    UL[4] = e1; UL[5] = bn1; UL[6] = bt1; UL[7] = bm1;
+    UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2;
+    UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2;
+
+    // HLL averages of the field (Gas_Dinamic.cu:1127-1130)
+    const double iS = fast_rcp(SR - SL);
+    const double bn = (SR * bn2 - SL * bn1) * iS;
+    const double bt = (SR * bt2 - SL * bt1 + FL[6] - FR[6]) * iS;
+    const double bm = (SR * bm2 - SL * bm1 + FL[7] - FR[7]) * iS;
+    const double bbn = bn * bn;
+    const double iLM = fast_rcp(SL - SM), iRM = fast_rcp(SR - SM);
+    const double ro_LL = ro_L * suL * iLM;
+    const double ro_RR = ro_R * suR * iRM;
+
+    FluxM P;
+    if constexpr (METOD == 2) {
+        // HLLC-MHD with HLL-averaged tangential velocity (Gas_Dinamic.cu:1137-1301)
+        const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
+        const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
+        const double UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) * iS;
+        const double UZ1 = (SR * UR[1] - SL * UL[1] + FL[1] - FR[1]) * iS;
+        const double UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) * iS;
+        const double UZ3 = (SR * UR[3] - SL * UL[3] + FL[3] - FR[3]) * iS;
+        const double suRm = suR * iRM, suLm = suL * iLM;
+        const double ptz = 0.5 * ((pTR + ro_R * suR * (SM - u2)) + (pTL + ro_L * suL * (SM - u1)));
+        const double iUZ0 = fast_rcp(UZ0);
+        const double vv = UZ2 * iUZ0, ww = UZ3 * iUZ0;
+        const double sbvz = (bn * UZ1 + bt * UZ2 + bm * UZ3) * iUZ0;
+        const double ee2 = e2 * suRm + (ptz * SM - pTR * u2 + bn * (sbv2 - sbvz)) * iRM;
+        const double ee1 = e1 * suLm + (ptz * SM - pTL * u1 + bn * (sbv1 - sbvz)) * iLM;
+        // branch order of Gas_Dinamic.cu:1250-1277: SL >= 0, SM >= 0, SR > 0, else
+        const bool left = (SL >= 0.0) || (SM >= 0.0);
+        const double S = (SL >= 0.0) ? 0.0 : (left ? SL : (SR > 0.0 ? SR : 0.0));
+        const double rs = left ? ro_LL : ro_RR;
+        const double es = left ? ee1 : ee2;
+        const double* F = left ? FL : FR;
+        const double* U = left ? UL : UR;
+        double Us[8] = {rs, rs * SM, rs * vv, rs * ww, es, bn, bt, bm};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) P.f[i] = F[i] + S * Us[i] - S * U[i];
+    } else {
+        // HLLD (Gas_Dinamic.cu:1303-1484)
+        const double PTT = (mR * pTL - mL * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) * idm;
+        const double ttL = ro_L * suL * (SL - SM) - bbn;
+        const double ttR = ro_R * suR * (SR - SM) - bbn;
+        double vLL, wLL, vRR, wRR, btt1, bmm1, btt2, bmm2;
+        if (fabs(ttL) >= 0.00001) {
+            const double it = fast_rcp(ttL);
+            vLL = v1 - bn * bt1 * (SM - u1) * it;
+            wLL = w1 - bn * bm1 * (SM - u1) * it;
+            const double q = (ro_L * suL * suL - bbn) * it;
+            btt1 = bt1 * q;
+            bmm1 = bm1 * q;
+        } else {
+            vLL = v1; wLL = w1; btt1 = 0.0; bmm1 = 0.0;
+        }
+        if (fabs(ttR) >= 0.00001) {
+            const double it = fast_rcp(ttR);
+            vRR = v2 - bn * bt2 * (SM - u2) * it;
+            wRR = w2 - bn * bm2 * (SM - u2) * it;
+            const double q = (ro_R * suR * suR - bbn) * it;
+            btt2 = bt2 * q;
+            bmm2 = bm2 * q;
+        } else {
+            vRR = v2; wRR = w2; btt2 = 0.0; bmm2 = 0.0;
+        }
+        const double eLL = (e1 * suL + PTT * SM - pTL * u1 +
+                            bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) * iLM;
+        const double eRR = (e2 * suR + PTT * SM - pTR * u2 +
+                            bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) * iRM;
+        const double sLL = fast_sqrt(ro_LL), sRR = fast_sqrt(ro_RR);
+        double SLL = SM - fabs(bn) * fast_rcp(sLL);
+        double SRR = SM + fabs(bn) * fast_rcp(sRR);
+        double idbn;
+        if (fabs(bn) > 0.000001) {
+            idbn = bn > 0.0 ? 1.0 : -1.0;
+        } else {
+            idbn = 0.0;
+            SLL = SM;
+            SRR = SM;
+        }
+        const double isum = fast_rcp(sLL + sRR);
+        const double vLLL = (sLL * vLL + sRR * vRR + idbn * (btt2 - btt1)) * isum;
+        const double wLLL = (sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1)) * isum;
+        const double bttt = (sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL)) * isum;
+        const double bmmm = (sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL)) * isum;
+        const double vb3 = SM * bn + vLLL * bttt + wLLL * bmmm;
+        const double eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - vb3);
+        const double eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - vb3);
+        // six regions (Gas_Dinamic.cu:1434-1484): F + S*(U* - U) for the
+        // single-star regions, F + S**U** - (S** - S)U* - S U for the double-star
+        const bool left = SM >= 0.0;
+        const double* F = left ? FL : FR;
+        const double* U = left ? UL : UR;
+        const double S1 = left ? SL : SR;
+        const double S2 = left ? SLL : SRR;
+        const double rs = left ? ro_LL : ro_RR;
+        const double vs = left ? vLL : vRR, ws = left ? wLL : wRR;
+        const double bts = left ? btt1 : btt2, bms = left ? bmm1 : bmm2;
+        const double es = left ? eLL : eRR;
+        const double ess = left ? eLLL : eRRR;
+        // region selectors in the reference's branch order: left side
+        // SL >= 0 / SLL >= 0 / double star; right side SRR > 0 (double star) /
+        // SR > 0 (single star) / outer
+        const bool outer = left ? (SL >= 0.0) : !(SRR > 0.0 || SR > 0.0);
+        const bool single = left ? (SLL >= 0.0) : !(SRR > 0.0);
+        const double Us[8] = {rs, rs * SM, rs * vs, rs * ws, es, bn, bts, bms};
+        const double Uss[8] = {rs, rs * SM, rs * vLLL, rs * wLLL, ess, bn, bttt, bmmm};
+        if (outer) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) P.f[i] = F[i];
+        } else if (single) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) P.f[i] = F[i] + S1 * Us[i] - S1 * U[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) P.f[i] = F[i] + S2 * Uss[i] - (S2 - S1) * Us[i] - S1 * U[i];
+        }
+    }
+    P.f[5] = -smax * (bn2 - bn1);
+    return P;
+}
+
+struct MhdArgs {
+    const double* Uin;
+    const double* Ubase;
+    double* Uout;
+    const double2* s;
+    const double2* u;
+    const double3* b;
+    double2* s2;
+    double2* u2;
+    double3* b2;
+    int nx, ny;
+    long long plane;
+    double dx, dy, idx, idy;
+    double gamma, g1, inv_g1, krit;
+    int bc[4];
+    double a, b_coef;
+    int plain;
+    double dt;
+    const double* dt_dev;
+    unsigned long long* wmax_bits;
+    unsigned long long* tmin_bits;
+    DeviceDiag* diag;
+    int tiles_x, rows;
+};
+
+template <int LAYOUT>
+__device__ __forceinline__ PrimM load_cell_m(const MhdArgs& A, long long k, bool fx, bool fy) {
+    PrimM q;
+    if constexpr (LAYOUT == kSoA) {
+        const long long P = A.plane;
+        const double r = A.Uin[k], mx = A.Uin[P + k], my = A.Uin[2 * P + k];
+        q.bx = A.Uin[3 * P + k];
+        q.by = A.Uin[4 * P + k];
+        q.bz = A.Uin[5 * P + k];
+        const double E = A.Uin[6 * P + k];
+        const double inv = fast_rcp(r);
+        q.r = r;
+        q.u = mx * inv;
+        q.v = my * inv;
+        q.p = A.g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
+    } else {
+        const double2 sv = A.s[k], uv = A.u[k];
+        const double3 bv = A.b[k];
+        q.r = sv.x; q.p = sv.y; q.u = uv.x; q.v = uv.y; q.bx = bv.x; q.by = bv.y; q.bz = bv.z;
+    }
+    // reflective sides negate the normal velocity and the normal field
+    if (fx) { q.u = -q.u; q.bx = -q.bx; }
+    if (fy) { q.v = -q.v; q.by = -q.by; }
+    return q;
+}
+
+constexpr int kMF = 7;  // LDS fields per cell: r, u, v, p, bx, by, bz
+
+__device__ __forceinline__ void lds_put_m(double (*ring)[kMF][kRowBuf], int slot, int e, const PrimM& q) {
+    ring[slot][0][e] = q.r;
+    ring[slot][1][e] = q.u;
+    ring[slot][2][e] = q.v;
+    ring[slot][3][e] = q.p;
+    ring[slot][4][e] = q.bx;
+    ring[slot][5][e] = q.by;
+    ring[slot][6][e] = q.bz;
+}
+
+__device__ __forceinline__ PrimM lds_get_m(double (*ring)[kMF][kRowBuf], int slot, int e) {
+    PrimM q;
+    q.r = ring[slot][0][e];
+    q.u = ring[slot][1][e];
+    q.v = ring[slot][2][e];
+    q.p = ring[slot][3][e];
+    q.bx = ring[slot][4][e];
+    q.by = ring[slot][5][e];
+    q.bz = ring[slot][6][e];
+    return q;
+}
+
+template <int ORDER>
+__device__ __forceinline__ PrimM slope3m(const PrimM& a, const PrimM& b, const PrimM& c) {
+    PrimM s;
+    if constexpr (ORDER == 2) {
+        s.r = minmod(b.r - a.r, c.r - b.r);
+        s.u = minmod(b.u - a.u, c.u - b.u);
+        s.v = minmod(b.v - a.v, c.v - b.v);
+        s.p = minmod(b.p - a.p, c.p - b.p);
+        s.bx = minmod(b.bx - a.bx, c.bx - b.bx);
+        s.by = minmod(b.by - a.by, c.by - b.by);
+        s.bz = minmod(b.bz - a.bz, c.bz - b.bz);
+    } else {
+        s.r = s.u = s.v = s.p = s.bx = s.by = s.bz = 0.0;
+    }
+    return s;
+}
+
+__device__ __forceinline__ PrimM face_m(const PrimM& q, const PrimM& s, double h) {
+    PrimM f;
+    f.r = q.r + h * s.r;
+    f.u = q.u + h * s.u;
+    f.v = q.v + h * s.v;
+    f.p = q.p + h * s.p;
+    f.bx = q.bx + h * s.bx;
+    f.by = q.by + h * s.by;
+    f.bz = q.bz + h * s.bz;
+    if (f.r <= 0.0) f.r = q.r;  // grad_p fall-back (kernel.cu:2407-2440)
+    if (f.p <= 0.0) f.p = q.p;
+    return f;
+}
+
+// Lab-frame face flux (7 conserved fields + normal-field flux) for an x-face
+// (frame n = x, t = y, m = z) or a y-face (n = y, t = z, m = x); Gaussian field
+// fluxes = sqrt(4 pi) * HL fluxes (Gas_Dinamic.cu:1285-1299).
+struct LabFluxM {
+    double r, mx, my, bx, by, bz, e;
+};
+
+template <int METOD, bool XFACE>
+__device__ __forceinline__ LabFluxM mhd_flux(const PrimM& L, const PrimM& R, double gamma, double inv_g1,
+                                             double& smax) {
+    const double is = 1.0 / kSpi4;
+    FluxM P;
+    if constexpr (XFACE) {
+        P = hlld_face<METOD>(L.r, L.p, L.u, L.v, 0.0, L.bx * is, L.by * is, L.bz * is, R.r, R.p, R.u, R.v, 0.0,
+                             R.bx * is, R.by * is, R.bz * is, gamma, inv_g1, smax);
+    } else {
+        P = hlld_face<METOD>(L.r, L.p, L.v, 0.0, L.u, L.by * is, L.bz * is, L.bx * is, R.r, R.p, R.v, 0.0, R.u,
+                             R.by * is, R.bz * is, R.bx * is, gamma, inv_g1, smax);
+    }
+    LabFluxM F;
+    F.r = P.f[0];
+    F.e = P.f[4];
+    if constexpr (XFACE) {
+        F.mx = P.f[1];
+        F.my = P.f[2];
+        F.bx = kSpi4 * P.f[5];
+        F.by = kSpi4 * P.f[6];
+        F.bz = kSpi4 * P.f[7];
+    } else {
+        F.my = P.f[1];
+        F.mx = P.f[3];
+        F.by = kSpi4 * P.f[5];
+        F.bz = kSpi4 * P.f[6];
+        F.bx = kSpi4 * P.f[7];
+    }
+    return F;
+}
+
+__device__ __forceinline__ LabFluxM shfl_down_m(const LabFluxM& F) {
+    LabFluxM G;
+    G.r = __shfl_down(F.r, 1, kWave);
+    G.mx = __shfl_down(F.mx, 1, kWave);
+    G.my = __shfl_down(F.my, 1, kWave);
+    G.bx = __shfl_down(F.bx, 1, kWave);
+    G.by = __shfl_down(F.by, 1, kWave);
+    G.bz = __shfl_down(F.bz, 1, kWave);
+    G.e = __shfl_down(F.e, 1, kWave);
+    return G;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ bool frozen_m(const MhdArgs& A, int c, int j) {
+    if constexpr (LAYOUT == kAoS) {
+        return (A.bc[0] == GD2_BC_FROZEN && c == 0) || (A.bc[1] == GD2_BC_FROZEN && c == A.nx - 1) ||
+               (A.bc[2] == GD2_BC_FROZEN && j == 0) || (A.bc[3] == GD2_BC_FROZEN && j == A.ny - 1);
+    } else {
+        return false;
+    }
+}
+
+template <int ORDER, int METOD, int LAYOUT, bool WANT_W>
+__global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
+    __shared__ double ring[3][kMF][kRowBuf];
+
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    const int cbase = tx * kTileW;
+    const int j0 = ty * A.rows;
+    const int j1 = min(j0 + A.rows, A.ny);
+    const int c = cbase + lane;
+
+    bool fx, hfx;
+    const int ci = map_index(c, A.nx, A.bc[0], A.bc[1], fx);
+    const int hcol = lane == 0 ? cbase - 2 : (lane == 1 ? cbase - 1 : cbase + 64);
+    const int hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
+    const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
+    const bool has_halo = lane < 3;
+    const bool owns = (lane < kTileW) && (c < A.nx);
+
+    const double dt = A.dt_dev ? *A.dt_dev : A.dt;
+    const double dx = A.dx, dy = A.dy;
+    const double dV = dx * dy;
+    // lagged face time step radius: dx on x-faces and dy on y-faces as in
+    // add2 / Kernel_TVD (kernel.cu:2070, 2086)
+    const double krx = A.krit * dx, kry = A.krit * dy;
+
+    auto load_row = [&](int jl, int slot, bool to_lds) -> PrimM {
+        bool fy;
+        const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
+        const long long rowk = (long long)jj * A.nx;
+        PrimM q = load_cell_m<LAYOUT>(A, rowk + ci, fx, fy);
+        if (to_lds) {
+            lds_put_m(ring, slot, lane + 2, q);
+            if (has_halo) {
+                PrimM h = load_cell_m<LAYOUT>(A, rowk + hci, hfx, fy);
+                lds_put_m(ring, slot, hslot, h);
+            }
+        }
+        return q;
+    };
+
+    PrimM qm2 = load_row(j0 - 2, 0, false);
+    PrimM qm1 = load_row(j0 - 1, 0, false);
+    PrimM q0 = load_row(j0, j0 % 3, true);
+    PrimM q1 = load_row(j0 + 1, (j0 + 1) % 3, true);
+    PrimM sm1 = slope3m<ORDER>(qm2, qm1, q0);
+    PrimM s0 = slope3m<ORDER>(qm1, q0, q1);
+    double smax;
+    LabFluxM Gm = mhd_flux<METOD, false>(face_m(qm1, sm1, 0.5), face_m(q0, s0, -0.5), A.gamma, A.inv_g1, smax);
+    double tGm = (LAYOUT == kAoS) ? kry / smax : 0.0;
+    double by_s = qm1.by;  // cell-centred By of the south neighbour (Powell average)
+
+    double wmax = 0.0, tmin = 1e300;
+
+    for (int j = j0; j < j1; ++j) {
+        __syncthreads();
+        PrimM q2 = load_row(j + 2, (j + 2) % 3, true);
+        PrimM s1 = slope3m<ORDER>(q0, q1, q2);
+        const LabFluxM Gp =
+            mhd_flux<METOD, false>(face_m(q0, s0, 0.5), face_m(q1, s1, -0.5), A.gamma, A.inv_g1, smax);
+        const double tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
+
+        // x-face F(c-1/2) from cells c-2, c-1, c, c+1 of row j (in LDS since two barriers)
+        const int sl = j % 3;
+        const PrimM qa = lds_get_m(ring, sl, lane);
+        const PrimM qb = lds_get_m(ring, sl, lane + 1);
+        const PrimM qd = lds_get_m(ring, sl, lane + 3);
+        const PrimM sb = slope3m<ORDER>(qa, qb, q0);
+        const PrimM sc = slope3m<ORDER>(qb, q0, qd);
+        const LabFluxM Fl = mhd_flux<METOD, true>(face_m(qb, sb, 0.5), face_m(q0, sc, -0.5), A.gamma, A.inv_g1, smax);
+        const double tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+        const LabFluxM Fr = shfl_down_m(Fl);
+        const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
+        const double bx_w = qb.bx, bx_e = qd.bx;
+
+        if (owns && !frozen_m<LAYOUT>(A, c, j)) {
+            // face sums differenced per direction (kernel.cu:2559-2604)
+            const double PSr = (Fr.r - Fl.r) * dy + (Gp.r - Gm.r) * dx;
+            const double PSe = (Fr.e - Fl.e) * dy + (Gp.e - Gm.e) * dx;
+            const double PUx = (Fr.mx - Fl.mx) * dy + (Gp.mx - Gm.mx) * dx;
+            const double PUy = (Fr.my - Fl.my) * dy + (Gp.my - Gm.my) * dx;
+            const double PBx = (Fr.bx - Fl.bx) * dy + (Gp.bx - Gm.bx) * dx;
+            const double PBy = (Fr.by - Fl.by) * dy + (Gp.by - Gm.by) * dx;
+            const double PBz = (Fr.bz - Fl.bz) * dy + (Gp.bz - Gm.bz) * dx;
+            // Powell term: face-averaged normal field of the cell-centred
+            // neighbours (kernel.cu:2568, 2580, 2592, 2604)
+            const double Pdiv = dy * (bx_e - bx_w) * 0.5 + dx * (q1.by - by_s) * 0.5;
+            const PrimM& q = q0;
+            const long long k = (long long)j * A.nx + c;
+            if constexpr (LAYOUT == kSoA) {
+                const long long P = A.plane;
+                const double dtV = dt / dV;
+                double o[7];
+                o[0] = A.Uin[k] - dtV * PSr;
+                o[1] = A.Uin[P + k] - dtV * (PUx + (q.bx * (1.0 / kCpi4)) * Pdiv);
+                o[2] = A.Uin[2 * P + k] - dtV * (PUy + (q.by * (1.0 / kCpi4)) * Pdiv);
+                o[3] = A.Uin[3 * P + k] - dtV * (PBx + q.u * Pdiv);
+                o[4] = A.Uin[4 * P + k] - dtV * (PBy + q.v * Pdiv);
+                o[5] = A.Uin[5 * P + k] - dtV * PBz;
+                o[6] = A.Uin[6 * P + k] - dtV * (PSe + ((q.u * q.bx + q.v * q.by) * (1.0 / kCpi4)) * Pdiv);
+                if (!A.plain) {
+#pragma unroll
+                    for (int f = 0; f < 7; ++f) o[f] = A.a * A.Ubase[f * P + k] + A.b_coef * o[f];
+                }
+                // clamps (kernel.cu:2144-2148 rho -> old rho; 2192-2195 p -> 1e-6)
+                if (o[0] <= 0.0) {
+                    atomicAdd(&A.diag->clamp_rho, 1ull);
+                    o[0] = A.Uin[k];
+                }
+                const double inv = fast_rcp(o[0]);
+                const double ux = o[1] * inv, uy = o[2] * inv;
+                const double ke = 0.5 * (o[1] * ux + o[2] * uy);
+                const double b2 = o[3] * o[3] + o[4] * o[4] + o[5] * o[5];
+                const double me = b2 * (1.0 / kCpi8);
+                double p = A.g1 * (o[6] - ke - me);
+                if (p <= 0.0) {
+                    atomicAdd(&A.diag->clamp_p, 1ull);
+                    o[6] = 0.000001 * A.inv_g1 + ke + me;
+                    p = A.g1 * (o[6] - ke - me);
+                }
+#pragma unroll
+                for (int f = 0; f < 7; ++f) A.Uout[f * P + k] = o[f];
+                if constexpr (WANT_W) {
+                    const double cf = fast_sqrt((A.gamma * p + b2 * (1.0 / kCpi4)) * inv);
+                    wmax = fmax(wmax, (fabs(ux) + cf) * A.idx + (fabs(uy) + cf) * A.idy);
+                }
+            } else {
+                // primitive-form update of add2 / Kernel_TVD, Cartesian
+                // (kernel.cu:2141-2195 without the 1/y terms)
+                const double gm1 = A.gamma - 1.0;
+                double r2 = q.r - dt * PSr / dV;
+                if (r2 <= 0.0) {
+                    atomicAdd(&A.diag->clamp_rho, 1ull);
+                    r2 = q.r;
+                }
+                const double ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pdiv) / dV) / r2;
+                const double uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pdiv) / dV) / r2;
+                const double bx = q.bx - dt * (PBx + q.u * Pdiv) / dV;
+                const double by = q.by - dt * (PBy + q.v * Pdiv) / dV;
+                const double bz = q.bz - dt * PBz / dV;
+                const double U8 = q.p / gm1 + 0.5 * q.r * (q.u * q.u + q.v * q.v) +
+                                  (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / kCpi8;
+                const double skk = q.u * q.bx + q.v * q.by;
+                double p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pdiv) / dV - 0.5 * r2 * (ux * ux + uy * uy) -
+                             (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                if (p2 <= 0.0) {
+                    atomicAdd(&A.diag->clamp_p, 1ull);
+                    p2 = 0.000001;
+                }
+                A.s2[k] = make_double2(r2, p2);
+                A.u2[k] = make_double2(ux, uy);
+                A.b2[k] = make_double3(bx, by, bz);
+                tmin = fmin(tmin, fmin(1000.0, fmin(fmin(tFr, tGm), fmin(tFl, tGp))));
+            }
+        } else if constexpr (LAYOUT == kAoS) {
+            if (owns) {
+                const long long k = (long long)j * A.nx + c;
+                A.s2[k] = A.s[k];
+                A.u2[k] = A.u[k];
+                A.b2[k] = A.b[k];
+            }
+        }
+
+        by_s = q0.by;
+        q0 = q1;
+        q1 = q2;
+        s0 = s1;
+        Gm = Gp;
+        tGm = tGp;
+    }
+
+    if constexpr (LAYOUT == kSoA && WANT_W) {
+        const double w = wave_max(wmax);
+        if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
+    }
+    if constexpr (LAYOUT == kAoS) {
+        const double t = wave_min(tmin);
+        if (lane == 0 && t < 1e300) atomic_min_nonneg(A.tmin_bits, t);
+    }
+}
+
+__global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, long long n, double gamma, double g1,
+                                                            double idx, double idy, unsigned long long* wmax_bits) {
+    double w = 0.0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const double r = U[k], mx = U[n + k], my = U[2 * n + k];
+        const double bx = U[3 * n + k], by = U[4 * n + k], bz = U[5 * n + k], E = U[6 * n + k];
+        const double inv = fast_rcp(r);
+        const double ux = mx * inv, uy = my * inv;
+        const double b2 = bx * bx + by * by + bz * bz;
+        const double p = g1 * (E - 0.5 * (mx * ux + my * uy) - b2 * (1.0 / kCpi8));
+        const double cf = fast_sqrt((gamma * p + b2 * (1.0 / kCpi4)) * inv);
+        w = fmax(w, (fabs(ux) + cf) * idx + (fabs(uy) + cf) * idy);
+    }
+    __shared__ double part[256 / kWave];
+    w = wave_max(w);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int i = 1; i < (int)(blockDim.x / kWave); ++i) m = fmax(m, part[i]);
+        atomic_max_nonneg(wmax_bits, m);
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+int tile_rows_for(int nx, int ny);
+int hip_error_code(hipError_t e);
+
+inline void fill_mhd(MhdArgs& A, const gd2_grid* g, DeviceDiag* diag) {
+    std::memset(&A, 0, sizeof A);
+    A.nx = g->nx;
+    A.ny = g->ny;
+    A.plane = (long long)g->nx * g->ny;
+    A.dx = g->dx;
+    A.dy = g->dy;
+    A.idx = 1.0 / g->dx;
+    A.idy = 1.0 / g->dy;
+    A.gamma = g->gamma;
+    A.g1 = g->gamma - 1.0;
+    A.inv_g1 = 1.0 / (g->gamma - 1.0);
+    A.krit = g->krit;
+    for (int s = 0; s < 4; ++s) A.bc[s] = g->bc[s];
+    A.diag = diag;
+    A.rows = tile_rows_for(g->nx, g->ny);
+    A.tiles_x = (g->nx + kTileW - 1) / kTileW;
+}
+
+template <int ORDER, int METOD, int LAYOUT, bool W>
+inline int launch_mhd(const MhdArgs& A, hipStream_t st) {
+    const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
+    hipLaunchKernelGGL((mhd_tile_kernel<ORDER, METOD, LAYOUT, W>), dim3(blocks), dim3(kWave), 0, st, A);
+    return hip_error_code(hipGetLastError());
+}
+
+template <int LAYOUT, bool W>
+inline int dispatch_mhd(int order, int metod, const MhdArgs& A, hipStream_t st) {
+    if (order == 2) {
+        if (metod == 3) return launch_mhd<2, 3, LAYOUT, W>(A, st);
+        if (metod == 2) return launch_mhd<2, 2, LAYOUT, W>(A, st);
+    } else if (order == 1) {
+        if (metod == 3) return launch_mhd<1, 3, LAYOUT, W>(A, st);
+        if (metod == 2) return launch_mhd<1, 2, LAYOUT, W>(A, st);
+    }
     return GD2_ERR_UNSUPPORTED;
 }
 
-inline int mhd_wavespeed(const gd2_grid*, const double*, unsigned long long*, hipStream_t) {
-    return GD2_ERR_UNSUPPORTED;
+inline int mhd_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const double* Ubase, double* Uout,
+                     double a, double b, double dt, const double* dt_dev, unsigned long long* wmax_bits,
+                     DeviceDiag* diag, hipStream_t st) {
+    if (!diag) return GD2_ERR_HIP;
+    MhdArgs A;
+    fill_mhd(A, g, diag);
+    A.Uin = Uin;
+    A.Ubase = Ubase;
+    A.Uout = Uout;
+    A.a = a;
+    A.b_coef = b;
+    A.plain = (a == 0.0 && b == 1.0) ? 1 : 0;
+    A.dt = dt;
+    A.dt_dev = dt_dev;
+    A.wmax_bits = wmax_bits;
+    const int metod = sc->flux == GD2_FLUX_HLLD ? 3 : 2;
+    if (wmax_bits) return dispatch_mhd<kSoA, true>(sc->order, metod, A, st);
+    return dispatch_mhd<kSoA, false>(sc->order, metod, A, st);
 }
 
-inline int mhd_ref_step(const gd2_grid*, int, const gd2_double2*, const gd2_double2*, const gd2_double3*,
-                        gd2_double2*, gd2_double2*, gd2_double3*, double*, const double*, int, DeviceDiag*,
-                        hipStream_t) {
-    return GD2_ERR_UNSUPPORTED;
+inline unsigned mhd_ew_blocks(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b > 2048 ? 2048 : (b < 1 ? 1 : b));
+}
+
+inline int mhd_wavespeed(const gd2_grid* g, const double* U, unsigned long long* wmax_bits, hipStream_t st) {
+    const long long n = (long long)g->nx * g->ny;
+    hipLaunchKernelGGL(mhd_wavespeed_kernel, dim3(mhd_ew_blocks(n)), dim3(256), 0, st, U, n, g->gamma, g->gamma - 1.0,
+                       1.0 / g->dx, 1.0 / g->dy, wmax_bits);
+    return hip_error_code(hipGetLastError());
+}
+
+inline int mhd_ref_step(const gd2_grid* g, int order, const gd2_double2* s, const gd2_double2* u,
+                        const gd2_double3* b, gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T,
+                        const double* T_do, int method, DeviceDiag* diag, hipStream_t st) {
+    if (!s || !u || !b || !s2 || !u2 || !b2 || !T || !T_do) return GD2_ERR_ARG;
+    if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2 || (const void*)b == (const void*)b2)
+        return GD2_ERR_ARG;
+    if (method != 2 && method != 3) return GD2_ERR_UNSUPPORTED;
+    if (!diag) return GD2_ERR_HIP;
+    MhdArgs A;
+    fill_mhd(A, g, diag);
+    A.s = reinterpret_cast<const double2*>(s);
+    A.u = reinterpret_cast<const double2*>(u);
+    A.b = reinterpret_cast<const double3*>(b);
+    A.s2 = reinterpret_cast<double2*>(s2);
+    A.u2 = reinterpret_cast<double2*>(u2);
+    A.b2 = reinterpret_cast<double3*>(b2);
+    A.dt_dev = T_do;
+    A.tmin_bits = reinterpret_cast<unsigned long long*>(T);
+    A.plain = 1;
+    A.a = 0.0;
+    A.b_coef = 1.0;
+    return dispatch_mhd<kAoS, false>(order, method, A, st);
 }
 
 }  // namespace gd2
