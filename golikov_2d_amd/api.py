@@ -91,6 +91,14 @@ def lib():
             "gd2_ctx_advance": (ip, [vp, vp, ip, dp, vp]),
             "gd2_ctx_time": (ip, [vp, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
             "gd2_get_diagnostics": (ip, [C.POINTER(Diag), ip]),
+            "gd2_ensemble_create": (ip, [ip, C.POINTER(Grid), sp, C.POINTER(vp), C.POINTER(vp)]),
+            "gd2_ensemble_destroy": (ip, [vp]),
+            "gd2_ensemble_set_tiling": (ip, [vp, ip, ip]),
+            "gd2_ensemble_get_tiling": (ip, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]),
+            "gd2_ensemble_start": (ip, [vp, vp]),
+            "gd2_ensemble_advance": (ip, [vp, ip, ip, vp]),
+            "gd2_ensemble_time": (ip, [vp, ip, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
+            "gd2_ensemble_tune": (ip, [vp, vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -103,7 +111,9 @@ def lib():
 EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd2_wavespeed_max",
             "gd2_ker_dekard", "gd2_kernel_tvd_old", "gd2_add2", "gd2_kernel_tvd", "gd2_funk_time",
             "gd2_prim_to_cons", "gd2_cons_to_prim", "gd2_ctx_create", "gd2_ctx_destroy", "gd2_ctx_start",
-            "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics"]
+            "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
+            "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
+            "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune"]
 
 
 def _check(rc, name):
@@ -247,6 +257,63 @@ class Solver:
     def close(self):
         if self._h:
             lib().gd2_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Batched ensemble of independent Euler problems (gd2_ensemble)."""
+
+    def __init__(self, grids, sc: Scheme, states):
+        if len(grids) != len(states) or not grids:
+            raise GD2Error("one state per grid expected")
+        n = len(grids)
+        for g, U in zip(grids, states):
+            if tuple(U.shape) != (4, g.ny, g.nx):
+                raise GD2Error("state shape must be (4, ny, nx)")
+            _f64(U)
+        self.grids, self.sc, self.states = list(grids), sc, list(states)
+        garr = (Grid * n)(*grids)
+        parr = (C.c_void_p * n)(*[U.data_ptr() for U in states])
+        h = C.c_void_p()
+        _check(lib().gd2_ensemble_create(n, garr, C.byref(sc), parr, C.byref(h)), "gd2_ensemble_create")
+        self._h = h
+
+    def set_tiling(self, rows=0, blocks=0):
+        _check(lib().gd2_ensemble_set_tiling(self._h, rows, blocks), "gd2_ensemble_set_tiling")
+
+    def tiling(self):
+        r, b, t = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().gd2_ensemble_get_tiling(self._h, C.byref(r), C.byref(b), C.byref(t)), "gd2_ensemble_get_tiling")
+        return r.value, b.value, t.value
+
+    def start(self, stream=None):
+        _check(lib().gd2_ensemble_start(self._h, _stream(stream)), "gd2_ensemble_start")
+
+    def advance(self, nsteps, use_graph=False, stream=None):
+        _check(lib().gd2_ensemble_advance(self._h, int(nsteps), 1 if use_graph else 0, _stream(stream)),
+               "gd2_ensemble_advance")
+
+    def time(self, p, stream=None):
+        t, dt, n = C.c_double(), C.c_double(), C.c_longlong()
+        _check(lib().gd2_ensemble_time(self._h, p, C.byref(t), C.byref(n), C.byref(dt), _stream(stream)),
+               "gd2_ensemble_time")
+        return t.value, n.value, dt.value
+
+    def tune(self, stream=None):
+        r, b, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().gd2_ensemble_tune(self._h, _stream(stream), C.byref(r), C.byref(b), C.byref(ms)),
+               "gd2_ensemble_tune")
+        return r.value, b.value, ms.value
+
+    def close(self):
+        if self._h:
+            lib().gd2_ensemble_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -4,13 +4,17 @@
 // gd2_euler.hpp / gd2_mhd.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <vector>
 
 #include "../../include/gd2.h"
 #include "gd2_euler.hpp"
 #include "gd2_mhd.hpp"
+#include "gd2_ensemble.hpp"
 
 namespace gd2 {
 
@@ -120,31 +124,7 @@ static bool scheme_ok(const gd2_scheme* sc) {
 }
 
 // ---------------------------------------------------------------- clock
-struct Clock {
-    double t, dt, cfl, pad;
-    unsigned long long w[2];
-    long long step;
-};
-
-__global__ void clock_begin_kernel(Clock* c, int cur, double t_end) {
-    const double w = __longlong_as_double((long long)c->w[cur]);
-    double dt = c->cfl / w;
-    bool live = true;
-    if (t_end > 0.0) {
-        if (!(c->t < t_end)) {
-            dt = 0.0;
-            live = false;
-        } else if (dt > t_end - c->t) {
-            dt = t_end - c->t;
-        }
-    }
-    c->dt = dt;
-    c->w[cur ^ 1] = 0ull;
-    if (live) {
-        c->t = c->t + dt;
-        c->step += 1;
-    }
-}
+__global__ void clock_begin_kernel(Clock* c, int cur, double t_end) { clock_begin(c, cur, t_end); }
 
 __global__ void clock_start_kernel(Clock* c, double t0, double cfl) {
     c->t = t0;
@@ -451,3 +431,270 @@ int gd2_get_diagnostics(gd2_diag* out, int reset) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- ensembles
+
+struct gd2_ensemble {
+    int n = 0;
+    gd2_scheme sc{};
+    std::vector<EnsProblem> host;     // descriptors (device pointers inside)
+    EnsProblem* dprobs = nullptr;
+    Clock* clocks = nullptr;
+    double* scratch = nullptr;
+    EnsTile* dtiles = nullptr;
+    int ntiles = 0;
+    int rows = 0;       // tile height (0 = per-problem default)
+    int blocks = 0;     // resident waves per stage launch (0 = one per tile)
+    long long host_steps = 0;
+    hipStream_t cap_stream = nullptr;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, parity) -> graph
+    std::vector<hipGraph_t> raw_graphs;
+};
+
+static void ens_drop_graphs(gd2_ensemble* e) {
+    for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto g : e->raw_graphs) (void)hipGraphDestroy(g);
+    e->graphs.clear();
+    e->raw_graphs.clear();
+}
+
+static int ens_build_tiles(gd2_ensemble* e) {
+    std::vector<EnsTile> t;
+    for (int p = 0; p < e->n; ++p) {
+        EulerArgs& A = e->host[p].args;
+        A.rows = e->rows > 0 ? e->rows : tile_rows_for(A.nx, A.ny);
+        const int ty_n = (A.ny + A.rows - 1) / A.rows;
+        for (int ty = 0; ty < ty_n; ++ty)
+            for (int tx = 0; tx < A.tiles_x; ++tx) t.push_back(EnsTile{p, tx, ty, 0});
+    }
+    // large problems first so the tail of a persistent launch holds small tiles
+    std::stable_sort(t.begin(), t.end(), [&](const EnsTile& a, const EnsTile& b) {
+        const long long ca = (long long)e->host[a.problem].args.nx * e->host[a.problem].args.ny;
+        const long long cb = (long long)e->host[b.problem].args.nx * e->host[b.problem].args.ny;
+        return ca > cb;
+    });
+    if (e->dtiles) (void)hipFree(e->dtiles);
+    e->dtiles = nullptr;
+    GD2_HIP(hipMalloc(&e->dtiles, sizeof(EnsTile) * t.size()));
+    GD2_HIP(hipMemcpy(e->dtiles, t.data(), sizeof(EnsTile) * t.size(), hipMemcpyHostToDevice));
+    GD2_HIP(hipMemcpy(e->dprobs, e->host.data(), sizeof(EnsProblem) * e->n, hipMemcpyHostToDevice));
+    e->ntiles = (int)t.size();
+    ens_drop_graphs(e);
+    return GD2_OK;
+}
+
+template <int STAGE>
+static int ens_launch_stage(gd2_ensemble* e, int slot, hipStream_t st) {
+    const unsigned grid = (unsigned)(e->blocks > 0 ? std::min(e->blocks, e->ntiles) : e->ntiles);
+    const int order = e->sc.order, flux = e->sc.flux;
+    if (order == 2 && flux == GD2_FLUX_HLLC)
+        hipLaunchKernelGGL((ens_stage_kernel<2, GD2_FLUX_HLLC, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    else if (order == 1 && flux == GD2_FLUX_HLLC)
+        hipLaunchKernelGGL((ens_stage_kernel<1, GD2_FLUX_HLLC, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    else if (order == 2 && flux == GD2_FLUX_EXACT)
+        hipLaunchKernelGGL((ens_stage_kernel<2, GD2_FLUX_EXACT, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    else
+        hipLaunchKernelGGL((ens_stage_kernel<1, GD2_FLUX_EXACT, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+static int ens_enqueue_steps(gd2_ensemble* e, int nsteps, long long first_step, hipStream_t st) {
+    for (int i = 0; i < nsteps; ++i) {
+        const int slot = (int)((first_step + i) & 1);
+        hipLaunchKernelGGL(ens_clock_kernel, dim3((e->n + 63) / 64), dim3(64), 0, st, e->dprobs, e->n, slot);
+        GD2_HIP(hipGetLastError());
+        int rc = ens_launch_stage<1>(e, slot, st);
+        if (rc) return rc;
+        rc = ens_launch_stage<2>(e, slot, st);
+        if (rc) return rc;
+    }
+    return GD2_OK;
+}
+
+extern "C" {
+
+int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, double* const* U, gd2_ensemble** out) {
+    if (!out) return GD2_ERR_ARG;
+    *out = nullptr;
+    if (n < 1 || !grids || !sc || !U || !scheme_ok(sc) || sc->physics != GD2_EULER) return GD2_ERR_ARG;
+    if (sc->integrator != GD2_SSPRK2 || !(sc->cfl > 0.0)) return GD2_ERR_UNSUPPORTED;
+    for (int p = 0; p < n; ++p)
+        if (!grid_ok(&grids[p], false) || !U[p]) return GD2_ERR_ARG;
+    if (!diag_ptr()) return GD2_ERR_HIP;
+    gd2_ensemble* e = new (std::nothrow) gd2_ensemble();
+    if (!e) return GD2_ERR_NOMEM;
+    e->n = n;
+    e->sc = *sc;
+    e->host.resize(n);
+    size_t total = 0;
+    for (int p = 0; p < n; ++p) total += (size_t)4 * grids[p].nx * grids[p].ny;
+    hipError_t err = hipMalloc(&e->scratch, sizeof(double) * total);
+    if (err == hipSuccess) err = hipMalloc(&e->clocks, sizeof(Clock) * n);
+    if (err == hipSuccess) err = hipMalloc(&e->dprobs, sizeof(EnsProblem) * n);
+    if (err != hipSuccess) {
+        g_last_hip = (int)err;
+        gd2_ensemble_destroy(e);
+        return GD2_ERR_NOMEM;
+    }
+    size_t off = 0;
+    for (int p = 0; p < n; ++p) {
+        EnsProblem& P = e->host[p];
+        fill_common(P.args, &grids[p]);
+        P.U = U[p];
+        P.S = e->scratch + off;
+        P.clk = e->clocks + p;
+        off += (size_t)4 * grids[p].nx * grids[p].ny;
+    }
+    int rc = ens_build_tiles(e);
+    if (rc) {
+        gd2_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return GD2_OK;
+}
+
+int gd2_ensemble_destroy(gd2_ensemble* e) {
+    if (!e) return GD2_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    ens_drop_graphs(e);
+    if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
+    if (e->dtiles) (void)hipFree(e->dtiles);
+    if (e->dprobs) (void)hipFree(e->dprobs);
+    if (e->clocks) (void)hipFree(e->clocks);
+    if (e->scratch) (void)hipFree(e->scratch);
+    delete e;
+    return GD2_OK;
+}
+
+int gd2_ensemble_set_tiling(gd2_ensemble* e, int rows, int blocks) {
+    if (!e || rows < 0 || blocks < 0 || rows > 4096) return GD2_ERR_ARG;
+    e->rows = rows;
+    e->blocks = blocks;
+    return ens_build_tiles(e);
+}
+
+int gd2_ensemble_get_tiling(gd2_ensemble* e, int* rows, int* blocks, int* ntiles) {
+    if (!e) return GD2_ERR_ARG;
+    if (rows) *rows = e->rows;
+    if (blocks) *blocks = e->blocks;
+    if (ntiles) *ntiles = e->ntiles;
+    return GD2_OK;
+}
+
+int gd2_ensemble_start(gd2_ensemble* e, void* stream) {
+    if (!e) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(ens_clock_start_kernel, dim3((e->n + 63) / 64), dim3(64), 0, st, e->clocks, e->n, e->sc.cfl);
+    GD2_HIP(hipGetLastError());
+    for (int p = 0; p < e->n; ++p) {
+        const EulerArgs& A = e->host[p].args;
+        const long long n = A.plane;
+        hipLaunchKernelGGL(euler_wavespeed_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, e->host[p].U, n, n, A.gamma,
+                           A.g1, A.idx, A.idy, &e->clocks[p].w[0]);
+        GD2_HIP(hipGetLastError());
+    }
+    e->host_steps = 0;
+    return GD2_OK;
+}
+
+int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* stream) {
+    if (!e || nsteps < 0) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    if (nsteps == 0) return GD2_OK;
+    if (!use_graph) {
+        int rc = ens_enqueue_steps(e, nsteps, e->host_steps, st);
+        if (rc) return rc;
+        e->host_steps += nsteps;
+        return GD2_OK;
+    }
+    const int parity = (int)(e->host_steps & 1);
+    auto key = std::make_pair(nsteps, parity);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+        if (!e->cap_stream) GD2_HIP(hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking));
+        GD2_HIP(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = ens_enqueue_steps(e, nsteps, parity, e->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        GD2_HIP(ce);
+        hipGraphExec_t ge = nullptr;
+        GD2_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        e->raw_graphs.push_back(g);
+        it = e->graphs.emplace(key, ge).first;
+    }
+    GD2_HIP(hipGraphLaunch(it->second, st));
+    e->host_steps += nsteps;
+    return GD2_OK;
+}
+
+int gd2_ensemble_time(gd2_ensemble* e, int p, double* t, long long* steps, double* dt, void* stream) {
+    if (!e || p < 0 || p >= e->n) return GD2_ERR_ARG;
+    Clock h;
+    GD2_HIP(hipMemcpyAsync(&h, e->clocks + p, sizeof(Clock), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    GD2_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (t) *t = h.t;
+    if (steps) *steps = h.step;
+    if (dt) *dt = h.dt;
+    return GD2_OK;
+}
+
+int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows_out, int* blocks_out, double* best_ms) {
+    if (!e) return GD2_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    // candidate tile heights and resident-wave counts; each candidate is
+    // timed on stage-1 launches, which write only the scratch states
+    const int rows_c[] = {8, 16, 32, 64};
+    const int blocks_c[] = {0, 2048, 4096, 8192, 16384};
+    hipEvent_t a, b;
+    GD2_HIP(hipEventCreate(&a));
+    GD2_HIP(hipEventCreate(&b));
+    double best = 1e300;
+    int br = e->rows, bb = e->blocks;
+    for (int r : rows_c) {
+        e->rows = r;
+        int rc = ens_build_tiles(e);
+        if (rc) return rc;
+        for (int blk : blocks_c) {
+            e->blocks = blk;
+            rc = ens_launch_stage<1>(e, 0, st);  // warm-up
+            if (rc) return rc;
+            GD2_HIP(hipEventRecord(a, st));
+            for (int k = 0; k < 3; ++k) {
+                rc = ens_launch_stage<1>(e, 0, st);
+                if (rc) return rc;
+            }
+            GD2_HIP(hipEventRecord(b, st));
+            GD2_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            GD2_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                br = r;
+                bb = blk;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->rows = br;
+    e->blocks = bb;
+    int rc = ens_build_tiles(e);
+    if (rc) return rc;
+    if (rows_out) *rows_out = br;
+    if (blocks_out) *blocks_out = bb;
+    if (best_ms) *best_ms = best / 3.0;
+    return GD2_OK;
+}
+
+}  // extern "C"
+

@@ -20,6 +20,34 @@ struct DeviceDiag {
     unsigned long long clamp_rho, clamp_p, newton_cap, bisect_cap;
 };
 
+// Device clock of a solver context: the next step's dt is cfl / w[cur]
+// (w = wave-speed maximum of the current state, accumulated as double bits).
+struct Clock {
+    double t, dt, cfl, pad;
+    unsigned long long w[2];
+    long long step;
+};
+
+__device__ __forceinline__ void clock_begin(Clock* c, int cur, double t_end) {
+    const double w = __longlong_as_double((long long)c->w[cur]);
+    double dt = c->cfl / w;
+    bool live = true;
+    if (t_end > 0.0) {
+        if (!(c->t < t_end)) {
+            dt = 0.0;
+            live = false;
+        } else if (dt > t_end - c->t) {
+            dt = t_end - c->t;
+        }
+    }
+    c->dt = dt;
+    c->w[cur ^ 1] = 0ull;
+    if (live) {
+        c->t = c->t + dt;
+        c->step += 1;
+    }
+}
+
 // Logical index k (may be -2..n+1, or further on padding lanes) -> stored
 // index; flip is set where the reflective rule negates the normal vector
 // components. Transmissive and frozen sides clamp (kernel.cu:1762-1793,

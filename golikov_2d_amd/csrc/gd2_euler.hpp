@@ -153,14 +153,12 @@ __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
     }
 }
 
+// One wave's strip: columns tx*63 .. tx*63+62, rows ty*rows .. +rows-1.
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
-__global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
-    __shared__ double ring[3][4][kRowBuf];
-
+__device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, double (*ring)[4][kRowBuf]) {
     const int lane = threadIdx.x;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
     const int cbase = tx * kTileW;
+    __syncthreads();  // a wave may run several tiles: the previous tile's LDS reads are done
     const int j0 = ty * A.rows;
     const int j1 = min(j0 + A.rows, A.ny);
     const int c = cbase + lane;
@@ -344,6 +342,13 @@ __global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
         const double t = wave_min(tmin);
         if (lane == 0 && t < 1e300) atomic_min_nonneg(A.tmin_bits, t);
     }
+}
+
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
+__global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
+    __shared__ double ring[3][4][kRowBuf];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    euler_tile<ORDER, FLUX, LAYOUT, WANT_W>(A, tile % A.tiles_x, tile / A.tiles_x, ring);
 }
 
 // Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA

@@ -151,6 +151,38 @@ int gd2_ctx_advance(gd2_ctx* ctx, double* U, int nsteps, double t_end, void* str
 /* Synchronises the stream and reads the clock: time, steps taken, last dt. */
 int gd2_ctx_time(gd2_ctx* ctx, double* t, long long* steps, double* dt, void* stream);
 
+/* ----------------------------------------------------------------------
+ * Ensembles: n independent Euler problems (SSP-RK2, each with its own grid,
+ * boundary kinds and CFL step) advanced together. Each step is three launches
+ * for the whole ensemble (per-problem clocks, stage 1, stage 2) over a device
+ * work list of (problem, tile) entries; gd2_ensemble_advance can capture the
+ * steps in a hipGraph and replay it. No reference counterpart (the reference
+ * steps a single grid); results are bitwise those of running each problem
+ * alone through gd2_ctx_advance.
+ * ---------------------------------------------------------------------- */
+typedef struct gd2_ensemble gd2_ensemble;
+
+/* U[p] is problem p's caller-owned SoA conserved state (4 planes). Allocates
+ * the stage scratch for all problems and the device descriptors. */
+int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, double* const* U,
+                        gd2_ensemble** out);
+int gd2_ensemble_destroy(gd2_ensemble* e);
+/* Tile height (rows per wave strip, 0 = per-problem default) and number of
+ * resident waves per stage launch (0 = one wave per tile). */
+int gd2_ensemble_set_tiling(gd2_ensemble* e, int rows, int blocks);
+int gd2_ensemble_get_tiling(gd2_ensemble* e, int* rows, int* blocks, int* ntiles);
+/* Reset every clock to t = 0 and compute each problem's first dt. */
+int gd2_ensemble_start(gd2_ensemble* e, void* stream);
+/* Advance every problem by nsteps; use_graph != 0 replays a captured hipGraph
+ * of the nsteps steps (captured on first use, cached). */
+int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* stream);
+/* Synchronises the stream; reads problem p's clock. */
+int gd2_ensemble_time(gd2_ensemble* e, int p, double* t, long long* steps, double* dt, void* stream);
+/* Times stage launches over candidate tile heights and resident-wave counts
+ * (writing only the scratch states) and keeps the fastest. Call after
+ * gd2_ensemble_start. Synchronises the stream. */
+int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, double* best_stage_ms);
+
 /* Layout conversion between the reference AoS primitives and SoA conserved. */
 int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
                      const gd2_double3* b, double* U, void* stream);
