@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--copy-ref-mib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -147,6 +149,15 @@ def main():
         "stage2_ms": round(t2, 4),
         "bytes_per_launch": BYTES_PER_CELL_UPDATE / 2.0 * cells,
     }
+
+    if args.copy_ref_mib > 0:
+        nel = args.copy_ref_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

@@ -70,7 +70,7 @@ int tile_rows_for(int nx, int ny) {
     if (env > 0) return env;
     const long long tiles_x = (nx + kTileW - 1) / kTileW;
     int rows = 64;
-    while (rows > 8 && tiles_x * ((ny + rows - 1) / rows) < 4096) rows /= 2;
+    while (rows > 8 && tiles_x * ((ny + rows - 1) / rows) < 16384) rows /= 2;
     return rows;
 }
 

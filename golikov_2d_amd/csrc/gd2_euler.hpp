@@ -194,26 +194,23 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
         return v;
     };
 
-    // prologue: rows j0-2, j0-1 (y-stencil only), j0, j0+1 (also in LDS)
+    // prologue: rows j0-2, j0-1 (y-stencil only) and j0 (also in LDS). The
+    // first y-face G(j0-1/2) is produced by the loop's own pass j = j0-1, so
+    // every face goes through one instruction sequence and results do not
+    // depend on where the strips start.
     RowVal<LAYOUT> vm2 = load_row(j0 - 2, 0, false);
-    RowVal<LAYOUT> vm1 = load_row(j0 - 1, 0, false);
-    RowVal<LAYOUT> v0 = load_row(j0, j0 % 3, true);
-    RowVal<LAYOUT> v1 = load_row(j0 + 1, (j0 + 1) % 3, true);
-
-    Slope sm1 = slope3<ORDER>(vm2.q, vm1.q, v0.q);
-    Slope s0 = slope3<ORDER>(vm1.q, v0.q, v1.q);
+    RowVal<LAYOUT> v0 = load_row(j0 - 1, 0, false);
+    RowVal<LAYOUT> v1 = load_row(j0, j0 % 3, true);
+    Slope s0 = slope3<ORDER>(vm2.q, v0.q, v1.q);
     double smax;
     FluxE Gm;  // y-face flux G(j-1/2), frame (n = +y, t = +x)
-    {
-        PrimE L = face_plus(vm1.q, sm1), R = face_minus(v0.q, s0);
-        Gm = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
-    }
-    double tGm = (LAYOUT == kAoS) ? kry / smax : 0.0;
+    Gm.fr = Gm.fn = Gm.ft = Gm.fe = 0.0;
+    double tGm = 0.0;
 
     double wmax = 0.0;
     double tmin = 1e300;
 
-    for (int j = j0; j < j1; ++j) {
+    for (int j = j0 - 1; j < j1; ++j) {
         __syncthreads();  // slot (j+2)%3 = slot of row j-1, whose x-faces are done
         RowVal<LAYOUT> v2 = load_row(j + 2, (j + 2) % 3, true);
         // y-face G(j+1/2)
@@ -224,6 +221,14 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             PrimE L = face_plus(v0.q, s0), R = face_minus(v1.q, s1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
             tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
+        }
+        if (j < j0) {  // lead-in pass: only G(j0-1/2)
+            v0 = v1;
+            v1 = v2;
+            s0 = s1;
+            Gm = Gp;
+            tGm = tGp;
+            continue;
         }
         // row j reached LDS two iterations ago (before two barriers)
         // x-face F(c-1/2) of row j from cells c-2, c-1, c, c+1

@@ -400,26 +400,36 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         return q;
     };
 
+    // prologue: rows j0-2, j0-1 (y-stencil only) and j0 (also in LDS); the
+    // first y-face comes from the loop's lead-in pass j = j0-1
     PrimM qm2 = load_row(j0 - 2, 0, false);
-    PrimM qm1 = load_row(j0 - 1, 0, false);
-    PrimM q0 = load_row(j0, j0 % 3, true);
-    PrimM q1 = load_row(j0 + 1, (j0 + 1) % 3, true);
-    PrimM sm1 = slope3m<ORDER>(qm2, qm1, q0);
-    PrimM s0 = slope3m<ORDER>(qm1, q0, q1);
+    PrimM q0 = load_row(j0 - 1, 0, false);
+    PrimM q1 = load_row(j0, j0 % 3, true);
+    PrimM s0 = slope3m<ORDER>(qm2, q0, q1);
     double smax;
-    LabFluxM Gm = mhd_flux<METOD, false>(face_m(qm1, sm1, 0.5), face_m(q0, s0, -0.5), A.gamma, A.inv_g1, smax);
-    double tGm = (LAYOUT == kAoS) ? kry / smax : 0.0;
-    double by_s = qm1.by;  // cell-centred By of the south neighbour (Powell average)
+    LabFluxM Gm;
+    Gm.r = Gm.mx = Gm.my = Gm.bx = Gm.by = Gm.bz = Gm.e = 0.0;
+    double tGm = 0.0;
+    double by_s = qm2.by;  // cell-centred By of the south neighbour (Powell average)
 
     double wmax = 0.0, tmin = 1e300;
 
-    for (int j = j0; j < j1; ++j) {
+    for (int j = j0 - 1; j < j1; ++j) {
         __syncthreads();
         PrimM q2 = load_row(j + 2, (j + 2) % 3, true);
         PrimM s1 = slope3m<ORDER>(q0, q1, q2);
         const LabFluxM Gp =
             mhd_flux<METOD, false>(face_m(q0, s0, 0.5), face_m(q1, s1, -0.5), A.gamma, A.inv_g1, smax);
         const double tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
+        if (j < j0) {  // lead-in pass: only G(j0-1/2)
+            by_s = q0.by;
+            q0 = q1;
+            q1 = q2;
+            s0 = s1;
+            Gm = Gp;
+            tGm = tGp;
+            continue;
+        }
 
         // x-face F(c-1/2) from cells c-2, c-1, c, c+1 of row j (in LDS since two barriers)
         const int sl = j % 3;
