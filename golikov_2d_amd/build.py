@@ -14,6 +14,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
+    # contract a*b+c only within one source expression: the rounding of every
+    # expression is then fixed by the source, not by later code motion
+    "-ffp-contract=on",
     "--offload-arch=gfx950",
     # no packed-FP32 instructions (device code is FP64 throughout)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

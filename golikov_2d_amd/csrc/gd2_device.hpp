@@ -142,7 +142,9 @@ __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, d
     smax = fmax(fabs(SL), fabs(SR));
     const double suL = SL - unL, suR = SR - unR;
     const double mL = suL * rL, mR = suR * rR;
-    const double SM = (mR * unR - pR + pL - mL * unL) * fast_rcp(mR - mL);
+    const double smR = mR * unR;  // separately rounded products, as the reference
+    const double smL = mL * unL;
+    const double SM = (smR - pR + pL - smL) * fast_rcp(mR - mL);
     // averaged star total pressure (kernel.cu:1069-1071)
     const double ptz = 0.5 * ((pR + mR * (SM - unR)) + (pL + mL * (SM - unL)));
     const bool left = SM >= 0.0;

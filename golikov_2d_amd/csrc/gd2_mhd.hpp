@@ -47,7 +47,11 @@ __device__ __forceinline__ FluxM hlld_face(double ro_L, double p_L, double u1, d
     const double suR = SR - u2, suL = SL - u1;
     const double mR = suR * ro_R, mL = suL * ro_L;
     const double idm = fast_rcp(mR - mL);
-    const double SM = (mR * u2 - mL * u1 - pTR + pTL) * idm;
+    // separately rounded products: mirror-image states give SM = 0 exactly,
+    // as in the reference's evaluation order (Gas_Dinamic.cu:1071)
+    const double smR = mR * u2;
+    const double smL = mL * u1;
+    const double SM = (smR - smL - pTR + pTL) * idm;
     const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
     const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
 
@@ -348,6 +352,39 @@ __device__ __forceinline__ LabFluxM shfl_down_m(const LabFluxM& F) {
     return G;
 }
 
+// Mirror image across an x- (y-) wall: normal velocity and normal field negated.
+__device__ __forceinline__ PrimM mirror_x(PrimM q) {
+    q.u = -q.u;
+    q.bx = -q.bx;
+    return q;
+}
+
+__device__ __forceinline__ PrimM mirror_y(PrimM q) {
+    q.v = -q.v;
+    q.by = -q.by;
+    return q;
+}
+
+// Flux of the mirrored problem mapped back: the components odd under the
+// mirror change sign (mass, tangential momentum, energy, tangential field).
+__device__ __forceinline__ LabFluxM unmirror_x(LabFluxM F) {
+    F.r = -F.r;
+    F.my = -F.my;
+    F.by = -F.by;
+    F.bz = -F.bz;
+    F.e = -F.e;
+    return F;
+}
+
+__device__ __forceinline__ LabFluxM unmirror_y(LabFluxM F) {
+    F.r = -F.r;
+    F.mx = -F.mx;
+    F.bx = -F.bx;
+    F.bz = -F.bz;
+    F.e = -F.e;
+    return F;
+}
+
 template <int LAYOUT>
 __device__ __forceinline__ bool frozen_m(const MhdArgs& A, int c, int j) {
     if constexpr (LAYOUT == kAoS) {
@@ -418,8 +455,13 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         __syncthreads();
         PrimM q2 = load_row(j + 2, (j + 2) % 3, true);
         PrimM s1 = slope3m<ORDER>(q0, q1, q2);
-        const LabFluxM Gp =
-            mhd_flux<METOD, false>(face_m(q0, s0, 0.5), face_m(q1, s1, -0.5), A.gamma, A.inv_g1, smax);
+        // A reflective south wall face is evaluated from the interior cell's
+        // side (outward normal), as the reference's cell does; its HLLC-MHD /
+        // HLLD flux is discontinuous at SM = 0 when the normal field jumps.
+        const bool smir = (j == -1) && (A.bc[2] == GD2_BC_REFLECTIVE);
+        const PrimM yl = face_m(q0, s0, 0.5), yr = face_m(q1, s1, -0.5);
+        LabFluxM Gp = smir ? unmirror_y(mhd_flux<METOD, false>(mirror_y(yr), mirror_y(yl), A.gamma, A.inv_g1, smax))
+                           : mhd_flux<METOD, false>(yl, yr, A.gamma, A.inv_g1, smax);
         const double tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             by_s = q0.by;
@@ -438,7 +480,12 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         const PrimM qd = lds_get_m(ring, sl, lane + 3);
         const PrimM sb = slope3m<ORDER>(qa, qb, q0);
         const PrimM sc = slope3m<ORDER>(qb, q0, qd);
-        const LabFluxM Fl = mhd_flux<METOD, true>(face_m(qb, sb, 0.5), face_m(q0, sc, -0.5), A.gamma, A.inv_g1, smax);
+        // reflective west wall face from the interior side (see the south wall)
+        const bool wmir = (c == 0) && (A.bc[0] == GD2_BC_REFLECTIVE);
+        const PrimM xl = face_m(qb, sb, 0.5), xr = face_m(q0, sc, -0.5);
+        LabFluxM Fl = mhd_flux<METOD, true>(wmir ? mirror_x(xr) : xl, wmir ? mirror_x(xl) : xr, A.gamma, A.inv_g1,
+                                            smax);
+        if (wmir) Fl = unmirror_x(Fl);
         const double tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
         const LabFluxM Fr = shfl_down_m(Fl);
         const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
