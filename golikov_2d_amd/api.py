@@ -99,6 +99,9 @@ def lib():
             "gd2_ensemble_advance": (ip, [vp, ip, ip, vp]),
             "gd2_ensemble_time": (ip, [vp, ip, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
             "gd2_ensemble_tune": (ip, [vp, vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]),
+            "gd2_ctx_log_dt": (ip, [vp, vp, C.c_longlong]),
+            "gd2_vjp_workspace_bytes": (C.c_longlong, [gp]),
+            "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -113,7 +116,8 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_prim_to_cons", "gd2_cons_to_prim", "gd2_ctx_create", "gd2_ctx_destroy", "gd2_ctx_start",
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
-            "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune"]
+            "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
+            "gd2_vjp_workspace_bytes", "gd2_stage_vjp"]
 
 
 def _check(rc, name):
@@ -191,6 +195,30 @@ def stage(g, sc, Uin, Ubase, Uout, a=0.0, b=1.0, dt=0.0, dt_dev=None, wmax_bits=
                            float(dt), _f64(dt_dev), _p(wmax_bits), _stream(stream)), "gd2_stage")
 
 
+def vjp_workspace_bytes(g) -> int:
+    n = lib().gd2_vjp_workspace_bytes(C.byref(g))
+    if n < 0:
+        raise GD2Error(f"gd2_vjp_workspace_bytes returned {n}")
+    return n
+
+
+def vjp_workspace(g, device):
+    return torch.empty(vjp_workspace_bytes(g) // 8, dtype=torch.float64, device=device)
+
+
+def stage_vjp(g, sc, U, V, W, c=1.0, A=None, a=0.0, B=None, b=0.0, workspace=None, stream=None):
+    """W = a*A + b*B + c * J_L(U)^T V for the Euler HLLC stage operator L."""
+    for t in (U, V, W, A, B):
+        if t is not None and tuple(t.shape) != (4, g.ny, g.nx):
+            raise GD2Error(f"state shape {tuple(t.shape)} is not (4, {g.ny}, {g.nx})")
+    if workspace is None:
+        workspace = vjp_workspace(g, U.device)
+    elif workspace.numel() * workspace.element_size() < vjp_workspace_bytes(g):
+        raise GD2Error("workspace smaller than gd2_vjp_workspace_bytes")
+    _check(lib().gd2_stage_vjp(C.byref(g), C.byref(sc), _f64(U), _f64(V), _f64(A), float(a), _f64(B), float(b),
+                               float(c), _f64(W), _p(workspace), _stream(stream)), "gd2_stage_vjp")
+
+
 def bits_to_double(t):
     """View an int64 device scalar holding double bits as a Python float."""
     return t.view(torch.float64).item()
@@ -245,6 +273,14 @@ class Solver:
         t, dt, n = C.c_double(), C.c_double(), C.c_longlong()
         _check(lib().gd2_ctx_time(self._h, C.byref(t), C.byref(n), C.byref(dt), _stream(stream)), "gd2_ctx_time")
         return t.value, n.value, dt.value
+
+    def log_dt(self, dt_log):
+        """Record the dt of every following step into dt_log[step] (float64 device tensor; None: off)."""
+        if dt_log is None:
+            _check(lib().gd2_ctx_log_dt(self._h, None, 0), "gd2_ctx_log_dt")
+        else:
+            _check(lib().gd2_ctx_log_dt(self._h, _f64(dt_log), dt_log.numel()), "gd2_ctx_log_dt")
+        self._dt_log = dt_log
 
     def run_to(self, U, t_end, chunk=32, max_steps=10 ** 7):
         """Advance until t_end, checking the device clock every `chunk` steps."""

@@ -1,0 +1,519 @@
+// gd2_adjoint.hpp -- discrete adjoint (vector-Jacobian product) of the Euler
+// HLLC + minmod-MUSCL stage operator on gfx950.
+//
+// For a stage operator L(U) = -(1/dV) sum_faces (outward flux x face length)
+// the kernel computes, per cell,
+//     W = a*A + b*B + c * J_L(U)^T V
+// with V a cotangent field (4 SoA planes). J_L^T V is formed face by face:
+// the face cotangent phi = (V_R - V_L)/h goes through the reverse-mode HLLC
+// (hllc_vjp) to cotangents on the two reconstructed face states, which each
+// cell spreads over its 3-cell minmod stencil with the branch taken in the
+// forward pass (subgradient 1/2-1/2 on ties, 0 at the zero branch). Cells
+// gather their contributions without atomics: x-neighbours by cross-lane
+// shuffles, y-neighbours by a register march that finishes row j one row
+// later. Cotangents on ghost cells go to a small side buffer and are folded
+// into the cells they copy (transmissive / reflective / periodic) by
+// adj_fold_kernel.
+#pragma once
+
+#include "gd2_euler.hpp"
+
+namespace gd2 {
+
+constexpr int kAdjW = 61;    // interior columns per wave (lanes 1..61)
+constexpr int kAdjBuf = 68;  // LDS row: columns cbase-3 .. cbase+63
+
+struct FaceE {
+    double r, un, ut, p;
+};
+
+// Reverse-mode HLLC (same forward formulas as hllc_euler): given the flux
+// cotangent Fb (mass, normal, tangential, energy) return the cotangents of
+// the left and right face states.
+__device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double gamma, double inv_g1, const FluxE& Fb,
+                                         FaceE& Lb, FaceE& Rb) {
+    // forward
+    const double iL = fast_rcp(L.r), iR = fast_rcp(R.r);
+    const double cL = fast_sqrt(gamma * L.p * iL);
+    const double cR = fast_sqrt(gamma * R.p * iR);
+    const double aL = L.un - cL, aR = R.un - cR;
+    const double bL = L.un + cL, bR = R.un + cR;
+    const double SL = fmin(aL, aR), SR = fmax(bL, bR);
+    const bool slL = aL <= aR, srL = bL >= bR;
+    const double suL = SL - L.un, suR = SR - R.un;
+    const double mL = suL * L.r, mR = suR * R.r;
+    const double smR = mR * R.un;
+    const double smL = mL * L.un;
+    const double D = mR - mL;
+    const double iD = fast_rcp(D);
+    const double Nn = smR - R.p + L.p - smL;
+    const double SM = Nn * iD;
+    const double ptz = 0.5 * ((R.p + mR * (SM - R.un)) + (L.p + mL * (SM - L.un)));
+    const bool left = SM >= 0.0;
+    const double S = left ? SL : SR;
+    const double r = left ? L.r : R.r;
+    const double un = left ? L.un : R.un;
+    const double ut = left ? L.ut : R.ut;
+    const double p = left ? L.p : R.p;
+    const double su = left ? suL : suR;
+    const bool super = (SL > 0.0) || (SR < 0.0);
+    const double ke = 0.5 * (un * un + ut * ut);
+    const double e = p * inv_g1 + r * ke;
+    const double den = super ? 1.0 : (S - SM);
+    const double inv = fast_rcp(den);
+    const double sum = su * inv;
+    const double rz = r * sum;
+    const double t = ptz * SM - p * un;
+    const double ez = e * sum + t * inv;
+    const double Se = super ? 0.0 : S;
+    const double run = r * un;
+
+    // reverse
+    const double b_Se = Fb.fr * (rz - r) + Fb.fn * (SM * rz - run) + Fb.ft * (ut * rz - r * ut) + Fb.fe * (ez - e);
+    double b_run = Fb.fr + Fb.fn * (un - Se) + Fb.ft * ut;
+    const double b_rz = Se * (Fb.fr + Fb.fn * SM + Fb.ft * ut);
+    double b_r = -Se * (Fb.fr + Fb.ft * ut);
+    double b_un = Fb.fn * run + Fb.fe * (e + p);
+    double b_p = Fb.fn + Fb.fe * un;
+    double b_SM = Fb.fn * Se * rz;
+    double b_ut = Fb.ft * (run + Se * (rz - r));
+    const double b_ez = Fb.fe * Se;
+    double b_e = Fb.fe * (un - Se);
+    b_r += b_run * un;
+    b_un += b_run * r;
+    double b_S = super ? 0.0 : b_Se;
+    b_e += b_ez * sum;
+    double b_sum = b_ez * e;
+    const double b_t = b_ez * inv;
+    double b_inv = b_ez * t;
+    const double b_ptz = b_t * SM;
+    b_SM += b_t * ptz;
+    b_p -= b_t * un;
+    b_un -= b_t * p;
+    b_r += b_rz * sum;
+    b_sum += b_rz * r;
+    const double b_su = b_sum * inv;
+    b_inv += b_sum * su;
+    const double b_den = -inv * inv * b_inv;
+    if (!super) {
+        b_S += b_den;
+        b_SM -= b_den;
+    }
+    b_p += b_e * inv_g1;
+    b_r += b_e * ke;
+    b_un += b_e * r * un;
+    b_ut += b_e * r * ut;
+
+    FaceE bl = {0.0, 0.0, 0.0, 0.0}, br = {0.0, 0.0, 0.0, 0.0};
+    double b_SL = 0.0, b_SR = 0.0, b_suL = 0.0, b_suR = 0.0;
+    if (left) {
+        bl.r += b_r; bl.un += b_un; bl.ut += b_ut; bl.p += b_p;
+        b_SL += b_S;
+        b_suL += b_su;
+    } else {
+        br.r += b_r; br.un += b_un; br.ut += b_ut; br.p += b_p;
+        b_SR += b_S;
+        b_suR += b_su;
+    }
+    const double hp = 0.5 * b_ptz;
+    br.p += hp;
+    double b_mR = hp * (SM - R.un);
+    b_SM += hp * (mR + mL);
+    br.un -= hp * mR;
+    bl.p += hp;
+    double b_mL = hp * (SM - L.un);
+    bl.un -= hp * mL;
+    const double b_N = b_SM * iD;
+    const double b_iD = b_SM * Nn;
+    br.p -= b_N;
+    bl.p += b_N;
+    const double b_D = -iD * iD * b_iD;
+    b_mR += b_D;
+    b_mL -= b_D;
+    b_mR += b_N * R.un;
+    br.un += b_N * mR;
+    b_mL -= b_N * L.un;
+    bl.un -= b_N * mL;
+    b_suL += b_mL * L.r;
+    bl.r += b_mL * suL;
+    b_suR += b_mR * R.r;
+    br.r += b_mR * suR;
+    b_SL += b_suL;
+    bl.un -= b_suL;
+    b_SR += b_suR;
+    br.un -= b_suR;
+    double b_cL = 0.0, b_cR = 0.0;
+    if (slL) { bl.un += b_SL; b_cL -= b_SL; } else { br.un += b_SL; b_cR -= b_SL; }
+    if (srL) { bl.un += b_SR; b_cL += b_SR; } else { br.un += b_SR; b_cR += b_SR; }
+    // c = sqrt(gamma p / r): dc/dp = gamma / (2 r c), dc/dr = -c / (2 r)
+    const double icL = fast_rcp(cL), icR = fast_rcp(cR);
+    bl.p += b_cL * 0.5 * gamma * iL * icL;
+    bl.r -= b_cL * 0.5 * cL * iL;
+    br.p += b_cR * 0.5 * gamma * iR * icR;
+    br.r -= b_cR * 0.5 * cR * iR;
+    Lb = bl;
+    Rb = br;
+}
+
+// minmod branch weights of the middle cell's slope: the slope is
+// alpha*(q_c - q_{c-1}) + beta*(q_{c+1} - q_c) on the branch taken
+// (alpha = 1: first argument, beta = 1: second, 1/2-1/2 on a tie, 0-0 when the
+// arguments' signs differ or one is zero).
+struct MMW {
+    double a, b;
+};
+
+__device__ __forceinline__ MMW mm_weights(double dm, double dp) {
+    MMW w;
+    const double am = fabs(dm), ap = fabs(dp);
+    const bool zero = !(dm * dp > 0.0);
+    w.a = zero ? 0.0 : (am < ap ? 1.0 : (am > ap ? 0.0 : 0.5));
+    w.b = zero ? 0.0 : (ap < am ? 1.0 : (ap > am ? 0.0 : 0.5));
+    return w;
+}
+
+// One cell's share of the reconstruction adjoint along one direction and one
+// field: given the cotangents of its plus-face value (qp = q + mm/2) and its
+// minus-face value (qm = q - mm/2), with fp / fm set where the rho/p <= 0
+// fall-back replaced the face value by the cell value, return the parts that
+// belong to the cell itself, to the previous cell and to the next cell.
+__device__ __forceinline__ void spread(double dqp, double dqm, const MMW& w, bool fp, bool fm, double& own,
+                                       double& to_prev, double& to_next) {
+    const double g = 0.5 * ((fp ? 0.0 : dqp) - (fm ? 0.0 : dqm));
+    own = dqp + dqm + (w.a - w.b) * g;
+    to_prev = -w.a * g;
+    to_next = w.b * g;
+}
+
+struct AdjArgs {
+    const double* U;    // state the stage operator is linearised at
+    const double* V;    // cotangent
+    const double* A;    // optional
+    const double* B;    // optional
+    double* W;          // output
+    double a, b, c;     // W = a*A + b*B + c*J^T V
+    double* ghost;      // side buffer: [4 sides][2 layers][len][4]
+    int nx, ny;
+    long long plane;
+    double dx, dy, idx, idy;
+    double gamma, g1, inv_g1;
+    int bc[4];
+    int tiles_x, rows;
+    int gstride;        // max(nx, ny)
+};
+
+struct Q4 {
+    double r, u, v, p;
+};
+
+__device__ __forceinline__ Q4 q4_zero() {
+    Q4 z = {0.0, 0.0, 0.0, 0.0};
+    return z;
+}
+
+__device__ __forceinline__ Q4 q4_add(const Q4& a, const Q4& b) {
+    Q4 s = {a.r + b.r, a.u + b.u, a.v + b.v, a.p + b.p};
+    return s;
+}
+
+__device__ __forceinline__ Q4 shfl_down_q(const Q4& x, int d) {
+    Q4 y;
+    y.r = __shfl_down(x.r, d, kWave);
+    y.u = __shfl_down(x.u, d, kWave);
+    y.v = __shfl_down(x.v, d, kWave);
+    y.p = __shfl_down(x.p, d, kWave);
+    return y;
+}
+
+__device__ __forceinline__ Q4 shfl_up_q(const Q4& x, int d) {
+    Q4 y;
+    y.r = __shfl_up(x.r, d, kWave);
+    y.u = __shfl_up(x.u, d, kWave);
+    y.v = __shfl_up(x.v, d, kWave);
+    y.p = __shfl_up(x.p, d, kWave);
+    return y;
+}
+
+// Cell-level reconstruction adjoint for one direction: slopes from
+// (qm1, q, qp1), face-value fall-backs from the forward formulas.
+template <int ORDER>
+__device__ __forceinline__ void spread_cell(const PrimE& qm1, const PrimE& q, const PrimE& qp1, const Q4& dqp,
+                                            const Q4& dqm, Q4& own, Q4& to_prev, Q4& to_next) {
+    if constexpr (ORDER == 1) {
+        own = q4_add(dqp, dqm);
+        to_prev = q4_zero();
+        to_next = q4_zero();
+    } else {
+        const MMW wr = mm_weights(q.r - qm1.r, qp1.r - q.r);
+        const MMW wu = mm_weights(q.u - qm1.u, qp1.u - q.u);
+        const MMW wv = mm_weights(q.v - qm1.v, qp1.v - q.v);
+        const MMW wp = mm_weights(q.p - qm1.p, qp1.p - q.p);
+        const Slope s = slope3<2>(qm1, q, qp1);
+        const bool fpr = !(q.r + 0.5 * s.r > 0.0), fmr = !(q.r - 0.5 * s.r > 0.0);
+        const bool fpp = !(q.p + 0.5 * s.p > 0.0), fmp = !(q.p - 0.5 * s.p > 0.0);
+        spread(dqp.r, dqm.r, wr, fpr, fmr, own.r, to_prev.r, to_next.r);
+        spread(dqp.u, dqm.u, wu, false, false, own.u, to_prev.u, to_next.u);
+        spread(dqp.v, dqm.v, wv, false, false, own.v, to_prev.v, to_next.v);
+        spread(dqp.p, dqm.p, wp, fpp, fmp, own.p, to_prev.p, to_next.p);
+    }
+}
+
+// d(prim)/d(cons) transposed at a cell with density r and velocity (u, v):
+// prim = (r, mx/r, my/r, g1 (E - (mx^2 + my^2)/(2 r))).
+__device__ __forceinline__ void cons_vjp(double r, double u, double v, double g1, const Q4& dq, double out[4]) {
+    const double inv = fast_rcp(r);
+    out[3] = g1 * dq.p;
+    out[1] = dq.u * inv - g1 * u * dq.p;
+    out[2] = dq.v * inv - g1 * v * dq.p;
+    out[0] = dq.r - (u * dq.u + v * dq.v) * inv + g1 * 0.5 * (u * u + v * v) * dq.p;
+}
+
+__device__ __forceinline__ void lds_put8(double (*ring)[8][kAdjBuf], int slot, int e, const PrimE& q, const double* V) {
+    ring[slot][0][e] = q.r;
+    ring[slot][1][e] = q.u;
+    ring[slot][2][e] = q.v;
+    ring[slot][3][e] = q.p;
+    ring[slot][4][e] = V[0];
+    ring[slot][5][e] = V[1];
+    ring[slot][6][e] = V[2];
+    ring[slot][7][e] = V[3];
+}
+
+__device__ __forceinline__ PrimE lds_q(double (*ring)[8][kAdjBuf], int slot, int e) {
+    PrimE q;
+    q.r = ring[slot][0][e];
+    q.u = ring[slot][1][e];
+    q.v = ring[slot][2][e];
+    q.p = ring[slot][3][e];
+    return q;
+}
+
+// ghost side buffer index: side 0 west, 1 east, 2 south, 3 north; layer 0 is
+// the ghost next to the boundary, layer 1 the outer one
+__device__ __forceinline__ double* ghost_at(const AdjArgs& A, int side, int layer, int idx) {
+    return A.ghost + (((long long)(side * 2 + layer) * A.gstride + idx) * 4);
+}
+
+__device__ __forceinline__ void ghost_store(const AdjArgs& A, int side, int layer, int idx, const Q4& d) {
+    double* g = ghost_at(A, side, layer, idx);
+    g[0] = d.r;
+    g[1] = d.u;
+    g[2] = d.v;
+    g[3] = d.p;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
+    __shared__ double ring[3][8][kAdjBuf];
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    const int cbase = tx * kAdjW;
+    const int j0 = ty * A.rows;
+    const int j1 = min(j0 + A.rows, A.ny);
+    const int c = cbase - 1 + lane;
+    const int nx = A.nx, ny = A.ny;
+    const long long P = A.plane;
+
+    bool fx, hfx;
+    const int ci = map_index(c, nx, A.bc[0], A.bc[1], fx);
+    const int hcol = lane == 0 ? cbase - 3 : (lane == 1 ? cbase - 2 : cbase + 63);
+    const int hci = map_index(hcol, nx, A.bc[0], A.bc[1], hfx);
+    const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
+    const bool has_halo = lane < 3;
+    const bool col_real = c >= 0 && c < nx;
+    const bool xface_on = c >= 0 && c <= nx;  // face c-1/2 belongs to the forward map
+    const bool interior_lane = lane >= 1 && lane <= kAdjW;
+
+    // the strip's output rows include the ghost rows on the domain's edges
+    const int ro0 = (j0 == 0) ? -2 : j0;
+    const int ro1 = (j1 == ny) ? ny + 2 : j1;
+
+    auto load = [&](int jl, int slot, bool to_lds, double Vv[4]) -> PrimE {
+        bool fy;
+        const int jj = map_index(jl, ny, A.bc[2], A.bc[3], fy);
+        const long long rowk = (long long)jj * nx;
+        const long long k = rowk + ci;
+        PrimE q = cons_to_prim_e(A.U[k], A.U[P + k], A.U[2 * P + k], A.U[3 * P + k], A.g1);
+        if (fx) q.u = -q.u;
+        if (fy) q.v = -q.v;
+        // ghost cells have no output in the forward map: their cotangent is 0
+        const bool row_real = jl >= 0 && jl < ny;
+        const bool real = row_real && col_real;
+        Vv[0] = real ? A.V[k] : 0.0;
+        Vv[1] = real ? A.V[P + k] : 0.0;
+        Vv[2] = real ? A.V[2 * P + k] : 0.0;
+        Vv[3] = real ? A.V[3 * P + k] : 0.0;
+        if (to_lds) {
+            lds_put8(ring, slot, lane + 2, q, Vv);
+            if (has_halo) {
+                const long long kh = rowk + hci;
+                PrimE h = cons_to_prim_e(A.U[kh], A.U[P + kh], A.U[2 * P + kh], A.U[3 * P + kh], A.g1);
+                if (hfx) h.u = -h.u;
+                if (fy) h.v = -h.v;
+                const bool hreal = row_real && hcol >= 0 && hcol < nx;
+                double Vh[4] = {hreal ? A.V[kh] : 0.0, hreal ? A.V[P + kh] : 0.0, hreal ? A.V[2 * P + kh] : 0.0,
+                                hreal ? A.V[3 * P + kh] : 0.0};
+                lds_put8(ring, slot, hslot, h, Vh);
+            }
+        }
+        return q;
+    };
+
+    // rows ro0-3 .. ro0: prims; ro0-2, ro0-1 cotangents
+    double Vd[4];
+    PrimE qa = load(ro0 - 3, 0, false, Vd);  // q_{j-1} at j = ro0-2
+    PrimE qb = load(ro0 - 2, 0, false, Vd);  // q_j
+    double Vb[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
+    PrimE qc = load(ro0 - 1, ((ro0 - 1) % 3 + 3) % 3, true, Vd);  // q_{j+1}
+    double Vc[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
+    PrimE qd;
+
+    Q4 dqm_j = q4_zero();     // cotangent of row j's minus (south) face value, from G(j-1/2)
+    Q4 own_prev = q4_zero();  // row j-1: own y part
+    Q4 x_prev = q4_zero();    // row j-1: x gather
+    Q4 rsp_prev2 = q4_zero(); // spill of row j-2 to row j-1
+    Q4 rsp_prev = q4_zero();  // spill of row j-1 to row j
+
+    for (int j = ro0 - 2; j <= ro1; ++j) {
+        __syncthreads();
+        qd = load(j + 2, ((j + 2) % 3 + 3) % 3, true, Vd);  // q_{j+2}
+        const double Vn[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
+
+        // y-face G(j+1/2) between rows j and j+1 (forward face iff -1 <= j <= ny-1, real column)
+        Q4 dqp_j, dqm_n;
+        {
+            const Slope sj = slope3<ORDER>(qa, qb, qc);
+            const Slope sn = slope3<ORDER>(qb, qc, qd);
+            const PrimE L = face_plus(qb, sj), R = face_minus(qc, sn);
+            const bool on = col_real && j >= -1 && j <= ny - 1;
+            FluxE Fb;  // face cotangent (V_top - V_bottom)/dy in the y-frame (n = y, t = x)
+            Fb.fr = on ? (Vc[0] - Vb[0]) * A.idy : 0.0;
+            Fb.ft = on ? (Vc[1] - Vb[1]) * A.idy : 0.0;
+            Fb.fn = on ? (Vc[2] - Vb[2]) * A.idy : 0.0;
+            Fb.fe = on ? (Vc[3] - Vb[3]) * A.idy : 0.0;
+            FaceE fl = {L.r, L.v, L.u, L.p}, fr = {R.r, R.v, R.u, R.p}, bl, br;
+            hllc_vjp(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
+            dqp_j.r = bl.r; dqp_j.u = bl.ut; dqp_j.v = bl.un; dqp_j.p = bl.p;
+            dqm_n.r = br.r; dqm_n.u = br.ut; dqm_n.v = br.un; dqm_n.p = br.p;
+        }
+        // row j's y spread
+        Q4 own_j, lsp_j, rsp_j;
+        spread_cell<ORDER>(qa, qb, qc, dqp_j, dqm_j, own_j, lsp_j, rsp_j);
+
+        // row j's x gather (real rows only)
+        Q4 x_j = q4_zero();
+        if (j >= j0 && j < j1) {
+            const int sl = ((j % 3) + 3) % 3;
+            // x-face F(c-1/2) from cells c-2, c-1 (LDS), c (own), c+1 (LDS)
+            const PrimE q2 = lds_q(ring, sl, lane), q1 = lds_q(ring, sl, lane + 1), q3 = lds_q(ring, sl, lane + 3);
+            const Slope sb = slope3<ORDER>(q2, q1, qb);
+            const Slope sc = slope3<ORDER>(q1, qb, q3);
+            const PrimE L = face_plus(q1, sb), R = face_minus(qb, sc);
+            const double Vw0 = ring[sl][4][lane + 1], Vw1 = ring[sl][5][lane + 1];
+            const double Vw2 = ring[sl][6][lane + 1], Vw3 = ring[sl][7][lane + 1];
+            FluxE Fb;
+            Fb.fr = xface_on ? (Vb[0] - Vw0) * A.idx : 0.0;
+            Fb.fn = xface_on ? (Vb[1] - Vw1) * A.idx : 0.0;
+            Fb.ft = xface_on ? (Vb[2] - Vw2) * A.idx : 0.0;
+            Fb.fe = xface_on ? (Vb[3] - Vw3) * A.idx : 0.0;
+            FaceE fl = {L.r, L.u, L.v, L.p}, fr = {R.r, R.u, R.v, R.p}, bl, br;
+            hllc_vjp(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
+            const Q4 dqL = {bl.r, bl.un, bl.ut, bl.p};  // cotangent of q_plus of cell c-1
+            const Q4 dqR = {br.r, br.un, br.ut, br.p};  // cotangent of q_minus of cell c
+            // this lane's cell c: dqp from face c+1/2 (next lane's dqL), dqm = dqR
+            Q4 dqp_c = shfl_down_q(dqL, 1);
+            if (lane == kWave - 1) dqp_c = q4_zero();
+            Q4 own_c, lsp_c, rsp_c;
+            spread_cell<ORDER>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
+            Q4 from_next = shfl_down_q(lsp_c, 1);
+            if (lane == kWave - 1) from_next = q4_zero();
+            Q4 from_prev = shfl_up_q(rsp_c, 1);
+            if (lane == 0) from_prev = q4_zero();
+            x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            // ghost columns of the domain edges (east ones from the last tile only)
+            if (c == -1) {
+                ghost_store(A, 0, 0, j, x_j);
+                ghost_store(A, 0, 1, j, lsp_c);
+            }
+            if (tx == A.tiles_x - 1) {
+                if (c == nx) ghost_store(A, 1, 0, j, x_j);
+                if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
+            }
+        }
+
+        // finish row r = j-1: x gather + own y + spill from r-1 + spill from r+1 (= row j)
+        const int r = j - 1;
+        if (r >= ro0 && r < ro1) {
+            Q4 tot = q4_add(q4_add(x_prev, own_prev), q4_add(rsp_prev2, lsp_j));
+            if (r >= 0 && r < ny) {
+                if (col_real && interior_lane) {
+                    double du[4];
+                    cons_vjp(qa.r, qa.u, qa.v, A.g1, tot, du);  // qa = prims of row j-1
+                    const long long k = (long long)r * nx + c;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        double w = A.c * du[f];
+                        if (A.A) w += A.a * A.A[f * P + k];
+                        if (A.B) w += A.b * A.B[f * P + k];
+                        A.W[f * P + k] = w;
+                    }
+                }
+            } else if (col_real && interior_lane) {
+                // ghost rows: y contributions only
+                if (r == -1) ghost_store(A, 2, 0, c, tot);
+                if (r == -2) ghost_store(A, 2, 1, c, tot);
+                if (r == ny) ghost_store(A, 3, 0, c, tot);
+                if (r == ny + 1) ghost_store(A, 3, 1, c, tot);
+            }
+        }
+
+        // advance the window
+        x_prev = x_j;
+        own_prev = own_j;
+        rsp_prev2 = rsp_prev;
+        rsp_prev = rsp_j;
+        dqm_j = dqm_n;
+        qa = qb;
+        qb = qc;
+        qc = qd;
+        Vb[0] = Vc[0]; Vb[1] = Vc[1]; Vb[2] = Vc[2]; Vb[3] = Vc[3];
+        Vc[0] = Vn[0]; Vc[1] = Vn[1]; Vc[2] = Vn[2]; Vc[3] = Vn[3];
+    }
+}
+
+// Fold the ghost-cell cotangents of one side into the cells they copy:
+// W(target) += c * (d prim/d cons)^T (M dq_ghost), M negating the normal
+// velocity on reflective sides. One thread per boundary index; the two
+// layers of a side are summed by the same thread, sides run one after another.
+__global__ void adj_fold_kernel(const AdjArgs A, int side) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool xs = side < 2;
+    const int len = xs ? A.ny : A.nx;
+    if (i >= len) return;
+    const int n = xs ? A.nx : A.ny;
+    const int bc = A.bc[side];
+    const long long P = A.plane;
+    for (int layer = 0; layer < 2; ++layer) {
+        const double* g = ghost_at(A, side, layer, i);
+        Q4 d = {g[0], g[1], g[2], g[3]};
+        // logical ghost index along the normal direction
+        const int k = (side == 0 || side == 2) ? -1 - layer : n + layer;
+        bool flip;
+        const int src = map_index(k, n, side == 0 || side == 2 ? bc : A.bc[side - 1],
+                                  side == 1 || side == 3 ? bc : A.bc[side + 1], flip);
+        if (flip) {
+            if (xs) d.u = -d.u;
+            else d.v = -d.v;
+        }
+        const long long cell = xs ? (long long)i * A.nx + src : (long long)src * A.nx + i;
+        const double r = A.U[cell], mx = A.U[P + cell], my = A.U[2 * P + cell], E = A.U[3 * P + cell];
+        const PrimE q = cons_to_prim_e(r, mx, my, E, A.g1);
+        double du[4];
+        cons_vjp(q.r, q.u, q.v, A.g1, d, du);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) A.W[f * P + cell] += A.c * du[f];
+    }
+}
+
+}  // namespace gd2

@@ -15,6 +15,7 @@
 #include "gd2_euler.hpp"
 #include "gd2_mhd.hpp"
 #include "gd2_ensemble.hpp"
+#include "gd2_adjoint.hpp"
 
 namespace gd2 {
 
@@ -124,7 +125,11 @@ static bool scheme_ok(const gd2_scheme* sc) {
 }
 
 // ---------------------------------------------------------------- clock
-__global__ void clock_begin_kernel(Clock* c, int cur, double t_end) { clock_begin(c, cur, t_end); }
+__global__ void clock_begin_kernel(Clock* c, int cur, double t_end, double* dt_log, long long cap) {
+    const long long n = c->step;
+    clock_begin(c, cur, t_end);
+    if (dt_log && c->step > n && n < cap) dt_log[n] = c->dt;  // dt of step n (taken steps only)
+}
 
 __global__ void clock_start_kernel(Clock* c, double t0, double cfl) {
     c->t = t0;
@@ -201,6 +206,8 @@ struct gd2_ctx {
     double* scratch;  // one state (nfields planes)
     Clock* clock;
     long long host_steps;  // steps enqueued (parity of the wave-speed slots)
+    double* dt_log;        // optional device log of each step's dt
+    long long dt_cap;
 };
 
 extern "C" {
@@ -348,7 +355,67 @@ int gd2_ctx_create(const gd2_grid* g, const gd2_scheme* sc, gd2_ctx** out) {
         return GD2_ERR_NOMEM;
     }
     c->host_steps = 0;
+    c->dt_log = nullptr;
+    c->dt_cap = 0;
     *out = c;
+    return GD2_OK;
+}
+
+int gd2_ctx_log_dt(gd2_ctx* c, double* dt_log, long long capacity) {
+    if (!c || capacity < 0 || (capacity > 0 && !dt_log)) return GD2_ERR_ARG;
+    c->dt_log = capacity > 0 ? dt_log : nullptr;
+    c->dt_cap = capacity;
+    return GD2_OK;
+}
+
+long long gd2_vjp_workspace_bytes(const gd2_grid* g) {
+    if (!g || g->nx < 1 || g->ny < 1) return GD2_ERR_ARG;
+    const long long len = g->nx > g->ny ? g->nx : g->ny;
+    return (long long)sizeof(double) * 4 * 2 * len * 4;
+}
+
+int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V, const double* A,
+                  double a, const double* B, double b, double c, double* W, void* workspace, void* stream) {
+    if (!grid_ok(g, false) || !scheme_ok(sc) || !U || !V || !W || !workspace) return GD2_ERR_ARG;
+    if (W == U || W == V) return GD2_ERR_ARG;
+    if (sc->physics != GD2_EULER || sc->flux != GD2_FLUX_HLLC) return GD2_ERR_UNSUPPORTED;
+    AdjArgs J;
+    std::memset(&J, 0, sizeof J);
+    J.U = U;
+    J.V = V;
+    J.A = A;
+    J.B = B;
+    J.W = W;
+    J.a = a;
+    J.b = b;
+    J.c = c;
+    J.ghost = static_cast<double*>(workspace);
+    J.nx = g->nx;
+    J.ny = g->ny;
+    J.plane = (long long)g->nx * g->ny;
+    J.dx = g->dx;
+    J.dy = g->dy;
+    J.idx = 1.0 / g->dx;
+    J.idy = 1.0 / g->dy;
+    J.gamma = g->gamma;
+    J.g1 = g->gamma - 1.0;
+    J.inv_g1 = 1.0 / (g->gamma - 1.0);
+    for (int s2 = 0; s2 < 4; ++s2) J.bc[s2] = g->bc[s2];
+    J.rows = tile_rows_for(g->nx, g->ny);
+    J.tiles_x = (g->nx + kAdjW - 1) / kAdjW;
+    J.gstride = g->nx > g->ny ? g->nx : g->ny;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned blocks = (unsigned)(J.tiles_x * ((g->ny + J.rows - 1) / J.rows));
+    if (sc->order == 2)
+        hipLaunchKernelGGL(euler_vjp_kernel<2>, dim3(blocks), dim3(kWave), 0, st, J);
+    else
+        hipLaunchKernelGGL(euler_vjp_kernel<1>, dim3(blocks), dim3(kWave), 0, st, J);
+    GD2_HIP(hipGetLastError());
+    for (int side = 0; side < 4; ++side) {
+        const int len = side < 2 ? g->ny : g->nx;
+        hipLaunchKernelGGL(adj_fold_kernel, dim3((len + 255) / 256), dim3(256), 0, st, J, side);
+        GD2_HIP(hipGetLastError());
+    }
     return GD2_OK;
 }
 
@@ -378,7 +445,7 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
     double* cur = U;  // forward Euler ping-pongs between U and S
     for (int i = 0; i < nsteps; ++i) {
         const int slot = (int)(c->host_steps & 1);
-        hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end);
+        hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log, c->dt_cap);
         GD2_HIP(hipGetLastError());
         unsigned long long* wn = &c->clock->w[slot ^ 1];
         int rc;
