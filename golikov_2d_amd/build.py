@@ -1,4 +1,5 @@
-"""Build libgd2.so (HIP, gfx950) in-tree with hipcc."""
+"""Build libgd2.so (HIP, gfx950) with hipcc and the LibTorch adjoint module
+gd2_torch (host C++ over the libgd2 C ABI) with g++, both in-tree."""
 from __future__ import annotations
 
 import os
@@ -25,6 +26,37 @@ FLAGS = [
 ]
 
 
+TORCH_SRC = "gd2_torch.cpp"
+
+
+def torch_module_path() -> str:
+    import sysconfig
+    return os.path.join(HERE, "gd2_torch" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_torch_module(verbose: bool = False) -> str:
+    """g++ build of the pybind11/LibTorch module, linked against libgd2.so."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    out = torch_module_path()
+    tlib = ce.library_paths()[0]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-w",
+           "-DTORCH_EXTENSION_NAME=gd2_torch", "-DTORCH_API_INCLUDE_EXTENSION_H",
+           f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}",
+           *[f"-I{p}" for p in ce.include_paths()], f"-I{sysconfig.get_paths()['include']}",
+           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, TORCH_SRC),
+           f"-L{tlib}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-Wl,-rpath,{tlib}",
+           f"-L{HERE}", "-lgd2", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose: bool = False, extra=()) -> str:
     cmd = [HIPCC, *FLAGS, *extra, "-I", os.path.join(ROOT, "include"),
            *[os.path.join(CSRC, s) for s in SOURCES], "-o", SO + ".tmp"]
@@ -36,4 +68,6 @@ def build(verbose: bool = False, extra=()) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True, extra=sys.argv[1:])
+    build(verbose=True, extra=[a for a in sys.argv[1:] if a != "--torch"])
+    if "--torch" in sys.argv[1:]:
+        build_torch_module(verbose=True)

@@ -1,0 +1,195 @@
+// gd2_torch.cpp -- LibTorch autograd Function over the libgd2 C ABI: an
+// SSP-RK2 rollout of the Euler HLLC-MUSCL scheme with a prescribed dt
+// schedule, differentiable with respect to the initial conserved state.
+//
+// Forward: nsteps SSP-RK2 steps (gd2_stage x 2 per step), keeping a copy of
+// the state every `every` steps. Backward: per segment, from the last one
+// down, recompute the segment's states from its checkpoint (the step's
+// input U^n and its stage-1 state U^(1)), then run the step adjoint
+//     lam1 = lam/2 + (dt/2) J_L(U^(1))^T lam
+//     lam  = lam/2 + lam1 + dt J_L(U^n)^T lam1
+// backwards through the segment (two gd2_stage_vjp launches per step).
+// The dt schedule is data, not a function of the state: derivatives are
+// those of the map with the schedule held fixed.
+//
+// No reference counterpart (the reference has no adjoint).
+#include <torch/extension.h>
+
+#include <string>
+#include <vector>
+
+#include "gd2.h"
+
+namespace {
+
+using torch::Tensor;
+using torch::autograd::AutogradContext;
+using torch::autograd::variable_list;
+
+struct Setup {
+    gd2_grid g;
+    gd2_scheme sc;
+};
+
+Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc, int64_t order) {
+    TORCH_CHECK(geom.size() == 9, "geom = (nx, ny, dx, dy, x0, y0, gamma, krit, cfl)");
+    TORCH_CHECK(bc.size() == 4, "bc = (west, east, south, north)");
+    Setup s{};
+    s.g.nx = static_cast<int>(geom[0]);
+    s.g.ny = static_cast<int>(geom[1]);
+    s.g.dx = geom[2];
+    s.g.dy = geom[3];
+    s.g.x0 = geom[4];
+    s.g.y0 = geom[5];
+    s.g.gamma = geom[6];
+    s.g.krit = geom[7];
+    for (int i = 0; i < 4; ++i) s.g.bc[i] = static_cast<int>(bc[i]);
+    s.sc.physics = GD2_EULER;
+    s.sc.flux = GD2_FLUX_HLLC;
+    s.sc.order = static_cast<int>(order);
+    s.sc.integrator = GD2_SSPRK2;
+    s.sc.cfl = geom[8];
+    return s;
+}
+
+void check(int rc, const char* what) {
+    TORCH_CHECK(rc == 0, what, " returned ", rc, " (hip error ", gd2_last_hip_error(), ")");
+}
+
+void check_state(const Setup& s, const Tensor& U, const char* name) {
+    TORCH_CHECK(U.is_cuda(), name, " must be a device tensor");
+    TORCH_CHECK(U.scalar_type() == torch::kFloat64, name, " must be float64");
+    TORCH_CHECK(U.is_contiguous(), name, " must be contiguous");
+    TORCH_CHECK(U.dim() == 3 && U.size(0) == 4 && U.size(1) == s.g.ny && U.size(2) == s.g.nx, name,
+                " must have shape (4, ny, nx)");
+}
+
+double* dp(const Tensor& t) { return t.data_ptr<double>(); }
+
+void stage1(const Setup& s, const Tensor& U, Tensor& U1, double dt, void* st) {
+    check(gd2_stage(&s.g, &s.sc, dp(U), nullptr, dp(U1), 0.0, 1.0, dt, nullptr, nullptr, st), "gd2_stage");
+}
+
+void stage2(const Setup& s, const Tensor& U1, const Tensor& Un, Tensor& Uout, double dt, void* st) {
+    check(gd2_stage(&s.g, &s.sc, dp(U1), dp(Un), dp(Uout), 0.5, 0.5, dt, nullptr, nullptr, st), "gd2_stage");
+}
+
+// One step adjoint in place: lam <- (d U^{n+1} / d U^n)^T lam.
+void step_vjp(const Setup& s, const Tensor& Un, const Tensor& U1, Tensor& lam, Tensor& lam1, Tensor& ws,
+              double dt, void* st) {
+    check(gd2_stage_vjp(&s.g, &s.sc, dp(U1), dp(lam), dp(lam), 0.5, nullptr, 0.0, 0.5 * dt, dp(lam1),
+                        ws.data_ptr(), st),
+          "gd2_stage_vjp");
+    check(gd2_stage_vjp(&s.g, &s.sc, dp(Un), dp(lam1), dp(lam), 0.5, dp(lam1), 1.0, dt, dp(lam), ws.data_ptr(),
+                        st),
+          "gd2_stage_vjp");
+}
+
+std::vector<double> schedule(const Tensor& dts) {
+    Tensor d = dts.detach().to(torch::kCPU, torch::kFloat64).contiguous();
+    TORCH_CHECK(d.dim() == 1, "dts must be a 1-D tensor of step sizes");
+    const double* p = d.data_ptr<double>();
+    return std::vector<double>(p, p + d.numel());
+}
+
+// Forward rollout; returns the final state and fills the checkpoints.
+Tensor run_forward(const Setup& s, const Tensor& U0, const std::vector<double>& dt, int64_t every, void* st,
+                   std::vector<Tensor>* ckpt) {
+    Tensor U = U0.detach().clone();
+    Tensor S = torch::empty_like(U);
+    const int64_t n = static_cast<int64_t>(dt.size());
+    for (int64_t k = 0; k < n; ++k) {
+        if (ckpt && k % every == 0) ckpt->push_back(U.clone());
+        stage1(s, U, S, dt[k], st);
+        stage2(s, S, U, U, dt[k], st);
+    }
+    return U;
+}
+
+// Reverse sweep from the checkpoints: returns (d U_T / d U_0)^T lamT.
+Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::vector<double>& dt, int64_t every,
+                    const Tensor& lamT, void* st) {
+    const int64_t n = static_cast<int64_t>(dt.size());
+    Tensor lam = lamT.to(torch::kFloat64).contiguous().clone();
+    Tensor lam1 = torch::empty_like(lam);
+    const long long wsb = gd2_vjp_workspace_bytes(&s.g);
+    TORCH_CHECK(wsb > 0, "gd2_vjp_workspace_bytes failed");
+    Tensor ws = torch::empty({(wsb + 7) / 8}, lam.options());
+    std::vector<Tensor> Un, U1;
+    for (int64_t seg = static_cast<int64_t>(ckpt.size()) - 1; seg >= 0; --seg) {
+        const int64_t k0 = seg * every, k1 = std::min(n, k0 + every);
+        const int64_t m = k1 - k0;
+        while (static_cast<int64_t>(Un.size()) < m) {
+            Un.push_back(torch::empty_like(lam));
+            U1.push_back(torch::empty_like(lam));
+        }
+        Un[0].copy_(ckpt[seg]);
+        for (int64_t i = 0; i < m; ++i) {
+            stage1(s, Un[i], U1[i], dt[k0 + i], st);
+            if (i + 1 < m) stage2(s, U1[i], Un[i], Un[i + 1], dt[k0 + i], st);
+        }
+        for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, Un[i], U1[i], lam, lam1, ws, dt[k0 + i], st);
+    }
+    return lam;
+}
+
+struct Rollout : public torch::autograd::Function<Rollout> {
+    static Tensor forward(AutogradContext* ctx, Tensor U0, Tensor dts, std::vector<double> geom,
+                          std::vector<int64_t> bc, int64_t order, int64_t every, int64_t stream) {
+        const Setup s = make_setup(geom, bc, order);
+        check_state(s, U0, "U0");
+        TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
+        const std::vector<double> dt = schedule(dts);
+        std::vector<Tensor> ckpt;
+        Tensor UT = run_forward(s, U0, dt, every, reinterpret_cast<void*>(stream), &ckpt);
+        ctx->saved_data["ckpt"] = ckpt;
+        ctx->saved_data["dt"] = dt;
+        ctx->saved_data["geom"] = geom;
+        ctx->saved_data["bc"] = bc;
+        ctx->saved_data["order"] = order;
+        ctx->saved_data["every"] = every;
+        ctx->saved_data["stream"] = stream;
+        return UT;
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list grads) {
+        const Setup s = make_setup(ctx->saved_data["geom"].toDoubleVector(), ctx->saved_data["bc"].toIntVector(),
+                                   ctx->saved_data["order"].toInt());
+        const std::vector<Tensor> ckpt = ctx->saved_data["ckpt"].toTensorVector();
+        const std::vector<double> dt = ctx->saved_data["dt"].toDoubleVector();
+        const int64_t every = ctx->saved_data["every"].toInt();
+        void* st = reinterpret_cast<void*>(ctx->saved_data["stream"].toInt());
+        Tensor g = grads[0];
+        if (!g.defined()) return {Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+        Tensor lam0 = run_backward(s, ckpt, dt, every, g, st);
+        return {lam0, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    }
+};
+
+Tensor rollout(Tensor U0, Tensor dts, std::vector<double> geom, std::vector<int64_t> bc, int64_t order,
+               int64_t every, int64_t stream) {
+    return Rollout::apply(U0, dts, geom, bc, order, every, stream);
+}
+
+// The same two sweeps without the autograd graph: returns (U_T, lam_0).
+std::vector<Tensor> rollout_vjp(Tensor U0, Tensor dts, Tensor lamT, std::vector<double> geom,
+                                std::vector<int64_t> bc, int64_t order, int64_t every, int64_t stream) {
+    const Setup s = make_setup(geom, bc, order);
+    check_state(s, U0, "U0");
+    check_state(s, lamT, "lamT");
+    TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
+    const std::vector<double> dt = schedule(dts);
+    std::vector<Tensor> ckpt;
+    void* st = reinterpret_cast<void*>(stream);
+    Tensor UT = run_forward(s, U0, dt, every, st, &ckpt);
+    Tensor lam0 = run_backward(s, ckpt, dt, every, lamT, st);
+    return {UT, lam0};
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    m.doc() = "Differentiable SSP-RK2 rollouts of the Euler HLLC-MUSCL scheme on libgd2";
+    m.def("rollout", &rollout, "U_T = rollout(U0, dts, geom, bc, order, every, stream) (autograd-aware)");
+    m.def("rollout_vjp", &rollout_vjp, "(U_T, lam_0) = rollout_vjp(U0, dts, lamT, geom, bc, order, every, stream)");
+}

@@ -113,6 +113,20 @@ def smooth_perturbation(g, seed, amp, nmodes=4):
     return amp * d / nmodes
 
 
+def smooth_perturbed_euler(g, U, seed, amp):
+    """Euler state U with seeded smooth perturbations of relative amplitude amp
+    on rho and p and absolute amplitude amp on u and v."""
+    gm1 = g.gamma - 1.0
+    rho = U[0]
+    u, v = U[1] / rho, U[2] / rho
+    p = gm1 * (U[3] - 0.5 * rho * (u * u + v * v))
+    rho = rho * (1.0 + smooth_perturbation(g, seed + 1, amp))
+    p = p * (1.0 + smooth_perturbation(g, seed + 2, amp))
+    u = u + smooth_perturbation(g, seed + 3, amp)
+    v = v + smooth_perturbation(g, seed + 4, amp)
+    return np.ascontiguousarray(euler_from_prim(g.gamma, rho, p, u, v))
+
+
 def orszag_tang(g, seed=0, amp=1e-3):
     """Orszag-Tang vortex on [0,1]^2 (periodic), gamma from g:
     rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x),
