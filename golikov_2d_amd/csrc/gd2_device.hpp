@@ -106,6 +106,59 @@ __device__ __forceinline__ double fast_sqrt(double x) {
     return x == 0.0 ? 0.0 : s;
 }
 
+// sqrt(a / b) for a, b > 0 as a * rsq(a b): one reciprocal square root,
+// refined by two Goldschmidt steps (h -> 1/(2 sqrt(x))), instead of a
+// reciprocal and a square root. Returns 0 when a b == 0.
+__device__ __forceinline__ double sqrt_ratio(double a, double b) {
+    const double x = a * b;
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    h = fma(h, r, h);
+    return x == 0.0 ? 0.0 : (a + a) * h;
+}
+
+// A wave-uniform pointer pinned to scalar registers. Row base pointers pass
+// through this so that loop strength reduction cannot turn them into per-lane
+// 64-bit induction variables: loads then take the scalar-base + 32-bit
+// vector-offset form.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+
+// Raw buffer resource over a wave-uniform base pointer (scalar registers):
+// accesses then take a 32-bit per-lane byte offset, with no 64-bit address
+// arithmetic per lane; offsets at or beyond 2^31 - 1 read 0 / drop the store.
+constexpr int kBufFlags = 0x00020000;  // raw dword access on CDNA3/CDNA4
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, kBufFlags);
+}
+
+__device__ __forceinline__ double ld_buf(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, unsigned byte_off, double v) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, byte_off, 0, 0);
+}
+
+// Value held by lane + 1 (lane 63 receives 0): DPP wave_shl:1 on both halves.
+__device__ __forceinline__ double from_next_lane(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Primitive Euler state.
 struct PrimE {
     double r, u, v, p;
@@ -135,8 +188,8 @@ struct FluxE {
 __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, double pL,
                                             double rR, double unR, double utR, double pR,
                                             double gamma, double inv_g1, double& smax) {
-    const double cL = fast_sqrt(gamma * pL * fast_rcp(rL));
-    const double cR = fast_sqrt(gamma * pR * fast_rcp(rR));
+    const double cL = sqrt_ratio(gamma * pL, rL);
+    const double cR = sqrt_ratio(gamma * pR, rR);
     const double SL = fmin(unL - cL, unR - cR);
     const double SR = fmax(unL + cL, unR + cR);
     smax = fmax(fabs(SL), fabs(SR));

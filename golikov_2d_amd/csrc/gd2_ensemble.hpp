@@ -29,7 +29,7 @@ struct EnsTile {
 };
 
 template <int ORDER, int FLUX, int STAGE>
-__global__ __launch_bounds__(64) void ens_stage_kernel(const EnsProblem* __restrict__ probs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 4 : 1))) void ens_stage_kernel(const EnsProblem* __restrict__ probs,
                                                        const EnsTile* __restrict__ tiles, int ntiles, int slot) {
     __shared__ double ring[3][4][kRowBuf];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {

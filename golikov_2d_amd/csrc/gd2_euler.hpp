@@ -61,27 +61,59 @@ struct Slope {
 template <int LAYOUT>
 struct RowVal {
     PrimE q;
-    double U[4];  // conserved (kSoA) or unused
+};
+
+// Raw values of one cell as stored: conserved (kSoA) or (rho, p, vx, vy) (kAoS).
+struct Raw {
+    double a[4];
+};
+
+// Buffer resources of one logical row (after the boundary map): one per
+// conserved plane (kSoA), or s and u twice (kAoS, second double at +8 bytes).
+struct RowPtr {
+    __amdgpu_buffer_rsrc_t f[4];
+    unsigned add[4];
 };
 
 template <int LAYOUT>
-__device__ __forceinline__ RowVal<LAYOUT> load_cell(const EulerArgs& A, long long k, bool fx, bool fy) {
+__device__ __forceinline__ RowPtr row_ptr(const EulerArgs& A, int jj) {
+    RowPtr r;
+    if constexpr (LAYOUT == kSoA) {
+        const double* b = A.Uin + (long long)jj * A.nx;
+        r.f[0] = buf_rsrc(uniform_ptr(b));
+        r.f[1] = buf_rsrc(uniform_ptr(b + A.plane));
+        r.f[2] = buf_rsrc(uniform_ptr(b + 2 * A.plane));
+        r.f[3] = buf_rsrc(uniform_ptr(b + 3 * A.plane));
+        r.add[0] = r.add[1] = r.add[2] = r.add[3] = 0u;
+    } else {
+        r.f[0] = r.f[1] = buf_rsrc(uniform_ptr(A.s + (long long)jj * A.nx));
+        r.f[2] = r.f[3] = buf_rsrc(uniform_ptr(A.u + (long long)jj * A.nx));
+        r.add[0] = r.add[2] = 0u;
+        r.add[1] = r.add[3] = 8u;
+    }
+    return r;
+}
+
+// off: byte offset of the cell within the row (8 * column for kSoA, 16 * column for kAoS)
+__device__ __forceinline__ Raw load_raw(const RowPtr& r, unsigned off) {
+    Raw v;
+    v.a[0] = ld_buf(r.f[0], off + r.add[0]);
+    v.a[1] = ld_buf(r.f[1], off + r.add[1]);
+    v.a[2] = ld_buf(r.f[2], off + r.add[2]);
+    v.a[3] = ld_buf(r.f[3], off + r.add[3]);
+    return v;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ RowVal<LAYOUT> finish_cell(const EulerArgs& A, const Raw& w, bool fx, bool fy) {
     RowVal<LAYOUT> v;
     if constexpr (LAYOUT == kSoA) {
-        const long long P = A.plane;
-        v.U[0] = A.Uin[k];
-        v.U[1] = A.Uin[P + k];
-        v.U[2] = A.Uin[2 * P + k];
-        v.U[3] = A.Uin[3 * P + k];
-        v.q = cons_to_prim_e(v.U[0], v.U[1], v.U[2], v.U[3], A.g1);
+        v.q = cons_to_prim_e(w.a[0], w.a[1], w.a[2], w.a[3], A.g1);
     } else {
-        const double2 sv = A.s[k];
-        const double2 uv = A.u[k];
-        v.q.r = sv.x;
-        v.q.p = sv.y;
-        v.q.u = uv.x;
-        v.q.v = uv.y;
-        v.U[0] = v.U[1] = v.U[2] = v.U[3] = 0.0;
+        v.q.r = w.a[0];
+        v.q.p = w.a[1];
+        v.q.u = w.a[2];
+        v.q.v = w.a[3];
     }
     if (fx) v.q.u = -v.q.u;
     if (fy) v.q.v = -v.q.v;
@@ -170,6 +202,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
     const bool has_halo = lane < 3;
     const bool owns = (lane < kTileW) && (c < A.nx);
+    constexpr unsigned kCellBytes = LAYOUT == kSoA ? 8u : 16u;
+    const unsigned oc = (unsigned)ci * kCellBytes;   // own cell within a row
+    const unsigned oh = (unsigned)hci * kCellBytes;  // halo cell within a row
+    const unsigned oo = (unsigned)c * 8u;            // output cell within a row (kSoA)
 
     const double dt = A.dt_dev ? *A.dt_dev : A.dt;
     const double dx = A.dx, dy = A.dy;
@@ -178,16 +214,26 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     // Ker_Dekard / Kernel_TVD_old (kernel.cu:1798-1801, 3549-3552)
     const double krx = A.krit * dy, kry = A.krit * dx;
 
-    // load one logical row jl into registers (own column) and the LDS ring
-    auto load_row = [&](int jl, int slot, bool to_lds) -> RowVal<LAYOUT> {
+    // logical row jl: raw loads of the own cell and (lanes 0-2) the halo cell
+    struct Pending {
+        Raw own, halo;
         bool fy;
-        const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
-        const long long rowk = (long long)jj * A.nx;
-        RowVal<LAYOUT> v = load_cell<LAYOUT>(A, rowk + ci, fx, fy);
+    };
+    auto issue_row = [&](int jl) -> Pending {
+        Pending pr;
+        const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], pr.fy);
+        const RowPtr rp = row_ptr<LAYOUT>(A, jj);
+        pr.own = load_raw(rp, oc);
+        if (has_halo) pr.halo = load_raw(rp, oh);
+        return pr;
+    };
+    // convert a pending row; to_lds: also publish it (and the halo) in ring slot
+    auto finish_row = [&](const Pending& pr, int slot, bool to_lds) -> RowVal<LAYOUT> {
+        RowVal<LAYOUT> v = finish_cell<LAYOUT>(A, pr.own, fx, pr.fy);
         if (to_lds) {
             lds_put(ring, slot, lane + 2, v.q);
             if (has_halo) {
-                RowVal<LAYOUT> h = load_cell<LAYOUT>(A, rowk + hci, hfx, fy);
+                RowVal<LAYOUT> h = finish_cell<LAYOUT>(A, pr.halo, hfx, pr.fy);
                 lds_put(ring, slot, hslot, h.q);
             }
         }
@@ -198,9 +244,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     // first y-face G(j0-1/2) is produced by the loop's own pass j = j0-1, so
     // every face goes through one instruction sequence and results do not
     // depend on where the strips start.
-    RowVal<LAYOUT> vm2 = load_row(j0 - 2, 0, false);
-    RowVal<LAYOUT> v0 = load_row(j0 - 1, 0, false);
-    RowVal<LAYOUT> v1 = load_row(j0, j0 % 3, true);
+    const Pending pm2 = issue_row(j0 - 2), pm1 = issue_row(j0 - 1), pj0 = issue_row(j0);
+    RowVal<LAYOUT> vm2 = finish_row(pm2, 0, false);
+    RowVal<LAYOUT> v0 = finish_row(pm1, 0, false);
+    RowVal<LAYOUT> v1 = finish_row(pj0, j0 % 3, true);
     Slope s0 = slope3<ORDER>(vm2.q, v0.q, v1.q);
     double smax;
     FluxE Gm;  // y-face flux G(j-1/2), frame (n = +y, t = +x)
@@ -212,7 +259,47 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
 
     for (int j = j0 - 1; j < j1; ++j) {
         __syncthreads();  // slot (j+2)%3 = slot of row j-1, whose x-faces are done
-        RowVal<LAYOUT> v2 = load_row(j + 2, (j + 2) % 3, true);
+        // loads first: row j+2, row j and the RK base of row j arrive while
+        // the x-face of row j is computed
+        const Pending pn = issue_row(j + 2);
+        double ub[4] = {0.0, 0.0, 0.0, 0.0};
+        Raw uc;  // conserved state of row j (re-read: cheaper than carrying it in registers)
+        if constexpr (LAYOUT == kSoA) {
+            if (j >= j0) uc = load_raw(row_ptr<LAYOUT>(A, j), oc);
+            if (!A.plain && j >= j0 && owns) {
+                const double* b = A.Ubase + (long long)j * A.nx;
+                ub[0] = ld_buf(buf_rsrc(uniform_ptr(b)), oo);
+                ub[1] = ld_buf(buf_rsrc(uniform_ptr(b + A.plane)), oo);
+                ub[2] = ld_buf(buf_rsrc(uniform_ptr(b + 2 * A.plane)), oo);
+                ub[3] = ld_buf(buf_rsrc(uniform_ptr(b + 3 * A.plane)), oo);
+            }
+        }
+
+        // x-face F(c-1/2) of row j from cells c-2, c-1 (LDS), c (own), c+1
+        // (LDS); row j reached LDS two iterations ago
+        FluxE Fl, Fr;
+        double tFl = 0.0, tFr = 0.0;
+        if (j >= j0) {
+            const int sl = j % 3;
+            PrimE qa, qb, qd;
+            qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
+            qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
+            qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
+            const PrimE& qc = v0.q;
+            Slope sb = slope3<ORDER>(qa, qb, qc);
+            Slope sc = slope3<ORDER>(qb, qc, qd);
+            PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
+            Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+            // F(c+1/2) from the next lane
+            Fr.fr = from_next_lane(Fl.fr);
+            Fr.fn = from_next_lane(Fl.fn);
+            Fr.ft = from_next_lane(Fl.ft);
+            Fr.fe = from_next_lane(Fl.fe);
+            if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
+        }
+
+        RowVal<LAYOUT> v2 = finish_row(pn, (j + 2) % 3, true);
         // y-face G(j+1/2)
         Slope s1 = slope3<ORDER>(v0.q, v1.q, v2.q);
         FluxE Gp;
@@ -230,30 +317,6 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             tGm = tGp;
             continue;
         }
-        // row j reached LDS two iterations ago (before two barriers)
-        // x-face F(c-1/2) of row j from cells c-2, c-1, c, c+1
-        FluxE Fl;
-        double tFl;
-        {
-            const int sl = j % 3;
-            PrimE qa, qb, qd;
-            qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
-            qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
-            qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
-            const PrimE& qc = v0.q;
-            Slope sb = slope3<ORDER>(qa, qb, qc);
-            Slope sc = slope3<ORDER>(qb, qc, qd);
-            PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
-            Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
-            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
-        }
-        // F(c+1/2) from the next lane
-        FluxE Fr;
-        Fr.fr = __shfl_down(Fl.fr, 1, kWave);
-        Fr.fn = __shfl_down(Fl.fn, 1, kWave);
-        Fr.ft = __shfl_down(Fl.ft, 1, kWave);
-        Fr.fe = __shfl_down(Fl.fe, 1, kWave);
-        const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
 
         if (owns && !frozen_cell<LAYOUT>(A, c, j)) {
             // sum of outward fluxes times face lengths (kernel.cu:1854-1857),
@@ -263,45 +326,52 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             const double PSe = (Fr.fe - Fl.fe) * dy + (Gp.fe - Gm.fe) * dx;
             const double PUx = (Fr.fn - Fl.fn) * dy + (Gp.ft - Gm.ft) * dx;
             const double PUy = (Fr.ft - Fl.ft) * dy + (Gp.fn - Gm.fn) * dx;
-            const long long k = (long long)j * A.nx + c;
             if constexpr (LAYOUT == kSoA) {
-                double o0 = v0.U[0] - dtdV * PSr;
-                double o1 = v0.U[1] - dtdV * PUx;
-                double o2 = v0.U[2] - dtdV * PUy;
-                double o3 = v0.U[3] - dtdV * PSe;
+                double o0 = uc.a[0] - dtdV * PSr;
+                double o1 = uc.a[1] - dtdV * PUx;
+                double o2 = uc.a[2] - dtdV * PUy;
+                double o3 = uc.a[3] - dtdV * PSe;
                 if (!A.plain) {
-                    const long long P = A.plane;
-                    o0 = A.a * A.Ubase[k] + A.b * o0;
-                    o1 = A.a * A.Ubase[P + k] + A.b * o1;
-                    o2 = A.a * A.Ubase[2 * P + k] + A.b * o2;
-                    o3 = A.a * A.Ubase[3 * P + k] + A.b * o3;
+                    o0 = A.a * ub[0] + A.b * o0;
+                    o1 = A.a * ub[1] + A.b * o1;
+                    o2 = A.a * ub[2] + A.b * o2;
+                    o3 = A.a * ub[3] + A.b * o3;
                 }
                 // clamps (kernel.cu:1862-1866, 1871-1874) in conserved form
                 if (o0 <= 0.0) {
                     atomicAdd(&A.diag->clamp_rho, 1ull);
                     o0 = 0.0001;
                 }
-                const double inv = fast_rcp(o0);
-                const double ux = o1 * inv, uy = o2 * inv;
-                const double ke = 0.5 * (o1 * ux + o2 * uy);
-                double p = A.g1 * (o3 - ke);
-                if (p <= 0.0) {
-                    atomicAdd(&A.diag->clamp_p, 1ull);
-                    o3 = 0.000001 / A.g1 + ke;
-                    p = A.g1 * (o3 - ke);
-                }
-                const long long P = A.plane;
-                A.Uout[k] = o0;
-                A.Uout[P + k] = o1;
-                A.Uout[2 * P + k] = o2;
-                A.Uout[3 * P + k] = o3;
+                double* ob = A.Uout + (long long)j * A.nx;
                 if constexpr (WANT_W) {
-                    const double cs = fast_sqrt(A.gamma * p * inv);
+                    const double inv = fast_rcp(o0);
+                    const double ux = o1 * inv, uy = o2 * inv;
+                    const double ke = 0.5 * (o1 * ux + o2 * uy);
+                    double p = A.g1 * (o3 - ke);
+                    if (p <= 0.0) {
+                        atomicAdd(&A.diag->clamp_p, 1ull);
+                        o3 = 0.000001 / A.g1 + ke;
+                        p = A.g1 * (o3 - ke);
+                    }
+                    const double cs = sqrt_ratio(A.gamma * p, o0);
                     const double w = (fabs(ux) + cs) * A.idx + (fabs(uy) + cs) * A.idy;
                     wmax = fmax(wmax, w);
+                } else {
+                    // p <= 0 exactly when rho E - |m|^2 / 2 <= 0 (rho > 0): no division
+                    if (o3 * o0 - 0.5 * (o1 * o1 + o2 * o2) <= 0.0) {
+                        atomicAdd(&A.diag->clamp_p, 1ull);
+                        const double inv = fast_rcp(o0);
+                        const double ke = 0.5 * (o1 * (o1 * inv) + o2 * (o2 * inv));
+                        o3 = 0.000001 / A.g1 + ke;
+                    }
                 }
+                st_buf(buf_rsrc(uniform_ptr(ob)), oo, o0);
+                st_buf(buf_rsrc(uniform_ptr(ob + A.plane)), oo, o1);
+                st_buf(buf_rsrc(uniform_ptr(ob + 2 * A.plane)), oo, o2);
+                st_buf(buf_rsrc(uniform_ptr(ob + 3 * A.plane)), oo, o3);
             } else {
                 // primitive-form update of Ker_Dekard (kernel.cu:1859-1874)
+                const long long k = (long long)j * A.nx + c;
                 const PrimE& q = v0.q;
                 const double gm1 = A.gamma - 1.0;
                 const double Tdv = dt / (dx * dy);
@@ -350,7 +420,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
 }
 
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
-__global__ __launch_bounds__(64) void euler_tile_kernel(const EulerArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 4 : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ double ring[3][4][kRowBuf];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     euler_tile<ORDER, FLUX, LAYOUT, WANT_W>(A, tile % A.tiles_x, tile / A.tiles_x, ring);
