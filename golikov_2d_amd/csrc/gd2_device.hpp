@@ -109,6 +109,8 @@ __device__ __forceinline__ double fast_sqrt(double x) {
 // sqrt(a / b) for a, b > 0 as a * rsq(a b): one reciprocal square root,
 // refined by two Goldschmidt steps (h -> 1/(2 sqrt(x))), instead of a
 // reciprocal and a square root. Returns 0 when a b == 0.
+// GUARD = false: the caller guarantees a b > 0 (normal).
+template <bool GUARD = true>
 __device__ __forceinline__ double sqrt_ratio(double a, double b) {
     const double x = a * b;
     const double y = __builtin_amdgcn_rsq(x);
@@ -118,7 +120,10 @@ __device__ __forceinline__ double sqrt_ratio(double a, double b) {
     h = fma(h, r, h);
     r = fma(-g, h, 0.5);
     h = fma(h, r, h);
-    return x == 0.0 ? 0.0 : (a + a) * h;
+    if constexpr (GUARD)
+        return x == 0.0 ? 0.0 : (a + a) * h;
+    else
+        return (a + a) * h;
 }
 
 // A wave-uniform pointer pinned to scalar registers. Row base pointers pass
@@ -185,11 +190,12 @@ struct FluxE {
 // side chosen by sign(SM) is formed (the reference forms both); the branch
 // choice SL > 0 / SL <= 0 <= SM / SM <= 0 <= SR / SR < 0 (kernel.cu:1119-1158)
 // becomes a select. smax returns max(|SL|, |SR|).
+template <bool GUARD = true>
 __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, double pL,
                                             double rR, double unR, double utR, double pR,
                                             double gamma, double inv_g1, double& smax) {
-    const double cL = sqrt_ratio(gamma * pL, rL);
-    const double cR = sqrt_ratio(gamma * pR, rR);
+    const double cL = sqrt_ratio<GUARD>(gamma * pL, rL);
+    const double cR = sqrt_ratio<GUARD>(gamma * pR, rR);
     const double SL = fmin(unL - cL, unR - cR);
     const double SR = fmax(unL + cL, unR + cR);
     smax = fmax(fabs(SL), fabs(SR));
@@ -206,7 +212,7 @@ __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, d
     const double un = left ? unL : unR;
     const double ut = left ? utL : utR;
     const double p = left ? pL : pR;
-    const double su = left ? suL : suR;
+    const double su = S - un;  // = suL or suR, the same operation on the same operands
     const bool super = (SL > 0.0) || (SR < 0.0);
     const double e = p * inv_g1 + r * (0.5 * (un * un + ut * ut));
     const double den = super ? 1.0 : (S - SM);

@@ -127,14 +127,22 @@ __device__ __forceinline__ void lds_put(double (*ring)[4][kRowBuf], int slot, in
     ring[slot][3][e] = q.p;
 }
 
-template <int FLUX>
+// GUARD = false: every cell of the face's stencil has rho, p > kPosMin, so the
+// face states are positive and the zero guards can be skipped.
+template <int FLUX, bool GUARD = true>
 __device__ __forceinline__ FluxE face_flux(const EulerArgs& A, double rL, double unL, double utL, double pL,
                                            double rR, double unR, double utR, double pR, double& smax) {
     if constexpr (FLUX == GD2_FLUX_EXACT)
         return exact_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, smax, A.diag);
     else
-        return hllc_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
+        return hllc_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
 }
+
+// Cells with rho and p above this bound make every minmod face value
+// positive: for q, q_- , q_+ > 0 the half slope is at most q/2 towards zero,
+// so q +- s/2 >= q/2 > 0 in exact and in rounded arithmetic (rounding is
+// monotone), and the products in the sound speed stay normal.
+constexpr double kPosMin = 1e-150;
 
 // minmod slope of the middle value; half-slope face values with the
 // reference's rho/p <= 0 fall-back to the cell value (grad_p, kernel.cu:28,
@@ -153,25 +161,32 @@ __device__ __forceinline__ Slope slope3(const PrimE& a, const PrimE& b, const Pr
     return s;
 }
 
+// GUARD = false: the stencil is positive (kPosMin), the fall-back cannot fire.
+template <bool GUARD = true>
 __device__ __forceinline__ PrimE face_plus(const PrimE& q, const Slope& s) {
     PrimE f;
     f.r = q.r + 0.5 * s.r;
     f.u = q.u + 0.5 * s.u;
     f.v = q.v + 0.5 * s.v;
     f.p = q.p + 0.5 * s.p;
-    if (f.r <= 0.0) f.r = q.r;
-    if (f.p <= 0.0) f.p = q.p;
+    if constexpr (GUARD) {
+        if (f.r <= 0.0) f.r = q.r;
+        if (f.p <= 0.0) f.p = q.p;
+    }
     return f;
 }
 
+template <bool GUARD = true>
 __device__ __forceinline__ PrimE face_minus(const PrimE& q, const Slope& s) {
     PrimE f;
     f.r = q.r - 0.5 * s.r;
     f.u = q.u - 0.5 * s.u;
     f.v = q.v - 0.5 * s.v;
     f.p = q.p - 0.5 * s.p;
-    if (f.r <= 0.0) f.r = q.r;
-    if (f.p <= 0.0) f.p = q.p;
+    if constexpr (GUARD) {
+        if (f.r <= 0.0) f.r = q.r;
+        if (f.p <= 0.0) f.p = q.p;
+    }
     return f;
 }
 
@@ -227,16 +242,21 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
         if (has_halo) pr.halo = load_raw(rp, oh);
         return pr;
     };
-    // convert a pending row; to_lds: also publish it (and the halo) in ring slot
-    auto finish_row = [&](const Pending& pr, int slot, bool to_lds) -> RowVal<LAYOUT> {
+    // convert a pending row; to_lds: also publish it (and the halo) in ring
+    // slot. pos: whether every cell of the row (and halo) has rho, p > kPosMin
+    // (wave-uniform).
+    auto finish_row = [&](const Pending& pr, int slot, bool to_lds, bool& pos) -> RowVal<LAYOUT> {
         RowVal<LAYOUT> v = finish_cell<LAYOUT>(A, pr.own, fx, pr.fy);
+        bool ok = v.q.r > kPosMin && v.q.p > kPosMin;
         if (to_lds) {
             lds_put(ring, slot, lane + 2, v.q);
             if (has_halo) {
                 RowVal<LAYOUT> h = finish_cell<LAYOUT>(A, pr.halo, hfx, pr.fy);
                 lds_put(ring, slot, hslot, h.q);
+                ok = ok && h.q.r > kPosMin && h.q.p > kPosMin;
             }
         }
+        pos = __all(ok) != 0;
         return v;
     };
 
@@ -245,9 +265,13 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     // every face goes through one instruction sequence and results do not
     // depend on where the strips start.
     const Pending pm2 = issue_row(j0 - 2), pm1 = issue_row(j0 - 1), pj0 = issue_row(j0);
-    RowVal<LAYOUT> vm2 = finish_row(pm2, 0, false);
-    RowVal<LAYOUT> v0 = finish_row(pm1, 0, false);
-    RowVal<LAYOUT> v1 = finish_row(pj0, j0 % 3, true);
+    bool pa, pb, pc;
+    RowVal<LAYOUT> vm2 = finish_row(pm2, 0, false, pa);
+    RowVal<LAYOUT> v0 = finish_row(pm1, 0, false, pb);
+    RowVal<LAYOUT> v1 = finish_row(pj0, j0 % 3, true, pc);
+    // positivity of the last rows seen, newest in bit 0: rows j-1 .. j+2
+    // (the whole stencil of the faces of pass j) are in bits 3 .. 0
+    unsigned pos_hist = (pa ? 4u : 0u) | (pb ? 2u : 0u) | (pc ? 1u : 0u);
     Slope s0 = slope3<ORDER>(vm2.q, v0.q, v1.q);
     double smax;
     FluxE Gm;  // y-face flux G(j-1/2), frame (n = +y, t = +x)
@@ -259,9 +283,40 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
 
     for (int j = j0 - 1; j < j1; ++j) {
         __syncthreads();  // slot (j+2)%3 = slot of row j-1, whose x-faces are done
-        // loads first: row j+2, row j and the RK base of row j arrive while
-        // the x-face of row j is computed
+        // loads first: row j+2 arrives while the x-face of row j is computed
         const Pending pn = issue_row(j + 2);
+
+        // x-face F(c-1/2) of row j from cells c-2, c-1 (LDS), c (own), c+1
+        // (LDS); row j reached LDS two iterations ago
+        FluxE Fl, Fr;
+        double tFl = 0.0, tFr = 0.0;
+        // row j (with its halo) was vetted when it was converted: bit 1 now
+        const bool xpos = (pos_hist & 2u) != 0u;
+        if (j >= j0) {
+            const int sl = j % 3;
+            PrimE qa, qb, qd;
+            qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
+            qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
+            qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
+            const PrimE& qc = v0.q;
+            Slope sb = slope3<ORDER>(qa, qb, qc);
+            Slope sc = slope3<ORDER>(qb, qc, qd);
+            if (xpos) {
+                PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
+                Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+            } else {
+                PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
+                Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+            }
+            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+            // F(c+1/2) from the next lane
+            Fr.fr = from_next_lane(Fl.fr);
+            Fr.fn = from_next_lane(Fl.fn);
+            Fr.ft = from_next_lane(Fl.ft);
+            Fr.fe = from_next_lane(Fl.fe);
+            if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
+        }
+        // the update's inputs arrive while the y-face is computed
         double ub[4] = {0.0, 0.0, 0.0, 0.0};
         Raw uc;  // conserved state of row j (re-read: cheaper than carrying it in registers)
         if constexpr (LAYOUT == kSoA) {
@@ -275,40 +330,21 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             }
         }
 
-        // x-face F(c-1/2) of row j from cells c-2, c-1 (LDS), c (own), c+1
-        // (LDS); row j reached LDS two iterations ago
-        FluxE Fl, Fr;
-        double tFl = 0.0, tFr = 0.0;
-        if (j >= j0) {
-            const int sl = j % 3;
-            PrimE qa, qb, qd;
-            qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
-            qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
-            qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
-            const PrimE& qc = v0.q;
-            Slope sb = slope3<ORDER>(qa, qb, qc);
-            Slope sc = slope3<ORDER>(qb, qc, qd);
-            PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
-            Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
-            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
-            // F(c+1/2) from the next lane
-            Fr.fr = from_next_lane(Fl.fr);
-            Fr.fn = from_next_lane(Fl.fn);
-            Fr.ft = from_next_lane(Fl.ft);
-            Fr.fe = from_next_lane(Fl.fe);
-            if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
-        }
-
-        RowVal<LAYOUT> v2 = finish_row(pn, (j + 2) % 3, true);
-        // y-face G(j+1/2)
+        bool pn_pos;
+        RowVal<LAYOUT> v2 = finish_row(pn, (j + 2) % 3, true, pn_pos);
+        pos_hist = ((pos_hist << 1) | (pn_pos ? 1u : 0u)) & 15u;
+        // y-face G(j+1/2) from rows j-1 .. j+2
         Slope s1 = slope3<ORDER>(v0.q, v1.q, v2.q);
         FluxE Gp;
         double tGp;
-        {
+        if (pos_hist == 15u) {
+            PrimE L = face_plus<false>(v0.q, s0), R = face_minus<false>(v1.q, s1);
+            Gp = face_flux<FLUX, false>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
+        } else {
             PrimE L = face_plus(v0.q, s0), R = face_minus(v1.q, s1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
-            tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         }
+        tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             v0 = v1;
             v1 = v2;
