@@ -33,9 +33,8 @@ struct FaceE {
 __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double gamma, double inv_g1, const FluxE& Fb,
                                          FaceE& Lb, FaceE& Rb) {
     // forward
-    const double iL = fast_rcp(L.r), iR = fast_rcp(R.r);
-    const double cL = fast_sqrt(gamma * L.p * iL);
-    const double cR = fast_sqrt(gamma * R.p * iR);
+    const SoundSpeed sL = sound_speed(gamma * L.p, L.r), sR = sound_speed(gamma * R.p, R.r);
+    const double cL = sL.c, cR = sR.c;
     const double aL = L.un - cL, aR = R.un - cR;
     const double bL = L.un + cL, bR = R.un + cR;
     const double SL = fmin(aL, aR), SR = fmax(bL, bR);
@@ -55,7 +54,7 @@ __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double 
     const double un = left ? L.un : R.un;
     const double ut = left ? L.ut : R.ut;
     const double p = left ? L.p : R.p;
-    const double su = left ? suL : suR;
+    const double su = S - un;
     const bool super = (SL > 0.0) || (SR < 0.0);
     const double ke = 0.5 * (un * un + ut * ut);
     const double e = p * inv_g1 + r * ke;
@@ -145,12 +144,11 @@ __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double 
     double b_cL = 0.0, b_cR = 0.0;
     if (slL) { bl.un += b_SL; b_cL -= b_SL; } else { br.un += b_SL; b_cR -= b_SL; }
     if (srL) { bl.un += b_SR; b_cL += b_SR; } else { br.un += b_SR; b_cR += b_SR; }
-    // c = sqrt(gamma p / r): dc/dp = gamma / (2 r c), dc/dr = -c / (2 r)
-    const double icL = fast_rcp(cL), icR = fast_rcp(cR);
-    bl.p += b_cL * 0.5 * gamma * iL * icL;
-    bl.r -= b_cL * 0.5 * cL * iL;
-    br.p += b_cR * 0.5 * gamma * iR * icR;
-    br.r -= b_cR * 0.5 * cR * iR;
+    // c = sqrt(gamma p / r): dc/dp = gamma h, dc/dr = -c^2 h (h = 1 / (2 sqrt(gamma p r)))
+    bl.p += b_cL * (gamma * sL.h);
+    bl.r -= b_cL * (cL * cL * sL.h);
+    br.p += b_cR * (gamma * sR.h);
+    br.r -= b_cR * (cR * cR * sR.h);
     Lb = bl;
     Rb = br;
 }
@@ -216,21 +214,14 @@ __device__ __forceinline__ Q4 q4_add(const Q4& a, const Q4& b) {
     return s;
 }
 
-__device__ __forceinline__ Q4 shfl_down_q(const Q4& x, int d) {
-    Q4 y;
-    y.r = __shfl_down(x.r, d, kWave);
-    y.u = __shfl_down(x.u, d, kWave);
-    y.v = __shfl_down(x.v, d, kWave);
-    y.p = __shfl_down(x.p, d, kWave);
+// Q4 of lane + 1 (lane 63 receives 0) and of lane - 1 (lane 0 receives 0).
+__device__ __forceinline__ Q4 q4_from_next(const Q4& x) {
+    Q4 y = {from_next_lane(x.r), from_next_lane(x.u), from_next_lane(x.v), from_next_lane(x.p)};
     return y;
 }
 
-__device__ __forceinline__ Q4 shfl_up_q(const Q4& x, int d) {
-    Q4 y;
-    y.r = __shfl_up(x.r, d, kWave);
-    y.u = __shfl_up(x.u, d, kWave);
-    y.v = __shfl_up(x.v, d, kWave);
-    y.p = __shfl_up(x.p, d, kWave);
+__device__ __forceinline__ Q4 q4_from_prev(const Q4& x) {
+    Q4 y = {from_prev_lane(x.r), from_prev_lane(x.u), from_prev_lane(x.v), from_prev_lane(x.p)};
     return y;
 }
 
@@ -329,31 +320,39 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     const int ro0 = (j0 == 0) ? -2 : j0;
     const int ro1 = (j1 == ny) ? ny + 2 : j1;
 
+    const unsigned oc = (unsigned)ci * 8u, oh = (unsigned)hci * 8u;
     auto load = [&](int jl, int slot, bool to_lds, double Vv[4]) -> PrimE {
         bool fy;
         const int jj = map_index(jl, ny, A.bc[2], A.bc[3], fy);
         const long long rowk = (long long)jj * nx;
-        const long long k = rowk + ci;
-        PrimE q = cons_to_prim_e(A.U[k], A.U[P + k], A.U[2 * P + k], A.U[3 * P + k], A.g1);
+        __amdgpu_buffer_rsrc_t ru[4], rv[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            ru[f] = buf_rsrc(uniform_ptr(A.U + f * P + rowk));
+            rv[f] = buf_rsrc(uniform_ptr(A.V + f * P + rowk));
+        }
+        PrimE q = cons_to_prim_e(ld_buf(ru[0], oc), ld_buf(ru[1], oc), ld_buf(ru[2], oc), ld_buf(ru[3], oc), A.g1);
         if (fx) q.u = -q.u;
         if (fy) q.v = -q.v;
         // ghost cells have no output in the forward map: their cotangent is 0
         const bool row_real = jl >= 0 && jl < ny;
-        const bool real = row_real && col_real;
-        Vv[0] = real ? A.V[k] : 0.0;
-        Vv[1] = real ? A.V[P + k] : 0.0;
-        Vv[2] = real ? A.V[2 * P + k] : 0.0;
-        Vv[3] = real ? A.V[3 * P + k] : 0.0;
+        Vv[0] = Vv[1] = Vv[2] = Vv[3] = 0.0;
+        if (row_real && col_real) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) Vv[f] = ld_buf(rv[f], oc);
+        }
         if (to_lds) {
             lds_put8(ring, slot, lane + 2, q, Vv);
             if (has_halo) {
-                const long long kh = rowk + hci;
-                PrimE h = cons_to_prim_e(A.U[kh], A.U[P + kh], A.U[2 * P + kh], A.U[3 * P + kh], A.g1);
+                PrimE h = cons_to_prim_e(ld_buf(ru[0], oh), ld_buf(ru[1], oh), ld_buf(ru[2], oh), ld_buf(ru[3], oh),
+                                         A.g1);
                 if (hfx) h.u = -h.u;
                 if (fy) h.v = -h.v;
-                const bool hreal = row_real && hcol >= 0 && hcol < nx;
-                double Vh[4] = {hreal ? A.V[kh] : 0.0, hreal ? A.V[P + kh] : 0.0, hreal ? A.V[2 * P + kh] : 0.0,
-                                hreal ? A.V[3 * P + kh] : 0.0};
+                double Vh[4] = {0.0, 0.0, 0.0, 0.0};
+                if (row_real && hcol >= 0 && hcol < nx) {
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) Vh[f] = ld_buf(rv[f], oh);
+                }
                 lds_put8(ring, slot, hslot, h, Vh);
             }
         }
@@ -370,9 +369,7 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     PrimE qd;
 
     Q4 dqm_j = q4_zero();     // cotangent of row j's minus (south) face value, from G(j-1/2)
-    Q4 own_prev = q4_zero();  // row j-1: own y part
-    Q4 x_prev = q4_zero();    // row j-1: x gather
-    Q4 rsp_prev2 = q4_zero(); // spill of row j-2 to row j-1
+    Q4 acc_prev = q4_zero();  // row j-1: everything but the spill from row j
     Q4 rsp_prev = q4_zero();  // spill of row j-1 to row j
 
     for (int j = ro0 - 2; j <= ro1; ++j) {
@@ -401,8 +398,36 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
         Q4 own_j, lsp_j, rsp_j;
         spread_cell<ORDER>(qa, qb, qc, dqp_j, dqm_j, own_j, lsp_j, rsp_j);
 
+        // finish row r = j-1: its accumulated parts + the spill from row j
+        const int r = j - 1;
+        if (r >= ro0 && r < ro1) {
+            const Q4 tot = q4_add(acc_prev, lsp_j);
+            if (r >= 0 && r < ny) {
+                if (col_real && interior_lane) {
+                    double du[4];
+                    cons_vjp(qa.r, qa.u, qa.v, A.g1, tot, du);  // qa = prims of row j-1
+                    const long long rowk = (long long)r * nx;
+                    const unsigned o = (unsigned)c * 8u;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        double w = A.c * du[f];
+                        if (A.A) w += A.a * ld_buf(buf_rsrc(uniform_ptr(A.A + f * P + rowk)), o);
+                        if (A.B) w += A.b * ld_buf(buf_rsrc(uniform_ptr(A.B + f * P + rowk)), o);
+                        st_buf(buf_rsrc(uniform_ptr(A.W + f * P + rowk)), o, w);
+                    }
+                }
+            } else if (col_real && interior_lane) {
+                // ghost rows: y contributions only
+                if (r == -1) ghost_store(A, 2, 0, c, tot);
+                if (r == -2) ghost_store(A, 2, 1, c, tot);
+                if (r == ny) ghost_store(A, 3, 0, c, tot);
+                if (r == ny + 1) ghost_store(A, 3, 1, c, tot);
+            }
+        }
+        // row j so far: own y part + spill from row j-1
+        Q4 acc = q4_add(own_j, rsp_prev);
+
         // row j's x gather (real rows only)
-        Q4 x_j = q4_zero();
         if (j >= j0 && j < j1) {
             const int sl = ((j % 3) + 3) % 3;
             // x-face F(c-1/2) from cells c-2, c-1 (LDS), c (own), c+1 (LDS)
@@ -422,15 +447,13 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             const Q4 dqL = {bl.r, bl.un, bl.ut, bl.p};  // cotangent of q_plus of cell c-1
             const Q4 dqR = {br.r, br.un, br.ut, br.p};  // cotangent of q_minus of cell c
             // this lane's cell c: dqp from face c+1/2 (next lane's dqL), dqm = dqR
-            Q4 dqp_c = shfl_down_q(dqL, 1);
-            if (lane == kWave - 1) dqp_c = q4_zero();
+            const Q4 dqp_c = q4_from_next(dqL);
             Q4 own_c, lsp_c, rsp_c;
             spread_cell<ORDER>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
-            Q4 from_next = shfl_down_q(lsp_c, 1);
-            if (lane == kWave - 1) from_next = q4_zero();
-            Q4 from_prev = shfl_up_q(rsp_c, 1);
-            if (lane == 0) from_prev = q4_zero();
-            x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            const Q4 from_next = q4_from_next(lsp_c);
+            const Q4 from_prev = q4_from_prev(rsp_c);
+            const Q4 x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            acc = q4_add(acc, x_j);
             // ghost columns of the domain edges (east ones from the last tile only)
             if (c == -1) {
                 ghost_store(A, 0, 0, j, x_j);
@@ -442,36 +465,8 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             }
         }
 
-        // finish row r = j-1: x gather + own y + spill from r-1 + spill from r+1 (= row j)
-        const int r = j - 1;
-        if (r >= ro0 && r < ro1) {
-            Q4 tot = q4_add(q4_add(x_prev, own_prev), q4_add(rsp_prev2, lsp_j));
-            if (r >= 0 && r < ny) {
-                if (col_real && interior_lane) {
-                    double du[4];
-                    cons_vjp(qa.r, qa.u, qa.v, A.g1, tot, du);  // qa = prims of row j-1
-                    const long long k = (long long)r * nx + c;
-#pragma unroll
-                    for (int f = 0; f < 4; ++f) {
-                        double w = A.c * du[f];
-                        if (A.A) w += A.a * A.A[f * P + k];
-                        if (A.B) w += A.b * A.B[f * P + k];
-                        A.W[f * P + k] = w;
-                    }
-                }
-            } else if (col_real && interior_lane) {
-                // ghost rows: y contributions only
-                if (r == -1) ghost_store(A, 2, 0, c, tot);
-                if (r == -2) ghost_store(A, 2, 1, c, tot);
-                if (r == ny) ghost_store(A, 3, 0, c, tot);
-                if (r == ny + 1) ghost_store(A, 3, 1, c, tot);
-            }
-        }
-
         // advance the window
-        x_prev = x_j;
-        own_prev = own_j;
-        rsp_prev2 = rsp_prev;
+        acc_prev = acc;
         rsp_prev = rsp_j;
         dqm_j = dqm_n;
         qa = qb;
@@ -482,38 +477,78 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     }
 }
 
-// Fold the ghost-cell cotangents of one side into the cells they copy:
-// W(target) += c * (d prim/d cons)^T (M dq_ghost), M negating the normal
-// velocity on reflective sides. One thread per boundary index; the two
-// layers of a side are summed by the same thread, sides run one after another.
-__global__ void adj_fold_kernel(const AdjArgs A, int side) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool xs = side < 2;
-    const int len = xs ? A.ny : A.nx;
-    if (i >= len) return;
-    const int n = xs ? A.nx : A.ny;
-    const int bc = A.bc[side];
-    const long long P = A.plane;
-    for (int layer = 0; layer < 2; ++layer) {
-        const double* g = ghost_at(A, side, layer, i);
-        Q4 d = {g[0], g[1], g[2], g[3]};
-        // logical ghost index along the normal direction
-        const int k = (side == 0 || side == 2) ? -1 - layer : n + layer;
-        bool flip;
-        const int src = map_index(k, n, side == 0 || side == 2 ? bc : A.bc[side - 1],
-                                  side == 1 || side == 3 ? bc : A.bc[side + 1], flip);
-        if (flip) {
-            if (xs) d.u = -d.u;
-            else d.v = -d.v;
-        }
-        const long long cell = xs ? (long long)i * A.nx + src : (long long)src * A.nx + i;
-        const double r = A.U[cell], mx = A.U[P + cell], my = A.U[2 * P + cell], E = A.U[3 * P + cell];
-        const PrimE q = cons_to_prim_e(r, mx, my, E, A.g1);
-        double du[4];
-        cons_vjp(q.r, q.u, q.v, A.g1, d, du);
-#pragma unroll
-        for (int f = 0; f < 4; ++f) A.W[f * P + cell] += A.c * du[f];
+// Fold the ghost-cell cotangents into the cells they copy:
+// W(target) += c * (d prim/d cons)^T (sum of M dq_ghost over the ghosts that
+// copy the target), M negating the normal velocity on reflective sides.
+// One thread per cell of the boundary bands (columns 0, 1, nx-2, nx-1 and
+// rows 0, 1, ny-2, ny-1); every ghost maps into these bands, each band cell
+// has exactly one thread (no races), and the four sides are summed in a fixed
+// order.
+__device__ __forceinline__ int band_index(int b, int n) {
+    return b < 2 ? b : n - 4 + b;  // 0, 1, n-2, n-1
+}
+
+__device__ __forceinline__ bool band_first(int b, int n) {
+    // the first of the four band slots that names this index (small n repeats indices)
+    const int k = band_index(b, n);
+    if (k < 0 || k >= n) return false;
+    for (int e = 0; e < b; ++e)
+        if (band_index(e, n) == k) return false;
+    return true;
+}
+
+__global__ void adj_fold_kernel(const AdjArgs A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = A.nx, ny = A.ny;
+    int col, row;
+    if (t < 4 * ny) {
+        const int b = t / ny;
+        if (!band_first(b, nx)) return;
+        col = band_index(b, nx);
+        row = t - b * ny;
+    } else if (t < 4 * ny + 4 * nx) {
+        const int u = t - 4 * ny;
+        const int b = u / nx;
+        if (!band_first(b, ny)) return;
+        row = band_index(b, ny);
+        col = u - b * nx;
+        // cells of the column bands are done by the column-band threads
+        for (int e = 0; e < 4; ++e)
+            if (band_index(e, nx) == col) return;
+    } else {
+        return;
     }
+    Q4 d = q4_zero();
+    bool any = false;
+    for (int side = 0; side < 4; ++side) {
+        const bool xs = side < 2;
+        const int n = xs ? nx : ny;
+        const int target = xs ? col : row;
+        const int along = xs ? row : col;
+        for (int layer = 0; layer < 2; ++layer) {
+            const int k = (side == 0 || side == 2) ? -1 - layer : n + layer;
+            bool flip;
+            const int src = map_index(k, n, A.bc[xs ? 0 : 2], A.bc[xs ? 1 : 3], flip);
+            if (src != target) continue;
+            const double* g = ghost_at(A, side, layer, along);
+            Q4 e = {g[0], g[1], g[2], g[3]};
+            if (flip) {
+                if (xs) e.u = -e.u;
+                else e.v = -e.v;
+            }
+            d = q4_add(d, e);
+            any = true;
+        }
+    }
+    if (!any) return;
+    const long long P = A.plane;
+    const long long cell = (long long)row * nx + col;
+    const double r = A.U[cell], mx = A.U[P + cell], my = A.U[2 * P + cell], E = A.U[3 * P + cell];
+    const PrimE q = cons_to_prim_e(r, mx, my, E, A.g1);
+    double du[4];
+    cons_vjp(q.r, q.u, q.v, A.g1, d, du);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) A.W[f * P + cell] += A.c * du[f];
 }
 
 }  // namespace gd2

@@ -411,11 +411,9 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
     else
         hipLaunchKernelGGL(euler_vjp_kernel<1>, dim3(blocks), dim3(kWave), 0, st, J);
     GD2_HIP(hipGetLastError());
-    for (int side = 0; side < 4; ++side) {
-        const int len = side < 2 ? g->ny : g->nx;
-        hipLaunchKernelGGL(adj_fold_kernel, dim3((len + 255) / 256), dim3(256), 0, st, J, side);
-        GD2_HIP(hipGetLastError());
-    }
+    const long long nfold = 4ll * g->nx + 4ll * g->ny;
+    hipLaunchKernelGGL(adj_fold_kernel, dim3((unsigned)((nfold + 255) / 256)), dim3(256), 0, st, J);
+    GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
 

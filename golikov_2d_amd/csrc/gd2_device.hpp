@@ -164,6 +164,35 @@ __device__ __forceinline__ double from_next_lane(double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Value held by lane - 1 (lane 0 receives 0): DPP wave_shr:1 on both halves.
+__device__ __forceinline__ double from_prev_lane(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sound speed c = sqrt(a / b) (a = gamma p, b = rho) with h = 1 / (2 sqrt(a b)):
+// dc/dp = gamma h and dc/drho = -c^2 h, so the derivatives need no division.
+struct SoundSpeed {
+    double c, h;
+};
+
+__device__ __forceinline__ SoundSpeed sound_speed(double a, double b) {
+    const double x = a * b;
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    h = fma(h, r, h);
+    SoundSpeed s;
+    s.c = x == 0.0 ? 0.0 : (a + a) * h;
+    s.h = x == 0.0 ? 0.0 : h;
+    return s;
+}
+
 // Primitive Euler state.
 struct PrimE {
     double r, u, v, p;
