@@ -16,6 +16,8 @@
 // adj_fold_kernel.
 #pragma once
 
+#include <type_traits>
+
 #include "gd2_euler.hpp"
 
 namespace gd2 {
@@ -30,10 +32,11 @@ struct FaceE {
 // Reverse-mode HLLC (same forward formulas as hllc_euler): given the flux
 // cotangent Fb (mass, normal, tangential, energy) return the cotangents of
 // the left and right face states.
+template <bool GUARD = true>
 __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double gamma, double inv_g1, const FluxE& Fb,
                                          FaceE& Lb, FaceE& Rb) {
     // forward
-    const SoundSpeed sL = sound_speed(gamma * L.p, L.r), sR = sound_speed(gamma * R.p, R.r);
+    const SoundSpeed sL = sound_speed<GUARD>(gamma * L.p, L.r), sR = sound_speed<GUARD>(gamma * R.p, R.r);
     const double cL = sL.c, cR = sR.c;
     const double aL = L.un - cL, aR = R.un - cR;
     const double bL = L.un + cL, bR = R.un + cR;
@@ -103,17 +106,11 @@ __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double 
     b_un += b_e * r * un;
     b_ut += b_e * r * ut;
 
-    FaceE bl = {0.0, 0.0, 0.0, 0.0}, br = {0.0, 0.0, 0.0, 0.0};
-    double b_SL = 0.0, b_SR = 0.0, b_suL = 0.0, b_suR = 0.0;
-    if (left) {
-        bl.r += b_r; bl.un += b_un; bl.ut += b_ut; bl.p += b_p;
-        b_SL += b_S;
-        b_suL += b_su;
-    } else {
-        br.r += b_r; br.un += b_un; br.ut += b_ut; br.p += b_p;
-        b_SR += b_S;
-        b_suR += b_su;
-    }
+    // route the selected side's cotangents with 0/1 weights (exact: x*1, x*0)
+    const double wl = left ? 1.0 : 0.0, wr = left ? 0.0 : 1.0;
+    FaceE bl = {b_r * wl, b_un * wl, b_ut * wl, b_p * wl};
+    FaceE br = {b_r * wr, b_un * wr, b_ut * wr, b_p * wr};
+    double b_SL = b_S * wl, b_SR = b_S * wr, b_suL = b_su * wl, b_suR = b_su * wr;
     const double hp = 0.5 * b_ptz;
     br.p += hp;
     double b_mR = hp * (SM - R.un);
@@ -141,9 +138,13 @@ __device__ __forceinline__ void hllc_vjp(const FaceE& L, const FaceE& R, double 
     bl.un -= b_suL;
     b_SR += b_suR;
     br.un -= b_suR;
-    double b_cL = 0.0, b_cR = 0.0;
-    if (slL) { bl.un += b_SL; b_cL -= b_SL; } else { br.un += b_SL; b_cR -= b_SL; }
-    if (srL) { bl.un += b_SR; b_cL += b_SR; } else { br.un += b_SR; b_cR += b_SR; }
+    // SL = min(uL - cL, uR - cR), SR = max(uL + cL, uR + cR): the winning side's
+    // share, split exactly with 0/1 weights
+    const double t1 = b_SL * (slL ? 1.0 : 0.0), t2 = b_SR * (srL ? 1.0 : 0.0);
+    const double o1 = b_SL - t1, o2 = b_SR - t2;
+    bl.un += t1 + t2;
+    br.un += o1 + o2;
+    const double b_cL = t2 - t1, b_cR = o2 - o1;
     // c = sqrt(gamma p / r): dc/dp = gamma h, dc/dr = -c^2 h (h = 1 / (2 sqrt(gamma p r)))
     bl.p += b_cL * (gamma * sL.h);
     bl.r -= b_cL * (cL * cL * sL.h);
@@ -227,7 +228,7 @@ __device__ __forceinline__ Q4 q4_from_prev(const Q4& x) {
 
 // Cell-level reconstruction adjoint for one direction: slopes from
 // (qm1, q, qp1), face-value fall-backs from the forward formulas.
-template <int ORDER>
+template <int ORDER, bool GUARD = true>
 __device__ __forceinline__ void spread_cell(const PrimE& qm1, const PrimE& q, const PrimE& qp1, const Q4& dqp,
                                             const Q4& dqm, Q4& own, Q4& to_prev, Q4& to_next) {
     if constexpr (ORDER == 1) {
@@ -239,9 +240,14 @@ __device__ __forceinline__ void spread_cell(const PrimE& qm1, const PrimE& q, co
         const MMW wu = mm_weights(q.u - qm1.u, qp1.u - q.u);
         const MMW wv = mm_weights(q.v - qm1.v, qp1.v - q.v);
         const MMW wp = mm_weights(q.p - qm1.p, qp1.p - q.p);
-        const Slope s = slope3<2>(qm1, q, qp1);
-        const bool fpr = !(q.r + 0.5 * s.r > 0.0), fmr = !(q.r - 0.5 * s.r > 0.0);
-        const bool fpp = !(q.p + 0.5 * s.p > 0.0), fmp = !(q.p - 0.5 * s.p > 0.0);
+        bool fpr = false, fmr = false, fpp = false, fmp = false;
+        if constexpr (GUARD) {
+            const Slope s = slope3<2>(qm1, q, qp1);
+            fpr = !(q.r + 0.5 * s.r > 0.0);
+            fmr = !(q.r - 0.5 * s.r > 0.0);
+            fpp = !(q.p + 0.5 * s.p > 0.0);
+            fmp = !(q.p - 0.5 * s.p > 0.0);
+        }
         spread(dqp.r, dqm.r, wr, fpr, fmr, own.r, to_prev.r, to_next.r);
         spread(dqp.u, dqm.u, wu, false, false, own.u, to_prev.u, to_next.u);
         spread(dqp.v, dqm.v, wv, false, false, own.v, to_prev.v, to_next.v);
@@ -321,7 +327,8 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     const int ro1 = (j1 == ny) ? ny + 2 : j1;
 
     const unsigned oc = (unsigned)ci * 8u, oh = (unsigned)hci * 8u;
-    auto load = [&](int jl, int slot, bool to_lds, double Vv[4]) -> PrimE {
+    // pos: every loaded cell of the row (and halo) has rho, p > kPosMin (wave-uniform)
+    auto load = [&](int jl, int slot, bool to_lds, double Vv[4], bool& pos) -> PrimE {
         bool fy;
         const int jj = map_index(jl, ny, A.bc[2], A.bc[3], fy);
         const long long rowk = (long long)jj * nx;
@@ -334,6 +341,7 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
         PrimE q = cons_to_prim_e(ld_buf(ru[0], oc), ld_buf(ru[1], oc), ld_buf(ru[2], oc), ld_buf(ru[3], oc), A.g1);
         if (fx) q.u = -q.u;
         if (fy) q.v = -q.v;
+        bool ok = q.r > kPosMin && q.p > kPosMin;
         // ghost cells have no output in the forward map: their cotangent is 0
         const bool row_real = jl >= 0 && jl < ny;
         Vv[0] = Vv[1] = Vv[2] = Vv[3] = 0.0;
@@ -348,6 +356,7 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                                          A.g1);
                 if (hfx) h.u = -h.u;
                 if (fy) h.v = -h.v;
+                ok = ok && h.r > kPosMin && h.p > kPosMin;
                 double Vh[4] = {0.0, 0.0, 0.0, 0.0};
                 if (row_real && hcol >= 0 && hcol < nx) {
 #pragma unroll
@@ -356,17 +365,21 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                 lds_put8(ring, slot, hslot, h, Vh);
             }
         }
+        pos = __all(ok) != 0;
         return q;
     };
 
     // rows ro0-3 .. ro0: prims; ro0-2, ro0-1 cotangents
     double Vd[4];
-    PrimE qa = load(ro0 - 3, 0, false, Vd);  // q_{j-1} at j = ro0-2
-    PrimE qb = load(ro0 - 2, 0, false, Vd);  // q_j
+    bool pa, pb, pc;
+    PrimE qa = load(ro0 - 3, 0, false, Vd, pa);  // q_{j-1} at j = ro0-2
+    PrimE qb = load(ro0 - 2, 0, false, Vd, pb);  // q_j
     double Vb[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
-    PrimE qc = load(ro0 - 1, ((ro0 - 1) % 3 + 3) % 3, true, Vd);  // q_{j+1}
+    PrimE qc = load(ro0 - 1, ((ro0 - 1) % 3 + 3) % 3, true, Vd, pc);  // q_{j+1}
     double Vc[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
     PrimE qd;
+    // positivity of the rows seen, newest in bit 0; in pass j bits 3..0 are rows j-1 .. j+2
+    unsigned pos_hist = (pa ? 4u : 0u) | (pb ? 2u : 0u) | (pc ? 1u : 0u);
 
     Q4 dqm_j = q4_zero();     // cotangent of row j's minus (south) face value, from G(j-1/2)
     Q4 acc_prev = q4_zero();  // row j-1: everything but the spill from row j
@@ -374,15 +387,19 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
 
     for (int j = ro0 - 2; j <= ro1; ++j) {
         __syncthreads();
-        qd = load(j + 2, ((j + 2) % 3 + 3) % 3, true, Vd);  // q_{j+2}
+        bool pd;
+        qd = load(j + 2, ((j + 2) % 3 + 3) % 3, true, Vd, pd);  // q_{j+2}
         const double Vn[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
+        pos_hist = ((pos_hist << 1) | (pd ? 1u : 0u)) & 15u;
 
-        // y-face G(j+1/2) between rows j and j+1 (forward face iff -1 <= j <= ny-1, real column)
-        Q4 dqp_j, dqm_n;
-        {
+        // y-face G(j+1/2) between rows j and j+1 (forward face iff -1 <= j <= ny-1,
+        // real column), then row j's y spread
+        Q4 dqp_j, dqm_n, own_j, lsp_j, rsp_j;
+        auto ypass = [&](auto guard) {
+            constexpr bool G = decltype(guard)::value;
             const Slope sj = slope3<ORDER>(qa, qb, qc);
             const Slope sn = slope3<ORDER>(qb, qc, qd);
-            const PrimE L = face_plus(qb, sj), R = face_minus(qc, sn);
+            const PrimE L = face_plus<G>(qb, sj), R = face_minus<G>(qc, sn);
             const bool on = col_real && j >= -1 && j <= ny - 1;
             FluxE Fb;  // face cotangent (V_top - V_bottom)/dy in the y-frame (n = y, t = x)
             Fb.fr = on ? (Vc[0] - Vb[0]) * A.idy : 0.0;
@@ -390,13 +407,15 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             Fb.fn = on ? (Vc[2] - Vb[2]) * A.idy : 0.0;
             Fb.fe = on ? (Vc[3] - Vb[3]) * A.idy : 0.0;
             FaceE fl = {L.r, L.v, L.u, L.p}, fr = {R.r, R.v, R.u, R.p}, bl, br;
-            hllc_vjp(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
+            hllc_vjp<G>(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
             dqp_j.r = bl.r; dqp_j.u = bl.ut; dqp_j.v = bl.un; dqp_j.p = bl.p;
             dqm_n.r = br.r; dqm_n.u = br.ut; dqm_n.v = br.un; dqm_n.p = br.p;
-        }
-        // row j's y spread
-        Q4 own_j, lsp_j, rsp_j;
-        spread_cell<ORDER>(qa, qb, qc, dqp_j, dqm_j, own_j, lsp_j, rsp_j);
+            spread_cell<ORDER, G>(qa, qb, qc, dqp_j, dqm_j, own_j, lsp_j, rsp_j);
+        };
+        if (pos_hist == 15u)
+            ypass(std::false_type{});
+        else
+            ypass(std::true_type{});
 
         // finish row r = j-1: its accumulated parts + the spill from row j
         const int r = j - 1;
@@ -427,14 +446,15 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
         // row j so far: own y part + spill from row j-1
         Q4 acc = q4_add(own_j, rsp_prev);
 
-        // row j's x gather (real rows only)
-        if (j >= j0 && j < j1) {
+        // row j's x gather (real rows only); row j (with halo) is bit 2
+        auto xpass = [&](auto guard) {
+            constexpr bool G = decltype(guard)::value;
             const int sl = ((j % 3) + 3) % 3;
             // x-face F(c-1/2) from cells c-2, c-1 (LDS), c (own), c+1 (LDS)
             const PrimE q2 = lds_q(ring, sl, lane), q1 = lds_q(ring, sl, lane + 1), q3 = lds_q(ring, sl, lane + 3);
             const Slope sb = slope3<ORDER>(q2, q1, qb);
             const Slope sc = slope3<ORDER>(q1, qb, q3);
-            const PrimE L = face_plus(q1, sb), R = face_minus(qb, sc);
+            const PrimE L = face_plus<G>(q1, sb), R = face_minus<G>(qb, sc);
             const double Vw0 = ring[sl][4][lane + 1], Vw1 = ring[sl][5][lane + 1];
             const double Vw2 = ring[sl][6][lane + 1], Vw3 = ring[sl][7][lane + 1];
             FluxE Fb;
@@ -443,13 +463,13 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             Fb.ft = xface_on ? (Vb[2] - Vw2) * A.idx : 0.0;
             Fb.fe = xface_on ? (Vb[3] - Vw3) * A.idx : 0.0;
             FaceE fl = {L.r, L.u, L.v, L.p}, fr = {R.r, R.u, R.v, R.p}, bl, br;
-            hllc_vjp(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
+            hllc_vjp<G>(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
             const Q4 dqL = {bl.r, bl.un, bl.ut, bl.p};  // cotangent of q_plus of cell c-1
             const Q4 dqR = {br.r, br.un, br.ut, br.p};  // cotangent of q_minus of cell c
             // this lane's cell c: dqp from face c+1/2 (next lane's dqL), dqm = dqR
             const Q4 dqp_c = q4_from_next(dqL);
             Q4 own_c, lsp_c, rsp_c;
-            spread_cell<ORDER>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
+            spread_cell<ORDER, G>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
             const Q4 from_next = q4_from_next(lsp_c);
             const Q4 from_prev = q4_from_prev(rsp_c);
             const Q4 x_j = q4_add(q4_add(own_c, from_next), from_prev);
@@ -463,6 +483,12 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                 if (c == nx) ghost_store(A, 1, 0, j, x_j);
                 if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
             }
+        };
+        if (j >= j0 && j < j1) {
+            if (pos_hist & 4u)
+                xpass(std::false_type{});
+            else
+                xpass(std::true_type{});
         }
 
         // advance the window

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <map>
 #include <new>
 #include <vector>
@@ -75,6 +76,53 @@ int tile_rows_for(int nx, int ny) {
     return rows;
 }
 
+// Rows per strip for a launch of one-wave strips of a kernel: waves run in
+// rounds of (CUs x resident waves per CU); a round takes about as long as
+// one strip (its rows plus the rows re-read around it, `overhead`), so pick
+// the height minimising rounds x (rows + overhead). GD2_TILE_ROWS overrides.
+int strip_rows(const void* kernel, int tiles_x, int ny, int overhead) {
+    static int env = -1;
+    if (env < 0) {
+        const char* e = std::getenv("GD2_TILE_ROWS");
+        env = e ? std::atoi(e) : 0;
+    }
+    if (env > 0) return env;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    struct Entry {
+        const void* k;
+        int dev, cus, per_cu;
+    };
+    static Entry cache[64];
+    static int ncache = 0;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < ncache; ++i)
+        if (cache[i].k == kernel && cache[i].dev == dev) {
+            cus = cache[i].cus;
+            per_cu = cache[i].per_cu;
+        }
+    if (cus == 0) {
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess || per_cu <= 0)
+            per_cu = 4;
+        if (ncache < 64) cache[ncache++] = Entry{kernel, dev, cus, per_cu};
+    }
+    const long long slots = (long long)cus * per_cu;
+    int best = 16;
+    long long best_cost = -1;
+    for (int rows = 4; rows <= 64; ++rows) {
+        const long long waves = (long long)tiles_x * ((ny + rows - 1) / rows);
+        const long long cost = ((waves + slots - 1) / slots) * (rows + overhead);
+        if (best_cost < 0 || cost <= best_cost) {  // ties: the taller strip (less halo)
+            best_cost = cost;
+            best = rows;
+        }
+    }
+    return best;
+}
+
 static void fill_common(EulerArgs& A, const gd2_grid* g) {
     std::memset(&A, 0, sizeof A);
     A.nx = g->nx;
@@ -99,7 +147,9 @@ static unsigned euler_blocks(const EulerArgs& A) {
 }
 
 template <int ORDER, int FLUX, int LAYOUT, bool W>
-static int launch_euler(const EulerArgs& A, hipStream_t st) {
+static int launch_euler(const EulerArgs& A0, hipStream_t st) {
+    EulerArgs A = A0;
+    A.rows = strip_rows(reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W>), A.tiles_x, A.ny, 2);
     hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W>), dim3(euler_blocks(A)), dim3(kWave), 0, st, A);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
@@ -401,8 +451,10 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
     J.g1 = g->gamma - 1.0;
     J.inv_g1 = 1.0 / (g->gamma - 1.0);
     for (int s2 = 0; s2 < 4; ++s2) J.bc[s2] = g->bc[s2];
-    J.rows = tile_rows_for(g->nx, g->ny);
     J.tiles_x = (g->nx + kAdjW - 1) / kAdjW;
+    J.rows = strip_rows(sc->order == 2 ? reinterpret_cast<const void*>(&euler_vjp_kernel<2>)
+                                       : reinterpret_cast<const void*>(&euler_vjp_kernel<1>),
+                        J.tiles_x, g->ny, 3);
     J.gstride = g->nx > g->ny ? g->nx : g->ny;
     hipStream_t st = (hipStream_t)stream;
     const unsigned blocks = (unsigned)(J.tiles_x * ((g->ny + J.rows - 1) / J.rows));

@@ -178,6 +178,7 @@ struct SoundSpeed {
     double c, h;
 };
 
+template <bool GUARD = true>
 __device__ __forceinline__ SoundSpeed sound_speed(double a, double b) {
     const double x = a * b;
     const double y = __builtin_amdgcn_rsq(x);
@@ -188,8 +189,13 @@ __device__ __forceinline__ SoundSpeed sound_speed(double a, double b) {
     r = fma(-g, h, 0.5);
     h = fma(h, r, h);
     SoundSpeed s;
-    s.c = x == 0.0 ? 0.0 : (a + a) * h;
-    s.h = x == 0.0 ? 0.0 : h;
+    if constexpr (GUARD) {
+        s.c = x == 0.0 ? 0.0 : (a + a) * h;
+        s.h = x == 0.0 ? 0.0 : h;
+    } else {
+        s.c = (a + a) * h;
+        s.h = h;
+    }
     return s;
 }
 
