@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference for HBM counters)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-ensemble and adjoint legs")
+    ap.add_argument("--only-extras", action="store_true",
+                    help="run only the batched-ensemble and adjoint legs (for profiling them)")
     return ap.parse_args()
 
 
@@ -78,6 +82,114 @@ def cpu_baseline(n, seed):
     }
 
 
+def batched_vs_serial(seed=0, count=256, steps=50):
+    """Ensemble of `count` C1-style problems (sides log-uniform on [64, 1024]):
+    one tuned, graph-captured batched advance against a serial loop of
+    per-problem solvers, same steps."""
+    import torch
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    sizes = PB.ensemble_sizes(seed, count=count, lo=64, hi=1024)
+    grids = [api.make_grid(nx, ny) for nx, ny in sizes]
+    sc = api.make_scheme(api.EULER, api.FLUX_HLLC, 2, api.SSPRK2, 0.4)
+    init = [torch.from_numpy(PB.four_quadrant(g, seed=seed * 1000 + p)).to(dev) for p, g in enumerate(grids)]
+    cells = sum(g.nx * g.ny for g in grids)
+
+    ens_U = [u.clone() for u in init]
+    e = api.Ensemble(grids, sc, ens_U)
+    e.start()
+    rows, blocks, _ = e.tune()
+    e.advance(steps, use_graph=True)  # capture + warm-up
+    for u, u0 in zip(ens_U, init):
+        u.copy_(u0)
+    e.start()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.advance(steps, use_graph=True)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+
+    ser_U = [u.clone() for u in init]
+    solvers = [api.Solver(g, sc) for g in grids]
+    for s_, u in zip(solvers, ser_U):  # warm-up pass
+        s_.start(u)
+        s_.advance(u, 1)
+    for u, u0 in zip(ser_U, init):
+        u.copy_(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s_, u in zip(solvers, ser_U):
+        s_.start(u)
+        s_.advance(u, steps)
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    same = all(torch.equal(a, b) for a, b in zip(ens_U, ser_U))
+    for s_ in solvers:
+        s_.close()
+    e.close()
+    return {
+        "value": round(t_serial / t_batched, 3),
+        "unit": "x (serial wall time / batched wall time)",
+        "batched_ms": round(1e3 * t_batched, 3),
+        "serial_ms": round(1e3 * t_serial, 3),
+        "batched_gcells": round(cells * steps / t_batched / 1e9, 3),
+        "problems": count,
+        "steps": steps,
+        "cells": cells,
+        "tiling": {"rows": rows, "blocks": blocks},
+        "bitwise_equal_to_serial": same,
+        "workload": f"{count} problems, sides log-uniform on [64, 1024] (seed {seed}), C1-style four-quadrant "
+                    "states, HLLC + minmod, SSP-RK2, own CFL dt each; batched = tuned persistent launch per "
+                    "stage replayed from a hipGraph",
+    }
+
+
+def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
+    """Differentiable rollout (LibTorch autograd Function): backward (recompute
+    from checkpoints + stage VJPs) against the forward rollout, both timed."""
+    import torch
+    from golikov_2d_amd import adjoint as AD
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    g = api.make_grid(n, n)
+    sc = api.make_scheme(api.EULER, api.FLUX_HLLC, 2, api.SSPRK2, 0.4)
+    quad = PB.four_quadrant(g, seed=seed)
+    U0 = torch.from_numpy(PB.smooth_perturbed_euler(g, quad, seed + 100, 1e-3)).to(dev)
+    Ut = torch.from_numpy(PB.smooth_perturbed_euler(g, quad, seed + 200, 1e-2)).to(dev)
+    dts, _ = AD.dt_schedule(g, sc, U0, steps)
+    with torch.no_grad():
+        rho_t = AD.rollout(g, sc, Ut, dts, every=steps + 1)[0].clone()
+    fwd, bwd = [], []
+    for _ in range(reps + 1):
+        X = U0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        UT = AD.rollout(g, sc, X, dts, every=every)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        AD.density_loss(UT, rho_t).backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f, b = sorted(fwd[1:])[reps // 2], sorted(bwd[1:])[reps // 2]  # medians after one warm-up
+    return {
+        "value": round(b / f, 3),
+        "unit": "x (backward wall time / forward wall time)",
+        "forward_ms": round(1e3 * f, 3),
+        "backward_ms": round(1e3 * b, 3),
+        "n": n,
+        "steps": steps,
+        "checkpoint_every": every,
+        "workload": f"{n}x{n} four-quadrant state (seed {seed}) with a 1e-3 smooth perturbation, HLLC + minmod, "
+                    f"SSP-RK2, {steps} recorded CFL steps, loss 1/2 ||rho(T) - rho_target||^2",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -89,6 +201,11 @@ def main():
     import torch
     from golikov_2d_amd import api
     from golikov_2d_amd import problems as PB
+
+    if args.only_extras:
+        print(json.dumps({"batched_vs_serial": batched_vs_serial(args.seed),
+                          "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed)}), flush=True)
+        return
 
     n = args.n
     dev = torch.device("cuda:0")
@@ -117,24 +234,23 @@ def main():
     p = 0.4 * (Uh[3] - 0.5 * (Uh[1] ** 2 + Uh[2] ** 2) / rho)
     healthy = bool(np.isfinite(Uh).all() and rho.min() > 0 and p.min() > 0)
 
-    # stage-kernel duration with HIP events on the launch stream
+    # stage-kernel durations with HIP events on the launch stream, launches
+    # back to back as in the steps above (one synchronisation at the end)
     stream = torch.cuda.current_stream()
     S = torch.empty_like(U)
     dtd = torch.tensor([dt_last], dtype=torch.float64, device=dev)
     w = torch.zeros(1, dtype=torch.int64, device=dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    t1 = t2 = 0.0
-    for _ in range(args.kernel_reps):
-        ev[0].record(stream)
+    reps = args.kernel_reps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps + 1)]
+    ev[0].record(stream)
+    for i in range(reps):
         api.stage(g, sc, U, None, S, 0.0, 1.0, dt_dev=dtd)
-        ev[1].record(stream)
+        ev[2 * i + 1].record(stream)
         api.stage(g, sc, S, U, U, 0.5, 0.5, dt_dev=dtd, wmax_bits=w)
-        ev[2].record(stream)
-        torch.cuda.synchronize()
-        t1 += ev[0].elapsed_time(ev[1])
-        t2 += ev[1].elapsed_time(ev[2])
-    t1 /= args.kernel_reps
-    t2 /= args.kernel_reps
+        ev[2 * i + 2].record(stream)
+    torch.cuda.synchronize()
+    t1 = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps)) / reps
+    t2 = sum(ev[2 * i + 1].elapsed_time(ev[2 * i + 2]) for i in range(reps)) / reps
     stage_ms = 0.5 * (t1 + t2)
     achieved = BYTES_PER_CELL_UPDATE / 2.0 * cells / (stage_ms / 1e3) / 1e9  # GB/s per average launch
     roofline = {
@@ -162,6 +278,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.seed)
+    extras = None
+    if not args.no_extras:
+        del S
+        extras = {"batched_vs_serial": batched_vs_serial(args.seed),
+                  "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed)}
 
     out = {
         "metric": "Gcell-updates/s FP64 HLLC 4096^2 (% HBM peak)",
@@ -181,6 +302,7 @@ def main():
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extras": extras,
         "healthy": healthy,
         "sim_time": t_sim,
         "sim_steps": nsteps,
