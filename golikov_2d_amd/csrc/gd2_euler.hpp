@@ -6,7 +6,7 @@
 //     registers, so every y-face flux G(j+1/2) is computed once and carried
 //     to the next row;
 //   * x-direction: each loaded row (cbase-2 .. cbase+64, converted to
-//     primitives once) goes to a 3-row LDS ring; lane l computes the x-face
+//     primitives once) goes to a 2-row LDS ring; lane l computes the x-face
 //     F(c-1/2) from LDS and receives F(c+1/2) from lane l+1 (lane 63 only
 //     supplies that face), so every x-face is also computed once.
 // A stage reads each state value of the strip once from HBM (plus a 2-cell
@@ -120,6 +120,14 @@ __device__ __forceinline__ RowVal<LAYOUT> finish_cell(const EulerArgs& A, const 
     return v;
 }
 
+// Per-wave LDS: primitive rows (own cells and halo) for the x-faces and the
+// conserved own cells (kSoA) waiting for their update, each a ring of two
+// rows (row r in slot r & 1): 8.4 KB, so four waves per SIMD fit in LDS.
+struct TileLds {
+    double q[2][4][kRowBuf];
+    double u[2][4][kWave];
+};
+
 __device__ __forceinline__ void lds_put(double (*ring)[4][kRowBuf], int slot, int e, const PrimE& q) {
     ring[slot][0][e] = q.r;
     ring[slot][1][e] = q.u;
@@ -202,7 +210,7 @@ __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
 
 // One wave's strip: columns tx*63 .. tx*63+62, rows ty*rows .. +rows-1.
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
-__device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, double (*ring)[4][kRowBuf]) {
+__device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, TileLds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kTileW;
     __syncthreads();  // a wave may run several tiles: the previous tile's LDS reads are done
@@ -242,17 +250,21 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
         if (has_halo) pr.halo = load_raw(rp, oh);
         return pr;
     };
-    // convert a pending row; to_lds: also publish it (and the halo) in ring
-    // slot. pos: whether every cell of the row (and halo) has rho, p > kPosMin
-    // (wave-uniform).
+    // convert a pending row; to_lds: also publish it (and the halo) in LDS
+    // slot, with the own conserved values (kSoA). pos: whether every cell of
+    // the row (and halo) has rho, p > kPosMin (wave-uniform).
     auto finish_row = [&](const Pending& pr, int slot, bool to_lds, bool& pos) -> RowVal<LAYOUT> {
         RowVal<LAYOUT> v = finish_cell<LAYOUT>(A, pr.own, fx, pr.fy);
         bool ok = v.q.r > kPosMin && v.q.p > kPosMin;
         if (to_lds) {
-            lds_put(ring, slot, lane + 2, v.q);
+            lds_put(lds.q, slot, lane + 2, v.q);
+            if constexpr (LAYOUT == kSoA) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) lds.u[slot][f][lane] = pr.own.a[f];
+            }
             if (has_halo) {
                 RowVal<LAYOUT> h = finish_cell<LAYOUT>(A, pr.halo, hfx, pr.fy);
-                lds_put(ring, slot, hslot, h.q);
+                lds_put(lds.q, slot, hslot, h.q);
                 ok = ok && h.q.r > kPosMin && h.q.p > kPosMin;
             }
         }
@@ -268,7 +280,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     bool pa, pb, pc;
     RowVal<LAYOUT> vm2 = finish_row(pm2, 0, false, pa);
     RowVal<LAYOUT> v0 = finish_row(pm1, 0, false, pb);
-    RowVal<LAYOUT> v1 = finish_row(pj0, j0 % 3, true, pc);
+    RowVal<LAYOUT> v1 = finish_row(pj0, j0 & 1, true, pc);
     // positivity of the last rows seen, newest in bit 0: rows j-1 .. j+2
     // (the whole stencil of the faces of pass j) are in bits 3 .. 0
     unsigned pos_hist = (pa ? 4u : 0u) | (pb ? 2u : 0u) | (pc ? 1u : 0u);
@@ -282,7 +294,6 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
     double tmin = 1e300;
 
     for (int j = j0 - 1; j < j1; ++j) {
-        __syncthreads();  // slot (j+2)%3 = slot of row j-1, whose x-faces are done
         // loads first: row j+2 arrives while the x-face of row j is computed
         const Pending pn = issue_row(j + 2);
 
@@ -293,7 +304,8 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
         // row j (with its halo) was vetted when it was converted: bit 1 now
         const bool xpos = (pos_hist & 2u) != 0u;
         if (j >= j0) {
-            const int sl = j % 3;
+            const int sl = j & 1;
+            const double (*ring)[4][kRowBuf] = lds.q;
             PrimE qa, qb, qd;
             qa.r = ring[sl][0][lane]; qa.u = ring[sl][1][lane]; qa.v = ring[sl][2][lane]; qa.p = ring[sl][3][lane];
             qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
@@ -316,11 +328,15 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             Fr.fe = from_next_lane(Fl.fe);
             if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
         }
-        // the update's inputs arrive while the y-face is computed
+        // the update's inputs: row j's conserved values from LDS (stored when
+        // the row was converted), the RK base from memory
         double ub[4] = {0.0, 0.0, 0.0, 0.0};
-        Raw uc;  // conserved state of row j (re-read: cheaper than carrying it in registers)
+        Raw uc;
         if constexpr (LAYOUT == kSoA) {
-            if (j >= j0) uc = load_raw(row_ptr<LAYOUT>(A, j), oc);
+            if (j >= j0) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) uc.a[f] = lds.u[j & 1][f][lane];
+            }
             if (!A.plain && j >= j0 && owns) {
                 const double* b = A.Ubase + (long long)j * A.nx;
                 ub[0] = ld_buf(buf_rsrc(uniform_ptr(b)), oo);
@@ -330,8 +346,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
             }
         }
 
+        // row j's LDS slot is read: row j+2 may take it
+        __syncthreads();
         bool pn_pos;
-        RowVal<LAYOUT> v2 = finish_row(pn, (j + 2) % 3, true, pn_pos);
+        RowVal<LAYOUT> v2 = finish_row(pn, (j + 2) & 1, true, pn_pos);
         pos_hist = ((pos_hist << 1) | (pn_pos ? 1u : 0u)) & 15u;
         // y-face G(j+1/2) from rows j-1 .. j+2
         Slope s1 = slope3<ORDER>(v0.q, v1.q, v2.q);
@@ -457,9 +475,9 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, d
 
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 4 : 1))) void euler_tile_kernel(const EulerArgs A) {
-    __shared__ double ring[3][4][kRowBuf];
+    __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    euler_tile<ORDER, FLUX, LAYOUT, WANT_W>(A, tile % A.tiles_x, tile / A.tiles_x, ring);
+    euler_tile<ORDER, FLUX, LAYOUT, WANT_W>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
 }
 
 // Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA
