@@ -146,23 +146,25 @@ static unsigned euler_blocks(const EulerArgs& A) {
     return (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool W>
+template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE>
 static int launch_euler(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
-    A.rows = strip_rows(reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W>), A.tiles_x, A.ny, 2);
-    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W>), dim3(euler_blocks(A)), dim3(kWave), 0, st, A);
+    const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE>);
+    A.rows = strip_rows(k, A.tiles_x, A.ny, 2);
+    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE>), dim3(euler_blocks(A)), dim3(kWave), 0, st,
+                       A);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
 
-template <int LAYOUT, bool W>
+template <int LAYOUT, bool W, bool BASE>
 static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t st) {
     if (order == 2) {
-        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, LAYOUT, W>(A, st);
-        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, LAYOUT, W>(A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, LAYOUT, W, BASE>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
     } else if (order == 1) {
-        if (flux == GD2_FLUX_HLLC) return launch_euler<1, GD2_FLUX_HLLC, LAYOUT, W>(A, st);
-        if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W>(A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_euler<1, GD2_FLUX_HLLC, LAYOUT, W, BASE>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
     }
     return GD2_ERR_UNSUPPORTED;
 }
@@ -282,13 +284,16 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
     A.Uout = Uout;
     A.a = a;
     A.b = b;
-    A.plain = plain ? 1 : 0;
     A.dt = dt;
     A.dt_dev = dt_dev;
     A.wmax_bits = wmax_bits;
     if (!A.diag) return GD2_ERR_HIP;
-    if (wmax_bits) return dispatch_euler<kSoA, true>(sc->order, sc->flux, A, st);
-    return dispatch_euler<kSoA, false>(sc->order, sc->flux, A, st);
+    if (plain) {
+        if (wmax_bits) return dispatch_euler<kSoA, true, false>(sc->order, sc->flux, A, st);
+        return dispatch_euler<kSoA, false, false>(sc->order, sc->flux, A, st);
+    }
+    if (wmax_bits) return dispatch_euler<kSoA, true, true>(sc->order, sc->flux, A, st);
+    return dispatch_euler<kSoA, false, true>(sc->order, sc->flux, A, st);
 }
 
 int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, unsigned long long* wmax_bits,
@@ -316,11 +321,10 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     A.u2 = reinterpret_cast<double2*>(u2);
     A.dt_dev = T_do;
     A.tmin_bits = reinterpret_cast<unsigned long long*>(T);
-    A.plain = 1;
     A.a = 0.0;
     A.b = 1.0;
     if (!A.diag) return GD2_ERR_HIP;
-    return dispatch_euler<kAoS, false>(order, method == 2 ? GD2_FLUX_EXACT : GD2_FLUX_HLLC, A, (hipStream_t)stream);
+    return dispatch_euler<kAoS, false, false>(order, method == 2 ? GD2_FLUX_EXACT : GD2_FLUX_HLLC, A, (hipStream_t)stream);
 }
 
 int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2, gd2_double2* u2,

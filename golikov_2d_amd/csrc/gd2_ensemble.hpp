@@ -29,7 +29,7 @@ struct EnsTile {
 };
 
 template <int ORDER, int FLUX, int STAGE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 4 : 1))) void ens_stage_kernel(const EnsProblem* __restrict__ probs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 3 : 1))) void ens_stage_kernel(const EnsProblem* __restrict__ probs,
                                                        const EnsTile* __restrict__ tiles, int ntiles, int slot) {
     __shared__ TileLds lds;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -43,18 +43,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_
             A.Uout = P.S;
             A.a = 0.0;
             A.b = 1.0;
-            A.plain = 1;
             A.wmax_bits = nullptr;
-            euler_tile<ORDER, FLUX, kSoA, false>(A, tl.tx, tl.ty, lds);
+            euler_tile<ORDER, FLUX, kSoA, false, false>(A, tl.tx, tl.ty, lds);
         } else {
             A.Uin = P.S;
             A.Ubase = P.U;
             A.Uout = P.U;
             A.a = 0.5;
             A.b = 0.5;
-            A.plain = 0;
             A.wmax_bits = &P.clk->w[slot ^ 1];
-            euler_tile<ORDER, FLUX, kSoA, true>(A, tl.tx, tl.ty, lds);
+            euler_tile<ORDER, FLUX, kSoA, true, true>(A, tl.tx, tl.ty, lds);
         }
     }
 }

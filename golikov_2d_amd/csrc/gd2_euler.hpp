@@ -44,7 +44,6 @@ struct EulerArgs {
     int bc[4];
     // time stepping
     double a, b;
-    int plain;                     // a == 0 && b == 1
     double dt;                     // used when dt_dev == null
     const double* dt_dev;
     unsigned long long* wmax_bits; // kSoA: wave-speed max of the output (may be null)
@@ -67,6 +66,12 @@ struct RowVal {
 struct Raw {
     double a[4];
 };
+
+// Byte offset that puts a buffer access out of range: the load returns 0 and
+// touches no memory. Lanes without a cell to load use it, so every load of a
+// row is issued by the whole wave without a branch and the wait counts stay
+// exact.
+constexpr unsigned kOffNone = 0x80000000u;
 
 // Buffer resources of one logical row (after the boundary map): one per
 // conserved plane (kSoA), or s and u twice (kAoS, second double at +8 bytes).
@@ -122,7 +127,7 @@ __device__ __forceinline__ RowVal<LAYOUT> finish_cell(const EulerArgs& A, const 
 
 // Per-wave LDS: primitive rows (own cells and halo) for the x-faces and the
 // conserved own cells (kSoA) waiting for their update, each a ring of two
-// rows (row r in slot r & 1): 8.4 KB, so four waves per SIMD fit in LDS.
+// rows (row r in slot r & 1): 8.4 KB per wave.
 struct TileLds {
     double q[2][4][kRowBuf];
     double u[2][4][kWave];
@@ -209,7 +214,9 @@ __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
 }
 
 // One wave's strip: columns tx*63 .. tx*63+62, rows ty*rows .. +rows-1.
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
+// BASE: the output is a*Ubase + b*(stage result) (an SSP-RK combination);
+// otherwise it is the stage result itself.
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE>
 __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, TileLds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kTileW;
@@ -227,8 +234,9 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     const bool owns = (lane < kTileW) && (c < A.nx);
     constexpr unsigned kCellBytes = LAYOUT == kSoA ? 8u : 16u;
     const unsigned oc = (unsigned)ci * kCellBytes;   // own cell within a row
-    const unsigned oh = (unsigned)hci * kCellBytes;  // halo cell within a row
+    const unsigned oh = has_halo ? (unsigned)hci * kCellBytes : kOffNone;  // halo cell within a row
     const unsigned oo = (unsigned)c * 8u;            // output cell within a row (kSoA)
+    const unsigned ob_off = owns ? oo : kOffNone;    // RK base cell within a row
 
     const double dt = A.dt_dev ? *A.dt_dev : A.dt;
     const double dx = A.dx, dy = A.dy;
@@ -247,7 +255,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], pr.fy);
         const RowPtr rp = row_ptr<LAYOUT>(A, jj);
         pr.own = load_raw(rp, oc);
-        if (has_halo) pr.halo = load_raw(rp, oh);
+        pr.halo = load_raw(rp, oh);
         return pr;
     };
     // convert a pending row; to_lds: also publish it (and the halo) in LDS
@@ -329,7 +337,8 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
         }
         // the update's inputs: row j's conserved values from LDS (stored when
-        // the row was converted), the RK base from memory
+        // the row was converted), the RK base from memory (issued by the whole
+        // wave; the lead-in pass and lanes without a cell load nothing)
         double ub[4] = {0.0, 0.0, 0.0, 0.0};
         Raw uc;
         if constexpr (LAYOUT == kSoA) {
@@ -337,12 +346,13 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
 #pragma unroll
                 for (int f = 0; f < 4; ++f) uc.a[f] = lds.u[j & 1][f][lane];
             }
-            if (!A.plain && j >= j0 && owns) {
-                const double* b = A.Ubase + (long long)j * A.nx;
-                ub[0] = ld_buf(buf_rsrc(uniform_ptr(b)), oo);
-                ub[1] = ld_buf(buf_rsrc(uniform_ptr(b + A.plane)), oo);
-                ub[2] = ld_buf(buf_rsrc(uniform_ptr(b + 2 * A.plane)), oo);
-                ub[3] = ld_buf(buf_rsrc(uniform_ptr(b + 3 * A.plane)), oo);
+            if constexpr (BASE) {
+                const unsigned off = j >= j0 ? ob_off : kOffNone;
+                const double* b = A.Ubase + (long long)max(j, j0) * A.nx;
+                ub[0] = ld_buf(buf_rsrc(uniform_ptr(b)), off);
+                ub[1] = ld_buf(buf_rsrc(uniform_ptr(b + A.plane)), off);
+                ub[2] = ld_buf(buf_rsrc(uniform_ptr(b + 2 * A.plane)), off);
+                ub[3] = ld_buf(buf_rsrc(uniform_ptr(b + 3 * A.plane)), off);
             }
         }
 
@@ -385,7 +395,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 double o1 = uc.a[1] - dtdV * PUx;
                 double o2 = uc.a[2] - dtdV * PUy;
                 double o3 = uc.a[3] - dtdV * PSe;
-                if (!A.plain) {
+                if constexpr (BASE) {
                     o0 = A.a * ub[0] + A.b * o0;
                     o1 = A.a * ub[1] + A.b * o1;
                     o2 = A.a * ub[2] + A.b * o2;
@@ -473,11 +483,11 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     }
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 4 : 1))) void euler_tile_kernel(const EulerArgs A) {
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 3 : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    euler_tile<ORDER, FLUX, LAYOUT, WANT_W>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
+    euler_tile<ORDER, FLUX, LAYOUT, WANT_W, BASE>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
 }
 
 // Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA
