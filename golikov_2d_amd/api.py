@@ -77,6 +77,7 @@ def lib():
             "gd2_last_hip_error": (ip, []),
             "gd2_version": (C.c_char_p, []),
             "gd2_stage": (ip, [gp, sp, vp, vp, vp, dp, dp, dp, vp, vp, vp]),
+            "gd2_ssprk2_step": (ip, [gp, sp, vp, vp, dp, vp, vp, vp]),
             "gd2_wavespeed_max": (ip, [gp, sp, vp, vp, vp]),
             "gd2_ker_dekard": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
             "gd2_kernel_tvd_old": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
@@ -117,7 +118,7 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
-            "gd2_vjp_workspace_bytes", "gd2_stage_vjp"]
+            "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step"]
 
 
 def _check(rc, name):
@@ -193,6 +194,13 @@ def funk_time(T, T_do, TT, i, stream=None):
 def stage(g, sc, Uin, Ubase, Uout, a=0.0, b=1.0, dt=0.0, dt_dev=None, wmax_bits=None, stream=None):
     _check(lib().gd2_stage(C.byref(g), C.byref(sc), _f64(Uin), _f64(Ubase), _f64(Uout), float(a), float(b),
                            float(dt), _f64(dt_dev), _p(wmax_bits), _stream(stream)), "gd2_stage")
+
+
+def ssprk2_step(g, sc, Uin, Uout, dt=0.0, dt_dev=None, wmax_bits=None, stream=None):
+    """One SSP-RK2 step Uin -> Uout in a single pass (Euler); bitwise equal to
+    stage(Uin -> S) followed by stage(S, Ubase=Uin -> Uout, a=b=1/2)."""
+    _check(lib().gd2_ssprk2_step(C.byref(g), C.byref(sc), _f64(Uin), _f64(Uout), float(dt), _f64(dt_dev),
+                                 _p(wmax_bits), _stream(stream)), "gd2_ssprk2_step")
 
 
 def vjp_workspace_bytes(g) -> int:

@@ -14,6 +14,7 @@
 
 #include "../../include/gd2.h"
 #include "gd2_euler.hpp"
+#include "gd2_rk2.hpp"
 #include "gd2_mhd.hpp"
 #include "gd2_ensemble.hpp"
 #include "gd2_adjoint.hpp"
@@ -169,6 +170,31 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
     return GD2_ERR_UNSUPPORTED;
 }
 
+// One-pass SSP-RK2 step: strips of 59 columns; the strip height balances
+// rounds of resident waves against the four extra U1 rows each strip forms.
+template <int ORDER, int FLUX>
+static int launch_rk2(const EulerArgs& A0, hipStream_t st) {
+    EulerArgs A = A0;
+    A.tiles_x = (A.nx + kRk2W - 1) / kRk2W;
+    const void* k = reinterpret_cast<const void*>(&euler_rk2_kernel<ORDER, FLUX>);
+    A.rows = max(2, strip_rows(k, A.tiles_x, A.ny, 5));  // strips start at row 0 or row >= 2
+    const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
+    hipLaunchKernelGGL((euler_rk2_kernel<ORDER, FLUX>), dim3(blocks), dim3(kWave), 0, st, A);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+static int dispatch_rk2(int order, int flux, const EulerArgs& A, hipStream_t st) {
+    if (order == 2) {
+        if (flux == GD2_FLUX_HLLC) return launch_rk2<2, GD2_FLUX_HLLC>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_rk2<2, GD2_FLUX_EXACT>(A, st);
+    } else if (order == 1) {
+        if (flux == GD2_FLUX_HLLC) return launch_rk2<1, GD2_FLUX_HLLC>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_rk2<1, GD2_FLUX_EXACT>(A, st);
+    }
+    return GD2_ERR_UNSUPPORTED;
+}
+
 static bool scheme_ok(const gd2_scheme* sc) {
     if (!sc || (sc->order != 1 && sc->order != 2)) return false;
     if (sc->physics == GD2_EULER) return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT;
@@ -294,6 +320,26 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
     }
     if (wmax_bits) return dispatch_euler<kSoA, true, true>(sc->order, sc->flux, A, st);
     return dispatch_euler<kSoA, false, true>(sc->order, sc->flux, A, st);
+}
+
+int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double dt,
+                    const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
+    if (!grid_ok(g, false) || !scheme_ok(sc) || !Uin || !Uout || Uin == Uout) return GD2_ERR_ARG;
+    if (sc->physics != GD2_EULER) return GD2_ERR_UNSUPPORTED;
+    // the boundary rows and columns of the intermediate state are built from
+    // its first and last two rows / columns
+    if (g->nx < 2 || g->ny < 2) return GD2_ERR_UNSUPPORTED;
+    EulerArgs A;
+    fill_common(A, g);
+    A.Uin = Uin;
+    A.Uout = Uout;
+    A.a = 0.5;
+    A.b = 0.5;
+    A.dt = dt;
+    A.dt_dev = dt_dev;
+    A.wmax_bits = wmax_bits;
+    if (!A.diag) return GD2_ERR_HIP;
+    return dispatch_rk2(sc->order, sc->flux, A, (hipStream_t)stream);
 }
 
 int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, unsigned long long* wmax_bits,
@@ -491,19 +537,37 @@ int gd2_ctx_start(gd2_ctx* c, const double* U, double t0, void* stream) {
     return gd2_wavespeed_max(&c->g, &c->sc, U, &c->clock->w[0], stream);
 }
 
+// GD2_TWO_PASS=1 makes the solver context step SSP-RK2 with two stage
+// launches instead of the one-pass kernel (same results, for comparison).
+static bool one_pass_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("GD2_TWO_PASS");
+        v = (e && std::atoi(e) == 1) ? 0 : 1;
+    }
+    return v == 1;
+}
+
 int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* stream) {
     if (!c || !U || nsteps < 0) return GD2_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     double* S = c->scratch;
     const double* dtp = &c->clock->dt;
-    double* cur = U;  // forward Euler ping-pongs between U and S
+    double* cur = U;  // forward Euler and the one-pass SSP-RK2 step ping-pong between U and S
+    const bool one_pass = c->sc.physics == GD2_EULER && c->g.nx >= 2 && c->g.ny >= 2 && one_pass_enabled();
     for (int i = 0; i < nsteps; ++i) {
         const int slot = (int)(c->host_steps & 1);
         hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log, c->dt_cap);
         GD2_HIP(hipGetLastError());
         unsigned long long* wn = &c->clock->w[slot ^ 1];
         int rc;
-        if (c->sc.integrator == GD2_SSPRK2) {
+        if (c->sc.integrator == GD2_SSPRK2 && one_pass) {
+            // one pass per step, ping-ponging between U and the scratch state
+            double* nxt = (cur == U) ? S : U;
+            rc = gd2_ssprk2_step(&c->g, &c->sc, cur, nxt, 0.0, dtp, wn, stream);
+            if (rc) return rc;
+            cur = nxt;
+        } else if (c->sc.integrator == GD2_SSPRK2) {
             rc = gd2_stage(&c->g, &c->sc, U, nullptr, S, 0.0, 1.0, 0.0, dtp, nullptr, stream);
             if (rc) return rc;
             rc = gd2_stage(&c->g, &c->sc, S, U, U, 0.5, 0.5, 0.0, dtp, wn, stream);

@@ -203,6 +203,80 @@ __device__ __forceinline__ PrimE face_minus(const PrimE& q, const Slope& s) {
     return f;
 }
 
+// Flux balance of a cell per conserved field (rho, m_x, m_y, E): the sum of
+// outward fluxes times face lengths (kernel.cu:1854-1857), differenced per
+// direction so that equal opposite fluxes cancel exactly. The x part
+// (F(c+1/2) - F(c-1/2)) dy is formed as soon as the x-faces are known; the y
+// part is added with one explicit multiply-add, so every rounding is fixed by
+// the source and the one-pass step reproduces the stage-by-stage one bitwise.
+struct Bal {
+    double r, mx, my, e;
+};
+
+__device__ __forceinline__ Bal x_balance(const FluxE& Fl, const FluxE& Fr, double dy) {
+    Bal X;
+    X.r = (Fr.fr - Fl.fr) * dy;
+    X.mx = (Fr.fn - Fl.fn) * dy;
+    X.my = (Fr.ft - Fl.ft) * dy;
+    X.e = (Fr.fe - Fl.fe) * dy;
+    return X;
+}
+
+// y-face fluxes are in the face frame (n = +y): fn is m_y, ft is m_x
+__device__ __forceinline__ Bal full_balance(const Bal& X, const FluxE& Gm, const FluxE& Gp, double dx) {
+    Bal P;
+    P.r = fma(Gp.fr - Gm.fr, dx, X.r);
+    P.mx = fma(Gp.ft - Gm.ft, dx, X.mx);
+    P.my = fma(Gp.fn - Gm.fn, dx, X.my);
+    P.e = fma(Gp.fe - Gm.fe, dx, X.e);
+    return P;
+}
+
+// Stage outputs in conserved form: plain o = u - dtdV PS; with an RK base,
+// o = H - (b dtdV) PS with H = a ub + b u (formed when u is produced).
+__device__ __forceinline__ double out_plain(double u, double dtdV, double ps) { return fma(-dtdV, ps, u); }
+__device__ __forceinline__ double rk_base(double a, double ub, double b, double u) { return fma(a, ub, b * u); }
+__device__ __forceinline__ double out_base(double h, double bdt, double ps) { return fma(-bdt, ps, h); }
+
+// Clamps of a stage output (kernel.cu:1862-1866, 1871-1874) in conserved
+// form, counted when `count`. Without the wave-speed epilogue the pressure
+// test needs no division: p <= 0 exactly when rho E - |m|^2 / 2 <= 0.
+__device__ __forceinline__ void clamp_plain(const EulerArgs& A, double& o0, double o1, double o2, double& o3,
+                                            bool count) {
+    if (o0 <= 0.0) {
+        if (count) atomicAdd(&A.diag->clamp_rho, 1ull);
+        o0 = 0.0001;
+    }
+    if (o3 * o0 - 0.5 * (o1 * o1 + o2 * o2) <= 0.0) {
+        if (count) atomicAdd(&A.diag->clamp_p, 1ull);
+        const double inv = fast_rcp(o0);
+        const double ke = 0.5 * (o1 * (o1 * inv) + o2 * (o2 * inv));
+        o3 = 0.000001 / A.g1 + ke;
+    }
+}
+
+// The same clamps with the output's cell wave-speed bound
+// (|u| + c) / dx + (|v| + c) / dy folded into wmax.
+__device__ __forceinline__ void clamp_wavespeed(const EulerArgs& A, double& o0, double o1, double o2, double& o3,
+                                                bool count, double& wmax) {
+    if (o0 <= 0.0) {
+        if (count) atomicAdd(&A.diag->clamp_rho, 1ull);
+        o0 = 0.0001;
+    }
+    const double inv = fast_rcp(o0);
+    const double ux = o1 * inv, uy = o2 * inv;
+    const double ke = 0.5 * (o1 * ux + o2 * uy);
+    double p = A.g1 * (o3 - ke);
+    if (p <= 0.0) {
+        if (count) atomicAdd(&A.diag->clamp_p, 1ull);
+        o3 = 0.000001 / A.g1 + ke;
+        p = A.g1 * (o3 - ke);
+    }
+    const double cs = sqrt_ratio(A.gamma * p, o0);
+    const double w = (fabs(ux) + cs) * A.idx + (fabs(uy) + cs) * A.idy;
+    if (count) wmax = fmax(wmax, w);
+}
+
 template <int LAYOUT>
 __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
     if constexpr (LAYOUT == kAoS) {
@@ -308,6 +382,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         // x-face F(c-1/2) of row j from cells c-2, c-1 (LDS), c (own), c+1
         // (LDS); row j reached LDS two iterations ago
         FluxE Fl, Fr;
+        Bal X;
         double tFl = 0.0, tFr = 0.0;
         // row j (with its halo) was vetted when it was converted: bit 1 now
         const bool xpos = (pos_hist & 2u) != 0u;
@@ -335,6 +410,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             Fr.ft = from_next_lane(Fl.ft);
             Fr.fe = from_next_lane(Fl.fe);
             if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
+            X = x_balance(Fl, Fr, dy);
         }
         // the update's inputs: row j's conserved values from LDS (stored when
         // the row was converted), the RK base from memory (issued by the whole
@@ -383,52 +459,26 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         }
 
         if (owns && !frozen_cell<LAYOUT>(A, c, j)) {
-            // sum of outward fluxes times face lengths (kernel.cu:1854-1857),
-            // differenced per direction so that equal opposite fluxes cancel
-            // exactly whatever the multiply-add contraction
-            const double PSr = (Fr.fr - Fl.fr) * dy + (Gp.fr - Gm.fr) * dx;
-            const double PSe = (Fr.fe - Fl.fe) * dy + (Gp.fe - Gm.fe) * dx;
-            const double PUx = (Fr.fn - Fl.fn) * dy + (Gp.ft - Gm.ft) * dx;
-            const double PUy = (Fr.ft - Fl.ft) * dy + (Gp.fn - Gm.fn) * dx;
+            const Bal P = full_balance(X, Gm, Gp, dx);
             if constexpr (LAYOUT == kSoA) {
-                double o0 = uc.a[0] - dtdV * PSr;
-                double o1 = uc.a[1] - dtdV * PUx;
-                double o2 = uc.a[2] - dtdV * PUy;
-                double o3 = uc.a[3] - dtdV * PSe;
+                double o0, o1, o2, o3;
                 if constexpr (BASE) {
-                    o0 = A.a * ub[0] + A.b * o0;
-                    o1 = A.a * ub[1] + A.b * o1;
-                    o2 = A.a * ub[2] + A.b * o2;
-                    o3 = A.a * ub[3] + A.b * o3;
-                }
-                // clamps (kernel.cu:1862-1866, 1871-1874) in conserved form
-                if (o0 <= 0.0) {
-                    atomicAdd(&A.diag->clamp_rho, 1ull);
-                    o0 = 0.0001;
-                }
-                double* ob = A.Uout + (long long)j * A.nx;
-                if constexpr (WANT_W) {
-                    const double inv = fast_rcp(o0);
-                    const double ux = o1 * inv, uy = o2 * inv;
-                    const double ke = 0.5 * (o1 * ux + o2 * uy);
-                    double p = A.g1 * (o3 - ke);
-                    if (p <= 0.0) {
-                        atomicAdd(&A.diag->clamp_p, 1ull);
-                        o3 = 0.000001 / A.g1 + ke;
-                        p = A.g1 * (o3 - ke);
-                    }
-                    const double cs = sqrt_ratio(A.gamma * p, o0);
-                    const double w = (fabs(ux) + cs) * A.idx + (fabs(uy) + cs) * A.idy;
-                    wmax = fmax(wmax, w);
+                    const double bdt = A.b * dtdV;
+                    o0 = out_base(rk_base(A.a, ub[0], A.b, uc.a[0]), bdt, P.r);
+                    o1 = out_base(rk_base(A.a, ub[1], A.b, uc.a[1]), bdt, P.mx);
+                    o2 = out_base(rk_base(A.a, ub[2], A.b, uc.a[2]), bdt, P.my);
+                    o3 = out_base(rk_base(A.a, ub[3], A.b, uc.a[3]), bdt, P.e);
                 } else {
-                    // p <= 0 exactly when rho E - |m|^2 / 2 <= 0 (rho > 0): no division
-                    if (o3 * o0 - 0.5 * (o1 * o1 + o2 * o2) <= 0.0) {
-                        atomicAdd(&A.diag->clamp_p, 1ull);
-                        const double inv = fast_rcp(o0);
-                        const double ke = 0.5 * (o1 * (o1 * inv) + o2 * (o2 * inv));
-                        o3 = 0.000001 / A.g1 + ke;
-                    }
+                    o0 = out_plain(uc.a[0], dtdV, P.r);
+                    o1 = out_plain(uc.a[1], dtdV, P.mx);
+                    o2 = out_plain(uc.a[2], dtdV, P.my);
+                    o3 = out_plain(uc.a[3], dtdV, P.e);
                 }
+                if constexpr (WANT_W)
+                    clamp_wavespeed(A, o0, o1, o2, o3, true, wmax);
+                else
+                    clamp_plain(A, o0, o1, o2, o3, true);
+                double* ob = A.Uout + (long long)j * A.nx;
                 st_buf(buf_rsrc(uniform_ptr(ob)), oo, o0);
                 st_buf(buf_rsrc(uniform_ptr(ob + A.plane)), oo, o1);
                 st_buf(buf_rsrc(uniform_ptr(ob + 2 * A.plane)), oo, o2);
@@ -439,6 +489,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 const PrimE& q = v0.q;
                 const double gm1 = A.gamma - 1.0;
                 const double Tdv = dt / (dx * dy);
+                const double PSr = P.r, PUx = P.mx, PUy = P.my, PSe = P.e;
                 double r2 = q.r - Tdv * PSr;
                 if (r2 <= 0.0) {
                     atomicAdd(&A.diag->clamp_rho, 1ull);

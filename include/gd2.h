@@ -132,6 +132,17 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
               double* Uout, double a, double b, double dt, const double* dt_dev,
               unsigned long long* wmax_bits, void* stream);
 
+/* One SSP-RK2 step of the Euler scheme in a single pass, out of place:
+ *   U1 = Uin + dt L(Uin),  Uout = Uin/2 + (U1 + dt L(U1))/2
+ * with the intermediate state U1 kept on chip. Bitwise equal to
+ *   gd2_stage(Uin -> S, a=0, b=1); gd2_stage(S, Ubase=Uin -> Uout, a=b=1/2)
+ * (including the wave-speed fold into wmax_bits and the clamp counts).
+ * Uout must not alias Uin. Euler physics (HLLC or exact flux) only; other
+ * schemes return GD2_ERR_UNSUPPORTED. No reference counterpart (the reference
+ * steps with forward Euler); used by gd2_ctx_advance for SSP-RK2 Euler. */
+int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout,
+                    double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream);
+
 /* max over cells of (|u|+c)/dx + (|v|+c)/dy of U, folded into *wmax_bits
  * (device) as above; the caller zeroes *wmax_bits first. */
 int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U,

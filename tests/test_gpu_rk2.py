@@ -1,0 +1,127 @@
+"""GPU: the one-pass SSP-RK2 step (gd2_ssprk2_step) against two stage launches.
+
+The one-pass kernel keeps the intermediate state on chip; its contract is
+bitwise equality with stage(U -> S) followed by stage(S, base U -> U', a = b =
+1/2), including the wave-speed fold and the clamp counts. The stage kernels
+are checked against the oracle in test_gpu_euler.py; here the shapes are
+chosen so that strips of every height (the launch uses short strips on small
+grids), the first and last strips, columns split across waves and grids
+narrower than one wave all occur, on every boundary kind.
+"""
+import numpy as np
+import pytest
+import torch
+
+from golikov_2d_amd import api
+from golikov_2d_amd import problems as PB
+from oracle import oracle as O
+from tests.gpu_util import dev, grids, rel_l1, say, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+BCS = {
+    "transmissive": (O.BC_TRANSMISSIVE,) * 4,
+    "periodic": (O.BC_PERIODIC,) * 4,
+    "reflective": (O.BC_REFLECTIVE,) * 4,
+    "refl_x_per_y": (O.BC_REFLECTIVE, O.BC_TRANSMISSIVE, O.BC_PERIODIC, O.BC_PERIODIC),
+    "per_x_mixed_y": (O.BC_PERIODIC, O.BC_PERIODIC, O.BC_TRANSMISSIVE, O.BC_REFLECTIVE),
+    "mixed_y": (O.BC_TRANSMISSIVE, O.BC_REFLECTIVE, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE),
+}
+
+
+def two_stage(g, sc, U, dt):
+    S = torch.empty_like(U)
+    out = torch.empty_like(U)
+    w = torch.zeros(1, dtype=torch.int64, device=dev())
+    api.stage(g, sc, U, None, S, 0.0, 1.0, dt)
+    api.stage(g, sc, S, U, out, 0.5, 0.5, dt, wmax_bits=w)
+    return out, w
+
+
+def one_pass(g, sc, U, dt):
+    out = torch.empty_like(U)
+    w = torch.zeros(1, dtype=torch.int64, device=dev())
+    api.ssprk2_step(g, sc, U, out, dt, wmax_bits=w)
+    return out, w
+
+
+@pytest.mark.parametrize("shape", [(130, 97), (63, 64), (64, 63), (2, 5), (200, 3), (59, 2), (118, 41),
+                                   (60, 9), (177, 13)])
+@pytest.mark.parametrize("bc", list(BCS))
+@pytest.mark.parametrize("flux,order", [(api.FLUX_HLLC, 2), (api.FLUX_HLLC, 1), (api.FLUX_EXACT, 2),
+                                        (api.FLUX_EXACT, 1)])
+def test_one_pass_step_equals_two_stages(shape, bc, flux, order):
+    nx, ny = shape
+    say(f"one-pass step {nx}x{ny} bc={bc} flux={flux} order={order}")
+    g, og = grids(nx, ny, BCS[bc])
+    sc = api.make_scheme(api.EULER, flux, order, api.SSPRK2, 0.4)
+    U = to_dev(PB.four_quadrant(og, seed=nx * 7 + ny))
+    dt = 0.3 / (api.wavespeed_max(g, sc, U))
+    ref, wref = two_stage(g, sc, U, dt)
+    got, w = one_pass(g, sc, U, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref), float((got - ref).abs().max())
+    assert int(w.item()) == int(wref.item())
+
+
+def test_one_pass_step_matches_oracle():
+    say("one-pass step vs oracle 150x70")
+    g, og = grids(150, 70, BCS["refl_x_per_y"])
+    osc = O.Scheme(O.PHYS_EULER, O.FLUX_HLLC, 2, 0.4)
+    U0 = PB.four_quadrant(og, seed=21)
+    dt = 1e-3
+    U1 = O.stage(og, osc, U0, None, 0.0, 1.0, dt)
+    ref = O.stage(og, osc, U1, U0, 0.5, 0.5, dt)
+    got, _ = one_pass(g, api.make_scheme(), to_dev(U0), dt)
+    assert np.all(rel_l1(to_host(got), ref) <= 1e-12)
+
+
+def test_one_pass_clamp_counts_equal_two_stages():
+    """A near-vacuum expansion stepped with too large a dt fires the clamps;
+    both paths count every clamped cell once."""
+    say("clamp counts")
+    g, og = grids(150, 90, BCS["transmissive"])
+    sc = api.make_scheme(api.EULER, api.FLUX_HLLC, 2, api.SSPRK2, 0.4)
+    U = PB.uniform_euler(og, rho=1.0, p=1e-4, u=0.0, v=0.0)
+    U[1] = np.where(np.arange(150)[None, :] < 75, -2.0, 2.0) * U[0]   # rho u: pulled apart
+    U[3] = 1e-4 / 0.4 + 0.5 * U[1] ** 2 / U[0]
+    Ud = to_dev(U)
+    dt = 3.0 / api.wavespeed_max(g, sc, Ud)
+    api.diagnostics(reset=True)
+    ref, wref = two_stage(g, sc, Ud, dt)
+    d_ref = api.diagnostics(reset=True)
+    got, w = one_pass(g, sc, Ud, dt)
+    d = api.diagnostics(reset=True)
+    assert d_ref["clamp_rho"] + d_ref["clamp_p"] > 0
+    assert (d["clamp_rho"], d["clamp_p"]) == (d_ref["clamp_rho"], d_ref["clamp_p"])
+    assert torch.equal(got, ref) and int(w.item()) == int(wref.item())
+
+
+@pytest.mark.parametrize("nsteps", [1, 4, 7])
+def test_solver_context_one_pass_equals_two_pass(nsteps):
+    """gd2_ctx_advance ping-pongs one-pass steps between the state and its
+    scratch (copying back after an odd count): same state, clock and steps
+    as stage-by-stage stepping."""
+    say(f"ctx one-pass vs two-pass, {nsteps} steps")
+    g, og = grids(257, 130, BCS["transmissive"])
+    sc = api.make_scheme()
+    U0 = to_dev(PB.four_quadrant(og, seed=4))
+    # reference: explicit two-stage steps with the context's CFL rule (the
+    # first bound from the state, later ones from the previous step's fold)
+    Ur = U0.clone()
+    t = 0.0
+    wb = api.wavespeed_max(g, sc, Ur)
+    for _ in range(nsteps):
+        dt = sc.cfl / wb
+        Ur, w = two_stage(g, sc, Ur, dt)
+        wb = api.bits_to_double(w)
+        t += dt
+    s = api.Solver(g, sc)
+    U = U0.clone()
+    s.start(U)
+    s.advance(U, nsteps)
+    tg, ng, _ = s.time()
+    s.close()
+    assert ng == nsteps
+    assert torch.equal(U, Ur)
+    assert tg == t
