@@ -81,7 +81,7 @@ int tile_rows_for(int nx, int ny) {
 // rounds of (CUs x resident waves per CU); a round takes about as long as
 // one strip (its rows plus the rows re-read around it, `overhead`), so pick
 // the height minimising rounds x (rows + overhead). GD2_TILE_ROWS overrides.
-int strip_rows(const void* kernel, int tiles_x, int ny, int overhead) {
+int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows) {
     static int env = -1;
     if (env < 0) {
         const char* e = std::getenv("GD2_TILE_ROWS");
@@ -113,7 +113,7 @@ int strip_rows(const void* kernel, int tiles_x, int ny, int overhead) {
     const long long slots = (long long)cus * per_cu;
     int best = 16;
     long long best_cost = -1;
-    for (int rows = 4; rows <= 64; ++rows) {
+    for (int rows = 4; rows <= max_rows; ++rows) {
         const long long waves = (long long)tiles_x * ((ny + rows - 1) / rows);
         const long long cost = ((waves + slots - 1) / slots) * (rows + overhead);
         if (best_cost < 0 || cost <= best_cost) {  // ties: the taller strip (less halo)
@@ -171,13 +171,15 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
 }
 
 // One-pass SSP-RK2 step: strips of 59 columns; the strip height balances
-// rounds of resident waves against the four extra U1 rows each strip forms.
+// rounds of resident waves against the extra passes of each strip (four U1
+// rows below and above it and the lead-in, about three rows of work), up to
+// 256 rows so that a large grid fits in one round.
 template <int ORDER, int FLUX>
 static int launch_rk2(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
     A.tiles_x = (A.nx + kRk2W - 1) / kRk2W;
     const void* k = reinterpret_cast<const void*>(&euler_rk2_kernel<ORDER, FLUX>);
-    A.rows = max(2, strip_rows(k, A.tiles_x, A.ny, 5));  // strips start at row 0 or row >= 2
+    A.rows = max(2, strip_rows(k, A.tiles_x, A.ny, 3, 256));  // strips start at row 0 or row >= 2
     const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
     hipLaunchKernelGGL((euler_rk2_kernel<ORDER, FLUX>), dim3(blocks), dim3(kWave), 0, st, A);
     GD2_HIP(hipGetLastError());

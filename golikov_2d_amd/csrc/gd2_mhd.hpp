@@ -625,7 +625,7 @@ __global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, lon
 // ------------------------------------------------------------------ launchers
 
 int tile_rows_for(int nx, int ny);
-int strip_rows(const void* kernel, int tiles_x, int ny, int overhead);
+int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows = 64);
 int hip_error_code(hipError_t e);
 
 inline void fill_mhd(MhdArgs& A, const gd2_grid* g, DeviceDiag* diag) {
