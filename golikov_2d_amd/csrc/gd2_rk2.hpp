@@ -172,183 +172,121 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
     double wmax = 0.0;
     double smax;
 
-    // U1 rows fed to the consumer: j0-2 .. j1+1. Pass r forms U1 row r (when
-    // the producer owns it) and feeds it; the pass before r_first is the
-    // producer's lead-in.
-    for (int r = min(r_first - 1, j0 - 2); r <= j1 + 1; ++r) {
-        // ================= producer: U1 row r
-        PrimE q1;            // U1 row r, primitive
-        bool have = false;   // q1 holds a formed U1 row
-        if (r >= r_first - 1 && r <= r_last) {
-            const bool lead = r < r_first;
-            FluxE Fl, Fr;
-            Bal X;
-            double smx;
-            Raw uc;
-            if (!lead) {
-                // x-face F1(c-1/2) of row r from LDS; F1(c+1/2) from the next lane
-                const int sl = r & 1;
-                const PrimE qa = lds_get68(lds.q, sl, lane);
-                const PrimE qb = lds_get68(lds.q, sl, lane + 1);
-                const PrimE qd = lds_get68(lds.q, sl, lane + 3);
-                const PrimE& qc = pv0;
-                const Slope sb = slope3<ORDER>(qa, qb, qc);
-                const Slope sc = slope3<ORDER>(qb, qc, qd);
-                if ((phist & 2u) != 0u) {
-                    const PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
-                    Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smx);
-                } else {
-                    const PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
-                    Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smx);
-                }
-                Fr.fr = from_next_lane(Fl.fr);
-                Fr.fn = from_next_lane(Fl.fn);
-                Fr.ft = from_next_lane(Fl.ft);
-                Fr.fe = from_next_lane(Fl.fe);
-                X = x_balance(Fl, Fr, dy);
-#pragma unroll
-                for (int f = 0; f < 4; ++f) uc.a[f] = lds.u[r & 1][f][lane];
-            }
-            __syncthreads();  // slot r & 1 is read: row r+2 may take it
-            bool npos;
-            const PrimE pv2 = finish_row(pn, r & 1, true, npos);
-            phist = ((phist << 1) | (npos ? 1u : 0u)) & 15u;
-            pn = issue_row(r + 3);  // in flight during the consumer's pass
-            const Slope ps1 = slope3<ORDER>(pv0, pv1, pv2);
-            FluxE Gp;
-            if (phist == 15u) {
-                const PrimE L = face_plus<false>(pv0, ps0), R = face_minus<false>(pv1, ps1);
-                Gp = face_flux<FLUX, false>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smx);
-            } else {
-                const PrimE L = face_plus(pv0, ps0), R = face_minus(pv1, ps1);
-                Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smx);
-            }
-            if (!lead) {
-                const Bal P = full_balance(X, pGm, Gp, dx);
-                double o0 = out_plain(uc.a[0], dtdV, P.r);
-                double o1 = out_plain(uc.a[1], dtdV, P.mx);
-                double o2 = out_plain(uc.a[2], dtdV, P.my);
-                double o3 = out_plain(uc.a[3], dtdV, P.e);
-                // every U1 cell is counted once: by the wave that updates it
-                clamp_plain(A, o0, o1, o2, o3, owns && r >= j0 && r < j1);
-                q1 = cons_to_prim_e(o0, o1, o2, o3, A.g1);
-                const int hs = (r + 3) % 3;
-                lds.h[hs][0][lane] = rk_base(A.a, uc.a[0], A.b, o0);
-                lds.h[hs][1][lane] = rk_base(A.a, uc.a[1], A.b, o1);
-                lds.h[hs][2][lane] = rk_base(A.a, uc.a[2], A.b, o2);
-                lds.h[hs][3][lane] = rk_base(A.a, uc.a[3], A.b, o3);
-                have = true;
-            }
-            pv0 = pv1;
-            pv1 = pv2;
-            ps0 = ps1;
-            pGm = Gp;
-        }
-
-        // ================= consumer: feed U1 row r
-        const int k = r - (j0 - 2);
-        if (k < 0) continue;
-        bool real = have;
-        if (!have) {
-            // a row outside the grid on a non-periodic side
-            if (r >= A.ny) {
-                // top: row ny is row ny-1 (the newest row held), row ny+1 is
-                // row ny-1 (transmissive) or row ny-2 (reflective, from LDS)
-                if (r == A.ny) {
-                    q1 = cw1;
-                } else if (!refl_hi) {
-                    q1 = cw0;
-                } else {
-                    bool f;
-                    const int src = map_index(r, A.ny, A.bc[2], A.bc[3], f);
-                    q1 = lds_get68(lds.w, src & 1, lane + 2);
-                }
-                if (refl_hi) q1.v = -q1.v;
-            } else {
-                // bottom rows -2, -1: formed once rows 0 and 1 exist (k = 2, 3)
-                if (k < 2) continue;
-            }
-        }
-        // x-face of row j = r - 2 (passes k >= 4): LDS slot j & 1 is read
-        // before row r takes it
-        const int j = r - 2;
-        const bool upd = k >= 4;
-        Bal X2;
-        double hj[4];
-        if (upd) {
-            const int sl = j & 1;
-            const PrimE qa = lds_get68(lds.w, sl, lane);
-            const PrimE qb = lds_get68(lds.w, sl, lane + 1);
-            const PrimE qc = lds_get68(lds.w, sl, lane + 2);
-            const PrimE qd = lds_get68(lds.w, sl, lane + 3);
+    // ================= producer pass r: x-face F1 of row r (unless `lead`),
+    // the next U^n row into LDS, y-face G1(r+1/2), and U1 row r (unless `lead`)
+    // with its H = a U^n + b U1 in LDS
+    auto produce = [&](int r, bool lead, PrimE& q1) {
+        FluxE Fl, Fr;
+        Bal X;
+        double smx;
+        Raw uc;
+        if (!lead) {
+            // x-face F1(c-1/2) of row r from LDS; F1(c+1/2) from the next lane
+            const int sl = r & 1;
+            const PrimE qa = lds_get68(lds.q, sl, lane);
+            const PrimE qb = lds_get68(lds.q, sl, lane + 1);
+            const PrimE qd = lds_get68(lds.q, sl, lane + 3);
+            const PrimE& qc = pv0;
             const Slope sb = slope3<ORDER>(qa, qb, qc);
             const Slope sc = slope3<ORDER>(qb, qc, qd);
-            FluxE Fl, Fr;
-            if ((chist & 2u) != 0u) {
+            if ((phist & 2u) != 0u) {
                 const PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
-                Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+                Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smx);
             } else {
                 const PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
-                Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+                Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smx);
             }
             Fr.fr = from_next_lane(Fl.fr);
             Fr.fn = from_next_lane(Fl.fn);
             Fr.ft = from_next_lane(Fl.ft);
             Fr.fe = from_next_lane(Fl.fe);
-            X2 = x_balance(Fl, Fr, dy);
-            const int hs = (j + 3) % 3;
+            X = x_balance(Fl, Fr, dy);
 #pragma unroll
-            for (int f = 0; f < 4; ++f) hj[f] = lds.h[hs][f][lane];
+            for (int f = 0; f < 4; ++f) uc.a[f] = lds.u[r & 1][f][lane];
         }
-        __syncthreads();
-        if (real) {
-            lds_put68(lds.w, r & 1, lane + 2, q1);
-            if (wave_xfix) {
-                // columns outside the grid take the (mirrored) value of their
-                // source column, in LDS and in the lane's own registers
-                __syncthreads();
-                if (do_xfix) {
-                    PrimE sv = lds_get68(lds.w, r & 1, gsrc_e);
-                    if (gfx) sv.u = -sv.u;
-                    q1 = sv;
-                }
-                __syncthreads();
-                if (do_xfix) lds_put68(lds.w, r & 1, lane + 2, q1);
-            }
+        __syncthreads();  // slot r & 1 is read: row r+2 may take it
+        bool npos;
+        const PrimE pv2 = finish_row(pn, r & 1, true, npos);
+        phist = ((phist << 1) | (npos ? 1u : 0u)) & 15u;
+        pn = issue_row(r + 3);  // in flight during the rest of the pass
+        const Slope ps1 = slope3<ORDER>(pv0, pv1, pv2);
+        FluxE Gp;
+        if (phist == 15u) {
+            const PrimE L = face_plus<false>(pv0, ps0), R = face_minus<false>(pv1, ps1);
+            Gp = face_flux<FLUX, false>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smx);
+        } else {
+            const PrimE L = face_plus(pv0, ps0), R = face_minus(pv1, ps1);
+            Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smx);
         }
-        const bool qpos = __all(!in_u1 || prim_pos(q1)) != 0;
+        if (!lead) {
+            const Bal P = full_balance(X, pGm, Gp, dx);
+            double o0 = out_plain(uc.a[0], dtdV, P.r);
+            double o1 = out_plain(uc.a[1], dtdV, P.mx);
+            double o2 = out_plain(uc.a[2], dtdV, P.my);
+            double o3 = out_plain(uc.a[3], dtdV, P.e);
+            // every U1 cell is counted once: by the wave that updates it
+            clamp_plain(A, o0, o1, o2, o3, owns && r >= j0 && r < j1);
+            q1 = cons_to_prim_e(o0, o1, o2, o3, A.g1);
+            const int hs = (r + 3) % 3;
+            lds.h[hs][0][lane] = rk_base(A.a, uc.a[0], A.b, o0);
+            lds.h[hs][1][lane] = rk_base(A.a, uc.a[1], A.b, o1);
+            lds.h[hs][2][lane] = rk_base(A.a, uc.a[2], A.b, o2);
+            lds.h[hs][3][lane] = rk_base(A.a, uc.a[3], A.b, o3);
+        }
+        pv0 = pv1;
+        pv1 = pv2;
+        ps0 = ps1;
+        pGm = Gp;
+    };
 
-        if (k < 2) {  // rows j0-2, j0-1
-            if (k == 0)
-                cw0 = q1;
-            else
-                cw1 = q1;
-            chist = ((chist << 1) | (qpos ? 1u : 0u)) & 15u;
-            continue;
+    // ================= consumer x-face F2 of row j (LDS slot j & 1) and H_j
+    auto consume_xface = [&](int j, Bal& X2, double (&hj)[4]) {
+        const int sl = j & 1;
+        const PrimE qa = lds_get68(lds.w, sl, lane);
+        const PrimE qb = lds_get68(lds.w, sl, lane + 1);
+        const PrimE qc = lds_get68(lds.w, sl, lane + 2);
+        const PrimE qd = lds_get68(lds.w, sl, lane + 3);
+        const Slope sb = slope3<ORDER>(qa, qb, qc);
+        const Slope sc = slope3<ORDER>(qb, qc, qd);
+        FluxE Fl, Fr;
+        if ((chist & 2u) != 0u) {
+            const PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
+            Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
+        } else {
+            const PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
+            Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
         }
-        if (k == 2) {  // row j0
-            if (ghost_lo) {
-                // row -1 is row 0 (mirrored v when reflective); the slope of
-                // row -1 waits for row 1 (reflective row -2 is row 1)
-                cw0 = q1;
-                if (refl_lo) cw0.v = -cw0.v;
-                cw1 = q1;
-                chist = qpos ? 3u : 0u;
-            } else {
-                cs0 = slope3<ORDER>(cw0, cw1, q1);
-                cw0 = cw1;
-                cw1 = q1;
-                chist = ((chist << 1) | (qpos ? 1u : 0u)) & 15u;
-            }
-            continue;
-        }
+        Fr.fr = from_next_lane(Fl.fr);
+        Fr.fn = from_next_lane(Fl.fn);
+        Fr.ft = from_next_lane(Fl.ft);
+        Fr.fe = from_next_lane(Fl.fe);
+        X2 = x_balance(Fl, Fr, dy);
+        const int hs = (j + 3) % 3;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) hj[f] = lds.h[hs][f][lane];
+    };
 
-        // k >= 3: y-face G2(j+1/2); k == 3 is the lead-in (G2(j0-1/2) only)
-        if (k == 3 && ghost_lo) {
-            PrimE m2 = refl_lo ? q1 : cw1;  // row -2: row 1 mirrored, or row 0
-            if (refl_lo) m2.v = -m2.v;
-            cs0 = slope3<ORDER>(m2, cw0, cw1);
+    // ================= consumer: publish U1 row r in LDS slot r & 1 (after
+    // the x-face of row r-2 read that slot); columns outside the grid take the
+    // (mirrored) value of their source column, in LDS and in the lane's own
+    // registers
+    auto publish = [&](int r, PrimE& q1) {
+        lds_put68(lds.w, r & 1, lane + 2, q1);
+        if (wave_xfix) {
+            __syncthreads();
+            if (do_xfix) {
+                PrimE sv = lds_get68(lds.w, r & 1, gsrc_e);
+                if (gfx) sv.u = -sv.u;
+                q1 = sv;
+            }
+            __syncthreads();
+            if (do_xfix) lds_put68(lds.w, r & 1, lane + 2, q1);
         }
+    };
+
+    // ================= consumer y-face G2(j+1/2) from rows j-1 .. j+2 (q1 =
+    // row j+2), the update of row j when `upd`, and the window advance
+    auto consume_y = [&](int j, const PrimE& q1, bool qpos, bool upd, const Bal& X2, const double (&hj)[4]) {
         chist = ((chist << 1) | (qpos ? 1u : 0u)) & 15u;
         const Slope cs1 = slope3<ORDER>(cw0, cw1, q1);
         FluxE Gp;
@@ -376,6 +314,104 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         cw1 = q1;
         cs0 = cs1;
         cGm = Gp;
+    };
+
+    // ================= a pass of any kind: U1 rows fed to the consumer are
+    // j0-2 .. j1+1; pass r forms U1 row r when the producer owns it and feeds
+    // it; the pass before r_first is the producer's lead-in
+    auto pass_general = [&](int r) {
+        PrimE q1;           // U1 row r, primitive
+        bool have = false;  // q1 holds a formed U1 row
+        if (r >= r_first - 1 && r <= r_last) {
+            const bool lead = r < r_first;
+            produce(r, lead, q1);
+            have = !lead;
+        }
+        const int k = r - (j0 - 2);
+        if (k < 0) return;
+        if (!have) {
+            // a row outside the grid on a non-periodic side
+            if (r >= A.ny) {
+                // top: row ny is row ny-1 (the newest row held), row ny+1 is
+                // row ny-1 (transmissive) or row ny-2 (reflective, from LDS)
+                if (r == A.ny) {
+                    q1 = cw1;
+                } else if (!refl_hi) {
+                    q1 = cw0;
+                } else {
+                    bool f;
+                    const int src = map_index(r, A.ny, A.bc[2], A.bc[3], f);
+                    q1 = lds_get68(lds.w, src & 1, lane + 2);
+                }
+                if (refl_hi) q1.v = -q1.v;
+            } else if (k < 2) {
+                return;  // bottom rows -2, -1: formed once rows 0 and 1 exist (k = 2, 3)
+            }
+        }
+        const int j = r - 2;
+        const bool upd = k >= 4;
+        Bal X2;
+        double hj[4];
+        if (upd) consume_xface(j, X2, hj);
+        __syncthreads();
+        if (have) publish(r, q1);
+        const bool qpos = __all(!in_u1 || prim_pos(q1)) != 0;
+        if (k < 2) {  // rows j0-2, j0-1
+            if (k == 0)
+                cw0 = q1;
+            else
+                cw1 = q1;
+            chist = ((chist << 1) | (qpos ? 1u : 0u)) & 15u;
+            return;
+        }
+        if (k == 2) {  // row j0
+            if (ghost_lo) {
+                // row -1 is row 0 (mirrored v when reflective); the slope of
+                // row -1 waits for row 1 (reflective row -2 is row 1)
+                cw0 = q1;
+                if (refl_lo) cw0.v = -cw0.v;
+                cw1 = q1;
+                chist = qpos ? 3u : 0u;
+            } else {
+                cs0 = slope3<ORDER>(cw0, cw1, q1);
+                cw0 = cw1;
+                cw1 = q1;
+                chist = ((chist << 1) | (qpos ? 1u : 0u)) & 15u;
+            }
+            return;
+        }
+        if (k == 3 && ghost_lo) {
+            PrimE m2 = refl_lo ? q1 : cw1;  // row -2: row 1 mirrored, or row 0
+            if (refl_lo) m2.v = -m2.v;
+            cs0 = slope3<ORDER>(m2, cw0, cw1);
+        }
+        consume_y(j, q1, qpos, upd, X2, hj);  // k == 3: the lead-in, G2(j0-1/2) only
+    };
+
+    // ================= steady pass r: the producer forms U1 row r inside the
+    // grid and the consumer updates row r - 2
+    auto pass_steady = [&](int r) {
+        PrimE q1;
+        produce(r, false, q1);
+        Bal X2;
+        double hj[4];
+        consume_xface(r - 2, X2, hj);
+        __syncthreads();
+        publish(r, q1);
+        const bool qpos = __all(!in_u1 || prim_pos(q1)) != 0;
+        consume_y(r - 2, q1, qpos, true, X2, hj);
+    };
+
+    const int s_begin = j0 + 2;            // first pass that updates a row
+    const int s_end = min(r_last, j1 + 1);  // last pass whose U1 row the producer forms
+    int r = min(r_first - 1, j0 - 2);
+    while (r <= j1 + 1) {
+        if (r == s_begin && s_begin <= s_end) {
+            for (; r <= s_end; ++r) pass_steady(r);
+            continue;
+        }
+        pass_general(r);
+        ++r;
     }
 
     const double w = wave_max(wmax);
