@@ -402,17 +402,34 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         consume_y(r - 2, q1, qpos, true, X2, hj);
     };
 
+    // Two waves share a SIMD and the older one wins issue ties, so it would
+    // finish far ahead and leave the other to run alone at half the issue
+    // rate. The waves trade the higher issue priority every pass instead
+    // (odd/even hardware wave slot, HW_ID bit 0), so both stay busy to the end.
+    const unsigned wslot = __builtin_amdgcn_s_getreg(4) & 1u;
+    auto set_turn = [&](int pass) {
+        if (((unsigned)pass ^ wslot) & 1u)
+            __builtin_amdgcn_s_setprio(1);
+        else
+            __builtin_amdgcn_s_setprio(0);
+    };
+
     const int s_begin = j0 + 2;            // first pass that updates a row
     const int s_end = min(r_last, j1 + 1);  // last pass whose U1 row the producer forms
     int r = min(r_first - 1, j0 - 2);
     while (r <= j1 + 1) {
         if (r == s_begin && s_begin <= s_end) {
-            for (; r <= s_end; ++r) pass_steady(r);
+            for (; r <= s_end; ++r) {
+                set_turn(r);
+                pass_steady(r);
+            }
             continue;
         }
+        set_turn(r);
         pass_general(r);
         ++r;
     }
+    __builtin_amdgcn_s_setprio(0);
 
     const double w = wave_max(wmax);
     if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
