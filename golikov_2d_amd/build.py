@@ -47,9 +47,10 @@ def build_torch_module(verbose: bool = False) -> str:
            "-DTORCH_EXTENSION_NAME=gd2_torch", "-DTORCH_API_INCLUDE_EXTENSION_H",
            f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}",
            *[f"-I{p}" for p in ce.include_paths()], f"-I{sysconfig.get_paths()['include']}",
+           "-D__HIP_PLATFORM_AMD__=1", "-I", "/opt/rocm/include",
            "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, TORCH_SRC),
            f"-L{tlib}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-Wl,-rpath,{tlib}",
-           f"-L{HERE}", "-lgd2", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+           f"-L{HERE}", "-lgd2", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

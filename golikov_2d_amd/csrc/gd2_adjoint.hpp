@@ -199,6 +199,11 @@ struct AdjArgs {
     int bc[4];
     int tiles_x, rows;
     int gstride;        // max(nx, ny)
+    // 1: the kernel adds the ghost-cell cotangents of transmissive and
+    // reflective sides into the cells they copy itself (every such cell lies
+    // in the same wave or strip as its ghosts); 0: they go to the side buffer
+    // for adj_fold_kernel
+    int fold_in_kernel;
 };
 
 struct Q4 {
@@ -299,9 +304,28 @@ __device__ __forceinline__ void ghost_store(const AdjArgs& A, int side, int laye
     g[3] = d.p;
 }
 
+// x of lane `src` (wave-uniform) in every lane.
+__device__ __forceinline__ double lane_value(double x, int src) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ Q4 q4_lane(const Q4& x, int src) {
+    Q4 y = {lane_value(x.r, src), lane_value(x.u, src), lane_value(x.v, src), lane_value(x.p, src)};
+    return y;
+}
+
+__device__ __forceinline__ Q4 q4_flip_u(const Q4& x, bool f) {
+    Q4 y = {x.r, f ? -x.u : x.u, x.v, x.p};
+    return y;
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     __shared__ double ring[3][8][kAdjBuf];
+    __shared__ double gyb[2][4][kWave];  // ghost-row cotangents (layer 0 next to the edge)
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
@@ -325,6 +349,19 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     // the strip's output rows include the ghost rows on the domain's edges
     const int ro0 = (j0 == 0) ? -2 : j0;
     const int ro1 = (j1 == ny) ? ny + 2 : j1;
+
+    // in-kernel folding: the cells each ghost layer copies (layer 0 is next
+    // to the edge) and whether the copy mirrors the normal velocity
+    int tgt_w[2], tgt_e[2], tgt_lo[2], tgt_hi[2];
+    bool flip_w[2], flip_e[2], flip_lo[2], flip_hi[2];
+    for (int layer = 0; layer < 2; ++layer) {
+        tgt_w[layer] = map_index(-1 - layer, nx, A.bc[0], A.bc[1], flip_w[layer]);
+        tgt_e[layer] = map_index(nx + layer, nx, A.bc[0], A.bc[1], flip_e[layer]);
+        tgt_lo[layer] = map_index(-1 - layer, ny, A.bc[2], A.bc[3], flip_lo[layer]);
+        tgt_hi[layer] = map_index(ny + layer, ny, A.bc[2], A.bc[3], flip_hi[layer]);
+    }
+    const bool fold_lo = A.fold_in_kernel && j0 == 0;
+    const bool fold_hi = A.fold_in_kernel && j1 == ny;
 
     const unsigned oc = (unsigned)ci * 8u, oh = (unsigned)hci * 8u;
     // pos: every loaded cell of the row (and halo) has rho, p > kPosMin (wave-uniform)
@@ -385,7 +422,15 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     Q4 acc_prev = q4_zero();  // row j-1: everything but the spill from row j
     Q4 rsp_prev = q4_zero();  // spill of row j-1 to row j
 
+    // two waves share a SIMD: they trade the higher issue priority every pass
+    // (odd/even hardware wave slot, HW_ID bit 0) instead of letting the older
+    // one finish first and leave the other to run alone
+    const unsigned wslot = __builtin_amdgcn_s_getreg(4) & 1u;
     for (int j = ro0 - 2; j <= ro1; ++j) {
+        if (((unsigned)j ^ wslot) & 1u)
+            __builtin_amdgcn_s_setprio(1);
+        else
+            __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         bool pd;
         qd = load(j + 2, ((j + 2) % 3 + 3) % 3, true, Vd, pd);  // q_{j+2}
@@ -420,9 +465,19 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
         // finish row r = j-1: its accumulated parts + the spill from row j
         const int r = j - 1;
         if (r >= ro0 && r < ro1) {
-            const Q4 tot = q4_add(acc_prev, lsp_j);
+            Q4 tot = q4_add(acc_prev, lsp_j);
             if (r >= 0 && r < ny) {
                 if (col_real && interior_lane) {
+                    if (fold_lo && r <= 1) {
+                        // rows -1, -2 copy rows map(-1), map(-2) (mirrored v when reflective)
+                        for (int layer = 0; layer < 2; ++layer) {
+                            if (tgt_lo[layer] != r) continue;
+                            Q4 e = {gyb[layer][0][lane], gyb[layer][1][lane], gyb[layer][2][lane],
+                                    gyb[layer][3][lane]};
+                            if (flip_lo[layer]) e.v = -e.v;
+                            tot = q4_add(tot, e);
+                        }
+                    }
                     double du[4];
                     cons_vjp(qa.r, qa.u, qa.v, A.g1, tot, du);  // qa = prims of row j-1
                     const long long rowk = (long long)r * nx;
@@ -437,10 +492,19 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                 }
             } else if (col_real && interior_lane) {
                 // ghost rows: y contributions only
-                if (r == -1) ghost_store(A, 2, 0, c, tot);
-                if (r == -2) ghost_store(A, 2, 1, c, tot);
-                if (r == ny) ghost_store(A, 3, 0, c, tot);
-                if (r == ny + 1) ghost_store(A, 3, 1, c, tot);
+                if (A.fold_in_kernel) {
+                    // kept for the rows they copy (rows 0/1 below, ny-1/ny-2 after the march)
+                    const int layer = (r == -1 || r == ny) ? 0 : 1;
+                    gyb[layer][0][lane] = tot.r;
+                    gyb[layer][1][lane] = tot.u;
+                    gyb[layer][2][lane] = tot.v;
+                    gyb[layer][3][lane] = tot.p;
+                } else {
+                    if (r == -1) ghost_store(A, 2, 0, c, tot);
+                    if (r == -2) ghost_store(A, 2, 1, c, tot);
+                    if (r == ny) ghost_store(A, 3, 0, c, tot);
+                    if (r == ny + 1) ghost_store(A, 3, 1, c, tot);
+                }
             }
         }
         // row j so far: own y part + spill from row j-1
@@ -472,17 +536,34 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             spread_cell<ORDER, G>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
             const Q4 from_next = q4_from_next(lsp_c);
             const Q4 from_prev = q4_from_prev(rsp_c);
-            const Q4 x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            Q4 x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            if (A.fold_in_kernel) {
+                // ghost columns -1, -2 (lane 0: its own part and its spill to
+                // column -2) and nx, nx+1 go to the columns they copy, with u
+                // mirrored when reflective
+                if (tx == 0) {
+                    const Q4 g1 = q4_lane(x_j, 0), g2 = q4_lane(lsp_c, 0);
+                    if (c == tgt_w[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_w[0]));
+                    if (c == tgt_w[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_w[1]));
+                }
+                if (tx == A.tiles_x - 1) {
+                    const int le = nx - cbase + 1;  // lane of column nx
+                    const Q4 g1 = q4_lane(x_j, le), g2 = q4_lane(x_j, le + 1);
+                    if (c == tgt_e[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_e[0]));
+                    if (c == tgt_e[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_e[1]));
+                }
+            } else {
+                // ghost columns of the domain edges (east ones from the last tile only)
+                if (c == -1) {
+                    ghost_store(A, 0, 0, j, x_j);
+                    ghost_store(A, 0, 1, j, lsp_c);
+                }
+                if (tx == A.tiles_x - 1) {
+                    if (c == nx) ghost_store(A, 1, 0, j, x_j);
+                    if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
+                }
+            }
             acc = q4_add(acc, x_j);
-            // ghost columns of the domain edges (east ones from the last tile only)
-            if (c == -1) {
-                ghost_store(A, 0, 0, j, x_j);
-                ghost_store(A, 0, 1, j, lsp_c);
-            }
-            if (tx == A.tiles_x - 1) {
-                if (c == nx) ghost_store(A, 1, 0, j, x_j);
-                if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
-            }
         };
         if (j >= j0 && j < j1) {
             if (pos_hist & 4u)
@@ -500,6 +581,40 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
         qc = qd;
         Vb[0] = Vc[0]; Vb[1] = Vc[1]; Vb[2] = Vc[2]; Vb[3] = Vc[3];
         Vc[0] = Vn[0]; Vc[1] = Vn[1]; Vc[2] = Vn[2]; Vc[3] = Vn[3];
+    }
+    __builtin_amdgcn_s_setprio(0);
+
+    if (fold_hi && col_real && interior_lane) {
+        // rows ny, ny+1 were finished after the rows they copy: add their
+        // cotangents to W of those rows (stored by this wave: wait for the
+        // stores, then read W back past the L1)
+        __builtin_amdgcn_s_waitcnt(0);
+        for (int t = ny - 1; t >= ny - 2 && t >= 0; --t) {
+            Q4 d = q4_zero();
+            bool any = false;
+            for (int layer = 0; layer < 2; ++layer) {
+                if (tgt_hi[layer] != t) continue;
+                Q4 e = {gyb[layer][0][lane], gyb[layer][1][lane], gyb[layer][2][lane], gyb[layer][3][lane]};
+                if (flip_hi[layer]) e.v = -e.v;
+                d = q4_add(d, e);
+                any = true;
+            }
+            if (!any) continue;
+            const long long rowk = (long long)t * nx;
+            const unsigned o = (unsigned)c * 8u;
+            double u4[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) u4[f] = ld_buf(buf_rsrc(uniform_ptr(A.U + f * P + rowk)), o);
+            const PrimE q = cons_to_prim_e(u4[0], u4[1], u4[2], u4[3], A.g1);
+            double du[4];
+            cons_vjp(q.r, q.u, q.v, A.g1, d, du);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const __amdgpu_buffer_rsrc_t rw = buf_rsrc(uniform_ptr(A.W + f * P + rowk));
+                const double w0 = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rw, o, 0, 1));
+                st_buf(rw, o, w0 + A.c * du[f]);
+            }
+        }
     }
 }
 

@@ -508,6 +508,17 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
                                        : reinterpret_cast<const void*>(&euler_vjp_kernel<1>),
                         J.tiles_x, g->ny, 3);
     J.gstride = g->nx > g->ny ? g->nx : g->ny;
+    // ghost cotangents are folded inside the kernel when every ghost copies a
+    // cell of its own wave or strip: no periodic side, and the first / last
+    // column tile and row strip hold at least two columns / rows
+    {
+        bool in_kernel = g->nx >= 2 && g->ny >= 2 && J.rows >= 2;
+        for (int s2 = 0; s2 < 4; ++s2) in_kernel = in_kernel && g->bc[s2] != GD2_BC_PERIODIC;
+        in_kernel = in_kernel && g->nx - (J.tiles_x - 1) * kAdjW >= 2;
+        const int nstrips = (g->ny + J.rows - 1) / J.rows;
+        in_kernel = in_kernel && g->ny - (nstrips - 1) * J.rows >= 2;
+        J.fold_in_kernel = in_kernel ? 1 : 0;
+    }
     hipStream_t st = (hipStream_t)stream;
     const unsigned blocks = (unsigned)(J.tiles_x * ((g->ny + J.rows - 1) / J.rows));
     if (sc->order == 2)
@@ -515,9 +526,11 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
     else
         hipLaunchKernelGGL(euler_vjp_kernel<1>, dim3(blocks), dim3(kWave), 0, st, J);
     GD2_HIP(hipGetLastError());
-    const long long nfold = 4ll * g->nx + 4ll * g->ny;
-    hipLaunchKernelGGL(adj_fold_kernel, dim3((unsigned)((nfold + 255) / 256)), dim3(256), 0, st, J);
-    GD2_HIP(hipGetLastError());
+    if (!J.fold_in_kernel) {
+        const long long nfold = 4ll * g->nx + 4ll * g->ny;
+        hipLaunchKernelGGL(adj_fold_kernel, dim3((unsigned)((nfold + 255) / 256)), dim3(256), 0, st, J);
+        GD2_HIP(hipGetLastError());
+    }
     return GD2_OK;
 }
 

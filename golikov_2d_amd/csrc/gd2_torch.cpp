@@ -13,6 +13,7 @@
 // those of the map with the schedule held fixed.
 //
 // No reference counterpart (the reference has no adjoint).
+#include <hip/hip_runtime_api.h>
 #include <torch/extension.h>
 
 #include <string>
@@ -106,30 +107,81 @@ Tensor run_forward(const Setup& s, const Tensor& U0, const std::vector<double>& 
     return U;
 }
 
+void hip_check(hipError_t e, const char* what) { TORCH_CHECK(e == hipSuccess, what, ": ", hipGetErrorString(e)); }
+
+// A second stream (default priority) and its events, created once per
+// process: the backward sweep recomputes the next segment's states on it
+// while the caller's stream runs the current segment's adjoint.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, rec[2] = {nullptr, nullptr}, vjp[2] = {nullptr, nullptr};
+    SideStream() {
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        hip_check(hipEventCreateWithFlags(&start, hipEventDisableTiming), "hipEventCreateWithFlags");
+        for (int k = 0; k < 2; ++k) {
+            hip_check(hipEventCreateWithFlags(&rec[k], hipEventDisableTiming), "hipEventCreateWithFlags");
+            hip_check(hipEventCreateWithFlags(&vjp[k], hipEventDisableTiming), "hipEventCreateWithFlags");
+        }
+    }
+};
+
+SideStream& side_stream() {
+    static SideStream s;
+    return s;
+}
+
 // Reverse sweep from the checkpoints: returns (d U_T / d U_0)^T lamT.
+// Segment seg's states U^n, U^(1) are recomputed from its checkpoint into one
+// of two buffer sets on the side stream; the adjoint of segment seg (caller's
+// stream) overlaps the recompute of segment seg-1 into the other set.
 Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::vector<double>& dt, int64_t every,
                     const Tensor& lamT, void* st) {
     const int64_t n = static_cast<int64_t>(dt.size());
+    const int64_t nseg = static_cast<int64_t>(ckpt.size());
     Tensor lam = lamT.to(torch::kFloat64).contiguous().clone();
     Tensor lam1 = torch::empty_like(lam);
     const long long wsb = gd2_vjp_workspace_bytes(&s.g);
     TORCH_CHECK(wsb > 0, "gd2_vjp_workspace_bytes failed");
     Tensor ws = torch::empty({(wsb + 7) / 8}, lam.options());
-    std::vector<Tensor> Un, U1;
-    for (int64_t seg = static_cast<int64_t>(ckpt.size()) - 1; seg >= 0; --seg) {
-        const int64_t k0 = seg * every, k1 = std::min(n, k0 + every);
-        const int64_t m = k1 - k0;
-        while (static_cast<int64_t>(Un.size()) < m) {
-            Un.push_back(torch::empty_like(lam));
-            U1.push_back(torch::empty_like(lam));
+    const int64_t seg_len = std::min<int64_t>(every, n);
+    std::vector<Tensor> Un[2], U1[2];
+    for (int k = 0; k < 2 && k < nseg; ++k)
+        for (int64_t i = 0; i < seg_len; ++i) {
+            Un[k].push_back(torch::empty_like(lam));
+            U1[k].push_back(torch::empty_like(lam));
         }
-        Un[0].copy_(ckpt[seg]);
+    SideStream& ss = side_stream();
+    hipStream_t main = reinterpret_cast<hipStream_t>(st);
+    // the checkpoints and lamT come from work on the caller's stream
+    hip_check(hipEventRecord(ss.start, main), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(ss.stream, ss.start, 0), "hipStreamWaitEvent");
+    const size_t bytes = static_cast<size_t>(lam.numel()) * sizeof(double);
+    auto recompute = [&](int64_t seg) {
+        const int set = static_cast<int>(seg & 1);
+        const int64_t k0 = seg * every, m = std::min(n, k0 + every) - k0;
+        hip_check(hipMemcpyAsync(dp(Un[set][0]), dp(ckpt[seg]), bytes, hipMemcpyDeviceToDevice, ss.stream),
+                  "hipMemcpyAsync");
         for (int64_t i = 0; i < m; ++i) {
-            stage1(s, Un[i], U1[i], dt[k0 + i], st);
-            if (i + 1 < m) stage2(s, U1[i], Un[i], Un[i + 1], dt[k0 + i], st);
+            stage1(s, Un[set][i], U1[set][i], dt[k0 + i], ss.stream);
+            if (i + 1 < m) stage2(s, U1[set][i], Un[set][i], Un[set][i + 1], dt[k0 + i], ss.stream);
         }
-        for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, Un[i], U1[i], lam, lam1, ws, dt[k0 + i], st);
+        hip_check(hipEventRecord(ss.rec[set], ss.stream), "hipEventRecord");
+    };
+    if (nseg > 0) recompute(nseg - 1);
+    for (int64_t seg = nseg - 1; seg >= 0; --seg) {
+        const int set = static_cast<int>(seg & 1);
+        if (seg >= 1) {
+            // segment seg-1 reuses the buffers of segment seg+1: after its adjoint
+            if (seg + 1 < nseg) hip_check(hipStreamWaitEvent(ss.stream, ss.vjp[set ^ 1], 0), "hipStreamWaitEvent");
+            recompute(seg - 1);
+        }
+        hip_check(hipStreamWaitEvent(main, ss.rec[set], 0), "hipStreamWaitEvent");
+        const int64_t k0 = seg * every, m = std::min(n, k0 + every) - k0;
+        for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, Un[set][i], U1[set][i], lam, lam1, ws, dt[k0 + i], st);
+        hip_check(hipEventRecord(ss.vjp[set], main), "hipEventRecord");
     }
+    // every side-stream launch was waited for by the caller's stream above, so
+    // the buffers may be released in the caller's stream order
     return lam;
 }
 

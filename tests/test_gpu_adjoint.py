@@ -66,6 +66,10 @@ CASES = [
     (61, 7, (R, R, T, T), 2, "smooth"),
     (3, 5, (T, R, P, P), 2, "smooth"),
     (128, 128, (T, T, T, T), 2, "smooth"),
+    # strips of at least two rows at both ends: ghosts folded inside the kernel
+    (70, 46, (R, R, R, R), 2, "smooth"),
+    (37, 130, (R, T, T, R), 2, "smooth"),
+    (62, 42, (T, R, R, T), 1, "quadrants"),
 ]
 
 
