@@ -363,58 +363,73 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
     const bool fold_lo = A.fold_in_kernel && j0 == 0;
     const bool fold_hi = A.fold_in_kernel && j1 == ny;
 
-    const unsigned oc = (unsigned)ci * 8u, oh = (unsigned)hci * 8u;
-    // pos: every loaded cell of the row (and halo) has rho, p > kPosMin (wave-uniform)
-    auto load = [&](int jl, int slot, bool to_lds, double Vv[4], bool& pos) -> PrimE {
+    const unsigned oc = (unsigned)ci * 8u;
+    const unsigned oh = has_halo ? (unsigned)hci * 8u : kOffNone;
+    // ghost cells have no output in the forward map: their cotangent is 0
+    // (an out-of-range offset loads 0)
+    const unsigned ocv = col_real ? oc : kOffNone;
+    const unsigned ohv = (has_halo && hcol >= 0 && hcol < nx) ? oh : kOffNone;
+
+    // raw U and V values of one logical row (own cell and, lanes 0-2, the halo
+    // cell); issued by the whole wave without branches
+    struct RawRow {
+        double u[4], uh[4], v[4], vh[4];
         bool fy;
-        const int jj = map_index(jl, ny, A.bc[2], A.bc[3], fy);
+    };
+    auto issue = [&](int jl) -> RawRow {
+        RawRow w;
+        const int jj = map_index(jl, ny, A.bc[2], A.bc[3], w.fy);
+        const bool row_real = jl >= 0 && jl < ny;
+        const unsigned ov = row_real ? ocv : kOffNone, ohw = row_real ? ohv : kOffNone;
         const long long rowk = (long long)jj * nx;
-        __amdgpu_buffer_rsrc_t ru[4], rv[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
-            ru[f] = buf_rsrc(uniform_ptr(A.U + f * P + rowk));
-            rv[f] = buf_rsrc(uniform_ptr(A.V + f * P + rowk));
+            const __amdgpu_buffer_rsrc_t ru = buf_rsrc(uniform_ptr(A.U + f * P + rowk));
+            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(uniform_ptr(A.V + f * P + rowk));
+            w.u[f] = ld_buf(ru, oc);
+            w.v[f] = ld_buf(rv, ov);
+            w.uh[f] = ld_buf(ru, oh);
+            w.vh[f] = ld_buf(rv, ohw);
         }
-        PrimE q = cons_to_prim_e(ld_buf(ru[0], oc), ld_buf(ru[1], oc), ld_buf(ru[2], oc), ld_buf(ru[3], oc), A.g1);
+        return w;
+    };
+    // convert a raw row; to_lds: publish it (and the halo) in LDS slot.
+    // pos: every cell of the row (and halo) has rho, p > kPosMin (wave-uniform)
+    auto finish = [&](const RawRow& w, int slot, bool to_lds, double Vv[4], bool& pos) -> PrimE {
+        PrimE q = cons_to_prim_e(w.u[0], w.u[1], w.u[2], w.u[3], A.g1);
         if (fx) q.u = -q.u;
-        if (fy) q.v = -q.v;
+        if (w.fy) q.v = -q.v;
         bool ok = q.r > kPosMin && q.p > kPosMin;
-        // ghost cells have no output in the forward map: their cotangent is 0
-        const bool row_real = jl >= 0 && jl < ny;
-        Vv[0] = Vv[1] = Vv[2] = Vv[3] = 0.0;
-        if (row_real && col_real) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) Vv[f] = ld_buf(rv[f], oc);
-        }
+        for (int f = 0; f < 4; ++f) Vv[f] = w.v[f];
         if (to_lds) {
             lds_put8(ring, slot, lane + 2, q, Vv);
             if (has_halo) {
-                PrimE h = cons_to_prim_e(ld_buf(ru[0], oh), ld_buf(ru[1], oh), ld_buf(ru[2], oh), ld_buf(ru[3], oh),
-                                         A.g1);
+                PrimE h = cons_to_prim_e(w.uh[0], w.uh[1], w.uh[2], w.uh[3], A.g1);
                 if (hfx) h.u = -h.u;
-                if (fy) h.v = -h.v;
+                if (w.fy) h.v = -h.v;
                 ok = ok && h.r > kPosMin && h.p > kPosMin;
-                double Vh[4] = {0.0, 0.0, 0.0, 0.0};
-                if (row_real && hcol >= 0 && hcol < nx) {
-#pragma unroll
-                    for (int f = 0; f < 4; ++f) Vh[f] = ld_buf(rv[f], oh);
-                }
-                lds_put8(ring, slot, hslot, h, Vh);
+                lds_put8(ring, slot, hslot, h, w.vh);
             }
         }
         pos = __all(ok) != 0;
         return q;
     };
 
-    // rows ro0-3 .. ro0: prims; ro0-2, ro0-1 cotangents
+    // rows ro0-3 .. ro0-1: prims; ro0-2, ro0-1 cotangents (all loads issued
+    // before the first conversion)
     double Vd[4];
     bool pa, pb, pc;
-    PrimE qa = load(ro0 - 3, 0, false, Vd, pa);  // q_{j-1} at j = ro0-2
-    PrimE qb = load(ro0 - 2, 0, false, Vd, pb);  // q_j
-    double Vb[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
-    PrimE qc = load(ro0 - 1, ((ro0 - 1) % 3 + 3) % 3, true, Vd, pc);  // q_{j+1}
-    double Vc[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
-    PrimE qd;
+    PrimE qa, qb, qc, qd;
+    double Vb[4], Vc[4];
+    {
+        const RawRow ra = issue(ro0 - 3), rb = issue(ro0 - 2), rc = issue(ro0 - 1);
+        qa = finish(ra, 0, false, Vd, pa);  // q_{j-1} at j = ro0-2
+        qb = finish(rb, 0, false, Vd, pb);  // q_j
+        Vb[0] = Vd[0]; Vb[1] = Vd[1]; Vb[2] = Vd[2]; Vb[3] = Vd[3];
+        qc = finish(rc, ((ro0 - 1) % 3 + 3) % 3, true, Vd, pc);  // q_{j+1}
+        Vc[0] = Vd[0]; Vc[1] = Vd[1]; Vc[2] = Vd[2]; Vc[3] = Vd[3];
+    }
     // positivity of the rows seen, newest in bit 0; in pass j bits 3..0 are rows j-1 .. j+2
     unsigned pos_hist = (pa ? 4u : 0u) | (pb ? 2u : 0u) | (pc ? 1u : 0u);
 
@@ -431,9 +446,74 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
             __builtin_amdgcn_s_setprio(1);
         else
             __builtin_amdgcn_s_setprio(0);
-        __syncthreads();
+        __syncthreads();  // slot (j+2) % 3 (row j-1) was read by the previous pass
+        const RawRow nr = issue(j + 2);
+        // row j's x gather (real rows only), while row j+2's loads are in
+        // flight; row j (with halo) is bit 1 before row j+2 is added
+        Q4 x_j = q4_zero();
+        const bool do_x = j >= j0 && j < j1;
+        auto xpass = [&](auto guard) {
+            constexpr bool G = decltype(guard)::value;
+            const int sl = ((j % 3) + 3) % 3;
+            // x-face F(c-1/2) from cells c-2, c-1 (LDS), c (own), c+1 (LDS)
+            const PrimE q2 = lds_q(ring, sl, lane), q1 = lds_q(ring, sl, lane + 1), q3 = lds_q(ring, sl, lane + 3);
+            const Slope sb = slope3<ORDER>(q2, q1, qb);
+            const Slope sc = slope3<ORDER>(q1, qb, q3);
+            const PrimE L = face_plus<G>(q1, sb), R = face_minus<G>(qb, sc);
+            const double Vw0 = ring[sl][4][lane + 1], Vw1 = ring[sl][5][lane + 1];
+            const double Vw2 = ring[sl][6][lane + 1], Vw3 = ring[sl][7][lane + 1];
+            FluxE Fb;
+            Fb.fr = xface_on ? (Vb[0] - Vw0) * A.idx : 0.0;
+            Fb.fn = xface_on ? (Vb[1] - Vw1) * A.idx : 0.0;
+            Fb.ft = xface_on ? (Vb[2] - Vw2) * A.idx : 0.0;
+            Fb.fe = xface_on ? (Vb[3] - Vw3) * A.idx : 0.0;
+            FaceE fl = {L.r, L.u, L.v, L.p}, fr = {R.r, R.u, R.v, R.p}, bl, br;
+            hllc_vjp<G>(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
+            const Q4 dqL = {bl.r, bl.un, bl.ut, bl.p};  // cotangent of q_plus of cell c-1
+            const Q4 dqR = {br.r, br.un, br.ut, br.p};  // cotangent of q_minus of cell c
+            // this lane's cell c: dqp from face c+1/2 (next lane's dqL), dqm = dqR
+            const Q4 dqp_c = q4_from_next(dqL);
+            Q4 own_c, lsp_c, rsp_c;
+            spread_cell<ORDER, G>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
+            const Q4 from_next = q4_from_next(lsp_c);
+            const Q4 from_prev = q4_from_prev(rsp_c);
+            x_j = q4_add(q4_add(own_c, from_next), from_prev);
+            if (A.fold_in_kernel) {
+                // ghost columns -1, -2 (lane 0: its own part and its spill to
+                // column -2) and nx, nx+1 go to the columns they copy, with u
+                // mirrored when reflective
+                if (tx == 0) {
+                    const Q4 g1 = q4_lane(x_j, 0), g2 = q4_lane(lsp_c, 0);
+                    if (c == tgt_w[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_w[0]));
+                    if (c == tgt_w[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_w[1]));
+                }
+                if (tx == A.tiles_x - 1) {
+                    const int le = nx - cbase + 1;  // lane of column nx
+                    const Q4 g1 = q4_lane(x_j, le), g2 = q4_lane(x_j, le + 1);
+                    if (c == tgt_e[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_e[0]));
+                    if (c == tgt_e[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_e[1]));
+                }
+            } else {
+                // ghost columns of the domain edges (east ones from the last tile only)
+                if (c == -1) {
+                    ghost_store(A, 0, 0, j, x_j);
+                    ghost_store(A, 0, 1, j, lsp_c);
+                }
+                if (tx == A.tiles_x - 1) {
+                    if (c == nx) ghost_store(A, 1, 0, j, x_j);
+                    if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
+                }
+            }
+        };
+        if (do_x) {
+            if (pos_hist & 2u)
+                xpass(std::false_type{});
+            else
+                xpass(std::true_type{});
+        }
+
         bool pd;
-        qd = load(j + 2, ((j + 2) % 3 + 3) % 3, true, Vd, pd);  // q_{j+2}
+        qd = finish(nr, ((j + 2) % 3 + 3) % 3, true, Vd, pd);  // q_{j+2}
         const double Vn[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
         pos_hist = ((pos_hist << 1) | (pd ? 1u : 0u)) & 15u;
 
@@ -507,70 +587,9 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                 }
             }
         }
-        // row j so far: own y part + spill from row j-1
+        // row j so far: own y part + spill from row j-1, then its x gather
         Q4 acc = q4_add(own_j, rsp_prev);
-
-        // row j's x gather (real rows only); row j (with halo) is bit 2
-        auto xpass = [&](auto guard) {
-            constexpr bool G = decltype(guard)::value;
-            const int sl = ((j % 3) + 3) % 3;
-            // x-face F(c-1/2) from cells c-2, c-1 (LDS), c (own), c+1 (LDS)
-            const PrimE q2 = lds_q(ring, sl, lane), q1 = lds_q(ring, sl, lane + 1), q3 = lds_q(ring, sl, lane + 3);
-            const Slope sb = slope3<ORDER>(q2, q1, qb);
-            const Slope sc = slope3<ORDER>(q1, qb, q3);
-            const PrimE L = face_plus<G>(q1, sb), R = face_minus<G>(qb, sc);
-            const double Vw0 = ring[sl][4][lane + 1], Vw1 = ring[sl][5][lane + 1];
-            const double Vw2 = ring[sl][6][lane + 1], Vw3 = ring[sl][7][lane + 1];
-            FluxE Fb;
-            Fb.fr = xface_on ? (Vb[0] - Vw0) * A.idx : 0.0;
-            Fb.fn = xface_on ? (Vb[1] - Vw1) * A.idx : 0.0;
-            Fb.ft = xface_on ? (Vb[2] - Vw2) * A.idx : 0.0;
-            Fb.fe = xface_on ? (Vb[3] - Vw3) * A.idx : 0.0;
-            FaceE fl = {L.r, L.u, L.v, L.p}, fr = {R.r, R.u, R.v, R.p}, bl, br;
-            hllc_vjp<G>(fl, fr, A.gamma, A.inv_g1, Fb, bl, br);
-            const Q4 dqL = {bl.r, bl.un, bl.ut, bl.p};  // cotangent of q_plus of cell c-1
-            const Q4 dqR = {br.r, br.un, br.ut, br.p};  // cotangent of q_minus of cell c
-            // this lane's cell c: dqp from face c+1/2 (next lane's dqL), dqm = dqR
-            const Q4 dqp_c = q4_from_next(dqL);
-            Q4 own_c, lsp_c, rsp_c;
-            spread_cell<ORDER, G>(q1, qb, q3, dqp_c, dqR, own_c, lsp_c, rsp_c);
-            const Q4 from_next = q4_from_next(lsp_c);
-            const Q4 from_prev = q4_from_prev(rsp_c);
-            Q4 x_j = q4_add(q4_add(own_c, from_next), from_prev);
-            if (A.fold_in_kernel) {
-                // ghost columns -1, -2 (lane 0: its own part and its spill to
-                // column -2) and nx, nx+1 go to the columns they copy, with u
-                // mirrored when reflective
-                if (tx == 0) {
-                    const Q4 g1 = q4_lane(x_j, 0), g2 = q4_lane(lsp_c, 0);
-                    if (c == tgt_w[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_w[0]));
-                    if (c == tgt_w[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_w[1]));
-                }
-                if (tx == A.tiles_x - 1) {
-                    const int le = nx - cbase + 1;  // lane of column nx
-                    const Q4 g1 = q4_lane(x_j, le), g2 = q4_lane(x_j, le + 1);
-                    if (c == tgt_e[0]) x_j = q4_add(x_j, q4_flip_u(g1, flip_e[0]));
-                    if (c == tgt_e[1]) x_j = q4_add(x_j, q4_flip_u(g2, flip_e[1]));
-                }
-            } else {
-                // ghost columns of the domain edges (east ones from the last tile only)
-                if (c == -1) {
-                    ghost_store(A, 0, 0, j, x_j);
-                    ghost_store(A, 0, 1, j, lsp_c);
-                }
-                if (tx == A.tiles_x - 1) {
-                    if (c == nx) ghost_store(A, 1, 0, j, x_j);
-                    if (c == nx + 1) ghost_store(A, 1, 1, j, x_j);
-                }
-            }
-            acc = q4_add(acc, x_j);
-        };
-        if (j >= j0 && j < j1) {
-            if (pos_hist & 4u)
-                xpass(std::false_type{});
-            else
-                xpass(std::true_type{});
-        }
+        if (do_x) acc = q4_add(acc, x_j);
 
         // advance the window
         acc_prev = acc;
