@@ -2,8 +2,10 @@
 
 `rollout` is a LibTorch autograd Function (gd2_torch, C++ over the libgd2 C
 ABI): the forward pass runs the HIP stage kernels with a prescribed dt
-schedule and keeps a checkpoint every `every` steps; the backward pass
-recomputes one segment at a time from its checkpoint and runs the stage VJP
+schedule and keeps a checkpoint every `every` steps (and the stage states of
+the last segment, where the reverse sweep starts); the backward pass
+recomputes one segment at a time from its checkpoint, on a second stream
+while the adjoint of the segment after it runs, and runs the stage VJP
 kernels (gd2_stage_vjp) backwards through it. Gradients are with respect to
 the initial conserved state, with the dt schedule held fixed.
 

@@ -3,12 +3,14 @@
 // schedule, differentiable with respect to the initial conserved state.
 //
 // Forward: nsteps SSP-RK2 steps (gd2_stage x 2 per step), keeping a copy of
-// the state every `every` steps. Backward: per segment, from the last one
-// down, recompute the segment's states from its checkpoint (the step's
-// input U^n and its stage-1 state U^(1)), then run the step adjoint
+// the state every `every` steps, and every step's states of the last
+// segment. Backward: per segment, from the last one down, the segment's
+// states (each step's input U^n and its stage-1 state U^(1)) are recomputed
+// from its checkpoint on a second stream while the previous segment's
+// adjoint runs, then the step adjoint
 //     lam1 = lam/2 + (dt/2) J_L(U^(1))^T lam
 //     lam  = lam/2 + lam1 + dt J_L(U^n)^T lam1
-// backwards through the segment (two gd2_stage_vjp launches per step).
+// runs backwards through the segment (two gd2_stage_vjp launches per step).
 // The dt schedule is data, not a function of the state: derivatives are
 // those of the map with the schedule held fixed.
 //
@@ -93,14 +95,35 @@ std::vector<double> schedule(const Tensor& dts) {
     return std::vector<double>(p, p + d.numel());
 }
 
-// Forward rollout; returns the final state and fills the checkpoints.
+// Forward rollout; returns the final state and fills the checkpoints (U^n of
+// every `every`-th step). The last segment also keeps each step's U^n and
+// U^(1) (in last_un / last_u1): the reverse sweep starts there, so its
+// recompute would have nothing to overlap with; the forward computes these
+// states anyway and only writes them to separate buffers.
 Tensor run_forward(const Setup& s, const Tensor& U0, const std::vector<double>& dt, int64_t every, void* st,
-                   std::vector<Tensor>* ckpt) {
+                   std::vector<Tensor>* ckpt, std::vector<Tensor>* last_un, std::vector<Tensor>* last_u1) {
     Tensor U = U0.detach().clone();
     Tensor S = torch::empty_like(U);
     const int64_t n = static_cast<int64_t>(dt.size());
+    const int64_t k_last = n > 0 ? ((n - 1) / every) * every : 0;
+    const bool keep = ckpt && last_un && last_u1;
     for (int64_t k = 0; k < n; ++k) {
         if (ckpt && k % every == 0) ckpt->push_back(U.clone());
+        if (keep && k >= k_last) {
+            if (k == k_last) last_un->push_back(ckpt->back());
+            const Tensor& un = last_un->back();
+            Tensor u1 = torch::empty_like(U);
+            stage1(s, un, u1, dt[k], st);
+            last_u1->push_back(u1);
+            if (k + 1 < n) {
+                Tensor next = torch::empty_like(U);
+                stage2(s, u1, un, next, dt[k], st);
+                last_un->push_back(next);
+            } else {
+                stage2(s, u1, un, U, dt[k], st);
+            }
+            continue;
+        }
         stage1(s, U, S, dt[k], st);
         stage2(s, S, U, U, dt[k], st);
     }
@@ -135,7 +158,8 @@ SideStream& side_stream() {
 // of two buffer sets on the side stream; the adjoint of segment seg (caller's
 // stream) overlaps the recompute of segment seg-1 into the other set.
 Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::vector<double>& dt, int64_t every,
-                    const Tensor& lamT, void* st) {
+                    const Tensor& lamT, void* st, const std::vector<Tensor>& last_un,
+                    const std::vector<Tensor>& last_u1) {
     const int64_t n = static_cast<int64_t>(dt.size());
     const int64_t nseg = static_cast<int64_t>(ckpt.size());
     Tensor lam = lamT.to(torch::kFloat64).contiguous().clone();
@@ -167,17 +191,26 @@ Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::
         }
         hip_check(hipEventRecord(ss.rec[set], ss.stream), "hipEventRecord");
     };
-    if (nseg > 0) recompute(nseg - 1);
+    // the last segment's states come from the forward when it kept them
+    const bool have_last = nseg > 0 && static_cast<int64_t>(last_un.size()) == n - (nseg - 1) * every &&
+                           last_u1.size() == last_un.size();
+    if (nseg > 0 && !have_last) recompute(nseg - 1);
     for (int64_t seg = nseg - 1; seg >= 0; --seg) {
         const int set = static_cast<int>(seg & 1);
+        const bool kept = have_last && seg == nseg - 1;
         if (seg >= 1) {
             // segment seg-1 reuses the buffers of segment seg+1: after its adjoint
-            if (seg + 1 < nseg) hip_check(hipStreamWaitEvent(ss.stream, ss.vjp[set ^ 1], 0), "hipStreamWaitEvent");
+            const bool seg1_used_set = seg + 1 < nseg && !(have_last && seg + 1 == nseg - 1);
+            if (seg1_used_set) hip_check(hipStreamWaitEvent(ss.stream, ss.vjp[set ^ 1], 0), "hipStreamWaitEvent");
             recompute(seg - 1);
         }
-        hip_check(hipStreamWaitEvent(main, ss.rec[set], 0), "hipStreamWaitEvent");
         const int64_t k0 = seg * every, m = std::min(n, k0 + every) - k0;
-        for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, Un[set][i], U1[set][i], lam, lam1, ws, dt[k0 + i], st);
+        if (kept) {
+            for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, last_un[i], last_u1[i], lam, lam1, ws, dt[k0 + i], st);
+        } else {
+            hip_check(hipStreamWaitEvent(main, ss.rec[set], 0), "hipStreamWaitEvent");
+            for (int64_t i = m - 1; i >= 0; --i) step_vjp(s, Un[set][i], U1[set][i], lam, lam1, ws, dt[k0 + i], st);
+        }
         hip_check(hipEventRecord(ss.vjp[set], main), "hipEventRecord");
     }
     // every side-stream launch was waited for by the caller's stream above, so
@@ -192,9 +225,11 @@ struct Rollout : public torch::autograd::Function<Rollout> {
         check_state(s, U0, "U0");
         TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
         const std::vector<double> dt = schedule(dts);
-        std::vector<Tensor> ckpt;
-        Tensor UT = run_forward(s, U0, dt, every, reinterpret_cast<void*>(stream), &ckpt);
+        std::vector<Tensor> ckpt, last_un, last_u1;
+        Tensor UT = run_forward(s, U0, dt, every, reinterpret_cast<void*>(stream), &ckpt, &last_un, &last_u1);
         ctx->saved_data["ckpt"] = ckpt;
+        ctx->saved_data["last_un"] = last_un;
+        ctx->saved_data["last_u1"] = last_u1;
         ctx->saved_data["dt"] = dt;
         ctx->saved_data["geom"] = geom;
         ctx->saved_data["bc"] = bc;
@@ -213,7 +248,8 @@ struct Rollout : public torch::autograd::Function<Rollout> {
         void* st = reinterpret_cast<void*>(ctx->saved_data["stream"].toInt());
         Tensor g = grads[0];
         if (!g.defined()) return {Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
-        Tensor lam0 = run_backward(s, ckpt, dt, every, g, st);
+        Tensor lam0 = run_backward(s, ckpt, dt, every, g, st, ctx->saved_data["last_un"].toTensorVector(),
+                                   ctx->saved_data["last_u1"].toTensorVector());
         return {lam0, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
     }
 };
@@ -231,10 +267,10 @@ std::vector<Tensor> rollout_vjp(Tensor U0, Tensor dts, Tensor lamT, std::vector<
     check_state(s, lamT, "lamT");
     TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
     const std::vector<double> dt = schedule(dts);
-    std::vector<Tensor> ckpt;
+    std::vector<Tensor> ckpt, last_un, last_u1;
     void* st = reinterpret_cast<void*>(stream);
-    Tensor UT = run_forward(s, U0, dt, every, st, &ckpt);
-    Tensor lam0 = run_backward(s, ckpt, dt, every, lamT, st);
+    Tensor UT = run_forward(s, U0, dt, every, st, &ckpt, &last_un, &last_u1);
+    Tensor lam0 = run_backward(s, ckpt, dt, every, lamT, st, last_un, last_u1);
     return {UT, lam0};
 }
 
