@@ -77,7 +77,7 @@ def lib():
             "gd2_last_hip_error": (ip, []),
             "gd2_version": (C.c_char_p, []),
             "gd2_stage": (ip, [gp, sp, vp, vp, vp, dp, dp, dp, vp, vp, vp]),
-            "gd2_ssprk2_step": (ip, [gp, sp, vp, vp, dp, vp, vp, vp]),
+            "gd2_ssprk2_step": (ip, [gp, sp, vp, vp, vp, dp, vp, vp, vp]),
             "gd2_wavespeed_max": (ip, [gp, sp, vp, vp, vp]),
             "gd2_ker_dekard": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
             "gd2_kernel_tvd_old": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
@@ -103,6 +103,8 @@ def lib():
             "gd2_ctx_log_dt": (ip, [vp, vp, C.c_longlong]),
             "gd2_vjp_workspace_bytes": (C.c_longlong, [gp]),
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
+            "gd2_set_step_path": (ip, [ip]),
+            "gd2_one_pass_preferred": (ip, [gp, sp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -118,7 +120,8 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
-            "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step"]
+            "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
+            "gd2_one_pass_preferred"]
 
 
 def _check(rc, name):
@@ -196,11 +199,32 @@ def stage(g, sc, Uin, Ubase, Uout, a=0.0, b=1.0, dt=0.0, dt_dev=None, wmax_bits=
                            float(dt), _f64(dt_dev), _p(wmax_bits), _stream(stream)), "gd2_stage")
 
 
-def ssprk2_step(g, sc, Uin, Uout, dt=0.0, dt_dev=None, wmax_bits=None, stream=None):
+def ssprk2_step(g, sc, Uin, Uout, dt=0.0, dt_dev=None, wmax_bits=None, stream=None, ustage=None):
     """One SSP-RK2 step Uin -> Uout in a single pass (Euler); bitwise equal to
-    stage(Uin -> S) followed by stage(S, Ubase=Uin -> Uout, a=b=1/2)."""
-    _check(lib().gd2_ssprk2_step(C.byref(g), C.byref(sc), _f64(Uin), _f64(Uout), float(dt), _f64(dt_dev),
-                                 _p(wmax_bits), _stream(stream)), "gd2_ssprk2_step")
+    stage(Uin -> S) followed by stage(S, Ubase=Uin -> Uout, a=b=1/2). When
+    `ustage` is given it receives the stage-1 state S."""
+    _check(lib().gd2_ssprk2_step(C.byref(g), C.byref(sc), _f64(Uin), _f64(Uout), _f64(ustage), float(dt),
+                                 _f64(dt_dev), _p(wmax_bits), _stream(stream)), "gd2_ssprk2_step")
+
+
+STEP_AUTO, STEP_TWO_PASS, STEP_ONE_PASS = 0, 1, 2
+
+
+def set_step_path(mode) -> int:
+    """Process-wide choice between the one-pass SSP-RK2 kernel and two stage
+    launches where the library steps by itself (the solver context); returns
+    the previous mode. Both paths give the same bits."""
+    rc = lib().gd2_set_step_path(int(mode))
+    if rc < 0:
+        raise GD2Error(f"gd2_set_step_path({mode}) failed ({rc})")
+    return rc
+
+
+def one_pass_preferred(g, sc) -> bool:
+    rc = lib().gd2_one_pass_preferred(C.byref(g), C.byref(sc))
+    if rc < 0:
+        raise GD2Error(f"gd2_one_pass_preferred failed ({rc})")
+    return rc == 1
 
 
 def vjp_workspace_bytes(g) -> int:

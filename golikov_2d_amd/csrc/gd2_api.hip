@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <map>
 #include <new>
@@ -174,25 +175,26 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
 // rounds of resident waves against the extra passes of each strip (four U1
 // rows below and above it and the lead-in, about three rows of work), up to
 // 256 rows so that a large grid fits in one round.
-template <int ORDER, int FLUX>
+template <int ORDER, int FLUX, bool KEEP>
 static int launch_rk2(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
     A.tiles_x = (A.nx + kRk2W - 1) / kRk2W;
-    const void* k = reinterpret_cast<const void*>(&euler_rk2_kernel<ORDER, FLUX>);
+    const void* k = reinterpret_cast<const void*>(&euler_rk2_kernel<ORDER, FLUX, KEEP>);
     A.rows = max(2, strip_rows(k, A.tiles_x, A.ny, 3, 256));  // strips start at row 0 or row >= 2
     const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
-    hipLaunchKernelGGL((euler_rk2_kernel<ORDER, FLUX>), dim3(blocks), dim3(kWave), 0, st, A);
+    hipLaunchKernelGGL((euler_rk2_kernel<ORDER, FLUX, KEEP>), dim3(blocks), dim3(kWave), 0, st, A);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
 
+template <bool KEEP>
 static int dispatch_rk2(int order, int flux, const EulerArgs& A, hipStream_t st) {
     if (order == 2) {
-        if (flux == GD2_FLUX_HLLC) return launch_rk2<2, GD2_FLUX_HLLC>(A, st);
-        if (flux == GD2_FLUX_EXACT) return launch_rk2<2, GD2_FLUX_EXACT>(A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_rk2<2, GD2_FLUX_HLLC, KEEP>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_rk2<2, GD2_FLUX_EXACT, KEEP>(A, st);
     } else if (order == 1) {
-        if (flux == GD2_FLUX_HLLC) return launch_rk2<1, GD2_FLUX_HLLC>(A, st);
-        if (flux == GD2_FLUX_EXACT) return launch_rk2<1, GD2_FLUX_EXACT>(A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_rk2<1, GD2_FLUX_HLLC, KEEP>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_rk2<1, GD2_FLUX_EXACT, KEEP>(A, st);
     }
     return GD2_ERR_UNSUPPORTED;
 }
@@ -324,9 +326,10 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
     return dispatch_euler<kSoA, false, true>(sc->order, sc->flux, A, st);
 }
 
-int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double dt,
-                    const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
+int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double* Ustage,
+                    double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
     if (!grid_ok(g, false) || !scheme_ok(sc) || !Uin || !Uout || Uin == Uout) return GD2_ERR_ARG;
+    if (Ustage && (Ustage == Uin || Ustage == Uout)) return GD2_ERR_ARG;
     if (sc->physics != GD2_EULER) return GD2_ERR_UNSUPPORTED;
     // the boundary rows and columns of the intermediate state are built from
     // its first and last two rows / columns
@@ -335,13 +338,15 @@ int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, 
     fill_common(A, g);
     A.Uin = Uin;
     A.Uout = Uout;
+    A.Ustage = Ustage;
     A.a = 0.5;
     A.b = 0.5;
     A.dt = dt;
     A.dt_dev = dt_dev;
     A.wmax_bits = wmax_bits;
     if (!A.diag) return GD2_ERR_HIP;
-    return dispatch_rk2(sc->order, sc->flux, A, (hipStream_t)stream);
+    if (Ustage) return dispatch_rk2<true>(sc->order, sc->flux, A, (hipStream_t)stream);
+    return dispatch_rk2<false>(sc->order, sc->flux, A, (hipStream_t)stream);
 }
 
 int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, unsigned long long* wmax_bits,
@@ -552,15 +557,41 @@ int gd2_ctx_start(gd2_ctx* c, const double* U, double t0, void* stream) {
     return gd2_wavespeed_max(&c->g, &c->sc, U, &c->clock->w[0], stream);
 }
 
-// GD2_TWO_PASS=1 makes the solver context step SSP-RK2 with two stage
-// launches instead of the one-pass kernel (same results, for comparison).
-static bool one_pass_enabled() {
-    static int v = -1;
+// How SSP-RK2 Euler steps are taken: GD2_STEP_AUTO picks the one-pass kernel
+// on grids of at least kOnePassCells cells, where it is measured faster than
+// two stage launches (on MI355X: 0.80x at 4096^2, 0.90x at 2048^2, 1.05x at
+// 1024^2, 1.17x at 512^2: a small grid needs short strips to fill the chip,
+// and the one-pass strip re-forms four more rows of U1 than a stage strip
+// re-reads). Both paths give the same bits. GD2_TWO_PASS=1 in the
+// environment starts in GD2_STEP_TWO_PASS.
+constexpr long long kOnePassCells = 1ll << 21;
+static std::atomic<int> g_step_path{-1};
+
+static int step_path() {
+    int v = g_step_path.load();
     if (v < 0) {
         const char* e = std::getenv("GD2_TWO_PASS");
-        v = (e && std::atoi(e) == 1) ? 0 : 1;
+        int init = (e && std::atoi(e) == 1) ? GD2_STEP_TWO_PASS : GD2_STEP_AUTO;
+        g_step_path.compare_exchange_strong(v, init);
+        v = g_step_path.load();
     }
-    return v == 1;
+    return v;
+}
+
+int gd2_set_step_path(int mode) {
+    if (mode != GD2_STEP_AUTO && mode != GD2_STEP_TWO_PASS && mode != GD2_STEP_ONE_PASS) return GD2_ERR_ARG;
+    const int prev = step_path();
+    g_step_path.store(mode);
+    return prev;
+}
+
+int gd2_one_pass_preferred(const gd2_grid* g, const gd2_scheme* sc) {
+    if (!g || !sc) return GD2_ERR_ARG;
+    if (sc->physics != GD2_EULER || sc->integrator != GD2_SSPRK2 || g->nx < 2 || g->ny < 2) return 0;
+    const int mode = step_path();
+    if (mode == GD2_STEP_ONE_PASS) return 1;
+    if (mode == GD2_STEP_TWO_PASS) return 0;
+    return (long long)g->nx * g->ny >= kOnePassCells ? 1 : 0;
 }
 
 int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* stream) {
@@ -569,7 +600,7 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
     double* S = c->scratch;
     const double* dtp = &c->clock->dt;
     double* cur = U;  // forward Euler and the one-pass SSP-RK2 step ping-pong between U and S
-    const bool one_pass = c->sc.physics == GD2_EULER && c->g.nx >= 2 && c->g.ny >= 2 && one_pass_enabled();
+    const bool one_pass = gd2_one_pass_preferred(&c->g, &c->sc) == 1;
     for (int i = 0; i < nsteps; ++i) {
         const int slot = (int)(c->host_steps & 1);
         hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log, c->dt_cap);
@@ -579,7 +610,7 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
         if (c->sc.integrator == GD2_SSPRK2 && one_pass) {
             // one pass per step, ping-ponging between U and the scratch state
             double* nxt = (cur == U) ? S : U;
-            rc = gd2_ssprk2_step(&c->g, &c->sc, cur, nxt, 0.0, dtp, wn, stream);
+            rc = gd2_ssprk2_step(&c->g, &c->sc, cur, nxt, nullptr, 0.0, dtp, wn, stream);
             if (rc) return rc;
             cur = nxt;
         } else if (c->sc.integrator == GD2_SSPRK2) {

@@ -32,6 +32,7 @@ struct EulerArgs {
     const double* Uin;     // kSoA: 4 planes
     const double* Ubase;   // kSoA: 4 planes or null (a == 0)
     double* Uout;          // kSoA: 4 planes
+    double* Ustage;        // one-pass step: receives the stage-1 state U1 (may be null)
     const double2* s;      // kAoS: (rho, p)
     const double2* u;      // kAoS: (vx, vy)
     double2* s2;

@@ -24,7 +24,9 @@
 //
 // Every expression is the one the stage kernel uses (same face functions,
 // x_balance / full_balance, out_plain, rk_base, out_base, clamps), so the
-// result is bitwise that of the two stage launches.
+// result is bitwise that of the two stage launches. With KEEP the producer
+// also stores the U1 rows of its strip (bitwise the first stage launch's
+// output), for callers that need the stage state, such as an adjoint sweep.
 #pragma once
 
 #include "gd2_euler.hpp"
@@ -59,7 +61,7 @@ __device__ __forceinline__ PrimE lds_get68(const double (*ring)[4][kRk2Row], int
 
 __device__ __forceinline__ bool prim_pos(const PrimE& q) { return q.r > kPosMin && q.p > kPosMin; }
 
-template <int ORDER, int FLUX>
+template <int ORDER, int FLUX, bool KEEP>
 __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int ty, Rk2Lds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kRk2W;
@@ -226,6 +228,15 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
             double o3 = out_plain(uc.a[3], dtdV, P.e);
             // every U1 cell is counted once: by the wave that updates it
             clamp_plain(A, o0, o1, o2, o3, owns && r >= j0 && r < j1);
+            if constexpr (KEEP) {
+                if (owns && r >= j0 && r < j1) {
+                    double* sb = A.Ustage + (long long)r * A.nx;
+                    st_buf(buf_rsrc(uniform_ptr(sb)), oo, o0);
+                    st_buf(buf_rsrc(uniform_ptr(sb + A.plane)), oo, o1);
+                    st_buf(buf_rsrc(uniform_ptr(sb + 2 * A.plane)), oo, o2);
+                    st_buf(buf_rsrc(uniform_ptr(sb + 3 * A.plane)), oo, o3);
+                }
+            }
             q1 = cons_to_prim_e(o0, o1, o2, o3, A.g1);
             const int hs = (r + 3) % 3;
             lds.h[hs][0][lane] = rk_base(A.a, uc.a[0], A.b, o0);
@@ -435,11 +446,11 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
     if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
 }
 
-template <int ORDER, int FLUX>
+template <int ORDER, int FLUX, bool KEEP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 2 : 1))) void euler_rk2_kernel(const EulerArgs A) {
     __shared__ Rk2Lds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    euler_rk2_tile<ORDER, FLUX>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
+    euler_rk2_tile<ORDER, FLUX, KEEP>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
 }
 
 }  // namespace gd2

@@ -2,9 +2,9 @@
 // SSP-RK2 rollout of the Euler HLLC-MUSCL scheme with a prescribed dt
 // schedule, differentiable with respect to the initial conserved state.
 //
-// Forward: nsteps SSP-RK2 steps (gd2_stage x 2 per step), keeping a copy of
-// the state every `every` steps, and every step's states of the last
-// segment. Backward: per segment, from the last one down, the segment's
+// Forward: nsteps SSP-RK2 steps (gd2_ssprk2_step, or two gd2_stage launches
+// where the library prefers them: small grids), keeping a copy of the state
+// every `every` steps, and every step's states of the last segment. Backward: per segment, from the last one down, the segment's
 // states (each step's input U^n and its stage-1 state U^(1)) are recomputed
 // from its checkpoint on a second stream while the previous segment's
 // adjoint runs, then the step adjoint
@@ -19,6 +19,7 @@
 #include <torch/extension.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gd2.h"
@@ -32,6 +33,7 @@ using torch::autograd::variable_list;
 struct Setup {
     gd2_grid g;
     gd2_scheme sc;
+    bool one_pass;  // SSP-RK2 steps through gd2_ssprk2_step
 };
 
 Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc, int64_t order) {
@@ -52,6 +54,9 @@ Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc
     s.sc.order = static_cast<int>(order);
     s.sc.integrator = GD2_SSPRK2;
     s.sc.cfl = geom[8];
+    const int pref = gd2_one_pass_preferred(&s.g, &s.sc);
+    TORCH_CHECK(pref >= 0, "gd2_one_pass_preferred failed");
+    s.one_pass = pref == 1;
     return s;
 }
 
@@ -75,6 +80,20 @@ void stage1(const Setup& s, const Tensor& U, Tensor& U1, double dt, void* st) {
 
 void stage2(const Setup& s, const Tensor& U1, const Tensor& Un, Tensor& Uout, double dt, void* st) {
     check(gd2_stage(&s.g, &s.sc, dp(U1), dp(Un), dp(Uout), 0.5, 0.5, dt, nullptr, nullptr, st), "gd2_stage");
+}
+
+// One SSP-RK2 step Un -> Uout (Uout != Un); U1 (may be null) receives the
+// stage-1 state. One pass where the library prefers it, else two stage
+// launches (same bits); `tmp` is the stage state of the latter when U1 is null.
+void step(const Setup& s, const Tensor& Un, Tensor& Uout, Tensor* U1, Tensor& tmp, double dt, void* st) {
+    if (s.one_pass) {
+        check(gd2_ssprk2_step(&s.g, &s.sc, dp(Un), dp(Uout), U1 ? dp(*U1) : nullptr, dt, nullptr, nullptr, st),
+              "gd2_ssprk2_step");
+        return;
+    }
+    Tensor& S = U1 ? *U1 : tmp;
+    stage1(s, Un, S, dt, st);
+    stage2(s, S, Un, Uout, dt, st);
 }
 
 // One step adjoint in place: lam <- (d U^{n+1} / d U^n)^T lam.
@@ -103,7 +122,8 @@ std::vector<double> schedule(const Tensor& dts) {
 Tensor run_forward(const Setup& s, const Tensor& U0, const std::vector<double>& dt, int64_t every, void* st,
                    std::vector<Tensor>* ckpt, std::vector<Tensor>* last_un, std::vector<Tensor>* last_u1) {
     Tensor U = U0.detach().clone();
-    Tensor S = torch::empty_like(U);
+    Tensor N = torch::empty_like(U);  // next state
+    Tensor S = torch::empty_like(U);  // stage state of a two-launch step
     const int64_t n = static_cast<int64_t>(dt.size());
     const int64_t k_last = n > 0 ? ((n - 1) / every) * every : 0;
     const bool keep = ckpt && last_un && last_u1;
@@ -113,19 +133,17 @@ Tensor run_forward(const Setup& s, const Tensor& U0, const std::vector<double>& 
             if (k == k_last) last_un->push_back(ckpt->back());
             const Tensor& un = last_un->back();
             Tensor u1 = torch::empty_like(U);
-            stage1(s, un, u1, dt[k], st);
+            Tensor next = torch::empty_like(U);
+            step(s, un, next, &u1, S, dt[k], st);
             last_u1->push_back(u1);
-            if (k + 1 < n) {
-                Tensor next = torch::empty_like(U);
-                stage2(s, u1, un, next, dt[k], st);
+            if (k + 1 < n)
                 last_un->push_back(next);
-            } else {
-                stage2(s, u1, un, U, dt[k], st);
-            }
+            else
+                U = next;
             continue;
         }
-        stage1(s, U, S, dt[k], st);
-        stage2(s, S, U, U, dt[k], st);
+        step(s, U, N, nullptr, S, dt[k], st);
+        std::swap(U, N);
     }
     return U;
 }
@@ -186,8 +204,10 @@ Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::
         hip_check(hipMemcpyAsync(dp(Un[set][0]), dp(ckpt[seg]), bytes, hipMemcpyDeviceToDevice, ss.stream),
                   "hipMemcpyAsync");
         for (int64_t i = 0; i < m; ++i) {
-            stage1(s, Un[set][i], U1[set][i], dt[k0 + i], ss.stream);
-            if (i + 1 < m) stage2(s, U1[set][i], Un[set][i], Un[set][i + 1], dt[k0 + i], ss.stream);
+            if (i + 1 < m)
+                step(s, Un[set][i], Un[set][i + 1], &U1[set][i], U1[set][i], dt[k0 + i], ss.stream);
+            else
+                stage1(s, Un[set][i], U1[set][i], dt[k0 + i], ss.stream);
         }
         hip_check(hipEventRecord(ss.rec[set], ss.stream), "hipEventRecord");
     };

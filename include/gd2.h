@@ -137,11 +137,25 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
  * with the intermediate state U1 kept on chip. Bitwise equal to
  *   gd2_stage(Uin -> S, a=0, b=1); gd2_stage(S, Ubase=Uin -> Uout, a=b=1/2)
  * (including the wave-speed fold into wmax_bits and the clamp counts).
- * Uout must not alias Uin. Euler physics (HLLC or exact flux) only; other
- * schemes return GD2_ERR_UNSUPPORTED. No reference counterpart (the reference
- * steps with forward Euler); used by gd2_ctx_advance for SSP-RK2 Euler. */
+ * Ustage may be null; otherwise it receives U1 (bitwise S above), e.g. for
+ * an adjoint sweep that needs the stage state. Uout and Ustage must not alias
+ * Uin or each other. Euler physics (HLLC or exact flux) only; other schemes
+ * return GD2_ERR_UNSUPPORTED. No reference counterpart (the reference steps
+ * with forward Euler); used by gd2_ctx_advance for SSP-RK2 Euler. */
 int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout,
-                    double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream);
+                    double* Ustage, double dt, const double* dt_dev, unsigned long long* wmax_bits,
+                    void* stream);
+
+/* How SSP-RK2 Euler steps are taken where the library chooses (the solver
+ * context): GD2_STEP_AUTO uses gd2_ssprk2_step on grids of at least 2^21
+ * cells and two gd2_stage launches on smaller ones (the faster of the two on
+ * MI355X); the others force one path. Both paths give the same bits.
+ * Process-wide; returns the previous mode. No reference counterpart. */
+enum gd2_step_path { GD2_STEP_AUTO = 0, GD2_STEP_TWO_PASS = 1, GD2_STEP_ONE_PASS = 2 };
+int gd2_set_step_path(int mode);
+/* 1 when a step of this grid and scheme goes through gd2_ssprk2_step under
+ * the current mode, 0 when through two gd2_stage launches. */
+int gd2_one_pass_preferred(const gd2_grid* g, const gd2_scheme* sc);
 
 /* max over cells of (|u|+c)/dx + (|v|+c)/dy of U, folded into *wmax_bits
  * (device) as above; the caller zeroes *wmax_bits first. */

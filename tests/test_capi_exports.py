@@ -46,3 +46,25 @@ def test_pure_host_entry_points():
     assert L.gd2_stage(ctypes.byref(bad), ctypes.byref(sc), x, None, ctypes.c_void_p(32), 0.0, 1.0, 0.0, None,
                        None, None) == -1
     assert L.gd2_ker_dekard(ctypes.byref(g), x, x, ctypes.c_void_p(32), ctypes.c_void_p(48), x, x, 0, None) == -2
+
+
+@pytest.mark.skipif(not os.path.exists(api.SO_PATH), reason="libgd2.so not built")
+def test_step_path_policy():
+    """AUTO takes the one-pass SSP-RK2 kernel from 2^21 cells up; the forced
+    modes override it; non-Euler schemes, forward Euler and grids under 2 x 2
+    never take it; a bad mode is rejected (host logic only)."""
+    sc = api.make_scheme()
+    big, small = api.make_grid(2048, 1024), api.make_grid(1024, 1024)
+    prev = api.set_step_path(api.STEP_AUTO)
+    try:
+        assert api.one_pass_preferred(big, sc) and not api.one_pass_preferred(small, sc)
+        assert api.set_step_path(api.STEP_ONE_PASS) == api.STEP_AUTO
+        assert api.one_pass_preferred(small, sc)
+        assert not api.one_pass_preferred(api.make_grid(1, 64), sc)
+        assert not api.one_pass_preferred(big, api.make_scheme(integrator=api.FORWARD_EULER))
+        assert not api.one_pass_preferred(big, api.make_scheme(api.MHD, api.FLUX_HLLD))
+        api.set_step_path(api.STEP_TWO_PASS)
+        assert not api.one_pass_preferred(big, sc)
+        assert api.lib().gd2_set_step_path(7) == -1
+    finally:
+        api.set_step_path(prev)

@@ -29,20 +29,21 @@ BCS = {
 }
 
 
-def two_stage(g, sc, U, dt):
+def two_stage(g, sc, U, dt, want_stage=False):
     S = torch.empty_like(U)
     out = torch.empty_like(U)
     w = torch.zeros(1, dtype=torch.int64, device=dev())
     api.stage(g, sc, U, None, S, 0.0, 1.0, dt)
     api.stage(g, sc, S, U, out, 0.5, 0.5, dt, wmax_bits=w)
-    return out, w
+    return (out, w, S) if want_stage else (out, w)
 
 
-def one_pass(g, sc, U, dt):
+def one_pass(g, sc, U, dt, want_stage=False):
     out = torch.empty_like(U)
     w = torch.zeros(1, dtype=torch.int64, device=dev())
-    api.ssprk2_step(g, sc, U, out, dt, wmax_bits=w)
-    return out, w
+    S = torch.full_like(U, float("nan")) if want_stage else None
+    api.ssprk2_step(g, sc, U, out, dt, wmax_bits=w, ustage=S)
+    return (out, w, S) if want_stage else (out, w)
 
 
 @pytest.mark.parametrize("shape", [(130, 97), (63, 64), (64, 63), (2, 5), (200, 3), (59, 2), (118, 41),
@@ -57,11 +58,16 @@ def test_one_pass_step_equals_two_stages(shape, bc, flux, order):
     sc = api.make_scheme(api.EULER, flux, order, api.SSPRK2, 0.4)
     U = to_dev(PB.four_quadrant(og, seed=nx * 7 + ny))
     dt = 0.3 / (api.wavespeed_max(g, sc, U))
-    ref, wref = two_stage(g, sc, U, dt)
+    ref, wref, Sref = two_stage(g, sc, U, dt, want_stage=True)
     got, w = one_pass(g, sc, U, dt)
     torch.cuda.synchronize()
     assert torch.equal(got, ref), float((got - ref).abs().max())
     assert int(w.item()) == int(wref.item())
+    # the variant that also stores the stage-1 state: every cell written once,
+    # bitwise the first stage launch's output
+    got2, w2, S = one_pass(g, sc, U, dt, want_stage=True)
+    assert torch.equal(got2, ref) and int(w2.item()) == int(wref.item())
+    assert torch.equal(S, Sref), float((S - Sref).abs().nan_to_num(1e300).max())
 
 
 def test_one_pass_step_matches_oracle():
@@ -95,6 +101,10 @@ def test_one_pass_clamp_counts_equal_two_stages():
     assert d_ref["clamp_rho"] + d_ref["clamp_p"] > 0
     assert (d["clamp_rho"], d["clamp_p"]) == (d_ref["clamp_rho"], d_ref["clamp_p"])
     assert torch.equal(got, ref) and int(w.item()) == int(wref.item())
+    got2, _, S = one_pass(g, sc, Ud, dt, want_stage=True)
+    d2 = api.diagnostics(reset=True)
+    assert (d2["clamp_rho"], d2["clamp_p"]) == (d_ref["clamp_rho"], d_ref["clamp_p"])
+    assert torch.equal(got2, ref)
 
 
 @pytest.mark.parametrize("nsteps", [1, 4, 7])
@@ -116,12 +126,17 @@ def test_solver_context_one_pass_equals_two_pass(nsteps):
         Ur, w = two_stage(g, sc, Ur, dt)
         wb = api.bits_to_double(w)
         t += dt
-    s = api.Solver(g, sc)
-    U = U0.clone()
-    s.start(U)
-    s.advance(U, nsteps)
-    tg, ng, _ = s.time()
-    s.close()
+    prev = api.set_step_path(api.STEP_ONE_PASS)  # the grid is below the size where AUTO picks it
+    try:
+        assert api.one_pass_preferred(g, sc)
+        s = api.Solver(g, sc)
+        U = U0.clone()
+        s.start(U)
+        s.advance(U, nsteps)
+        tg, ng, _ = s.time()
+        s.close()
+    finally:
+        api.set_step_path(prev)
     assert ng == nsteps
     assert torch.equal(U, Ur)
     assert tg == t
