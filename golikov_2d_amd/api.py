@@ -101,7 +101,7 @@ def lib():
             "gd2_ensemble_time": (ip, [vp, ip, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
             "gd2_ensemble_tune": (ip, [vp, vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]),
             "gd2_ctx_log_dt": (ip, [vp, vp, C.c_longlong]),
-            "gd2_vjp_workspace_bytes": (C.c_longlong, [gp]),
+            "gd2_vjp_workspace_bytes": (C.c_longlong, [gp, sp]),
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
             "gd2_set_step_path": (ip, [ip]),
             "gd2_one_pass_preferred": (ip, [gp, sp]),
@@ -227,25 +227,28 @@ def one_pass_preferred(g, sc) -> bool:
     return rc == 1
 
 
-def vjp_workspace_bytes(g) -> int:
-    n = lib().gd2_vjp_workspace_bytes(C.byref(g))
+def vjp_workspace_bytes(g, sc=None) -> int:
+    sc = make_scheme() if sc is None else sc
+    n = lib().gd2_vjp_workspace_bytes(C.byref(g), C.byref(sc))
     if n < 0:
         raise GD2Error(f"gd2_vjp_workspace_bytes returned {n}")
     return n
 
 
-def vjp_workspace(g, device):
-    return torch.empty(vjp_workspace_bytes(g) // 8, dtype=torch.float64, device=device)
+def vjp_workspace(g, device, sc=None):
+    return torch.empty(vjp_workspace_bytes(g, sc) // 8, dtype=torch.float64, device=device)
 
 
 def stage_vjp(g, sc, U, V, W, c=1.0, A=None, a=0.0, B=None, b=0.0, workspace=None, stream=None):
-    """W = a*A + b*B + c * J_L(U)^T V for the Euler HLLC stage operator L."""
+    """W = a*A + b*B + c * J_L(U)^T V for the stage operator L (Euler HLLC, or
+    MHD HLLC-MHD / HLLD with the Powell term)."""
+    nf = 7 if sc.physics == MHD else 4
     for t in (U, V, W, A, B):
-        if t is not None and tuple(t.shape) != (4, g.ny, g.nx):
-            raise GD2Error(f"state shape {tuple(t.shape)} is not (4, {g.ny}, {g.nx})")
+        if t is not None and tuple(t.shape) != (nf, g.ny, g.nx):
+            raise GD2Error(f"state shape {tuple(t.shape)} is not ({nf}, {g.ny}, {g.nx})")
     if workspace is None:
-        workspace = vjp_workspace(g, U.device)
-    elif workspace.numel() * workspace.element_size() < vjp_workspace_bytes(g):
+        workspace = vjp_workspace(g, U.device, sc)
+    elif workspace.numel() * workspace.element_size() < vjp_workspace_bytes(g, sc):
         raise GD2Error("workspace smaller than gd2_vjp_workspace_bytes")
     _check(lib().gd2_stage_vjp(C.byref(g), C.byref(sc), _f64(U), _f64(V), _f64(A), float(a), _f64(B), float(b),
                                float(c), _f64(W), _p(workspace), _stream(stream)), "gd2_stage_vjp")

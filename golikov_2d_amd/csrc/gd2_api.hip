@@ -16,6 +16,7 @@
 #include "../../include/gd2.h"
 #include "gd2_euler.hpp"
 #include "gd2_rk2.hpp"
+#include "gd2_mhd_adjoint.hpp"
 #include "gd2_mhd.hpp"
 #include "gd2_ensemble.hpp"
 #include "gd2_adjoint.hpp"
@@ -278,6 +279,58 @@ static unsigned ew_blocks(long long n) {
     return (unsigned)b;
 }
 
+static unsigned grid_stride_blocks(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+template <int ORDER, int METOD>
+static int launch_mhd_vjp(const MhdAdjArgs& J, hipStream_t st) {
+    hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, true>), dim3(grid_stride_blocks(J.nfx)), dim3(256), 0, st,
+                       J);
+    GD2_HIP(hipGetLastError());
+    hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, false>), dim3(grid_stride_blocks(J.nfy)), dim3(256), 0, st,
+                       J);
+    GD2_HIP(hipGetLastError());
+    hipLaunchKernelGGL((mhd_vjp_cell_kernel<ORDER>), dim3(grid_stride_blocks(J.plane)), dim3(256), 0, st, J);
+    GD2_HIP(hipGetLastError());
+    return GD2_OK;
+}
+
+static int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V, const double* A,
+                         double a, const double* B, double b, double c, double* W, void* workspace,
+                         hipStream_t st) {
+    MhdAdjArgs J;
+    std::memset(&J, 0, sizeof J);
+    J.U = U;
+    J.V = V;
+    J.A = A;
+    J.B = B;
+    J.W = W;
+    J.a = a;
+    J.b = b;
+    J.c = c;
+    J.nx = g->nx;
+    J.ny = g->ny;
+    J.plane = (long long)g->nx * g->ny;
+    J.nfx = (long long)(g->nx + 1) * g->ny;
+    J.nfy = (long long)g->nx * (g->ny + 1);
+    J.wx = static_cast<double*>(workspace);
+    J.wy = J.wx + 14 * J.nfx;
+    J.dx = g->dx;
+    J.dy = g->dy;
+    J.dV = g->dx * g->dy;
+    J.gamma = g->gamma;
+    J.g1 = g->gamma - 1.0;
+    J.inv_g1 = 1.0 / (g->gamma - 1.0);
+    for (int s2 = 0; s2 < 4; ++s2) J.bc[s2] = g->bc[s2];
+    const bool hlld = sc->flux == GD2_FLUX_HLLD;
+    if (sc->order == 2) return hlld ? launch_mhd_vjp<2, 3>(J, st) : launch_mhd_vjp<2, 2>(J, st);
+    return hlld ? launch_mhd_vjp<1, 3>(J, st) : launch_mhd_vjp<1, 2>(J, st);
+}
+
 }  // namespace gd2
 
 using namespace gd2;
@@ -475,8 +528,13 @@ int gd2_ctx_log_dt(gd2_ctx* c, double* dt_log, long long capacity) {
     return GD2_OK;
 }
 
-long long gd2_vjp_workspace_bytes(const gd2_grid* g) {
-    if (!g || g->nx < 1 || g->ny < 1) return GD2_ERR_ARG;
+long long gd2_vjp_workspace_bytes(const gd2_grid* g, const gd2_scheme* sc) {
+    if (!g || !sc || g->nx < 1 || g->ny < 1) return GD2_ERR_ARG;
+    if (sc->physics == GD2_MHD) {
+        // face-state cotangents: 14 planes over the x-faces and the y-faces
+        const long long nf = (long long)(g->nx + 1) * g->ny + (long long)g->nx * (g->ny + 1);
+        return (long long)sizeof(double) * 14 * nf;
+    }
     const long long len = g->nx > g->ny ? g->nx : g->ny;
     return (long long)sizeof(double) * 4 * 2 * len * 4;
 }
@@ -485,6 +543,8 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
                   double a, const double* B, double b, double c, double* W, void* workspace, void* stream) {
     if (!grid_ok(g, false) || !scheme_ok(sc) || !U || !V || !W || !workspace) return GD2_ERR_ARG;
     if (W == U || W == V) return GD2_ERR_ARG;
+    if (sc->physics == GD2_MHD)
+        return mhd_stage_vjp(g, sc, U, V, A, a, B, b, c, W, workspace, (hipStream_t)stream);
     if (sc->physics != GD2_EULER || sc->flux != GD2_FLUX_HLLC) return GD2_ERR_UNSUPPORTED;
     AdjArgs J;
     std::memset(&J, 0, sizeof J);

@@ -221,21 +221,27 @@ struct MhdArgs {
     int tiles_x, rows;
 };
 
+// Primitives of cell k of a SoA conserved MHD state (P = plane size).
+__device__ __forceinline__ PrimM cons_to_prim_m(const double* U, long long P, long long k, double g1) {
+    PrimM q;
+    const double r = U[k], mx = U[P + k], my = U[2 * P + k];
+    q.bx = U[3 * P + k];
+    q.by = U[4 * P + k];
+    q.bz = U[5 * P + k];
+    const double E = U[6 * P + k];
+    const double inv = fast_rcp(r);
+    q.r = r;
+    q.u = mx * inv;
+    q.v = my * inv;
+    q.p = g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
+    return q;
+}
+
 template <int LAYOUT>
 __device__ __forceinline__ PrimM load_cell_m(const MhdArgs& A, long long k, bool fx, bool fy) {
     PrimM q;
     if constexpr (LAYOUT == kSoA) {
-        const long long P = A.plane;
-        const double r = A.Uin[k], mx = A.Uin[P + k], my = A.Uin[2 * P + k];
-        q.bx = A.Uin[3 * P + k];
-        q.by = A.Uin[4 * P + k];
-        q.bz = A.Uin[5 * P + k];
-        const double E = A.Uin[6 * P + k];
-        const double inv = fast_rcp(r);
-        q.r = r;
-        q.u = mx * inv;
-        q.v = my * inv;
-        q.p = A.g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
+        q = cons_to_prim_m(A.Uin, A.plane, k, A.g1);
     } else {
         const double2 sv = A.s[k], uv = A.u[k];
         const double3 bv = A.b[k];

@@ -182,7 +182,7 @@ Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::
     const int64_t nseg = static_cast<int64_t>(ckpt.size());
     Tensor lam = lamT.to(torch::kFloat64).contiguous().clone();
     Tensor lam1 = torch::empty_like(lam);
-    const long long wsb = gd2_vjp_workspace_bytes(&s.g);
+    const long long wsb = gd2_vjp_workspace_bytes(&s.g, &s.sc);
     TORCH_CHECK(wsb > 0, "gd2_vjp_workspace_bytes failed");
     Tensor ws = torch::empty({(wsb + 7) / 8}, lam.options());
     const int64_t seg_len = std::min<int64_t>(every, n);

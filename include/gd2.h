@@ -213,19 +213,21 @@ int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, dou
 int gd2_ctx_log_dt(gd2_ctx* ctx, double* dt_log, long long capacity);
 
 /* ----------------------------------------------------------------------
- * Discrete adjoint of the Euler stage operator (HLLC flux, order 1 or 2).
+ * Discrete adjoint of the stage operator: Euler with the HLLC flux, or MHD
+ * with the HLLC-MHD or HLLD flux (Powell term included); order 1 or 2.
  * No reference counterpart. For L(U) = -(1/(dx dy)) * (sum of outward face
- * fluxes times face lengths), computes
+ * fluxes times face lengths [+ Powell source]), computes
  *     W = a*A + b*B + c * J_L(U)^T V
  * (A, B may be NULL; W may alias A or B but not U or V). The minmod branch of
  * every slope is the one the forward stage takes on U (weights 1/2-1/2 on a
  * tie, 0 on the zero branch); the rho/p <= 0 face fall-back and the boundary
- * ghost fill are differentiated as copies. workspace: device memory of
- * gd2_vjp_workspace_bytes(g) bytes. One SSP-RK2 step adjoint, with
- * U1 = U + dt L(U):  lam1 = lam/2 + (dt/2) J_L(U1)^T lam;
- *                    lam0 = lam/2 + lam1 + dt J_L(U)^T lam1.
+ * ghost fill are differentiated as copies; the clamps are not differentiated.
+ * workspace: device memory of gd2_vjp_workspace_bytes(g, sc) bytes. One
+ * SSP-RK2 step adjoint, with U1 = U + dt L(U):
+ *     lam1 = lam/2 + (dt/2) J_L(U1)^T lam;
+ *     lam0 = lam/2 + lam1 + dt J_L(U)^T lam1.
  * ---------------------------------------------------------------------- */
-long long gd2_vjp_workspace_bytes(const gd2_grid* g);
+long long gd2_vjp_workspace_bytes(const gd2_grid* g, const gd2_scheme* sc);
 int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V,
                   const double* A, double a, const double* B, double b, double c, double* W,
                   void* workspace, void* stream);
