@@ -1,4 +1,5 @@
-"""Differentiable SSP-RK2 rollouts of the Euler HLLC-MUSCL scheme.
+"""Differentiable SSP-RK2 rollouts of the Euler HLLC and MHD HLLC-MHD / HLLD
+MUSCL schemes.
 
 `rollout` is a LibTorch autograd Function (gd2_torch, C++ over the libgd2 C
 ABI): the forward pass runs the HIP stage kernels with a prescribed dt
@@ -43,8 +44,14 @@ def _geom(g, sc):
 
 
 def _check(g, sc):
-    if sc.physics != api.EULER or sc.flux != api.FLUX_HLLC or sc.integrator != api.SSPRK2:
-        raise api.GD2Error("differentiable rollouts cover Euler + HLLC + SSP-RK2")
+    euler = sc.physics == api.EULER and sc.flux == api.FLUX_HLLC
+    mhd = sc.physics == api.MHD and sc.flux in (api.FLUX_HLLC_MHD, api.FLUX_HLLD)
+    if not (euler or mhd) or sc.integrator != api.SSPRK2:
+        raise api.GD2Error("differentiable rollouts cover SSP-RK2 with Euler + HLLC or MHD + HLLC-MHD / HLLD")
+
+
+def _scheme(sc):
+    return [int(sc.physics), int(sc.flux), int(sc.order)]
 
 
 def _stream(stream):
@@ -53,17 +60,16 @@ def _stream(stream):
 
 
 def rollout(g, sc, U0, dts, every=20, stream=None):
-    """U_T after len(dts) SSP-RK2 steps from U0 (shape (4, ny, nx), float64, device).
-
-    Differentiable with respect to U0 through torch autograd."""
+    """U_T after len(dts) SSP-RK2 steps from U0 (shape (nfields, ny, nx), float64,
+    device). Differentiable with respect to U0 through torch autograd."""
     _check(g, sc)
-    return module().rollout(U0, dts, _geom(g, sc), list(g.bc), sc.order, int(every), _stream(stream))
+    return module().rollout(U0, dts, _geom(g, sc), list(g.bc), _scheme(sc), int(every), _stream(stream))
 
 
 def rollout_vjp(g, sc, U0, dts, lamT, every=20, stream=None):
     """(U_T, (dU_T/dU0)^T lamT) without building an autograd graph."""
     _check(g, sc)
-    UT, lam0 = module().rollout_vjp(U0, dts, lamT, _geom(g, sc), list(g.bc), sc.order, int(every),
+    UT, lam0 = module().rollout_vjp(U0, dts, lamT, _geom(g, sc), list(g.bc), _scheme(sc), int(every),
                                     _stream(stream))
     return UT, lam0
 

@@ -1,6 +1,7 @@
 // gd2_torch.cpp -- LibTorch autograd Function over the libgd2 C ABI: an
-// SSP-RK2 rollout of the Euler HLLC-MUSCL scheme with a prescribed dt
-// schedule, differentiable with respect to the initial conserved state.
+// SSP-RK2 rollout of the Euler HLLC scheme or the MHD HLLC-MHD / HLLD scheme
+// (MUSCL or first order) with a prescribed dt schedule, differentiable with
+// respect to the initial conserved state.
 //
 // Forward: nsteps SSP-RK2 steps (gd2_ssprk2_step, or two gd2_stage launches
 // where the library prefers them: small grids), keeping a copy of the state
@@ -33,12 +34,15 @@ using torch::autograd::variable_list;
 struct Setup {
     gd2_grid g;
     gd2_scheme sc;
+    int nfields;
     bool one_pass;  // SSP-RK2 steps through gd2_ssprk2_step
 };
 
-Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc, int64_t order) {
+Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc,
+                 const std::vector<int64_t>& scheme) {
     TORCH_CHECK(geom.size() == 9, "geom = (nx, ny, dx, dy, x0, y0, gamma, krit, cfl)");
     TORCH_CHECK(bc.size() == 4, "bc = (west, east, south, north)");
+    TORCH_CHECK(scheme.size() == 3, "scheme = (physics, flux, order)");
     Setup s{};
     s.g.nx = static_cast<int>(geom[0]);
     s.g.ny = static_cast<int>(geom[1]);
@@ -49,11 +53,15 @@ Setup make_setup(const std::vector<double>& geom, const std::vector<int64_t>& bc
     s.g.gamma = geom[6];
     s.g.krit = geom[7];
     for (int i = 0; i < 4; ++i) s.g.bc[i] = static_cast<int>(bc[i]);
-    s.sc.physics = GD2_EULER;
-    s.sc.flux = GD2_FLUX_HLLC;
-    s.sc.order = static_cast<int>(order);
+    s.sc.physics = static_cast<int>(scheme[0]);
+    s.sc.flux = static_cast<int>(scheme[1]);
+    s.sc.order = static_cast<int>(scheme[2]);
     s.sc.integrator = GD2_SSPRK2;
     s.sc.cfl = geom[8];
+    const bool euler = s.sc.physics == GD2_EULER && s.sc.flux == GD2_FLUX_HLLC;
+    const bool mhd = s.sc.physics == GD2_MHD && (s.sc.flux == GD2_FLUX_HLLC_MHD || s.sc.flux == GD2_FLUX_HLLD);
+    TORCH_CHECK(euler || mhd, "differentiable rollouts cover Euler + HLLC and MHD + HLLC-MHD / HLLD");
+    s.nfields = gd2_nfields(s.sc.physics);
     const int pref = gd2_one_pass_preferred(&s.g, &s.sc);
     TORCH_CHECK(pref >= 0, "gd2_one_pass_preferred failed");
     s.one_pass = pref == 1;
@@ -68,8 +76,8 @@ void check_state(const Setup& s, const Tensor& U, const char* name) {
     TORCH_CHECK(U.is_cuda(), name, " must be a device tensor");
     TORCH_CHECK(U.scalar_type() == torch::kFloat64, name, " must be float64");
     TORCH_CHECK(U.is_contiguous(), name, " must be contiguous");
-    TORCH_CHECK(U.dim() == 3 && U.size(0) == 4 && U.size(1) == s.g.ny && U.size(2) == s.g.nx, name,
-                " must have shape (4, ny, nx)");
+    TORCH_CHECK(U.dim() == 3 && U.size(0) == s.nfields && U.size(1) == s.g.ny && U.size(2) == s.g.nx, name,
+                " must have shape (nfields, ny, nx)");
 }
 
 double* dp(const Tensor& t) { return t.data_ptr<double>(); }
@@ -240,8 +248,8 @@ Tensor run_backward(const Setup& s, const std::vector<Tensor>& ckpt, const std::
 
 struct Rollout : public torch::autograd::Function<Rollout> {
     static Tensor forward(AutogradContext* ctx, Tensor U0, Tensor dts, std::vector<double> geom,
-                          std::vector<int64_t> bc, int64_t order, int64_t every, int64_t stream) {
-        const Setup s = make_setup(geom, bc, order);
+                          std::vector<int64_t> bc, std::vector<int64_t> scheme, int64_t every, int64_t stream) {
+        const Setup s = make_setup(geom, bc, scheme);
         check_state(s, U0, "U0");
         TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
         const std::vector<double> dt = schedule(dts);
@@ -253,7 +261,7 @@ struct Rollout : public torch::autograd::Function<Rollout> {
         ctx->saved_data["dt"] = dt;
         ctx->saved_data["geom"] = geom;
         ctx->saved_data["bc"] = bc;
-        ctx->saved_data["order"] = order;
+        ctx->saved_data["scheme"] = scheme;
         ctx->saved_data["every"] = every;
         ctx->saved_data["stream"] = stream;
         return UT;
@@ -261,7 +269,7 @@ struct Rollout : public torch::autograd::Function<Rollout> {
 
     static variable_list backward(AutogradContext* ctx, variable_list grads) {
         const Setup s = make_setup(ctx->saved_data["geom"].toDoubleVector(), ctx->saved_data["bc"].toIntVector(),
-                                   ctx->saved_data["order"].toInt());
+                                   ctx->saved_data["scheme"].toIntVector());
         const std::vector<Tensor> ckpt = ctx->saved_data["ckpt"].toTensorVector();
         const std::vector<double> dt = ctx->saved_data["dt"].toDoubleVector();
         const int64_t every = ctx->saved_data["every"].toInt();
@@ -274,15 +282,16 @@ struct Rollout : public torch::autograd::Function<Rollout> {
     }
 };
 
-Tensor rollout(Tensor U0, Tensor dts, std::vector<double> geom, std::vector<int64_t> bc, int64_t order,
-               int64_t every, int64_t stream) {
-    return Rollout::apply(U0, dts, geom, bc, order, every, stream);
+Tensor rollout(Tensor U0, Tensor dts, std::vector<double> geom, std::vector<int64_t> bc,
+               std::vector<int64_t> scheme, int64_t every, int64_t stream) {
+    return Rollout::apply(U0, dts, geom, bc, scheme, every, stream);
 }
 
 // The same two sweeps without the autograd graph: returns (U_T, lam_0).
 std::vector<Tensor> rollout_vjp(Tensor U0, Tensor dts, Tensor lamT, std::vector<double> geom,
-                                std::vector<int64_t> bc, int64_t order, int64_t every, int64_t stream) {
-    const Setup s = make_setup(geom, bc, order);
+                                std::vector<int64_t> bc, std::vector<int64_t> scheme, int64_t every,
+                                int64_t stream) {
+    const Setup s = make_setup(geom, bc, scheme);
     check_state(s, U0, "U0");
     check_state(s, lamT, "lamT");
     TORCH_CHECK(every >= 1, "checkpoint interval must be >= 1");
@@ -297,7 +306,9 @@ std::vector<Tensor> rollout_vjp(Tensor U0, Tensor dts, Tensor lamT, std::vector<
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
-    m.doc() = "Differentiable SSP-RK2 rollouts of the Euler HLLC-MUSCL scheme on libgd2";
-    m.def("rollout", &rollout, "U_T = rollout(U0, dts, geom, bc, order, every, stream) (autograd-aware)");
-    m.def("rollout_vjp", &rollout_vjp, "(U_T, lam_0) = rollout_vjp(U0, dts, lamT, geom, bc, order, every, stream)");
+    m.doc() = "Differentiable SSP-RK2 rollouts of the Euler and MHD MUSCL schemes on libgd2";
+    m.def("rollout", &rollout,
+          "U_T = rollout(U0, dts, geom, bc, (physics, flux, order), every, stream) (autograd-aware)");
+    m.def("rollout_vjp", &rollout_vjp,
+          "(U_T, lam_0) = rollout_vjp(U0, dts, lamT, geom, bc, (physics, flux, order), every, stream)");
 }
