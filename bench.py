@@ -10,7 +10,8 @@ Prints one JSON line: Gcell-updates/s over the timed steps, the roofline of
 the one-pass step kernel (160 algorithmic bytes per cell-update / its
 HIP-event-timed launch duration, against the 8 TB/s HBM peak), a CPU
 baseline (the oracle on a bounded sample of the same workload, timed on this
-host) and the batched-ensemble and adjoint legs.
+host) and the batched-ensemble and adjoint legs, plus the ideal-MHD HLLD
+Orszag-Tang run at 2048^2 and the exact-Riemann Sod run at 200^2.
 """
 from __future__ import annotations
 
@@ -40,9 +41,11 @@ def parse():
     ap.add_argument("--copy-ref-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference for HBM counters)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched-ensemble and adjoint legs")
+                    help="skip the batched-ensemble, adjoint, MHD and Sod legs")
     ap.add_argument("--only-extras", action="store_true",
-                    help="run only the batched-ensemble and adjoint legs (for profiling them)")
+                    help="run only the batched-ensemble, adjoint, MHD and Sod legs (for profiling them)")
+    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod"], default=None,
+                    help="run only one of those legs (for profiling it)")
     return ap.parse_args()
 
 
@@ -62,7 +65,7 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(n, seed):
-    """Oracle (OpenMP) on a bounded sample: 3 SSP-RK2 steps of the full grid."""
+    """Oracle (OpenMP) on a bounded sample: the first 25 SSP-RK2 steps of the full grid."""
     import numpy as np
     from oracle import oracle as O
     from golikov_2d_amd import problems as PB
@@ -150,6 +153,95 @@ def batched_vs_serial(seed=0, count=256, steps=50):
     }
 
 
+def mhd_orszag_tang(n=2048, steps=100, seed=0, reps=10):
+    """Ideal MHD Orszag-Tang vortex (gamma 5/3, periodic, seeded 1e-3 Fourier
+    perturbations), HLLD + minmod MUSCL, SSP-RK2, CFL 0.3: Gcell-updates/s
+    over `steps` solver steps, and the two stage kernels' HIP-event durations
+    against the 280 algorithmic bytes per cell-update (7 fields: stage 1
+    reads and writes 56 B, stage 2 reads 112 B and writes 56 B)."""
+    import torch
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    g = api.make_grid(n, n, gamma=5.0 / 3.0, bc=(api.BC_PERIODIC,) * 4)
+    sc = api.make_scheme(api.MHD, api.FLUX_HLLD, 2, api.SSPRK2, 0.3)
+    U = torch.from_numpy(PB.orszag_tang(g, seed=seed, amp=1e-3)).to(dev)
+    s = api.Solver(g, sc)
+    s.start(U)
+    s.advance(U, 5)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    s.advance(U, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    _, _, dt_last = s.time()
+    s.close()
+    stream = torch.cuda.current_stream()
+    S = torch.empty_like(U)
+    O = torch.empty_like(U)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps + 1)]
+    api.stage(g, sc, U, None, S, 0.0, 1.0, dt_last)
+    api.stage(g, sc, S, U, O, 0.5, 0.5, dt_last)
+    ev[0].record(stream)
+    for i in range(reps):
+        api.stage(g, sc, U, None, S, 0.0, 1.0, dt_last)
+        ev[2 * i + 1].record(stream)
+        api.stage(g, sc, S, U, O, 0.5, 0.5, dt_last)
+        ev[2 * i + 2].record(stream)
+    torch.cuda.synchronize()
+    t1 = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps)) / reps
+    t2 = sum(ev[2 * i + 1].elapsed_time(ev[2 * i + 2]) for i in range(reps)) / reps
+    cells = n * n
+    achieved = 280.0 * cells / ((t1 + t2) / 1e3) / 1e9
+    return {
+        "value": round(cells * steps / (ms / 1e3) / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(ms / steps, 4),
+        "stage_kernel_ms": [round(t1, 4), round(t2, 4)],
+        "bytes_per_cell_update": 280,
+        "achieved_GBs": round(achieved, 1),
+        "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
+        "workload": f"ideal MHD Orszag-Tang {n}x{n}, gamma 5/3, periodic, HLLD + minmod MUSCL, SSP-RK2, CFL 0.3, "
+                    f"{steps} steps (seed {seed})",
+    }
+
+
+def sod_godunov(n=200):
+    """Sod shock tube along x on n x n cells, exact Riemann (Godunov) flux,
+    first order, forward Euler, CFL 0.4, transmissive, run to t = 0.2 (the
+    second of two identical runs is timed)."""
+    import torch
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    g = api.make_grid(n, n)
+    sc = api.make_scheme(api.EULER, api.FLUX_EXACT, 1, api.FORWARD_EULER, 0.4)
+    U0 = torch.from_numpy(PB.sod_x(g)).to(dev)
+    for rep in range(2):
+        U = U0.clone()
+        s = api.Solver(g, sc)
+        s.start(U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_end, nsteps = s.run_to(U, 0.2, chunk=64)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        s.close()
+    return {
+        "value": round(n * n * nsteps / wall / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "steps": nsteps,
+        "wall_ms": round(1e3 * wall, 3),
+        "t_end": t_end,
+        "workload": f"Sod shock tube {n}x{n} (x-directed), exact Riemann flux, first order, forward Euler, "
+                    "CFL 0.4, run to t = 0.2",
+    }
+
+
 def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
     """Differentiable rollout (LibTorch autograd Function): backward (recompute
     from checkpoints + stage VJPs) against the forward rollout, both timed."""
@@ -194,6 +286,18 @@ def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
     }
 
 
+def copy_reference(mib):
+    """One plain device-to-device copy of `mib` MiB: the reference dispatch
+    that scales the HBM byte counters."""
+    import torch
+    src = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -206,9 +310,18 @@ def main():
     from golikov_2d_amd import api
     from golikov_2d_amd import problems as PB
 
+    if args.only_leg:
+        if args.copy_ref_mib > 0:
+            copy_reference(args.copy_ref_mib)
+        legs = {"batched": lambda: batched_vs_serial(args.seed), "adjoint": lambda: adjoint_vs_forward(seed=args.seed),
+                "mhd": lambda: mhd_orszag_tang(seed=args.seed), "sod": sod_godunov}
+        print(json.dumps({args.only_leg: legs[args.only_leg]()}), flush=True)
+        return
     if args.only_extras:
         print(json.dumps({"batched_vs_serial": batched_vs_serial(args.seed),
-                          "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed)}), flush=True)
+                          "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
+                          "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed),
+                          "sod_godunov_200": sod_godunov()}), flush=True)
         return
 
     n = args.n
@@ -286,13 +399,7 @@ def main():
     healthy = bool(np.isfinite(Uh).all() and rho.min() > 0 and p.min() > 0)
 
     if args.copy_ref_mib > 0:
-        nel = args.copy_ref_mib * (1 << 20) // 8
-        src = torch.ones(nel, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_reference(args.copy_ref_mib)
 
     cpu = None
     if not args.no_cpu:
@@ -301,7 +408,9 @@ def main():
     if not args.no_extras:
         del S
         extras = {"batched_vs_serial": batched_vs_serial(args.seed),
-                  "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed)}
+                  "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
+                  "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed),
+                  "sod_godunov_200": sod_godunov()}
 
     out = {
         "metric": METRIC,
