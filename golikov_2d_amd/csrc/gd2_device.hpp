@@ -508,6 +508,68 @@ __device__ __forceinline__ void atomic_min_nonneg(unsigned long long* dst, doubl
 // XCD-aware tile order: the hardware hands consecutive workgroups to the 8
 // XCDs in turn; remap so that each XCD walks a contiguous range of tiles (its
 // neighbours' halo rows and columns then sit in the same L2).
+// Issue priority between the waves sharing a SIMD. The older of two waves
+// wins issue ties, so one would finish far ahead and leave the other to run
+// alone. Each pass a wave publishes its pass count in its SIMD's row of
+// g_simd_passes (slot = hardware wave slot; passes + 1 while it runs, 0 when
+// it has none; 16 slots for each of the 8 x 256 x 4 SIMDs) and reads the row
+// one pass earlier: the wave that is behind takes the higher priority, level
+// waves alternate by pass and slot parity. (The table costs a store and a
+// load per pass, which suits the one-pass step's long passes; the stage VJP's
+// short passes measured slower with it than with a plain parity alternation.)
+constexpr int kSimdSlots = 16;
+__device__ unsigned g_simd_passes[8 * 256 * 4 * kSimdSlots];
+
+struct SimdTurn {
+    unsigned* slots;
+    unsigned wid;
+    unsigned seen;  // lane l < 16: slot l as read one pass earlier
+    unsigned done;  // passes done by this wave
+
+    __device__ __forceinline__ void init() {
+        const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);        // HW_ID
+        const unsigned xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7u; // XCC_ID
+        wid = hw & 15u;
+        const unsigned key = (((xcc * 8u + ((hw >> 13) & 7u)) * 2u + ((hw >> 12) & 1u)) * 16u + ((hw >> 8) & 15u)) * 4u +
+                             ((hw >> 4) & 3u);
+        slots = g_simd_passes + key * kSimdSlots;
+        seen = 0u;
+        done = 0u;
+    }
+
+    __device__ __forceinline__ void step(int pass) {
+        const unsigned lane = threadIdx.x & 63u;
+        unsigned v = (lane < (unsigned)kSimdSlots && lane != wid) ? seen : 0u;
+        // maximum over the 16 lanes of DPP row 0 (row_ror 8, 4, 2, 1)
+        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));
+        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));
+        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false));
+        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false));
+        const unsigned other = __builtin_amdgcn_readfirstlane(v);
+        const unsigned mine = done + 1u;
+        bool high;
+        if (other == 0u)
+            high = false;  // alone on the SIMD
+        else if (other != mine)
+            high = other > mine;
+        else
+            high = (((unsigned)pass ^ wid) & 1u) != 0u;
+        if (high)
+            __builtin_amdgcn_s_setprio(1);
+        else
+            __builtin_amdgcn_s_setprio(0);
+        if (lane == 0) __hip_atomic_store(slots + wid, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        seen = lane < (unsigned)kSimdSlots ? __hip_atomic_load(slots + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                           : 0u;
+        done = mine;
+    }
+
+    __device__ __forceinline__ void finish() {
+        __builtin_amdgcn_s_setprio(0);
+        if ((threadIdx.x & 63u) == 0u) __hip_atomic_store(slots + wid, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
 __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     constexpr int kXcd = 8;
     const int xcd = b % kXcd, local = b / kXcd;

@@ -33,13 +33,6 @@
 
 namespace gd2 {
 
-// Per-SIMD table of the passes done by the waves running this kernel on it
-// (slot = the wave's hardware slot; pass count + 1 while it runs, 0 when it
-// has none), 16 slots for each of the 8 x 256 x 4 SIMDs. The waves sharing a
-// SIMD read it to decide which of them takes the higher issue priority.
-constexpr int kSimdSlots = 16;
-__device__ unsigned g_simd_passes[8 * 256 * 4 * kSimdSlots];
-
 constexpr int kRk2W = 59;     // columns of U^{n+1} per wave
 constexpr int kRk2Row = 68;   // LDS row entries: entry e <-> column cbase - 4 + e
 
@@ -420,45 +413,10 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         consume_y(r - 2, q1, qpos, true, X2, hj);
     };
 
-    // Two waves share a SIMD and the older one wins issue ties, so one would
-    // finish far ahead and leave the other to run alone. Each pass a wave
-    // publishes its pass count in the SIMD's slot table and reads the other
-    // slots (loaded one pass earlier): the wave that is behind takes the
-    // higher issue priority; level waves alternate by pass and slot parity.
-    const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);        // HW_ID
-    const unsigned xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7u; // XCC_ID
-    const unsigned wid = hw & 15u;
-    const unsigned simd_key = (((xcc * 8u + ((hw >> 13) & 7u)) * 2u + ((hw >> 12) & 1u)) * 16u + ((hw >> 8) & 15u)) * 4u +
-                              ((hw >> 4) & 3u);
-    unsigned* const slots = g_simd_passes + simd_key * kSimdSlots;
-    unsigned seen = 0u;   // lane l < 16: slot l as read one pass earlier
-    unsigned done = 0u;   // passes done by this wave
-    auto set_turn = [&](int pass) {
-        unsigned v = ((unsigned)lane < (unsigned)kSimdSlots && (unsigned)lane != wid) ? seen : 0u;
-        // maximum over the 16 lanes of DPP row 0 (row_ror 8, 4, 2, 1)
-        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));
-        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));
-        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false));
-        v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false));
-        const unsigned other = __builtin_amdgcn_readfirstlane(v);
-        const unsigned mine = done + 1u;
-        bool high;
-        if (other == 0u)
-            high = false;  // alone on the SIMD
-        else if (other != mine)
-            high = other > mine;
-        else
-            high = (((unsigned)pass ^ wid) & 1u) != 0u;
-        if (high)
-            __builtin_amdgcn_s_setprio(1);
-        else
-            __builtin_amdgcn_s_setprio(0);
-        if (lane == 0) __hip_atomic_store(slots + wid, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        seen = (unsigned)lane < (unsigned)kSimdSlots
-                   ? __hip_atomic_load(slots + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                   : 0u;
-        done = mine;
-    };
+    // the wave behind on its SIMD takes the higher issue priority each pass
+    SimdTurn turn;
+    turn.init();
+    auto set_turn = [&](int pass) { turn.step(pass); };
 
     const int s_begin = j0 + 2;            // first pass that updates a row
     const int s_end = min(r_last, j1 + 1);  // last pass whose U1 row the producer forms
@@ -475,8 +433,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         pass_general(r);
         ++r;
     }
-    __builtin_amdgcn_s_setprio(0);
-    if (lane == 0) __hip_atomic_store(slots + wid, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    turn.finish();
 
     const double w = wave_max(wmax);
     if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
