@@ -79,14 +79,37 @@ def cpu_baseline(n, seed):
     O.run(og, sc, 2, U, nsteps=steps)
     dt = time.perf_counter() - t0
     assert np.all(np.isfinite(U))
+    threads = O.num_threads()
+    # one thread: one step of the same grid
+    U1 = PB.four_quadrant(og, seed=seed)
+    O.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        O.run(og, sc, 2, U1, nsteps=1)
+        dt1 = time.perf_counter() - t0
+    finally:
+        O.set_num_threads(threads)
     return {
         "value": round(n * rows * steps / dt / 1e9, 6),
         "unit": "Gcell-updates/s",
-        "cores": O.num_threads(),
+        "cores": threads,
         "kind": "port",
         "sample": f"oracle (C, OpenMP, -O3 -ffp-contract=off): the first {steps} SSP-RK2 steps of the same "
                   f"{n}x{rows} workload, {dt:.2f} s wall",
+        "one_thread": round(n * rows / dt1 / 1e9, 6),
+        "one_thread_sample": f"the first SSP-RK2 step of the same workload on 1 thread, {dt1:.2f} s wall",
     }
+
+
+def cpu_leg(g_kw, scheme, U, integrator, nsteps=0, t_end=0.0):
+    """The oracle on all host threads over a given run: Gcell-updates/s."""
+    from oracle import oracle as O
+    og = O.make_grid(**g_kw)
+    t0 = time.perf_counter()
+    steps, _ = O.run(og, O.Scheme(*scheme), integrator, U, nsteps=nsteps, t_end=t_end)
+    dt = time.perf_counter() - t0
+    return {"value": round(og.nx * og.ny * steps / dt / 1e9, 6), "unit": "Gcell-updates/s",
+            "cores": O.num_threads(), "kind": "port", "steps": steps, "wall_s": round(dt, 3)}
 
 
 def batched_vs_serial(seed=0, count=256, steps=50):
@@ -153,7 +176,7 @@ def batched_vs_serial(seed=0, count=256, steps=50):
     }
 
 
-def mhd_orszag_tang(n=2048, steps=100, seed=0, reps=10):
+def mhd_orszag_tang(n=2048, steps=100, seed=0, reps=10, with_cpu=True):
     """Ideal MHD Orszag-Tang vortex (gamma 5/3, periodic, seeded 1e-3 Fourier
     perturbations), HLLD + minmod MUSCL, SSP-RK2, CFL 0.3: Gcell-updates/s
     over `steps` solver steps, and the two stage kernels' HIP-event durations
@@ -196,6 +219,12 @@ def mhd_orszag_tang(n=2048, steps=100, seed=0, reps=10):
     t2 = sum(ev[2 * i + 1].elapsed_time(ev[2 * i + 2]) for i in range(reps)) / reps
     cells = n * n
     achieved = 280.0 * cells / ((t1 + t2) / 1e3) / 1e9
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        cpu = cpu_leg(dict(nx=n, ny=n, gamma=5.0 / 3.0, bc=(O.BC_PERIODIC,) * 4),
+                      (O.PHYS_MHD, O.FLUX_HLLD, 2, 0.3), PB.orszag_tang(g, seed=seed, amp=1e-3), 2, nsteps=3)
+        cpu["sample"] = "oracle (C, OpenMP): the first 3 SSP-RK2 steps of the same Orszag-Tang workload"
     return {
         "value": round(cells * steps / (ms / 1e3) / 1e9, 4),
         "unit": "Gcell-updates/s",
@@ -206,10 +235,11 @@ def mhd_orszag_tang(n=2048, steps=100, seed=0, reps=10):
         "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
         "workload": f"ideal MHD Orszag-Tang {n}x{n}, gamma 5/3, periodic, HLLD + minmod MUSCL, SSP-RK2, CFL 0.3, "
                     f"{steps} steps (seed {seed})",
+        "cpu_baseline": cpu,
     }
 
 
-def sod_godunov(n=200):
+def sod_godunov(n=200, with_cpu=True):
     """Sod shock tube along x on n x n cells, exact Riemann (Godunov) flux,
     first order, forward Euler, CFL 0.4, transmissive, run to t = 0.2 (the
     second of two identical runs is timed)."""
@@ -231,9 +261,15 @@ def sod_godunov(n=200):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         s.close()
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        cpu = cpu_leg(dict(nx=n, ny=n), (O.PHYS_EULER, O.FLUX_EXACT, 1, 0.4), PB.sod_x(g), 1, t_end=0.2)
+        cpu["sample"] = "oracle (C, OpenMP): the whole run to t = 0.2"
     return {
         "value": round(n * n * nsteps / wall / 1e9, 4),
         "unit": "Gcell-updates/s",
+        "cpu_baseline": cpu,
         "steps": nsteps,
         "wall_ms": round(1e3 * wall, 3),
         "t_end": t_end,
@@ -314,14 +350,15 @@ def main():
         if args.copy_ref_mib > 0:
             copy_reference(args.copy_ref_mib)
         legs = {"batched": lambda: batched_vs_serial(args.seed), "adjoint": lambda: adjoint_vs_forward(seed=args.seed),
-                "mhd": lambda: mhd_orszag_tang(seed=args.seed), "sod": sod_godunov}
+                "mhd": lambda: mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
+                "sod": lambda: sod_godunov(with_cpu=not args.no_cpu)}
         print(json.dumps({args.only_leg: legs[args.only_leg]()}), flush=True)
         return
     if args.only_extras:
         print(json.dumps({"batched_vs_serial": batched_vs_serial(args.seed),
                           "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
-                          "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed),
-                          "sod_godunov_200": sod_godunov()}), flush=True)
+                          "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
+                          "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu)}), flush=True)
         return
 
     n = args.n
@@ -409,8 +446,8 @@ def main():
         del S
         extras = {"batched_vs_serial": batched_vs_serial(args.seed),
                   "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
-                  "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed),
-                  "sod_godunov_200": sod_godunov()}
+                  "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
+                  "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu)}
 
     out = {
         "metric": METRIC,
