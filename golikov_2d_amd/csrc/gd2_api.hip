@@ -572,6 +572,15 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
     J.rows = strip_rows(sc->order == 2 ? reinterpret_cast<const void*>(&euler_vjp_kernel<2>)
                                        : reinterpret_cast<const void*>(&euler_vjp_kernel<1>),
                         J.tiles_x, g->ny, 3);
+    {
+        // GD2_VJP_ROWS overrides the strip height of the VJP launches alone
+        static int env = -1;
+        if (env < 0) {
+            const char* e = std::getenv("GD2_VJP_ROWS");
+            env = e ? std::atoi(e) : 0;
+        }
+        if (env > 0) J.rows = env;
+    }
     J.gstride = g->nx > g->ny ? g->nx : g->ny;
     // ghost cotangents are folded inside the kernel when every ghost copies a
     // cell of its own wave or strip: no periodic side, and the first / last
