@@ -260,7 +260,9 @@ __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, d
     FluxE F;
     F.fr = run + Se * (rz - r);
     F.fn = run * un + p + Se * (SM * rz - run);
-    F.ft = run * ut + Se * (ut * rz - r * ut);
+    // the reference's run ut + S (ut rz - r ut) is ut times the mass flux
+    // (the tangential velocity is carried across the contact unchanged)
+    F.ft = ut * F.fr;
     F.fe = (e + p) * un + Se * (ez - e);
     return F;
 }

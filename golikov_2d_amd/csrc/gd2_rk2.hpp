@@ -250,15 +250,20 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         pGm = Gp;
     };
 
-    // ================= consumer x-face F2 of row j (LDS slot j & 1) and H_j
+    // ================= consumer x-face F2 of row j (LDS slot j & 1) and H_j.
+    // Only lanes 2..61 need their face F2(c-1/2), so the slope of column c-1
+    // is lane l-1's slope of its own column, shifted over (bit for bit).
     auto consume_xface = [&](int j, Bal& X2, double (&hj)[4]) {
         const int sl = j & 1;
-        const PrimE qa = lds_get68(lds.w, sl, lane);
         const PrimE qb = lds_get68(lds.w, sl, lane + 1);
         const PrimE qc = lds_get68(lds.w, sl, lane + 2);
         const PrimE qd = lds_get68(lds.w, sl, lane + 3);
-        const Slope sb = slope3<ORDER>(qa, qb, qc);
         const Slope sc = slope3<ORDER>(qb, qc, qd);
+        Slope sb;
+        sb.r = from_prev_lane(sc.r);
+        sb.u = from_prev_lane(sc.u);
+        sb.v = from_prev_lane(sc.v);
+        sb.p = from_prev_lane(sc.p);
         FluxE Fl, Fr;
         if ((chist & 2u) != 0u) {
             const PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
