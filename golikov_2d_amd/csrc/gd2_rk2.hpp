@@ -92,6 +92,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
                         ((c == A.nx || c == A.nx + 1) && A.bc[1] != GD2_BC_PERIODIC);
     const int gsrc_e = gsrc - cbase + 4;  // LDS entry of the source column
     const bool do_xfix = xghost && gsrc_e >= 2 && gsrc_e <= 64;
+    const int src_lane = do_xfix ? gsrc_e - 2 : lane;  // the lane holding the source column
     const bool wave_xfix = __any(do_xfix) != 0;
     const bool in_u1 = lane <= 62;  // lanes whose U1 is formed (lane 63 has no right face)
 
@@ -287,17 +288,17 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
     // (mirrored) value of their source column, in LDS and in the lane's own
     // registers
     auto publish = [&](int r, PrimE& q1) {
-        lds_put68(lds.w, r & 1, lane + 2, q1);
         if (wave_xfix) {
-            __syncthreads();
+            // ghost lanes take their source column's value straight from
+            // the lane holding it (a lane permute, no LDS round trip)
+            const PrimE sv = {__shfl(q1.r, src_lane), __shfl(q1.u, src_lane), __shfl(q1.v, src_lane),
+                              __shfl(q1.p, src_lane)};
             if (do_xfix) {
-                PrimE sv = lds_get68(lds.w, r & 1, gsrc_e);
-                if (gfx) sv.u = -sv.u;
                 q1 = sv;
+                if (gfx) q1.u = -q1.u;
             }
-            __syncthreads();
-            if (do_xfix) lds_put68(lds.w, r & 1, lane + 2, q1);
         }
+        lds_put68(lds.w, r & 1, lane + 2, q1);
     };
 
     // ================= consumer y-face G2(j+1/2) from rows j-1 .. j+2 (q1 =
