@@ -21,7 +21,7 @@ SO_PATH = os.path.join(HERE, "libgd2.so")
 
 BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
 EULER, MHD = 0, 1
-FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD = 1, 2, 3, 4
+FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
 FORWARD_EULER, SSPRK2 = 1, 2
 
 _lib = None

@@ -165,9 +165,11 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
     if (order == 2) {
         if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, LAYOUT, W, BASE>(A, st);
         if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
+        if (flux == GD2_FLUX_HLL) return launch_euler<2, GD2_FLUX_HLL, LAYOUT, W, BASE>(A, st);
     } else if (order == 1) {
         if (flux == GD2_FLUX_HLLC) return launch_euler<1, GD2_FLUX_HLLC, LAYOUT, W, BASE>(A, st);
         if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
+        if (flux == GD2_FLUX_HLL) return launch_euler<1, GD2_FLUX_HLL, LAYOUT, W, BASE>(A, st);
     }
     return GD2_ERR_UNSUPPORTED;
 }
@@ -193,16 +195,19 @@ static int dispatch_rk2(int order, int flux, const EulerArgs& A, hipStream_t st)
     if (order == 2) {
         if (flux == GD2_FLUX_HLLC) return launch_rk2<2, GD2_FLUX_HLLC, KEEP>(A, st);
         if (flux == GD2_FLUX_EXACT) return launch_rk2<2, GD2_FLUX_EXACT, KEEP>(A, st);
+        if (flux == GD2_FLUX_HLL) return launch_rk2<2, GD2_FLUX_HLL, KEEP>(A, st);
     } else if (order == 1) {
         if (flux == GD2_FLUX_HLLC) return launch_rk2<1, GD2_FLUX_HLLC, KEEP>(A, st);
         if (flux == GD2_FLUX_EXACT) return launch_rk2<1, GD2_FLUX_EXACT, KEEP>(A, st);
+        if (flux == GD2_FLUX_HLL) return launch_rk2<1, GD2_FLUX_HLL, KEEP>(A, st);
     }
     return GD2_ERR_UNSUPPORTED;
 }
 
 static bool scheme_ok(const gd2_scheme* sc) {
     if (!sc || (sc->order != 1 && sc->order != 2)) return false;
-    if (sc->physics == GD2_EULER) return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT;
+    if (sc->physics == GD2_EULER)
+        return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT || sc->flux == GD2_FLUX_HLL;
     if (sc->physics == GD2_MHD) return sc->flux == GD2_FLUX_HLLC_MHD || sc->flux == GD2_FLUX_HLLD;
     return false;
 }
@@ -418,7 +423,7 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
                      gd2_double2* u2, double* T, const double* T_do, int method, void* stream) {
     if (!grid_ok(g, true) || !s || !u || !s2 || !u2 || !T || !T_do) return GD2_ERR_ARG;
     if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2) return GD2_ERR_ARG;
-    if (method != 1 && method != 2) return GD2_ERR_UNSUPPORTED;
+    if (method != 0 && method != 1 && method != 2) return GD2_ERR_UNSUPPORTED;
     EulerArgs A;
     fill_common(A, g);
     A.s = reinterpret_cast<const double2*>(s);
@@ -430,7 +435,8 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     A.a = 0.0;
     A.b = 1.0;
     if (!A.diag) return GD2_ERR_HIP;
-    return dispatch_euler<kAoS, false, false>(order, method == 2 ? GD2_FLUX_EXACT : GD2_FLUX_HLLC, A, (hipStream_t)stream);
+    const int flux = method == 2 ? GD2_FLUX_EXACT : (method == 0 ? GD2_FLUX_HLL : GD2_FLUX_HLLC);
+    return dispatch_euler<kAoS, false, false>(order, flux, A, (hipStream_t)stream);
 }
 
 int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2, gd2_double2* u2,
@@ -796,8 +802,14 @@ static int ens_launch_stage(gd2_ensemble* e, int slot, hipStream_t st) {
     else if (order == 2 && flux == GD2_FLUX_EXACT)
         hipLaunchKernelGGL((ens_stage_kernel<2, GD2_FLUX_EXACT, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
                            e->dtiles, e->ntiles, slot);
-    else
+    else if (order == 1 && flux == GD2_FLUX_EXACT)
         hipLaunchKernelGGL((ens_stage_kernel<1, GD2_FLUX_EXACT, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    else if (order == 2)
+        hipLaunchKernelGGL((ens_stage_kernel<2, GD2_FLUX_HLL, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
+                           e->dtiles, e->ntiles, slot);
+    else
+        hipLaunchKernelGGL((ens_stage_kernel<1, GD2_FLUX_HLL, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
                            e->dtiles, e->ntiles, slot);
     GD2_HIP(hipGetLastError());
     return GD2_OK;

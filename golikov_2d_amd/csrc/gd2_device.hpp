@@ -267,6 +267,55 @@ __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, d
     return F;
 }
 
+// HLL flux of kernel.cu:1485-1609 in the face frame: wave-speed bounds
+// DL = min(unL, unR) - max(cL, cR), DR = max(unL, unR) + max(cL, cR) (sound
+// speed 0 where rho <= 0); the left flux when DL > 0, the right one when
+// DR < 0, else the HLL average (DR FL - DL FR + DL DR (UR - UL)) / (DR - DL).
+// smax returns max(|DL|, |DR|). GUARD = false: positive densities.
+template <bool GUARD = true>
+__device__ __forceinline__ FluxE hll_euler(double rL, double unL, double utL, double pL,
+                                           double rR, double unR, double utR, double pR,
+                                           double gamma, double inv_g1, double& smax) {
+    double cL, cR;
+    if constexpr (GUARD) {
+        cL = rL <= 0.0 ? 0.0 : sqrt_ratio(gamma * pL, rL);
+        cR = rR <= 0.0 ? 0.0 : sqrt_ratio(gamma * pR, rR);
+    } else {
+        cL = sqrt_ratio<false>(gamma * pL, rL);
+        cR = sqrt_ratio<false>(gamma * pR, rR);
+    }
+    const double cm = fmax(cL, cR);
+    const double DL = fmin(unL, unR) - cm;
+    const double DR = fmax(unL, unR) + cm;
+    smax = fmax(fabs(DL), fabs(DR));
+    const double eL = pL * inv_g1 + rL * (0.5 * (unL * unL + utL * utL));
+    const double eR = pR * inv_g1 + rR * (0.5 * (unR * unR + utR * utR));
+    const double mL = rL * unL, mR = rR * unR;
+    FluxE FL, FR;
+    FL.fr = mL;
+    FL.fn = mL * unL + pL;
+    FL.ft = mL * utL;
+    FL.fe = (eL + pL) * unL;
+    FR.fr = mR;
+    FR.fn = mR * unR + pR;
+    FR.ft = mR * utR;
+    FR.fe = (eR + pR) * unR;
+    const double den = DR - DL;
+    const double inv = den > 0.0 ? fast_rcp(den) : 0.0;  // den = 0 only for two states at rest with c = 0
+    const double dd = DL * DR;
+    FluxE F;
+    F.fr = (DR * FL.fr - DL * FR.fr + dd * (rR - rL)) * inv;
+    F.fn = (DR * FL.fn - DL * FR.fn + dd * (mR - mL)) * inv;
+    F.ft = (DR * FL.ft - DL * FR.ft + dd * (rR * utR - rL * utL)) * inv;
+    F.fe = (DR * FL.fe - DL * FR.fe + dd * (eR - eL)) * inv;
+    const bool left = DL > 0.0, right = DR < 0.0;
+    F.fr = left ? FL.fr : (right ? FR.fr : F.fr);
+    F.fn = left ? FL.fn : (right ? FR.fn : F.fn);
+    F.ft = left ? FL.ft : (right ? FR.ft : F.ft);
+    F.fe = left ? FL.fe : (right ? FR.fe : F.fe);
+    return F;
+}
+
 // ---------------------------------------------------------------------------
 // Exact Riemann solver (Godunov_Solver_Alexashov, kernel.cu:214-656; newton
 // 721-766; devtwo 768-835; lev 837-851) with gamma at run time. The

@@ -148,6 +148,8 @@ __device__ __forceinline__ FluxE face_flux(const EulerArgs& A, double rL, double
                                            double rR, double unR, double utR, double pR, double& smax) {
     if constexpr (FLUX == GD2_FLUX_EXACT)
         return exact_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, smax, A.diag);
+    else if constexpr (FLUX == GD2_FLUX_HLL)
+        return hll_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
     else
         return hllc_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
 }

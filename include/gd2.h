@@ -70,8 +70,9 @@ typedef struct gd2_grid {
  * ---------------------------------------------------------------------- */
 
 /* Euler, first order. Replaces Ker_Dekard (kernel.cu:1736).
- * method 1 = HLLC (HLLC_Aleksashov, kernel.cu:853), 2 = exact Riemann
- * (Godunov_Solver_Alexashov, kernel.cu:214). Face time step radius as in the
+ * method 0 = HLL (kernel.cu:1485), 1 = HLLC (HLLC_Aleksashov, kernel.cu:853),
+ * 2 = exact Riemann (Godunov_Solver_Alexashov, kernel.cu:214; the reference's
+ * Ker_Dekard has no method 2). Face time step radius as in the
  * reference: dy on x-faces, dx on y-faces (kernel.cu:1798-1801). */
 int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
                    gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
@@ -106,12 +107,12 @@ int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);
  * cell-centred CFL step computed on the device.
  * ---------------------------------------------------------------------- */
 enum gd2_physics { GD2_EULER = 0, GD2_MHD = 1 };
-enum gd2_flux { GD2_FLUX_HLLC = 1, GD2_FLUX_EXACT = 2, GD2_FLUX_HLLC_MHD = 3, GD2_FLUX_HLLD = 4 };
+enum gd2_flux { GD2_FLUX_HLLC = 1, GD2_FLUX_EXACT = 2, GD2_FLUX_HLLC_MHD = 3, GD2_FLUX_HLLD = 4, GD2_FLUX_HLL = 5 };
 enum gd2_integrator { GD2_FORWARD_EULER = 1, GD2_SSPRK2 = 2 };
 
 typedef struct gd2_scheme {
     int physics;      /* enum gd2_physics */
-    int flux;         /* enum gd2_flux (HLLC/EXACT for Euler, HLLC_MHD/HLLD for MHD) */
+    int flux;         /* enum gd2_flux (HLLC/EXACT/HLL for Euler, HLLC_MHD/HLLD for MHD) */
     int order;        /* 1 = piecewise constant, 2 = minmod MUSCL on primitives */
     int integrator;   /* enum gd2_integrator */
     double cfl;       /* dt = cfl / max_cells((|u|+c)/dx + (|v|+c)/dy) */

@@ -95,6 +95,58 @@ void o_linear2(double x1, double t1, double x2, double t2, double x3, double t3,
 }
 
 /* ------------------------------------------------------------------------ */
+/* HLL (Euler), kernel.cu:1485-1609: sound speeds 0 where rho <= 0, bounds   */
+/* D_L = min(Vn_L, Vn_R) - max(c_L, c_R), D_R = max(Vn_L, Vn_R) +            */
+/* max(c_L, c_R); the left flux when D_L > Omega (= 0), the right one when   */
+/* D_R < Omega, else the HLL average; fluxes in the lab frame along n.       */
+/* Returns the face time step krit rad / max(|D_L|, |D_R|).                  */
+/* ------------------------------------------------------------------------ */
+double o_hll(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+             double n1, double n2, o_d2* Ps, o_d2* Pu, double rad)
+{
+    const double ro_L = Ls.x, p_L = Ls.y, v1_L = Lu.x, v2_L = Lu.y;
+    const double ro_R = Rs.x, p_R = Rs.y, v1_R = Ru.x, v2_R = Ru.y;
+    const double Vkv_L = v1_L * v1_L + v2_L * v2_L;
+    const double Vkv_R = v1_R * v1_R + v2_R * v2_R;
+    const double c_L = ro_L <= 0 ? 0.0 : sqrt(gamma * p_L / ro_L);
+    const double c_R = ro_R <= 0 ? 0.0 : sqrt(gamma * p_R / ro_R);
+    const double e_L = p_L / (gamma - 1.0) + ro_L * Vkv_L / 2.0;
+    const double e_R = p_R / (gamma - 1.0) + ro_R * Vkv_R / 2.0;
+    const double Vn_L = v1_L * n1 + v2_L * n2;
+    const double Vn_R = v1_R * n1 + v2_R * n2;
+    const double D_L = fmin(Vn_L, Vn_R) - fmax(c_L, c_R);
+    const double D_R = fmax(Vn_L, Vn_R) + fmax(c_L, c_R);
+    double t = 10000000;
+    t = fmin(t, krit * rad / fmax(fabs(D_L), fabs(D_R)));
+    /* lab-frame physical fluxes projected on n */
+    const double fl_1 = ro_L * v1_L * n1 + ro_L * v2_L * n2;
+    const double fl_2 = (ro_L * v1_L * v1_L + p_L) * n1 + ro_L * v1_L * v2_L * n2;
+    const double fl_3 = ro_L * v1_L * v2_L * n1 + (ro_L * v2_L * v2_L + p_L) * n2;
+    const double fl_5 = (e_L + p_L) * v1_L * n1 + (e_L + p_L) * v2_L * n2;
+    if (D_L > 0.0) {
+        Ps->x = fl_1; Pu->x = fl_2; Pu->y = fl_3; Ps->y = fl_5;
+        return t;
+    }
+    const double fr_1 = ro_R * v1_R * n1 + ro_R * v2_R * n2;
+    const double fr_2 = (ro_R * v1_R * v1_R + p_R) * n1 + ro_R * v1_R * v2_R * n2;
+    const double fr_3 = ro_R * v1_R * v2_R * n1 + (ro_R * v2_R * v2_R + p_R) * n2;
+    const double fr_5 = (e_R + p_R) * v1_R * n1 + (e_R + p_R) * v2_R * n2;
+    if (D_R < 0.0) {
+        Ps->x = fr_1; Pu->x = fr_2; Pu->y = fr_3; Ps->y = fr_5;
+        return t;
+    }
+    const double dq1 = ro_R - ro_L;
+    const double dq2 = ro_R * v1_R - ro_L * v1_L;
+    const double dq3 = ro_R * v2_R - ro_L * v2_L;
+    const double dq5 = e_R - e_L;
+    Ps->x = (D_R * fl_1 - D_L * fr_1 + D_L * D_R * dq1) / (D_R - D_L);
+    Pu->x = (D_R * fl_2 - D_L * fr_2 + D_L * D_R * dq2) / (D_R - D_L);
+    Pu->y = (D_R * fl_3 - D_L * fr_3 + D_L * D_R * dq3) / (D_R - D_L);
+    Ps->y = (D_R * fl_5 - D_L * fr_5 + D_L * D_R * dq5) / (D_R - D_L);
+    return t;
+}
+
+/* ------------------------------------------------------------------------ */
 /* HLLC (Euler): HLLC_Aleksashov, kernel.cu:853-1165, called with B = 0.     */
 /* With B == 0 the frame of kernel.cu:915-945 is (n, e_z, t) with            */
 /* t = (-n2, n1), every magnetic term is an exact zero, the fast speed       */
@@ -858,6 +910,7 @@ static inline double euler_flux(const o_grid* g, int method, const prim* L, cons
 {
     o_d2 Ls = {L->r, L->p}, Lu = {L->u, L->v}, Rs = {R->r, R->p}, Ru = {R->u, R->v};
     if (method == 2) return o_godunov(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    if (method == 0) return o_hll(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     return o_hllc(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
 }
 
@@ -898,7 +951,7 @@ static int is_frozen(const o_grid* g, int i, int j)
 static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u, o_d2* s2, o_d2* u2,
                           double* T, const double* T_do, int method)
 {
-    if (method != 1 && method != 2) return -1;
+    if (method != 0 && method != 1 && method != 2) return -1;
     src_t S = {g, 0, 0, s, u, NULL, NULL};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
@@ -1073,7 +1126,7 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
             double* Uout, double a, double b, double dt, double* wmax)
 {
     const int mhd = sc->physics == O_PHYS_MHD;
-    if (!mhd && sc->flux != O_FLUX_HLLC && sc->flux != O_FLUX_EXACT) return -1;
+    if (!mhd && sc->flux != O_FLUX_HLLC && sc->flux != O_FLUX_EXACT && sc->flux != O_FLUX_HLL) return -1;
     if (mhd && sc->flux != O_FLUX_HLLC_MHD && sc->flux != O_FLUX_HLLD) return -1;
     for (int s = 0; s < 4; ++s)
         if (g->bc[s] == O_BC_FROZEN) return -1;
@@ -1082,7 +1135,8 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
     const double dx = g->dx, dy = g->dy, gam = g->gamma, dV = dx * dy;
     const long plane = (long)g->nx * g->ny;
     const int nf = fields_of(sc->physics);
-    const int method = mhd ? (sc->flux == O_FLUX_HLLD ? 3 : 2) : (sc->flux == O_FLUX_EXACT ? 2 : 1);
+    const int method = mhd ? (sc->flux == O_FLUX_HLLD ? 3 : 2)
+                           : (sc->flux == O_FLUX_EXACT ? 2 : (sc->flux == O_FLUX_HLL ? 0 : 1));
     double wm = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : wm)
     for (int j = 0; j < g->ny; ++j) {

@@ -35,7 +35,7 @@ typedef struct {
 
 /* Scheme selection for the native (SoA conserved) path. */
 enum { O_PHYS_EULER = 0, O_PHYS_MHD = 1 };
-enum { O_FLUX_HLLC = 1, O_FLUX_EXACT = 2, O_FLUX_HLLC_MHD = 3, O_FLUX_HLLD = 4 };
+enum { O_FLUX_HLLC = 1, O_FLUX_EXACT = 2, O_FLUX_HLLC_MHD = 3, O_FLUX_HLLD = 4, O_FLUX_HLL = 5 };
 
 typedef struct {
     int physics;         /* O_PHYS_* */
@@ -59,6 +59,8 @@ void   o_linear2(double x1, double t1, double x2, double t2, double x3, double t
                  double y1, double y2, double* A, double* B);
 
 /* ---- interface fluxes; each returns the face time step krit*rad/S ---- */
+double o_hll(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+             double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
 double o_hllc(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
               double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
 double o_godunov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,

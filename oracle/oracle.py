@@ -17,7 +17,7 @@ _LIB = None
 
 BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
 PHYS_EULER, PHYS_MHD = 0, 1
-FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD = 1, 2, 3, 4
+FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
 
 
 class D2(C.Structure):
@@ -66,7 +66,7 @@ def lib():
         L.o_linear.argtypes = [C.c_double] * 7
         L.o_linear2.restype = None
         L.o_linear2.argtypes = [C.c_double] * 8 + [dp, dp]
-        for fn in (L.o_hllc, L.o_godunov):
+        for fn in (L.o_hllc, L.o_godunov, L.o_hll):
             fn.restype = C.c_double
             fn.argtypes = [C.c_double, C.c_double, D2, D2, D2, D2, C.c_double, C.c_double,
                            C.POINTER(D2), C.POINTER(D2), C.c_double]
@@ -141,7 +141,7 @@ def linear2(x1, t1, x2, t2, x3, t3, y1, y2):
 def euler_flux(kind, gamma, krit, L, R, n, rad=1.0):
     """L, R = (rho, p, u, v). Returns (F_rho, F_E, F_mx, F_my), dt_face."""
     Ps, Pu = D2(), D2()
-    fn = lib().o_hllc if kind == "hllc" else lib().o_godunov
+    fn = {"hllc": lib().o_hllc, "hll": lib().o_hll}.get(kind, lib().o_godunov)
     t = fn(gamma, krit, D2(L[0], L[1]), D2(L[2], L[3]), D2(R[0], R[1]), D2(R[2], R[3]),
            float(n[0]), float(n[1]), C.byref(Ps), C.byref(Pu), rad)
     return (Ps.x, Ps.y, Pu.x, Pu.y), t

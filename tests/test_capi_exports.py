@@ -45,7 +45,7 @@ def test_pure_host_entry_points():
     x = ctypes.c_void_p(16)
     assert L.gd2_stage(ctypes.byref(bad), ctypes.byref(sc), x, None, ctypes.c_void_p(32), 0.0, 1.0, 0.0, None,
                        None, None) == -1
-    assert L.gd2_ker_dekard(ctypes.byref(g), x, x, ctypes.c_void_p(32), ctypes.c_void_p(48), x, x, 0, None) == -2
+    assert L.gd2_ker_dekard(ctypes.byref(g), x, x, ctypes.c_void_p(32), ctypes.c_void_p(48), x, x, 7, None) == -2
 
 
 @pytest.mark.skipif(not os.path.exists(api.SO_PATH), reason="libgd2.so not built")

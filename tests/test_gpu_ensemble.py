@@ -27,10 +27,11 @@ def make_problems(count, lo, hi, seed):
     return grids, ogrids, states
 
 
-def test_ensemble_bitwise_equals_individual_runs():
-    say("ensemble vs individual")
+@pytest.mark.parametrize("flux", [api.FLUX_HLLC, api.FLUX_HLL])
+def test_ensemble_bitwise_equals_individual_runs(flux):
+    say(f"ensemble vs individual flux={flux}")
     grids, _, states = make_problems(12, 16, 200, seed=3)
-    sc = api.make_scheme()
+    sc = api.make_scheme(api.EULER, flux)
     ens_U = [to_dev(U) for U in states]
     ind_U = [to_dev(U) for U in states]
     e = api.Ensemble(grids, sc, ens_U)

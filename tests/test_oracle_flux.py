@@ -315,3 +315,56 @@ def test_hlld_rejects_unknown_metod():
     P, t = O.mhd_flux(5 / 3, 0.25, (1, 1, 0, 0, 0, 0, 0, 0), (1, 1, 0, 0, 0, 0, 0, 0), (1.0, 0, 0), 1)
     assert t == -1.0 and np.all(P == 0)
     assert O.diag()["bad_branch"] == 1
+
+
+# ------------------------------------------------------------------ HLL
+
+def _hll_frame(L, n):
+    """(rho, p, u, v) -> (rho, p, un, ut) with t = (-n2, n1)."""
+    return (L[0], L[1], L[2] * n[0] + L[3] * n[1], -L[2] * n[1] + L[3] * n[0])
+
+
+def _hll_lab(F, n):
+    """face-frame (mass, mom n, mom t, energy) -> the oracle's (F_rho, F_E, F_mx, F_my)."""
+    fx = F[1] * n[0] - F[2] * n[1]
+    fy = F[1] * n[1] + F[2] * n[0]
+    return (F[0], F[3], fx, fy)
+
+
+def test_hll_matches_textbook_hll():
+    rng = random.Random(11)
+    gamma = 1.4
+    for _ in range(3000):
+        L = (rng.uniform(0.1, 3), rng.uniform(0.1, 3), rng.uniform(-3, 3), rng.uniform(-3, 3))
+        R = (rng.uniform(0.1, 3), rng.uniform(0.1, 3), rng.uniform(-3, 3), rng.uniform(-3, 3))
+        for n in NORMALS:
+            F, _ = O.euler_flux("hll", gamma, 0.25, L, R, n)
+            ref = _hll_lab(TB.hll(gamma, _hll_frame(L, n), _hll_frame(R, n)), n)
+            assert rel(F, ref) < 1e-13, (L, R, n)
+
+
+def test_hll_consistency_upwinding_and_timestep():
+    for n in NORMALS:
+        q = (1.3, 0.7, 0.4, -0.2)
+        F, _ = O.euler_flux("hll", 1.4, 0.25, q, q, n)
+        ref = _hll_lab(TB.euler_phys_flux(1.4, q[0], q[1], *_hll_frame(q, n)[2:]), n)
+        assert rel(F, ref) < 1e-15
+    # supersonic to the right: the left state's physical flux
+    L, R = (1.0, 1.0, 5.0, 0.3), (0.5, 0.4, 4.0, -0.1)
+    F, t = O.euler_flux("hll", 1.4, 0.25, L, R, (1.0, 0.0))
+    assert F == tuple(_hll_lab(TB.euler_phys_flux(1.4, 1.0, 1.0, 5.0, 0.3), (1.0, 0.0)))
+    # face time step krit rad / max(|D_L|, |D_R|)
+    cL, cR = math.sqrt(1.4), math.sqrt(1.4 * 0.4 / 0.5)
+    DL, DR = 4.0 - max(cL, cR), 5.0 + max(cL, cR)
+    _, t = O.euler_flux("hll", 1.4, 0.4, L, R, (1.0, 0.0), rad=0.01)
+    assert abs(t - 0.4 * 0.01 / max(abs(DL), abs(DR))) <= 1e-16
+
+
+def test_hll_mirror_invariance():
+    rng = random.Random(12)
+    for _ in range(500):
+        L = (rng.uniform(0.1, 3), rng.uniform(0.1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1))
+        R = (rng.uniform(0.1, 3), rng.uniform(0.1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1))
+        F, _ = O.euler_flux("hll", 1.4, 0.25, L, R, (1.0, 0.0))
+        Fm, _ = O.euler_flux("hll", 1.4, 0.25, R, L, (-1.0, 0.0))
+        assert rel(F, tuple(-x for x in Fm)) < 1e-14

@@ -7,7 +7,10 @@ These follow published methods, not the reference source:
   * HLLC: Toro ch. 10.4, star state eq. (10.39), wave speeds from the Davis
     estimate S_L = min(uL - cL, uR - cR), S_R = max(uL + cL, uR + cR);
   * HLLD: Miyoshi & Kusano, J. Comput. Phys. 208 (2005) 315, eqs. (38)-(63),
-    Heaviside-Lorentz units, constant normal field.
+    Heaviside-Lorentz units, constant normal field;
+  * HLL: Harten, Lax & van Leer (SIAM Rev. 25, 1983) as in Toro eq. (10.21),
+    with the bounds S_L = min(uL, uR) - max(cL, cR), S_R = max(uL, uR) +
+    max(cL, cR) (Davis 1988, his simplest estimate).
 All in pure Python floats (small inputs only).
 """
 from __future__ import annotations
@@ -20,6 +23,23 @@ import math
 def euler_phys_flux(gamma, rho, p, un, ut):
     E = p / (gamma - 1.0) + 0.5 * rho * (un * un + ut * ut)
     return (rho * un, rho * un * un + p, rho * un * ut, un * (E + p))
+
+
+def hll(gamma, L, R):
+    """L, R = (rho, p, un, ut). HLL flux (mass, normal mom., tangential mom., energy)."""
+    rL, pL, uL, tL = L
+    rR, pR, uR, tR = R
+    cL, cR = math.sqrt(gamma * pL / rL), math.sqrt(gamma * pR / rR)
+    SL = min(uL, uR) - max(cL, cR)
+    SR = max(uL, uR) + max(cL, cR)
+    FL, FR = euler_phys_flux(gamma, rL, pL, uL, tL), euler_phys_flux(gamma, rR, pR, uR, tR)
+    if SL >= 0:
+        return FL
+    if SR <= 0:
+        return FR
+    UL = (rL, rL * uL, rL * tL, pL / (gamma - 1) + 0.5 * rL * (uL * uL + tL * tL))
+    UR = (rR, rR * uR, rR * tR, pR / (gamma - 1) + 0.5 * rR * (uR * uR + tR * tR))
+    return tuple((SR * fl - SL * fr + SL * SR * (ur - ul)) / (SR - SL) for fl, fr, ul, ur in zip(FL, FR, UL, UR))
 
 
 def toro_exact(gamma, L, R, tol=1e-15):
