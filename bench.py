@@ -44,7 +44,7 @@ def parse():
                     help="skip the batched-ensemble, adjoint, MHD and Sod legs")
     ap.add_argument("--only-extras", action="store_true",
                     help="run only the batched-ensemble, adjoint, MHD and Sod legs (for profiling them)")
-    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod"], default=None,
+    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod", "reflayout"], default=None,
                     help="run only one of those legs (for profiling it)")
     return ap.parse_args()
 
@@ -278,6 +278,80 @@ def sod_godunov(n=200, with_cpu=True):
     }
 
 
+def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
+    """The drop-in entry points driven the way the reference's step loop drives
+    its kernels: one reference-layout step (AoS primitives, lagged face time
+    step) plus gd2_funk_time per step, ping-ponging the state buffers.
+    MHD: gd2_kernel_tvd (MUSCL, HLLD = method 3) on the Orszag-Tang state;
+    Euler: gd2_kernel_tvd_old (MUSCL, HLLC = method 1) on the four-quadrant
+    state. Algorithmic bytes per cell-update: the AoS state read and written
+    once (MHD 56 + 56 B, Euler 32 + 32 B)."""
+    import torch
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    out = {}
+    for kind, n in (("mhd_kernel_tvd", n_mhd), ("euler_kernel_tvd_old", n_euler)):
+        mhd = kind.startswith("mhd")
+        g = api.make_grid(n, n, gamma=5.0 / 3.0 if mhd else 1.4,
+                          bc=(api.BC_PERIODIC,) * 4 if mhd else (api.BC_TRANSMISSIVE,) * 4)
+        U0 = PB.orszag_tang(g, seed=seed, amp=1e-3) if mhd else PB.four_quadrant(g, seed=seed)
+        phys = api.MHD if mhd else api.EULER
+        s, u, b = api.cons_to_prim(g, phys, torch.from_numpy(U0).to(dev))
+        s2, u2 = torch.empty_like(s), torch.empty_like(u)
+        b2 = torch.empty_like(b) if mhd else None
+        T = torch.tensor([1e7], dtype=torch.float64, device=dev)
+        T_do = torch.tensor([1e-8], dtype=torch.float64, device=dev)  # the reference's first step
+        TT = torch.zeros(1, dtype=torch.float64, device=dev)
+        i = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def step():
+            nonlocal s, u, b, s2, u2, b2
+            if mhd:
+                api.kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, 0, 3)
+                b, b2 = b2, b
+            else:
+                api.kernel_tvd_old(g, s, u, s2, u2, T, T_do, 1)
+            s, s2 = s2, s
+            u, u2 = u2, u
+            api.funk_time(T, T_do, TT, i)
+
+        for _ in range(5):
+            step()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        bpc = 112 if mhd else 64
+        r = {"value": round(n * n / (ms / 1e3) / 1e9, 4), "unit": "Gcell-updates/s", "ms_per_step": round(ms, 4),
+             "bytes_per_cell_update": bpc, "hbm_frac": round(bpc * n * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+             "finite": bool(torch.isfinite(s).all()),
+             "workload": (f"gd2_kernel_tvd (MUSCL + HLLD, method 3) + gd2_funk_time, Orszag-Tang {n}x{n}, periodic"
+                          if mhd else f"gd2_kernel_tvd_old (MUSCL + HLLC, method 1) + gd2_funk_time, four-quadrant "
+                          f"{n}x{n}, transmissive") + f", {steps} steps after 5"}
+        if with_cpu:
+            from oracle import oracle as O
+            og = O.make_grid(n, n, gamma=g.gamma, bc=tuple(g.bc))
+            sh, uh = s.cpu().numpy().copy(), u.cpu().numpy().copy()
+            Tdo = float(T_do.item())
+            t0 = time.perf_counter()
+            if mhd:
+                O.ref_mhd_step("kernel_tvd", og, sh, uh, b.cpu().numpy().copy(), 1e7, Tdo, 0, 3)
+            else:
+                O.ref_euler_step("kernel_tvd_old", og, sh, uh, 1e7, Tdo, 1)
+            dt = time.perf_counter() - t0
+            r["cpu_baseline"] = {"value": round(n * n / dt / 1e9, 6), "unit": "Gcell-updates/s",
+                                 "cores": O.num_threads(), "kind": "port",
+                                 "sample": "oracle (C, OpenMP): one step of the same reference-layout workload"}
+        out[kind] = r
+    return out
+
+
 def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
     """Differentiable rollout (LibTorch autograd Function): backward (recompute
     from checkpoints + stage VJPs) against the forward rollout, both timed."""
@@ -351,14 +425,16 @@ def main():
             copy_reference(args.copy_ref_mib)
         legs = {"batched": lambda: batched_vs_serial(args.seed), "adjoint": lambda: adjoint_vs_forward(seed=args.seed),
                 "mhd": lambda: mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
-                "sod": lambda: sod_godunov(with_cpu=not args.no_cpu)}
+                "sod": lambda: sod_godunov(with_cpu=not args.no_cpu),
+                "reflayout": lambda: reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}
         print(json.dumps({args.only_leg: legs[args.only_leg]()}), flush=True)
         return
     if args.only_extras:
         print(json.dumps({"batched_vs_serial": batched_vs_serial(args.seed),
                           "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
                           "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
-                          "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu)}), flush=True)
+                          "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
+                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}), flush=True)
         return
 
     n = args.n
@@ -447,7 +523,8 @@ def main():
         extras = {"batched_vs_serial": batched_vs_serial(args.seed),
                   "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
                   "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
-                  "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu)}
+                  "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
+                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}
 
     out = {
         "metric": METRIC,
