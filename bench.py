@@ -292,13 +292,25 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
 
     dev = torch.device("cuda:0")
     out = {}
-    for kind, n in (("mhd_kernel_tvd", n_mhd), ("euler_kernel_tvd_old", n_euler)):
+    for kind, n in (("mhd_kernel_tvd", n_mhd), ("euler_kernel_tvd_old", n_euler), ("mhd_wind_kernel_tvd", 0)):
         mhd = kind.startswith("mhd")
-        g = api.make_grid(n, n, gamma=5.0 / 3.0 if mhd else 1.4,
-                          bc=(api.BC_PERIODIC,) * 4 if mhd else (api.BC_TRANSMISSIVE,) * 4)
-        U0 = PB.orszag_tang(g, seed=seed, amp=1e-3) if mhd else PB.four_quadrant(g, seed=seed)
-        phys = api.MHD if mhd else api.EULER
-        s, u, b = api.cons_to_prim(g, phys, torch.from_numpy(U0).to(dev))
+        wind = kind == "mhd_wind_kernel_tvd"
+        method = 2 if wind else 3
+        if wind:
+            # the reference's production run: 3226 x 1792, symmetry / outflow /
+            # axis / inflow sides, frozen inner sphere, axisymmetric sources,
+            # HLLC-MHD (method 2), stellar-wind initial state
+            nx, ny = PB.WIND_NX, PB.WIND_NY
+            g = api.make_grid(nx, ny, **PB.wind_grid_args())
+            sh0, uh0, bh0 = PB.stellar_wind_prim(g)
+            s, u, b = (torch.from_numpy(a).to(dev) for a in (sh0, uh0, bh0))
+        else:
+            nx = ny = n
+            g = api.make_grid(n, n, gamma=5.0 / 3.0 if mhd else 1.4,
+                              bc=(api.BC_PERIODIC,) * 4 if mhd else (api.BC_TRANSMISSIVE,) * 4)
+            U0 = PB.orszag_tang(g, seed=seed, amp=1e-3) if mhd else PB.four_quadrant(g, seed=seed)
+            phys = api.MHD if mhd else api.EULER
+            s, u, b = api.cons_to_prim(g, phys, torch.from_numpy(U0).to(dev))
         s2, u2 = torch.empty_like(s), torch.empty_like(u)
         b2 = torch.empty_like(b) if mhd else None
         T = torch.tensor([1e7], dtype=torch.float64, device=dev)
@@ -309,7 +321,7 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
         def step():
             nonlocal s, u, b, s2, u2, b2
             if mhd:
-                api.kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, 0, 3)
+                api.kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, 0, method)
                 b, b2 = b2, b
             else:
                 api.kernel_tvd_old(g, s, u, s2, u2, T, T_do, 1)
@@ -328,24 +340,32 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
         bpc = 112 if mhd else 64
-        r = {"value": round(n * n / (ms / 1e3) / 1e9, 4), "unit": "Gcell-updates/s", "ms_per_step": round(ms, 4),
-             "bytes_per_cell_update": bpc, "hbm_frac": round(bpc * n * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-             "finite": bool(torch.isfinite(s).all()),
-             "workload": (f"gd2_kernel_tvd (MUSCL + HLLD, method 3) + gd2_funk_time, Orszag-Tang {n}x{n}, periodic"
-                          if mhd else f"gd2_kernel_tvd_old (MUSCL + HLLC, method 1) + gd2_funk_time, four-quadrant "
-                          f"{n}x{n}, transmissive") + f", {steps} steps after 5"}
+        cells = nx * ny
+        if wind:
+            work = (f"gd2_kernel_tvd (MUSCL + HLLC-MHD, method 2) + gd2_funk_time, stellar wind {nx}x{ny}, "
+                    f"symmetry/outflow/axis/inflow sides, frozen sphere r < {PB.WIND_DDIST}, axisymmetric")
+        elif mhd:
+            work = f"gd2_kernel_tvd (MUSCL + HLLD, method 3) + gd2_funk_time, Orszag-Tang {n}x{n}, periodic"
+        else:
+            work = f"gd2_kernel_tvd_old (MUSCL + HLLC, method 1) + gd2_funk_time, four-quadrant {n}x{n}, transmissive"
+        r = {"value": round(cells / (ms / 1e3) / 1e9, 4), "unit": "Gcell-updates/s", "ms_per_step": round(ms, 4),
+             "bytes_per_cell_update": bpc, "hbm_frac": round(bpc * cells / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+             "finite": bool(torch.isfinite(s).all()), "workload": work + f", {steps} steps after 5"}
         if with_cpu:
             from oracle import oracle as O
-            og = O.make_grid(n, n, gamma=g.gamma, bc=tuple(g.bc))
+            if wind:
+                og = O.make_grid(nx, ny, **PB.wind_grid_args())
+            else:
+                og = O.make_grid(n, n, gamma=g.gamma, bc=tuple(g.bc))
             sh, uh = s.cpu().numpy().copy(), u.cpu().numpy().copy()
             Tdo = float(T_do.item())
             t0 = time.perf_counter()
             if mhd:
-                O.ref_mhd_step("kernel_tvd", og, sh, uh, b.cpu().numpy().copy(), 1e7, Tdo, 0, 3)
+                O.ref_mhd_step("kernel_tvd", og, sh, uh, b.cpu().numpy().copy(), 1e7, Tdo, 0, method)
             else:
                 O.ref_euler_step("kernel_tvd_old", og, sh, uh, 1e7, Tdo, 1)
             dt = time.perf_counter() - t0
-            r["cpu_baseline"] = {"value": round(n * n / dt / 1e9, 6), "unit": "Gcell-updates/s",
+            r["cpu_baseline"] = {"value": round(cells / dt / 1e9, 6), "unit": "Gcell-updates/s",
                                  "cores": O.num_threads(), "kind": "port",
                                  "sample": "oracle (C, OpenMP): one step of the same reference-layout workload"}
         out[kind] = r

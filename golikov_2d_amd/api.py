@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SO_PATH = os.path.join(HERE, "libgd2.so")
 
 BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
+BC_SYMMETRY, BC_AXIS, BC_OUTFLOW, BC_INFLOW = 4, 5, 6, 7
 EULER, MHD = 0, 1
 FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
 FORWARD_EULER, SSPRK2 = 1, 2
@@ -30,7 +31,7 @@ _lib = None
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("x0", C.c_double), ("y0", C.c_double), ("gamma", C.c_double),
-                ("krit", C.c_double), ("bc", C.c_int * 4)]
+                ("krit", C.c_double), ("bc", C.c_int * 4), ("r_frozen", C.c_double), ("axisym", C.c_int)]
 
 
 class Scheme(C.Structure):
@@ -48,13 +49,15 @@ class GD2Error(RuntimeError):
 
 
 def make_grid(nx, ny, *, dx=None, dy=None, x0=None, y0=None, gamma=1.4, krit=0.25,
-              bc=(BC_TRANSMISSIVE,) * 4) -> Grid:
+              bc=(BC_TRANSMISSIVE,) * 4, r_frozen=0.0, axisym=False) -> Grid:
     """Uniform grid on [0, nx*dx] x [0, ny*dy]; defaults to the unit square."""
     dx = 1.0 / nx if dx is None else dx
     dy = 1.0 / ny if dy is None else dy
     g = Grid(nx, ny, dx, dy, dx / 2.0 if x0 is None else x0, dy / 2.0 if y0 is None else y0, gamma, krit)
     for i in range(4):
         g.bc[i] = bc[i]
+    g.r_frozen = float(r_frozen)
+    g.axisym = 1 if axisym else 0
     return g
 
 

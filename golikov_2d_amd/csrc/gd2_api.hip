@@ -48,14 +48,23 @@ static DeviceDiag* diag_ptr() {
     return p;
 }
 
-static bool grid_ok(const gd2_grid* g, bool allow_frozen) {
+// allow_prod: the production options of the MHD reference-layout steps
+// (side kinds SYMMETRY / AXIS / OUTFLOW / INFLOW, each on its own side, the
+// frozen sphere and the axisymmetric sources); everywhere else they are
+// rejected rather than ignored.
+static bool grid_ok(const gd2_grid* g, bool allow_frozen, bool allow_prod = false) {
     if (!g || g->nx < 1 || g->ny < 1) return false;
     if (!(g->dx > 0.0) || !(g->dy > 0.0) || !(g->gamma > 1.0)) return false;
     if ((long long)g->nx * g->ny > (1ll << 40)) return false;
+    static const int own_side[4] = {GD2_BC_SYMMETRY, GD2_BC_OUTFLOW, GD2_BC_AXIS, GD2_BC_INFLOW};
     for (int s = 0; s < 4; ++s) {
-        if (g->bc[s] < GD2_BC_TRANSMISSIVE || g->bc[s] > GD2_BC_FROZEN) return false;
+        if (g->bc[s] < GD2_BC_TRANSMISSIVE || g->bc[s] > GD2_BC_INFLOW) return false;
         if (g->bc[s] == GD2_BC_FROZEN && !allow_frozen) return false;
+        if (g->bc[s] >= GD2_BC_SYMMETRY && (!allow_prod || g->bc[s] != own_side[s])) return false;
     }
+    if (!allow_prod && (g->r_frozen != 0.0 || g->axisym != 0)) return false;
+    if (!(g->r_frozen >= 0.0)) return false;
+    if (g->axisym && !(g->y0 > 0.0)) return false;
     // a periodic side needs a periodic opposite side
     if ((g->bc[0] == GD2_BC_PERIODIC) != (g->bc[1] == GD2_BC_PERIODIC)) return false;
     if ((g->bc[2] == GD2_BC_PERIODIC) != (g->bc[3] == GD2_BC_PERIODIC)) return false;
@@ -452,7 +461,7 @@ int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double
 int gd2_add2(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b, gd2_double2* s2,
              gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i, int method, void* stream) {
     (void)i;
-    if (!grid_ok(g, true)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, true)) return GD2_ERR_ARG;
     return mhd_ref_step(g, 1, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
 }
 
@@ -460,7 +469,7 @@ int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
                    gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i,
                    int method, void* stream) {
     (void)i;
-    if (!grid_ok(g, true)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, true)) return GD2_ERR_ARG;
     return mhd_ref_step(g, 2, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
 }
 
@@ -473,7 +482,7 @@ int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream) {
 
 int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
                      const gd2_double3* b, double* U, void* stream) {
-    if (!grid_ok(g, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
     if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
     const long long n = (long long)g->nx * g->ny;
     hipLaunchKernelGGL(prim_to_cons_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
@@ -485,7 +494,7 @@ int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const
 
 int gd2_cons_to_prim(const gd2_grid* g, int physics, const double* U, gd2_double2* s, gd2_double2* u,
                      gd2_double3* b, void* stream) {
-    if (!grid_ok(g, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
     if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
     const long long n = (long long)g->nx * g->ny;
     hipLaunchKernelGGL(cons_to_prim_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, U,

@@ -219,6 +219,11 @@ struct MhdArgs {
     unsigned long long* tmin_bits;
     DeviceDiag* diag;
     int tiles_x, rows;
+    // production options of the reference-layout steps (kernel.cu:1918,
+    // 2137-2190): inner frozen sphere, axisymmetric sources, INFLOW ghosts
+    // with B = 0 (Kernel_TVD) instead of the edge cell's (add2)
+    double x0, y0, r_frozen, inv_gamma;
+    int axisym, inflow_b_zero;
 };
 
 // Primitives of cell k of a SoA conserved MHD state (P = plane size).
@@ -250,6 +255,50 @@ __device__ __forceinline__ PrimM load_cell_m(const MhdArgs& A, long long k, bool
     // reflective sides negate the normal velocity and the normal field
     if (fx) { q.u = -q.u; q.bx = -q.bx; }
     if (fy) { q.v = -q.v; q.by = -q.by; }
+    return q;
+}
+
+// Logical index k -> stored index for the production side kinds of the
+// reference-layout steps; side = the kind of the side k lies beyond (-1
+// inside). SYMMETRY / AXIS / REFLECTIVE mirror, the others clamp.
+__device__ __forceinline__ int map_index_side(int k, int n, int lo, int hi, int& side) {
+    side = -1;
+    if (k < 0) {
+        if (lo == GD2_BC_PERIODIC) return ((k % n) + n) % n;
+        side = lo;
+        if (lo == GD2_BC_REFLECTIVE || lo == GD2_BC_SYMMETRY || lo == GD2_BC_AXIS) {
+            const int m = -1 - k;
+            return m < n ? m : n - 1;
+        }
+        return 0;
+    }
+    if (k >= n) {
+        if (hi == GD2_BC_PERIODIC) return k % n;
+        side = hi;
+        if (hi == GD2_BC_REFLECTIVE || hi == GD2_BC_SYMMETRY || hi == GD2_BC_AXIS) {
+            const int m = 2 * n - 1 - k;
+            return m >= 0 ? m : 0;
+        }
+        return n - 1;
+    }
+    return k;
+}
+
+// Ghost state of a cell reached through side kinds sx (x) and sy (y), the
+// rules of add2 (kernel.cu:1938-2016) and Kernel_TVD (kernel.cu:2227-2405).
+__device__ __forceinline__ PrimM ghost_m(PrimM q, int sx, int sy, const MhdArgs& A) {
+    if (sx == GD2_BC_REFLECTIVE) { q.u = -q.u; q.bx = -q.bx; }
+    else if (sx == GD2_BC_SYMMETRY) { q.u = -q.u; q.bx = -q.bx; q.by = -q.by; }
+    else if (sx == GD2_BC_OUTFLOW) { if (q.u < 0.0) q.u = 0.01; }
+    if (sy == GD2_BC_REFLECTIVE) { q.v = -q.v; q.by = -q.by; }
+    else if (sy == GD2_BC_AXIS) { q.v = -q.v; q.bz = -q.bz; }
+    else if (sy == GD2_BC_INFLOW) {
+        q.r = 1.0;
+        q.p = A.inv_gamma;
+        q.u = 0.0;
+        q.v = 0.0;
+        if (A.inflow_b_zero) { q.bx = 0.0; q.by = 0.0; q.bz = 0.0; }
+    }
     return q;
 }
 
@@ -401,7 +450,9 @@ __device__ __forceinline__ bool frozen_m(const MhdArgs& A, int c, int j) {
     }
 }
 
-template <int ORDER, int METOD, int LAYOUT, bool WANT_W>
+// PROD (reference layout only): production side kinds, inner frozen sphere
+// and axisymmetric sources.
+template <int ORDER, int METOD, int LAYOUT, bool WANT_W, bool PROD = false>
 __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
     __shared__ double ring[3][kMF][kRowBuf];
 
@@ -414,9 +465,17 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
     const int c = cbase + lane;
 
     bool fx, hfx;
-    const int ci = map_index(c, A.nx, A.bc[0], A.bc[1], fx);
+    int sxc = -1, sxh = -1;
     const int hcol = lane == 0 ? cbase - 2 : (lane == 1 ? cbase - 1 : cbase + 64);
-    const int hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
+    int ci, hci;
+    if constexpr (PROD) {
+        ci = map_index_side(c, A.nx, A.bc[0], A.bc[1], sxc);
+        hci = map_index_side(hcol, A.nx, A.bc[0], A.bc[1], sxh);
+        fx = hfx = false;
+    } else {
+        ci = map_index(c, A.nx, A.bc[0], A.bc[1], fx);
+        hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
+    }
     const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
     const bool has_halo = lane < 3;
     const bool owns = (lane < kTileW) && (c < A.nx);
@@ -430,13 +489,22 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
 
     auto load_row = [&](int jl, int slot, bool to_lds) -> PrimM {
         bool fy;
-        const int jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
+        int sy = -1;
+        int jj;
+        if constexpr (PROD) {
+            jj = map_index_side(jl, A.ny, A.bc[2], A.bc[3], sy);
+            fy = false;
+        } else {
+            jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
+        }
         const long long rowk = (long long)jj * A.nx;
         PrimM q = load_cell_m<LAYOUT>(A, rowk + ci, fx, fy);
+        if constexpr (PROD) q = ghost_m(q, sxc, sy, A);
         if (to_lds) {
             lds_put_m(ring, slot, lane + 2, q);
             if (has_halo) {
                 PrimM h = load_cell_m<LAYOUT>(A, rowk + hci, hfx, fy);
+                if constexpr (PROD) h = ghost_m(h, sxh, sy, A);
                 lds_put_m(ring, slot, hslot, h);
             }
         }
@@ -464,7 +532,7 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         // A reflective south wall face is evaluated from the interior cell's
         // side (outward normal), as the reference's cell does; its HLLC-MHD /
         // HLLD flux is discontinuous at SM = 0 when the normal field jumps.
-        const bool smir = (j == -1) && (A.bc[2] == GD2_BC_REFLECTIVE);
+        const bool smir = (j == -1) && (A.bc[2] == GD2_BC_REFLECTIVE || (PROD && A.bc[2] == GD2_BC_AXIS));
         const PrimM yl = face_m(q0, s0, 0.5), yr = face_m(q1, s1, -0.5);
         LabFluxM Gp = smir ? unmirror_y(mhd_flux<METOD, false>(mirror_y(yr), mirror_y(yl), A.gamma, A.inv_g1, smax))
                            : mhd_flux<METOD, false>(yl, yr, A.gamma, A.inv_g1, smax);
@@ -487,7 +555,7 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         const PrimM sb = slope3m<ORDER>(qa, qb, q0);
         const PrimM sc = slope3m<ORDER>(qb, q0, qd);
         // reflective west wall face from the interior side (see the south wall)
-        const bool wmir = (c == 0) && (A.bc[0] == GD2_BC_REFLECTIVE);
+        const bool wmir = (c == 0) && (A.bc[0] == GD2_BC_REFLECTIVE || (PROD && A.bc[0] == GD2_BC_SYMMETRY));
         const PrimM xl = face_m(qb, sb, 0.5), xr = face_m(q0, sc, -0.5);
         LabFluxM Fl = mhd_flux<METOD, true>(wmir ? mirror_x(xr) : xl, wmir ? mirror_x(xl) : xr, A.gamma, A.inv_g1,
                                             smax);
@@ -497,7 +565,14 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
         const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
         const double bx_w = qb.bx, bx_e = qd.bx;
 
-        if (owns && !frozen_m<LAYOUT>(A, c, j)) {
+        bool sphere = false;
+        double yc = 0.0;
+        if constexpr (PROD) {
+            const double xc = A.x0 + c * dx;
+            yc = A.y0 + j * dy;
+            sphere = A.r_frozen > 0.0 && sqrt(xc * xc + yc * yc) < A.r_frozen;
+        }
+        if (owns && !frozen_m<LAYOUT>(A, c, j) && !sphere) {
             // face sums differenced per direction (kernel.cu:2559-2604)
             const double PSr = (Fr.r - Fl.r) * dy + (Gp.r - Gm.r) * dx;
             const double PSe = (Fr.e - Fl.e) * dy + (Gp.e - Gm.e) * dx;
@@ -549,24 +624,46 @@ __global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
                     wmax = fmax(wmax, (fabs(ux) + cf) * A.idx + (fabs(uy) + cf) * A.idy);
                 }
             } else {
-                // primitive-form update of add2 / Kernel_TVD, Cartesian
-                // (kernel.cu:2141-2195 without the 1/y terms)
+                // primitive-form update of add2 / Kernel_TVD (kernel.cu:2137-2195);
+                // the 1/y terms only with the axisymmetric option
                 const double gm1 = A.gamma - 1.0;
-                double r2 = q.r - dt * PSr / dV;
-                if (r2 <= 0.0) {
-                    atomicAdd(&A.diag->clamp_rho, 1ull);
-                    r2 = q.r;
-                }
-                const double ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pdiv) / dV) / r2;
-                const double uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pdiv) / dV) / r2;
-                const double bx = q.bx - dt * (PBx + q.u * Pdiv) / dV;
-                const double by = q.by - dt * (PBy + q.v * Pdiv) / dV;
-                const double bz = q.bz - dt * PBz / dV;
                 const double U8 = q.p / gm1 + 0.5 * q.r * (q.u * q.u + q.v * q.v) +
                                   (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / kCpi8;
                 const double skk = q.u * q.bx + q.v * q.by;
-                double p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pdiv) / dV - 0.5 * r2 * (ux * ux + uy * uy) -
-                             (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                double r2, ux, uy, bx, by, bz, p2;
+                if (PROD && A.axisym) {
+                    const double y = yc;
+                    const double Pd = Pdiv + dV * q.by / y;
+                    r2 = q.r - dt * PSr / dV - dt * q.r * q.v / y;
+                    if (r2 <= 0.0) {
+                        atomicAdd(&A.diag->clamp_rho, 1ull);
+                        r2 = q.r;
+                    }
+                    ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pd) / dV -
+                          dt * (q.r * q.u * q.v - q.bx * q.by / kCpi4) / y) / r2;
+                    uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pd) / dV -
+                          dt * (q.r * q.v * q.v + (q.bz * q.bz - q.by * q.by) / kCpi4) / y) / r2;
+                    bx = q.bx - dt * (PBx + q.u * Pd) / dV - dt * (q.v * q.bx - q.by * q.u) / y;
+                    by = q.by - dt * (PBy + q.v * Pd) / dV;
+                    bz = q.bz - dt * PBz / dV;
+                    p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pd) / dV -
+                          dt * (((U8 + q.p + (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / kCpi8) * q.v -
+                                 q.by * skk / kCpi4) / y) -
+                          0.5 * r2 * (ux * ux + uy * uy) - (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                } else {
+                    r2 = q.r - dt * PSr / dV;
+                    if (r2 <= 0.0) {
+                        atomicAdd(&A.diag->clamp_rho, 1ull);
+                        r2 = q.r;
+                    }
+                    ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pdiv) / dV) / r2;
+                    uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pdiv) / dV) / r2;
+                    bx = q.bx - dt * (PBx + q.u * Pdiv) / dV;
+                    by = q.by - dt * (PBy + q.v * Pdiv) / dV;
+                    bz = q.bz - dt * PBz / dV;
+                    p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pdiv) / dV - 0.5 * r2 * (ux * ux + uy * uy) -
+                          (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                }
                 if (p2 <= 0.0) {
                     atomicAdd(&A.diag->clamp_p, 1ull);
                     p2 = 0.000001;
@@ -653,23 +750,24 @@ inline void fill_mhd(MhdArgs& A, const gd2_grid* g, DeviceDiag* diag) {
     A.tiles_x = (g->nx + kTileW - 1) / kTileW;
 }
 
-template <int ORDER, int METOD, int LAYOUT, bool W>
+template <int ORDER, int METOD, int LAYOUT, bool W, bool PROD = false>
 inline int launch_mhd(const MhdArgs& A0, hipStream_t st) {
     MhdArgs A = A0;
-    A.rows = strip_rows(reinterpret_cast<const void*>(&mhd_tile_kernel<ORDER, METOD, LAYOUT, W>), A.tiles_x, A.ny, 2);
+    A.rows = strip_rows(reinterpret_cast<const void*>(&mhd_tile_kernel<ORDER, METOD, LAYOUT, W, PROD>), A.tiles_x,
+                        A.ny, 2);
     const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
-    hipLaunchKernelGGL((mhd_tile_kernel<ORDER, METOD, LAYOUT, W>), dim3(blocks), dim3(kWave), 0, st, A);
+    hipLaunchKernelGGL((mhd_tile_kernel<ORDER, METOD, LAYOUT, W, PROD>), dim3(blocks), dim3(kWave), 0, st, A);
     return hip_error_code(hipGetLastError());
 }
 
-template <int LAYOUT, bool W>
+template <int LAYOUT, bool W, bool PROD = false>
 inline int dispatch_mhd(int order, int metod, const MhdArgs& A, hipStream_t st) {
     if (order == 2) {
-        if (metod == 3) return launch_mhd<2, 3, LAYOUT, W>(A, st);
-        if (metod == 2) return launch_mhd<2, 2, LAYOUT, W>(A, st);
+        if (metod == 3) return launch_mhd<2, 3, LAYOUT, W, PROD>(A, st);
+        if (metod == 2) return launch_mhd<2, 2, LAYOUT, W, PROD>(A, st);
     } else if (order == 1) {
-        if (metod == 3) return launch_mhd<1, 3, LAYOUT, W>(A, st);
-        if (metod == 2) return launch_mhd<1, 2, LAYOUT, W>(A, st);
+        if (metod == 3) return launch_mhd<1, 3, LAYOUT, W, PROD>(A, st);
+        if (metod == 2) return launch_mhd<1, 2, LAYOUT, W, PROD>(A, st);
     }
     return GD2_ERR_UNSUPPORTED;
 }
@@ -727,6 +825,15 @@ inline int mhd_ref_step(const gd2_grid* g, int order, const gd2_double2* s, cons
     A.plain = 1;
     A.a = 0.0;
     A.b_coef = 1.0;
+    A.x0 = g->x0;
+    A.y0 = g->y0;
+    A.r_frozen = g->r_frozen;
+    A.inv_gamma = 1.0 / g->gamma;
+    A.axisym = g->axisym ? 1 : 0;
+    A.inflow_b_zero = order == 2 ? 1 : 0;
+    bool prod = g->axisym || g->r_frozen > 0.0;
+    for (int s = 0; s < 4; ++s) prod = prod || g->bc[s] >= GD2_BC_SYMMETRY;
+    if (prod) return dispatch_mhd<kAoS, false, true>(order, method, A, st);
     return dispatch_mhd<kAoS, false>(order, method, A, st);
 }
 

@@ -160,3 +160,50 @@ def ensemble_sizes(seed, count=256, lo=64, hi=1024):
         ny = int(round(math.exp(r.uniform(math.log(lo), math.log(hi)))))
         out.append((nx, ny))
     return out
+
+
+# ------------------------------------------------- the reference's production run
+# Constants of the stellar-wind run (Header.h:25-37): wind speed phi_0, wind
+# Mach number M_0, Alfven Mach number M_alf, reference radius rr_0, inner
+# frozen sphere radius ddist; grid N x M on (0, 9] x (0, 5] with cell
+# centres at (n + 1/2) dx, (m + 1/2) dy (kernel.cu:12-20).
+WIND_PHI0, WIND_M0, WIND_MALF, WIND_RR0, WIND_DDIST = 4.0, 10.0, 8.0, 0.07, 0.3
+WIND_NX, WIND_NY, WIND_XMAX, WIND_YMAX = 3226, 1792, 9.0, 5.0
+
+
+def wind_grid_args(nx=WIND_NX, ny=WIND_NY, x_max=WIND_XMAX, y_max=WIND_YMAX):
+    """Keyword arguments of make_grid for the production run: the four
+    production side kinds (symmetry west, outflow east, axis south, inflow
+    north), the frozen inner sphere and the axisymmetric sources."""
+    return dict(dx=x_max / nx, dy=y_max / ny, x0=x_max / (2.0 * nx), y0=y_max / (2.0 * ny),
+                gamma=5.0 / 3.0, krit=0.25, bc=(4, 6, 5, 7), r_frozen=WIND_DDIST, axisym=True)
+
+
+def stellar_wind_prim(g):
+    """Initial primitive state of the production run (kernel.cu:3901-3941):
+    inside 1.01 ddist a radial wind rho ~ r^-2, p ~ r^-2gamma, |v| = phi_0 with
+    an azimuthal field B_phi ~ sin(theta) / r; outside, rho = 1, p = 1/gamma,
+    v = 0, B = 0. Returns AoS arrays s (ny, nx, 2) = (rho, p), u (ny, nx, 2),
+    b (ny, nx, 3)."""
+    gam = g.gamma
+    x, y = _centres(g)
+    dist = np.sqrt(x * x + y * y)
+    ro_e = 1.0 / (WIND_PHI0 * WIND_PHI0 * WIND_RR0 * WIND_RR0)
+    p_e = ro_e * WIND_PHI0 * WIND_PHI0 / (gam * WIND_M0 * WIND_M0)
+    b_e = math.sqrt(4.0 * math.pi) / (WIND_MALF * WIND_RR0)
+    inner = dist < WIND_DDIST * 1.01
+    s = np.empty((g.ny, g.nx, 2))
+    u = np.zeros((g.ny, g.nx, 2))
+    b = np.zeros((g.ny, g.nx, 3))
+    s[..., 0] = 1.0
+    s[..., 1] = 1.0 / gam
+    d = dist[inner]
+    s[inner, 0] = ro_e / (d / WIND_RR0) ** 2
+    s[inner, 1] = p_e * (WIND_RR0 / d) ** (2.0 * gam)
+    u[inner, 0] = WIND_PHI0 * x[inner] / d
+    u[inner, 1] = WIND_PHI0 * y[inner] / d
+    # Cartesian image of the azimuthal component B_E (rr_0 / r) sin(theta)
+    # (kernel.cu:155-173 at phi = 0): it lands in the third component
+    bphi = b_e / (d / WIND_RR0) * np.sin(np.arccos(x[inner] / d))
+    b[inner, 2] = np.where(np.abs(y[inner]) < 1e-6, 0.0, bphi)
+    return s, u, b

@@ -42,8 +42,20 @@ enum gd2_status {
  * cell (kernel.cu:1762-1793); its "hard" sides keep the edge cells frozen
  * (kernel.cu:1752-1758); its symmetry sides mirror with the normal velocity
  * (and normal B) negated (kernel.cu:2263-2277). Periodic has no reference
- * counterpart. FROZEN is accepted by the reference-layout entry points only. */
-enum gd2_bc { GD2_BC_TRANSMISSIVE = 0, GD2_BC_PERIODIC = 1, GD2_BC_REFLECTIVE = 2, GD2_BC_FROZEN = 3 };
+ * counterpart. FROZEN is accepted by the reference-layout entry points only.
+ * The last four are the ghost rules of the reference's production MHD steps
+ * (add2 / Kernel_TVD), accepted by gd2_add2 and gd2_kernel_tvd only, each on
+ * its own side (add2 kernel.cu:1938-2016, Kernel_TVD kernel.cu:2227-2405):
+ *   SYMMETRY (west only): mirror, negating vx, Bx and By;
+ *   AXIS (south only): mirror, negating vy and Bz;
+ *   OUTFLOW (east only): copy of the edge cell with vx raised to 0.01
+ *     where it is negative;
+ *   INFLOW (north only): rho = 1, p = 1/gamma, v = 0, and B of the edge
+ *     cell for gd2_add2, B = 0 for gd2_kernel_tvd. */
+enum gd2_bc {
+    GD2_BC_TRANSMISSIVE = 0, GD2_BC_PERIODIC = 1, GD2_BC_REFLECTIVE = 2, GD2_BC_FROZEN = 3,
+    GD2_BC_SYMMETRY = 4, GD2_BC_AXIS = 5, GD2_BC_OUTFLOW = 6, GD2_BC_INFLOW = 7
+};
 
 /* Grid and physics descriptor. The reference has all of these as
  * compile-time macros: N, M, dx, dy, x_min, y_min (kernel.cu:12-20), gamma
@@ -55,6 +67,13 @@ typedef struct gd2_grid {
     double gamma;        /* ratio of specific heats */
     double krit;         /* CFL factor of the reference-layout steps' face time step */
     int bc[4];           /* west, east, south, north: enum gd2_bc */
+    /* Production options of gd2_add2 / gd2_kernel_tvd (0 elsewhere):
+     * cells with sqrt(x^2 + y^2) < r_frozen keep their state and give no
+     * time step (the inner sphere, kernel.cu:1918, 2218); axisym != 0 adds
+     * the axisymmetric (y = r) source terms of kernel.cu:2137-2190 and
+     * 2615-2650 (requires y0 > 0). */
+    double r_frozen;
+    int axisym;
 } gd2_grid;
 
 /* ----------------------------------------------------------------------

@@ -743,17 +743,23 @@ double o_hlld(double gamma, double krit,
  * reference's transmissive sides copy the edge cell for both ghost layers
  * (kernel.cu:1762-1793, 3374-3417); its symmetry sides mirror (kernel.cu:
  * 2263-2277, 2356-2405). Periodic sides wrap (no reference counterpart). */
-static inline int map_idx(int k, int n, int lo, int hi, int* flip)
+/* Source index of position k on a line of n cells, and the kind of side it
+ * came through (*side = the side's bc kind, or -1 inside the line): mirrored
+ * for the reflective / symmetry / axis kinds, wrapped for periodic, the
+ * edge cell otherwise. */
+static inline int map_idx(int k, int n, int lo, int hi, int* side)
 {
-    *flip = 0;
+    *side = -1;
     if (k < 0) {
         if (lo == O_BC_PERIODIC) return ((k % n) + n) % n;
-        if (lo == O_BC_REFLECTIVE) { *flip = 1; return (-1 - k < n) ? -1 - k : n - 1; }
+        *side = lo;
+        if (lo == O_BC_REFLECTIVE || lo == O_BC_SYMMETRY || lo == O_BC_AXIS) return (-1 - k < n) ? -1 - k : n - 1;
         return 0;
     }
     if (k >= n) {
         if (hi == O_BC_PERIODIC) return k % n;
-        if (hi == O_BC_REFLECTIVE) { *flip = 1; return (2 * n - 1 - k >= 0) ? 2 * n - 1 - k : 0; }
+        *side = hi;
+        if (hi == O_BC_REFLECTIVE || hi == O_BC_SYMMETRY || hi == O_BC_AXIS) return (2 * n - 1 - k >= 0) ? 2 * n - 1 - k : 0;
         return n - 1;
     }
     return k;
@@ -769,6 +775,7 @@ typedef struct {
     const o_d2 *s, *u;
     const o_d3* b;
     const double* U;
+    int inflow_b_zero;   /* INFLOW ghosts: B = 0 (Kernel_TVD) instead of the edge cell's (add2) */
 } src_t;
 
 static inline prim cons_cell_to_prim(const src_t* S, long k)
@@ -800,9 +807,9 @@ static inline prim cons_cell_to_prim(const src_t* S, long k)
 static inline prim fetch(const src_t* S, int i, int j)
 {
     const o_grid* g = S->g;
-    int fx, fy;
-    int ii = map_idx(i, g->nx, g->bc[0], g->bc[1], &fx);
-    int jj = map_idx(j, g->ny, g->bc[2], g->bc[3], &fy);
+    int sx, sy;
+    int ii = map_idx(i, g->nx, g->bc[0], g->bc[1], &sx);
+    int jj = map_idx(j, g->ny, g->bc[2], g->bc[3], &sy);
     long k = (long)jj * g->nx + ii;
     prim q;
     if (S->kind == 0) {
@@ -812,8 +819,16 @@ static inline prim fetch(const src_t* S, int i, int j)
     } else {
         q = cons_cell_to_prim(S, k);
     }
-    if (fx) { q.u = -q.u; q.bx = -q.bx; }
-    if (fy) { q.v = -q.v; q.by = -q.by; }
+    /* ghost rules by side kind (kernel.cu:1960-2032, 2232-2277, 2322-2405) */
+    if (sx == O_BC_REFLECTIVE) { q.u = -q.u; q.bx = -q.bx; }
+    else if (sx == O_BC_SYMMETRY) { q.u = -q.u; q.bx = -q.bx; q.by = -q.by; }
+    else if (sx == O_BC_OUTFLOW) { if (q.u < 0) q.u = 0.01; }
+    if (sy == O_BC_REFLECTIVE) { q.v = -q.v; q.by = -q.by; }
+    else if (sy == O_BC_AXIS) { q.v = -q.v; q.bz = -q.bz; }
+    else if (sy == O_BC_INFLOW) {
+        q.r = 1.0; q.p = 1.0 / g->gamma; q.u = 0.0; q.v = 0.0;
+        if (S->inflow_b_zero) { q.bx = 0.0; q.by = 0.0; q.bz = 0.0; }
+    }
     return q;
 }
 
@@ -952,7 +967,7 @@ static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2*
                           double* T, const double* T_do, int method)
 {
     if (method != 0 && method != 1 && method != 2) return -1;
-    src_t S = {g, 0, 0, s, u, NULL, NULL};
+    src_t S = {g, 0, 0, s, u, NULL, NULL, 0};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
     double tglob = *T;
@@ -1043,7 +1058,8 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
                         o_d2* s2, o_d2* u2, o_d3* b2, double* T, const double* T_do, int method)
 {
     if (method != 2 && method != 3) return -1;
-    src_t S = {g, 0, 1, s, u, b, NULL};
+    if (g->axisym && !(g->y0 > 0.0)) return -1;
+    src_t S = {g, 0, 1, s, u, b, NULL, order == 2};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
     double tglob = *T;
@@ -1051,7 +1067,9 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
     for (int j = 0; j < g->ny; ++j) {
         for (int i = 0; i < g->nx; ++i) {
             long k = (long)j * g->nx + i;
-            if (is_frozen(g, i, j)) { s2[k] = s[k]; u2[k] = u[k]; b2[k] = b[k]; continue; }
+            const double x = g->x0 + i * dx, y = g->y0 + j * dy;
+            const int sphere = g->r_frozen > 0.0 && sqrt(x * x + y * y) < g->r_frozen;  /* kernel.cu:1919 */
+            if (is_frozen(g, i, j) || sphere) { s2[k] = s[k]; u2[k] = u[k]; b2[k] = b[k]; continue; }
             stencil st;
             faces F;
             gather(&S, i, j, order, &st);
@@ -1060,20 +1078,40 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
             if (tglob > m.tmin) tglob = m.tmin;
             const prim* c = &st.c;
             double dV = dx * dy;
-            double r2 = c->r - Tdo * m.PS.x / dV;
-            if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = c->r; }
-            double ux = (c->r * c->u - Tdo * (m.PU.x + (c->bx / CPI4) * m.Pdiv) / dV) / r2;
-            double uy = (c->r * c->v - Tdo * (m.PU.y + (c->by / CPI4) * m.Pdiv) / dV) / r2;
-            double bx = (c->bx - Tdo * (m.PB.x + c->u * m.Pdiv) / dV);
-            double by = (c->by - Tdo * (m.PB.y + c->v * m.Pdiv) / dV);
-            double bz = (c->bz - Tdo * (m.PB.z) / dV);
             /* U8 (Header.h:16), skk (Header.h:14), kvv (Header.h:15) with w = 0 */
             double U8 = c->p / (gam - 1.0) + 0.5 * c->r * (c->u * c->u + c->v * c->v + 0.0 * 0.0) +
                         (c->bx * c->bx + c->by * c->by + c->bz * c->bz) / 25.13274122871834590768;
             double skk = c->u * c->bx + c->v * c->by + 0.0 * c->bz;
-            double p2 = (U8 - Tdo * (m.PS.y + (skk / CPI4) * m.Pdiv) / dV -
-                         0.5 * r2 * (ux * ux + uy * uy + 0.0 * 0.0) - (bx * bx + by * by + bz * bz) / CPI8) *
-                        (gam - 1.0);
+            double r2, ux, uy, bx, by, bz, p2;
+            if (g->axisym) {
+                /* axisymmetric sources, kernel.cu:2134-2190 (y = r > 0) */
+                const double Pdiv = m.Pdiv + dV * c->by / y;
+                r2 = c->r - Tdo * m.PS.x / dV - Tdo * c->r * c->v / y;
+                if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = c->r; }
+                ux = (c->r * c->u - Tdo * (m.PU.x + (c->bx / CPI4) * Pdiv) / dV -
+                      Tdo * (c->r * c->u * c->v - c->bx * c->by / CPI4) / y) / r2;
+                uy = (c->r * c->v - Tdo * (m.PU.y + (c->by / CPI4) * Pdiv) / dV -
+                      Tdo * (c->r * c->v * c->v + (c->bz * c->bz - c->by * c->by) / CPI4) / y) / r2;
+                bx = (c->bx - Tdo * (m.PB.x + c->u * Pdiv) / dV - Tdo * (c->v * c->bx - c->by * c->u) / y);
+                by = (c->by - Tdo * (m.PB.y + c->v * Pdiv) / dV);
+                bz = (c->bz - Tdo * (m.PB.z) / dV);
+                p2 = (U8 - Tdo * (m.PS.y + (skk / CPI4) * Pdiv) / dV -
+                      Tdo * (((U8 + c->p + (c->bx * c->bx + c->by * c->by + c->bz * c->bz) / CPI8) * c->v -
+                              c->by * skk / CPI4) / y) -
+                      0.5 * r2 * (ux * ux + uy * uy + 0.0 * 0.0) - (bx * bx + by * by + bz * bz) / CPI8) *
+                     (gam - 1.0);
+            } else {
+                r2 = c->r - Tdo * m.PS.x / dV;
+                if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = c->r; }
+                ux = (c->r * c->u - Tdo * (m.PU.x + (c->bx / CPI4) * m.Pdiv) / dV) / r2;
+                uy = (c->r * c->v - Tdo * (m.PU.y + (c->by / CPI4) * m.Pdiv) / dV) / r2;
+                bx = (c->bx - Tdo * (m.PB.x + c->u * m.Pdiv) / dV);
+                by = (c->by - Tdo * (m.PB.y + c->v * m.Pdiv) / dV);
+                bz = (c->bz - Tdo * (m.PB.z) / dV);
+                p2 = (U8 - Tdo * (m.PS.y + (skk / CPI4) * m.Pdiv) / dV -
+                      0.5 * r2 * (ux * ux + uy * uy + 0.0 * 0.0) - (bx * bx + by * by + bz * bz) / CPI8) *
+                     (gam - 1.0);
+            }
             if (p2 <= 0) { diag_inc(&g_diag.clamp_p); p2 = 0.000001; }
             s2[k].x = r2; s2[k].y = p2; u2[k].x = ux; u2[k].y = uy;
             b2[k].x = bx; b2[k].y = by; b2[k].z = bz;
@@ -1131,7 +1169,7 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
     for (int s = 0; s < 4; ++s)
         if (g->bc[s] == O_BC_FROZEN) return -1;
     const int plain = (a == 0.0 && b == 1.0);
-    src_t S = {g, 1, mhd, NULL, NULL, NULL, Uin};
+    src_t S = {g, 1, mhd, NULL, NULL, NULL, Uin, 0};
     const double dx = g->dx, dy = g->dy, gam = g->gamma, dV = dx * dy;
     const long plane = (long)g->nx * g->ny;
     const int nf = fields_of(sc->physics);
@@ -1190,7 +1228,7 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
             }
             for (int f = 0; f < nf; ++f) Uout[f * plane + k] = out[f];
             if (wmax) {
-                src_t So = {g, 1, mhd, NULL, NULL, NULL, Uout};
+                src_t So = {g, 1, mhd, NULL, NULL, NULL, Uout, 0};
                 prim q = cons_cell_to_prim(&So, k);
                 double w = cell_wavespeed(g, &q);
                 if (w > wm) wm = w;
@@ -1204,7 +1242,7 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
 double o_wavespeed_max(const o_grid* g, const o_scheme* sc, const double* U)
 {
     const int mhd = sc->physics == O_PHYS_MHD;
-    src_t S = {g, 1, mhd, NULL, NULL, NULL, U};
+    src_t S = {g, 1, mhd, NULL, NULL, NULL, U, 0};
     const long n = (long)g->nx * g->ny;
     double wm = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : wm)
@@ -1268,7 +1306,7 @@ void o_prim_to_cons(const o_grid* g, int physics, const o_d2* s, const o_d2* u, 
 
 void o_cons_to_prim(const o_grid* g, int physics, const double* U, o_d2* s, o_d2* u, o_d3* b)
 {
-    src_t S = {g, 1, physics == O_PHYS_MHD, NULL, NULL, NULL, U};
+    src_t S = {g, 1, physics == O_PHYS_MHD, NULL, NULL, NULL, U, 0};
     const long plane = (long)g->nx * g->ny;
     for (long k = 0; k < plane; ++k) {
         prim q = cons_cell_to_prim(&S, k);

@@ -20,7 +20,9 @@ typedef struct { double x, y; } o_d2;
 typedef struct { double x, y, z; } o_d3;
 
 /* Side boundary kinds (same numbering as the product's gd2_bc). */
-enum { O_BC_TRANSMISSIVE = 0, O_BC_PERIODIC = 1, O_BC_REFLECTIVE = 2, O_BC_FROZEN = 3 };
+enum { O_BC_TRANSMISSIVE = 0, O_BC_PERIODIC = 1, O_BC_REFLECTIVE = 2, O_BC_FROZEN = 3,
+       /* the production ghost rules of add2 / Kernel_TVD (one side each) */
+       O_BC_SYMMETRY = 4, O_BC_AXIS = 5, O_BC_OUTFLOW = 6, O_BC_INFLOW = 7 };
 
 /* Runtime grid/physics descriptor: the reference has these as macros
  * (Header.h:3-16 gamma/krit, kernel.cu:11-20 N, M, dx, dy, x_min, y_min). */
@@ -31,6 +33,8 @@ typedef struct {
     double gamma;        /* ga / ggg */
     double krit;         /* CFL factor of the reference-layout steps */
     int bc[4];           /* west, east, south, north */
+    double r_frozen;     /* MHD reference steps: cells inside this radius keep their state */
+    int axisym;          /* MHD reference steps: axisymmetric (y = r) source terms */
 } o_grid;
 
 /* Scheme selection for the native (SoA conserved) path. */

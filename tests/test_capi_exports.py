@@ -46,6 +46,19 @@ def test_pure_host_entry_points():
     assert L.gd2_stage(ctypes.byref(bad), ctypes.byref(sc), x, None, ctypes.c_void_p(32), 0.0, 1.0, 0.0, None,
                        None, None) == -1
     assert L.gd2_ker_dekard(ctypes.byref(g), x, x, ctypes.c_void_p(32), ctypes.c_void_p(48), x, x, 7, None) == -2
+    # production options: MHD reference-layout steps only, each side kind on
+    # its own side, axisym with y0 > 0, r_frozen >= 0
+    from golikov_2d_amd import problems as PB
+    prod = api.make_grid(8, 8, **PB.wind_grid_args(8, 8))
+    y = ctypes.c_void_p(64)
+    assert L.gd2_stage(ctypes.byref(prod), ctypes.byref(sc), x, None, y, 0.0, 1.0, 0.0, None, None, None) == -1
+    assert L.gd2_ker_dekard(ctypes.byref(prod), x, x, y, y, x, x, 1, None) == -1
+    for bad in (dict(bc=(api.BC_OUTFLOW, api.BC_SYMMETRY, api.BC_AXIS, api.BC_INFLOW)),
+                dict(bc=(api.BC_SYMMETRY, api.BC_OUTFLOW, api.BC_INFLOW, api.BC_AXIS)),
+                dict(y0=0.0), dict(r_frozen=-1.0)):
+        gb = api.make_grid(8, 8, **dict(PB.wind_grid_args(8, 8), **bad))
+        for fn in (L.gd2_add2, L.gd2_kernel_tvd):
+            assert fn(ctypes.byref(gb), x, x, x, y, y, y, x, x, 0, 2, None) == -1
 
 
 @pytest.mark.skipif(not os.path.exists(api.SO_PATH), reason="libgd2.so not built")

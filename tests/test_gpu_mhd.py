@@ -131,3 +131,103 @@ def test_orszag_tang_point_symmetry_2048():
     diff = (U - sign * rot).abs().sum(dim=(1, 2)) / U.abs().sum(dim=(1, 2)).clamp_min(1e-300)
     assert torch.all(diff[[0, 1, 2, 3, 4, 6]] <= 1e-12), diff
     s.close()
+
+
+PROD_BC = (O.BC_SYMMETRY, O.BC_OUTFLOW, O.BC_AXIS, O.BC_INFLOW)
+
+
+def _ref_step_dev(kind, g, s, u, b, T0, Tdo, method):
+    sd, ud, bd = to_dev(s), to_dev(u), to_dev(b)
+    s2, u2, b2 = torch.empty_like(sd), torch.empty_like(ud), torch.empty_like(bd)
+    T = torch.tensor([T0], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([Tdo], dtype=torch.float64, device=dev())
+    fn = api.add2 if kind == "add2" else api.kernel_tvd
+    fn(g, sd, ud, bd, s2, u2, b2, T, T_do, 0, method)
+    return to_host(s2), to_host(u2), to_host(b2), T.item()
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+@pytest.mark.parametrize("method", [2, 3])
+@pytest.mark.parametrize("axisym", [True, False])
+def test_production_options_match_oracle(kind, method, axisym):
+    """Production side kinds, frozen inner sphere and (optionally) the
+    axisymmetric sources: one step against the oracle, 1e-12."""
+    from tests.test_oracle_wind import smooth_state
+    nx, ny = 203, 131
+    say(f"production options {kind} method={method} axisym={axisym} {nx}x{ny}")
+    kw = PB.wind_grid_args(nx, ny, x_max=4.0, y_max=2.6)
+    kw["axisym"] = axisym
+    g, og = grids(nx, ny, kw.pop("bc"), **kw)
+    s, u, b = smooth_state(og, seed=method + 7)
+    T0, Tdo = 1e7, 1e-3
+    s2r, u2r, b2r, Tr = O.ref_mhd_step(kind, og, s, u, b, T0, Tdo, 0, method)
+    s2, u2, b2, T = _ref_step_dev(kind, g, s, u, b, T0, Tdo, method)
+    assert np.max(np.abs(s2 - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(u2 - u2r)) <= 1e-12
+    assert np.max(np.abs(b2 - b2r)) <= 1e-12
+    assert abs(T - Tr) <= 1e-14 * Tr
+    # the frozen sphere is copied bit for bit
+    x = og.x0 + np.arange(nx) * og.dx
+    y = og.y0 + np.arange(ny) * og.dy
+    X, Y = np.meshgrid(x, y)
+    inside = np.sqrt(X * X + Y * Y) < og.r_frozen
+    assert inside.sum() > 10
+    np.testing.assert_array_equal(s2[inside], s[inside])
+
+
+def test_production_options_rejected_elsewhere():
+    """The production side kinds and options are accepted by the MHD
+    reference-layout steps only, each side kind on its own side."""
+    say("production options rejected elsewhere")
+    kw = PB.wind_grid_args(64, 40)
+    g = api.make_grid(64, 40, **kw)
+    U = to_dev(PB.uniform_mhd(g))
+    out = torch.empty_like(U)
+    with pytest.raises(api.GD2Error):
+        api.stage(g, pscheme(3, 2), U, None, out, 0.0, 1.0, 1e-4)
+    sw = list(kw["bc"])
+    sw[0], sw[1] = sw[1], sw[0]
+    kw2 = dict(kw, bc=tuple(sw))
+    g2 = api.make_grid(64, 40, **kw2)
+    s = torch.ones((40, 64, 2), dtype=torch.float64, device=dev())
+    u = torch.zeros((40, 64, 2), dtype=torch.float64, device=dev())
+    b = torch.zeros((40, 64, 3), dtype=torch.float64, device=dev())
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    with pytest.raises(api.GD2Error):
+        api.kernel_tvd(g2, s, u, b, torch.empty_like(s), torch.empty_like(u), torch.empty_like(b), T, T.clone(), 0, 2)
+
+
+def test_stellar_wind_20_steps_match_oracle():
+    """The production run at a tenth of its size: stellar-wind state, add2
+    (first-order) then Kernel_TVD steps, each followed by funk_time, as the
+    reference's step loop does; 20 steps against the oracle, 1e-10."""
+    nx, ny = 322, 179
+    say(f"stellar wind {nx}x{ny} 20 steps")
+    kw = PB.wind_grid_args(nx, ny)
+    g, og = grids(nx, ny, kw.pop("bc"), **kw)
+    s, u, b = PB.stellar_wind_prim(og)
+    # host loop through the oracle
+    hs, hu, hb = s.copy(), u.copy(), b.copy()
+    T, Tdo, TT, it = 1e7, 1e-8, 0.0, 0
+    for k in range(20):
+        kind = "add2" if k < 10 else "kernel_tvd"
+        hs, hu, hb, T = O.ref_mhd_step(kind, og, hs, hu, hb, T, Tdo, it, 2)
+        T, Tdo, TT, it = O.funk_time(T, Tdo, TT, it)
+    # device loop through the C ABI
+    sd, ud, bd = to_dev(s), to_dev(u), to_dev(b)
+    s2, u2, b2 = torch.empty_like(sd), torch.empty_like(ud), torch.empty_like(bd)
+    Td = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    Tdod = torch.tensor([1e-8], dtype=torch.float64, device=dev())
+    TTd = torch.zeros(1, dtype=torch.float64, device=dev())
+    itd = torch.zeros(1, dtype=torch.int32, device=dev())
+    for k in range(20):
+        fn = api.add2 if k < 10 else api.kernel_tvd
+        fn(g, sd, ud, bd, s2, u2, b2, Td, Tdod, 0, 2)
+        sd, s2, ud, u2, bd, b2 = s2, sd, u2, ud, b2, bd
+        api.funk_time(Td, Tdod, TTd, itd)
+    gs, gu, gb = to_host(sd), to_host(ud), to_host(bd)
+    assert abs(TTd.item() - TT) <= 1e-12 * TT and itd.item() == it
+    assert np.max(np.abs(gs - hs) / np.abs(hs)) <= 1e-10
+    assert np.max(np.abs(gu - hu)) <= 1e-10
+    assert np.max(np.abs(gb - hb)) <= 1e-10
+    assert np.all(gs[..., 1] > 0)

@@ -1,0 +1,207 @@
+"""Pins of the oracle's production options of the MHD reference-layout steps
+(add2 / Kernel_TVD): axisymmetric source terms, the four production side
+kinds, the frozen inner sphere (kernel.cu:1918-2190, 2218-2650).
+
+* Sources: on a uniform state every face flux cancels, so one step is a
+  forward-Euler step of the geometric and Powell source terms alone. The
+  expected update is written from the axisymmetric ideal-MHD equations in
+  cylindrical (z = x, r = y) form: stress tensor Pi_rr - Pi_phiphi hoop term,
+  r-flux of each conserved field over r, Powell's 8-wave term with
+  div B = dBz/dz + dBr/dr + Br/r.
+* Side kinds: the step on a grid with the production sides equals, bit for
+  bit, the interior of the same step on a grid two cells larger on every
+  side whose extra cells hold the ghost states explicitly, filled here from
+  the reference's rules (symmetry west, outflow east, axis south, inflow
+  north).
+* Frozen sphere: cells inside keep their state, every other cell is the
+  step without the sphere bit for bit.
+* The quiescent outer state of the production run is a steady state.
+"""
+import numpy as np
+import pytest
+
+from golikov_2d_amd import problems as PB
+from oracle import oracle as O
+
+CPI4 = 4.0 * np.pi
+CPI8 = 8.0 * np.pi
+G53 = 5.0 / 3.0
+PROD_BC = (O.BC_SYMMETRY, O.BC_OUTFLOW, O.BC_AXIS, O.BC_INFLOW)
+
+
+def cyl_source(q, y, gamma):
+    """d/dt of (rho, rho u, rho v, Bx, By, Bz, E) for a uniform state at
+    radius y from the cylindrical-form equations (w = 0)."""
+    r, p, u, v, bx, by, bz = q
+    b2 = bx * bx + by * by + bz * bz
+    P = p + b2 / CPI8
+    E = p / (gamma - 1.0) + 0.5 * r * (u * u + v * v) + b2 / CPI8
+    pi_rr = r * v * v + P - by * by / CPI4
+    pi_pp = P - bz * bz / CPI4
+    divB = by / y                      # only the Br / r part is non-zero
+    ub = u * bx + v * by
+    return np.array([
+        -r * v / y,
+        -(r * u * v - bx * by / CPI4) / y - divB * bx / CPI4,
+        -(pi_rr - pi_pp) / y - divB * by / CPI4,
+        -(v * bx - u * by) / y - divB * u,
+        -divB * v,
+        0.0,
+        -((E + P) * v - by * ub / CPI4) / y - divB * ub / CPI4,
+    ])
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+def test_axisymmetric_sources_on_uniform_state(kind):
+    q = (1.3, 0.9, 0.4, -0.3, 0.5, 0.7, -0.6)
+    nx, ny, dx, dy, y0 = 6, 5, 0.1, 0.1, 0.45
+    g = O.make_grid(nx, ny, dx=dx, dy=dy, y0=y0, gamma=G53, axisym=True)
+    s = np.empty((ny, nx, 2)); u = np.empty((ny, nx, 2)); b = np.empty((ny, nx, 3))
+    s[...] = (q[0], q[1]); u[...] = (q[2], q[3]); b[...] = q[4:]
+    dt = 1e-3
+    s2, u2, b2, _ = O.ref_mhd_step(kind, g, s, u, b, 1e7, dt, 0, 3)
+    r, p = q[0], q[1]
+    U = np.array([r, r * q[2], r * q[3], q[4], q[5], q[6],
+                  p / (G53 - 1) + 0.5 * r * (q[2] ** 2 + q[3] ** 2) + (q[4] ** 2 + q[5] ** 2 + q[6] ** 2) / CPI8])
+    for j in range(ny):
+        y = y0 + j * dy
+        W = U + dt * cyl_source(q, y, G53)
+        ux, uy = W[1] / W[0], W[2] / W[0]
+        pe = (W[6] - 0.5 * W[0] * (ux * ux + uy * uy) - (W[3] ** 2 + W[4] ** 2 + W[5] ** 2) / CPI8) * (G53 - 1)
+        want = np.array([W[0], pe, ux, uy, W[3], W[4], W[5]])
+        for i in range(nx):
+            got = np.array([s2[j, i, 0], s2[j, i, 1], u2[j, i, 0], u2[j, i, 1], *b2[j, i]])
+            assert np.allclose(got, want, rtol=1e-12, atol=1e-14), (j, i, got - want)
+    # without the option the uniform state does not move
+    g.axisym = 0
+    s2, u2, b2, _ = O.ref_mhd_step(kind, g, s, u, b, 1e7, dt, 0, 3)
+    assert np.allclose(s2, s, rtol=1e-14) and np.allclose(u2, u, atol=1e-14) and np.allclose(b2, b, atol=1e-14)
+
+
+def smooth_state(g, seed):
+    """Smooth state with every primitive non-zero and vx < 0 somewhere on the
+    east edge (so the outflow rule acts)."""
+    x = g.x0 + np.arange(g.nx) * g.dx
+    y = g.y0 + np.arange(g.ny) * g.dy
+    X, Y = np.meshgrid(x, y)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.5, 8)
+    s = np.empty((g.ny, g.nx, 2)); u = np.empty((g.ny, g.nx, 2)); b = np.empty((g.ny, g.nx, 3))
+    s[..., 0] = 1.0 + 0.3 * np.sin(a[0] * X + 0.3) * np.cos(a[1] * Y)
+    s[..., 1] = 0.8 + 0.2 * np.cos(a[2] * X - a[3] * Y)
+    u[..., 0] = 0.5 * np.sin(a[4] * Y + 0.7) - 0.1
+    u[..., 1] = 0.4 * np.cos(a[5] * X)
+    b[..., 0] = 0.6 * np.cos(a[6] * Y)
+    b[..., 1] = 0.5 * np.sin(a[7] * X + 0.2)
+    b[..., 2] = 0.4 * np.cos(X + Y)
+    return s, u, b
+
+
+def explicit_ghosts(s, u, b, gamma, inflow_b_zero):
+    """Copy of (s, u, b) two cells larger on each side, the extra cells set by
+    the reference's production ghost rules (kernel.cu:1938-2016,
+    2227-2405); corner cells are not read by the cross stencil."""
+    ny, nx = s.shape[:2]
+    S = np.zeros((ny + 4, nx + 4, 2)); Uv = np.zeros((ny + 4, nx + 4, 2)); B = np.zeros((ny + 4, nx + 4, 3))
+    S[...] = (1.0, 1.0 / gamma)
+    S[2:-2, 2:-2], Uv[2:-2, 2:-2], B[2:-2, 2:-2] = s, u, b
+    for layer in (1, 2):
+        # west symmetry: mirror, vx, Bx, By negated
+        src = layer - 1
+        S[2:-2, 2 - layer] = s[:, src]
+        Uv[2:-2, 2 - layer] = u[:, src] * (-1, 1)
+        B[2:-2, 2 - layer] = b[:, src] * (-1, -1, 1)
+        # east outflow: edge cell, vx < 0 raised to 0.01
+        S[2:-2, nx + 1 + layer] = s[:, -1]
+        ue = u[:, -1].copy()
+        ue[ue[:, 0] < 0, 0] = 0.01
+        Uv[2:-2, nx + 1 + layer] = ue
+        B[2:-2, nx + 1 + layer] = b[:, -1]
+        # south axis: mirror, vy and Bz negated
+        S[2 - layer, 2:-2] = s[src]
+        Uv[2 - layer, 2:-2] = u[src] * (1, -1)
+        B[2 - layer, 2:-2] = b[src] * (1, 1, -1)
+        # north inflow: rho = 1, p = 1/gamma, v = 0, B of the edge cell or 0
+        S[ny + 1 + layer, 2:-2] = (1.0, 1.0 / gamma)
+        Uv[ny + 1 + layer, 2:-2] = 0.0
+        B[ny + 1 + layer, 2:-2] = 0.0 if inflow_b_zero else b[-1]
+    return S, Uv, B
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+@pytest.mark.parametrize("method", [2, 3])
+@pytest.mark.parametrize("axisym", [False, True])
+def test_production_sides_equal_explicit_ghost_cells(kind, method, axisym):
+    nx, ny, h = 13, 11, 0.125
+    x0, y0 = 0.0625, 3.0625                # exact in binary on both grids
+    g = O.make_grid(nx, ny, dx=h, dy=h, x0=x0, y0=y0, gamma=G53, bc=PROD_BC, axisym=axisym)
+    s, u, b = smooth_state(g, seed=nx + method)
+    dt = 2e-3
+    s2, u2, b2, _ = O.ref_mhd_step(kind, g, s, u, b, 1e7, dt, 0, method)
+    S, Uv, B = explicit_ghosts(s, u, b, G53, inflow_b_zero=(kind == "kernel_tvd"))
+    gb = O.make_grid(nx + 4, ny + 4, dx=h, dy=h, x0=x0 - 2 * h, y0=y0 - 2 * h, gamma=G53, axisym=axisym)
+    S2, U2, B2, _ = O.ref_mhd_step(kind, gb, S, Uv, B, 1e7, dt, 0, method)
+    np.testing.assert_array_equal(s2, S2[2:-2, 2:-2])
+    np.testing.assert_array_equal(u2, U2[2:-2, 2:-2])
+    np.testing.assert_array_equal(b2, B2[2:-2, 2:-2])
+    # the side kinds act: the step differs from the transmissive one at the edges
+    gt = O.make_grid(nx, ny, dx=h, dy=h, x0=x0, y0=y0, gamma=G53, axisym=axisym)
+    s2t, _, b2t, _ = O.ref_mhd_step(kind, gt, s, u, b, 1e7, dt, 0, method)
+    for sl in (np.s_[:, 0], np.s_[:, -1], np.s_[0, :], np.s_[-1, :]):
+        assert np.any(b2t[sl] != b2[sl]) or np.any(s2t[sl] != s2[sl])
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+def test_frozen_sphere_keeps_state_and_leaves_the_rest(kind):
+    nx, ny = 40, 24
+    kw = PB.wind_grid_args(nx, ny, x_max=2.0, y_max=1.2)
+    g = O.make_grid(nx, ny, **kw)
+    s, u, b = smooth_state(g, seed=3)
+    x = g.x0 + np.arange(nx) * g.dx
+    y = g.y0 + np.arange(ny) * g.dy
+    X, Y = np.meshgrid(x, y)
+    d = np.sqrt(X * X + Y * Y)
+    assert np.min(np.abs(d - g.r_frozen)) > 1e-6
+    inside = d < g.r_frozen
+    assert 5 < inside.sum() < inside.size // 4
+    s2, u2, b2, T = O.ref_mhd_step(kind, g, s, u, b, 1e7, 1e-3, 0, 3)
+    g.r_frozen = 0.0
+    s2n, u2n, b2n, Tn = O.ref_mhd_step(kind, g, s, u, b, 1e7, 1e-3, 0, 3)
+    np.testing.assert_array_equal(s2[inside], s[inside])
+    np.testing.assert_array_equal(b2[inside], b[inside])
+    np.testing.assert_array_equal(u2[~inside], u2n[~inside])
+    np.testing.assert_array_equal(b2[~inside], b2n[~inside])
+    assert np.any(s2n[inside] != s[inside])
+    assert T >= Tn
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+def test_quiescent_outer_state_is_steady(kind):
+    g = O.make_grid(30, 20, **PB.wind_grid_args(30, 20, x_max=3.0, y_max=2.0))
+    g.r_frozen = 0.0
+    s = np.empty((20, 30, 2)); s[...] = (1.0, 1.0 / G53)
+    u = np.zeros((20, 30, 2)); b = np.zeros((20, 30, 3))
+    s2, u2, b2, _ = O.ref_mhd_step(kind, g, s, u, b, 1e7, 1e-3, 0, 2)
+    np.testing.assert_array_equal(s2, s)
+    np.testing.assert_array_equal(u2, u)
+    np.testing.assert_array_equal(b2, b)
+
+
+def test_wind_initial_state_matches_closed_form():
+    g = O.make_grid(161, 90, **PB.wind_grid_args(161, 90))
+    s, u, b = PB.stellar_wind_prim(g)
+    x = g.x0 + np.arange(g.nx) * g.dx
+    y = g.y0 + np.arange(g.ny) * g.dy
+    X, Y = np.meshgrid(x, y)
+    d = np.hypot(X, Y)
+    inner = d < 0.303
+    assert inner.sum() > 10
+    # mass flux rho |v| r^2 is constant in the wind, p rho^-gamma too
+    flux = s[inner, 0] * np.hypot(u[inner, 0], u[inner, 1]) * d[inner] ** 2
+    assert np.allclose(flux, flux[0], rtol=1e-13)
+    ent = s[inner, 1] / s[inner, 0] ** G53
+    assert np.allclose(ent, ent[0], rtol=1e-12)
+    # B_phi r / sin(theta) constant, Alfven Mach number M_alf at r = rr_0
+    bphi = b[inner, 2] * d[inner] / (Y[inner] / d[inner])
+    assert np.allclose(bphi, np.sqrt(4 * np.pi) / PB.WIND_MALF, rtol=1e-12)
+    assert np.all(s[~inner, 0] == 1.0) and np.all(b[~inner] == 0.0) and np.all(u[~inner] == 0.0)
