@@ -44,7 +44,7 @@ def parse():
                     help="skip the batched-ensemble, adjoint, MHD and Sod legs")
     ap.add_argument("--only-extras", action="store_true",
                     help="run only the batched-ensemble, adjoint, MHD and Sod legs (for profiling them)")
-    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod", "reflayout"], default=None,
+    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod", "reflayout", "io"], default=None,
                     help="run only one of those legs (for profiling it)")
     return ap.parse_args()
 
@@ -372,6 +372,65 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
     return out
 
 
+def io_formats(with_cpu=True, sample_rows=112):
+    """The reference's on-disk formats written by the library's host writers
+    at the production size (stellar-wind state, 3226 x 1792, host buffers):
+    text state / restart, save_b3.dat, Tecplot every 3rd cell; and the text
+    restart read back. MB/s of file per second of wall time, best of two. The
+    oracle's one-cell-at-a-time stdio writer (the reference's ofstream loop)
+    is timed beside it on the first sample_rows rows."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    nx, ny = PB.WIND_NX, PB.WIND_NY
+    kw = PB.wind_grid_args()
+    g = api.make_grid(nx, ny, **kw)
+    s, u, b = PB.stellar_wind_prim(g)
+    d = tempfile.mkdtemp(prefix="gd2_io_")
+    out = {"workload": f"stellar-wind state {nx}x{ny} (host buffers), files under {tempfile.gettempdir()}",
+           "threads": min(16, os.cpu_count() or 1) if not os.environ.get("GD2_IO_THREADS")
+           else int(os.environ["GD2_IO_THREADS"])}
+    try:
+        def timed(fn, path):
+            best = None
+            for _ in range(2):
+                t0 = time.perf_counter()
+                fn(path)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            return best, os.path.getsize(path)
+
+        for name, fn in (("state_text", lambda p: api.write_state_text(g, s, u, b, p)),
+                         ("save_b3", lambda p: api.write_state_b3(g, s, u, b, p)),
+                         ("tecplot_nmin3", lambda p: api.write_tecplot(g, s, u, b, p, nmin=3, solution_time=0.0))):
+            path = os.path.join(d, name)
+            t, size = timed(fn, path)
+            out[name] = {"MB_per_s": round(size / t / 1e6, 1), "seconds": round(t, 4), "bytes": size}
+        path = os.path.join(d, "state_text")
+        t0 = time.perf_counter()
+        s2, _, _ = api.read_state_text(g, path)
+        t = time.perf_counter() - t0
+        out["read_state_text"] = {"MB_per_s": round(os.path.getsize(path) / t / 1e6, 1), "seconds": round(t, 4),
+                                  "rho_max_rel_err": float(np.max(np.abs(s2[..., 0] - s[..., 0]) / s[..., 0]))}
+        if with_cpu:
+            from oracle import oracle as O
+            og = O.make_grid(nx, sample_rows, **kw)
+            ss, us, bs = (np.ascontiguousarray(a[:sample_rows]) for a in (s, u, b))
+            path = os.path.join(d, "oracle_text")
+            t0 = time.perf_counter()
+            O.write_state_text(og, ss, us, bs, path)
+            t = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(os.path.getsize(path) / t / 1e6, 1), "unit": "MB/s", "cores": 1,
+                                   "kind": "port", "sample": f"oracle stdio text writer, first {sample_rows} rows"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
     """Differentiable rollout (LibTorch autograd Function): backward (recompute
     from checkpoints + stage VJPs) against the forward rollout, both timed."""
@@ -446,7 +505,8 @@ def main():
         legs = {"batched": lambda: batched_vs_serial(args.seed), "adjoint": lambda: adjoint_vs_forward(seed=args.seed),
                 "mhd": lambda: mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                 "sod": lambda: sod_godunov(with_cpu=not args.no_cpu),
-                "reflayout": lambda: reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}
+                "reflayout": lambda: reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
+                "io": lambda: io_formats(with_cpu=not args.no_cpu)}
         print(json.dumps({args.only_leg: legs[args.only_leg]()}), flush=True)
         return
     if args.only_extras:
@@ -454,7 +514,8 @@ def main():
                           "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
                           "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                           "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
-                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}), flush=True)
+                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
+                  "io_formats": io_formats(with_cpu=not args.no_cpu)}), flush=True)
         return
 
     n = args.n
@@ -544,7 +605,8 @@ def main():
                   "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
                   "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                   "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
-                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu)}
+                  "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
+                  "io_formats": io_formats(with_cpu=not args.no_cpu)}
 
     out = {
         "metric": METRIC,

@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -108,6 +109,10 @@ def lib():
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
             "gd2_set_step_path": (ip, [ip]),
             "gd2_one_pass_preferred": (ip, [gp, sp]),
+            "gd2_write_state_text": (ip, [gp, vp, vp, vp, C.c_char_p]),
+            "gd2_read_state_text": (ip, [gp, C.c_char_p, vp, vp, vp]),
+            "gd2_write_state_b3": (ip, [gp, vp, vp, vp, C.c_char_p]),
+            "gd2_write_tecplot": (ip, [gp, vp, vp, vp, ip, C.POINTER(dp), C.c_char_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -124,7 +129,8 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
             "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
-            "gd2_one_pass_preferred"]
+            "gd2_one_pass_preferred", "gd2_write_state_text", "gd2_read_state_text", "gd2_write_state_b3",
+            "gd2_write_tecplot"]
 
 
 def _check(rc, name):
@@ -395,3 +401,49 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------ on-disk formats (host buffers)
+
+def _host(g, a, last):
+    """Pointer to a C-contiguous float64 host array of shape (ny, nx, last)."""
+    if a is None:
+        return None
+    if isinstance(a, torch.Tensor):
+        if a.is_cuda:
+            raise GD2Error("host array expected (copy the state to the host first)")
+        a = a.numpy()
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"] or a.shape != (g.ny, g.nx, last):
+        raise GD2Error(f"C-contiguous float64 array of shape {(g.ny, g.nx, last)} expected")
+    return C.c_void_p(a.ctypes.data)
+
+
+def write_state_text(g, s, u, b, path):
+    """'x y rho p vx vy Bx By Bz' per cell, as the reference's end-of-run
+    writer (kernel.cu:4257-4260); b may be None."""
+    _check(lib().gd2_write_state_text(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3),
+                                      os.fsencode(path)), "gd2_write_state_text")
+
+
+def read_state_text(g, path, mhd=True):
+    """Restart reader (kernel.cu:3944-3969); returns host arrays s, u, b."""
+    s = np.empty((g.ny, g.nx, 2))
+    u = np.empty((g.ny, g.nx, 2))
+    b = np.empty((g.ny, g.nx, 3)) if mhd else None
+    _check(lib().gd2_read_state_text(C.byref(g), os.fsencode(path), _host(g, s, 2), _host(g, u, 2),
+                                     _host(g, b, 3)), "gd2_read_state_text")
+    return s, u, b
+
+
+def write_state_b3(g, s, u, b, path):
+    """save_b3.dat: x, y, rho, p, vx, vy, Bz per cell (kernel.cu:4261-4267)."""
+    _check(lib().gd2_write_state_b3(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3),
+                                    os.fsencode(path)), "gd2_write_state_b3")
+
+
+def write_tecplot(g, s, u, b, path, nmin=3, solution_time=None):
+    """Tecplot FEPOINT zone of every nmin-th cell (print_file_mini2 /
+    print_file_mini, kernel.cu:3682-3808)."""
+    t = None if solution_time is None else C.byref(C.c_double(solution_time))
+    _check(lib().gd2_write_tecplot(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3), int(nmin), t,
+                                   os.fsencode(path)), "gd2_write_tecplot")

@@ -265,6 +265,38 @@ typedef struct gd2_diag {
 } gd2_diag;
 int gd2_get_diagnostics(gd2_diag* out, int reset);
 
+/* ----------------------------------------------------------------------
+ * On-disk formats of the reference's host driver. HOST buffers in the
+ * reference layout (the reference copies its state to host_s / host_u /
+ * host_b before writing); no device work. Numbers are written as its
+ * std::ofstream writes them: printf "%g" (6 significant digits). Cell
+ * centres are x = x0 + n*dx, y = y0 + m*dy. GD2_ERR_ARG on a bad argument or
+ * an I/O failure. GD2_IO_THREADS caps the formatting threads (default: the
+ * host's, at most 16).
+ * ---------------------------------------------------------------------- */
+
+/* Text state, one cell per line: "x y rho p vx vy Bx By Bz"
+ * (instable_for_inner_b3.txt, kernel.cu:4257-4260); b may be NULL (zeros). */
+int gd2_write_state_text(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                         const gd2_double3* b, const char* path);
+/* Restart reader (kernel.cu:3944-3969): 9 whitespace-separated numbers per
+ * cell, the first two (coordinates) skipped; b may be NULL (Euler). */
+int gd2_read_state_text(const gd2_grid* g, const char* path, gd2_double2* s, gd2_double2* u,
+                        gd2_double3* b);
+/* save_b3.dat (kernel.cu:4261-4267): 7 native-endian doubles per cell,
+ * x, y, rho, p, vx, vy, Bz. */
+int gd2_write_state_b3(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                       const gd2_double3* b, const char* path);
+/* Tecplot FEPOINT quadrilateral zone of every nmin-th cell in x and y
+ * (print_file_mini2 kernel.cu:3724-3808 with b, print_file_mini
+ * kernel.cu:3682-3722 with b == NULL; the reference's Nmin is 3):
+ * X Y Ro P Vx Vy [Bx By Bz] Max T, Max = |v| / sqrt(gamma p / rho),
+ * T = p / rho, on that writer's own coordinates (cell centres stretched
+ * over [x0, right edge] with an (nx - 1) spacing); solution_time != NULL adds
+ * the SOLUTIONTIME clause. */
+int gd2_write_tecplot(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                      const gd2_double3* b, int nmin, const double* solution_time, const char* path);
+
 /* Last HIP error code seen by the library (hipError_t value). */
 int gd2_last_hip_error(void);
 /* Library version string. */

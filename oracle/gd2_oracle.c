@@ -16,6 +16,7 @@
 #include "gd2_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1313,4 +1314,112 @@ void o_cons_to_prim(const o_grid* g, int physics, const double* U, o_d2* s, o_d2
         s[k].x = q.r; s[k].y = q.p; u[k].x = q.u; u[k].y = q.v;
         if (b) { b[k].x = q.bx; b[k].y = q.by; b[k].z = q.bz; }
     }
+}
+
+/* ------------------------------------------------------------------ on-disk formats
+ * One cell at a time with stdio, as the reference's host driver writes with
+ * std::ofstream (whose default float output is printf "%g"). */
+
+/* instable_for_inner_b3.txt writer (kernel.cu:4252-4260) */
+int o_write_state_text(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path)
+{
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    const long K = (long)g->nx * g->ny;
+    for (long k = 0; k < K; ++k) {
+        int n = (int)(k % g->nx), m = (int)(k / g->nx);
+        double y = g->y0 + m * g->dy;
+        double x = g->x0 + n * g->dx;
+        fprintf(f, "%g %g %g %g %g %g %g %g %g\n", x, y, s[k].x, s[k].y, u[k].x, u[k].y,
+                b ? b[k].x : 0.0, b ? b[k].y : 0.0, b ? b[k].z : 0.0);
+    }
+    return fclose(f) == 0 ? 0 : -1;
+}
+
+/* restart reader (kernel.cu:3944-3969): c1 c2 a1 ... a7 per cell */
+int o_read_state_text(const o_grid* g, const char* path, o_d2* s, o_d2* u, o_d3* b)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) return -1;
+    const long K = (long)g->nx * g->ny;
+    for (long k = 0; k < K; ++k) {
+        double c1, c2, a[7];
+        if (fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf", &c1, &c2, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5],
+                   &a[6]) != 9) {
+            fclose(f);
+            return -1;
+        }
+        s[k].x = a[0]; s[k].y = a[1];
+        u[k].x = a[2]; u[k].y = a[3];
+        if (b) { b[k].x = a[4]; b[k].y = a[5]; b[k].z = a[6]; }
+    }
+    fclose(f);
+    return 0;
+}
+
+/* save_b3.dat (kernel.cu:4261-4267) */
+int o_write_state_b3(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path)
+{
+    FILE* f = fopen(path, "wb");
+    if (!f) return -1;
+    const long K = (long)g->nx * g->ny;
+    int ok = 1;
+    for (long k = 0; k < K && ok; ++k) {
+        int n = (int)(k % g->nx), m = (int)(k / g->nx);
+        double y = g->y0 + m * g->dy;
+        double x = g->x0 + n * g->dx;
+        ok = fwrite(&x, 8, 1, f) == 1 && fwrite(&y, 8, 1, f) == 1 && fwrite(&s[k].x, 8, 1, f) == 1 &&
+             fwrite(&s[k].y, 8, 1, f) == 1 && fwrite(&u[k].x, 8, 1, f) == 1 && fwrite(&u[k].y, 8, 1, f) == 1 &&
+             fwrite(&b[k].z, 8, 1, f) == 1;
+    }
+    return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+/* print_file_mini2 (kernel.cu:3724-3808) with b, print_file_mini
+ * (kernel.cu:3682-3722) without; x_max / y_max are the domain's edges */
+int o_write_tecplot(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int nmin,
+                    const double* solution_time, const char* path)
+{
+    if (nmin < 1) return -1;
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    const int N = g->nx, M = g->ny;
+    const long K = (long)N * M;
+    const int nn = (N + nmin - 1) / nmin, mm = (M + nmin - 1) / nmin;
+    const double x_min = g->x0, y_min = g->y0;
+    const double x_max = g->x0 + (N - 0.5) * g->dx, y_max = g->y0 + (M - 0.5) * g->dy;
+    if (b)
+        fprintf(f, "TITLE = \"HP\"  VARIABLES = \"X\", \"Y\", \"Ro\", \"P\", \"Vx\", \"Vy\", \"Bx\", \"By\", \"Bz\", "
+                   "\"Max\", \"T\", ZONE T = \"HP\", N = %d , E = %d, F = FEPOINT, ET = quadrilateral",
+                nn * mm, (nn - 1) * (mm - 1));
+    else
+        fprintf(f, "TITLE = \"HP\"  VARIABLES = \"X\", \"Y\", \"Ro\", \"P\", \"Vx\", \"Vy\", \"Max\", \"T\", "
+                   "ZONE T = \"HP\", N = %d , E = %d, F = FEPOINT, ET = quadrilateral",
+                nn * mm, (nn - 1) * (mm - 1));
+    if (solution_time) fprintf(f, ", SOLUTIONTIME = %g", *solution_time);
+    fprintf(f, "\n");
+    for (long k = 0; k < K; k++) {
+        int n = (int)(k % N);
+        int m = (int)((k - n) / N);
+        if ((n % nmin != 0) || (m % nmin != 0)) continue;
+        double y = M > 1 ? y_min + m * (y_max - y_min) / (M - 1) : y_min;
+        double x = N > 1 ? x_min + n * (x_max - x_min) / (N - 1) : x_min;
+        double Max = 0.0, Temp = 0.0;
+        if (s[k].x > 0.0) {
+            Max = sqrt((u[k].x * u[k].x + u[k].y * u[k].y) / (g->gamma * s[k].y / s[k].x));
+            Temp = s[k].y / s[k].x;
+        }
+        if (b)
+            fprintf(f, "%g %g %g %g %g %g %g %g %g %g %g\n", x, y, s[k].x, s[k].y, u[k].x, u[k].y, b[k].x, b[k].y,
+                    b[k].z, Max, Temp);
+        else
+            fprintf(f, "%g %g %g %g %g %g %g %g\n", x, y, s[k].x, s[k].y, u[k].x, u[k].y, Max, Temp);
+    }
+    for (int k = 0; k < nn * mm; k = k + 1) {
+        int n = k % nn;
+        int m = (k - n) / nn;
+        if ((m < mm - 1) && (n < nn - 1))
+            fprintf(f, "%d %d %d %d\n", m * nn + n + 1, m * nn + n + 2, (m + 1) * nn + n + 2, (m + 1) * nn + n + 1);
+    }
+    return fclose(f) == 0 ? 0 : -1;
 }

@@ -117,6 +117,15 @@ long o_run(const o_grid* g, const o_scheme* sc, int integrator, double* U, doubl
 void o_prim_to_cons(const o_grid* g, int physics, const o_d2* s, const o_d2* u, const o_d3* b, double* U);
 void o_cons_to_prim(const o_grid* g, int physics, const double* U, o_d2* s, o_d2* u, o_d3* b);
 
+/* The reference's host writers / reader restated with stdio (its ofstream
+ * default format is printf "%g"). 0 on success, -1 on a bad argument or an
+ * I/O failure. */
+int o_write_state_text(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path);
+int o_read_state_text(const o_grid* g, const char* path, o_d2* s, o_d2* u, o_d3* b);
+int o_write_state_b3(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path);
+int o_write_tecplot(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int nmin,
+                    const double* solution_time, const char* path);
+
 int o_num_threads(void);
 void o_set_num_threads(int n);
 

@@ -100,6 +100,13 @@ def lib():
         L.o_diag_reset.restype = None
         L.o_diag_get.restype = None
         L.o_diag_get.argtypes = [C.POINTER(Diag)]
+        for fn in (L.o_write_state_text, L.o_write_state_b3):
+            fn.restype = C.c_int
+            fn.argtypes = [gp, vp, vp, vp, C.c_char_p]
+        L.o_read_state_text.restype = C.c_int
+        L.o_read_state_text.argtypes = [gp, C.c_char_p, vp, vp, vp]
+        L.o_write_tecplot.restype = C.c_int
+        L.o_write_tecplot.argtypes = [gp, vp, vp, vp, C.c_int, dp, C.c_char_p]
         L.o_num_threads.restype = C.c_int
         L.o_set_num_threads.argtypes = [C.c_int]
         _LIB = L
@@ -253,3 +260,32 @@ def num_threads():
 
 def set_num_threads(n):
     lib().o_set_num_threads(n)
+
+
+# ---------------------------------------------------------------- on-disk formats
+
+def write_state_text(g, s, u, b, path):
+    if lib().o_write_state_text(C.byref(g), _ptr(s), _ptr(u), _ptr(b) if b is not None else None,
+                                os.fsencode(path)) != 0:
+        raise OSError(path)
+
+
+def read_state_text(g, path, mhd=True):
+    s = np.empty((g.ny, g.nx, 2)); u = np.empty((g.ny, g.nx, 2))
+    b = np.empty((g.ny, g.nx, 3)) if mhd else None
+    if lib().o_read_state_text(C.byref(g), os.fsencode(path), _ptr(s), _ptr(u),
+                               _ptr(b) if b is not None else None) != 0:
+        raise OSError(path)
+    return s, u, b
+
+
+def write_state_b3(g, s, u, b, path):
+    if lib().o_write_state_b3(C.byref(g), _ptr(s), _ptr(u), _ptr(b), os.fsencode(path)) != 0:
+        raise OSError(path)
+
+
+def write_tecplot(g, s, u, b, path, nmin=3, solution_time=None):
+    t = None if solution_time is None else C.byref(C.c_double(solution_time))
+    if lib().o_write_tecplot(C.byref(g), _ptr(s), _ptr(u), _ptr(b) if b is not None else None, nmin, t,
+                             os.fsencode(path)) != 0:
+        raise OSError(path)
