@@ -453,7 +453,7 @@ __device__ __forceinline__ bool frozen_m(const MhdArgs& A, int c, int j) {
 // PROD (reference layout only): production side kinds, inner frozen sphere
 // and axisymmetric sources.
 template <int ORDER, int METOD, int LAYOUT, bool WANT_W, bool PROD = false>
-__global__ __launch_bounds__(64) void mhd_tile_kernel(const MhdArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == kAoS && ORDER == 2 && METOD == 3) ? 2 : 1))) void mhd_tile_kernel(const MhdArgs A) {
     __shared__ double ring[3][kMF][kRowBuf];
 
     const int lane = threadIdx.x;
