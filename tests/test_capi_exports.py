@@ -81,3 +81,42 @@ def test_step_path_policy():
         assert api.lib().gd2_set_step_path(7) == -1
     finally:
         api.set_step_path(prev)
+
+
+@pytest.mark.skipif(not os.path.exists(api.SO_PATH), reason="libgd2.so not built")
+def test_header_is_plain_c_and_links(tmp_path):
+    """A C99 program that includes gd2.h compiles with -pedantic, links
+    against libgd2.so and runs its host-only calls (no device work): argument
+    validation of a drop-in step and the text-state writer."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use_gd2.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <string.h>
+#include "gd2.h"
+int main(int argc, char** argv) {
+    gd2_grid g;
+    memset(&g, 0, sizeof g);
+    g.nx = 2; g.ny = 1; g.dx = 0.5; g.dy = 1.0; g.x0 = 0.25; g.y0 = 0.5; g.gamma = 1.4; g.krit = 0.25;
+    gd2_double2 s[2] = {{1.0, 2.5}, {0.125, 0.1}}, u[2] = {{0.0, 0.0}, {-1.0, 3.0}};
+    /* a bad method is rejected before any device work */
+    if (gd2_ker_dekard(&g, s, u, s, u, NULL, NULL, 7, NULL) != GD2_ERR_ARG) return 2;
+    if (gd2_write_state_text(&g, s, u, NULL, argv[1]) != GD2_OK) return 3;
+    printf("%s\n", gd2_version());
+    return argc == 2 ? 0 : 4;
+}
+''')
+    exe = tmp_path / "use_gd2"
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(api.SO_PATH)
+    subprocess.run([cc, "-std=c99", "-pedantic", "-Wall", "-Werror", "-I", inc, str(src), "-o", str(exe),
+                    f"-L{libdir}", "-lgd2", f"-Wl,-rpath,{libdir}"], check=True)
+    out = tmp_path / "state.txt"
+    r = subprocess.run([str(exe), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    assert "gfx950" in r.stdout
+    assert out.read_text() == "0.25 0.5 1 2.5 0 0 0 0 0\n0.75 0.5 0.125 0.1 -1 3 0 0 0\n"
