@@ -538,7 +538,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
 }
 
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 3 : 1))) void euler_tile_kernel(const EulerArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? (LAYOUT == kAoS && ORDER == 2 ? 4 : 3) : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     euler_tile<ORDER, FLUX, LAYOUT, WANT_W, BASE>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
