@@ -231,3 +231,38 @@ def test_stellar_wind_20_steps_match_oracle():
     assert np.max(np.abs(gu - hu)) <= 1e-10
     assert np.max(np.abs(gb - hb)) <= 1e-10
     assert np.all(gs[..., 1] > 0)
+
+
+def test_stellar_wind_full_size_step_matches_oracle():
+    """The production run at its own size (3226 x 1792): one Kernel_TVD
+    (HLLC-MHD, method 2) step from the stellar-wind state after 30 add2
+    steps, against the oracle, 1e-12; the frozen sphere copied bit for bit."""
+    nx, ny = PB.WIND_NX, PB.WIND_NY
+    say(f"stellar wind full size {nx}x{ny}")
+    kw = PB.wind_grid_args()
+    g, og = grids(nx, ny, kw.pop("bc"), **kw)
+    s, u, b = PB.stellar_wind_prim(og)
+    sd, ud, bd = to_dev(s), to_dev(u), to_dev(b)
+    s2, u2, b2 = torch.empty_like(sd), torch.empty_like(ud), torch.empty_like(bd)
+    Td = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    Tdod = torch.tensor([1e-8], dtype=torch.float64, device=dev())
+    TTd = torch.zeros(1, dtype=torch.float64, device=dev())
+    itd = torch.zeros(1, dtype=torch.int32, device=dev())
+    for _ in range(30):
+        api.add2(g, sd, ud, bd, s2, u2, b2, Td, Tdod, 0, 2)
+        sd, s2, ud, u2, bd, b2 = s2, sd, u2, ud, b2, bd
+        api.funk_time(Td, Tdod, TTd, itd)
+    s0, u0, b0 = to_host(sd), to_host(ud), to_host(bd)
+    T0, Tdo = Td.item(), Tdod.item()
+    s2r, u2r, b2r, Tr = O.ref_mhd_step("kernel_tvd", og, s0, u0, b0, T0, Tdo, 0, 2)
+    api.kernel_tvd(g, sd, ud, bd, s2, u2, b2, Td, Tdod, 0, 2)
+    gs, gu, gb = to_host(s2), to_host(u2), to_host(b2)
+    assert np.max(np.abs(gs - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(gu - u2r)) <= 1e-12
+    assert np.max(np.abs(gb - b2r)) <= 1e-12
+    assert abs(Td.item() - Tr) <= 1e-14 * Tr
+    x = og.x0 + np.arange(nx) * og.dx
+    y = og.y0 + np.arange(ny) * og.dy
+    X, Y = np.meshgrid(x, y)
+    inside = np.sqrt(X * X + Y * Y) < og.r_frozen
+    np.testing.assert_array_equal(gs[inside], s0[inside])
