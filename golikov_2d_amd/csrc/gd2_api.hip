@@ -48,11 +48,13 @@ static DeviceDiag* diag_ptr() {
     return p;
 }
 
-// allow_prod: the production options of the MHD reference-layout steps
-// (side kinds SYMMETRY / AXIS / OUTFLOW / INFLOW, each on its own side, the
-// frozen sphere and the axisymmetric sources); everywhere else they are
-// rejected rather than ignored.
-static bool grid_ok(const gd2_grid* g, bool allow_frozen, bool allow_prod = false) {
+// prod: which production options of the reference-layout steps an entry
+// point takes: kProdSides (side kinds SYMMETRY / AXIS / OUTFLOW / INFLOW,
+// each on its own side: the MHD steps) and kProdOptions (the frozen sphere
+// and the axisymmetric sources: the MHD steps and Kernel_TVD_old);
+// everywhere else they are rejected rather than ignored.
+enum { kProdSides = 1, kProdOptions = 2, kProdAll = 3 };
+static bool grid_ok(const gd2_grid* g, bool allow_frozen, int prod = 0) {
     if (!g || g->nx < 1 || g->ny < 1) return false;
     if (!(g->dx > 0.0) || !(g->dy > 0.0) || !(g->gamma > 1.0)) return false;
     if ((long long)g->nx * g->ny > (1ll << 40)) return false;
@@ -60,9 +62,9 @@ static bool grid_ok(const gd2_grid* g, bool allow_frozen, bool allow_prod = fals
     for (int s = 0; s < 4; ++s) {
         if (g->bc[s] < GD2_BC_TRANSMISSIVE || g->bc[s] > GD2_BC_INFLOW) return false;
         if (g->bc[s] == GD2_BC_FROZEN && !allow_frozen) return false;
-        if (g->bc[s] >= GD2_BC_SYMMETRY && (!allow_prod || g->bc[s] != own_side[s])) return false;
+        if (g->bc[s] >= GD2_BC_SYMMETRY && (!(prod & kProdSides) || g->bc[s] != own_side[s])) return false;
     }
-    if (!allow_prod && (g->r_frozen != 0.0 || g->axisym != 0)) return false;
+    if (!(prod & kProdOptions) && (g->r_frozen != 0.0 || g->axisym != 0)) return false;
     if (!(g->r_frozen >= 0.0)) return false;
     if (g->axisym && !(g->y0 > 0.0)) return false;
     // a periodic side needs a periodic opposite side
@@ -158,13 +160,13 @@ static unsigned euler_blocks(const EulerArgs& A) {
     return (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE>
+template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE, bool PROD = false>
 static int launch_euler(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
-    const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE>);
+    const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>);
     A.rows = strip_rows(k, A.tiles_x, A.ny, 2);
-    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE>), dim3(euler_blocks(A)), dim3(kWave), 0, st,
-                       A);
+    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>), dim3(euler_blocks(A)), dim3(kWave), 0,
+                       st, A);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
@@ -430,7 +432,9 @@ int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, 
 
 static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2,
                      gd2_double2* u2, double* T, const double* T_do, int method, void* stream) {
-    if (!grid_ok(g, true) || !s || !u || !s2 || !u2 || !T || !T_do) return GD2_ERR_ARG;
+    // Kernel_TVD_old (order 2) takes the sphere and the 1/y terms, Ker_Dekard none
+    if (!grid_ok(g, true, order == 2 ? kProdOptions : 0) || !s || !u || !s2 || !u2 || !T || !T_do)
+        return GD2_ERR_ARG;
     if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2) return GD2_ERR_ARG;
     if (method != 0 && method != 1 && method != 2) return GD2_ERR_UNSUPPORTED;
     EulerArgs A;
@@ -443,8 +447,18 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     A.tmin_bits = reinterpret_cast<unsigned long long*>(T);
     A.a = 0.0;
     A.b = 1.0;
+    A.x0 = g->x0;
+    A.y0 = g->y0;
+    A.r_frozen = g->r_frozen;
+    A.axisym = g->axisym ? 1 : 0;
     if (!A.diag) return GD2_ERR_HIP;
     const int flux = method == 2 ? GD2_FLUX_EXACT : (method == 0 ? GD2_FLUX_HLL : GD2_FLUX_HLLC);
+    if (order == 2 && (A.axisym || A.r_frozen > 0.0)) {
+        const hipStream_t st = (hipStream_t)stream;
+        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, kAoS, false, false, true>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, kAoS, false, false, true>(A, st);
+        return launch_euler<2, GD2_FLUX_HLL, kAoS, false, false, true>(A, st);
+    }
     return dispatch_euler<kAoS, false, false>(order, flux, A, (hipStream_t)stream);
 }
 
@@ -461,7 +475,7 @@ int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double
 int gd2_add2(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b, gd2_double2* s2,
              gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i, int method, void* stream) {
     (void)i;
-    if (!grid_ok(g, true, true)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, kProdAll)) return GD2_ERR_ARG;
     return mhd_ref_step(g, 1, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
 }
 
@@ -469,7 +483,7 @@ int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
                    gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i,
                    int method, void* stream) {
     (void)i;
-    if (!grid_ok(g, true, true)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, kProdAll)) return GD2_ERR_ARG;
     return mhd_ref_step(g, 2, s, u, b, s2, u2, b2, T, T_do, method, diag_ptr(), (hipStream_t)stream);
 }
 
@@ -482,7 +496,7 @@ int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream) {
 
 int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
                      const gd2_double3* b, double* U, void* stream) {
-    if (!grid_ok(g, true, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, kProdAll) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
     if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
     const long long n = (long long)g->nx * g->ny;
     hipLaunchKernelGGL(prim_to_cons_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream,
@@ -494,7 +508,7 @@ int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const
 
 int gd2_cons_to_prim(const gd2_grid* g, int physics, const double* U, gd2_double2* s, gd2_double2* u,
                      gd2_double3* b, void* stream) {
-    if (!grid_ok(g, true, true) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
+    if (!grid_ok(g, true, kProdAll) || !s || !u || !U || (physics == GD2_MHD && !b)) return GD2_ERR_ARG;
     if (physics != GD2_EULER && physics != GD2_MHD) return GD2_ERR_ARG;
     const long long n = (long long)g->nx * g->ny;
     hipLaunchKernelGGL(cons_to_prim_kernel, dim3(ew_blocks(n)), dim3(256), 0, (hipStream_t)stream, U,

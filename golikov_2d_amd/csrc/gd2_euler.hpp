@@ -52,6 +52,10 @@ struct EulerArgs {
     DeviceDiag* diag;
     // tiling
     int tiles_x, rows;
+    // Kernel_TVD_old's options (kAoS, order 2): inner frozen sphere
+    // (kernel.cu:3303) and axisymmetric 1/y source terms (kernel.cu:3651-3662)
+    double x0, y0, r_frozen;
+    int axisym;
 };
 
 struct Slope {
@@ -280,11 +284,16 @@ __device__ __forceinline__ void clamp_wavespeed(const EulerArgs& A, double& o0, 
     if (count) wmax = fmax(wmax, w);
 }
 
-template <int LAYOUT>
+template <int LAYOUT, bool PROD>
 __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
     if constexpr (LAYOUT == kAoS) {
-        return (A.bc[0] == GD2_BC_FROZEN && c == 0) || (A.bc[1] == GD2_BC_FROZEN && c == A.nx - 1) ||
-               (A.bc[2] == GD2_BC_FROZEN && j == 0) || (A.bc[3] == GD2_BC_FROZEN && j == A.ny - 1);
+        bool f = (A.bc[0] == GD2_BC_FROZEN && c == 0) || (A.bc[1] == GD2_BC_FROZEN && c == A.nx - 1) ||
+                 (A.bc[2] == GD2_BC_FROZEN && j == 0) || (A.bc[3] == GD2_BC_FROZEN && j == A.ny - 1);
+        if (PROD && A.r_frozen > 0.0) {
+            const double x = A.x0 + c * A.dx, y = A.y0 + j * A.dy;
+            f = f || sqrt(x * x + y * y) < A.r_frozen;
+        }
+        return f;
     } else {
         return false;
     }
@@ -292,8 +301,9 @@ __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
 
 // One wave's strip: columns tx*63 .. tx*63+62, rows ty*rows .. +rows-1.
 // BASE: the output is a*Ubase + b*(stage result) (an SSP-RK combination);
-// otherwise it is the stage result itself.
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE>
+// otherwise it is the stage result itself. PROD (kAoS, order 2): Kernel_TVD_old's
+// inner sphere and axisymmetric sources.
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, bool PROD = false>
 __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, TileLds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kTileW;
@@ -461,7 +471,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             continue;
         }
 
-        if (owns && !frozen_cell<LAYOUT>(A, c, j)) {
+        if (owns && !frozen_cell<LAYOUT, PROD>(A, c, j)) {
             const Bal P = full_balance(X, Gm, Gp, dx);
             if constexpr (LAYOUT == kSoA) {
                 double o0, o1, o2, o3;
@@ -487,21 +497,38 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 st_buf(buf_rsrc(uniform_ptr(ob + 2 * A.plane)), oo, o2);
                 st_buf(buf_rsrc(uniform_ptr(ob + 3 * A.plane)), oo, o3);
             } else {
-                // primitive-form update of Ker_Dekard (kernel.cu:1859-1874)
+                // primitive-form update of Ker_Dekard (kernel.cu:1859-1874);
+                // Kernel_TVD_old's 1/y terms with the axisymmetric option
+                // (kernel.cu:3651-3662)
                 const long long k = (long long)j * A.nx + c;
                 const PrimE& q = v0.q;
                 const double gm1 = A.gamma - 1.0;
                 const double Tdv = dt / (dx * dy);
                 const double PSr = P.r, PUx = P.mx, PUy = P.my, PSe = P.e;
-                double r2 = q.r - Tdv * PSr;
-                if (r2 <= 0.0) {
-                    atomicAdd(&A.diag->clamp_rho, 1ull);
-                    r2 = 0.0001;
+                double r2, ux, uy, p2;
+                if (PROD && ORDER == 2 && A.axisym) {
+                    const double y = A.y0 + j * dy;
+                    r2 = q.r - Tdv * PSr - dt * q.r * q.v / y;
+                    if (r2 <= 0.0) {
+                        atomicAdd(&A.diag->clamp_rho, 1ull);
+                        r2 = 0.0001;
+                    }
+                    ux = (q.r * q.u - Tdv * PUx - dt * q.r * q.v * q.u / y) / r2;
+                    uy = (q.r * q.v - Tdv * PUy - dt * q.r * q.v * q.v / y) / r2;
+                    p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe -
+                           dt * q.v * (A.gamma * q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) / y) -
+                          0.5 * r2 * (ux * ux + uy * uy)) * gm1;
+                } else {
+                    r2 = q.r - Tdv * PSr;
+                    if (r2 <= 0.0) {
+                        atomicAdd(&A.diag->clamp_rho, 1ull);
+                        r2 = 0.0001;
+                    }
+                    ux = (q.r * q.u - Tdv * PUx) / r2;
+                    uy = (q.r * q.v - Tdv * PUy) / r2;
+                    p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe) -
+                          0.5 * r2 * (ux * ux + uy * uy)) * gm1;
                 }
-                const double ux = (q.r * q.u - Tdv * PUx) / r2;
-                const double uy = (q.r * q.v - Tdv * PUy) / r2;
-                double p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe) -
-                             0.5 * r2 * (ux * ux + uy * uy)) * gm1;
                 if (p2 <= 0.0) {
                     atomicAdd(&A.diag->clamp_p, 1ull);
                     p2 = 0.000001;
@@ -537,11 +564,11 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     }
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE>
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, bool PROD = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? (LAYOUT == kAoS && ORDER == 2 ? 4 : 3) : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    euler_tile<ORDER, FLUX, LAYOUT, WANT_W, BASE>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
+    euler_tile<ORDER, FLUX, LAYOUT, WANT_W, BASE, PROD>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
 }
 
 // Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA

@@ -67,11 +67,12 @@ typedef struct gd2_grid {
     double gamma;        /* ratio of specific heats */
     double krit;         /* CFL factor of the reference-layout steps' face time step */
     int bc[4];           /* west, east, south, north: enum gd2_bc */
-    /* Production options of gd2_add2 / gd2_kernel_tvd (0 elsewhere):
-     * cells with sqrt(x^2 + y^2) < r_frozen keep their state and give no
-     * time step (the inner sphere, kernel.cu:1918, 2218); axisym != 0 adds
-     * the axisymmetric (y = r) source terms of kernel.cu:2137-2190 and
-     * 2615-2650 (requires y0 > 0). */
+    /* Production options of gd2_add2 / gd2_kernel_tvd / gd2_kernel_tvd_old
+     * (0 elsewhere): cells with sqrt(x^2 + y^2) < r_frozen keep their state
+     * and give no time step (the inner sphere, kernel.cu:1918, 2218, 3303);
+     * axisym != 0 adds the axisymmetric (y = r) source terms of
+     * kernel.cu:2137-2190, 2615-2650 (MHD) and 3651-3662 (Euler)
+     * (requires y0 > 0). */
     double r_frozen;
     int axisym;
 } gd2_grid;
@@ -84,8 +85,9 @@ typedef struct gd2_grid {
  * face time step krit*rad/max(|S_L|,|S_R|) of this step into *T with an exact
  * (deterministic) device-wide minimum. As in the reference, the step used is
  * the one computed in the previous step (lagged); gd2_funk_time rotates it.
- * Cartesian forms: the reference's axisymmetric 1/y source terms and its
- * problem-specific inner-sphere and inflow cells are not part of these.
+ * Cartesian by default; the reference's axisymmetric 1/y source terms, inner
+ * sphere and production side rules apply where the grid asks for them
+ * (gd2_grid.axisym, r_frozen, side kinds 4-7; see each entry point).
  * ---------------------------------------------------------------------- */
 
 /* Euler, first order. Replaces Ker_Dekard (kernel.cu:1736).
@@ -97,22 +99,24 @@ int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
                    gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
                    int method, void* stream);
 
-/* Euler, minmod MUSCL on primitives. Replaces the Cartesian part of
- * Kernel_TVD_old (kernel.cu:3286); method as for gd2_ker_dekard. */
+/* Euler, minmod MUSCL on primitives. Replaces Kernel_TVD_old
+ * (kernel.cu:3286); method as for gd2_ker_dekard. Takes r_frozen and axisym
+ * (its inner sphere and 1/y terms); its west inflow ghost is not offered. */
 int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
                        gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
                        int method, void* stream);
 
-/* MHD (vz = 0), first order, Powell 8-wave source. Replaces the Cartesian part
- * of add2 (kernel.cu:1877). method 2 = HLLC-MHD, 3 = HLLD (HLLDQ_Korolkov
- * metod 2 / 3, Gas_Dinamic.cu:964). i is the step counter (unused, as in the
- * reference). */
+/* MHD (vz = 0), first order, Powell 8-wave source. Replaces add2
+ * (kernel.cu:1877); its production run sets the side kinds SYMMETRY, OUTFLOW,
+ * AXIS, INFLOW, r_frozen and axisym. method 2 = HLLC-MHD, 3 = HLLD
+ * (HLLDQ_Korolkov metod 2 / 3, Gas_Dinamic.cu:964). i is the step counter
+ * (unused, as in the reference). */
 int gd2_add2(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
              gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
              int i, int method, void* stream);
 
-/* MHD, minmod MUSCL on the 7 primitives + Powell source. Replaces the
- * Cartesian part of Kernel_TVD (kernel.cu:2198). */
+/* MHD, minmod MUSCL on the 7 primitives + Powell source. Replaces
+ * Kernel_TVD (kernel.cu:2198); production options as for gd2_add2. */
 int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
                    gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
                    int i, int method, void* stream);

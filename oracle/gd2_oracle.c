@@ -961,13 +961,17 @@ static int is_frozen(const o_grid* g, int i, int j)
            (g->bc[2] == O_BC_FROZEN && j == 0) || (g->bc[3] == O_BC_FROZEN && j == g->ny - 1);
 }
 
-/* Euler reference-layout step shared by Ker_Dekard (order 1) and the
- * Cartesian part of Kernel_TVD_old (order 2). Frozen edge cells (the
- * reference's "hard" boundary rows, kernel.cu:1752-1758) are copied. */
+/* Euler reference-layout step shared by Ker_Dekard (order 1) and
+ * Kernel_TVD_old (order 2). Frozen edge cells (the reference's "hard"
+ * boundary rows, kernel.cu:1752-1758) are copied. Kernel_TVD_old's inner
+ * sphere (kernel.cu:3303) and axisymmetric 1/y source terms
+ * (kernel.cu:3651-3662) apply when the grid asks for them (order 2 only). */
 static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u, o_d2* s2, o_d2* u2,
                           double* T, const double* T_do, int method)
 {
     if (method != 0 && method != 1 && method != 2) return -1;
+    if ((g->axisym || g->r_frozen != 0.0) && order != 2) return -1;
+    if (g->axisym && !(g->y0 > 0.0)) return -1;
     src_t S = {g, 0, 0, s, u, NULL, NULL, 0};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
@@ -976,22 +980,37 @@ static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2*
     for (int j = 0; j < g->ny; ++j) {
         for (int i = 0; i < g->nx; ++i) {
             long k = (long)j * g->nx + i;
-            if (is_frozen(g, i, j)) { s2[k] = s[k]; u2[k] = u[k]; continue; }
+            const double x = g->x0 + i * dx, y = g->y0 + j * dy;
+            const int sphere = g->r_frozen > 0.0 && sqrt(x * x + y * y) < g->r_frozen;
+            if (is_frozen(g, i, j) || sphere) { s2[k] = s[k]; u2[k] = u[k]; continue; }
             stencil st;
             faces F;
             gather(&S, i, j, order, &st);
             reconstruct(g, i, j, order, 0, &st, &F);
             euler_sum E = euler_cell_fluxes(g, method, &F, 1);
             if (tglob > E.tmin) tglob = E.tmin;   /* kernel.cu:1840-1844 (exact min) */
-            /* kernel.cu:1859-1874 */
+            /* kernel.cu:1859-1874; with the 1/y terms kernel.cu:3651-3662 */
             double dV = dx * dy;
-            double r2 = s[k].x - (Tdo / dV) * E.PS.x;
-            if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = 0.0001; }
-            double ux = (s[k].x * u[k].x - (Tdo / dV) * E.PU.x) / r2;
-            double uy = (s[k].x * u[k].y - (Tdo / dV) * E.PU.y) / r2;
-            double p2 = (((s[k].y / (gam - 1) + s[k].x * (u[k].x * u[k].x + u[k].y * u[k].y) * 0.5) -
-                          (Tdo / dV) * E.PS.y) -
-                         0.5 * r2 * (ux * ux + uy * uy)) * (gam - 1);
+            double r2, ux, uy, p2;
+            if (g->axisym) {
+                r2 = s[k].x - (Tdo / dV) * E.PS.x - Tdo * s[k].x * u[k].y / y;
+                if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = 0.0001; }
+                ux = (s[k].x * u[k].x - (Tdo / dV) * E.PU.x - Tdo * s[k].x * u[k].y * u[k].x / y) / r2;
+                uy = (s[k].x * u[k].y - (Tdo / dV) * E.PU.y - Tdo * s[k].x * u[k].y * u[k].y / y) / r2;
+                p2 = (((s[k].y / (gam - 1) + s[k].x * (u[k].x * u[k].x + u[k].y * u[k].y) * 0.5) -
+                       (Tdo / dV) * E.PS.y -
+                       Tdo * u[k].y * (gam * s[k].y / (gam - 1) + s[k].x * (u[k].x * u[k].x + u[k].y * u[k].y) * 0.5) /
+                           y) -
+                      0.5 * r2 * (ux * ux + uy * uy)) * (gam - 1);
+            } else {
+                r2 = s[k].x - (Tdo / dV) * E.PS.x;
+                if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = 0.0001; }
+                ux = (s[k].x * u[k].x - (Tdo / dV) * E.PU.x) / r2;
+                uy = (s[k].x * u[k].y - (Tdo / dV) * E.PU.y) / r2;
+                p2 = (((s[k].y / (gam - 1) + s[k].x * (u[k].x * u[k].x + u[k].y * u[k].y) * 0.5) -
+                       (Tdo / dV) * E.PS.y) -
+                      0.5 * r2 * (ux * ux + uy * uy)) * (gam - 1);
+            }
             if (p2 <= 0) { diag_inc(&g_diag.clamp_p); p2 = 0.000001; }
             s2[k].x = r2; s2[k].y = p2; u2[k].x = ux; u2[k].y = uy;
         }

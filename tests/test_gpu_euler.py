@@ -204,3 +204,35 @@ def test_full_size_invariants_4096():
     s2.advance(U, 10)
     assert torch.equal(U, U[:, :1, :].expand_as(U))
     s2.close()
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_kernel_tvd_old_axisymmetric_sphere_matches_oracle(method):
+    """Kernel_TVD_old's 1/y source terms and inner sphere (kernel.cu:3303,
+    3651-3662) with its frozen east side and axis (reflective) south side:
+    one step against the oracle, 1e-12."""
+    say(f"kernel_tvd_old axisymmetric method={method}")
+    bc = (O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    g, og = grids(157, 90, bc, gamma=1.4, krit=0.25, r_frozen=0.23, axisym=True)
+    U = PB.four_quadrant(og, seed=11)
+    s, u, _ = O.cons_to_prim(og, O.PHYS_EULER, U)
+    T0, Tdo = 1e7, 2e-4
+    s2r, u2r, Tr = O.ref_euler_step("kernel_tvd_old", og, s, u, T0, Tdo, method)
+    sd, ud = to_dev(s), to_dev(u)
+    s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+    T = torch.tensor([T0], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([Tdo], dtype=torch.float64, device=dev())
+    api.kernel_tvd_old(g, sd, ud, s2, u2, T, T_do, method)
+    s2g, u2g = to_host(s2), to_host(u2)
+    assert np.max(np.abs(s2g - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(u2g - u2r)) <= 1e-12
+    assert abs(T.item() - Tr) <= 1e-14 * Tr
+    x = og.x0 + np.arange(og.nx) * og.dx
+    y = og.y0 + np.arange(og.ny) * og.dy
+    X, Y = np.meshgrid(x, y)
+    inside = np.sqrt(X * X + Y * Y) < og.r_frozen
+    assert inside.sum() > 20
+    np.testing.assert_array_equal(s2g[inside], s[inside])
+    # Ker_Dekard takes neither option
+    with pytest.raises(api.GD2Error):
+        api.ker_dekard(g, sd, ud, s2, u2, T, T_do, method)

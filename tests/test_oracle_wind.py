@@ -1,6 +1,8 @@
-"""Pins of the oracle's production options of the MHD reference-layout steps
-(add2 / Kernel_TVD): axisymmetric source terms, the four production side
-kinds, the frozen inner sphere (kernel.cu:1918-2190, 2218-2650).
+"""Pins of the oracle's production options of the reference-layout steps:
+MHD add2 / Kernel_TVD (axisymmetric source terms, the four production side
+kinds, the frozen inner sphere; kernel.cu:1918-2190, 2218-2650) and Euler
+Kernel_TVD_old (axisymmetric source terms, inner sphere; kernel.cu:3303,
+3651-3662).
 
 * Sources: on a uniform state every face flux cancels, so one step is a
   forward-Euler step of the geometric and Powell source terms alone. The
@@ -205,3 +207,53 @@ def test_wind_initial_state_matches_closed_form():
     bphi = b[inner, 2] * d[inner] / (Y[inner] / d[inner])
     assert np.allclose(bphi, np.sqrt(4 * np.pi) / PB.WIND_MALF, rtol=1e-12)
     assert np.all(s[~inner, 0] == 1.0) and np.all(b[~inner] == 0.0) and np.all(u[~inner] == 0.0)
+
+
+def test_euler_axisymmetric_sources_on_uniform_state():
+    """Kernel_TVD_old's 1/y terms (kernel.cu:3651-3662): on a uniform state
+    one step is the forward-Euler step of the cylindrical Euler equations'
+    geometric source -G/r, G = (rho v, rho u v, rho v^2, v (E + p))."""
+    r, p, uu, vv = 1.2, 0.8, 0.35, -0.45
+    gam = 1.4
+    nx, ny, dx, dy, y0 = 7, 6, 0.1, 0.1, 0.35
+    g = O.make_grid(nx, ny, dx=dx, dy=dy, y0=y0, gamma=gam, axisym=True)
+    s = np.empty((ny, nx, 2)); u = np.empty((ny, nx, 2))
+    s[...] = (r, p); u[...] = (uu, vv)
+    dt = 2e-3
+    for method in (0, 1, 2):
+        s2, u2, _ = O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, dt, method)
+        E = p / (gam - 1) + 0.5 * r * (uu * uu + vv * vv)
+        for j in range(ny):
+            y = y0 + j * dy
+            G = np.array([r * vv, r * uu * vv, r * vv * vv, vv * (E + p)])
+            W = np.array([r, r * uu, r * vv, E]) - dt * G / y
+            ux, uy = W[1] / W[0], W[2] / W[0]
+            pe = (W[3] - 0.5 * W[0] * (ux * ux + uy * uy)) * (gam - 1)
+            want = np.array([W[0], pe, ux, uy])
+            for i in range(nx):
+                got = np.array([*s2[j, i], *u2[j, i]])
+                assert np.allclose(got, want, rtol=1e-12, atol=1e-14), (method, j, i, got - want)
+
+
+def test_euler_sphere_and_option_scope():
+    nx, ny = 30, 20
+    g = O.make_grid(nx, ny, dx=0.05, dy=0.05, gamma=1.4, r_frozen=0.42, axisym=True)
+    rng = np.random.default_rng(4)
+    s = np.empty((ny, nx, 2)); u = np.empty((ny, nx, 2))
+    s[..., 0] = rng.uniform(0.5, 1.5, (ny, nx)); s[..., 1] = rng.uniform(0.5, 1.5, (ny, nx))
+    u[...] = rng.uniform(-0.5, 0.5, (ny, nx, 2))
+    x = g.x0 + np.arange(nx) * g.dx
+    y = g.y0 + np.arange(ny) * g.dy
+    X, Y = np.meshgrid(x, y)
+    d = np.sqrt(X * X + Y * Y)
+    assert np.min(np.abs(d - 0.42)) > 1e-6
+    inside = d < 0.42
+    s2, u2, _ = O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, 1e-3, 1)
+    g.r_frozen = 0.0
+    s2n, u2n, _ = O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, 1e-3, 1)
+    np.testing.assert_array_equal(s2[inside], s[inside])
+    np.testing.assert_array_equal(s2[~inside], s2n[~inside])
+    np.testing.assert_array_equal(u2[~inside], u2n[~inside])
+    # Ker_Dekard has neither option (kernel.cu:1736-1874)
+    with pytest.raises(ValueError):
+        O.ref_euler_step("ker_dekard", g, s, u, 1e7, 1e-3, 1)
