@@ -446,7 +446,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
 }
 
 template <int ORDER, int FLUX, bool KEEP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? 2 : 1))) void euler_rk2_kernel(const EulerArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_HLLC || FLUX == GD2_FLUX_HLL) ? 2 : 1))) void euler_rk2_kernel(const EulerArgs A) {
     __shared__ Rk2Lds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     euler_rk2_tile<ORDER, FLUX, KEEP>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
