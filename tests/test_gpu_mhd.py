@@ -266,3 +266,43 @@ def test_stellar_wind_full_size_step_matches_oracle():
     X, Y = np.meshgrid(x, y)
     inside = np.sqrt(X * X + Y * Y) < og.r_frozen
     np.testing.assert_array_equal(gs[inside], s0[inside])
+
+
+def test_reference_layout_16384_square_rows_equal_one_row():
+    """64-bit indexing of the AoS reference layout at 16384^2 (the
+    reference's int byte counts overflow above ~1.3e8 cells; b alone is
+    6.4 GB here): an x-directed MHD state, constant along y with transmissive
+    sides, gives every row of a Kernel_TVD (HLLD) step bit for bit equal to
+    the step on 16384 x 4, and the same time step; that row matches the
+    oracle to 1e-12."""
+    n = 16384
+    say("reference layout 16384^2 rows")
+    x = (np.arange(n) + 0.5) / n
+    s_row = np.stack([1.0 + 0.3 * np.sin(6 * np.pi * x) + np.where(x < 0.5, 0.5, 0.0),
+                      0.8 + 0.2 * np.cos(4 * np.pi * x)], axis=-1)
+    u_row = np.stack([0.3 * np.sin(2 * np.pi * x), np.full(n, 0.1)], axis=-1)
+    b_row = np.stack([np.full(n, 0.7), 0.4 * np.cos(2 * np.pi * x), 0.3 * np.sin(4 * np.pi * x)], axis=-1)
+    res = {}
+    for ny in (4, n):
+        g = api.make_grid(n, ny, dx=1.0 / n, dy=1.0 / n, gamma=G53, krit=0.25)
+        sd = to_dev(s_row[None]).repeat(ny, 1, 1).contiguous()
+        ud = to_dev(u_row[None]).repeat(ny, 1, 1).contiguous()
+        bd = to_dev(b_row[None]).repeat(ny, 1, 1).contiguous()
+        s2, u2, b2 = torch.empty_like(sd), torch.empty_like(ud), torch.empty_like(bd)
+        T = torch.tensor([1e7], dtype=torch.float64, device=dev())
+        T_do = torch.tensor([2e-5], dtype=torch.float64, device=dev())
+        api.kernel_tvd(g, sd, ud, bd, s2, u2, b2, T, T_do, 0, 3)
+        torch.cuda.synchronize()
+        del sd, ud, bd
+        res[ny] = (s2, u2, b2, T.item())
+    s4, u4, b4, T4 = res[4]
+    sn, un, bn, Tn = res[n]
+    assert T4 == Tn
+    for big, small in ((sn, s4), (un, u4), (bn, b4)):
+        assert torch.equal(big, small[:1].expand_as(big))
+    og = O.make_grid(n, 4, dx=1.0 / n, dy=1.0 / n, gamma=G53, krit=0.25)
+    rep = lambda a: np.ascontiguousarray(np.repeat(a[None], 4, axis=0))
+    s2r, u2r, b2r, Tr = O.ref_mhd_step("kernel_tvd", og, rep(s_row), rep(u_row), rep(b_row), 1e7, 2e-5, 0, 3)
+    assert np.max(np.abs(to_host(s4) - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(to_host(b4) - b2r)) <= 1e-12
+    assert abs(T4 - Tr) <= 1e-14 * Tr

@@ -140,3 +140,45 @@ def test_solver_context_one_pass_equals_two_pass(nsteps):
     assert ng == nsteps
     assert torch.equal(U, Ur)
     assert tg == t
+
+
+@pytest.mark.parametrize("path", ["one_pass", "two_pass"])
+def test_16384_square_rows_equal_one_row_solution(path):
+    """64-bit indexing at 16384^2 (8.6 GB per state, plane offsets beyond
+    2^32 bytes): an x-directed state, constant along y, stays so under
+    SSP-RK2 with transmissive sides, and every row of the 16384^2 run equals
+    bit for bit the row of the same problem on 16384 x 4 (whose rows are
+    checked against the oracle)."""
+    n = 16384
+    say(f"16384^2 rows, {path}")
+    x = (np.arange(n) + 0.5) / n
+    rho = 1.0 + 0.3 * np.sin(6 * np.pi * x) + np.where(x < 0.5, 0.5, 0.0)
+    p = 1.0 + 0.2 * np.cos(4 * np.pi * x) + np.where(x < 0.3, 0.4, 0.0)
+    u = 0.3 * np.sin(2 * np.pi * x)
+    v = np.full(n, 0.2)
+    row = PB.euler_from_prim(1.4, rho, p, u, v)           # (4, n)
+    sc = api.make_scheme(api.EULER, api.FLUX_HLLC, 2, api.SSPRK2, 0.4)
+    prev = api.set_step_path(api.STEP_ONE_PASS if path == "one_pass" else api.STEP_TWO_PASS)
+    try:
+        out = {}
+        for ny in (4, n):
+            g = api.make_grid(n, ny, dx=1.0 / n, dy=1.0 / n)
+            U = to_dev(row[:, None, :]).repeat(1, ny, 1).contiguous()
+            s = api.Solver(g, sc)
+            s.start(U)
+            s.advance(U, 3)
+            out[ny] = (U, s.time())
+            s.close()
+        small, (t4, n4, _) = out[4]
+        big, (tb, nb, _) = out[n]
+        assert n4 == nb == 3 and t4 == tb
+        for f in range(4):
+            assert torch.equal(big[f], small[f, :1, :].expand(n, n)), f
+        # the one-row problem against the oracle
+        og = O.make_grid(n, 4, dx=1.0 / n, dy=1.0 / n)
+        Uo = np.ascontiguousarray(np.repeat(row[:, None, :], 4, axis=1))
+        steps, _ = O.run(og, O.Scheme(O.PHYS_EULER, O.FLUX_HLLC, 2, 0.4), 2, Uo, nsteps=3)
+        assert steps == 3
+        assert np.all(rel_l1(to_host(small), Uo) <= 1e-12)
+    finally:
+        api.set_step_path(prev)
