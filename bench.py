@@ -490,8 +490,8 @@ def copy_reference(mib):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on one "
-              "MI355X; the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (one MI355X per run; "
+              "a grid would split into row strips with a halo exchange per step)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
