@@ -94,7 +94,7 @@ int tile_rows_for(int nx, int ny) {
 // rounds of (CUs x resident waves per CU); a round takes about as long as
 // one strip (its rows plus the rows re-read around it, `overhead`), so pick
 // the height minimising rounds x (rows + overhead). GD2_TILE_ROWS overrides.
-int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows) {
+int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows, int min_rows) {
     static int env = -1;
     if (env < 0) {
         const char* e = std::getenv("GD2_TILE_ROWS");
@@ -126,7 +126,7 @@ int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_ro
     const long long slots = (long long)cus * per_cu;
     int best = 16;
     long long best_cost = -1;
-    for (int rows = 4; rows <= max_rows; ++rows) {
+    for (int rows = min_rows; rows <= max_rows; ++rows) {
         const long long waves = (long long)tiles_x * ((ny + rows - 1) / rows);
         const long long cost = ((waves + slots - 1) / slots) * (rows + overhead);
         if (best_cost < 0 || cost <= best_cost) {  // ties: the taller strip (less halo)
@@ -164,7 +164,9 @@ template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE, bool PROD = false>
 static int launch_euler(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
     const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>);
-    A.rows = strip_rows(k, A.tiles_x, A.ny, 2);
+    // strips down to one row: on grids too small to fill the chip each wave's
+    // latency-bound march is the step's critical path
+    A.rows = strip_rows(k, A.tiles_x, A.ny, 2, 64, 1);
     hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>), dim3(euler_blocks(A)), dim3(kWave), 0,
                        st, A);
     GD2_HIP(hipGetLastError());

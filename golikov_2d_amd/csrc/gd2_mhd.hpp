@@ -728,7 +728,7 @@ __global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, lon
 // ------------------------------------------------------------------ launchers
 
 int tile_rows_for(int nx, int ny);
-int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows = 64);
+int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows = 64, int min_rows = 4);
 int hip_error_code(hipError_t e);
 
 inline void fill_mhd(MhdArgs& A, const gd2_grid* g, DeviceDiag* diag) {
@@ -754,7 +754,7 @@ template <int ORDER, int METOD, int LAYOUT, bool W, bool PROD = false>
 inline int launch_mhd(const MhdArgs& A0, hipStream_t st) {
     MhdArgs A = A0;
     A.rows = strip_rows(reinterpret_cast<const void*>(&mhd_tile_kernel<ORDER, METOD, LAYOUT, W, PROD>), A.tiles_x,
-                        A.ny, 2);
+                        A.ny, 2, 64, 1);
     const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
     hipLaunchKernelGGL((mhd_tile_kernel<ORDER, METOD, LAYOUT, W, PROD>), dim3(blocks), dim3(kWave), 0, st, A);
     return hip_error_code(hipGetLastError());
