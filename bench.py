@@ -44,7 +44,8 @@ def parse():
                     help="skip the batched-ensemble, adjoint, MHD and Sod legs")
     ap.add_argument("--only-extras", action="store_true",
                     help="run only the batched-ensemble, adjoint, MHD and Sod legs (for profiling them)")
-    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod", "reflayout", "io"], default=None,
+    ap.add_argument("--only-leg", choices=["batched", "adjoint", "mhd", "sod", "reflayout", "io", "mhd_adjoint"],
+                    default=None,
                     help="run only one of those legs (for profiling it)")
     return ap.parse_args()
 
@@ -475,6 +476,49 @@ def adjoint_vs_forward(n=1024, steps=200, every=20, seed=0, reps=3):
     }
 
 
+def mhd_adjoint_vs_forward(n=1024, steps=100, every=20, seed=0, reps=3):
+    """The MHD (HLLD) differentiable rollout: backward against forward, as for
+    the Euler adjoint, on the Orszag-Tang state."""
+    import torch
+    from golikov_2d_amd import adjoint as AD
+    from golikov_2d_amd import api
+    from golikov_2d_amd import problems as PB
+
+    dev = torch.device("cuda:0")
+    g = api.make_grid(n, n, gamma=5.0 / 3.0, bc=(api.BC_PERIODIC,) * 4)
+    sc = api.make_scheme(api.MHD, api.FLUX_HLLD, 2, api.SSPRK2, 0.3)
+    U0 = torch.from_numpy(PB.orszag_tang(g, seed=seed, amp=1e-3)).to(dev)
+    Ut = torch.from_numpy(PB.orszag_tang(g, seed=seed + 1, amp=1e-2)).to(dev)
+    dts, _ = AD.dt_schedule(g, sc, U0, steps)
+    with torch.no_grad():
+        rho_t = AD.rollout(g, sc, Ut, dts, every=steps + 1)[0].clone()
+    fwd, bwd = [], []
+    for _ in range(reps + 1):
+        X = U0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        UT = AD.rollout(g, sc, X, dts, every=every)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        AD.density_loss(UT, rho_t).backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f, b = sorted(fwd[1:])[reps // 2], sorted(bwd[1:])[reps // 2]
+    return {
+        "value": round(b / f, 3),
+        "unit": "x (backward wall time / forward wall time)",
+        "forward_ms": round(1e3 * f, 3),
+        "backward_ms": round(1e3 * b, 3),
+        "n": n,
+        "steps": steps,
+        "checkpoint_every": every,
+        "workload": f"{n}x{n} Orszag-Tang (seed {seed}, 1e-3 perturbation), HLLD + minmod, Powell term, SSP-RK2, "
+                    f"{steps} recorded CFL steps, loss 1/2 ||rho(T) - rho_target||^2",
+    }
+
+
 def copy_reference(mib):
     """One plain device-to-device copy of `mib` MiB: the reference dispatch
     that scales the HBM byte counters."""
@@ -506,7 +550,8 @@ def main():
                 "mhd": lambda: mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                 "sod": lambda: sod_godunov(with_cpu=not args.no_cpu),
                 "reflayout": lambda: reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
-                "io": lambda: io_formats(with_cpu=not args.no_cpu)}
+                "io": lambda: io_formats(with_cpu=not args.no_cpu),
+                "mhd_adjoint": lambda: mhd_adjoint_vs_forward(seed=args.seed)}
         print(json.dumps({args.only_leg: legs[args.only_leg]()}), flush=True)
         return
     if args.only_extras:
@@ -515,7 +560,8 @@ def main():
                           "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                           "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
                   "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
-                  "io_formats": io_formats(with_cpu=not args.no_cpu)}), flush=True)
+                  "io_formats": io_formats(with_cpu=not args.no_cpu),
+                  "mhd_adjoint_vs_forward": mhd_adjoint_vs_forward(seed=args.seed)}), flush=True)
         return
 
     n = args.n
@@ -606,7 +652,8 @@ def main():
                   "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
                   "sod_godunov_200": sod_godunov(with_cpu=not args.no_cpu),
                   "reference_layout": reference_layout(seed=args.seed, with_cpu=not args.no_cpu),
-                  "io_formats": io_formats(with_cpu=not args.no_cpu)}
+                  "io_formats": io_formats(with_cpu=not args.no_cpu),
+                  "mhd_adjoint_vs_forward": mhd_adjoint_vs_forward(seed=args.seed)}
 
     out = {
         "metric": METRIC,
