@@ -7,11 +7,10 @@
 //
 //  1. one thread per face (x-faces, then y-faces): the face cotangent
 //     phi = (h/dV)(V_right - V_left) (zero on the side of a ghost cell) is
-//     taken through the face flux to its two reconstructed states. The flux
-//     Jacobian is formed by forward-mode tangents, one per reconstructed
-//     input (7 per side), through a restatement of hlld_face over tangent
-//     numbers that follows the same branches; each product phi . dF/dx_k goes
-//     to plane k of a face buffer (planes 0..6: left state, 7..13: right).
+//     taken through the face flux to its two reconstructed states by one
+//     reverse sweep of hlld_face (hand-reversed, the same branches as the
+//     forward); each product phi . dF/dx_k goes to plane k of a face buffer
+//     (planes 0..6: left state, 7..13: right).
 //     Reflective west / south wall faces are evaluated mirrored, as the
 //     forward stage does.
 //  2. one thread per cell: gathers the face-state cotangents through the
@@ -30,223 +29,654 @@
 
 namespace gd2 {
 
-// Forward-mode tangent number: value and derivative along one direction.
-struct Tg {
-    double v, d;
-};
-
-__device__ __forceinline__ Tg operator+(Tg a, Tg b) { return {a.v + b.v, a.d + b.d}; }
-__device__ __forceinline__ Tg operator-(Tg a, Tg b) { return {a.v - b.v, a.d - b.d}; }
-__device__ __forceinline__ Tg operator-(Tg a) { return {-a.v, -a.d}; }
-__device__ __forceinline__ Tg operator*(Tg a, Tg b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-__device__ __forceinline__ Tg operator*(double s, Tg a) { return {s * a.v, s * a.d}; }
-__device__ __forceinline__ Tg operator*(Tg a, double s) { return {a.v * s, a.d * s}; }
-__device__ __forceinline__ Tg operator+(Tg a, double s) { return {a.v + s, a.d}; }
-__device__ __forceinline__ Tg operator-(Tg a, double s) { return {a.v - s, a.d}; }
-__device__ __forceinline__ Tg trcp(Tg a) {
-    const double r = 1.0 / a.v;
-    return {r, -a.d * r * r};
-}
-__device__ __forceinline__ Tg tsqrt(Tg a) {
-    const double s = sqrt(a.v);
-    return {s, s > 0.0 ? 0.5 * a.d / s : 0.0};
-}
-__device__ __forceinline__ Tg tabs(Tg a) { return a.v < 0.0 ? -a : a; }
-__device__ __forceinline__ Tg tmin(Tg a, Tg b) { return a.v <= b.v ? a : b; }
-__device__ __forceinline__ Tg tmax(Tg a, Tg b) { return a.v >= b.v ? a : b; }
-
-// hlld_face over tangent numbers: same formulas and branch order, IEEE
-// division and square root (the branch tests see values within rounding of
-// the forward's).
+// Reverse-mode hlld_face: the same forward values as hlld_face_tg (IEEE
+// division and square root, every product rounded on its own), then one
+// reverse sweep taking the cotangent Pb of the face-frame flux to the
+// cotangents ib of the 16 face-frame inputs.
 template <int METOD>
-__device__ __forceinline__ void hlld_face_tg(const Tg (&in)[16], double gamma, double inv_g1, Tg (&P)[8]) {
-    const Tg ro_L = in[0], p_L = in[1], u1 = in[2], v1 = in[3], w1 = in[4], bn1 = in[5], bt1 = in[6],
-             bm1 = in[7];
-    const Tg ro_R = in[8], p_R = in[9], u2 = in[10], v2 = in[11], w2 = in[12], bn2 = in[13], bt2 = in[14],
-             bm2 = in[15];
-    const Tg bb_L = bn1 * bn1 + bt1 * bt1 + bm1 * bm1;
-    const Tg bb_R = bn2 * bn2 + bt2 * bt2 + bm2 * bm2;
-    const Tg uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
-    const Tg uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
-    const Tg gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R;
-    const Tg iroL = trcp(ro_L), iroR = trcp(ro_R);
-    const Tg cfL = tsqrt((gpL + tsqrt(gpL * gpL - (4.0 * gamma) * p_L * (bn1 * bn1))) * (0.5 * iroL));
-    const Tg cfR = tsqrt((gpR + tsqrt(gpR * gpR - (4.0 * gamma) * p_R * (bn2 * bn2))) * (0.5 * iroR));
-    const Tg cmax = tmax(cfL, cfR);
-    const Tg SL = tmin(u1, u2) - cmax;
-    const Tg SR = tmax(u1, u2) + cmax;
-    const Tg smax = tmax(tabs(SL), tabs(SR));
-    const Tg pTL = p_L + 0.5 * bb_L, pTR = p_R + 0.5 * bb_R;
-    const Tg suR = SR - u2, suL = SL - u1;
-    const Tg mR = suR * ro_R, mL = suL * ro_L;
-    const Tg idm = trcp(mR - mL);
-    const Tg SM = (mR * u2 - mL * u1 - pTR + pTL) * idm;
-    const Tg e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
-    const Tg e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
-
-    Tg FL[8], FR[8], UL[8], UR[8];
-    const Tg zero = {0.0, 0.0};
+__device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gamma, double inv_g1,
+                                              const double (&Pb)[8], double (&ib)[16]) {
+#pragma clang fp contract(off)
+    const double ro_L = in[0], p_L = in[1], u1 = in[2], v1 = in[3], w1 = in[4], bn1 = in[5], bt1 = in[6],
+                 bm1 = in[7];
+    const double ro_R = in[8], p_R = in[9], u2 = in[10], v2 = in[11], w2 = in[12], bn2 = in[13], bt2 = in[14],
+                 bm2 = in[15];
+    // ---------------------------------------------------------------- forward
+    const double bb_L = bn1 * bn1 + bt1 * bt1 + bm1 * bm1;
+    const double bb_R = bn2 * bn2 + bt2 * bt2 + bm2 * bm2;
+    const double uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
+    const double uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
+    const double gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R;
+    const double iroL = 1.0 / ro_L, iroR = 1.0 / ro_R;
+    const double g4 = 4.0 * gamma;
+    const double dL = gpL * gpL - g4 * p_L * (bn1 * bn1), dR = gpR * gpR - g4 * p_R * (bn2 * bn2);
+    const double sdL = sqrt(dL), sdR = sqrt(dR);
+    const double aL = (gpL + sdL) * (0.5 * iroL), aR = (gpR + sdR) * (0.5 * iroR);
+    const double cfL = sqrt(aL), cfR = sqrt(aR);
+    const bool cL = cfL >= cfR;  // tmax picks its first argument on ties
+    const double cmax = cL ? cfL : cfR;
+    const bool uminL = u1 <= u2, umaxL = u1 >= u2;
+    const double SL = (uminL ? u1 : u2) - cmax;
+    const double SR = (umaxL ? u1 : u2) + cmax;
+    const double aSL = SL < 0.0 ? -SL : SL, aSR = SR < 0.0 ? -SR : SR;
+    const bool smL = aSL >= aSR;
+    const double smax = smL ? aSL : aSR;
+    const double pTL = p_L + 0.5 * bb_L, pTR = p_R + 0.5 * bb_R;
+    const double suR = SR - u2, suL = SL - u1;
+    const double mR = suR * ro_R, mL = suL * ro_L;
+    const double Dm = mR - mL;
+    const double idm = 1.0 / Dm;
+    const double smR = mR * u2, smLv = mL * u1;
+    const double Nn = smR - smLv - pTR + pTL;
+    const double SM = Nn * idm;
+    const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
+    const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
+    double FL[8], FR[8], UL[8], UR[8];
     FL[0] = ro_L * u1;
     FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1;
     FL[2] = ro_L * u1 * v1 - bn1 * bt1;
     FL[3] = ro_L * u1 * w1 - bn1 * bm1;
-    FL[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
-    FL[5] = zero;
+    const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
+    FL[4] = (e1 + pTL) * u1 - bn1 * sbv1;
+    FL[5] = 0.0;
     FL[6] = u1 * bt1 - v1 * bn1;
     FL[7] = u1 * bm1 - w1 * bn1;
     FR[0] = ro_R * u2;
     FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2;
     FR[2] = ro_R * u2 * v2 - bn2 * bt2;
     FR[3] = ro_R * u2 * w2 - bn2 * bm2;
-    FR[4] = (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2);
-    FR[5] = zero;
+    const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
+    FR[4] = (e2 + pTR) * u2 - bn2 * sbv2;
+    FR[5] = 0.0;
     FR[6] = u2 * bt2 - v2 * bn2;
     FR[7] = u2 * bm2 - w2 * bn2;
     UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1;
     UL[4] = e1; UL[5] = bn1; UL[6] = bt1; UL[7] = bm1;
     UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2;
     UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2;
+    const double DS = SR - SL;
+    const double iS = 1.0 / DS;
+    const double nbn = SR * bn2 - SL * bn1;
+    const double nbt = SR * bt2 - SL * bt1 + FL[6] - FR[6];
+    const double nbm = SR * bm2 - SL * bm1 + FL[7] - FR[7];
+    const double bn = nbn * iS, bt = nbt * iS, bm = nbm * iS;
+    const double bbn = bn * bn;
+    const double DLM = SL - SM, DRM = SR - SM;
+    const double iLM = 1.0 / DLM, iRM = 1.0 / DRM;
+    const double ro_LL = ro_L * suL * iLM;
+    const double ro_RR = ro_R * suR * iRM;
 
-    const Tg iS = trcp(SR - SL);
-    const Tg bn = (SR * bn2 - SL * bn1) * iS;
-    const Tg bt = (SR * bt2 - SL * bt1 + FL[6] - FR[6]) * iS;
-    const Tg bm = (SR * bm2 - SL * bm1 + FL[7] - FR[7]) * iS;
-    const Tg bbn = bn * bn;
-    const Tg iLM = trcp(SL - SM), iRM = trcp(SR - SM);
-    const Tg ro_LL = ro_L * suL * iLM;
-    const Tg ro_RR = ro_R * suR * iRM;
+    // adjoints of every forward quantity
+    double b_ro_L = 0.0, b_p_L = 0.0, b_u1 = 0.0, b_v1 = 0.0, b_w1 = 0.0, b_bn1 = 0.0, b_bt1 = 0.0, b_bm1 = 0.0;
+    double b_ro_R = 0.0, b_p_R = 0.0, b_u2 = 0.0, b_v2 = 0.0, b_w2 = 0.0, b_bn2 = 0.0, b_bt2 = 0.0, b_bm2 = 0.0;
+    double bFL[8], bFR[8], bUL[8], bUR[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bFL[i] = bFR[i] = bUL[i] = bUR[i] = 0.0;
+    double b_SL = 0.0, b_SR = 0.0, b_SM = 0.0, b_bn = 0.0, b_bt = 0.0, b_bm = 0.0, b_ro_LL = 0.0, b_ro_RR = 0.0;
+    double b_iS = 0.0, b_iLM = 0.0, b_iRM = 0.0, b_e1 = 0.0, b_e2 = 0.0, b_pTL = 0.0, b_pTR = 0.0;
+    double b_suL = 0.0, b_suR = 0.0, b_mL = 0.0, b_mR = 0.0, b_idm = 0.0;
+
+    // P[5] = -(smax * (bn2 - bn1)) replaces the region formula's P[5]
+    {
+        const double g = Pb[5];
+        const double b_smax = -g * (bn2 - bn1);
+        b_bn2 -= g * smax;
+        b_bn1 += g * smax;
+        const double b_a = smL ? b_smax : 0.0, b_b = smL ? 0.0 : b_smax;
+        b_SL += SL < 0.0 ? -b_a : b_a;
+        b_SR += SR < 0.0 ? -b_b : b_b;
+    }
 
     if constexpr (METOD == 2) {
-        const Tg sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
-        const Tg sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
-        const Tg UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) * iS;
-        const Tg UZ1 = (SR * UR[1] - SL * UL[1] + FL[1] - FR[1]) * iS;
-        const Tg UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) * iS;
-        const Tg UZ3 = (SR * UR[3] - SL * UL[3] + FL[3] - FR[3]) * iS;
-        const Tg suRm = suR * iRM, suLm = suL * iLM;
-        const Tg ptz = 0.5 * ((pTR + ro_R * suR * (SM - u2)) + (pTL + ro_L * suL * (SM - u1)));
-        const Tg iUZ0 = trcp(UZ0);
-        const Tg vv = UZ2 * iUZ0, ww = UZ3 * iUZ0;
-        const Tg sbvz = (bn * UZ1 + bt * UZ2 + bm * UZ3) * iUZ0;
-        const Tg ee2 = e2 * suRm + (ptz * SM - pTR * u2 + bn * (sbv2 - sbvz)) * iRM;
-        const Tg ee1 = e1 * suLm + (ptz * SM - pTL * u1 + bn * (sbv1 - sbvz)) * iLM;
-        const bool left = (SL.v >= 0.0) || (SM.v >= 0.0);
-        const Tg S = (SL.v >= 0.0) ? zero : (left ? SL : (SR.v > 0.0 ? SR : zero));
-        const Tg rs = left ? ro_LL : ro_RR;
-        const Tg es = left ? ee1 : ee2;
-        const Tg Us[8] = {rs, rs * SM, rs * vv, rs * ww, es, bn, bt, bm};
+        double nUZ[4];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) P[i] = (left ? FL[i] : FR[i]) + S * Us[i] - S * (left ? UL[i] : UR[i]);
-    } else {
-        const Tg PTT = (mR * pTL - mL * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) * idm;
-        const Tg ttL = ro_L * suL * (SL - SM) - bbn;
-        const Tg ttR = ro_R * suR * (SR - SM) - bbn;
-        Tg vLL, wLL, vRR, wRR, btt1, bmm1, btt2, bmm2;
-        if (fabs(ttL.v) >= 0.00001) {
-            const Tg it = trcp(ttL);
-            vLL = v1 - bn * bt1 * (SM - u1) * it;
-            wLL = w1 - bn * bm1 * (SM - u1) * it;
-            const Tg q = (ro_L * suL * suL - bbn) * it;
-            btt1 = bt1 * q;
-            bmm1 = bm1 * q;
-        } else {
-            vLL = v1; wLL = w1; btt1 = zero; bmm1 = zero;
-        }
-        if (fabs(ttR.v) >= 0.00001) {
-            const Tg it = trcp(ttR);
-            vRR = v2 - bn * bt2 * (SM - u2) * it;
-            wRR = w2 - bn * bm2 * (SM - u2) * it;
-            const Tg q = (ro_R * suR * suR - bbn) * it;
-            btt2 = bt2 * q;
-            bmm2 = bm2 * q;
-        } else {
-            vRR = v2; wRR = w2; btt2 = zero; bmm2 = zero;
-        }
-        const Tg eLL = (e1 * suL + PTT * SM - pTL * u1 +
-                        bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) * iLM;
-        const Tg eRR = (e2 * suR + PTT * SM - pTR * u2 +
-                        bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) * iRM;
-        const Tg sLL = tsqrt(ro_LL), sRR = tsqrt(ro_RR);
-        Tg SLL = SM - tabs(bn) * trcp(sLL);
-        Tg SRR = SM + tabs(bn) * trcp(sRR);
-        double idbn;
-        if (fabs(bn.v) > 0.000001) {
-            idbn = bn.v > 0.0 ? 1.0 : -1.0;
-        } else {
-            idbn = 0.0;
-            SLL = SM;
-            SRR = SM;
-        }
-        const Tg isum = trcp(sLL + sRR);
-        const Tg vLLL = (sLL * vLL + sRR * vRR + idbn * (btt2 - btt1)) * isum;
-        const Tg wLLL = (sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1)) * isum;
-        const Tg bttt = (sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL)) * isum;
-        const Tg bmmm = (sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL)) * isum;
-        const Tg vb3 = SM * bn + vLLL * bttt + wLLL * bmmm;
-        const Tg eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - vb3);
-        const Tg eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - vb3);
-        const bool left = SM.v >= 0.0;
-        const Tg S1 = left ? SL : SR;
-        const Tg S2 = left ? SLL : SRR;
-        const Tg rs = left ? ro_LL : ro_RR;
-        const Tg vs = left ? vLL : vRR, ws = left ? wLL : wRR;
-        const Tg bts = left ? btt1 : btt2, bms = left ? bmm1 : bmm2;
-        const Tg es = left ? eLL : eRR;
-        const Tg ess = left ? eLLL : eRRR;
-        const bool outer = left ? (SL.v >= 0.0) : !(SRR.v > 0.0 || SR.v > 0.0);
-        const bool single = left ? (SLL.v >= 0.0) : !(SRR.v > 0.0);
-        const Tg Us[8] = {rs, rs * SM, rs * vs, rs * ws, es, bn, bts, bms};
-        const Tg Uss[8] = {rs, rs * SM, rs * vLLL, rs * wLLL, ess, bn, bttt, bmmm};
+        for (int k = 0; k < 4; ++k) nUZ[k] = SR * UR[k] - SL * UL[k] + FL[k] - FR[k];
+        const double UZ0 = nUZ[0] * iS, UZ1 = nUZ[1] * iS, UZ2 = nUZ[2] * iS, UZ3 = nUZ[3] * iS;
+        const double suRm = suR * iRM, suLm = suL * iLM;
+        const double ptR = pTR + ro_R * suR * (SM - u2), ptL = pTL + ro_L * suL * (SM - u1);
+        const double ptz = 0.5 * (ptR + ptL);
+        const double iUZ0 = 1.0 / UZ0;
+        const double vv = UZ2 * iUZ0, ww = UZ3 * iUZ0;
+        const double nz = bn * UZ1 + bt * UZ2 + bm * UZ3;
+        const double sbvz = nz * iUZ0;
+        const double tR = ptz * SM - pTR * u2 + bn * (sbv2 - sbvz);
+        const double tL = ptz * SM - pTL * u1 + bn * (sbv1 - sbvz);
+        const bool left = (SL >= 0.0) || (SM >= 0.0);
+        const int sS = (SL >= 0.0) ? 0 : (left ? 1 : (SR > 0.0 ? 2 : 0));  // S = 0, SL, SR
+        const double S = sS == 1 ? SL : (sS == 2 ? SR : 0.0);
+        const double rs = left ? ro_LL : ro_RR;
+        const double es = left ? e1 * suLm + tL * iLM : e2 * suRm + tR * iRM;
+        const double Us[8] = {rs, rs * SM, rs * vv, rs * ww, es, bn, bt, bm};
+        // P[i] = F[i] + S*Us[i] - S*U[i] (i != 5)
+        double b_S = 0.0, bUs[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const Tg Fi = left ? FL[i] : FR[i];
-            const Tg Ui = left ? UL[i] : UR[i];
-            if (outer)
-                P[i] = Fi;
-            else if (single)
-                P[i] = Fi + S1 * Us[i] - S1 * Ui;
-            else
-                P[i] = Fi + S2 * Uss[i] - (S2 - S1) * Us[i] - S1 * Ui;
+            const double g = i == 5 ? 0.0 : Pb[i];
+            const double Ui = left ? UL[i] : UR[i];
+            if (left) { bFL[i] += g; bUL[i] -= g * S; } else { bFR[i] += g; bUR[i] -= g * S; }
+            b_S += g * (Us[i] - Ui);
+            bUs[i] = g * S;
+        }
+        if (sS == 1) b_SL += b_S; else if (sS == 2) b_SR += b_S;
+        double b_rs = bUs[0] + bUs[1] * SM + bUs[2] * vv + bUs[3] * ww;
+        b_SM += bUs[1] * rs;
+        const double b_vv = bUs[2] * rs, b_ww = bUs[3] * rs;
+        const double b_es = bUs[4];
+        b_bn += bUs[5];
+        b_bt += bUs[6];
+        b_bm += bUs[7];
+        if (left) b_ro_LL += b_rs; else b_ro_RR += b_rs;
+        double b_ptz = 0.0, b_sbvz = 0.0;
+        if (left) {
+            // es = e1*suLm + tL*iLM
+            b_e1 += b_es * suLm;
+            const double b_suLm = b_es * e1;
+            const double b_tL = b_es * iLM;
+            b_iLM += b_es * tL;
+            b_suL += b_suLm * iLM;
+            b_iLM += b_suLm * suL;
+            // tL = ptz*SM - pTL*u1 + bn*(sbv1 - sbvz)
+            b_ptz += b_tL * SM;
+            b_SM += b_tL * ptz;
+            b_pTL -= b_tL * u1;
+            b_u1 -= b_tL * pTL;
+            b_bn += b_tL * (sbv1 - sbvz);
+            const double b_sb = b_tL * bn;
+            b_sbvz -= b_sb;
+            b_u1 += b_sb * bn1; b_bn1 += b_sb * u1;
+            b_v1 += b_sb * bt1; b_bt1 += b_sb * v1;
+            b_w1 += b_sb * bm1; b_bm1 += b_sb * w1;
+        } else {
+            b_e2 += b_es * suRm;
+            const double b_suRm = b_es * e2;
+            const double b_tR = b_es * iRM;
+            b_iRM += b_es * tR;
+            b_suR += b_suRm * iRM;
+            b_iRM += b_suRm * suR;
+            b_ptz += b_tR * SM;
+            b_SM += b_tR * ptz;
+            b_pTR -= b_tR * u2;
+            b_u2 -= b_tR * pTR;
+            b_bn += b_tR * (sbv2 - sbvz);
+            const double b_sb = b_tR * bn;
+            b_sbvz -= b_sb;
+            b_u2 += b_sb * bn2; b_bn2 += b_sb * u2;
+            b_v2 += b_sb * bt2; b_bt2 += b_sb * v2;
+            b_w2 += b_sb * bm2; b_bm2 += b_sb * w2;
+        }
+        // sbvz = nz * iUZ0; vv = UZ2*iUZ0; ww = UZ3*iUZ0
+        const double b_nz = b_sbvz * iUZ0;
+        double b_iUZ0 = b_sbvz * nz + b_vv * UZ2 + b_ww * UZ3;
+        double b_UZ2 = b_vv * iUZ0, b_UZ3 = b_ww * iUZ0;
+        b_bn += b_nz * UZ1; b_bt += b_nz * UZ2; b_bm += b_nz * UZ3;
+        const double b_UZ1 = b_nz * bn;
+        b_UZ2 += b_nz * bt;
+        b_UZ3 += b_nz * bm;
+        const double b_UZ0 = -b_iUZ0 * iUZ0 * iUZ0;
+        // ptz = 0.5*(ptR + ptL)
+        const double b_ptR = 0.5 * b_ptz, b_ptL = 0.5 * b_ptz;
+        b_pTR += b_ptR;
+        {   // ro_R*suR*(SM-u2): ((ro_R*suR)*(SM-u2))
+            const double a = ro_R * suR, d = SM - u2;
+            const double b_a = b_ptR * d, b_d = b_ptR * a;
+            b_ro_R += b_a * suR; b_suR += b_a * ro_R;
+            b_SM += b_d; b_u2 -= b_d;
+        }
+        b_pTL += b_ptL;
+        {
+            const double a = ro_L * suL, d = SM - u1;
+            const double b_a = b_ptL * d, b_d = b_ptL * a;
+            b_ro_L += b_a * suL; b_suL += b_a * ro_L;
+            b_SM += b_d; b_u1 -= b_d;
+        }
+        // UZk = (SR*UR[k] - SL*UL[k] + FL[k] - FR[k]) * iS
+        const double bUZ[4] = {b_UZ0, b_UZ1, b_UZ2, b_UZ3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double g = bUZ[k] * iS;
+            b_iS += bUZ[k] * nUZ[k];
+            b_SR += g * UR[k]; bUR[k] += g * SR;
+            b_SL -= g * UL[k]; bUL[k] -= g * SL;
+            bFL[k] += g; bFR[k] -= g;
+        }
+    } else {
+        // ---------------------------------------------------------- HLLD forward
+        const double a1 = ro_L * ro_R, a2 = a1 * suR, a3 = a2 * suL, du = u2 - u1;
+        const double nP = mR * pTL - mL * pTR + a3 * du;
+        const double PTT = nP * idm;
+        const double kL = ro_L * suL, kR = ro_R * suR;
+        const double ttL = kL * DLM - bbn, ttR = kR * DRM - bbn;
+        const bool gL = fabs(ttL) >= 0.00001, gR = fabs(ttR) >= 0.00001;
+        const double itL = gL ? 1.0 / ttL : 0.0, itR = gR ? 1.0 / ttR : 0.0;
+        const double dSL = SM - u1, dSR = SM - u2;
+        // vLL = v1 - ((bn*bt1)*dSL)*itL
+        const double hvL = bn * bt1, hwL = bn * bm1, hvR = bn * bt2, hwR = bn * bm2;
+        const double qnL = kL * suL - bbn, qnR = kR * suR - bbn;  // (ro*su)*su - bbn
+        const double qL = qnL * itL, qR = qnR * itR;
+        const double vLL = gL ? v1 - hvL * dSL * itL : v1, wLL = gL ? w1 - hwL * dSL * itL : w1;
+        const double vRR = gR ? v2 - hvR * dSR * itR : v2, wRR = gR ? w2 - hwR * dSR * itR : w2;
+        const double btt1 = gL ? bt1 * qL : 0.0, bmm1 = gL ? bm1 * qL : 0.0;
+        const double btt2 = gR ? bt2 * qR : 0.0, bmm2 = gR ? bm2 * qR : 0.0;
+        const double vbL = SM * bn + vLL * btt1 + wLL * bmm1;
+        const double vbR = SM * bn + vRR * btt2 + wRR * bmm2;
+        const double nEL = e1 * suL + PTT * SM - pTL * u1 + bn * (sbv1 - vbL);
+        const double nER = e2 * suR + PTT * SM - pTR * u2 + bn * (sbv2 - vbR);
+        const double eLL = nEL * iLM, eRR = nER * iRM;
+        const double sLL = sqrt(ro_LL), sRR = sqrt(ro_RR);
+        const double isLL = 1.0 / sLL, isRR = 1.0 / sRR;
+        const double abn = bn < 0.0 ? -bn : bn;
+        const bool big = fabs(bn) > 0.000001;
+        const double idbn = big ? (bn > 0.0 ? 1.0 : -1.0) : 0.0;
+        const double SLL = big ? SM - abn * isLL : SM;
+        const double SRR = big ? SM + abn * isRR : SM;
+        const double ssum = sLL + sRR;
+        const double isum = 1.0 / ssum;
+        const double sLR = sLL * sRR;
+        const double n1 = sLL * vLL + sRR * vRR + idbn * (btt2 - btt1);
+        const double n2 = sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1);
+        const double n3 = sLL * btt2 + sRR * btt1 + idbn * sLR * (vRR - vLL);
+        const double n4 = sLL * bmm2 + sRR * bmm1 + idbn * sLR * (wRR - wLL);
+        const double vLLL = n1 * isum, wLLL = n2 * isum, bttt = n3 * isum, bmmm = n4 * isum;
+        const double vb3 = SM * bn + vLLL * bttt + wLLL * bmmm;
+        const double eLLL = eLL - idbn * sLL * (vbL - vb3);
+        const double eRRR = eRR + idbn * sRR * (vbR - vb3);
+        const bool left = SM >= 0.0;
+        const double S1 = left ? SL : SR, S2 = left ? SLL : SRR;
+        const double rs = left ? ro_LL : ro_RR;
+        const double vs = left ? vLL : vRR, ws = left ? wLL : wRR;
+        const double bts = left ? btt1 : btt2, bms = left ? bmm1 : bmm2;
+        const double es = left ? eLL : eRR, ess = left ? eLLL : eRRR;
+        const bool outer = left ? (SL >= 0.0) : !(SRR > 0.0 || SR > 0.0);
+        const bool single = left ? (SLL >= 0.0) : !(SRR > 0.0);
+        const double Us[8] = {rs, rs * SM, rs * vs, rs * ws, es, bn, bts, bms};
+        const double Uss[8] = {rs, rs * SM, rs * vLLL, rs * wLLL, ess, bn, bttt, bmmm};
+        // ---------------------------------------------------------- HLLD reverse
+        double bUs[8], bUss[8], b_S1 = 0.0, b_S2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double g = i == 5 ? 0.0 : Pb[i];
+            const double Ui = left ? UL[i] : UR[i];
+            // region weights: outer F; single F + S1 (Us - U); double
+            // F + S2 Uss - (S2 - S1) Us - S1 U
+            const double w1 = outer ? 0.0 : 1.0, w2 = (outer || single) ? 0.0 : 1.0;
+            const double gU = -g * S1 * w1;
+            if (left) {
+                bFL[i] += g;
+                bUL[i] += gU;
+            } else {
+                bFR[i] += g;
+                bUR[i] += gU;
+            }
+            b_S1 += g * w1 * (Us[i] - Ui);
+            b_S2 += g * w2 * (Uss[i] - Us[i]);
+            bUss[i] = g * S2 * w2;
+            bUs[i] = w2 > 0.0 ? -g * (S2 - S1) : g * S1 * w1;
+        }
+        // Us = {rs, rs*SM, rs*vs, rs*ws, es, bn, bts, bms}
+        double b_rs = bUs[0] + bUs[1] * SM + bUs[2] * vs + bUs[3] * ws;
+        b_SM += bUs[1] * rs;
+        const double b_vs = bUs[2] * rs, b_ws = bUs[3] * rs;
+        const double b_es = bUs[4];
+        b_bn += bUs[5];
+        const double b_bts = bUs[6], b_bms = bUs[7];
+        // Uss = {rs, rs*SM, rs*vLLL, rs*wLLL, ess, bn, bttt, bmmm}
+        b_rs += bUss[0] + bUss[1] * SM + bUss[2] * vLLL + bUss[3] * wLLL;
+        b_SM += bUss[1] * rs;
+        double b_vLLL = bUss[2] * rs, b_wLLL = bUss[3] * rs;
+        const double b_ess = bUss[4];
+        b_bn += bUss[5];
+        double b_bttt = bUss[6], b_bmmm = bUss[7];
+        // route the side selections
+        double b_vLL = 0.0, b_wLL = 0.0, b_vRR = 0.0, b_wRR = 0.0;
+        double b_btt1 = 0.0, b_bmm1 = 0.0, b_btt2 = 0.0, b_bmm2 = 0.0;
+        double b_eLL = 0.0, b_eRR = 0.0, b_eLLL = 0.0, b_eRRR = 0.0, b_SLL = 0.0, b_SRR = 0.0;
+        if (left) {
+            b_SL += b_S1; b_SLL += b_S2; b_ro_LL += b_rs;
+            b_vLL += b_vs; b_wLL += b_ws; b_btt1 += b_bts; b_bmm1 += b_bms;
+            b_eLL += b_es; b_eLLL += b_ess;
+        } else {
+            b_SR += b_S1; b_SRR += b_S2; b_ro_RR += b_rs;
+            b_vRR += b_vs; b_wRR += b_ws; b_btt2 += b_bts; b_bmm2 += b_bms;
+            b_eRR += b_es; b_eRRR += b_ess;
+        }
+        // eRRR = eRR + ((idbn*sRR)*(vbR - vb3)); eLLL = eLL - ((idbn*sLL)*(vbL - vb3))
+        double b_vbL = 0.0, b_vbR = 0.0, b_vb3 = 0.0, b_sLL = 0.0, b_sRR = 0.0;
+        b_eRR += b_eRRR;
+        b_sRR += b_eRRR * idbn * (vbR - vb3);
+        b_vbR += b_eRRR * idbn * sRR;
+        b_vb3 -= b_eRRR * idbn * sRR;
+        b_eLL += b_eLLL;
+        b_sLL -= b_eLLL * idbn * (vbL - vb3);
+        b_vbL -= b_eLLL * idbn * sLL;
+        b_vb3 += b_eLLL * idbn * sLL;
+        // vb3 = SM*bn + vLLL*bttt + wLLL*bmmm
+        b_SM += b_vb3 * bn; b_bn += b_vb3 * SM;
+        b_vLLL += b_vb3 * bttt; b_bttt += b_vb3 * vLLL;
+        b_wLLL += b_vb3 * bmmm; b_bmmm += b_vb3 * wLLL;
+        // vLLL = n1*isum ...
+        const double b_n1 = b_vLLL * isum, b_n2 = b_wLLL * isum, b_n3 = b_bttt * isum, b_n4 = b_bmmm * isum;
+        const double b_isum = b_vLLL * n1 + b_wLLL * n2 + b_bttt * n3 + b_bmmm * n4;
+        const double b_ssum = -b_isum * isum * isum;
+        b_sLL += b_ssum; b_sRR += b_ssum;
+        // n1 = sLL*vLL + sRR*vRR + idbn*(btt2 - btt1)
+        b_sLL += b_n1 * vLL; b_vLL += b_n1 * sLL;
+        b_sRR += b_n1 * vRR; b_vRR += b_n1 * sRR;
+        b_btt2 += b_n1 * idbn; b_btt1 -= b_n1 * idbn;
+        b_sLL += b_n2 * wLL; b_wLL += b_n2 * sLL;
+        b_sRR += b_n2 * wRR; b_wRR += b_n2 * sRR;
+        b_bmm2 += b_n2 * idbn; b_bmm1 -= b_n2 * idbn;
+        // n3 = sLL*btt2 + sRR*btt1 + ((idbn*sLR)*(vRR - vLL))
+        double b_sLR = 0.0;
+        b_sLL += b_n3 * btt2; b_btt2 += b_n3 * sLL;
+        b_sRR += b_n3 * btt1; b_btt1 += b_n3 * sRR;
+        b_sLR += b_n3 * idbn * (vRR - vLL);
+        b_vRR += b_n3 * idbn * sLR; b_vLL -= b_n3 * idbn * sLR;
+        b_sLL += b_n4 * bmm2; b_bmm2 += b_n4 * sLL;
+        b_sRR += b_n4 * bmm1; b_bmm1 += b_n4 * sRR;
+        b_sLR += b_n4 * idbn * (wRR - wLL);
+        b_wRR += b_n4 * idbn * sLR; b_wLL -= b_n4 * idbn * sLR;
+        b_sLL += b_sLR * sRR; b_sRR += b_sLR * sLL;
+        // SLL = SM - abn*isLL, SRR = SM + abn*isRR (when big; else SM)
+        b_SM += b_SLL + b_SRR;
+        if (big) {
+            const double b_abn = -b_SLL * isLL + b_SRR * isRR;
+            const double b_isLL = -b_SLL * abn, b_isRR = b_SRR * abn;
+            b_sLL -= b_isLL * isLL * isLL;
+            b_sRR -= b_isRR * isRR * isRR;
+            b_bn += bn < 0.0 ? -b_abn : b_abn;
+        }
+        // sLL = sqrt(ro_LL)
+        b_ro_LL += sLL > 0.0 ? 0.5 * b_sLL / sLL : 0.0;
+        b_ro_RR += sRR > 0.0 ? 0.5 * b_sRR / sRR : 0.0;
+        // eLL = nEL*iLM, nEL = e1*suL + PTT*SM - pTL*u1 + bn*(sbv1 - vbL)
+        double b_PTT = 0.0;
+        {
+            const double b_n = b_eLL * iLM;
+            b_iLM += b_eLL * nEL;
+            b_e1 += b_n * suL; b_suL += b_n * e1;
+            b_PTT += b_n * SM; b_SM += b_n * PTT;
+            b_pTL -= b_n * u1; b_u1 -= b_n * pTL;
+            b_bn += b_n * (sbv1 - vbL);
+            const double b_sb = b_n * bn;
+            b_vbL -= b_sb;
+            b_u1 += b_sb * bn1; b_bn1 += b_sb * u1;
+            b_v1 += b_sb * bt1; b_bt1 += b_sb * v1;
+            b_w1 += b_sb * bm1; b_bm1 += b_sb * w1;
+        }
+        {
+            const double b_n = b_eRR * iRM;
+            b_iRM += b_eRR * nER;
+            b_e2 += b_n * suR; b_suR += b_n * e2;
+            b_PTT += b_n * SM; b_SM += b_n * PTT;
+            b_pTR -= b_n * u2; b_u2 -= b_n * pTR;
+            b_bn += b_n * (sbv2 - vbR);
+            const double b_sb = b_n * bn;
+            b_vbR -= b_sb;
+            b_u2 += b_sb * bn2; b_bn2 += b_sb * u2;
+            b_v2 += b_sb * bt2; b_bt2 += b_sb * v2;
+            b_w2 += b_sb * bm2; b_bm2 += b_sb * w2;
+        }
+        // vbL = SM*bn + vLL*btt1 + wLL*bmm1
+        b_SM += b_vbL * bn + b_vbR * bn;
+        b_bn += b_vbL * SM + b_vbR * SM;
+        b_vLL += b_vbL * btt1; b_btt1 += b_vbL * vLL;
+        b_wLL += b_vbL * bmm1; b_bmm1 += b_vbL * wLL;
+        b_vRR += b_vbR * btt2; b_btt2 += b_vbR * vRR;
+        b_wRR += b_vbR * bmm2; b_bmm2 += b_vbR * wRR;
+        // the tangential star states
+        double b_kL = 0.0, b_kR = 0.0, b_bbn = 0.0;
+        if (gL) {
+            // btt1 = bt1*qL, bmm1 = bm1*qL; qL = qnL*itL; vLL = v1 - ((hvL*dSL)*itL); wLL likewise
+            b_bt1 += b_btt1 * qL; b_bm1 += b_bmm1 * qL;
+            const double b_qL = b_btt1 * bt1 + b_bmm1 * bm1;
+            const double b_qn = b_qL * itL;
+            double b_it = b_qL * qnL;
+            b_kL += b_qn * suL; b_suL += b_qn * kL; b_bbn -= b_qn;
+            b_v1 += b_vLL; b_w1 += b_wLL;
+            const double gv = -b_vLL, gw = -b_wLL;
+            b_it += gv * hvL * dSL + gw * hwL * dSL;
+            const double b_dS = gv * hvL * itL + gw * hwL * itL;
+            const double b_hv = gv * dSL * itL, b_hw = gw * dSL * itL;
+            b_bn += b_hv * bt1 + b_hw * bm1;
+            b_bt1 += b_hv * bn; b_bm1 += b_hw * bn;
+            b_SM += b_dS; b_u1 -= b_dS;
+            const double b_tt = -b_it * itL * itL;
+            b_kL += b_tt * DLM;
+            const double b_DLM = b_tt * kL;
+            b_SL += b_DLM; b_SM -= b_DLM;
+            b_bbn -= b_tt;
+        } else {
+            b_v1 += b_vLL; b_w1 += b_wLL;
+        }
+        if (gR) {
+            b_bt2 += b_btt2 * qR; b_bm2 += b_bmm2 * qR;
+            const double b_qR = b_btt2 * bt2 + b_bmm2 * bm2;
+            const double b_qn = b_qR * itR;
+            double b_it = b_qR * qnR;
+            b_kR += b_qn * suR; b_suR += b_qn * kR; b_bbn -= b_qn;
+            b_v2 += b_vRR; b_w2 += b_wRR;
+            const double gv = -b_vRR, gw = -b_wRR;
+            b_it += gv * hvR * dSR + gw * hwR * dSR;
+            const double b_dS = gv * hvR * itR + gw * hwR * itR;
+            const double b_hv = gv * dSR * itR, b_hw = gw * dSR * itR;
+            b_bn += b_hv * bt2 + b_hw * bm2;
+            b_bt2 += b_hv * bn; b_bm2 += b_hw * bn;
+            b_SM += b_dS; b_u2 -= b_dS;
+            const double b_tt = -b_it * itR * itR;
+            b_kR += b_tt * DRM;
+            const double b_DRM = b_tt * kR;
+            b_SR += b_DRM; b_SM -= b_DRM;
+            b_bbn -= b_tt;
+        } else {
+            b_v2 += b_vRR; b_w2 += b_wRR;
+        }
+        b_ro_L += b_kL * suL; b_suL += b_kL * ro_L;
+        b_ro_R += b_kR * suR; b_suR += b_kR * ro_R;
+        b_bn += 2.0 * bn * b_bbn;
+        // PTT = nP*idm, nP = mR*pTL - mL*pTR + ((ro_L*ro_R)*suR*suL)*(u2 - u1)
+        {
+            const double b_n = b_PTT * idm;
+            b_idm += b_PTT * nP;
+            b_mR += b_n * pTL; b_pTL += b_n * mR;
+            b_mL -= b_n * pTR; b_pTR -= b_n * mL;
+            const double b_a3 = b_n * du, b_du = b_n * a3;
+            b_u2 += b_du; b_u1 -= b_du;
+            const double b_a2 = b_a3 * suL; b_suL += b_a3 * a2;
+            const double b_a1 = b_a2 * suR; b_suR += b_a2 * a1;
+            b_ro_L += b_a1 * ro_R; b_ro_R += b_a1 * ro_L;
         }
     }
-    P[5] = -(smax * (bn2 - bn1));
+    // ---------------------------------------------------------------- common reverse
+    // bbn = bn*bn is folded above for HLLD; HLLC-MHD does not use bbn
+    // ro_LL = (ro_L*suL)*iLM, ro_RR likewise
+    {
+        const double k = ro_L * suL;
+        b_iLM += b_ro_LL * k;
+        const double b_k = b_ro_LL * iLM;
+        b_ro_L += b_k * suL; b_suL += b_k * ro_L;
+    }
+    {
+        const double k = ro_R * suR;
+        b_iRM += b_ro_RR * k;
+        const double b_k = b_ro_RR * iRM;
+        b_ro_R += b_k * suR; b_suR += b_k * ro_R;
+    }
+    // iLM = 1/(SL - SM), iRM = 1/(SR - SM)
+    {
+        const double g = -b_iLM * iLM * iLM;
+        b_SL += g; b_SM -= g;
+        const double h = -b_iRM * iRM * iRM;
+        b_SR += h; b_SM -= h;
+    }
+    // bn = nbn*iS, bt = nbt*iS, bm = nbm*iS
+    {
+        const double gn = b_bn * iS, gt = b_bt * iS, gm = b_bm * iS;
+        b_iS += b_bn * nbn + b_bt * nbt + b_bm * nbm;
+        b_SR += gn * bn2 + gt * bt2 + gm * bm2;
+        b_SL -= gn * bn1 + gt * bt1 + gm * bm1;
+        b_bn2 += gn * SR; b_bn1 -= gn * SL;
+        b_bt2 += gt * SR; b_bt1 -= gt * SL;
+        b_bm2 += gm * SR; b_bm1 -= gm * SL;
+        bFL[6] += gt; bFR[6] -= gt;
+        bFL[7] += gm; bFR[7] -= gm;
+        const double g = -b_iS * iS * iS;
+        b_SR += g; b_SL -= g;
+    }
+    // U and F of each side
+    b_ro_L += bUL[0] + bUL[1] * u1 + bUL[2] * v1 + bUL[3] * w1;
+    b_u1 += bUL[1] * ro_L; b_v1 += bUL[2] * ro_L; b_w1 += bUL[3] * ro_L;
+    b_e1 += bUL[4]; b_bn1 += bUL[5]; b_bt1 += bUL[6]; b_bm1 += bUL[7];
+    b_ro_R += bUR[0] + bUR[1] * u2 + bUR[2] * v2 + bUR[3] * w2;
+    b_u2 += bUR[1] * ro_R; b_v2 += bUR[2] * ro_R; b_w2 += bUR[3] * ro_R;
+    b_e2 += bUR[4]; b_bn2 += bUR[5]; b_bt2 += bUR[6]; b_bm2 += bUR[7];
+    // FL: [0] ro u, [1] (ro u) u + pT - bn^2, [2] (ro u) v - bn bt, [3] (ro u) w - bn bm,
+    // [4] (e + pT) u - bn sbv, [6] u bt - v bn, [7] u bm - w bn
+    {
+        const double ru = ro_L * u1;
+        const double b_ru = bFL[0] + bFL[1] * u1 + bFL[2] * v1 + bFL[3] * w1;
+        b_ro_L += b_ru * u1; b_u1 += b_ru * ro_L;
+        b_u1 += bFL[1] * ru + bFL[4] * (e1 + pTL) + bFL[6] * bt1 + bFL[7] * bm1;
+        b_pTL += bFL[1] + bFL[4] * u1;
+        b_bn1 -= 2.0 * bn1 * bFL[1] + bFL[2] * bt1 + bFL[3] * bm1 + bFL[4] * sbv1 + bFL[6] * v1 + bFL[7] * w1;
+        b_v1 += bFL[2] * ru - bFL[6] * bn1;
+        b_w1 += bFL[3] * ru - bFL[7] * bn1;
+        b_bt1 += -bFL[2] * bn1 + bFL[6] * u1;
+        b_bm1 += -bFL[3] * bn1 + bFL[7] * u1;
+        b_e1 += bFL[4] * u1;
+        const double b_sb = -bFL[4] * bn1;
+        b_u1 += b_sb * bn1; b_bn1 += b_sb * u1;
+        b_v1 += b_sb * bt1; b_bt1 += b_sb * v1;
+        b_w1 += b_sb * bm1; b_bm1 += b_sb * w1;
+    }
+    {
+        const double ru = ro_R * u2;
+        const double b_ru = bFR[0] + bFR[1] * u2 + bFR[2] * v2 + bFR[3] * w2;
+        b_ro_R += b_ru * u2; b_u2 += b_ru * ro_R;
+        b_u2 += bFR[1] * ru + bFR[4] * (e2 + pTR) + bFR[6] * bt2 + bFR[7] * bm2;
+        b_pTR += bFR[1] + bFR[4] * u2;
+        b_bn2 -= 2.0 * bn2 * bFR[1] + bFR[2] * bt2 + bFR[3] * bm2 + bFR[4] * sbv2 + bFR[6] * v2 + bFR[7] * w2;
+        b_v2 += bFR[2] * ru - bFR[6] * bn2;
+        b_w2 += bFR[3] * ru - bFR[7] * bn2;
+        b_bt2 += -bFR[2] * bn2 + bFR[6] * u2;
+        b_bm2 += -bFR[3] * bn2 + bFR[7] * u2;
+        b_e2 += bFR[4] * u2;
+        const double b_sb = -bFR[4] * bn2;
+        b_u2 += b_sb * bn2; b_bn2 += b_sb * u2;
+        b_v2 += b_sb * bt2; b_bt2 += b_sb * v2;
+        b_w2 += b_sb * bm2; b_bm2 += b_sb * w2;
+    }
+    // e = p*inv_g1 + ro*uu + 0.5*bb
+    double b_bb_L = 0.5 * b_e1, b_bb_R = 0.5 * b_e2, b_uu_L = b_e1 * ro_L, b_uu_R = b_e2 * ro_R;
+    b_p_L += b_e1 * inv_g1; b_ro_L += b_e1 * uu_L;
+    b_p_R += b_e2 * inv_g1; b_ro_R += b_e2 * uu_R;
+    // SM = Nn*idm, Nn = mR*u2 - mL*u1 - pTR + pTL
+    {
+        const double b_N = b_SM * idm;
+        b_idm += b_SM * Nn;
+        b_mR += b_N * u2; b_u2 += b_N * mR;
+        b_mL -= b_N * u1; b_u1 -= b_N * mL;
+        b_pTR -= b_N; b_pTL += b_N;
+        const double b_D = -b_idm * idm * idm;
+        b_mR += b_D; b_mL -= b_D;
+    }
+    // mR = suR*ro_R, mL = suL*ro_L; suR = SR - u2, suL = SL - u1
+    b_suR += b_mR * ro_R; b_ro_R += b_mR * suR;
+    b_suL += b_mL * ro_L; b_ro_L += b_mL * suL;
+    b_SR += b_suR; b_u2 -= b_suR;
+    b_SL += b_suL; b_u1 -= b_suL;
+    // pT = p + 0.5*bb
+    b_p_L += b_pTL; b_bb_L += 0.5 * b_pTL;
+    b_p_R += b_pTR; b_bb_R += 0.5 * b_pTR;
+    // SL = umin - cmax, SR = umax + cmax
+    const double b_cmax = b_SR - b_SL;
+    if (uminL) b_u1 += b_SL; else b_u2 += b_SL;
+    if (umaxL) b_u1 += b_SR; else b_u2 += b_SR;
+    double b_cfL = cL ? b_cmax : 0.0, b_cfR = cL ? 0.0 : b_cmax;
+    // cf = sqrt(a), a = (gp + sqrt(d)) * (0.5*iro), d = gp^2 - (4 gamma p) bn^2
+    {
+        const double b_a = cfL > 0.0 ? 0.5 * b_cfL / cfL : 0.0;
+        const double h = 0.5 * iroL;
+        const double b_gs = b_a * h;
+        const double b_iro = b_a * (gpL + sdL) * 0.5;
+        const double b_d = sdL > 0.0 ? 0.5 * b_gs / sdL : 0.0;
+        const double b_gp = b_gs + b_d * 2.0 * gpL;
+        // d = gp*gp - (g4*p)*(bn^2)
+        b_p_L -= b_d * g4 * (bn1 * bn1);
+        b_bn1 -= b_d * (g4 * p_L) * 2.0 * bn1;
+        b_p_L += b_gp * gamma;
+        b_bb_L += b_gp;
+        b_ro_L -= b_iro * iroL * iroL;
+    }
+    {
+        const double b_a = cfR > 0.0 ? 0.5 * b_cfR / cfR : 0.0;
+        const double h = 0.5 * iroR;
+        const double b_gs = b_a * h;
+        const double b_iro = b_a * (gpR + sdR) * 0.5;
+        const double b_d = sdR > 0.0 ? 0.5 * b_gs / sdR : 0.0;
+        const double b_gp = b_gs + b_d * 2.0 * gpR;
+        b_p_R -= b_d * g4 * (bn2 * bn2);
+        b_bn2 -= b_d * (g4 * p_R) * 2.0 * bn2;
+        b_p_R += b_gp * gamma;
+        b_bb_R += b_gp;
+        b_ro_R -= b_iro * iroR * iroR;
+    }
+    // uu = 0.5*(u^2 + v^2 + w^2), bb = bn^2 + bt^2 + bm^2
+    b_u1 += b_uu_L * u1; b_v1 += b_uu_L * v1; b_w1 += b_uu_L * w1;
+    b_u2 += b_uu_R * u2; b_v2 += b_uu_R * v2; b_w2 += b_uu_R * w2;
+    b_bn1 += 2.0 * b_bb_L * bn1; b_bt1 += 2.0 * b_bb_L * bt1; b_bm1 += 2.0 * b_bb_L * bm1;
+    b_bn2 += 2.0 * b_bb_R * bn2; b_bt2 += 2.0 * b_bb_R * bt2; b_bm2 += 2.0 * b_bb_R * bm2;
+    ib[0] = b_ro_L; ib[1] = b_p_L; ib[2] = b_u1; ib[3] = b_v1; ib[4] = b_w1; ib[5] = b_bn1; ib[6] = b_bt1; ib[7] = b_bm1;
+    ib[8] = b_ro_R; ib[9] = b_p_R; ib[10] = b_u2; ib[11] = b_v2; ib[12] = b_w2; ib[13] = b_bn2; ib[14] = b_bt2;
+    ib[15] = b_bm2;
 }
 
-// mhd_flux over tangent numbers. L, R: r, u, v, p, bx, by, bz (Gaussian B);
-// F: lab-frame fluxes in conserved order r, mx, my, bx, by, bz, e.
+// mhd_flux reversed: L, R as r, u, v, p, bx, by, bz (Gaussian B); phi the
+// cotangent of the lab-frame fluxes in conserved order r, mx, my, bx, by,
+// bz, e; Lb, Rb the cotangents of L and R.
 template <int METOD, bool XFACE>
-__device__ __forceinline__ void mhd_flux_tg(const Tg (&L)[7], const Tg (&R)[7], double gamma, double inv_g1,
-                                            Tg (&F)[7]) {
+__device__ __forceinline__ void mhd_flux_vjp(const double (&L)[7], const double (&R)[7], double gamma,
+                                             double inv_g1, const double (&phi)[7], double (&Lb)[7],
+                                             double (&Rb)[7]) {
     const double is = 1.0 / kSpi4;
-    const Tg z = {0.0, 0.0};
-    Tg in[16];
+    double in[16], Pb[8];
     if constexpr (XFACE) {
-        const Tg a[16] = {L[0], L[3], L[1], L[2], z, L[4] * is, L[5] * is, L[6] * is,
-                          R[0], R[3], R[1], R[2], z, R[4] * is, R[5] * is, R[6] * is};
+        const double a[16] = {L[0], L[3], L[1], L[2], 0.0, L[4] * is, L[5] * is, L[6] * is,
+                              R[0], R[3], R[1], R[2], 0.0, R[4] * is, R[5] * is, R[6] * is};
 #pragma unroll
         for (int i = 0; i < 16; ++i) in[i] = a[i];
+        const double b[8] = {phi[0], phi[1], phi[2], 0.0, phi[6], kSpi4 * phi[3], kSpi4 * phi[4], kSpi4 * phi[5]};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Pb[i] = b[i];
     } else {
-        const Tg a[16] = {L[0], L[3], L[2], z, L[1], L[5] * is, L[6] * is, L[4] * is,
-                          R[0], R[3], R[2], z, R[1], R[5] * is, R[6] * is, R[4] * is};
+        const double a[16] = {L[0], L[3], L[2], 0.0, L[1], L[5] * is, L[6] * is, L[4] * is,
+                              R[0], R[3], R[2], 0.0, R[1], R[5] * is, R[6] * is, R[4] * is};
 #pragma unroll
         for (int i = 0; i < 16; ++i) in[i] = a[i];
+        const double b[8] = {phi[0], phi[2], 0.0, phi[1], phi[6], kSpi4 * phi[4], kSpi4 * phi[5], kSpi4 * phi[3]};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Pb[i] = b[i];
     }
-    Tg P[8];
-    hlld_face_tg<METOD>(in, gamma, inv_g1, P);
-    F[0] = P[0];
-    F[6] = P[4];
-    if constexpr (XFACE) {
-        F[1] = P[1];
-        F[2] = P[2];
-        F[3] = kSpi4 * P[5];
-        F[4] = kSpi4 * P[6];
-        F[5] = kSpi4 * P[7];
-    } else {
-        F[2] = P[1];
-        F[1] = P[3];
-        F[4] = kSpi4 * P[5];
-        F[5] = kSpi4 * P[6];
-        F[3] = kSpi4 * P[7];
+    double ib[16];
+    hlld_face_vjp<METOD>(in, gamma, inv_g1, Pb, ib);
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const double* g = ib + 8 * sd;
+        double* o = sd == 0 ? Lb : Rb;
+        o[0] = g[0];
+        o[3] = g[1];
+        if constexpr (XFACE) {
+            o[1] = g[2];
+            o[2] = g[3];
+            o[4] = g[5] * is;
+            o[5] = g[6] * is;
+            o[6] = g[7] * is;
+        } else {
+            o[2] = g[2];
+            o[1] = g[4];
+            o[5] = g[5] * is;
+            o[6] = g[6] * is;
+            o[4] = g[7] * is;
+        }
     }
 }
 
@@ -353,19 +783,16 @@ __global__ __launch_bounds__(256) void mhd_vjp_face_kernel(const MhdAdjArgs J) {
             }
         }
         const int mu = XFACE ? 1 : 2, mb = XFACE ? 4 : 5;  // components a mirror negates
-#pragma unroll 1
+        double Xv[7], Yv[7], Xb[7], Yb[7];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+            Xv[c] = comp_m(X, c);
+            Yv[c] = comp_m(Y, c);
+        }
+        mhd_flux_vjp<METOD, XFACE>(Xv, Yv, J.gamma, J.inv_g1, phi, Xb, Yb);
+#pragma unroll
         for (int k = 0; k < 14; ++k) {
-            Tg Xt[7], Yt[7];
-#pragma unroll
-            for (int c = 0; c < 7; ++c) {
-                Xt[c] = Tg{comp_m(X, c), k == c ? 1.0 : 0.0};
-                Yt[c] = Tg{comp_m(Y, c), k == c + 7 ? 1.0 : 0.0};
-            }
-            Tg F[7];
-            mhd_flux_tg<METOD, XFACE>(Xt, Yt, J.gamma, J.inv_g1, F);
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < 7; ++c) s += phi[c] * F[c].d;
+            double s = k < 7 ? Xb[k] : Yb[k - 7];
             int pl = k;
             if (wall) {
                 const int comp = k < 7 ? k : k - 7;
