@@ -236,3 +236,25 @@ def test_kernel_tvd_old_axisymmetric_sphere_matches_oracle(method):
     # Ker_Dekard takes neither option
     with pytest.raises(api.GD2Error):
         api.ker_dekard(g, sd, ud, s2, u2, T, T_do, method)
+
+
+@pytest.mark.parametrize("shape", [(1, 5), (5, 1), (2, 2), (3, 4), (66, 3)])
+def test_kernel_tvd_old_options_small_grids_match_oracle(shape):
+    """Kernel_TVD_old's sphere and 1/y terms on grids a few cells wide."""
+    nx, ny = shape
+    say(f"kernel_tvd_old options small grid {nx}x{ny}")
+    bc = (O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    g, og = grids(nx, ny, bc, gamma=1.4, dx=0.05, dy=0.05, x0=0.025, y0=0.025, r_frozen=0.06, axisym=True)
+    rng = np.random.default_rng(nx * 13 + ny)
+    s = np.ascontiguousarray(np.stack([rng.uniform(0.5, 1.5, (ny, nx)), rng.uniform(0.5, 1.5, (ny, nx))], -1))
+    u = np.ascontiguousarray(rng.uniform(-0.5, 0.5, (ny, nx, 2)))
+    for method in (0, 1, 2):
+        s2r, u2r, Tr = O.ref_euler_step("kernel_tvd_old", og, s, u, 1e7, 1e-4, method)
+        sd, ud = to_dev(s), to_dev(u)
+        s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+        T = torch.tensor([1e7], dtype=torch.float64, device=dev())
+        T_do = torch.tensor([1e-4], dtype=torch.float64, device=dev())
+        api.kernel_tvd_old(g, sd, ud, s2, u2, T, T_do, method)
+        assert np.max(np.abs(to_host(s2) - s2r) / np.abs(s2r)) <= 1e-12, method
+        assert np.max(np.abs(to_host(u2) - u2r)) <= 1e-12, method
+        assert T.item() == Tr or abs(T.item() - Tr) <= 1e-14 * Tr, method

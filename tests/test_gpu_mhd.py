@@ -306,3 +306,25 @@ def test_reference_layout_16384_square_rows_equal_one_row():
     assert np.max(np.abs(to_host(s4) - s2r) / np.abs(s2r)) <= 1e-12
     assert np.max(np.abs(to_host(b4) - b2r)) <= 1e-12
     assert abs(T4 - Tr) <= 1e-14 * Tr
+
+
+@pytest.mark.parametrize("shape", [(1, 5), (5, 1), (2, 2), (3, 4), (66, 3)])
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+def test_production_options_small_grids_match_oracle(shape, kind):
+    """Production sides on grids one or a few cells wide (ghost layers that
+    mirror across the whole grid, edge cells that are ghosts on two sides),
+    with the axisymmetric sources: one step against the oracle, 1e-12."""
+    from tests.test_oracle_wind import smooth_state
+    nx, ny = shape
+    say(f"production options small grid {kind} {nx}x{ny}")
+    g, og = grids(nx, ny, (O.BC_SYMMETRY, O.BC_OUTFLOW, O.BC_AXIS, O.BC_INFLOW), gamma=G53, dx=0.05, dy=0.05,
+                  x0=0.025, y0=0.025, axisym=True)
+    s, u, b = smooth_state(og, seed=nx * 7 + ny)
+    T0, Tdo = 1e7, 1e-4
+    for method in (2, 3):
+        s2r, u2r, b2r, Tr = O.ref_mhd_step(kind, og, s, u, b, T0, Tdo, 0, method)
+        s2, u2, b2, T = _ref_step_dev(kind, g, s, u, b, T0, Tdo, method)
+        assert np.max(np.abs(s2 - s2r) / np.abs(s2r)) <= 1e-12, method
+        assert np.max(np.abs(u2 - u2r)) <= 1e-12, method
+        assert np.max(np.abs(b2 - b2r)) <= 1e-12, method
+        assert abs(T - Tr) <= 1e-14 * Tr, method
