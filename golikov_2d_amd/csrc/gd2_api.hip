@@ -128,6 +128,10 @@ int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_ro
     long long best_cost = -1;
     for (int rows = min_rows; rows <= max_rows; ++rows) {
         const long long waves = (long long)tiles_x * ((ny + rows - 1) / rows);
+        // strips under four rows only where every wave has a SIMD of its own
+        // (a lone wave's march is then the critical path); with shared SIMDs
+        // their extra lead-in and halo rows cost more than they save
+        if (rows < 4 && waves > (long long)cus * 4) continue;
         const long long cost = ((waves + slots - 1) / slots) * (rows + overhead);
         if (best_cost < 0 || cost <= best_cost) {  // ties: the taller strip (less halo)
             best_cost = cost;
