@@ -113,6 +113,8 @@ def lib():
             "gd2_read_state_text": (ip, [gp, C.c_char_p, vp, vp, vp]),
             "gd2_write_state_b3": (ip, [gp, vp, vp, vp, C.c_char_p]),
             "gd2_write_tecplot": (ip, [gp, vp, vp, vp, ip, C.POINTER(dp), C.c_char_p]),
+            "gd2_write_tecplot_derived": (ip, [gp, vp, vp, vp, ip, C.c_char_p]),
+            "gd2_write_line_cuts": (ip, [gp, vp, vp, vp, C.c_char_p, C.c_char_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -130,7 +132,7 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
             "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
             "gd2_one_pass_preferred", "gd2_write_state_text", "gd2_read_state_text", "gd2_write_state_b3",
-            "gd2_write_tecplot"]
+            "gd2_write_tecplot", "gd2_write_tecplot_derived", "gd2_write_line_cuts"]
 
 
 def _check(rc, name):
@@ -447,3 +449,18 @@ def write_tecplot(g, s, u, b, path, nmin=3, solution_time=None):
     t = None if solution_time is None else C.byref(C.c_double(solution_time))
     _check(lib().gd2_write_tecplot(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3), int(nmin), t,
                                    os.fsencode(path)), "gd2_write_tecplot")
+
+
+def write_tecplot_derived(g, s, u, b, path, nmin=3):
+    """param_for_texplot_mini.txt: every nmin-th cell with PP, Fx, Fy, zav
+    (kernel.cu:4384-4460)."""
+    _check(lib().gd2_write_tecplot_derived(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3), int(nmin),
+                                           os.fsencode(path)), "gd2_write_tecplot_derived")
+
+
+def write_line_cuts(g, s, u, b, path_y0=None, path_x0=None):
+    """param_y=0.txt / param_x=0.txt: the first row's and first column's
+    cells 5 .. n-2 with PP, Fx, Fy, dP (kernel.cu:4283-4374)."""
+    enc = lambda p: None if p is None else os.fsencode(p)
+    _check(lib().gd2_write_line_cuts(C.byref(g), _host(g, s, 2), _host(g, u, 2), _host(g, b, 3), enc(path_y0),
+                                     enc(path_x0)), "gd2_write_line_cuts")

@@ -3,6 +3,8 @@
 //                                                                kernel.cu:3944-3969 reader)
 //   binary save_b3.dat    x, y, rho, p, vx, vy, Bz doubles     (kernel.cu:4261-4267)
 //   Tecplot FEPOINT zone  every nmin-th cell, quadrilaterals   (kernel.cu:3682-3808)
+//   end-of-run derived    Tecplot with PP, Fx, Fy, zav; line  (kernel.cu:4231-4460)
+//                         cuts along the first row / column
 //
 // Numbers are written as the reference's std::ofstream writes them: default
 // float format at precision 6, i.e. printf "%g" (std::to_chars with
@@ -69,6 +71,46 @@ int parallel_write(std::FILE* f, long long n, long long bytes_per_item, Body bod
 
 bool grid_shape_ok(const gd2_grid* g) { return g && g->nx >= 1 && g->ny >= 1; }
 
+}  // namespace
+
+// Derived fields of the reference's end-of-run output (kernel.cu:4283-4446):
+// PP = p + Bz^2/8pi, the forces of the azimuthal field Fx = Bz dBz/dx / 4pi,
+// Fy = (-Bz^2/y - Bz dBz/dy) / 4pi (central differences inside, forward at the
+// first column / row, 0 on the last), the vorticity zav.
+namespace {
+constexpr double kPiRef = 3.14159265358979323846;  // Header.h:18
+
+struct Derived {
+    double Fx, Fy, zav;
+};
+
+// kernel.cu:4400-4436 (cdiff_y: central y difference where m > 0, as the
+// Tecplot file; the x = 0 / y = 0 line cuts take the forward one)
+inline Derived derived_at(const gd2_grid* g, const gd2_double2* u, const gd2_double3* b, int n, int m,
+                          bool cdiff_y) {
+    const int N = g->nx, M = g->ny;
+    const long long k = (long long)m * N + n;
+    Derived d = {0.0, 0.0, 0.0};
+    if (m < M - 1 && n < N - 1) {
+        const double y = g->y0 + m * g->dy;
+        const long long kk = k + N, kk2 = k - N, e = k + 1, w = k - 1;
+        const double dAz = n > 0 ? (b[e].z - b[w].z) / (2.0 * g->dx) : (b[e].z - b[k].z) / (g->dx);
+        const double dAr = (cdiff_y && m > 0) ? (b[kk].z - b[kk2].z) / (2.0 * g->dy) : (b[kk].z - b[k].z) / (g->dy);
+        d.Fx = b[k].z * dAz / (4.0 * kPiRef);
+        d.Fy = (-b[k].z * b[k].z / y - b[k].z * dAr) / (4.0 * kPiRef);
+        if (n > 0 && m > 0) d.zav = (u[e].y - u[w].y) / (2.0 * g->dx) - (u[kk].x - u[kk2].x) / (2.0 * g->dy);
+    }
+    return d;
+}
+
+inline void mach_temp(const gd2_grid* g, const gd2_double2& s, const gd2_double2& u, double& Max, double& Temp) {
+    Max = 0.0;
+    Temp = 0.0;
+    if (s.x > 0.0) {
+        Max = std::sqrt((u.x * u.x + u.y * u.y) / (g->gamma * s.y / s.x));
+        Temp = s.y / s.x;
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -280,6 +322,111 @@ int gd2_write_tecplot(const gd2_grid* g, const gd2_double2* s, const gd2_double2
             return p;
         });
     if (std::fclose(f) != 0) rc = GD2_ERR_ARG;
+    return rc;
+}
+
+
+int gd2_write_tecplot_derived(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                              int nmin, const char* path) {
+    if (!grid_shape_ok(g) || !s || !u || !b || !path || nmin < 1) return GD2_ERR_ARG;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return GD2_ERR_ARG;
+    const int nx = g->nx, ny = g->ny;
+    const long long nn = (nx + nmin - 1) / nmin, mm = (ny + nmin - 1) / nmin;
+    std::string head = "TITLE = \"HP\"  VARIABLES = \"X\", \"Y\", \"Ro\", \"P\", \"Vx\", \"Vy\", \"Bx\", \"By\", \"Bz\", "
+                       "\"Max\", \"T\", \"PP\", \"Fx\", \"Fy\", \"zav\",  ZONE T = \"HP\", N = ";
+    char num[64];
+    head.append(num, put_i(num, nn * mm));
+    head += " , E = ";
+    head.append(num, put_i(num, (nn - 1) * (mm - 1)));
+    head += ", F = FEPOINT, ET = quadrilateral\n";
+    int rc = std::fwrite(head.data(), 1, head.size(), f) == head.size() ? GD2_OK : GD2_ERR_ARG;
+    if (rc == GD2_OK)
+        rc = parallel_write(f, nn * mm, 15 * 32 + 2, [&](long long lo, long long hi, char* p) {
+            for (long long q = lo; q < hi; ++q) {
+                const int n = (int)(q % nn) * nmin, m = (int)(q / nn) * nmin;
+                const long long k = (long long)m * nx + n;
+                const double y = g->y0 + m * g->dy, x = g->x0 + n * g->dx;
+                const Derived d = derived_at(g, u, b, n, m, true);
+                double Max, Temp;
+                mach_temp(g, s[k], u[k], Max, Temp);
+                const double v[15] = {x, y, s[k].x, s[k].y, u[k].x, u[k].y, b[k].x, b[k].y, b[k].z, Max, Temp,
+                                      s[k].y + b[k].z * b[k].z / (8.0 * kPiRef), d.Fx, d.Fy, d.zav};
+                for (int c = 0; c < 15; ++c) {
+                    p = put_g(p, v[c]);
+                    *p++ = c == 14 ? '\n' : ' ';
+                }
+            }
+            return p;
+        });
+    if (rc == GD2_OK && nn > 1 && mm > 1)
+        rc = parallel_write(f, (nn - 1) * (mm - 1), 4 * 21 + 4, [&](long long lo, long long hi, char* p) {
+            for (long long q = lo; q < hi; ++q) {
+                const long long n = q % (nn - 1), m = q / (nn - 1);
+                p = put_i(p, m * nn + n + 1); *p++ = ' ';
+                p = put_i(p, m * nn + n + 2); *p++ = ' ';
+                p = put_i(p, (m + 1) * nn + n + 2); *p++ = ' ';
+                p = put_i(p, (m + 1) * nn + n + 1); *p++ = '\n';
+            }
+            return p;
+        });
+    if (std::fclose(f) != 0) rc = GD2_ERR_ARG;
+    return rc;
+}
+
+int gd2_write_line_cuts(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                        const char* path_y0, const char* path_x0) {
+    if (!grid_shape_ok(g) || !s || !u || !b || (!path_y0 && !path_x0)) return GD2_ERR_ARG;
+    const int N = g->nx, M = g->ny;
+    auto pp = [&](long long k) { return s[k].y + b[k].z * b[k].z / (8.0 * kPiRef); };
+    int rc = GD2_OK;
+    char line[16 * 32];
+    // the first row (kernel.cu:4283-4327): cells 5 .. N-2
+    if (path_y0) {
+        std::FILE* f = std::fopen(path_y0, "wb");
+        if (!f) return GD2_ERR_ARG;
+        const char* head = "TITLE = \"HP\"  VARIABLES = \"X\", \"Ro\", \"P\", \"Vx\", \"Bz\", \"Max\", \"T\", \"PP\", "
+                           "\"Fx\", \"Fy\", \"dP\", ZONE T = \"HP\"\n";
+        if (std::fputs(head, f) < 0) rc = GD2_ERR_ARG;
+        for (int n = 5; n < N - 1 && rc == GD2_OK; ++n) {
+            const long long k = n;
+            const Derived d = derived_at(g, u, b, n, 0, false);
+            double Max, Temp;
+            mach_temp(g, s[k], u[k], Max, Temp);
+            const double v[11] = {g->x0 + n * g->dx, s[k].x, s[k].y, u[k].x, b[k].z, Max, Temp, pp(k), d.Fx, d.Fy,
+                                  (pp(k + 1) - pp(k)) / g->dx};
+            char* p = line;
+            for (int c = 0; c < 11; ++c) {
+                p = put_g(p, v[c]);
+                *p++ = c == 10 ? '\n' : ' ';
+            }
+            if (std::fwrite(line, 1, (size_t)(p - line), f) != (size_t)(p - line)) rc = GD2_ERR_ARG;
+        }
+        if (std::fclose(f) != 0) rc = GD2_ERR_ARG;
+    }
+    // the first column (kernel.cu:4330-4374): cells 5 .. M-2
+    if (path_x0 && rc == GD2_OK) {
+        std::FILE* f = std::fopen(path_x0, "wb");
+        if (!f) return GD2_ERR_ARG;
+        const char* head = "TITLE = \"HP\"  VARIABLES = \"Y\", \"Ro\", \"P\", \"Vy\", \"Bz\", \"Max\", \"T\", \"PP\", "
+                           "\"Fx\", \"Fy\", \"dP\", ZONE T = \"HP\"\n";
+        if (std::fputs(head, f) < 0) rc = GD2_ERR_ARG;
+        for (int m = 5; m < M - 1 && rc == GD2_OK; ++m) {
+            const long long k = (long long)m * N;
+            const Derived d = derived_at(g, u, b, 0, m, false);
+            double Max, Temp;
+            mach_temp(g, s[k], u[k], Max, Temp);
+            const double v[11] = {g->y0 + m * g->dy, s[k].x, s[k].y, u[k].y, b[k].z, Max, Temp, pp(k), d.Fx, d.Fy,
+                                  (pp(k + N) - pp(k)) / g->dy};
+            char* p = line;
+            for (int c = 0; c < 11; ++c) {
+                p = put_g(p, v[c]);
+                *p++ = c == 10 ? '\n' : ' ';
+            }
+            if (std::fwrite(line, 1, (size_t)(p - line), f) != (size_t)(p - line)) rc = GD2_ERR_ARG;
+        }
+        if (std::fclose(f) != 0) rc = GD2_ERR_ARG;
+    }
     return rc;
 }
 

@@ -301,6 +301,22 @@ int gd2_write_state_b3(const gd2_grid* g, const gd2_double2* s, const gd2_double
 int gd2_write_tecplot(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
                       const gd2_double3* b, int nmin, const double* solution_time, const char* path);
 
+/* The end-of-run derived Tecplot file param_for_texplot_mini.txt
+ * (kernel.cu:4231-4233, 4384-4460): every nmin-th cell on the cell centres,
+ * X Y Ro P Vx Vy Bx By Bz Max T PP Fx Fy zav with PP = p + Bz^2/8pi,
+ * Fx = Bz dBz/dx / 4pi, Fy = (-Bz^2/y - Bz dBz/dy) / 4pi (central
+ * differences, forward on the first column / row, 0 on the last) and the
+ * vorticity zav (0 on the first and last column / row). */
+int gd2_write_tecplot_derived(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                              const gd2_double3* b, int nmin, const char* path);
+/* The end-of-run line cuts (kernel.cu:4238-4246, 4283-4374): param_y=0.txt,
+ * the first row's cells 5 .. nx-2 (X Ro P Vx Bz Max T PP Fx Fy dP), and
+ * param_x=0.txt, the first column's cells 5 .. ny-2 (Y Ro P Vy Bz Max T PP Fx
+ * Fy dP), with forward y differences and dP the forward difference of PP;
+ * either path may be NULL. */
+int gd2_write_line_cuts(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
+                        const gd2_double3* b, const char* path_y0, const char* path_x0);
+
 /* Last HIP error code seen by the library (hipError_t value). */
 int gd2_last_hip_error(void);
 /* Library version string. */

@@ -1442,3 +1442,124 @@ int o_write_tecplot(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b
     }
     return fclose(f) == 0 ? 0 : -1;
 }
+
+/* End-of-run derived outputs (kernel.cu:4231-4460), one cell at a time as
+ * the reference's loops write them. pi as Header.h:18. */
+#define O_PI 3.14159265358979323846
+
+/* param_for_texplot_mini.txt (kernel.cu:4384-4460) */
+int o_write_tecplot_derived(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int nmin,
+                            const char* path)
+{
+    if (nmin < 1) return -1;
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    const int N = g->nx, M = g->ny;
+    const long K = (long)N * M;
+    const int nn = (N + nmin - 1) / nmin, mm = (M + nmin - 1) / nmin;
+    fprintf(f, "TITLE = \"HP\"  VARIABLES = \"X\", \"Y\", \"Ro\", \"P\", \"Vx\", \"Vy\", \"Bx\", \"By\", \"Bz\", "
+               "\"Max\", \"T\", \"PP\", \"Fx\", \"Fy\", \"zav\",  ZONE T = \"HP\", N = %d , E = %d, F = FEPOINT, "
+               "ET = quadrilateral\n", nn * mm, (nn - 1) * (mm - 1));
+    for (long k = 0; k < K; k++) {
+        int n = (int)(k % N);
+        int m = (int)((k - n) / N);
+        double y = g->y0 + m * g->dy;
+        double x = g->x0 + n * g->dx;
+        if ((n % nmin != 0) || (m % nmin != 0)) continue;
+        long kk = (long)(m + 1) * N + n, kk2 = (long)(m - 1) * N + n;
+        long e = (long)m * N + n + 1, w = (long)m * N + n - 1;
+        double zav = 0.0, Fx = 0.0, Fy = 0.0;
+        if (m < M - 1 && n < N - 1) {
+            double dAr, dAz;
+            if (n > 0) dAz = (b[e].z - b[w].z) / (2.0 * g->dx);
+            else dAz = (b[e].z - b[k].z) / (g->dx);
+            if (m > 0) dAr = (b[kk].z - b[kk2].z) / (2.0 * g->dy);
+            else dAr = (b[kk].z - b[k].z) / (g->dy);
+            Fx = b[k].z * dAz / (4.0 * O_PI);
+            Fy = (-b[k].z * b[k].z / y - b[k].z * dAr) / (4.0 * O_PI);
+            if (n > 0 && m > 0)
+                zav = (u[e].y - u[w].y) / (2.0 * g->dx) - (u[kk].x - u[kk2].x) / (2.0 * g->dy);
+        }
+        double Max = 0.0, Temp = 0.0;
+        if (s[k].x > 0.0) {
+            Max = sqrt((u[k].x * u[k].x + u[k].y * u[k].y) / (g->gamma * s[k].y / s[k].x));
+            Temp = s[k].y / s[k].x;
+        }
+        fprintf(f, "%g %g %g %g %g %g %g %g %g %g %g %g %g %g %g\n", x, y, s[k].x, s[k].y, u[k].x, u[k].y, b[k].x,
+                b[k].y, b[k].z, Max, Temp, s[k].y + (b[k].z * b[k].z) / (8.0 * O_PI), Fx, Fy, zav);
+    }
+    for (int k = 0; k < nn * mm; k = k + 1) {
+        int n = k % nn;
+        int m = (k - n) / nn;
+        if ((m < mm - 1) && (n < nn - 1))
+            fprintf(f, "%d %d %d %d\n", m * nn + n + 1, m * nn + n + 2, (m + 1) * nn + n + 2, (m + 1) * nn + n + 1);
+    }
+    return fclose(f) == 0 ? 0 : -1;
+}
+
+/* param_y=0.txt (kernel.cu:4283-4327) and param_x=0.txt (kernel.cu:4330-4374) */
+int o_write_line_cuts(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path_y0,
+                      const char* path_x0)
+{
+    const int N = g->nx, M = g->ny;
+    if (path_y0) {
+        FILE* f = fopen(path_y0, "w");
+        if (!f) return -1;
+        fprintf(f, "TITLE = \"HP\"  VARIABLES = \"X\", \"Ro\", \"P\", \"Vx\", \"Bz\", \"Max\", \"T\", \"PP\", \"Fx\", "
+                   "\"Fy\", \"dP\", ZONE T = \"HP\"\n");
+        for (int k = 5; k < N - 1; k++) {
+            int n = k % N;
+            int m = (k - n) / N;
+            double y = g->y0 + m * g->dy;
+            double x = g->x0 + n * g->dx;
+            long kk = (long)(m + 1) * N + n;
+            double Max = 0.0, Temp = 0.0;
+            if (s[k].x > 0.0) {
+                Max = sqrt((u[k].x * u[k].x + u[k].y * u[k].y) / (g->gamma * s[k].y / s[k].x));
+                Temp = s[k].y / s[k].x;
+            }
+            long e = (long)m * N + n + 1, w = (long)m * N + n - 1;
+            double Fx = 0.0, Fy = 0.0;
+            if (m < M - 1 && n < N - 1) {
+                double dAz = n > 0 ? (b[e].z - b[w].z) / (2.0 * g->dx) : (b[e].z - b[k].z) / (g->dx);
+                double dAr = (b[kk].z - b[k].z) / (g->dy);
+                Fx = b[k].z * dAz / (4.0 * O_PI);
+                Fy = (-b[k].z * b[k].z / y - b[k].z * dAr) / (4.0 * O_PI);
+            }
+            fprintf(f, "%g %g %g %g %g %g %g %g %g %g %g\n", x, s[k].x, s[k].y, u[k].x, b[k].z, Max, Temp,
+                    s[k].y + (b[k].z * b[k].z) / (8.0 * O_PI), Fx, Fy,
+                    ((s[e].y + (b[e].z * b[e].z) / (8.0 * O_PI)) - (s[k].y + (b[k].z * b[k].z) / (8.0 * O_PI))) / g->dx);
+        }
+        if (fclose(f) != 0) return -1;
+    }
+    if (path_x0) {
+        FILE* f = fopen(path_x0, "w");
+        if (!f) return -1;
+        fprintf(f, "TITLE = \"HP\"  VARIABLES = \"Y\", \"Ro\", \"P\", \"Vy\", \"Bz\", \"Max\", \"T\", \"PP\", \"Fx\", "
+                   "\"Fy\", \"dP\", ZONE T = \"HP\"\n");
+        for (int m = 5; m < M - 1; m++) {
+            int n = 0;
+            double y = g->y0 + m * g->dy;
+            long kk = (long)(m + 1) * N;
+            long e = (long)m * N + 0 + 1;
+            long k = (long)m * N + 0;
+            double Max = 0.0, Temp = 0.0;
+            if (s[k].x > 0.0) {
+                Max = sqrt((u[k].x * u[k].x + u[k].y * u[k].y) / (g->gamma * s[k].y / s[k].x));
+                Temp = s[k].y / s[k].x;
+            }
+            double Fx = 0.0, Fy = 0.0;
+            if (m < M - 1 && n < N - 1) {
+                double dAz = (b[e].z - b[k].z) / (g->dx);
+                double dAr = (b[kk].z - b[k].z) / (g->dy);
+                Fx = b[k].z * dAz / (4.0 * O_PI);
+                Fy = (-b[k].z * b[k].z / y - b[k].z * dAr) / (4.0 * O_PI);
+            }
+            fprintf(f, "%g %g %g %g %g %g %g %g %g %g %g\n", y, s[k].x, s[k].y, u[k].y, b[k].z, Max, Temp,
+                    s[k].y + (b[k].z * b[k].z) / (8.0 * O_PI), Fx, Fy,
+                    ((s[kk].y + (b[kk].z * b[kk].z) / (8.0 * O_PI)) - (s[k].y + (b[k].z * b[k].z) / (8.0 * O_PI))) / g->dy);
+        }
+        if (fclose(f) != 0) return -1;
+    }
+    return 0;
+}

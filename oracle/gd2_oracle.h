@@ -125,6 +125,10 @@ int o_read_state_text(const o_grid* g, const char* path, o_d2* s, o_d2* u, o_d3*
 int o_write_state_b3(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path);
 int o_write_tecplot(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int nmin,
                     const double* solution_time, const char* path);
+int o_write_tecplot_derived(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int nmin,
+                            const char* path);
+int o_write_line_cuts(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, const char* path_y0,
+                      const char* path_x0);
 
 int o_num_threads(void);
 void o_set_num_threads(int n);

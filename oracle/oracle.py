@@ -105,6 +105,10 @@ def lib():
             fn.argtypes = [gp, vp, vp, vp, C.c_char_p]
         L.o_read_state_text.restype = C.c_int
         L.o_read_state_text.argtypes = [gp, C.c_char_p, vp, vp, vp]
+        L.o_write_tecplot_derived.restype = C.c_int
+        L.o_write_tecplot_derived.argtypes = [gp, vp, vp, vp, C.c_int, C.c_char_p]
+        L.o_write_line_cuts.restype = C.c_int
+        L.o_write_line_cuts.argtypes = [gp, vp, vp, vp, C.c_char_p, C.c_char_p]
         L.o_write_tecplot.restype = C.c_int
         L.o_write_tecplot.argtypes = [gp, vp, vp, vp, C.c_int, dp, C.c_char_p]
         L.o_num_threads.restype = C.c_int
@@ -289,3 +293,14 @@ def write_tecplot(g, s, u, b, path, nmin=3, solution_time=None):
     if lib().o_write_tecplot(C.byref(g), _ptr(s), _ptr(u), _ptr(b) if b is not None else None, nmin, t,
                              os.fsencode(path)) != 0:
         raise OSError(path)
+
+
+def write_tecplot_derived(g, s, u, b, path, nmin=3):
+    if lib().o_write_tecplot_derived(C.byref(g), _ptr(s), _ptr(u), _ptr(b), nmin, os.fsencode(path)) != 0:
+        raise OSError(path)
+
+
+def write_line_cuts(g, s, u, b, path_y0=None, path_x0=None):
+    enc = lambda p: None if p is None else os.fsencode(p)
+    if lib().o_write_line_cuts(C.byref(g), _ptr(s), _ptr(u), _ptr(b), enc(path_y0), enc(path_x0)) != 0:
+        raise OSError(str(path_y0))

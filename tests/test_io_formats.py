@@ -196,3 +196,67 @@ def test_io_arguments_rejected(tmp_path):
         api.write_state_text(g, s[:, :2], u, b, tmp_path / "x.txt")
     with pytest.raises(api.GD2Error):
         api.write_state_text(g, s, u, b, tmp_path / "no" / "such" / "dir.txt")
+
+
+def _smooth(ny, nx, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 2.0, 6)
+    X, Y = np.meshgrid(np.arange(nx) * 0.1, np.arange(ny) * 0.1)
+    s = np.ascontiguousarray(np.stack([1 + 0.3 * np.sin(a[0] * X) * np.cos(a[1] * Y), 0.8 + 0.2 * np.cos(a[2] * X + Y)], -1))
+    u = np.ascontiguousarray(np.stack([0.4 * np.sin(a[3] * Y), 0.3 * np.cos(a[4] * X)], -1))
+    b = np.ascontiguousarray(np.stack([0.2 * np.cos(Y), 0.1 * np.sin(X), 0.5 * np.sin(a[5] * X + Y) + 0.6], -1))
+    return s, u, b
+
+
+@pytest.mark.parametrize("nx,ny,nmin", [(23, 17, 3), (12, 9, 1), (7, 4, 2)])
+def test_derived_tecplot_matches_oracle_and_finite_differences(tmp_path, nx, ny, nmin):
+    g, og = grids(nx, ny, dx=0.1, dy=0.1, gamma=5.0 / 3.0)
+    s, u, b = _smooth(ny, nx, nx + ny)
+    a, o = tmp_path / "lib.dat", tmp_path / "oracle.dat"
+    api.write_tecplot_derived(g, s, u, b, a, nmin=nmin)
+    O.write_tecplot_derived(og, s, u, b, o, nmin=nmin)
+    assert a.read_bytes() == o.read_bytes()
+    lines = o.read_text().splitlines()
+    nn, mm = -(-nx // nmin), -(-ny // nmin)
+    assert lines[0].endswith(f'"zav",  ZONE T = "HP", N = {nn * mm} , E = {(nn - 1) * (mm - 1)}, '
+                             'F = FEPOINT, ET = quadrilateral')
+    rows = np.array([[float(t) for t in l.split()] for l in lines[1:1 + nn * mm]])
+    bz, vx, vy = b[..., 2], u[..., 0], u[..., 1]
+    y = g.y0 + np.arange(ny) * g.dy
+    q = 0
+    for m in range(0, ny, nmin):
+        for n in range(0, nx, nmin):
+            Fx = Fy = zav = 0.0
+            if m < ny - 1 and n < nx - 1:
+                dz = (bz[m, n + 1] - bz[m, n - 1]) / 0.2 if n > 0 else (bz[m, n + 1] - bz[m, n]) / 0.1
+                dr = (bz[m + 1, n] - bz[m - 1, n]) / 0.2 if m > 0 else (bz[m + 1, n] - bz[m, n]) / 0.1
+                Fx = bz[m, n] * dz / (4 * np.pi)
+                Fy = (-bz[m, n] ** 2 / y[m] - bz[m, n] * dr) / (4 * np.pi)
+                if n > 0 and m > 0:
+                    zav = (vy[m, n + 1] - vy[m, n - 1]) / 0.2 - (vx[m + 1, n] - vx[m - 1, n]) / 0.2
+            PP = s[m, n, 1] + bz[m, n] ** 2 / (8 * np.pi)
+            assert np.allclose(rows[q, 11:], [PP, Fx, Fy, zav], rtol=1e-5, atol=1e-6), (m, n)
+            q += 1
+
+
+def test_line_cuts_match_oracle_and_finite_differences(tmp_path):
+    nx, ny = 19, 14
+    g, og = grids(nx, ny, dx=0.1, dy=0.1, gamma=5.0 / 3.0)
+    s, u, b = _smooth(ny, nx, 3)
+    ay, ax, oy, ox = (tmp_path / n for n in ("ly", "lx", "oy", "ox"))
+    api.write_line_cuts(g, s, u, b, ay, ax)
+    O.write_line_cuts(og, s, u, b, oy, ox)
+    assert ay.read_bytes() == oy.read_bytes() and ax.read_bytes() == ox.read_bytes()
+    ry = np.array([[float(t) for t in l.split()] for l in oy.read_text().splitlines()[1:]])
+    rx = np.array([[float(t) for t in l.split()] for l in ox.read_text().splitlines()[1:]])
+    assert len(ry) == nx - 6 and len(rx) == ny - 6
+    pp = s[..., 1] + b[..., 2] ** 2 / (8 * np.pi)
+    for i, n in enumerate(range(5, nx - 1)):
+        assert np.isclose(ry[i, 0], g.x0 + n * g.dx, rtol=1e-5)
+        assert np.isclose(ry[i, 10], (pp[0, n + 1] - pp[0, n]) / 0.1, rtol=1e-5, atol=1e-6)
+    for i, m in enumerate(range(5, ny - 1)):
+        assert np.isclose(rx[i, 0], g.y0 + m * g.dy, rtol=1e-5)
+        assert np.isclose(rx[i, 10], (pp[m + 1, 0] - pp[m, 0]) / 0.1, rtol=1e-5, atol=1e-6)
+    # either file alone
+    api.write_line_cuts(g, s, u, b, path_x0=tmp_path / "only_x")
+    assert (tmp_path / "only_x").read_bytes() == ox.read_bytes()
