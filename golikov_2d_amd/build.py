@@ -1,5 +1,6 @@
-"""Build libgd2.so (HIP, gfx950) with hipcc and the LibTorch adjoint module
-gd2_torch (host C++ over the libgd2 C ABI) with g++, both in-tree."""
+"""Build libgd2.so (HIP, gfx950) with hipcc, and with g++ the LibTorch adjoint
+module gd2_torch and the production-run driver gd2_production (both host C++
+over the libgd2 C ABI), all in-tree."""
 from __future__ import annotations
 
 import os
@@ -58,6 +59,23 @@ def build_torch_module(verbose: bool = False) -> str:
     return out
 
 
+PRODUCTION_SRC = "gd2_production.cpp"
+PRODUCTION = os.path.join(HERE, "gd2_production")
+
+
+def build_production(verbose: bool = False) -> str:
+    """g++ build of the production-run driver, an executable over libgd2.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__=1", "-I", "/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, PRODUCTION_SRC),
+           f"-L{HERE}", "-lgd2", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", PRODUCTION + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(PRODUCTION + ".tmp", PRODUCTION)
+    return PRODUCTION
+
+
 def build(verbose: bool = False, extra=()) -> str:
     cmd = [HIPCC, *FLAGS, *extra, "-I", os.path.join(ROOT, "include"),
            *[os.path.join(CSRC, s) for s in SOURCES], "-o", SO + ".tmp"]
@@ -70,5 +88,6 @@ def build(verbose: bool = False, extra=()) -> str:
 
 if __name__ == "__main__":
     build(verbose=True, extra=[a for a in sys.argv[1:] if a != "--torch"])
+    build_production(verbose=True)
     if "--torch" in sys.argv[1:]:
         build_torch_module(verbose=True)

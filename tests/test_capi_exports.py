@@ -120,3 +120,16 @@ int main(int argc, char** argv) {
     assert r.returncode == 0, (r.returncode, r.stderr)
     assert "gfx950" in r.stdout
     assert out.read_text() == "0.25 0.5 1 2.5 0 0 0 0 0\n0.75 0.5 0.125 0.1 -1 3 0 0 0\n"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(api.SO_PATH), "gd2_production")),
+                    reason="gd2_production not built")
+def test_production_driver_links_and_rejects_bad_arguments():
+    """The production-run driver resolves libgd2 and HIP at load and refuses an
+    unknown option or a degenerate grid with its usage line and status 2,
+    before any device call."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(api.SO_PATH), "gd2_production")
+    for args in (["--bogus", "1"], ["--nx", "1"], ["--steps1"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "usage:" in r.stderr, (args, r.returncode, r.stderr)
