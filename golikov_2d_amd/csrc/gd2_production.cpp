@@ -9,7 +9,7 @@
 //
 //   gd2_production [--nx 3226] [--ny 1792] [--x-max 9] [--y-max 5] [--steps1 200000]
 //                  [--steps2 400000] [--method 2] [--snap1 50000] [--snap2 100000]
-//                  [--progress 10000] [--out .]
+//                  [--progress 10000] [--restart FILE] [--out .]
 //
 // Every --progress steps (0: never) the host waits for the device and prints
 // the step counter and the model time TT, as the reference's loop prints its
@@ -40,7 +40,7 @@ struct Opts {
     double x_max = 9.0, y_max = 5.0;
     long steps1 = 200000, steps2 = 400000, snap1 = 50000, snap2 = 100000, progress = 10000;
     int method = 2;
-    std::string out = ".";
+    std::string out = ".", restart;
 };
 
 bool parse(int argc, char** argv, Opts& o) {
@@ -59,6 +59,7 @@ bool parse(int argc, char** argv, Opts& o) {
         else if (k == "--progress") o.progress = std::atol(v);
         else if (k == "--method") o.method = std::atoi(v);
         else if (k == "--out") o.out = v;
+        else if (k == "--restart") o.restart = v;
         else return false;
     }
     return o.nx >= 2 && o.ny >= 2 && o.x_max > 0.0 && o.y_max > 0.0 && o.steps1 >= 0 && o.steps2 >= 0;
@@ -87,7 +88,7 @@ int main(int argc, char** argv) {
     Opts o;
     if (!parse(argc, argv, o)) {
         std::fprintf(stderr, "usage: %s [--nx N] [--ny M] [--x-max X] [--y-max Y] [--steps1 S] [--steps2 S] "
-                             "[--method 2|3] [--snap1 S] [--snap2 S] [--progress S] [--out DIR]\n", argv[0]);
+                             "[--method 2|3] [--snap1 S] [--snap2 S] [--progress S] [--restart FILE] [--out DIR]\n", argv[0]);
         return 2;
     }
     const int N = o.nx, M = o.ny;
@@ -109,23 +110,30 @@ int main(int argc, char** argv) {
     g.r_frozen = kDdist;
     g.axisym = 1;
 
-    // initial state (kernel.cu:3901-3941)
+    // initial state: the stellar wind inside 1.01 ddist and the quiescent
+    // medium outside (kernel.cu:3901-3941); or, with --restart, a text state
+    // with the wind imposed again inside 1.001 ddist (the reference's restart
+    // branch, kernel.cu:3944-3996)
     std::vector<gd2_double2> s(K), u(K);
     std::vector<gd2_double3> b(K);
     const double gam = g.gamma;
     const double V_E = kPhi0, ro_E = 1.0 / (kPhi0 * kPhi0 * kRr0 * kRr0);
     const double P_E = ro_E * V_E * V_E / (gam * kM0 * kM0), B_E = std::sqrt(4.0 * kPi) / (kMalf * kRr0);
+    if (!o.restart.empty()) GD2CHK(gd2_read_state_text(&g, o.restart.c_str(), s.data(), u.data(), b.data()));
+    const double r_wind = kDdist * (o.restart.empty() ? 1.01 : 1.001);
     for (long long k = 0; k < K; ++k) {
         const int n = (int)(k % N), m = (int)(k / N);
         const double y = g.y0 + m * g.dy, x = g.x0 + n * g.dx;
         const double dist = std::sqrt(x * x + y * y);
-        if (dist < kDdist * 1.01) {
+        if (dist < r_wind) {
             s[k] = {ro_E / std::pow(dist / kRr0, 2.0), P_E * std::pow(kRr0 / dist, 2.0 * gam)};
             u[k] = {V_E * x / dist, V_E * y / dist};
+            // B_E (rr_0 / r) sin(theta) in the azimuthal direction, whose
+            // Cartesian image at phi = 0 is the third component (0 on the axis)
             const double BE = B_E / (dist / kRr0);
             const double bphi = std::fabs(y) < 0.000001 ? 0.0 : BE * std::sin(std::acos(x / dist));
             b[k] = {0.0, 0.0, bphi};
-        } else {
+        } else if (o.restart.empty()) {
             s[k] = {1.0, 1.0 / gam};
             u[k] = {0.0, 0.0};
             b[k] = {0.0, 0.0, 0.0};

@@ -133,3 +133,15 @@ def test_production_driver_links_and_rejects_bad_arguments():
     for args in (["--bogus", "1"], ["--nx", "1"], ["--steps1"]):
         r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage:" in r.stderr, (args, r.returncode, r.stderr)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(api.SO_PATH), "gd2_production")),
+                    reason="gd2_production not built")
+def test_production_driver_missing_restart_file(tmp_path):
+    """A restart file that cannot be read stops the driver with status 1 and
+    the failing call named, before any device call."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(api.SO_PATH), "gd2_production")
+    r = subprocess.run([exe, "--nx", "8", "--ny", "8", "--restart", str(tmp_path / "missing.txt"),
+                        "--out", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "gd2_read_state_text" in r.stderr, (r.returncode, r.stderr)

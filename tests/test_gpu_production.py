@@ -3,7 +3,7 @@ main step loop over the C ABI, kernel.cu:3810-4469) against the oracle's
 loop of the same steps.
 
 The driver runs as a child process. At a tenth of the production size it
-advances 10 add2 and 10 Kernel_TVD steps (HLLC-MHD), each followed by
+advances 10 add2 and 10 Kernel_TVD steps (HLLC-MHD and HLLD), each followed by
 funk_time, snapshotting every 4 steps of either phase. Its save_b3.dat must
 match the oracle's loop to 1e-10, like the in-process 20-step wind test. Its
 printed text files must match the oracle's writers on the oracle's state to
@@ -91,8 +91,10 @@ def test_driver_initial_state(tmp_path):
     np.testing.assert_allclose(a[..., 6], b[..., 2], rtol=1e-14, atol=1e-15)
 
 
-def test_driver_run_matches_oracle_loop(tmp_path):
-    out = run_driver(tmp_path, "--steps1", "10", "--steps2", "10", "--snap1", "4", "--snap2", "4")
+@pytest.mark.parametrize("method", [2, 3])
+def test_driver_run_matches_oracle_loop(tmp_path, method):
+    out = run_driver(tmp_path, "--steps1", "10", "--steps2", "10", "--snap1", "4", "--snap2", "4",
+                     "--method", str(method))
     og = oracle_grid()
     s, u, b = PB.stellar_wind_prim(og)
     T, Tdo, TT, it = 1e7, 1e-8, 0.0, 0
@@ -101,7 +103,7 @@ def test_driver_run_matches_oracle_loop(tmp_path):
     tt_at = {}
     for k in range(20):
         kind = "add2" if k < 10 else "kernel_tvd"
-        s, u, b, T = O.ref_mhd_step(kind, og, s, u, b, T, Tdo, it, 2)
+        s, u, b, T = O.ref_mhd_step(kind, og, s, u, b, T, Tdo, it, method)
         T, Tdo, TT, it = O.funk_time(T, Tdo, TT, it)
         # the pass with counter i ends after step i + 2 of its phase
         i = (k % 10) - 1
@@ -126,3 +128,38 @@ def test_driver_run_matches_oracle_loop(tmp_path):
     O.write_line_cuts(og, s, u, b, str(ref / "param_y=0.txt"), str(ref / "param_x=0.txt"))
     for name in ["instable_for_inner_b3.txt", "param_for_texplot_mini.txt", "param_y=0.txt", "param_x=0.txt"]:
         same_table(tmp_path / name, ref / name, rtol=1e-4)
+
+
+def test_driver_restart_reimposes_wind(tmp_path):
+    """--restart: the text state is read back (6 significant digits) and the
+    wind imposed again inside 1.001 ddist (kernel.cu:3944-3996); with no steps
+    save_b3.dat holds exactly that."""
+    og = oracle_grid()
+    s, u, b = PB.stellar_wind_prim(og)
+    T, Tdo, TT, it = 1e7, 1e-8, 0.0, 0
+    for _ in range(4):
+        s, u, b, T = O.ref_mhd_step("add2", og, s, u, b, T, Tdo, it, 2)
+        T, Tdo, TT, it = O.funk_time(T, Tdo, TT, it)
+    path = str(tmp_path / "instable_cp14.txt")
+    O.write_state_text(og, s, u, b, path)
+    rs, ru, rb = O.read_state_text(og, path)
+    run_driver(tmp_path, "--steps1", "0", "--steps2", "0", "--restart", path)
+    ws, wu, wb = PB.stellar_wind_prim(og)
+    x = og.x0 + np.arange(NX) * og.dx
+    y = og.y0 + np.arange(NY) * og.dy
+    X, Y = np.meshgrid(x, y)
+    inner = np.sqrt(X * X + Y * Y) < PB.WIND_DDIST * 1.001
+    assert inner.any() and not inner.all()
+    es = np.where(inner[..., None], ws, rs)
+    eu = np.where(inner[..., None], wu, ru)
+    eb = np.where(inner, wb[..., 2], rb[..., 2])
+    a = read_b3(tmp_path / "save_b3.dat")
+    np.testing.assert_array_equal(a[~inner][:, 2:4], es[~inner])
+    np.testing.assert_array_equal(a[~inner][:, 4:6], eu[~inner])
+    np.testing.assert_array_equal(a[~inner][:, 6], eb[~inner])
+    np.testing.assert_allclose(a[inner][:, 2:4], es[inner], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(a[inner][:, 4:6], eu[inner], rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(a[inner][:, 6], eb[inner], rtol=1e-14, atol=1e-15)
+    # the ring 1.001 ddist <= r < 1.01 ddist keeps the file's (evolved) values
+    ring = (np.sqrt(X * X + Y * Y) < PB.WIND_DDIST * 1.01) & ~inner
+    assert ring.any()
