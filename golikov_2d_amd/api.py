@@ -105,6 +105,8 @@ def lib():
             "gd2_ensemble_time": (ip, [vp, ip, C.POINTER(dp), C.POINTER(C.c_longlong), C.POINTER(dp), vp]),
             "gd2_ensemble_tune": (ip, [vp, vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]),
             "gd2_ctx_log_dt": (ip, [vp, vp, C.c_longlong]),
+            "gd2_ctx_time_kernels": (ip, [vp, ip, ip]),
+            "gd2_ctx_kernel_ms": (ip, [vp, C.POINTER(dp), C.POINTER(ip)]),
             "gd2_vjp_workspace_bytes": (C.c_longlong, [gp, sp]),
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
             "gd2_set_step_path": (ip, [ip]),
@@ -130,7 +132,7 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
-            "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
+            "gd2_ctx_time_kernels", "gd2_ctx_kernel_ms", "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
             "gd2_one_pass_preferred", "gd2_write_state_text", "gd2_read_state_text", "gd2_write_state_b3",
             "gd2_write_tecplot", "gd2_write_tecplot_derived", "gd2_write_line_cuts"]
 
@@ -327,6 +329,17 @@ class Solver:
         else:
             _check(lib().gd2_ctx_log_dt(self._h, _f64(dt_log), dt_log.numel()), "gd2_ctx_log_dt")
         self._dt_log = dt_log
+
+    def time_kernels(self, capacity, stride=1):
+        """HIP events around the kernel launches of every `stride`-th following step until
+        `capacity` steps are timed (0: off)."""
+        _check(lib().gd2_ctx_time_kernels(self._h, int(capacity), int(stride)), "gd2_ctx_time_kernels")
+
+    def kernel_ms(self):
+        """(mean ms per timed step's kernels, number of steps timed) since the last call; waits for them."""
+        ms, n = C.c_double(), C.c_int()
+        _check(lib().gd2_ctx_kernel_ms(self._h, C.byref(ms), C.byref(n)), "gd2_ctx_kernel_ms")
+        return ms.value, n.value
 
     def run_to(self, U, t_end, chunk=32, max_steps=10 ** 7):
         """Advance until t_end, checking the device clock every `chunk` steps."""

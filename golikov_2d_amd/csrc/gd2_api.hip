@@ -365,7 +365,19 @@ struct gd2_ctx {
     long long host_steps;  // steps enqueued (parity of the wave-speed slots)
     double* dt_log;        // optional device log of each step's dt
     long long dt_cap;
+    // optional per-step kernel timing: a pair of events around each step's
+    // kernel launches, for the first ev_cap steps after gd2_ctx_time_kernels
+    hipEvent_t* ev = nullptr;
+    int ev_cap = 0, ev_n = 0, ev_stride = 1;
+    long long ev_seen = 0;  // steps enqueued since timing was switched on
 };
+
+static void drop_events(gd2_ctx* c) {
+    for (int i = 0; i < 2 * c->ev_cap; ++i) (void)hipEventDestroy(c->ev[i]);
+    delete[] c->ev;
+    c->ev = nullptr;
+    c->ev_cap = c->ev_n = 0;
+}
 
 extern "C" {
 
@@ -556,6 +568,43 @@ int gd2_ctx_create(const gd2_grid* g, const gd2_scheme* sc, gd2_ctx** out) {
     return GD2_OK;
 }
 
+int gd2_ctx_time_kernels(gd2_ctx* c, int capacity, int stride) {
+    if (!c || capacity < 0 || stride < 1) return GD2_ERR_ARG;
+    drop_events(c);
+    if (capacity == 0) return GD2_OK;
+    c->ev = new hipEvent_t[2 * (size_t)capacity];
+    for (int i = 0; i < 2 * capacity; ++i) {
+        const hipError_t e = hipEventCreate(&c->ev[i]);
+        if (e != hipSuccess) {
+            for (int k = 0; k < i; ++k) (void)hipEventDestroy(c->ev[k]);
+            delete[] c->ev;
+            c->ev = nullptr;
+            g_last_hip = (int)e;
+            return GD2_ERR_HIP;
+        }
+    }
+    c->ev_cap = capacity;
+    c->ev_n = 0;
+    c->ev_stride = stride;
+    c->ev_seen = 0;
+    return GD2_OK;
+}
+
+int gd2_ctx_kernel_ms(gd2_ctx* c, double* mean_ms, int* count) {
+    if (!c || !mean_ms) return GD2_ERR_ARG;
+    double sum = 0.0;
+    for (int i = 0; i < c->ev_n; ++i) {
+        float ms = 0.0f;
+        GD2_HIP(hipEventSynchronize(c->ev[2 * i + 1]));
+        GD2_HIP(hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
+        sum += ms;
+    }
+    *mean_ms = c->ev_n > 0 ? sum / c->ev_n : 0.0;
+    if (count) *count = c->ev_n;
+    c->ev_n = 0;
+    return GD2_OK;
+}
+
 int gd2_ctx_log_dt(gd2_ctx* c, double* dt_log, long long capacity) {
     if (!c || capacity < 0 || (capacity > 0 && !dt_log)) return GD2_ERR_ARG;
     c->dt_log = capacity > 0 ? dt_log : nullptr;
@@ -646,6 +695,7 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
 int gd2_ctx_destroy(gd2_ctx* c) {
     if (!c) return GD2_ERR_ARG;
     (void)hipDeviceSynchronize();
+    drop_events(c);
     (void)hipFree(c->scratch);
     (void)hipFree(c->clock);
     delete c;
@@ -710,6 +760,8 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
         hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log, c->dt_cap);
         GD2_HIP(hipGetLastError());
         unsigned long long* wn = &c->clock->w[slot ^ 1];
+        const bool timed = c->ev_n < c->ev_cap && c->ev_seen++ % c->ev_stride == 0;
+        if (timed) GD2_HIP(hipEventRecord(c->ev[2 * c->ev_n], st));
         int rc;
         if (c->sc.integrator == GD2_SSPRK2 && one_pass) {
             // one pass per step, ping-ponging between U and the scratch state
@@ -727,6 +779,10 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
             rc = gd2_stage(&c->g, &c->sc, cur, nullptr, nxt, 0.0, 1.0, 0.0, dtp, wn, stream);
             if (rc) return rc;
             cur = nxt;
+        }
+        if (timed) {
+            GD2_HIP(hipEventRecord(c->ev[2 * c->ev_n + 1], st));
+            c->ev_n += 1;
         }
         c->host_steps += 1;
     }

@@ -5,8 +5,8 @@
  * Plain C: pointers, sizes and an opaque stream handle (a hipStream_t passed as
  * void*; NULL = the default stream). Every device pointer is caller-owned
  * device memory; no entry point allocates except gd2_ctx_create. Entry points
- * are stream-ordered and never synchronise the host, except gd2_ctx_time and
- * gd2_get_diagnostics, which say so.
+ * are stream-ordered and never synchronise the host, except gd2_ctx_time,
+ * gd2_ctx_kernel_ms and gd2_get_diagnostics, which say so.
  *
  * Return value: GD2_OK (0) or a negative gd2_status.
  *
@@ -235,6 +235,19 @@ int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, dou
 /* Record each taken step's dt into dt_log[step] (device, capacity entries);
  * capacity 0 switches the log off. */
 int gd2_ctx_log_dt(gd2_ctx* ctx, double* dt_log, long long capacity);
+
+/* Kernel timing (no reference counterpart; the reference times whole phases
+ * with events around its loop, kernel.cu:4087-4091): gd2_ctx_advance records
+ * a pair of HIP events around the kernel launches of every stride-th step
+ * (stride >= 1; the first step after this call included) until `capacity`
+ * steps are timed: the one-pass step kernel, or the step's stage kernels, not
+ * its one-thread clock kernel. capacity 0 switches it off. Allocates
+ * host-side events only. */
+int gd2_ctx_time_kernels(gd2_ctx* ctx, int capacity, int stride);
+/* Mean duration in ms of the steps timed since the last call (waits for the
+ * last of them: synchronises with the stream), their number in *count (may
+ * be NULL); starts the next count. */
+int gd2_ctx_kernel_ms(gd2_ctx* ctx, double* mean_ms, int* count);
 
 /* ----------------------------------------------------------------------
  * Discrete adjoint of the stage operator: Euler with the HLLC flux, or MHD

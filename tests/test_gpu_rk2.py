@@ -143,6 +143,42 @@ def test_solver_context_one_pass_equals_two_pass(nsteps):
 
 
 @pytest.mark.parametrize("path", ["one_pass", "two_pass"])
+def test_solver_kernel_timing_changes_nothing(path):
+    """gd2_ctx_time_kernels / gd2_ctx_kernel_ms: events around every
+    stride-th step's kernels up to `capacity` steps; the state and the clock
+    are bitwise those of an untimed run; the count restarts after each read."""
+    say(f"ctx kernel timing, {path}")
+    g, og = grids(300, 200, BCS["transmissive"])
+    sc = api.make_scheme()
+    U0 = to_dev(PB.four_quadrant(og, seed=6))
+    prev = api.set_step_path(api.STEP_ONE_PASS if path == "one_pass" else api.STEP_TWO_PASS)
+    try:
+        runs = []
+        for timed in (False, True):
+            s = api.Solver(g, sc)
+            U = U0.clone()
+            s.start(U)
+            if timed:
+                s.time_kernels(3, stride=3)
+            s.advance(U, 8)
+            if timed:
+                ms, n = s.kernel_ms()
+                assert n == 3 and ms > 0.0  # steps 0, 3, 6 of 8
+                assert s.kernel_ms() == (0.0, 0)
+                s.time_kernels(0)
+                s.advance(U, 2)
+                assert s.kernel_ms() == (0.0, 0)
+            else:
+                s.advance(U, 2)
+            runs.append((U, s.time()))
+            s.close()
+    finally:
+        api.set_step_path(prev)
+    assert torch.equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+
+
+@pytest.mark.parametrize("path", ["one_pass", "two_pass"])
 def test_16384_square_rows_equal_one_row_solution(path):
     """64-bit indexing at 16384^2 (8.6 GB per state, plane offsets beyond
     2^32 bytes): an x-directed state, constant along y, stays so under
