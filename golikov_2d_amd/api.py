@@ -109,6 +109,7 @@ def lib():
             "gd2_ctx_kernel_ms": (ip, [vp, C.POINTER(dp), C.POINTER(ip)]),
             "gd2_vjp_workspace_bytes": (C.c_longlong, [gp, sp]),
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
+            "gd2_stage_vjp_noalloc": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp]),
             "gd2_set_step_path": (ip, [ip]),
             "gd2_one_pass_preferred": (ip, [gp, sp]),
             "gd2_write_state_text": (ip, [gp, vp, vp, vp, C.c_char_p]),
@@ -132,7 +133,7 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
-            "gd2_ctx_time_kernels", "gd2_ctx_kernel_ms", "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_ssprk2_step", "gd2_set_step_path",
+            "gd2_ctx_time_kernels", "gd2_ctx_kernel_ms", "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_stage_vjp_noalloc", "gd2_ssprk2_step", "gd2_set_step_path",
             "gd2_one_pass_preferred", "gd2_write_state_text", "gd2_read_state_text", "gd2_write_state_b3",
             "gd2_write_tecplot", "gd2_write_tecplot_derived", "gd2_write_line_cuts"]
 
@@ -265,6 +266,21 @@ def stage_vjp(g, sc, U, V, W, c=1.0, A=None, a=0.0, B=None, b=0.0, workspace=Non
         raise GD2Error("workspace smaller than gd2_vjp_workspace_bytes")
     _check(lib().gd2_stage_vjp(C.byref(g), C.byref(sc), _f64(U), _f64(V), _f64(A), float(a), _f64(B), float(b),
                                float(c), _f64(W), _p(workspace), _stream(stream)), "gd2_stage_vjp")
+
+
+ERR_TOO_LARGE = -5
+NOALLOC_SCRATCH_BYTES = 32 << 20  # GD2_NOALLOC_SCRATCH_BYTES
+
+
+def stage_vjp_noalloc(g, sc, U, V, W, c=1.0, A=None, a=0.0, B=None, b=0.0, stream=None):
+    """stage_vjp through gd2_stage_vjp_noalloc: no workspace, no device
+    allocation (the library's fixed per-device scratch array is used)."""
+    nf = 7 if sc.physics == MHD else 4
+    for t in (U, V, W, A, B):
+        if t is not None and tuple(t.shape) != (nf, g.ny, g.nx):
+            raise GD2Error(f"state shape {tuple(t.shape)} is not ({nf}, {g.ny}, {g.nx})")
+    _check(lib().gd2_stage_vjp_noalloc(C.byref(g), C.byref(sc), _f64(U), _f64(V), _f64(A), float(a), _f64(B),
+                                       float(b), float(c), _f64(W), _stream(stream)), "gd2_stage_vjp_noalloc")
 
 
 def bits_to_double(t):

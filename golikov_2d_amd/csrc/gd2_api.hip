@@ -25,6 +25,14 @@ namespace gd2 {
 
 __device__ DeviceDiag g_diag;
 
+// Fixed device scratch of the allocation-free adjoint entry point
+// (gd2_stage_vjp_noalloc): GD2_NOALLOC_SCRATCH_BYTES per device, held by
+// every device the library runs on until the process ends. Host code never
+// passes this name to a kernel (on the host it is a stand-in, not a device
+// address); it uses the address hipGetSymbolAddress returns for the current
+// device (noalloc_scratch below).
+static __device__ __align__(256) double scratch[GD2_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
 static thread_local int g_last_hip = 0;
 
 static int hip_fail(hipError_t e) {
@@ -37,6 +45,31 @@ static int hip_fail(hipError_t e) {
         hipError_t e_ = (call);                       \
         if (e_ != hipSuccess) return hip_fail(e_);    \
     } while (0)
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current and kept per device (each device holds its
+// own copy of the array at its own address). Null when the lookup fails.
+static void* noalloc_scratch() {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        hip_fail(e);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol);
+        return nullptr;
+    }
+    addr[dev] = p;
+    return p;
+}
 
 static DeviceDiag* diag_ptr() {
     static DeviceDiag* p = nullptr;
@@ -690,6 +723,24 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
         GD2_HIP(hipGetLastError());
     }
     return GD2_OK;
+}
+
+int gd2_stage_vjp_noalloc(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V,
+                          const double* A, double a, const double* B, double b, double c, double* W,
+                          void* stream) {
+    // the same checks as gd2_stage_vjp, then the size limit, all before any HIP call
+    if (!grid_ok(g, false) || !scheme_ok(sc) || !U || !V || !W) return GD2_ERR_ARG;
+    if (W == U || W == V) return GD2_ERR_ARG;
+    if (sc->physics == GD2_EULER && sc->flux != GD2_FLUX_HLLC) return GD2_ERR_UNSUPPORTED;
+    const long long need = gd2_vjp_workspace_bytes(g, sc);
+    if (need < 0) return GD2_ERR_ARG;
+    if (need > GD2_NOALLOC_SCRATCH_BYTES) return GD2_ERR_TOO_LARGE;
+    // every workspace entry the kernels read is written earlier in the same
+    // call (ghost cotangents, face-state cotangents), so the contents left by
+    // the last call need no clearing
+    void* ws = noalloc_scratch();
+    if (!ws) return GD2_ERR_HIP;
+    return gd2_stage_vjp(g, sc, U, V, A, a, B, b, c, W, ws, stream);
 }
 
 int gd2_ctx_destroy(gd2_ctx* c) {

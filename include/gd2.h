@@ -35,8 +35,13 @@ enum gd2_status {
     GD2_ERR_ARG = -1,          /* bad pointer, size or enum value */
     GD2_ERR_UNSUPPORTED = -2,  /* method / flux / boundary kind not offered by this entry point */
     GD2_ERR_HIP = -3,          /* a HIP runtime call failed (see gd2_last_hip_error) */
-    GD2_ERR_NOMEM = -4
+    GD2_ERR_NOMEM = -4,
+    GD2_ERR_TOO_LARGE = -5     /* needs more than the fixed scratch of an allocation-free entry point */
 };
+
+/* Size of the fixed device scratch array of the allocation-free entry point
+ * gd2_stage_vjp_noalloc (per device, held until the process ends). */
+#define GD2_NOALLOC_SCRATCH_BYTES (32ll << 20)
 
 /* Boundary kind of one side. The reference's transmissive sides copy the edge
  * cell (kernel.cu:1762-1793); its "hard" sides keep the edge cells frozen
@@ -268,6 +273,22 @@ long long gd2_vjp_workspace_bytes(const gd2_grid* g, const gd2_scheme* sc);
 int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V,
                   const double* A, double a, const double* B, double b, double c, double* W,
                   void* workspace, void* stream);
+
+/* gd2_stage_vjp without the workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. It calls
+ * no allocating routine: the workspace is a fixed device array of
+ * GD2_NOALLOC_SCRATCH_BYTES bytes inside the library (one per device, held
+ * until the process ends), looked up on the first call on each device.
+ * Returns GD2_ERR_TOO_LARGE when gd2_vjp_workspace_bytes(g, sc) exceeds it:
+ * Euler up to max(nx, ny) = 131072, MHD up to about 150 000 cells. Every call
+ * on one device shares that device's array, so calls that overlap on one
+ * device (concurrent streams or host threads) race; calls in one stream are
+ * safe, since each call writes every entry it reads. Results are those of
+ * gd2_stage_vjp. */
+int gd2_stage_vjp_noalloc(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V,
+                          const double* A, double a, const double* B, double b, double c, double* W,
+                          void* stream);
 
 /* Layout conversion between the reference AoS primitives and SoA conserved. */
 int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,
