@@ -310,6 +310,207 @@ def test_hlld_normal_field_dissipation():
     assert abs(P[4] - (-SN * (0.9 - 0.5) / SPI4) * SPI4) < 1e-14
 
 
+def _to_face(Q, n2):
+    """lab state (rho, p, vx, vy, vz, Bx, By, Bz), Gaussian B -> face-frame
+    (rho, p, un, vt, wm, Bn, Bt, Bm), Heaviside-Lorentz B"""
+    n = (n2[0], n2[1], 0.0)
+    t, m = _frame(n2)
+    dot = lambda a, b: a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+    v, b = Q[2:5], [x / SPI4 for x in Q[5:8]]
+    return (Q[0], Q[1], dot(v, n), dot(v, t), dot(v, m), dot(b, n), dot(b, t), dot(b, m))
+
+
+def _to_lab(F8, n2):
+    """face-frame flux (rho, mn, mt, mm, Bn, Bt, Bm, E), Heaviside-Lorentz ->
+    the oracle's P = (rho, mx, my, mz, Bx, By, Bz, E) with Gaussian B fluxes"""
+    n = (n2[0], n2[1], 0.0)
+    t, m = _frame(n2)
+    mom = [F8[1] * n[i] + F8[2] * t[i] + F8[3] * m[i] for i in range(3)]
+    bfl = [SPI4 * (F8[4] * n[i] + F8[5] * t[i] + F8[6] * m[i]) for i in range(3)]
+    return np.array([F8[0], *mom, *bfl, F8[7]])
+
+
+def _textbook_mhd(gamma, L, R, n2, metod, **kw):
+    """The textbook flux of the oracle's metod in the oracle's lab layout;
+    the normal-field component carries the reference's dissipation
+    -max(|SL|, |SR|)(Bn_R - Bn_L), pinned on its own above."""
+    ql, qr = _to_face(L, n2), _to_face(R, n2)
+    if metod == 2:
+        F8, (SL, SR) = TB.li_hllc(gamma, ql, qr)
+        F8 = list(F8)
+    else:
+        F7, (SL, SR) = TB.mk_hlld(gamma, ql, qr, **kw)
+        F8 = [F7[0], F7[1], F7[2], F7[3], 0.0, F7[4], F7[5], F7[6]]
+    F8[4] = -max(abs(SL), abs(SR)) * (qr[5] - ql[5])
+    return _to_lab(F8, n2)
+
+
+def _rand_pair(rng, n2, same_bn):
+    rL, pL, vL, BL = _rand_mhd(rng)
+    rR, pR, vR, BR = _rand_mhd(rng)
+    if same_bn:
+        k = 0 if n2[0] != 0 else 1
+        BR[k] = BL[k]
+    return (rL, pL, vL[0], vL[1], 0.0, *BL), (rR, pR, vR[0], vR[1], 0.0, *BR)
+
+
+# the reference's degeneracy thresholds (Gas_Dinamic.cu:1311, 1327, 1358),
+# pinned on their own by the two threshold tests below
+REF_THRESHOLDS = dict(deg_abs=1e-5, alfven_abs=1e-6)
+
+
+def test_hllc_mhd_matches_li_hllc_c():
+    """metod 2 (the production run's flux) against Li's HLLC-C with a
+    continuous normal field and B != 0: all 8 components (tangential momenta
+    and the B fluxes included), both gammas, all four normals."""
+    rng = random.Random(21)
+    worst = 0.0
+    for gamma in (5.0 / 3.0, 1.4):
+        for i in range(2400):
+            n2 = NORMALS[i % 4]
+            L, R = _rand_pair(rng, n2, True)
+            P, _ = O.mhd_flux(gamma, 0.25, L, R, (n2[0], n2[1], 0.0), 2)
+            exp = _textbook_mhd(gamma, L, R, n2, 2)
+            worst = max(worst, rel(P, exp))
+    assert worst < 1e-11, worst
+
+
+@pytest.mark.parametrize("metod", [2, 3])
+def test_mhd_fluxes_with_jumping_normal_field_use_the_hll_normal_field(metod):
+    """bn_L != bn_R (every MUSCL face of a 2-D run): both fluxes against the
+    textbook solvers whose star normal field is the HLL average, the value the
+    integral form of the normal-field equation fixes."""
+    rng = random.Random(22 + metod)
+    worst = 0.0
+    for gamma in (5.0 / 3.0, 1.4):
+        for i in range(1600):
+            n2 = NORMALS[i % 4]
+            L, R = _rand_pair(rng, n2, False)
+            P, _ = O.mhd_flux(gamma, 0.25, L, R, (n2[0], n2[1], 0.0), metod)
+            exp = _textbook_mhd(gamma, L, R, n2, metod, **({} if metod == 2 else REF_THRESHOLDS))
+            worst = max(worst, rel(P, exp))
+    assert worst < 1e-11, worst
+
+
+def test_hlld_matches_miyoshi_kusano_on_strong_fields():
+    """metod 3 with a continuous normal field on states where the field
+    dominates (|B|^2 up to 50 p): every region of the five-wave fan."""
+    rng = random.Random(24)
+    worst, regions = 0.0, set()
+    for i in range(3000):
+        n2 = NORMALS[i % 4]
+        L, R = _rand_pair(rng, n2, True)
+        L = (L[0], 0.1 * L[1], *L[2:5], *[3.0 * b for b in L[5:8]])
+        R = (R[0], 0.1 * R[1], *R[2:5], *[3.0 * b for b in R[5:8]])
+        if n2[0] != 0:
+            R = (*R[:5], L[5], *R[6:])
+        else:
+            R = (*R[:6], L[6], R[7])
+        if i % 7 == 0:  # a supersonic pair moving along +-n
+            k, sh = (2 if n2[0] != 0 else 3), (12.0 if i % 14 == 0 else -12.0)
+            L = tuple(v + sh if j == k else v for j, v in enumerate(L))
+            R = tuple(v + sh if j == k else v for j, v in enumerate(R))
+        P, _ = O.mhd_flux(5 / 3, 0.25, L, R, (n2[0], n2[1], 0.0), 3)
+        exp = _textbook_mhd(5 / 3, L, R, n2, 3, **REF_THRESHOLDS)
+        worst = max(worst, rel(P, exp))
+        # which region x/t = 0 falls in
+        ql, qr = _to_face(L, n2), _to_face(R, n2)
+        A, B = TB._mhd_side(5 / 3, ql), TB._mhd_side(5 / 3, qr)
+        SL, SR = TB._outer_speeds(A, B)
+        SM = TB._contact_speed(A, B, SL, SR)
+        bx = abs(ql[5])
+        rsL, rsR = A["r"] * (SL - A["u"]) / (SL - SM), B["r"] * (SR - B["u"]) / (SR - SM)
+        edges = [SL, SM - bx / math.sqrt(rsL), SM, SM + bx / math.sqrt(rsR), SR]
+        regions.add(sum(1 for e in edges if e < 0))
+    assert worst < 1e-11, worst
+    assert regions == {0, 1, 2, 3, 4, 5}, regions
+
+
+@pytest.mark.parametrize("metod", [2, 3])
+def test_mhd_flux_mirror_invariance(metod):
+    """F(L, R, n) = -F(R, L, -n) in the lab frame: the face seen from the
+    other cell (the reference evaluates every face from both cells), with
+    jumping normal field."""
+    rng = random.Random(25 + metod)
+    for i in range(1500):
+        n2 = NORMALS[i % 4]
+        L, R = _rand_pair(rng, n2, i % 3 == 0)
+        P, t = O.mhd_flux(5 / 3, 0.25, L, R, (n2[0], n2[1], 0.0), metod)
+        Pm, tm = O.mhd_flux(5 / 3, 0.25, R, L, (-n2[0], -n2[1], 0.0), metod)
+        assert rel(Pm, -P, 1e-12) < 1e-12, (i, P, Pm)
+        assert abs(t - tm) <= 1e-15 * t
+
+
+@pytest.mark.parametrize("metod", [2, 3])
+def test_mhd_flux_rotation_invariance(metod):
+    """A quarter turn about z of both states and the normal turns the flux:
+    F(rot L, rot R, rot n) = rot F(L, R, n), with jumping normal field. The
+    four normals use four different face frames (Gas_Dinamic.cu:978-1019)."""
+    rot3 = lambda a: (-a[1], a[0], a[2])
+    rot_state = lambda Q: (Q[0], Q[1], *rot3(Q[2:5]), *rot3(Q[5:8]))
+    rng = random.Random(27 + metod)
+    for i in range(1500):
+        n2 = NORMALS[i % 4]
+        L, R = _rand_pair(rng, n2, i % 3 == 0)
+        P, t = O.mhd_flux(5 / 3, 0.25, L, R, (n2[0], n2[1], 0.0), metod)
+        rn = rot3((n2[0], n2[1], 0.0))
+        Pr, tr = O.mhd_flux(5 / 3, 0.25, rot_state(L), rot_state(R), rn, metod)
+        exp = np.array([P[0], *rot3(P[1:4]), *rot3(P[4:7]), P[7]])
+        assert rel(Pr, exp, 1e-12) < 1e-11, (i, Pr, exp)
+        assert abs(t - tr) <= 1e-14 * t
+
+
+def _degenerate_pair(bt, un=0.0):
+    """Face-x states with bn^2 > gamma p and a weak tangential field +/-bt:
+    rho_L (S_L - u_L)(S_L - S_M) - bn^2 ~ bt^2 bn^2 / (bn^2 - gamma p) -> 0,
+    the degenerate case of the HLLD star state (the fast speed meets the
+    Alfven speed). Mirror-symmetric speeds make S_M = 0 exactly."""
+    bn = 1.5 * SPI4  # Gaussian; 1.5 in Heaviside-Lorentz units, gamma p = 1
+    L = (1.0, 0.6, un, 0.2, 0.0, bn, bt * SPI4, 0.0)
+    R = (1.0, 0.6, -un, -0.3, 0.0, bn, -bt * SPI4, 0.0)
+    return L, R
+
+
+def _den_left(L, R, gamma=5 / 3):
+    A, B = TB._mhd_side(gamma, _to_face(L, (1.0, 0.0))), TB._mhd_side(gamma, _to_face(R, (1.0, 0.0)))
+    SL, SR = TB._outer_speeds(A, B)
+    SM = TB._contact_speed(A, B, SL, SR)
+    bx = (SR * B["bn"] - SL * A["bn"]) / (SR - SL)
+    return A["r"] * (SL - A["u"]) * (SL - SM) - bx * bx
+
+
+def test_hlld_degenerate_star_state_threshold():
+    """|rho (S - u)(S - S_M) - bn^2| < 1e-5 takes the degenerate star state
+    (v* = v, B*_t = 0; Gas_Dinamic.cu:1311-1340, M-K sec. 4.3), from 1e-5 up
+    the regular one; the two differ by O(bt) here, so the side of the
+    threshold shows in the flux."""
+    n = (1.0, 0.0, 0.0)
+    for bt, degenerate in ((2.0e-3, True), (5.0e-3, False)):
+        L, R = _degenerate_pair(bt)
+        den = _den_left(L, R)
+        assert (abs(den) < 1e-5) == degenerate and abs(den) > 1e-7, den
+        P, _ = O.mhd_flux(5 / 3, 0.25, L, R, n, 3)
+        regular = _textbook_mhd(5 / 3, L, R, (1.0, 0.0), 3)
+        degen = _textbook_mhd(5 / 3, L, R, (1.0, 0.0), 3, deg_abs=1.0)
+        assert rel(regular, degen) > 1e-5  # the branches are far apart here
+        assert rel(P, degen if degenerate else regular) < 1e-11
+
+
+def test_hlld_alfven_waves_merge_below_1e6():
+    """|bn| <= 1e-6 (Heaviside-Lorentz) merges the Alfven waves with the
+    contact (Gas_Dinamic.cu:1358-1369); above, x/t = 0 at S_M = 0 lies in
+    the double-star region. The flux then differs by ~|bn| / sqrt(rho*)."""
+    n = (1.0, 0.0, 0.0)
+    for bn_hl, merged in ((0.5e-6, True), (2.0e-6, False), (-2.0e-6, False), (1.0e-6, True)):
+        L = (0.8, 0.5, 0.3, 0.4, 0.0, bn_hl * SPI4, 1.0 * SPI4, 0.3 * SPI4)
+        R = (0.8, 0.5, -0.3, -0.2, 0.0, bn_hl * SPI4, -1.0 * SPI4, 0.3 * SPI4)
+        P, _ = O.mhd_flux(5 / 3, 0.25, L, R, n, 3)
+        full = _textbook_mhd(5 / 3, L, R, (1.0, 0.0), 3, deg_abs=1e-5, alfven_abs=0.0)
+        merge = _textbook_mhd(5 / 3, L, R, (1.0, 0.0), 3, deg_abs=1e-5, alfven_abs=1.0)
+        assert rel(full, merge) > 1e-8
+        assert rel(P, merge if merged else full) < 1e-11
+
+
 def test_hlld_rejects_unknown_metod():
     O.diag_reset()
     P, t = O.mhd_flux(5 / 3, 0.25, (1, 1, 0, 0, 0, 0, 0, 0), (1, 1, 0, 0, 0, 0, 0, 0), (1.0, 0, 0), 1)

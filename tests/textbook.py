@@ -7,7 +7,18 @@ These follow published methods, not the reference source:
   * HLLC: Toro ch. 10.4, star state eq. (10.39), wave speeds from the Davis
     estimate S_L = min(uL - cL, uR - cR), S_R = max(uL + cL, uR + cR);
   * HLLD: Miyoshi & Kusano, J. Comput. Phys. 208 (2005) 315, eqs. (38)-(63),
-    Heaviside-Lorentz units, constant normal field;
+    Heaviside-Lorentz units;
+  * HLLC for MHD: Li, J. Comput. Phys. 203 (2005) 344 ("HLLC-C"): HLLC
+    with the contact speed S_M and star total pressure of the Euler HLLC,
+    and the HLL averages for the star magnetic field, the star tangential
+    velocities and the star (v.B);
+  * both MHD solvers take the outer wave speeds S_L = min(uL, uR) -
+    max(cfL, cfR), S_R = max(uL, uR) + max(cfL, cfR) (fast magnetosonic
+    speeds). Where the two states' normal fields differ (a multi-dimensional
+    scheme with cell-centred B), the star normal field is the HLL average
+    (S_R BnR - S_L BnL) / (S_R - S_L), the value consistent with the integral
+    form of the normal-field equation, whose physical flux is zero; the outer
+    states keep their own normal field in their physical fluxes;
   * HLL: Harten, Lax & van Leer (SIAM Rev. 25, 1983) as in Toro eq. (10.21),
     with the bounds S_L = min(uL, uR) - max(cL, cR), S_R = max(uL, uR) +
     max(cL, cR) (Davis 1988, his simplest estimate).
@@ -155,54 +166,131 @@ def toro_hllc(gamma, L, R):
 
 # ------------------------------------------------------------------ MHD
 
-def mk_hlld(gamma, L, R):
-    """Miyoshi-Kusano HLLD, Heaviside-Lorentz B, constant normal field.
+def _mhd_side(g, Q):
+    """Face-frame state Q = (rho, p, un, vt, wm, Bn, Bt, Bm), Heaviside-Lorentz B:
+    its conserved vector and physical normal flux over the 8 components
+    (rho, mn, mt, mm, Bn, Bt, Bm, E) and the derived scalars."""
+    r, p, u, v, w, bn, bt, bm = Q
+    b2 = bn * bn + bt * bt + bm * bm
+    pt = p + 0.5 * b2
+    E = p / (g - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
+    cf = math.sqrt((g * p + b2 + math.sqrt((g * p + b2) ** 2 - 4 * g * p * bn * bn)) / (2 * r))
+    vb = u * bn + v * bt + w * bm
+    U = (r, r * u, r * v, r * w, bn, bt, bm, E)
+    F = (r * u, r * u * u + pt - bn * bn, r * u * v - bn * bt, r * u * w - bn * bm,
+         0.0, u * bt - v * bn, u * bm - w * bn, (E + pt) * u - bn * vb)
+    return dict(r=r, p=p, u=u, v=v, w=w, bn=bn, bt=bt, bm=bm, pt=pt, E=E, cf=cf, vb=vb, U=U, F=F)
 
-    L, R = (rho, p, un, vt, wm, Bn, Bt, Bm) in the face frame with the same Bn.
-    Returns the 7 fluxes (rho, mn, mt, mm, Bt, Bm, E) and (SL, SR).
+
+def _outer_speeds(A, B):
+    cf = max(A["cf"], B["cf"])
+    return min(A["u"], B["u"]) - cf, max(A["u"], B["u"]) + cf
+
+
+def _hll_state(A, B, SL, SR):
+    return tuple((SR * ur - SL * ul + fl - fr) / (SR - SL) for ul, ur, fl, fr in zip(A["U"], B["U"], A["F"], B["F"]))
+
+
+def _contact_speed(A, B, SL, SR):
+    sL, sR = (SL - A["u"]) * A["r"], (SR - B["u"]) * B["r"]
+    return (sR * B["u"] - sL * A["u"] - B["pt"] + A["pt"]) / (sR - sL)
+
+
+def li_hllc(gamma, L, R):
+    """Li (2005) HLLC-C for MHD in the face frame, Heaviside-Lorentz B.
+
+    L, R = (rho, p, un, vt, wm, Bn, Bt, Bm). Returns the 8 normal-frame fluxes
+    (rho, mn, mt, mm, Bn, Bt, Bm, E), with the physical (zero) normal-field
+    flux, and (S_L, S_R).
+
+    Star states, K = L, R:
+      rho*_K = rho_K (S_K - u_K) / (S_K - S_M), normal velocity S_M;
+      B* = B_hll (all three components);
+      tangential velocities v*, w* = (rho v)_hll / rho_hll, (rho w)_hll / rho_hll;
+      p*_T = p_TK + rho_K (S_K - u_K)(S_M - u_K) (the same from both sides
+        when the normal field is continuous; their mean otherwise);
+      E*_K = [(S_K - u_K) E_K - p_TK u_K + p*_T S_M
+              + B*_n ((v.B)_K - (v.B)_hll)] / (S_K - S_M),
+        (v.B)_hll = v_hll . B_hll with v_hll = (rho v)_hll / rho_hll;
+    and the flux F_K + S_K (U*_K - U_K) on either side of the contact.
+    """
+    A, B = _mhd_side(gamma, L), _mhd_side(gamma, R)
+    SL, SR = _outer_speeds(A, B)
+    Uh = _hll_state(A, B, SL, SR)
+    SM = _contact_speed(A, B, SL, SR)
+    vh = (Uh[1] / Uh[0], Uh[2] / Uh[0], Uh[3] / Uh[0])
+    bh = (Uh[4], Uh[5], Uh[6])
+    vb_h = vh[0] * bh[0] + vh[1] * bh[1] + vh[2] * bh[2]
+    pts = 0.5 * ((A["pt"] + A["r"] * (SL - A["u"]) * (SM - A["u"])) +
+                 (B["pt"] + B["r"] * (SR - B["u"]) * (SM - B["u"])))
+
+    def star(K, S):
+        rs = K["r"] * (S - K["u"]) / (S - SM)
+        Es = ((S - K["u"]) * K["E"] - K["pt"] * K["u"] + pts * SM + bh[0] * (K["vb"] - vb_h)) / (S - SM)
+        return (rs, rs * SM, rs * vh[1], rs * vh[2], bh[0], bh[1], bh[2], Es)
+
+    if SL >= 0:
+        F = A["F"]
+    elif SM >= 0:
+        Us = star(A, SL)
+        F = tuple(A["F"][k] + SL * (Us[k] - A["U"][k]) for k in range(8))
+    elif SR > 0:
+        Us = star(B, SR)
+        F = tuple(B["F"][k] + SR * (Us[k] - B["U"][k]) for k in range(8))
+    else:
+        F = B["F"]
+    return F, (SL, SR)
+
+
+def mk_hlld(gamma, L, R, deg_abs=None, alfven_abs=0.0):
+    """Miyoshi-Kusano HLLD, Heaviside-Lorentz B.
+
+    L, R = (rho, p, un, vt, wm, Bn, Bt, Bm) in the face frame. Returns the 7
+    fluxes (rho, mn, mt, mm, Bt, Bm, E) and (SL, SR). The star normal field
+    B_x is the HLL average of the two sides' normal fields (their common
+    value when they agree).
+
+    Degenerate cases (M-K sec. 4.3): where rho_K (S_K - u_K)(S_K - S_M) - B_x^2
+    vanishes, v*_K = v_K and B*_tK = 0; with deg_abs given, "vanishes" means
+    |.| < deg_abs, else |.| below 1e-12 of its first term. Where B_x vanishes
+    the Alfven waves merge with the contact (S*_L = S*_R = S_M); "vanishes"
+    means |B_x| <= alfven_abs.
     """
     g = gamma
-
-    def unpack(Q):
-        r, p, u, v, w, bn, bt, bm = Q
-        b2 = bn * bn + bt * bt + bm * bm
-        pt = p + 0.5 * b2
-        E = p / (g - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
-        cf = math.sqrt((g * p + b2 + math.sqrt((g * p + b2) ** 2 - 4 * g * p * bn * bn)) / (2 * r))
-        vb = u * bn + v * bt + w * bm
-        F = (r * u, r * u * u + pt - bn * bn, r * u * v - bn * bt, r * u * w - bn * bm,
-             u * bt - v * bn, u * bm - w * bn, (E + pt) * u - bn * vb)
-        U = (r, r * u, r * v, r * w, bt, bm, E)
-        return r, p, u, v, w, bn, bt, bm, pt, E, cf, F, U
-
-    rL, pL, uL, vL, wL, Bx, btL, bmL, ptL, EL, cfL, FL, UL = unpack(L)
-    rR, pR, uR, vR, wR, _, btR, bmR, ptR, ER, cfR, FR, UR = unpack(R)
-    SL = min(uL, uR) - max(cfL, cfR)
-    SR = max(uL, uR) + max(cfL, cfR)
-    SM = ((SR - uR) * rR * uR - (SL - uL) * rL * uL - ptR + ptL) / ((SR - uR) * rR - (SL - uL) * rL)
-    pts = ((SR - uR) * rR * ptL - (SL - uL) * rL * ptR + rL * rR * (SR - uR) * (SL - uL) * (uR - uL)) / \
+    A, B = _mhd_side(g, L), _mhd_side(g, R)
+    SL, SR = _outer_speeds(A, B)
+    SM = _contact_speed(A, B, SL, SR)
+    Bx = (SR * B["bn"] - SL * A["bn"]) / (SR - SL)
+    rL, uL, rR, uR = A["r"], A["u"], B["r"], B["u"]
+    pts = ((SR - uR) * rR * A["pt"] - (SL - uL) * rL * B["pt"] + rL * rR * (SR - uR) * (SL - uL) * (uR - uL)) / \
           ((SR - uR) * rR - (SL - uL) * rL)
 
-    def star(r, u, v, w, bt, bm, pt, E, S):
+    def star(K, S):
+        r, u, v, w, bt, bm = K["r"], K["u"], K["v"], K["w"], K["bt"], K["bm"]
         rs = r * (S - u) / (S - SM)
         den = r * (S - u) * (S - SM) - Bx * Bx
-        if abs(den) < 1e-12 * max(1.0, abs(r * (S - u) * (S - SM))):
-            vs, ws, bts, bms = v, w, bt, bm
+        tol = deg_abs if deg_abs is not None else 1e-12 * max(1.0, abs(r * (S - u) * (S - SM)))
+        if abs(den) < tol:
+            vs, ws, bts, bms = v, w, 0.0, 0.0
         else:
             vs = v - Bx * bt * (SM - u) / den
             ws = w - Bx * bm * (SM - u) / den
             bts = bt * (r * (S - u) ** 2 - Bx * Bx) / den
             bms = bm * (r * (S - u) ** 2 - Bx * Bx) / den
         vbs = SM * Bx + vs * bts + ws * bms
-        Es = ((S - u) * E - pt * u + pts * SM + Bx * (u * Bx + v * bt + w * bm - vbs)) / (S - SM)
+        Es = ((S - u) * K["E"] - K["pt"] * u + pts * SM + Bx * (K["vb"] - vbs)) / (S - SM)
         return rs, vs, ws, bts, bms, Es, (rs, rs * SM, rs * vs, rs * ws, bts, bms, Es)
 
-    rsL, vsL, wsL, btsL, bmsL, EsL, UsL = star(rL, uL, vL, wL, btL, bmL, ptL, EL, SL)
-    rsR, vsR, wsR, btsR, bmsR, EsR, UsR = star(rR, uR, vR, wR, btR, bmR, ptR, ER, SR)
+    def seven(t8):  # drop the normal-field component
+        return (t8[0], t8[1], t8[2], t8[3], t8[5], t8[6], t8[7])
+
+    FL, UL, FR, UR = seven(A["F"]), seven(A["U"]), seven(B["F"]), seven(B["U"])
+    rsL, vsL, wsL, btsL, bmsL, EsL, UsL = star(A, SL)
+    rsR, vsR, wsR, btsR, bmsR, EsR, UsR = star(B, SR)
     sL, sR = math.sqrt(rsL), math.sqrt(rsR)
     SsL = SM - abs(Bx) / sL
     SsR = SM + abs(Bx) / sR
-    if Bx == 0.0:
+    if abs(Bx) <= alfven_abs:
         SsL = SsR = SM
         sgn = 0.0
     else:
