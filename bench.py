@@ -6,12 +6,15 @@ minmod MUSCL on primitives, 2-stage SSP-RK2, CFL 0.4, transmissive sides,
 seeded four-quadrant Riemann problem (SplitMix64 seed 0). One step = both
 RK stages over the whole grid with the device-side CFL time step.
 
-Prints one JSON line: Gcell-updates/s over the timed steps, the roofline of
-the one-pass step kernel (160 algorithmic bytes per cell-update / its
-HIP-event-timed launch duration, against the 8 TB/s HBM peak), a CPU
-baseline (the oracle on a bounded sample of the same workload, timed on this
-host) and the batched-ensemble and adjoint legs, plus the ideal-MHD HLLD
-Orszag-Tang run at 2048^2 and the exact-Riemann Sod run at 200^2.
+Prints one JSON line: Gcell-updates/s over the timed steps (each step is one
+launch of the one-pass step kernel, which also updates the device clock), the
+roofline of that kernel (160 algorithmic bytes per cell-update / its average
+launch duration in the timed region, from HIP events on the launch stream,
+against the 8 TB/s HBM peak), a CPU baseline (the oracle on a bounded sample
+of the same workload, timed on this host), the oracle parity of the first
+steps of the same workload, the batched-ensemble and adjoint legs, plus the
+ideal-MHD HLLD Orszag-Tang run at 2048^2 and the exact-Riemann Sod run at
+200^2.
 """
 from __future__ import annotations
 
@@ -65,8 +68,13 @@ def traffic_from_profiles():
         return None
 
 
+PARITY_STEPS = 25
+
+
 def cpu_baseline(n, seed):
-    """Oracle (OpenMP) on a bounded sample: the first 25 SSP-RK2 steps of the full grid."""
+    """Oracle (OpenMP) on a bounded sample: the first PARITY_STEPS SSP-RK2
+    steps of the full grid. Returns the baseline record and the oracle's state
+    after those steps."""
     import numpy as np
     from oracle import oracle as O
     from golikov_2d_amd import problems as PB
@@ -75,7 +83,7 @@ def cpu_baseline(n, seed):
     og = O.make_grid(n, rows, dy=1.0 / n)
     U = PB.four_quadrant(og, seed=seed)
     sc = O.Scheme(O.PHYS_EULER, O.FLUX_HLLC, 2, 0.4)
-    steps = 25
+    steps = PARITY_STEPS
     t0 = time.perf_counter()
     O.run(og, sc, 2, U, nsteps=steps)
     dt = time.perf_counter() - t0
@@ -94,12 +102,14 @@ def cpu_baseline(n, seed):
         "value": round(n * rows * steps / dt / 1e9, 6),
         "unit": "Gcell-updates/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": f"oracle (C, OpenMP, -O3 -ffp-contract=off): the first {steps} SSP-RK2 steps of the same "
                   f"{n}x{rows} workload, {dt:.2f} s wall",
         "one_thread": round(n * rows / dt1 / 1e9, 6),
         "one_thread_sample": f"the first SSP-RK2 step of the same workload on 1 thread, {dt1:.2f} s wall",
-    }
+    }, U
 
 
 def cpu_leg(g_kw, scheme, U, integrator, nsteps=0, t_end=0.0):
@@ -589,34 +599,35 @@ def main():
     dev = torch.device("cuda:0")
     g = api.make_grid(n, n, gamma=1.4)
     sc = api.make_scheme(api.EULER, api.FLUX_HLLC, 2, api.SSPRK2, 0.4)
+    if not api.one_pass_preferred(g, sc):
+        raise SystemExit("the 4096^2 step is expected to take the one-pass kernel")
     U0 = PB.four_quadrant(g, seed=args.seed)
     U = torch.from_numpy(U0).to(dev)
+    stream = torch.cuda.current_stream()
+    # the measured copy ceiling first: it also brings the GPU to its working
+    # clocks before the warm-up steps
+    copy_gbs = copy_ceiling_gbs()
+
     solver = api.Solver(g, sc)
     solver.start(U)
     solver.advance(U, args.warmup)
     torch.cuda.synchronize()
-
+    # timed region: K steps, each one launch of the one-pass step kernel (the
+    # kernel takes its dt from the device clock and updates it), back to back
+    # on the launch stream; an odd K adds one device copy back into U at the end
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # the library brackets the kernel of every 5th timed step with HIP events
-    # on the launch stream: the step kernel's own duration inside the timed
-    # region (each event pair costs ~7 us of the step it brackets)
-    solver.time_kernels(max(1, args.steps // 5), stride=5)
-    torch.cuda.synchronize()
-    start.record()
+    start.record(stream)
     solver.advance(U, args.steps)
-    end.record()
+    end.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(end)
-    live_kernel_ms, live_n = solver.kernel_ms()
-    solver.time_kernels(0)
     t_sim, nsteps, dt_last = solver.time()
     cells = n * n
     value = cells * args.steps / (ms / 1e3) / 1e9
 
-    # for comparison, with HIP events on the launch stream: the one-pass step
-    # kernel back to back outside the timed region, then the two stage
-    # kernels of the stage-by-stage step
-    stream = torch.cuda.current_stream()
+    # for comparison, with HIP events on the launch stream, outside the timed
+    # region: the one-pass step kernel back to back, then the two stage kernels
+    # of the stage-by-stage step
     S = torch.empty_like(U)
     dtd = torch.tensor([dt_last], dtype=torch.float64, device=dev)
     w = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -645,8 +656,19 @@ def main():
     torch.cuda.synchronize()
     t1 = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps)) / reps
     t2 = sum(ev[2 * i + 1].elapsed_time(ev[2 * i + 2]) for i in range(reps)) / reps
+    del S
+
+    # the dominant kernel's average launch duration inside the timed region:
+    # the region holds exactly K launches of it and nothing else (even K)
+    if args.steps % 2 == 0:
+        kernel_ms = ms / args.steps
+        kernel_src = (f"timed region / {args.steps}: {args.steps} launches of the one-pass step kernel back to "
+                      "back between two HIP events on the launch stream, nothing else enqueued")
+    else:
+        kernel_ms = step_kernel_ms
+        kernel_src = f"{reps} launches back to back after the timed region (odd K: the region ends with a copy)"
     bytes_per_launch = BYTES_PER_CELL_UPDATE * cells  # one launch advances every cell one full step
-    achieved = bytes_per_launch / (live_kernel_ms / 1e3) / 1e9
+    achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -655,30 +677,48 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(),
         "kernel": "euler_rk2_kernel<2,HLLC> (one-pass SSP-RK2 step)",
-        "kernel_ms": round(live_kernel_ms, 4),
-        "kernel_ms_source": f"mean of {live_n} launches in the timed steps, HIP events on the launch stream",
+        "kernel_ms": round(kernel_ms, 4),
+        "kernel_ms_source": kernel_src,
         "kernel_ms_back_to_back": round(step_kernel_ms, 4),
         "bytes_per_launch": bytes_per_launch,
         "two_pass_stage_ms": [round(t1, 4), round(t2, 4)],
+        "copy_ceiling": round(copy_gbs, 1),
+        "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
     }
-    copy_gbs = copy_ceiling_gbs()
-    roofline["copy_ceiling"] = round(copy_gbs, 1)
-    roofline["frac_of_copy_ceiling"] = round(achieved / copy_gbs, 4)
+    if kernel_ms > ms / args.steps * (1 + 1e-9):
+        raise SystemExit(f"kernel time {kernel_ms} ms exceeds the step time {ms / args.steps} ms")
 
     Uh = U.cpu().numpy()
     rho = Uh[0]
     p = 0.4 * (Uh[3] - 0.5 * (Uh[1] ** 2 + Uh[2] ** 2) / rho)
     healthy = bool(np.isfinite(Uh).all() and rho.min() > 0 and p.min() > 0)
 
+    # parity on the benchmarked workload: a fresh run of the first
+    # PARITY_STEPS steps (one-pass kernel, fused clock) against the oracle's
+    # state after the same steps, computed by the CPU baseline leg below
+    Up = torch.from_numpy(U0).to(dev)
+    psolver = api.Solver(g, sc)
+    psolver.start(Up)
+    psolver.advance(Up, PARITY_STEPS)
+    gpu_after = Up.cpu().numpy()
+    _, psteps, _ = psolver.time()
+    psolver.close()
+    del Up
+
     if args.copy_ref_mib > 0:
         copy_reference(args.copy_ref_mib)
 
     cpu = None
+    parity = {"steps": PARITY_STEPS, "gpu_steps": psteps, "checked": False}
     if not args.no_cpu:
-        cpu = cpu_baseline(n, args.seed)
+        cpu, oracle_after = cpu_baseline(n, args.seed)
+        err = np.abs(gpu_after - oracle_after).reshape(4, -1).sum(1) / np.abs(oracle_after).reshape(4, -1).sum(1)
+        parity.update({"checked": True, "rel_l1_per_field": [float(f"{e:.3e}") for e in err],
+                       "gate": 1e-10, "ok": bool(np.all(err <= 1e-10)) and psteps == PARITY_STEPS,
+                       "against": "oracle (C) after the same steps from the same seeded state"})
+        healthy = healthy and parity["ok"]
     extras = None
     if not args.no_extras:
-        del S
         extras = {"batched_vs_serial": batched_vs_serial(args.seed),
                   "adjoint_vs_forward": adjoint_vs_forward(seed=args.seed),
                   "mhd_orszag_tang_2048": mhd_orszag_tang(seed=args.seed, with_cpu=not args.no_cpu),
@@ -705,6 +745,7 @@ def main():
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
         "extras": extras,
         "healthy": healthy,
         "sim_time": t_sim,

@@ -275,6 +275,7 @@ __global__ void clock_start_kernel(Clock* c, double t0, double cfl) {
     c->cfl = cfl;
     c->w[0] = 0ull;
     c->w[1] = 0ull;
+    c->w[2] = 0ull;
     c->step = 0;
 }
 
@@ -446,8 +447,11 @@ int gd2_stage(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, const 
     return dispatch_euler<kSoA, false, true>(sc->order, sc->flux, A, st);
 }
 
-int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double* Ustage,
-                    double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
+// One-pass SSP-RK2 step; with clk != null the kernel takes its dt from the
+// clock (slot clk_slot) and one of its threads does the clock update.
+static int ssprk2_launch(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double* Ustage,
+                         double dt, const double* dt_dev, unsigned long long* wmax_bits, Clock* clk, int clk_slot,
+                         double* dt_log, long long dt_cap, hipStream_t st) {
     if (!grid_ok(g, false) || !scheme_ok(sc) || !Uin || !Uout || Uin == Uout) return GD2_ERR_ARG;
     if (Ustage && (Ustage == Uin || Ustage == Uout)) return GD2_ERR_ARG;
     if (sc->physics != GD2_EULER) return GD2_ERR_UNSUPPORTED;
@@ -464,9 +468,19 @@ int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, 
     A.dt = dt;
     A.dt_dev = dt_dev;
     A.wmax_bits = wmax_bits;
+    A.clk = clk;
+    A.clk_slot = clk_slot;
+    A.dt_log = dt_log;
+    A.dt_cap = dt_cap;
     if (!A.diag) return GD2_ERR_HIP;
-    if (Ustage) return dispatch_rk2<true>(sc->order, sc->flux, A, (hipStream_t)stream);
-    return dispatch_rk2<false>(sc->order, sc->flux, A, (hipStream_t)stream);
+    if (Ustage) return dispatch_rk2<true>(sc->order, sc->flux, A, st);
+    return dispatch_rk2<false>(sc->order, sc->flux, A, st);
+}
+
+int gd2_ssprk2_step(const gd2_grid* g, const gd2_scheme* sc, const double* Uin, double* Uout, double* Ustage,
+                    double dt, const double* dt_dev, unsigned long long* wmax_bits, void* stream) {
+    return ssprk2_launch(g, sc, Uin, Uout, Ustage, dt, dt_dev, wmax_bits, nullptr, 0, nullptr, 0,
+                         (hipStream_t)stream);
 }
 
 int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, unsigned long long* wmax_bits,
@@ -806,18 +820,36 @@ int gd2_ctx_advance(gd2_ctx* c, double* U, int nsteps, double t_end, void* strea
     const double* dtp = &c->clock->dt;
     double* cur = U;  // forward Euler and the one-pass SSP-RK2 step ping-pong between U and S
     const bool one_pass = gd2_one_pass_preferred(&c->g, &c->sc) == 1;
+    // events on a stream under capture would become capture dependencies,
+    // not timestamps: no kernel timing while the stream is captured
+    bool capturing = false;
+    if (c->ev_cap > 0) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GD2_HIP(hipStreamIsCapturing(st, &cs));
+        capturing = cs != hipStreamCaptureStatusNone;
+    }
+    // the one-pass step with no end time does its own clock update (one
+    // launch per step); otherwise a one-thread clock kernel starts the step
+    const bool fused_clock = c->sc.integrator == GD2_SSPRK2 && one_pass && !(t_end > 0.0);
     for (int i = 0; i < nsteps; ++i) {
-        const int slot = (int)(c->host_steps & 1);
-        hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log, c->dt_cap);
-        GD2_HIP(hipGetLastError());
-        unsigned long long* wn = &c->clock->w[slot ^ 1];
-        const bool timed = c->ev_n < c->ev_cap && c->ev_seen++ % c->ev_stride == 0;
+        const int slot = (int)(c->host_steps % 3);
+        if (!fused_clock) {
+            hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(1), 0, st, c->clock, slot, t_end, c->dt_log,
+                               c->dt_cap);
+            GD2_HIP(hipGetLastError());
+        }
+        unsigned long long* wn = &c->clock->w[slot == 2 ? 0 : slot + 1];
+        const bool timed = !capturing && c->ev_n < c->ev_cap && c->ev_seen++ % c->ev_stride == 0;
         if (timed) GD2_HIP(hipEventRecord(c->ev[2 * c->ev_n], st));
         int rc;
         if (c->sc.integrator == GD2_SSPRK2 && one_pass) {
             // one pass per step, ping-ponging between U and the scratch state
             double* nxt = (cur == U) ? S : U;
-            rc = gd2_ssprk2_step(&c->g, &c->sc, cur, nxt, nullptr, 0.0, dtp, wn, stream);
+            if (fused_clock)
+                rc = ssprk2_launch(&c->g, &c->sc, cur, nxt, nullptr, 0.0, nullptr, wn, c->clock, slot, c->dt_log,
+                                   c->dt_cap, st);
+            else
+                rc = gd2_ssprk2_step(&c->g, &c->sc, cur, nxt, nullptr, 0.0, dtp, wn, stream);
             if (rc) return rc;
             cur = nxt;
         } else if (c->sc.integrator == GD2_SSPRK2) {
@@ -953,7 +985,7 @@ static int ens_launch_stage(gd2_ensemble* e, int slot, hipStream_t st) {
 
 static int ens_enqueue_steps(gd2_ensemble* e, int nsteps, long long first_step, hipStream_t st) {
     for (int i = 0; i < nsteps; ++i) {
-        const int slot = (int)((first_step + i) & 1);
+        const int slot = (int)((first_step + i) % 3);
         hipLaunchKernelGGL(ens_clock_kernel, dim3((e->n + 63) / 64), dim3(64), 0, st, e->dprobs, e->n, slot);
         GD2_HIP(hipGetLastError());
         int rc = ens_launch_stage<1>(e, slot, st);
@@ -1061,7 +1093,7 @@ int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* strea
         e->host_steps += nsteps;
         return GD2_OK;
     }
-    const int parity = (int)(e->host_steps & 1);
+    const int parity = (int)(e->host_steps % 3);  // the first step's clock slot
     auto key = std::make_pair(nsteps, parity);
     auto it = e->graphs.find(key);
     if (it == e->graphs.end()) {

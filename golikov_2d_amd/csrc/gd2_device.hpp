@@ -22,15 +22,28 @@ struct DeviceDiag {
 
 // Device clock of a solver context: the next step's dt is cfl / w[cur]
 // (w = wave-speed maximum of the current state, accumulated as double bits).
+// Device clock of a native solver. Step n takes dt = cfl / w[n % 3] (the
+// wave-speed bound of U^n) and folds the bound of U^{n+1} into w[(n+1) % 3];
+// w[(n+1) % 3] is zero when step n starts (three slots: a step clears the
+// slot of the step after it, so no clearing races with a fold).
 struct Clock {
     double t, dt, cfl, pad;
-    unsigned long long w[2];
+    unsigned long long w[3];
     long long step;
 };
 
+__device__ __forceinline__ int clock_next(int cur) { return cur == 2 ? 0 : cur + 1; }
+
+__device__ __forceinline__ double clock_dt(const Clock* c, int cur) {
+    return c->cfl / __longlong_as_double((long long)c->w[cur]);
+}
+
+// The step's clock update, done by one thread before any wave of the step
+// folds into w[(cur+1) % 3]: dt (clipped to t_end when t_end > 0; 0 and no
+// step once t reached it), t, the step count, and the slots of the next
+// steps cleared.
 __device__ __forceinline__ void clock_begin(Clock* c, int cur, double t_end) {
-    const double w = __longlong_as_double((long long)c->w[cur]);
-    double dt = c->cfl / w;
+    double dt = clock_dt(c, cur);
     bool live = true;
     if (t_end > 0.0) {
         if (!(c->t < t_end)) {
@@ -41,11 +54,26 @@ __device__ __forceinline__ void clock_begin(Clock* c, int cur, double t_end) {
         }
     }
     c->dt = dt;
-    c->w[cur ^ 1] = 0ull;
+    c->w[clock_next(cur)] = 0ull;
+    c->w[clock_next(clock_next(cur))] = 0ull;
     if (live) {
         c->t = c->t + dt;
         c->step += 1;
     }
+}
+
+// The clock update of a step whose kernel computes dt itself (no end time):
+// every wave reads only cfl and w[cur]; one thread of the launch writes dt,
+// t, the step count and clears w[(cur+2) % 3] (w[(cur+1) % 3], which the
+// waves fold into, was cleared by the step before).
+__device__ __forceinline__ void clock_commit(Clock* c, int cur, double* dt_log, long long cap) {
+    const double dt = clock_dt(c, cur);
+    const long long n = c->step;
+    c->dt = dt;
+    c->t = c->t + dt;
+    c->step = n + 1;
+    c->w[clock_next(clock_next(cur))] = 0ull;
+    if (dt_log && n < cap) dt_log[n] = dt;
 }
 
 // Logical index k (may be -2..n+1, or further on padding lanes) -> stored

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_
             A.Uout = P.U;
             A.a = 0.5;
             A.b = 0.5;
-            A.wmax_bits = &P.clk->w[slot ^ 1];
+            A.wmax_bits = &P.clk->w[clock_next(slot)];
             euler_tile<ORDER, FLUX, kSoA, true, true>(A, tl.tx, tl.ty, lds);
         }
     }
@@ -71,6 +71,7 @@ __global__ void ens_clock_start_kernel(Clock* clks, int n, double cfl) {
         c->cfl = cfl;
         c->w[0] = 0ull;
         c->w[1] = 0ull;
+        c->w[2] = 0ull;
         c->step = 0;
     }
 }

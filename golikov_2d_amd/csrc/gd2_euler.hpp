@@ -45,8 +45,14 @@ struct EulerArgs {
     int bc[4];
     // time stepping
     double a, b;
-    double dt;                     // used when dt_dev == null
+    double dt;                     // used when dt_dev == null and clk == null
     const double* dt_dev;
+    // one-pass step with its clock update inside (no end time): dt =
+    // clock_dt(clk, clk_slot) in every wave, clock_commit by one thread
+    Clock* clk;
+    int clk_slot;
+    double* dt_log;
+    long long dt_cap;
     unsigned long long* wmax_bits; // kSoA: wave-speed max of the output (may be null)
     unsigned long long* tmin_bits; // kAoS: face time-step min (the reference's *T)
     DeviceDiag* diag;

@@ -107,7 +107,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
     const int r_first = ghost_lo ? 0 : j0 - 2;
     const int r_last = A.bc[3] != GD2_BC_PERIODIC ? min(j1 + 1, A.ny - 1) : j1 + 1;
 
-    const double dt = A.dt_dev ? *A.dt_dev : A.dt;
+    const double dt = A.clk ? clock_dt(A.clk, A.clk_slot) : (A.dt_dev ? *A.dt_dev : A.dt);
     const double dx = A.dx, dy = A.dy;
     const double dtdV = dt / (dx * dy);
     const double bdt = A.b * dtdV;
@@ -448,6 +448,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
 template <int ORDER, int FLUX, bool KEEP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_HLLC || FLUX == GD2_FLUX_HLL) ? 2 : 1))) void euler_rk2_kernel(const EulerArgs A) {
     __shared__ Rk2Lds lds;
+    if (A.clk && blockIdx.x == 0 && threadIdx.x == 0) clock_commit(A.clk, A.clk_slot, A.dt_log, A.dt_cap);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     euler_rk2_tile<ORDER, FLUX, KEEP>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
 }

@@ -142,6 +142,42 @@ def test_solver_context_one_pass_equals_two_pass(nsteps):
     assert tg == t
 
 
+def test_fused_clock_steps_mix_with_clock_kernel_steps():
+    """The one-pass step with no end time updates the device clock inside the
+    step kernel; steps with an end time go through the clock kernel. Mixed
+    in any order (each switch moves between the two three-slot schemes),
+    they give the states, clocks and dt logs of stage-by-stage stepping, bit
+    for bit, including the no-op steps after the end time."""
+    say("fused clock vs clock kernel")
+    g, og = grids(300, 170, BCS["refl_x_per_y"])
+    sc = api.make_scheme()
+    U0 = to_dev(PB.four_quadrant(og, seed=11))
+    res = []
+    for mode in (api.STEP_ONE_PASS, api.STEP_TWO_PASS):
+        prev = api.set_step_path(mode)
+        try:
+            s = api.Solver(g, sc)
+            log = torch.full((40,), -1.0, dtype=torch.float64, device=dev())
+            s.log_dt(log)
+            U = U0.clone()
+            s.start(U)
+            s.advance(U, 3)                  # fused (one-pass)
+            t3, _, _ = s.time()
+            s.advance(U, 2, t_end=1.0)       # clock kernel
+            s.advance(U, 4)                  # fused
+            t9, n9, _ = s.time()
+            s.advance(U, 3, t_end=t9 + 0.5 * (t9 - t3) / 6)  # reaches t_end, then no-ops
+            s.advance(U, 1)                  # fused
+            res.append((U, s.time(), to_host(log)))
+            s.close()
+        finally:
+            api.set_step_path(prev)
+    (Ua, ca, la), (Ub, cb, lb) = res
+    assert torch.equal(Ua, Ub)
+    assert ca == cb and ca[1] == 9 + 1 + 1
+    assert np.array_equal(la, lb) and np.all(la[:11] > 0) and np.all(la[11:] == -1.0)
+
+
 @pytest.mark.parametrize("path", ["one_pass", "two_pass"])
 def test_solver_kernel_timing_changes_nothing(path):
     """gd2_ctx_time_kernels / gd2_ctx_kernel_ms: events around every
