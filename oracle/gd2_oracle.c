@@ -235,6 +235,272 @@ double o_hllc(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
 }
 
 /* ------------------------------------------------------------------------ */
+/* HLLC_Aleksashov2, kernel.cu:1167-1348. The HLLC of HLLC_Aleksashov with   */
+/* B = 0 in the frame (n, t = (-n2, n1)), except for its contact speed: the  */
+/* numerator takes the left state's TANGENTIAL velocity vL[1] in the term    */
+/* suL r1 vL[1] (kernel.cu:1222), and SM = 0 when the denominator is 0       */
+/* (kernel.cu:1219-1223). Restated as written.                               */
+/* ------------------------------------------------------------------------ */
+double o_hllc_aleksashov2(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+                          double n1, double n2, o_d2* Ps, o_d2* Pu, double rad)
+{
+    const double g1 = gamma - 1.0;
+    double r1 = Ls.x, p1 = Ls.y, u1 = Lu.x, v1 = Lu.y;
+    double r2 = Rs.x, p2 = Rs.y, u2 = Ru.x, v2 = Ru.y;
+    /* kernel.cu:1187-1201: aco = [[n1, -n2], [n2, n1]] */
+    double vL0 = n1 * u1 + n2 * v1, vL1 = -n2 * u1 + n1 * v1;
+    double vR0 = n1 * u2 + n2 * v2, vR1 = -n2 * u2 + n1 * v2;
+    /* kernel.cu:1208-1217 */
+    double cL = sqrt(gamma * p1 / r1);
+    double cR = sqrt(gamma * p2 / r2);
+    double SL = fmin(vL0 - cL, vR0 - cR);
+    double SR = fmax(vL0 + cL, vR0 + cR);
+    double t = 10000000;
+    t = fmin(t, krit * rad / fmax(fabs(SL), fabs(SR)));
+    /* kernel.cu:1219-1225 */
+    double suR = SR - vR0;
+    double suL = SL - vL0;
+    double SM = 0.0;
+    if (fabs(suR * r2 - suL * r1) > 0) SM = (suR * r2 * vR0 - p2 + p1 - suL * r1 * vL1) / (suR * r2 - suL * r1);
+    /* kernel.cu:1232-1261 */
+    double upt1 = (u1 * u1 + v1 * v1) / 2.0;
+    double upt2 = (u2 * u2 + v2 * v2) / 2.0;
+    double e1 = p1 / g1 + r1 * upt1;
+    double e2 = p2 / g1 + r2 * upt2;
+    double FL0 = r1 * vL0, FL1 = r1 * vL0 * vL0 + p1, FL2 = r1 * vL0 * vL1, FL3 = (e1 + p1) * vL0;
+    double FR0 = r2 * vR0, FR1 = r2 * vR0 * vR0 + p2, FR2 = r2 * vR0 * vR1, FR3 = (e2 + p2) * vR0;
+    double UL1 = r1 * vL0, UL2 = r1 * vL1, UR1 = r2 * vR0, UR2 = r2 * vR1;
+    /* kernel.cu:1264-1297 */
+    double suRm = suR / (SR - SM);
+    double suLm = suL / (SL - SM);
+    double rzR = r2 * suRm;
+    double rzL = r1 * suLm;
+    double ptzR = p2 + r2 * suR * (SM - vR0);
+    double ptzL = p1 + r1 * suL * (SM - vL0);
+    double ptz = (ptzR + ptzL) / 2.0;
+    double ezR = e2 * suRm + (ptz * SM - p2 * vR0) / (SR - SM);
+    double ezL = e1 * suLm + (ptz * SM - p1 * vL0) / (SL - SM);
+    double q0, q1;
+    /* kernel.cu:1301-1341 */
+    if (SL > 0.0) {
+        Ps->x = FL0; Ps->y = FL3; q0 = FL1; q1 = FL2;
+    } else if (SL <= 0.0 && SM >= 0.0) {
+        Ps->x = FL0 + SL * (rzL - r1);
+        Ps->y = FL3 + SL * (ezL - e1);
+        q0 = FL1 + SL * (SM * rzL - UL1);
+        q1 = FL2 + SL * (vL1 * rzL - UL2);
+    } else if (SM <= 0.0 && SR >= 0.0) {
+        Ps->x = FR0 + SR * (rzR - r2);
+        Ps->y = FR3 + SR * (ezR - e2);
+        q0 = FR1 + SR * (SM * rzR - UR1);
+        q1 = FR2 + SR * (vR1 * rzR - UR2);
+    } else if (SR < 0.0) {
+        Ps->x = FR0; Ps->y = FR3; q0 = FR1; q1 = FR2;
+    } else {
+        diag_inc(&g_diag.bad_branch);
+        Ps->x = Ps->y = NAN; q0 = q1 = NAN;
+    }
+    /* kernel.cu:1343-1344 */
+    Pu->x = n1 * q0 + -n2 * q1;
+    Pu->y = n2 * q0 + n1 * q1;
+    return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLC_Korolkov, kernel.cu:1349-1483: lab-frame fluxes projected on n; the  */
+/* outer speeds of the HLL solver (D_L = min(Vn) - max(c), D_R = max(Vn) +   */
+/* max(c)); contact speed D_C and star pressure P_T of the two-state HLLC;   */
+/* the star state takes the normal velocity D_C, keeps the tangential one,   */
+/* and its energy from the equation of state, P_T/(gamma-1) + rho*|V*|^2/2   */
+/* (kernel.cu:1448, 1468). Omega = 0 (kernel.cu:11).                         */
+/* ------------------------------------------------------------------------ */
+double o_hllc_korolkov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+                       double n1, double n2, o_d2* Ps, o_d2* Pu, double rad)
+{
+    const double ro_L = Ls.x, p_L = Ls.y, v1_L = Lu.x, v2_L = Lu.y;
+    const double ro_R = Rs.x, p_R = Rs.y, v1_R = Ru.x, v2_R = Ru.y;
+    /* kernel.cu:1366-1382 */
+    const double Vkv_L = v1_L * v1_L + v2_L * v2_L;
+    const double Vkv_R = v1_R * v1_R + v2_R * v2_R;
+    const double c_L = sqrt(gamma * p_L / ro_L);
+    const double c_R = sqrt(gamma * p_R / ro_R);
+    const double e_L = p_L / (gamma - 1.0) + ro_L * Vkv_L / 2.0;
+    const double e_R = p_R / (gamma - 1.0) + ro_R * Vkv_R / 2.0;
+    const double Vn_L = v1_L * n1 + v2_L * n2;
+    const double Vn_R = v1_R * n1 + v2_R * n2;
+    const double D_L = fmin(Vn_L, Vn_R) - fmax(c_L, c_R);
+    const double D_R = fmax(Vn_L, Vn_R) + fmax(c_L, c_R);
+    double t = 10000000;
+    t = fmin(t, krit * rad / fmax(fabs(D_L), fabs(D_R)));
+    /* kernel.cu:1384-1397 */
+    const double fl_1 = ro_L * v1_L * n1 + ro_L * v2_L * n2;
+    const double fl_2 = (ro_L * v1_L * v1_L + p_L) * n1 + ro_L * v1_L * v2_L * n2;
+    const double fl_3 = ro_L * v1_L * v2_L * n1 + (ro_L * v2_L * v2_L + p_L) * n2;
+    const double fl_5 = (e_L + p_L) * v1_L * n1 + (e_L + p_L) * v2_L * n2;
+    if (D_L > 0.0) {
+        Ps->x = fl_1; Pu->x = fl_2; Pu->y = fl_3; Ps->y = fl_5;
+        return t;
+    }
+    /* kernel.cu:1408-1421 */
+    const double fr_1 = ro_R * v1_R * n1 + ro_R * v2_R * n2;
+    const double fr_2 = (ro_R * v1_R * v1_R + p_R) * n1 + ro_R * v1_R * v2_R * n2;
+    const double fr_3 = ro_R * v1_R * v2_R * n1 + (ro_R * v2_R * v2_R + p_R) * n2;
+    const double fr_5 = (e_R + p_R) * v1_R * n1 + (e_R + p_R) * v2_R * n2;
+    if (D_R < 0.0) {
+        Ps->x = fr_1; Pu->x = fr_2; Pu->y = fr_3; Ps->y = fr_5;
+        return t;
+    }
+    /* kernel.cu:1432-1441 */
+    const double u_L = Vn_L, u_R = Vn_R;
+    const double D_C = ((D_R - u_R) * ro_R * u_R - (D_L - u_L) * ro_L * u_L - p_R + p_L) /
+                       ((D_R - u_R) * ro_R - (D_L - u_L) * ro_L);
+    const double roro_L = ro_L * ((D_L - u_L) / (D_L - D_C));
+    const double roro_R = ro_R * ((D_R - u_R) / (D_R - D_C));
+    const double P_T = (p_L * ro_R * (u_R - D_R) - p_R * ro_L * (u_L - D_L) -
+                        ro_L * ro_R * (u_L - D_L) * (u_R - D_R) * (u_R - u_L)) /
+                       (ro_R * (u_R - D_R) - ro_L * (u_L - D_L));
+    if (D_L <= 0.0 && D_C >= 0.0) {
+        /* kernel.cu:1443-1461 */
+        const double Vx = v1_L + (D_C - Vn_L) * n1;
+        const double Vy = v2_L + (D_C - Vn_L) * n2;
+        const double ee_L = P_T / (gamma - 1.0) + roro_L * (Vx * Vx + Vy * Vy) / 2.0;
+        Ps->x = D_L * (roro_L - ro_L) + fl_1;
+        Pu->x = D_L * (roro_L * Vx - ro_L * v1_L) + fl_2;
+        Pu->y = D_L * (roro_L * Vy - ro_L * v2_L) + fl_3;
+        Ps->y = D_L * (ee_L - e_L) + fl_5;
+    } else if (D_R >= 0.0 && D_C <= 0.0) {
+        /* kernel.cu:1463-1480 */
+        const double Vx = v1_R + (D_C - Vn_R) * n1;
+        const double Vy = v2_R + (D_C - Vn_R) * n2;
+        const double ee_R = P_T / (gamma - 1.0) + roro_R * (Vx * Vx + Vy * Vy) / 2.0;
+        Ps->x = D_R * (roro_R - ro_R) + fr_1;
+        Pu->x = D_R * (roro_R * Vx - ro_R * v1_R) + fr_2;
+        Pu->y = D_R * (roro_R * Vy - ro_R * v2_R) + fr_3;
+        Ps->y = D_R * (ee_R - e_R) + fr_5;
+    } else {
+        /* no branch (NaN input): the reference leaves the flux unset */
+        diag_inc(&g_diag.bad_branch);
+        Ps->x = Ps->y = Pu->x = Pu->y = NAN;
+    }
+    return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLCQ_Korolkov_2D, Gas_Dinamic.cu:359-464 (HLLC_Korolkov_2D, 265-358, is  */
+/* the same arithmetic without the scalar: PQ = NULL). Face frame (n, t =     */
+/* (-n2, n1)), outer speeds min(uL - cL, uR - cR) / max(uL + cL, uR + cR),    */
+/* the one-sided star pressure pp = p1 + rho1 (SL - uL)(SM - uL), the star    */
+/* energy from the Rankine-Hugoniot condition, SM <= 0 on the right.          */
+/* PQ: flux of the passive scalar Q carried with the density.                 */
+/* ------------------------------------------------------------------------ */
+double o_hllcq_korolkov_2d(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru, double Q_L,
+                           double Q_R, double n1, double n2, o_d2* Ps, o_d2* Pu, double* PQ, double rad)
+{
+    const double g1 = gamma - 1.0;
+    const double ro1 = Ls.x, u1 = Lu.x, v1 = Lu.y, p1 = Ls.y;
+    const double ro2 = Rs.x, u2 = Ru.x, v2 = Ru.y, p2 = Rs.y;
+    const double t1 = -n2, t2 = n1;
+    /* Gas_Dinamic.cu:383-400 */
+    const double u_L = u1 * n1 + v1 * n2, v_L = u1 * t1 + v1 * t2;
+    const double u_R = u2 * n1 + v2 * n2, v_R = u2 * t1 + v2 * t2;
+    const double cL = sqrt(gamma * p1 / ro1);
+    const double cR = sqrt(gamma * p2 / ro2);
+    const double SL = fmin(u_L - cL, u_R - cR);
+    const double SR = fmax(u_L + cL, u_R + cR);
+    const double UU = fmax(fabs(SL), fabs(SR));
+    const double time = krit * rad / UU;
+    /* Gas_Dinamic.cu:405-409 */
+    const double FL1 = ro1 * u_L * u_L + p1, FL2 = ro1 * u_L * v_L;
+    const double FR1 = ro2 * u_R * u_R + p2, FR2 = ro2 * u_R * v_R;
+    double q = 0.0;
+    if (SL >= 0.0) {
+        /* Gas_Dinamic.cu:411-419 */
+        q = Q_L * u_L;
+        Ps->x = ro1 * u_L;
+        Ps->y = (gamma * p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1)) * u_L;
+        Pu->x = FL1 * n1 + FL2 * t1;
+        Pu->y = FL1 * n2 + FL2 * t2;
+    } else if (SR <= 0.0) {
+        /* Gas_Dinamic.cu:420-428 */
+        q = Q_R * u_R;
+        Ps->x = ro2 * u_R;
+        Ps->y = (gamma * p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2)) * u_R;
+        Pu->x = FR1 * n1 + FR2 * t1;
+        Pu->y = FR1 * n2 + FR2 * t2;
+    } else {
+        /* Gas_Dinamic.cu:429-431 */
+        const double SM = ((SR - u_R) * ro2 * u_R - (SL - u_L) * ro1 * u_L - p2 + p1) /
+                          ((SR - u_R) * ro2 - (SL - u_L) * ro1);
+        const double pp = p1 + ro1 * (SL - u_L) * (SM - u_L);
+        if (SM <= 0.0) {
+            /* Gas_Dinamic.cu:433-447 */
+            const double rr = ro2 * (SR - u_R) / (SR - SM);
+            const double e = p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2);
+            const double ee = ((SR - u_R) * e - p2 * u_R + pp * SM) / (SR - SM);
+            q = SR * (rr * Q_R / ro2 - Q_R) + Q_R * u_R;
+            Ps->x = SR * (rr - ro2) + ro2 * u_R;
+            Ps->y = SR * (ee - e) + (gamma * p2 / g1 + 0.5 * ro2 * (u2 * u2 + v2 * v2)) * u_R;
+            const double F1 = FR1 + SR * (rr * SM - ro2 * u_R);
+            const double F2 = FR2 + SR * (rr * v_R - ro2 * v_R);
+            Pu->x = F1 * n1 + F2 * t1;
+            Pu->y = F1 * n2 + F2 * t2;
+        } else if (SM >= 0.0) {
+            /* Gas_Dinamic.cu:449-462 */
+            const double rr = ro1 * (SL - u_L) / (SL - SM);
+            const double e = p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1);
+            const double ee = ((SL - u_L) * e - p1 * u_L + pp * SM) / (SL - SM);
+            q = SL * (rr * Q_L / ro1 - Q_L) + Q_L * u_L;
+            Ps->x = SL * (rr - ro1) + ro1 * u_L;
+            Ps->y = SL * (ee - e) + (gamma * p1 / g1 + 0.5 * ro1 * (u1 * u1 + v1 * v1)) * u_L;
+            const double F1 = FL1 + SL * (rr * SM - ro1 * u_L);
+            const double F2 = FL2 + SL * (rr * v_L - ro1 * v_L);
+            Pu->x = F1 * n1 + F2 * t1;
+            Pu->y = F1 * n2 + F2 * t2;
+        } else {
+            diag_inc(&g_diag.bad_branch);
+            q = NAN;
+            Ps->x = Ps->y = Pu->x = Pu->y = NAN;
+        }
+    }
+    if (PQ) *PQ = q;
+    return time;
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLCQ_Aleksashov, Gas_Dinamic.cu:465-785, as the reference calls it: B = 0 */
+/* (bx1..bz2 = 0, Gas_Dinamic.cu:484-499). Then d = 0 picks the frame        */
+/* (n, e_z, (-n2, n1)) (Gas_Dinamic.cu:525-552), the fast speeds equal c     */
+/* (sqrt(c*c) = c in IEEE arithmetic), every magnetic term is an exact zero, */
+/* the repeated speed block (Gas_Dinamic.cu:592-608) recomputes the same     */
+/* values, and |UZ[5]| < 1e-9 keeps the tangential velocities: the gas       */
+/* fluxes are HLLC_Aleksashov's (o_hllc) term for term. PQ (Gas_Dinamic.cu:   */
+/* 740-772) carries the passive scalar with the density; wv = 0.             */
+/* ------------------------------------------------------------------------ */
+double o_hllcq_aleksashov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru, double Q_L,
+                          double Q_R, double n1, double n2, o_d2* Ps, o_d2* Pu, double* PQ, double rad)
+{
+    const double time = o_hllc(gamma, krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    /* the scalar's branch quantities, as in Gas_Dinamic.cu:569-579, 676-686 */
+    const double r1 = Ls.x, p1 = Ls.y, r2 = Rs.x, p2 = Rs.y;
+    const double vL0 = n1 * Lu.x + n2 * Lu.y, vR0 = n1 * Ru.x + n2 * Ru.y;
+    const double cL = sqrt(gamma * p1 / r1), cR = sqrt(gamma * p2 / r2);
+    const double SL = fmin(vL0 - cL, vR0 - cR);
+    const double SR = fmax(vL0 + cL, vR0 + cR);
+    const double suR = SR - vR0, suL = SL - vL0;
+    const double SM = (suR * r2 * vR0 - p2 + p1 - suL * r1 * vL0) / (suR * r2 - suL * r1);
+    const double rzR = r2 * (suR / (SR - SM)), rzL = r1 * (suL / (SL - SM));
+    const double FL0 = Q_L * vL0, FR0 = Q_R * vR0;
+    double q;
+    if (SL > 0.0) q = FL0;
+    else if (SL <= 0.0 && SM >= 0.0) q = FL0 + SL * (rzL * Q_L / r1 - Q_L);
+    else if (SM <= 0.0 && SR >= 0.0) q = FR0 + SR * (rzR * Q_R / r2 - Q_R);
+    else if (SR < 0.0) q = FR0;
+    else q = NAN;
+    if (PQ) *PQ = q;
+    return time;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Exact Riemann solver: Godunov_Solver_Alexashov + newton + devtwo + lev     */
 /* (kernel.cu:214-656, 721-851), gamma as a runtime value.                    */
 /* ------------------------------------------------------------------------ */

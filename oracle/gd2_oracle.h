@@ -69,6 +69,16 @@ double o_hllc(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
               double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
 double o_godunov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
                  double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
+/* the remaining HLLC variants (kernel.cu:1167, 1349; Gas_Dinamic.cu:265, 359,
+ * 465); the Q variants also return the passive-scalar flux in *PQ (may be NULL) */
+double o_hllc_aleksashov2(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+                          double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
+double o_hllc_korolkov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
+                       double n1, double n2, o_d2* Ps, o_d2* Pu, double rad);
+double o_hllcq_korolkov_2d(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru, double Q_L,
+                           double Q_R, double n1, double n2, o_d2* Ps, o_d2* Pu, double* PQ, double rad);
+double o_hllcq_aleksashov(double gamma, double krit, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru, double Q_L,
+                          double Q_R, double n1, double n2, o_d2* Ps, o_d2* Pu, double* PQ, double rad);
 /* star pressure / velocity of the exact solver, in the caller's frame along n */
 void   o_godunov_star(double gamma, o_d2 Ls, o_d2 Lu, o_d2 Rs, o_d2 Ru,
                       double n1, double n2, double* pstar, double* ustar, int* pattern);

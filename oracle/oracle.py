@@ -67,10 +67,14 @@ def lib():
         L.o_linear.argtypes = [C.c_double] * 7
         L.o_linear2.restype = None
         L.o_linear2.argtypes = [C.c_double] * 8 + [dp, dp]
-        for fn in (L.o_hllc, L.o_godunov, L.o_hll):
+        for fn in (L.o_hllc, L.o_godunov, L.o_hll, L.o_hllc_aleksashov2, L.o_hllc_korolkov):
             fn.restype = C.c_double
             fn.argtypes = [C.c_double, C.c_double, D2, D2, D2, D2, C.c_double, C.c_double,
                            C.POINTER(D2), C.POINTER(D2), C.c_double]
+        for fn in (L.o_hllcq_korolkov_2d, L.o_hllcq_aleksashov):
+            fn.restype = C.c_double
+            fn.argtypes = [C.c_double, C.c_double, D2, D2, D2, D2, C.c_double, C.c_double, C.c_double,
+                           C.c_double, C.POINTER(D2), C.POINTER(D2), dp, C.c_double]
         L.o_godunov_star.restype = None
         L.o_godunov_star.argtypes = [C.c_double, D2, D2, D2, D2, C.c_double, C.c_double, dp, dp,
                                      C.POINTER(C.c_int)]
@@ -152,13 +156,33 @@ def linear2(x1, t1, x2, t2, x3, t3, y1, y2):
     return A.value, B.value
 
 
+EULER_FLUX_KINDS = ("hllc", "hll", "godunov", "exact", "hllc_aleksashov2", "hllc_korolkov", "hllc_korolkov_2d",
+                    "hllcq_korolkov_2d", "hllcq_aleksashov")
+
+
 def euler_flux(kind, gamma, krit, L, R, n, rad=1.0):
     """L, R = (rho, p, u, v). Returns (F_rho, F_E, F_mx, F_my), dt_face."""
+    if kind in ("hllc_korolkov_2d", "hllcq_korolkov_2d", "hllcq_aleksashov"):
+        F, _, t = euler_flux_q(kind, gamma, krit, L, R, 1.0, 1.0, n, rad)
+        return F, t
     Ps, Pu = D2(), D2()
-    fn = {"hllc": lib().o_hllc, "hll": lib().o_hll}.get(kind, lib().o_godunov)
+    fn = {"hllc": lib().o_hllc, "hll": lib().o_hll, "godunov": lib().o_godunov, "exact": lib().o_godunov,
+          "hllc_aleksashov2": lib().o_hllc_aleksashov2, "hllc_korolkov": lib().o_hllc_korolkov}[kind]
     t = fn(gamma, krit, D2(L[0], L[1]), D2(L[2], L[3]), D2(R[0], R[1]), D2(R[2], R[3]),
            float(n[0]), float(n[1]), C.byref(Ps), C.byref(Pu), rad)
     return (Ps.x, Ps.y, Pu.x, Pu.y), t
+
+
+def euler_flux_q(kind, gamma, krit, L, R, QL, QR, n, rad=1.0):
+    """Passive-scalar variants (HLLCQ_Korolkov_2D, HLLCQ_Aleksashov; kind
+    hllc_korolkov_2d is the former without the scalar). Returns
+    ((F_rho, F_E, F_mx, F_my), F_Q, dt_face)."""
+    Ps, Pu, PQ = D2(), D2(), C.c_double()
+    fn = {"hllc_korolkov_2d": lib().o_hllcq_korolkov_2d, "hllcq_korolkov_2d": lib().o_hllcq_korolkov_2d,
+          "hllcq_aleksashov": lib().o_hllcq_aleksashov}[kind]
+    t = fn(gamma, krit, D2(L[0], L[1]), D2(L[2], L[3]), D2(R[0], R[1]), D2(R[2], R[3]), float(QL), float(QR),
+           float(n[0]), float(n[1]), C.byref(Ps), C.byref(Pu), C.byref(PQ), rad)
+    return (Ps.x, Ps.y, Pu.x, Pu.y), PQ.value, t
 
 
 def godunov_star(gamma, L, R, n=(1.0, 0.0)):

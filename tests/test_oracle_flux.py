@@ -206,6 +206,129 @@ def test_hllc_face_timestep_closed_form():
     assert abs(t - 0.4 * 0.01 / S) < 1e-16
 
 
+# ------------------------------------------------- the remaining HLLC variants
+
+def _face_pair(L, R, n):
+    t = (-n[1], n[0])
+    un = lambda q: q[2] * n[0] + q[3] * n[1]
+    ut = lambda q: q[2] * t[0] + q[3] * t[1]
+    return (L[0], L[1], un(L), ut(L)), (R[0], R[1], un(R), ut(R))
+
+
+def _lab4(Fb, n):
+    """face-frame (rho, n, t, E) -> the oracle's (F_rho, F_E, F_mx, F_my)"""
+    t = (-n[1], n[0])
+    return (Fb[0], Fb[3], Fb[1] * n[0] + Fb[2] * t[0], Fb[1] * n[1] + Fb[2] * t[1])
+
+
+def _close(F, exp, tol):
+    scale = max(abs(v) for v in exp) + 1e-12
+    return max(abs(a - b) for a, b in zip(F, exp)) <= tol * scale
+
+
+@pytest.mark.parametrize("kind", ["hllc_korolkov_2d", "hllcq_korolkov_2d", "hllcq_aleksashov"])
+def test_hllc_variants_that_are_toro_hllc(kind):
+    """HLLC_Korolkov_2D / HLLCQ_Korolkov_2D (Gas_Dinamic.cu:265, 359; one-sided
+    star pressure, S_M <= 0 on the right) and HLLCQ_Aleksashov with B = 0
+    (Gas_Dinamic.cu:465) against Toro's HLLC with Davis speeds."""
+    rng = random.Random(31)
+    for gamma in (1.4, 5.0 / 3.0):
+        for i in range(2000):
+            L, R = _rand_state(rng), _rand_state(rng)
+            if i % 9 == 0:  # supersonic pairs
+                sh = 4.0 if i % 2 else -4.0
+                L, R = (L[0], L[1], L[2] + sh, L[3] + sh), (R[0], R[1], R[2] + sh, R[3] + sh)
+            n = NORMALS[i % 4]
+            F, _ = O.euler_flux(kind, gamma, 0.25, L, R, n)
+            fl, fr = _face_pair(L, R, n)
+            assert _close(F, _lab4(TB.toro_hllc(gamma, fl, fr), n), 1e-12), (kind, i)
+
+
+@pytest.mark.parametrize("kind", ["hllcq_korolkov_2d", "hllcq_aleksashov"])
+def test_hllcq_scalar_is_carried_with_the_mass_flux(kind):
+    """The passive-scalar flux equals the mass flux times the concentration
+    Q/rho of the state on the upwind side of the contact (the star states
+    carry rho*_K Q_K / rho_K)."""
+    rng = random.Random(32)
+    for i in range(3000):
+        L, R = _rand_state(rng), _rand_state(rng)
+        QL, QR = rng.uniform(0.1, 3.0), rng.uniform(0.1, 3.0)
+        n = NORMALS[i % 4]
+        F, PQ, _ = O.euler_flux_q(kind, 1.4, 0.25, L, R, QL, QR, n)
+        side = TB.hllc_contact_side(1.4, *_face_pair(L, R, n))
+        conc = QL / L[0] if side > 0 else QR / R[0]
+        assert abs(PQ - conc * F[0]) <= 1e-13 * (abs(PQ) + abs(conc * F[0]) + 1e-12), (kind, i)
+
+
+def test_hllc_korolkov_is_hllc_with_hll_bounds_and_eos_star_energy():
+    """HLLC_Korolkov (kernel.cu:1349): Toro's HLLC with the HLL flux's bounds
+    min(uL, uR) - max(cL, cR), max(uL, uR) + max(cL, cR); mass and momentum
+    do not depend on the star energy, the energy flux takes the star energy
+    from the equation of state at the star pressure (kernel.cu:1448, 1468)."""
+    rng = random.Random(33)
+    for gamma in (1.4, 5.0 / 3.0):
+        for i in range(2000):
+            L, R = _rand_state(rng), _rand_state(rng)
+            if i % 9 == 0:
+                sh = 4.0 if i % 2 else -4.0
+                L, R = (L[0], L[1], L[2] + sh, L[3] + sh), (R[0], R[1], R[2] + sh, R[3] + sh)
+            n = NORMALS[i % 4]
+            F, _ = O.euler_flux("hllc_korolkov", gamma, 0.25, L, R, n)
+            fl, fr = _face_pair(L, R, n)
+            exp = _lab4(TB.toro_hllc(gamma, fl, fr, speeds="simple", energy="eos"), n)
+            assert _close(F, exp, 1e-12), i
+            # the RH energy closure differs where the star region is sampled
+            rh = _lab4(TB.toro_hllc(gamma, fl, fr, speeds="simple", energy="rh"), n)
+            assert _close((F[0], F[2], F[3]), (rh[0], rh[2], rh[3]), 1e-12)
+
+
+def test_hllc_korolkov_consistency_and_stationary_contact():
+    rng = random.Random(34)
+    for _ in range(500):
+        q = _rand_state(rng)
+        for n in NORMALS:
+            F, _ = O.euler_flux("hllc_korolkov", 1.4, 0.25, q, q, n)
+            un = q[2] * n[0] + q[3] * n[1]
+            E = q[1] / 0.4 + 0.5 * q[0] * (q[2] ** 2 + q[3] ** 2)
+            exp = (q[0] * un, un * (E + q[1]), q[0] * un * q[2] + q[1] * n[0], q[0] * un * q[3] + q[1] * n[1])
+            assert rel(F, exp) < 1e-13
+    for n in NORMALS:
+        t = (-n[1], n[0])
+        L = (1.0, 0.7, 0.3 * t[0], 0.3 * t[1])
+        R = (0.2, 0.7, -0.5 * t[0], -0.5 * t[1])
+        F, _ = O.euler_flux("hllc_korolkov", 1.4, 0.25, L, R, n)
+        assert abs(F[0]) < 1e-15 and abs(F[1]) < 1e-15
+        assert abs(F[2] - 0.7 * n[0]) < 1e-15 and abs(F[3] - 0.7 * n[1]) < 1e-15
+
+
+def test_hllc_aleksashov2_contact_speed_takes_the_left_tangential_velocity():
+    """HLLC_Aleksashov2 (kernel.cu:1167) is Toro's HLLC with Davis speeds,
+    except that the left momentum term of its contact speed carries the left
+    tangential velocity (kernel.cu:1222): equal to Toro's HLLC where that
+    velocity equals the normal one, and everywhere to the HLLC with that
+    substitution whose star energies take the mean of the two one-sided star
+    pressures (kernel.cu:1269-1276; they differ once S* is not eq. 10.37). It
+    is then not consistent (F(U, U) != F(U) when ut != un)."""
+    rng = random.Random(35)
+    for gamma in (1.4, 5.0 / 3.0):
+        for i in range(2000):
+            L, R = _rand_state(rng), _rand_state(rng)
+            n = NORMALS[i % 4]
+            t = (-n[1], n[0])
+            if i % 2 == 0:  # left tangential velocity = left normal velocity
+                w = L[2] * n[0] + L[3] * n[1]
+                L = (L[0], L[1], w * (n[0] + t[0]), w * (n[1] + t[1]))
+            F, _ = O.euler_flux("hllc_aleksashov2", gamma, 0.25, L, R, n)
+            fl, fr = _face_pair(L, R, n)
+            if i % 2 == 0:
+                assert _close(F, _lab4(TB.toro_hllc(gamma, fl, fr), n), 1e-12), i
+            exp = TB.toro_hllc(gamma, fl, fr, sm_left_u=fl[3], pstar="mean")
+            assert _close(F, _lab4(exp, n), 1e-11), i
+    q = (1.0, 1.0, 0.3, 0.8)
+    F, _ = O.euler_flux("hllc_aleksashov2", 1.4, 0.25, q, q, (1.0, 0.0))
+    assert abs(F[0] - 0.3) > 1e-3
+
+
 # ------------------------------------------------------------ HLLD / HLLC-MHD
 
 SPI4 = math.sqrt(4.0 * math.pi)

@@ -133,20 +133,50 @@ def toro_sample0(gamma, L, R):
     return r, pR * (c / cR) ** (2 * g / (g - 1)), u
 
 
-def toro_hllc(gamma, L, R):
-    """L, R = (rho, p, un, ut). Returns normal-frame flux (F_rho, F_n, F_t, F_E)."""
+def toro_hllc(gamma, L, R, speeds="davis", energy="rh", sm_left_u=None, pstar="side"):
+    """L, R = (rho, p, un, ut). Returns normal-frame flux (F_rho, F_n, F_t, F_E).
+
+    Toro's HLLC (ch. 10.4): contact speed S* (eq. 10.37), star states
+    rho*_K = rho_K (S_K - u_K)/(S_K - S*), normal velocity S*, tangential
+    velocity unchanged, flux F_K + S_K (U*_K - U_K) on either side of the
+    contact. Options for the variants that differ from it:
+      speeds = "davis": S_L = min(uL - cL, uR - cR), S_R = max(uL + cL, uR + cR);
+               "simple": S_L = min(uL, uR) - max(cL, cR), S_R = max(uL, uR) +
+               max(cL, cR) (Davis's simplest bounds, those of the HLL flux);
+      energy = "rh": E*_K from the Rankine-Hugoniot condition across S_K,
+               E*_K = ((S_K - u_K) E_K - p_K u_K + p*_K S*) / (S_K - S*) (eq.
+               10.39 rearranged); "eos": E*_K = p*/(gamma-1) + rho*_K (S*^2 +
+               ut_K^2)/2 with p* = pL + rhoL (S_L - uL)(S* - uL);
+      pstar = "side": p*_K = p_K + rho_K (S_K - u_K)(S* - u_K), the two being
+               equal when S* is eq. 10.37; "mean": both sides take the mean of
+               the two one-sided values;
+      sm_left_u: replaces uL in the left momentum term rho_L uL (S_L - uL) of
+               the numerator of S* (a variant that puts another velocity there).
+    """
     rL, pL, uL, vL = L
     rR, pR, uR, vR = R
     cL, cR = math.sqrt(gamma * pL / rL), math.sqrt(gamma * pR / rR)
-    SL, SR = min(uL - cL, uR - cR), max(uL + cL, uR + cR)
-    Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
+    if speeds == "davis":
+        SL, SR = min(uL - cL, uR - cR), max(uL + cL, uR + cR)
+    else:
+        SL, SR = min(uL, uR) - max(cL, cR), max(uL, uR) + max(cL, cR)
+    uLm = uL if sm_left_u is None else sm_left_u
+    Ss = (pR - pL + rL * uLm * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
     FL = euler_phys_flux(gamma, rL, pL, uL, vL)
     FR = euler_phys_flux(gamma, rR, pR, uR, vR)
+    ps = pL + rL * (SL - uL) * (Ss - uL)
+    psR = pR + rR * (SR - uR) * (Ss - uR)
+    pmean = 0.5 * (ps + psR)
 
     def ustar(r, p, u, v, S):
         E = p / (gamma - 1) + 0.5 * r * (u * u + v * v)
         f = r * (S - u) / (S - Ss)
-        return (f, f * Ss, f * v, f * (E / r + (Ss - u) * (Ss + p / (r * (S - u)))))
+        if energy == "rh":
+            pk = pmean if pstar == "mean" else p + r * (S - u) * (Ss - u)
+            Es = ((S - u) * E - p * u + pk * Ss) / (S - Ss)
+        else:
+            Es = ps / (gamma - 1) + 0.5 * f * (Ss * Ss + v * v)
+        return (f, f * Ss, f * v, Es)
 
     def cons(r, p, u, v):
         return (r, r * u, r * v, p / (gamma - 1) + 0.5 * r * (u * u + v * v))
@@ -162,6 +192,16 @@ def toro_hllc(gamma, L, R):
     else:
         F = FR
     return (F[0], F[1], F[2], F[3])
+
+
+def hllc_contact_side(gamma, L, R):
+    """+1 when x/t = 0 lies left of the contact of Toro's HLLC (Davis speeds), -1 right."""
+    rL, pL, uL, _ = L
+    rR, pR, uR, _ = R
+    cL, cR = math.sqrt(gamma * pL / rL), math.sqrt(gamma * pR / rR)
+    SL, SR = min(uL - cL, uR - cR), max(uL + cL, uR + cR)
+    Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
+    return 1 if Ss > 0 else -1
 
 
 # ------------------------------------------------------------------ MHD
