@@ -24,6 +24,10 @@ BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
 BC_SYMMETRY, BC_AXIS, BC_OUTFLOW, BC_INFLOW = 4, 5, 6, 7
 EULER, MHD = 0, 1
 FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
+FLUX_HLLC_KOROLKOV, FLUX_HLLC_2D = 6, 7  # HLLC_Korolkov (kernel.cu:1349), HLLC_Korolkov_2D (Gas_Dinamic.cu:265)
+# pointwise flux kinds of gd2_euler_face_fluxes
+(FACE_HLL, FACE_HLLC_ALEKSASHOV, FACE_GODUNOV, FACE_HLLC_ALEKSASHOV2, FACE_HLLC_KOROLKOV, FACE_HLLC_KOROLKOV_2D,
+ FACE_HLLCQ_KOROLKOV_2D, FACE_HLLCQ_ALEKSASHOV) = range(8)
 FORWARD_EULER, SSPRK2 = 1, 2
 
 _lib = None
@@ -110,6 +114,8 @@ def lib():
             "gd2_vjp_workspace_bytes": (C.c_longlong, [gp, sp]),
             "gd2_stage_vjp": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp, vp]),
             "gd2_stage_vjp_noalloc": (ip, [gp, sp, vp, vp, vp, dp, vp, dp, dp, vp, vp]),
+            "gd2_euler_face_fluxes": (ip, [ip, dp, dp, dp, C.c_longlong] + [vp] * 11),
+            "gd2_mhd_face_fluxes": (ip, [ip, dp, dp, dp, C.c_longlong] + [vp] * 6),
             "gd2_set_step_path": (ip, [ip]),
             "gd2_one_pass_preferred": (ip, [gp, sp]),
             "gd2_write_state_text": (ip, [gp, vp, vp, vp, C.c_char_p]),
@@ -133,7 +139,7 @@ EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
             "gd2_ensemble_advance", "gd2_ensemble_time", "gd2_ensemble_tune", "gd2_ctx_log_dt",
-            "gd2_ctx_time_kernels", "gd2_ctx_kernel_ms", "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_stage_vjp_noalloc", "gd2_ssprk2_step", "gd2_set_step_path",
+            "gd2_ctx_time_kernels", "gd2_ctx_kernel_ms", "gd2_vjp_workspace_bytes", "gd2_stage_vjp", "gd2_stage_vjp_noalloc", "gd2_euler_face_fluxes", "gd2_mhd_face_fluxes", "gd2_ssprk2_step", "gd2_set_step_path",
             "gd2_one_pass_preferred", "gd2_write_state_text", "gd2_read_state_text", "gd2_write_state_b3",
             "gd2_write_tecplot", "gd2_write_tecplot_derived", "gd2_write_line_cuts"]
 
@@ -281,6 +287,40 @@ def stage_vjp_noalloc(g, sc, U, V, W, c=1.0, A=None, a=0.0, B=None, b=0.0, strea
             raise GD2Error(f"state shape {tuple(t.shape)} is not ({nf}, {g.ny}, {g.nx})")
     _check(lib().gd2_stage_vjp_noalloc(C.byref(g), C.byref(sc), _f64(U), _f64(V), _f64(A), float(a), _f64(B),
                                        float(b), float(c), _f64(W), _stream(stream)), "gd2_stage_vjp_noalloc")
+
+
+def euler_face_fluxes(kind, gamma, krit, Ls, Lu, Rs, Ru, normal, rad=1.0, QL=None, QR=None, stream=None):
+    """The reference's Euler flux function `kind` (FACE_*) on n faces: Ls, Lu,
+    Rs, Ru, normal of shape (n, 2) (rho, p) / (vx, vy) / (n1, n2); QL, QR of
+    shape (n,) for the passive-scalar kinds. Returns Ps (n, 2) = (F_rho, F_E),
+    Pu (n, 2) = (F_mx, F_my), PQ (n,) or None, dt (n,)."""
+    n = Ls.shape[0]
+    for t in (Ls, Lu, Rs, Ru, normal):
+        if tuple(t.shape) != (n, 2):
+            raise GD2Error("face arrays of shape (n, 2) expected")
+    q = kind in (FACE_HLLCQ_KOROLKOV_2D, FACE_HLLCQ_ALEKSASHOV)
+    Ps, Pu = torch.empty_like(Ls), torch.empty_like(Ls)
+    dt = torch.empty(n, dtype=torch.float64, device=Ls.device)
+    PQ = torch.empty(n, dtype=torch.float64, device=Ls.device) if q else None
+    _check(lib().gd2_euler_face_fluxes(int(kind), float(gamma), float(krit), float(rad), n, _f64(Ls), _f64(Lu),
+                                       _f64(Rs), _f64(Ru), _f64(QL) if q else None, _f64(QR) if q else None,
+                                       _f64(normal), _f64(Ps), _f64(Pu), _f64(PQ), _f64(dt), _stream(stream)),
+           "gd2_euler_face_fluxes")
+    return Ps, Pu, PQ, dt
+
+
+def mhd_face_fluxes(metod, gamma, krit, L, R, normal, rad=1.0, stream=None):
+    """HLLDQ_Korolkov metod 2 / 3 on n faces: L, R of shape (n, 8) (rho, p,
+    vx, vy, vz, Bx, By, Bz; Gaussian B), normal (n, 3). Returns P (n, 8) =
+    (F_rho, F_mx, F_my, F_mz, F_Bx, F_By, F_Bz, F_E) and dt (n,)."""
+    n = L.shape[0]
+    if tuple(L.shape) != (n, 8) or tuple(R.shape) != (n, 8) or tuple(normal.shape) != (n, 3):
+        raise GD2Error("face arrays of shape (n, 8) and (n, 3) expected")
+    P = torch.empty_like(L)
+    dt = torch.empty(n, dtype=torch.float64, device=L.device)
+    _check(lib().gd2_mhd_face_fluxes(int(metod), float(gamma), float(krit), float(rad), n, _f64(L), _f64(R),
+                                     _f64(normal), _f64(P), _f64(dt), _stream(stream)), "gd2_mhd_face_fluxes")
+    return P, dt
 
 
 def bits_to_double(t):

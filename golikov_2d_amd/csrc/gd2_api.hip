@@ -15,6 +15,7 @@
 
 #include "../../include/gd2.h"
 #include "gd2_euler.hpp"
+#include "gd2_launch.hpp"
 #include "gd2_rk2.hpp"
 #include "gd2_mhd_adjoint.hpp"
 #include "gd2_mhd.hpp"
@@ -108,6 +109,8 @@ static bool grid_ok(const gd2_grid* g, bool allow_frozen, int prod = 0) {
 
 int hip_error_code(hipError_t e) { return e == hipSuccess ? GD2_OK : hip_fail(e); }
 
+DeviceDiag* device_diag() { return diag_ptr(); }
+
 // Rows per wave strip. Default: long enough to amortise the 4-row y-halo,
 // short enough to give every CU several waves; GD2_TILE_ROWS overrides it.
 int tile_rows_for(int nx, int ny) {
@@ -193,23 +196,6 @@ static void fill_common(EulerArgs& A, const gd2_grid* g) {
     A.tiles_x = (g->nx + kTileW - 1) / kTileW;
 }
 
-static unsigned euler_blocks(const EulerArgs& A) {
-    return (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
-}
-
-template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE, bool PROD = false>
-static int launch_euler(const EulerArgs& A0, hipStream_t st) {
-    EulerArgs A = A0;
-    const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>);
-    // strips down to one row: on grids too small to fill the chip each wave's
-    // latency-bound march is the step's critical path
-    A.rows = strip_rows(k, A.tiles_x, A.ny, 2, 64, 1);
-    hipLaunchKernelGGL((euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>), dim3(euler_blocks(A)), dim3(kWave), 0,
-                       st, A);
-    GD2_HIP(hipGetLastError());
-    return GD2_OK;
-}
-
 template <int LAYOUT, bool W, bool BASE>
 static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t st) {
     if (order == 2) {
@@ -221,23 +207,8 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
         if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
         if (flux == GD2_FLUX_HLL) return launch_euler<1, GD2_FLUX_HLL, LAYOUT, W, BASE>(A, st);
     }
+    if (is_hllc_variant(flux)) return launch_euler_variant(order, flux, LAYOUT, W, BASE, false, A, st);
     return GD2_ERR_UNSUPPORTED;
-}
-
-// One-pass SSP-RK2 step: strips of 59 columns; the strip height balances
-// rounds of resident waves against the extra passes of each strip (four U1
-// rows below and above it and the lead-in, about three rows of work), up to
-// 256 rows so that a large grid fits in one round.
-template <int ORDER, int FLUX, bool KEEP>
-static int launch_rk2(const EulerArgs& A0, hipStream_t st) {
-    EulerArgs A = A0;
-    A.tiles_x = (A.nx + kRk2W - 1) / kRk2W;
-    const void* k = reinterpret_cast<const void*>(&euler_rk2_kernel<ORDER, FLUX, KEEP>);
-    A.rows = max(2, strip_rows(k, A.tiles_x, A.ny, 3, 256));  // strips start at row 0 or row >= 2
-    const unsigned blocks = (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
-    hipLaunchKernelGGL((euler_rk2_kernel<ORDER, FLUX, KEEP>), dim3(blocks), dim3(kWave), 0, st, A);
-    GD2_HIP(hipGetLastError());
-    return GD2_OK;
 }
 
 template <bool KEEP>
@@ -251,13 +222,15 @@ static int dispatch_rk2(int order, int flux, const EulerArgs& A, hipStream_t st)
         if (flux == GD2_FLUX_EXACT) return launch_rk2<1, GD2_FLUX_EXACT, KEEP>(A, st);
         if (flux == GD2_FLUX_HLL) return launch_rk2<1, GD2_FLUX_HLL, KEEP>(A, st);
     }
+    if (is_hllc_variant(flux)) return launch_rk2_variant(order, flux, KEEP, A, st);
     return GD2_ERR_UNSUPPORTED;
 }
 
 static bool scheme_ok(const gd2_scheme* sc) {
     if (!sc || (sc->order != 1 && sc->order != 2)) return false;
     if (sc->physics == GD2_EULER)
-        return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT || sc->flux == GD2_FLUX_HLL;
+        return sc->flux == GD2_FLUX_HLLC || sc->flux == GD2_FLUX_EXACT || sc->flux == GD2_FLUX_HLL ||
+               is_hllc_variant(sc->flux);
     if (sc->physics == GD2_MHD) return sc->flux == GD2_FLUX_HLLC_MHD || sc->flux == GD2_FLUX_HLLD;
     return false;
 }
@@ -501,7 +474,7 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     if (!grid_ok(g, true, order == 2 ? kProdOptions : 0) || !s || !u || !s2 || !u2 || !T || !T_do)
         return GD2_ERR_ARG;
     if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2) return GD2_ERR_ARG;
-    if (method != 0 && method != 1 && method != 2) return GD2_ERR_UNSUPPORTED;
+    if (method < 0 || method > 4) return GD2_ERR_UNSUPPORTED;
     EulerArgs A;
     fill_common(A, g);
     A.s = reinterpret_cast<const double2*>(s);
@@ -517,9 +490,14 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     A.r_frozen = g->r_frozen;
     A.axisym = g->axisym ? 1 : 0;
     if (!A.diag) return GD2_ERR_HIP;
-    const int flux = method == 2 ? GD2_FLUX_EXACT : (method == 0 ? GD2_FLUX_HLL : GD2_FLUX_HLLC);
+    // methods 3 and 4: the HLLC variants HLLC_Korolkov and HLLC_Korolkov_2D
+    // (no reference kernel calls them)
+    static const int method_flux[5] = {GD2_FLUX_HLL, GD2_FLUX_HLLC, GD2_FLUX_EXACT, GD2_FLUX_HLLC_KOROLKOV,
+                                       GD2_FLUX_HLLC_2D};
+    const int flux = method_flux[method];
     if (order == 2 && (A.axisym || A.r_frozen > 0.0)) {
         const hipStream_t st = (hipStream_t)stream;
+        if (is_hllc_variant(flux)) return launch_euler_variant(2, flux, kAoS, false, false, true, A, st);
         if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, kAoS, false, false, true>(A, st);
         if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, kAoS, false, false, true>(A, st);
         return launch_euler<2, GD2_FLUX_HLL, kAoS, false, false, true>(A, st);
@@ -1002,6 +980,8 @@ int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, doub
     if (!out) return GD2_ERR_ARG;
     *out = nullptr;
     if (n < 1 || !grids || !sc || !U || !scheme_ok(sc) || sc->physics != GD2_EULER) return GD2_ERR_ARG;
+    // the ensemble's stage kernels are built for the HLLC, exact and HLL fluxes
+    if (is_hllc_variant(sc->flux)) return GD2_ERR_UNSUPPORTED;
     if (sc->integrator != GD2_SSPRK2 || !(sc->cfl > 0.0)) return GD2_ERR_UNSUPPORTED;
     for (int p = 0; p < n; ++p)
         if (!grid_ok(&grids[p], false) || !U[p]) return GD2_ERR_ARG;

@@ -252,11 +252,14 @@ struct FluxE {
 // face frame: normal velocity un, tangential ut. Only the star state of the
 // side chosen by sign(SM) is formed (the reference forms both); the branch
 // choice SL > 0 / SL <= 0 <= SM / SM <= 0 <= SR / SR < 0 (kernel.cu:1119-1158)
-// becomes a select. smax returns max(|SL|, |SR|).
-template <bool GUARD = true>
+// becomes a select. smax returns max(|SL|, |SR|). With Q, *pq receives the
+// flux of a passive scalar Q carried with the density, as HLLCQ_Aleksashov
+// forms it with B = 0 (Gas_Dinamic.cu:740-772).
+template <bool GUARD = true, bool Q = false>
 __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, double pL,
                                             double rR, double unR, double utR, double pR,
-                                            double gamma, double inv_g1, double& smax) {
+                                            double gamma, double inv_g1, double& smax,
+                                            double QL = 0.0, double QR = 0.0, double* pq = nullptr) {
     const double cL = sqrt_ratio<GUARD>(gamma * pL, rL);
     const double cR = sqrt_ratio<GUARD>(gamma * pR, rR);
     const double SL = fmin(unL - cL, unR - cR);
@@ -292,6 +295,10 @@ __device__ __forceinline__ FluxE hllc_euler(double rL, double unL, double utL, d
     // (the tangential velocity is carried across the contact unchanged)
     F.ft = ut * F.fr;
     F.fe = (e + p) * un + Se * (ez - e);
+    if constexpr (Q) {
+        const double q = left ? QL : QR;
+        *pq = q * un + Se * (rz * q / r - q);
+    }
     return F;
 }
 
@@ -341,6 +348,170 @@ __device__ __forceinline__ FluxE hll_euler(double rL, double unL, double utL, do
     F.fn = left ? FL.fn : (right ? FR.fn : F.fn);
     F.ft = left ? FL.ft : (right ? FR.ft : F.ft);
     F.fe = left ? FL.fe : (right ? FR.fe : F.fe);
+    return F;
+}
+
+// HLLC_Korolkov (kernel.cu:1349-1483) in the face frame: the HLL solver's
+// outer speeds D_L = min(unL, unR) - max(cL, cR), D_R = max(unL, unR) +
+// max(cL, cR); contact speed D_C and star pressure P_T of the two-state
+// HLLC; the star state takes normal velocity D_C, keeps the tangential one
+// and takes its energy from the equation of state, P_T/(gamma-1) +
+// rho* (D_C^2 + ut^2)/2 (kernel.cu:1448, 1468). The reference's lab-frame
+// projections become face-frame components (a rotation: the same flux).
+// Branches D_L > 0 / D_R < 0 / D_C >= 0 (kernel.cu:1399-1480) are selects.
+template <bool GUARD = true>
+__device__ __forceinline__ FluxE hllc_korolkov_euler(double rL, double unL, double utL, double pL,
+                                                     double rR, double unR, double utR, double pR,
+                                                     double gamma, double inv_g1, double& smax) {
+    const double cL = sqrt_ratio<GUARD>(gamma * pL, rL);
+    const double cR = sqrt_ratio<GUARD>(gamma * pR, rR);
+    const double cm = fmax(cL, cR);
+    const double DL = fmin(unL, unR) - cm;
+    const double DR = fmax(unL, unR) + cm;
+    smax = fmax(fabs(DL), fabs(DR));
+    const double mL = (DL - unL) * rL, mR = (DR - unR) * rR;
+    const double smR = mR * unR, smL = mL * unL;
+    const double idm = fast_rcp(mR - mL);
+    const double DC = (smR - smL - pR + pL) * idm;
+    // P_T = (pL mR - pR mL + mL mR (unR - unL)) / (mR - mL)   (kernel.cu:1441)
+    const double PT = (pL * mR - pR * mL + (mL * mR) * (unR - unL)) * idm;
+    const bool left = DC >= 0.0;
+    const double S = left ? DL : DR;
+    const double r = left ? rL : rR;
+    const double un = left ? unL : unR;
+    const double ut = left ? utL : utR;
+    const double p = left ? pL : pR;
+    const bool super = (DL > 0.0) || (DR < 0.0);
+    const bool sup_left = DL > 0.0;
+    // a supersonic face takes the upwind state's physical flux
+    const double r2 = super ? (sup_left ? rL : rR) : r;
+    const double un2 = super ? (sup_left ? unL : unR) : un;
+    const double ut2 = super ? (sup_left ? utL : utR) : ut;
+    const double p2 = super ? (sup_left ? pL : pR) : p;
+    const double e = p2 * inv_g1 + r2 * (0.5 * (un2 * un2 + ut2 * ut2));
+    const double den = super ? 1.0 : (S - DC);
+    const double rz = r * ((S - un) * fast_rcp(den));
+    const double ez = PT * inv_g1 + rz * (0.5 * (DC * DC + ut * ut));
+    const double Se = super ? 0.0 : S;
+    const double run = r2 * un2;
+    FluxE F;
+    F.fr = run + Se * (rz - r2);
+    F.fn = run * un2 + p2 + Se * (rz * DC - run);
+    F.ft = run * ut2 + Se * (rz * ut2 - r2 * ut2);
+    F.fe = (e + p2) * un2 + Se * (ez - e);
+    return F;
+}
+
+// A reflective west or south wall face evaluated as F(ghost | interior) with
+// the face normal along +x / +y. HLLC_Korolkov's star energy (equation of
+// state, not the Rankine-Hugoniot condition) makes its flux jump at D_C = 0,
+// where a mirrored pair puts the wall face, and the reference evaluates that
+// face from the interior cell with the outward normal (kernel.cu:1803-1834,
+// 3554-3587): by the mirror symmetry of the pair, that is the flux above with
+// the mass, tangential momentum and energy negated. The other fluxes are
+// continuous at their contact, where both evaluations agree.
+template <int FLUX>
+__device__ __forceinline__ void wall_face_fix(FluxE& F, bool wall) {
+    if constexpr (FLUX == GD2_FLUX_HLLC_KOROLKOV) {
+        if (wall) {
+            F.fr = -F.fr;
+            F.ft = -F.ft;
+            F.fe = -F.fe;
+        }
+    }
+}
+
+// HLLC_Korolkov_2D / HLLCQ_Korolkov_2D (Gas_Dinamic.cu:265-464) in the face
+// frame: Toro's HLLC with S_L = min(uL - cL, uR - cR), S_R = max(uL + cL,
+// uR + cR), the one-sided star pressure pp = pL + rhoL (S_L - uL)(S_M - uL),
+// the star energy ((S - u) e - p u + pp S_M) / (S - S_M), and S_M <= 0 on the
+// right star state (Gas_Dinamic.cu:433). With Q, *pq receives the flux of the
+// passive scalar Q (carried with the density; Gas_Dinamic.cu:413-455).
+template <bool GUARD = true, bool Q = false>
+__device__ __forceinline__ FluxE hllc_toro_euler(double rL, double unL, double utL, double pL,
+                                                 double rR, double unR, double utR, double pR,
+                                                 double gamma, double inv_g1, double& smax,
+                                                 double QL = 0.0, double QR = 0.0, double* pq = nullptr) {
+    const double cL = sqrt_ratio<GUARD>(gamma * pL, rL);
+    const double cR = sqrt_ratio<GUARD>(gamma * pR, rR);
+    const double SL = fmin(unL - cL, unR - cR);
+    const double SR = fmax(unL + cL, unR + cR);
+    smax = fmax(fabs(SL), fabs(SR));
+    const double mL = (SL - unL) * rL, mR = (SR - unR) * rR;
+    const double smR = mR * unR, smL = mL * unL;
+    const double SM = (smR - smL - pR + pL) * fast_rcp(mR - mL);
+    const double pp = pL + mL * (SM - unL);
+    const bool sup_left = SL >= 0.0, sup_right = SR <= 0.0;
+    const bool super = sup_left || sup_right;
+    const bool left = super ? sup_left : (SM > 0.0);
+    const double S = left ? SL : SR;
+    const double r = left ? rL : rR;
+    const double un = left ? unL : unR;
+    const double ut = left ? utL : utR;
+    const double p = left ? pL : pR;
+    const double e = p * inv_g1 + r * (0.5 * (un * un + ut * ut));
+    const double inv = fast_rcp(super ? 1.0 : (S - SM));
+    const double su = S - un;
+    const double rr = r * su * inv;
+    const double ee = (su * e - p * un + pp * SM) * inv;
+    const double Se = super ? 0.0 : S;
+    const double run = r * un;
+    FluxE F;
+    F.fr = Se * (rr - r) + run;
+    F.fe = Se * (ee - e) + (e + p) * un;
+    F.fn = (run * un + p) + Se * (rr * SM - run);
+    F.ft = run * ut + Se * (rr * ut - r * ut);
+    if constexpr (Q) {
+        const double q = left ? QL : QR;
+        *pq = Se * (rr * q / r - q) + q * un;
+    }
+    return F;
+}
+
+// HLLC_Aleksashov2 (kernel.cu:1167-1348) in the frame (n, t = (-n2, n1)) of
+// its caller's normal: HLLC_Aleksashov's HLLC with B = 0, except that the
+// contact speed's numerator takes the left state's tangential velocity in
+// the term suL r1 vL[1] (kernel.cu:1222). That breaks the mirror symmetry
+// F(L, R, n) = -F(R, L, -n) and the consistency F(U, U) = F(U), so it is
+// offered pointwise (gd2_euler_face_fluxes), not as a stage flux.
+__device__ __forceinline__ FluxE hllc_aleksashov2_euler(double rL, double unL, double utL, double pL,
+                                                        double rR, double unR, double utR, double pR,
+                                                        double gamma, double inv_g1, double& smax) {
+    const double cL = sqrt(gamma * pL / rL);
+    const double cR = sqrt(gamma * pR / rR);
+    const double SL = fmin(unL - cL, unR - cR);
+    const double SR = fmax(unL + cL, unR + cR);
+    smax = fmax(fabs(SL), fabs(SR));
+    const double suR = SR - unR, suL = SL - unL;
+    const double dd = suR * rR - suL * rL;
+    const double SM = fabs(dd) > 0.0 ? (suR * rR * unR - pR + pL - suL * rL * utL) / dd : 0.0;
+    const double ptz = ((pR + rR * suR * (SM - unR)) + (pL + rL * suL * (SM - unL))) / 2.0;
+    const bool left = SM >= 0.0;
+    const double S = left ? SL : SR;
+    const double r = left ? rL : rR;
+    const double un = left ? unL : unR;
+    const double ut = left ? utL : utR;
+    const double p = left ? pL : pR;
+    const double su = left ? suL : suR;
+    // branch order of kernel.cu:1301-1341: SL > 0; SL <= 0 <= SM (left star,
+    // whatever SR is: this SM can lie outside [SL, SR]); SM <= 0 <= SR; SR < 0
+    const bool sup_left = SL > 0.0, sup_right = !sup_left && !left && SR < 0.0;
+    const bool super = sup_left || sup_right;
+    const double r2 = super ? (sup_left ? rL : rR) : r;
+    const double un2 = super ? (sup_left ? unL : unR) : un;
+    const double ut2 = super ? (sup_left ? utL : utR) : ut;
+    const double p2 = super ? (sup_left ? pL : pR) : p;
+    const double e = p2 * inv_g1 + r2 * (0.5 * (un2 * un2 + ut2 * ut2));
+    const double den = super ? 1.0 : (S - SM);
+    const double sm = su / den;
+    const double rz = r * sm;
+    const double ez = e * sm + (ptz * SM - p * un) / den;
+    const double Se = super ? 0.0 : S;
+    FluxE F;
+    F.fr = r2 * un2 + Se * (rz - r2);
+    F.fe = (e + p2) * un2 + Se * (ez - e);
+    F.fn = r2 * un2 * un2 + p2 + Se * (SM * rz - r2 * un2);
+    F.ft = r2 * un2 * ut2 + Se * (ut2 * rz - r2 * ut2);
     return F;
 }
 
@@ -597,7 +768,8 @@ __device__ __forceinline__ void atomic_min_nonneg(unsigned long long* dst, doubl
 // load per pass, which suits the one-pass step's long passes; the stage VJP's
 // short passes measured slower with it than with a plain parity alternation.)
 constexpr int kSimdSlots = 16;
-__device__ unsigned g_simd_passes[8 * 256 * 4 * kSimdSlots];
+// one table per translation unit (waves of one kernel share its table)
+static __device__ unsigned g_simd_passes[8 * 256 * 4 * kSimdSlots];
 
 struct SimdTurn {
     unsigned* slots;

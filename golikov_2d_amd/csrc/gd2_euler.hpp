@@ -160,6 +160,10 @@ __device__ __forceinline__ FluxE face_flux(const EulerArgs& A, double rL, double
         return exact_euler(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, smax, A.diag);
     else if constexpr (FLUX == GD2_FLUX_HLL)
         return hll_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
+    else if constexpr (FLUX == GD2_FLUX_HLLC_KOROLKOV)
+        return hllc_korolkov_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
+    else if constexpr (FLUX == GD2_FLUX_HLLC_2D)
+        return hllc_toro_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
     else
         return hllc_euler<GUARD>(rL, unL, utL, pL, rR, unR, utR, pR, A.gamma, A.inv_g1, smax);
 }
@@ -422,6 +426,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
                 Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
             }
+            wall_face_fix<FLUX>(Fl, c == 0 && A.bc[0] == GD2_BC_REFLECTIVE);
             tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
             // F(c+1/2) from the next lane
             Fr.fr = from_next_lane(Fl.fr);
@@ -467,6 +472,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             PrimE L = face_plus(v0.q, s0), R = face_minus(v1.q, s1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
         }
+        wall_face_fix<FLUX>(Gp, j == -1 && A.bc[2] == GD2_BC_REFLECTIVE);
         tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             v0 = v1;
@@ -571,7 +577,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
 }
 
 template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, bool PROD = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_FLUX_HLLC ? (LAYOUT == kAoS && ORDER == 2 ? 4 : 3) : 1))) void euler_tile_kernel(const EulerArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_HLLC || FLUX == GD2_FLUX_HLLC_KOROLKOV || FLUX == GD2_FLUX_HLLC_2D) ? (LAYOUT == kAoS && ORDER == 2 ? 4 : 3) : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     euler_tile<ORDER, FLUX, LAYOUT, WANT_W, BASE, PROD>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
@@ -579,7 +585,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FLUX == GD2_
 
 // Cell-centred wave-speed bound max((|u|+c)/dx + (|v|+c)/dy) of a SoA
 // Euler state, folded into *wmax_bits.
-__global__ __launch_bounds__(256) void euler_wavespeed_kernel(const double* U, long long n, long long plane,
+static __global__ __launch_bounds__(256) void euler_wavespeed_kernel(const double* U, long long n, long long plane,
                                                               double gamma, double g1, double idx, double idy,
                                                               unsigned long long* wmax_bits) {
     double w = 0.0;

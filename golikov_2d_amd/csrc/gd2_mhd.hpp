@@ -8,6 +8,7 @@
 #pragma once
 
 #include "gd2_euler.hpp"
+#include "gd2_host.hpp"
 
 namespace gd2 {
 
@@ -700,7 +701,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
     }
 }
 
-__global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, long long n, double gamma, double g1,
+static __global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, long long n, double gamma, double g1,
                                                             double idx, double idy, unsigned long long* wmax_bits) {
     double w = 0.0;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -727,9 +728,6 @@ __global__ __launch_bounds__(256) void mhd_wavespeed_kernel(const double* U, lon
 
 // ------------------------------------------------------------------ launchers
 
-int tile_rows_for(int nx, int ny);
-int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows = 64, int min_rows = 4);
-int hip_error_code(hipError_t e);
 
 inline void fill_mhd(MhdArgs& A, const gd2_grid* g, DeviceDiag* diag) {
     std::memset(&A, 0, sizeof A);

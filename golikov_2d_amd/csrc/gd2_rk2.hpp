@@ -199,6 +199,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
                 const PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
                 Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smx);
             }
+            wall_face_fix<FLUX>(Fl, c == 0 && A.bc[0] == GD2_BC_REFLECTIVE);
             Fr.fr = from_next_lane(Fl.fr);
             Fr.fn = from_next_lane(Fl.fn);
             Fr.ft = from_next_lane(Fl.ft);
@@ -221,6 +222,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
             const PrimE L = face_plus(pv0, ps0), R = face_minus(pv1, ps1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smx);
         }
+        wall_face_fix<FLUX>(Gp, r == -1 && A.bc[2] == GD2_BC_REFLECTIVE);
         if (!lead) {
             const Bal P = full_balance(X, pGm, Gp, dx);
             double o0 = out_plain(uc.a[0], dtdV, P.r);
@@ -273,6 +275,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
             const PrimE L = face_plus(qb, sb), R = face_minus(qc, sc);
             Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
         }
+        wall_face_fix<FLUX>(Fl, c == 0 && A.bc[0] == GD2_BC_REFLECTIVE);
         Fr.fr = from_next_lane(Fl.fr);
         Fr.fn = from_next_lane(Fl.fn);
         Fr.ft = from_next_lane(Fl.ft);
@@ -314,6 +317,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
             const PrimE L = face_plus(cw0, cs0), R = face_minus(cw1, cs1);
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
         }
+        wall_face_fix<FLUX>(Gp, j == -1 && A.bc[2] == GD2_BC_REFLECTIVE);
         if (upd && owns) {
             const Bal P = full_balance(X2, cGm, Gp, dx);
             double o0 = out_base(hj[0], bdt, P.r);
@@ -446,7 +450,7 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
 }
 
 template <int ORDER, int FLUX, bool KEEP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_HLLC || FLUX == GD2_FLUX_HLL) ? 2 : 1))) void euler_rk2_kernel(const EulerArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_EXACT) ? 1 : 2))) void euler_rk2_kernel(const EulerArgs A) {
     __shared__ Rk2Lds lds;
     if (A.clk && blockIdx.x == 0 && threadIdx.x == 0) clock_commit(A.clk, A.clk_slot, A.dt_log, A.dt_cap);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);

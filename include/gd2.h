@@ -135,7 +135,15 @@ int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);
  * cell-centred CFL step computed on the device.
  * ---------------------------------------------------------------------- */
 enum gd2_physics { GD2_EULER = 0, GD2_MHD = 1 };
-enum gd2_flux { GD2_FLUX_HLLC = 1, GD2_FLUX_EXACT = 2, GD2_FLUX_HLLC_MHD = 3, GD2_FLUX_HLLD = 4, GD2_FLUX_HLL = 5 };
+/* Euler fluxes: HLLC = HLLC_Aleksashov (kernel.cu:853), EXACT =
+ * Godunov_Solver_Alexashov (kernel.cu:214), HLL (kernel.cu:1485),
+ * HLLC_KOROLKOV = HLLC_Korolkov (kernel.cu:1349: the HLL flux's wave-speed
+ * bounds, star energy from the equation of state), HLLC_2D =
+ * HLLC_Korolkov_2D (Gas_Dinamic.cu:265: Toro's HLLC with one-sided star
+ * pressure). MHD fluxes: HLLC_MHD and HLLD = HLLDQ_Korolkov metod 2 and 3
+ * (Gas_Dinamic.cu:964). */
+enum gd2_flux { GD2_FLUX_HLLC = 1, GD2_FLUX_EXACT = 2, GD2_FLUX_HLLC_MHD = 3, GD2_FLUX_HLLD = 4, GD2_FLUX_HLL = 5,
+                GD2_FLUX_HLLC_KOROLKOV = 6, GD2_FLUX_HLLC_2D = 7 };
 enum gd2_integrator { GD2_FORWARD_EULER = 1, GD2_SSPRK2 = 2 };
 
 typedef struct gd2_scheme {
@@ -289,6 +297,42 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
 int gd2_stage_vjp_noalloc(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V,
                           const double* A, double a, const double* B, double b, double c, double* W,
                           void* stream);
+
+/* ----------------------------------------------------------------------
+ * The reference's interface fluxes evaluated pointwise: one GPU thread per
+ * face calls the flux function with the reference's own lab-frame arguments
+ * and outputs (its device-function flux interface, Header.h:41-65 and the
+ * flux functions of kernel.cu / Gas_Dinamic.cu). Arrays of n entries in
+ * device memory; rad is the face size passed as `rad`, and dt[i] receives the
+ * function's return value, the face time step krit * rad / S.
+ * ---------------------------------------------------------------------- */
+enum gd2_face_flux {
+    GD2_FACE_HLL = 0,               /* HLL, kernel.cu:1485 */
+    GD2_FACE_HLLC_ALEKSASHOV = 1,   /* HLLC_Aleksashov, kernel.cu:853 */
+    GD2_FACE_GODUNOV = 2,           /* Godunov_Solver_Alexashov (exact), kernel.cu:214 */
+    GD2_FACE_HLLC_ALEKSASHOV2 = 3,  /* HLLC_Aleksashov2, kernel.cu:1167 */
+    GD2_FACE_HLLC_KOROLKOV = 4,     /* HLLC_Korolkov, kernel.cu:1349 */
+    GD2_FACE_HLLC_KOROLKOV_2D = 5,  /* HLLC_Korolkov_2D, Gas_Dinamic.cu:265 */
+    GD2_FACE_HLLCQ_KOROLKOV_2D = 6, /* HLLCQ_Korolkov_2D, Gas_Dinamic.cu:359 (passive scalar) */
+    GD2_FACE_HLLCQ_ALEKSASHOV = 7   /* HLLCQ_Aleksashov with B = 0 as its callers pass it, Gas_Dinamic.cu:465 */
+};
+
+/* Euler: Ls[i] = (rho, p), Lu[i] = (vx, vy) on the left, Rs, Ru on the right,
+ * normal[i] = (n1, n2); out Ps[i] = (F_rho, F_E), Pu[i] = (F_mx, F_my) along
+ * the normal. QL, QR, PQ: the passive scalar and its flux, for the two Q
+ * kinds only (NULL otherwise). */
+int gd2_euler_face_fluxes(int kind, double gamma, double krit, double rad, long long n,
+                          const gd2_double2* Ls, const gd2_double2* Lu, const gd2_double2* Rs,
+                          const gd2_double2* Ru, const double* QL, const double* QR,
+                          const gd2_double2* normal, gd2_double2* Ps, gd2_double2* Pu, double* PQ,
+                          double* dt, void* stream);
+/* MHD, HLLDQ_Korolkov (Gas_Dinamic.cu:964) metod 2 (HLLC-MHD) or 3 (HLLD)
+ * without its passive scalar: L, R of n x 8 doubles (rho, p, vx, vy, vz, Bx,
+ * By, Bz; Gaussian B), normal of n x 3 (its frame table, Gas_Dinamic.cu:
+ * 978-1019, covers the six axis directions); out P of n x 8 (F_rho, F_mx,
+ * F_my, F_mz, F_Bx, F_By, F_Bz, F_E). */
+int gd2_mhd_face_fluxes(int metod, double gamma, double krit, double rad, long long n, const double* L,
+                        const double* R, const double* normal, double* P, double* dt, void* stream);
 
 /* Layout conversion between the reference AoS primitives and SoA conserved. */
 int gd2_prim_to_cons(const gd2_grid* g, int physics, const gd2_double2* s, const gd2_double2* u,

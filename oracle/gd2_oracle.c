@@ -1193,6 +1193,9 @@ static inline double euler_flux(const o_grid* g, int method, const prim* L, cons
     o_d2 Ls = {L->r, L->p}, Lu = {L->u, L->v}, Rs = {R->r, R->p}, Ru = {R->u, R->v};
     if (method == 2) return o_godunov(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
     if (method == 0) return o_hll(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    /* the HLLC variants as further methods (no reference kernel calls them) */
+    if (method == 3) return o_hllc_korolkov(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
+    if (method == 4) return o_hllcq_korolkov_2d(g->gamma, g->krit, Ls, Lu, Rs, Ru, 0.0, 0.0, n1, n2, Ps, Pu, NULL, rad);
     return o_hllc(g->gamma, g->krit, Ls, Lu, Rs, Ru, n1, n2, Ps, Pu, rad);
 }
 
@@ -1235,7 +1238,7 @@ static int is_frozen(const o_grid* g, int i, int j)
 static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u, o_d2* s2, o_d2* u2,
                           double* T, const double* T_do, int method)
 {
-    if (method != 0 && method != 1 && method != 2) return -1;
+    if (method < 0 || method > 4) return -1;
     if ((g->axisym || g->r_frozen != 0.0) && order != 2) return -1;
     if (g->axisym && !(g->y0 > 0.0)) return -1;
     src_t S = {g, 0, 0, s, u, NULL, NULL, 0};
@@ -1450,7 +1453,9 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
             double* Uout, double a, double b, double dt, double* wmax)
 {
     const int mhd = sc->physics == O_PHYS_MHD;
-    if (!mhd && sc->flux != O_FLUX_HLLC && sc->flux != O_FLUX_EXACT && sc->flux != O_FLUX_HLL) return -1;
+    if (!mhd && sc->flux != O_FLUX_HLLC && sc->flux != O_FLUX_EXACT && sc->flux != O_FLUX_HLL &&
+        sc->flux != O_FLUX_HLLC_KOROLKOV && sc->flux != O_FLUX_HLLC_2D)
+        return -1;
     if (mhd && sc->flux != O_FLUX_HLLC_MHD && sc->flux != O_FLUX_HLLD) return -1;
     for (int s = 0; s < 4; ++s)
         if (g->bc[s] == O_BC_FROZEN) return -1;
@@ -1460,7 +1465,8 @@ int o_stage(const o_grid* g, const o_scheme* sc, const double* Uin, const double
     const long plane = (long)g->nx * g->ny;
     const int nf = fields_of(sc->physics);
     const int method = mhd ? (sc->flux == O_FLUX_HLLD ? 3 : 2)
-                           : (sc->flux == O_FLUX_EXACT ? 2 : (sc->flux == O_FLUX_HLL ? 0 : 1));
+                           : (sc->flux == O_FLUX_EXACT ? 2 : (sc->flux == O_FLUX_HLL ? 0 :
+                              (sc->flux == O_FLUX_HLLC_KOROLKOV ? 3 : (sc->flux == O_FLUX_HLLC_2D ? 4 : 1))));
     double wm = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : wm)
     for (int j = 0; j < g->ny; ++j) {

@@ -39,7 +39,8 @@ typedef struct {
 
 /* Scheme selection for the native (SoA conserved) path. */
 enum { O_PHYS_EULER = 0, O_PHYS_MHD = 1 };
-enum { O_FLUX_HLLC = 1, O_FLUX_EXACT = 2, O_FLUX_HLLC_MHD = 3, O_FLUX_HLLD = 4, O_FLUX_HLL = 5 };
+enum { O_FLUX_HLLC = 1, O_FLUX_EXACT = 2, O_FLUX_HLLC_MHD = 3, O_FLUX_HLLD = 4, O_FLUX_HLL = 5,
+       O_FLUX_HLLC_KOROLKOV = 6, O_FLUX_HLLC_2D = 7 };
 
 typedef struct {
     int physics;         /* O_PHYS_* */

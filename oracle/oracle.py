@@ -19,6 +19,7 @@ BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
 BC_SYMMETRY, BC_AXIS, BC_OUTFLOW, BC_INFLOW = 4, 5, 6, 7
 PHYS_EULER, PHYS_MHD = 0, 1
 FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
+FLUX_HLLC_KOROLKOV, FLUX_HLLC_2D = 6, 7
 
 
 class D2(C.Structure):

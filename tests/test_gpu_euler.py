@@ -38,7 +38,9 @@ def pscheme(flux, order, integ=api.SSPRK2, cfl=0.4):
 @pytest.mark.parametrize("shape", [(130, 97), (63, 64), (64, 63), (1, 5), (200, 3)])
 @pytest.mark.parametrize("bc", list(BCS))
 @pytest.mark.parametrize("flux,order", [(api.FLUX_HLLC, 2), (api.FLUX_HLLC, 1), (api.FLUX_EXACT, 1),
-                                        (api.FLUX_EXACT, 2), (api.FLUX_HLL, 2), (api.FLUX_HLL, 1)])
+                                        (api.FLUX_EXACT, 2), (api.FLUX_HLL, 2), (api.FLUX_HLL, 1),
+                                        (api.FLUX_HLLC_KOROLKOV, 2), (api.FLUX_HLLC_KOROLKOV, 1),
+                                        (api.FLUX_HLLC_2D, 2), (api.FLUX_HLLC_2D, 1)])
 def test_stage_matches_oracle(shape, bc, flux, order):
     nx, ny = shape
     say(f"stage {nx}x{ny} bc={bc} flux={flux} order={order}")
@@ -114,7 +116,7 @@ def test_sod_godunov_to_t02_matches_oracle():
 
 
 @pytest.mark.parametrize("kind,order", [("ker_dekard", 1), ("kernel_tvd_old", 2)])
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("bc", [(O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_TRANSMISSIVE, O.BC_FROZEN),
                                 (O.BC_REFLECTIVE, O.BC_PERIODIC, O.BC_TRANSMISSIVE, O.BC_TRANSMISSIVE)])
 def test_reference_layout_step_matches_oracle(kind, order, method, bc):
@@ -240,7 +242,7 @@ def test_full_size_invariants_4096():
     s2.close()
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
 def test_kernel_tvd_old_axisymmetric_sphere_matches_oracle(method):
     """Kernel_TVD_old's 1/y source terms and inner sphere (kernel.cu:3303,
     3651-3662) with its frozen east side and axis (reflective) south side:

@@ -50,7 +50,8 @@ def one_pass(g, sc, U, dt, want_stage=False):
                                    (60, 9), (177, 13)])
 @pytest.mark.parametrize("bc", list(BCS))
 @pytest.mark.parametrize("flux,order", [(api.FLUX_HLLC, 2), (api.FLUX_HLLC, 1), (api.FLUX_EXACT, 2),
-                                        (api.FLUX_EXACT, 1), (api.FLUX_HLL, 2)])
+                                        (api.FLUX_EXACT, 1), (api.FLUX_HLL, 2), (api.FLUX_HLLC_KOROLKOV, 2),
+                                        (api.FLUX_HLLC_2D, 2), (api.FLUX_HLLC_2D, 1)])
 def test_one_pass_step_equals_two_stages(shape, bc, flux, order):
     nx, ny = shape
     say(f"one-pass step {nx}x{ny} bc={bc} flux={flux} order={order}")
