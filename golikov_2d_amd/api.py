@@ -91,6 +91,7 @@ def lib():
             "gd2_kernel_tvd_old": (ip, [gp, vp, vp, vp, vp, vp, vp, ip, vp]),
             "gd2_add2": (ip, [gp, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip, vp]),
             "gd2_kernel_tvd": (ip, [gp, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip, vp]),
+            "gd2_kernel_gkr": (ip, [gp, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip, vp]),
             "gd2_funk_time": (ip, [vp, vp, vp, vp, vp]),
             "gd2_prim_to_cons": (ip, [gp, ip, vp, vp, vp, vp, vp]),
             "gd2_cons_to_prim": (ip, [gp, ip, vp, vp, vp, vp, vp]),
@@ -134,7 +135,7 @@ def lib():
 
 
 EXPORTED = ["gd2_nfields", "gd2_last_hip_error", "gd2_version", "gd2_stage", "gd2_wavespeed_max",
-            "gd2_ker_dekard", "gd2_kernel_tvd_old", "gd2_add2", "gd2_kernel_tvd", "gd2_funk_time",
+            "gd2_ker_dekard", "gd2_kernel_tvd_old", "gd2_add2", "gd2_kernel_tvd", "gd2_kernel_gkr", "gd2_funk_time",
             "gd2_prim_to_cons", "gd2_cons_to_prim", "gd2_ctx_create", "gd2_ctx_destroy", "gd2_ctx_start",
             "gd2_ctx_advance", "gd2_ctx_time", "gd2_get_diagnostics", "gd2_ensemble_create",
             "gd2_ensemble_destroy", "gd2_ensemble_set_tiling", "gd2_ensemble_get_tiling", "gd2_ensemble_start",
@@ -204,6 +205,13 @@ def kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, i, method, stream=None):
     _check_aos(g, s, u, b, s2, u2, b2)
     _check(lib().gd2_kernel_tvd(C.byref(g), _f64(s), _f64(u), _f64(b), _f64(s2), _f64(u2), _f64(b2),
                                 _f64(T), _f64(T_do), i, method, _stream(stream)), "gd2_kernel_tvd")
+
+
+def kernel_gkr(g, s, u, b, s2, u2, b2, T, T_do, i, method=3, stream=None):
+    """Kernel_GKR (kernel.cu:2663): the predictor-corrector MHD step."""
+    _check_aos(g, s, u, b, s2, u2, b2)
+    _check(lib().gd2_kernel_gkr(C.byref(g), _f64(s), _f64(u), _f64(b), _f64(s2), _f64(u2), _f64(b2), _f64(T),
+                                _f64(T_do), i, method, _stream(stream)), "gd2_kernel_gkr")
 
 
 def funk_time(T, T_do, TT, i, stream=None):

@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "../../include/gd2.h"
+#include "gd2_gkr.hpp"
 #include "gd2_launch.hpp"
 #include "gd2_mhd.hpp"
 
@@ -248,6 +249,53 @@ int gd2_euler_face_fluxes(int kind, double gamma, double krit, double rad, long 
             break;
         default: hipLaunchKernelGGL(euler_face_kernel<GD2_FACE_HLLCQ_ALEKSASHOV>, grid, block, 0, st, B); break;
     }
+    return hip_error_code(hipGetLastError());
+}
+
+int gd2_kernel_gkr(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                   gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do, int i, int method,
+                   void* stream) {
+    (void)i;
+    if (!g || g->nx < 1 || g->ny < 1 || !(g->dx > 0.0) || !(g->dy > 0.0) || !(g->gamma > 1.0)) return GD2_ERR_ARG;
+    if ((long long)g->nx * g->ny > (1ll << 40)) return GD2_ERR_ARG;
+    // Kernel_GKR's sides are its own (kernel.cu:2697-2770): symmetry west,
+    // outflow east, axis south, inflow north; its sources are axisymmetric
+    if (g->bc[0] != GD2_BC_SYMMETRY || g->bc[1] != GD2_BC_OUTFLOW || g->bc[2] != GD2_BC_AXIS ||
+        g->bc[3] != GD2_BC_INFLOW)
+        return GD2_ERR_UNSUPPORTED;
+    if (!g->axisym || !(g->y0 > 0.0) || !(g->r_frozen >= 0.0)) return GD2_ERR_ARG;
+    if (!s || !u || !b || !s2 || !u2 || !b2 || !T || !T_do) return GD2_ERR_ARG;
+    if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2 || (const void*)b == (const void*)b2)
+        return GD2_ERR_ARG;
+    if (method != 2 && method != 3) return GD2_ERR_UNSUPPORTED;
+    GkrArgs A;
+    A.s = reinterpret_cast<const double2*>(s);
+    A.u = reinterpret_cast<const double2*>(u);
+    A.b = reinterpret_cast<const double3*>(b);
+    A.s2 = reinterpret_cast<double2*>(s2);
+    A.u2 = reinterpret_cast<double2*>(u2);
+    A.b2 = reinterpret_cast<double3*>(b2);
+    A.tmin_bits = reinterpret_cast<unsigned long long*>(T);
+    A.T_do = T_do;
+    A.nx = g->nx;
+    A.ny = g->ny;
+    A.dx = g->dx;
+    A.dy = g->dy;
+    A.x0 = g->x0;
+    A.y0 = g->y0;
+    A.gamma = g->gamma;
+    A.inv_g1 = 1.0 / (g->gamma - 1.0);
+    A.krit = g->krit;
+    A.r_frozen = g->r_frozen;
+    A.diag = device_diag();
+    if (!A.diag) return GD2_ERR_HIP;
+    const long long cells = (long long)g->nx * g->ny;
+    const unsigned blocks = (unsigned)((cells + 255) / 256);
+    const hipStream_t st = (hipStream_t)stream;
+    if (method == 3)
+        hipLaunchKernelGGL(gkr_kernel<3>, dim3(blocks), dim3(256), 0, st, A);
+    else
+        hipLaunchKernelGGL(gkr_kernel<2>, dim3(blocks), dim3(256), 0, st, A);
     return hip_error_code(hipGetLastError());
 }
 

@@ -126,6 +126,22 @@ int gd2_kernel_tvd(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
                    gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
                    int i, int method, void* stream);
 
+/* MHD Godunov-Kolgan-Rodionov predictor-corrector step. Replaces Kernel_GKR
+ * (kernel.cu:2663; predictor kernel.cu:1611, POTOK_Korolkov
+ * Gas_Dinamic.cu:70): minmod slopes (the vy slot taking the pressure slope,
+ * as the reference writes it), a half-step predictor of the cell and its four
+ * neighbours from the physical fluxes of their half-slope face states, then
+ * the update from the mean of the cell and its predictor over dt/2 with the
+ * HLLD (method 3, the reference's) or HLLC-MHD (method 2) fluxes of the
+ * predictor-averaged face states; axisymmetric sources throughout, no Powell
+ * term. Its sides are its own: the grid must give SYMMETRY, OUTFLOW, AXIS,
+ * INFLOW (west, east, south, north; else GD2_ERR_UNSUPPORTED) and axisym;
+ * cells within r_frozen keep their state (2.0 in the reference). Face time
+ * steps krit * rad / S with rad = dx on x-faces, dy on y-faces. */
+int gd2_kernel_gkr(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u, const gd2_double3* b,
+                   gd2_double2* s2, gd2_double2* u2, gd2_double3* b2, double* T, const double* T_do,
+                   int i, int method, void* stream);
+
 /* Time-step rotation, replaces funk_time (kernel.cu:1722):
  * *T_do = *T; *TT += *T_do; *T = 1e7; ++*i. All four are device pointers. */
 int gd2_funk_time(double* T, double* T_do, double* TT, int* i, void* stream);

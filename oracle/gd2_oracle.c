@@ -1424,6 +1424,310 @@ int o_kernel_tvd(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b,
     return mhd_ref_step(g, 2, s, u, b, s2, u2, b2, T, T_do, method);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Godunov-Kolgan-Rodionov predictor-corrector step: Kernel_GKR,            */
+/* kernel.cu:2663-3284, with predictor (kernel.cu:1611-1720) and            */
+/* POTOK_Korolkov (Gas_Dinamic.cu:70-181).                                  */
+/* State vector q[7] = (rho, p, vx, vy, Bx, By, Bz), Gaussian B.            */
+/* Choices where the source does not define the behaviour:                  */
+/*  * POTOK_Korolkov fills P[0..7] (energy flux at P[7] after its SWAP,      */
+/*    Gas_Dinamic.cu:175-179) while predictor declares double P[7]           */
+/*    (kernel.cu:1613) and reads P[7] as the energy flux: P has 8 entries.   */
+/*  * cell centres at x0 + n dx, y0 + m dy (the reference's Kernel_GKR puts  */
+/*    x at x_min + n (x_max - x_min)/(N - 1)); the frozen radius is the      */
+/*    grid's r_frozen (2.0 in Kernel_GKR, kernel.cu:2688).                   */
+/* Kept as written: the vy slope is the pressure slope (DX*[3], DY*[3] from  */
+/* s.y, kernel.cu:2965-3038); the second-neighbour ghosts of kernel.cu:      */
+/* 2909-2957 (n = 1 takes cell n+1 mirrored, m = 0 takes the cell itself);   */
+/* the predictor's and the corrector's 1/y source terms (always on), with    */
+/* the ghost rows' predictors at y -/+ dy; the corrector starts from the     */
+/* mean of the cell and its predictor and takes dt/2 (kernel.cu:3248-3276);  */
+/* Powell's term computed and then zeroed (kernel.cu:3245).                  */
+/* ------------------------------------------------------------------------ */
+#define GKR_KV(x) ((x) * (x))
+#define GKR_KVV(x, y, z) (GKR_KV(x) + GKR_KV(y) + GKR_KV(z))
+#define GKR_SKK(u, v, w, bx, by, bz) ((u) * (bx) + (v) * (by) + (w) * (bz))
+static inline double gkr_u8(double gam, double ro, double p, double u, double v, double w, double bx, double by,
+                            double bz)
+{
+    /* Header.h:16 */
+    return p / (gam - 1.0) + 0.5 * ro * GKR_KVV(u, v, w) + GKR_KVV(bx, by, bz) / CPI8;
+}
+
+/* POTOK_Korolkov (Gas_Dinamic.cu:70-181) with Q_L = 0: the physical MHD
+ * flux of one state along an axis normal, P[8] = (F_rho, F_mx, F_my, F_mz,
+ * F_Bx, F_By, F_Bz, F_E). */
+void o_potok(double gam, double ro_L, double p_L, double v1_L, double v2_L, double v3_L, double Bx_L,
+             double By_L, double Bz_L, double* P, double n1, double n2, double n3)
+{
+    const double g1 = gam - 1.0;
+    double bx_L = Bx_L / SPI4, by_L = By_L / SPI4, bz_L = Bz_L / SPI4;
+    double t1 = 0, t2 = 0, t3 = 0, m1 = 0, m2 = 0, m3 = 0;
+    if (n1 > 0.1) { t2 = 1.0; m3 = 1.0; }
+    else if (n2 > 0.1) { t3 = 1.0; m1 = 1.0; }
+    else if (n3 > 0.1) { t1 = 1.0; m2 = 1.0; }
+    else if (n1 < -0.1) { t3 = -1.0; m2 = -1.0; }
+    else if (n2 < -0.1) { t1 = -1.0; m3 = -1.0; }
+    else if (n3 < -0.1) { t1 = -1.0; m2 = -1.0; }
+    else diag_inc(&g_diag.bad_branch);
+    double u1 = v1_L * n1 + v2_L * n2 + v3_L * n3;
+    double v1 = v1_L * t1 + v2_L * t2 + v3_L * t3;
+    double w1 = v1_L * m1 + v2_L * m2 + v3_L * m3;
+    double bn1 = bx_L * n1 + by_L * n2 + bz_L * n3;
+    double bt1 = bx_L * t1 + by_L * t2 + bz_L * t3;
+    double bm1 = bx_L * m1 + by_L * m2 + bz_L * m3;
+    double bb_L = GKR_KV(bx_L) + GKR_KV(by_L) + GKR_KV(bz_L);
+    double uu_L = (GKR_KV(v1_L) + GKR_KV(v2_L) + GKR_KV(v3_L)) / 2.0;
+    double pTL = p_L + bb_L / 2.0;
+    double e1 = p_L / g1 + ro_L * uu_L + bb_L / 2.0;
+    double PO[8];
+    PO[0] = ro_L * u1;
+    PO[1] = ro_L * u1 * u1 + pTL - GKR_KV(bn1);
+    PO[2] = ro_L * u1 * v1 - bn1 * bt1;
+    PO[3] = ro_L * u1 * w1 - bn1 * bm1;
+    PO[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
+    PO[5] = 0.0;
+    PO[6] = u1 * bt1 - v1 * bn1;
+    PO[7] = u1 * bm1 - w1 * bn1;
+    P[1] = n1 * PO[1] + t1 * PO[2] + m1 * PO[3];
+    P[2] = n2 * PO[1] + t2 * PO[2] + m2 * PO[3];
+    P[3] = n3 * PO[1] + t3 * PO[2] + m3 * PO[3];
+    P[4] = SPI4 * (n1 * PO[5] + t1 * PO[6] + m1 * PO[7]);
+    P[5] = SPI4 * (n2 * PO[5] + t2 * PO[6] + m2 * PO[7]);
+    P[6] = SPI4 * (n3 * PO[5] + t3 * PO[6] + m3 * PO[7]);
+    P[0] = PO[0];
+    P[7] = PO[4];
+}
+
+/* predictor (kernel.cu:1611-1720): physical fluxes at the four half-slope
+ * face states Q -/+ DX/2, Q +/- DY/2 (rho, p falling back to the cell value
+ * when not positive), a step of st with the axisymmetric source terms. */
+void o_gkr_predictor(const o_grid* g, const double* Q, const double* DX, const double* DY, double* QQ, double st,
+                     double y)
+{
+    const double gam = g->gamma, dx = g->dx, dy = g->dy;
+    double P[8];
+    double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
+    /* west (-DX/2, n = -x), east (+DX/2, +x), north (+DY/2, +y), south (-DY/2, -y) */
+    const double* D[4] = {DX, DX, DY, DY};
+    const double sg[4] = {-1.0, 1.0, 1.0, -1.0};
+    const double nn[4][2] = {{-1.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}, {0.0, -1.0}};
+    const double len[4] = {dy, dy, dx, dx};
+    for (int f = 0; f < 4; ++f) {
+        const double* d = D[f];
+        double ro_ = sg[f] < 0 ? Q[0] - d[0] / 2.0 : Q[0] + d[0] / 2.0;
+        double p_ = sg[f] < 0 ? Q[1] - d[1] / 2.0 : Q[1] + d[1] / 2.0;
+        if (ro_ <= 0.0) ro_ = Q[0];
+        if (p_ <= 0.0) p_ = Q[1];
+        double q[7];
+        for (int k = 2; k < 7; ++k) q[k] = sg[f] < 0 ? Q[k] - d[k] / 2.0 : Q[k] + d[k] / 2.0;
+        o_potok(gam, ro_, p_, q[2], q[3], 0.0, q[4], q[5], q[6], P, nn[f][0], nn[f][1], 0.0);
+        PSx = PSx + P[0] * len[f];
+        PSy = PSy + P[7] * len[f];
+        PUx = PUx + P[1] * len[f];
+        PUy = PUy + P[2] * len[f];
+        PBx = PBx + P[4] * len[f];
+        PBy = PBy + P[5] * len[f];
+        PBz = PBz + P[6] * len[f];
+    }
+    const double dV = dx * dy;
+    /* kernel.cu:1698-1719 */
+    QQ[0] = Q[0] - st * PSx / dV - st * Q[0] * Q[3] / y;
+    if (QQ[0] <= 0) {
+        diag_inc(&g_diag.clamp_rho);
+        QQ[0] = Q[0];
+    }
+    QQ[2] = (Q[0] * Q[2] - st * (PUx) / dV - st * (Q[0] * Q[2] * Q[3] - Q[4] * Q[5] / CPI4) / y) / QQ[0];
+    QQ[3] = (Q[0] * Q[3] - st * (PUy) / dV - st * (Q[0] * Q[3] * Q[3] + (GKR_KV(Q[6]) - GKR_KV(Q[5])) / CPI4) / y) /
+            QQ[0];
+    QQ[4] = (Q[4] - st * (PBx) / dV - st * (Q[3] * Q[4] - Q[5] * Q[2]) / y);
+    QQ[5] = (Q[5] - st * (PBy) / dV);
+    QQ[6] = (Q[6] - st * (PBz) / dV);
+    const double u8 = gkr_u8(gam, Q[0], Q[1], Q[2], Q[3], 0.0, Q[4], Q[5], Q[6]);
+    QQ[1] = (u8 - st * (PSy) / dV -
+             st * (((u8 + Q[1] + GKR_KVV(Q[4], Q[5], Q[6]) / CPI8) * Q[3] -
+                    Q[5] * GKR_SKK(Q[2], Q[3], 0.0, Q[4], Q[5], Q[6]) / CPI4) / y) -
+             0.5 * QQ[0] * GKR_KVV(QQ[2], QQ[3], 0.0) - GKR_KVV(QQ[4], QQ[5], QQ[6]) / CPI8) * (gam - 1.0);
+    if (QQ[1] <= 0) {
+        diag_inc(&g_diag.clamp_p);
+        QQ[1] = 0.000001;
+    }
+}
+
+static inline void gkr_load(const o_d2* s, const o_d2* u, const o_d3* b, long k, double* q)
+{
+    q[0] = s[k].x; q[1] = s[k].y; q[2] = u[k].x; q[3] = u[k].y; q[4] = b[k].x; q[5] = b[k].y; q[6] = b[k].z;
+}
+
+static inline void gkr_copy(double* d, const double* a) { for (int k = 0; k < 7; ++k) d[k] = a[k]; }
+
+/* mirror in x (vx, Bx, By negated), in y (vy, Bz negated), the outflow
+ * floor (vx raised to 0.01 where negative), the inflow state */
+static inline void gkr_mx(double* d, const double* a) { gkr_copy(d, a); d[2] = -a[2]; d[4] = -a[4]; d[5] = -a[5]; }
+static inline void gkr_my(double* d, const double* a) { gkr_copy(d, a); d[3] = -a[3]; d[6] = -a[6]; }
+static inline void gkr_out(double* d, const double* a) { gkr_copy(d, a); if (d[2] < 0) d[2] = 0.01; }
+static inline void gkr_in(double* d, double gam, const double* bsrc)
+{
+    d[0] = 1.0; d[1] = 1.0 / gam; d[2] = 0.0; d[3] = 0.0; d[4] = bsrc[4]; d[5] = bsrc[5]; d[6] = bsrc[6];
+}
+
+/* the 13-point stencil of kernel.cu:2678-2957, st[k] for the reference's
+ * s_k / u_k / b_k (k = 1..13) */
+void o_gkr_stencil(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int n, int m, double st[14][7])
+{
+    const int N = g->nx, M = g->ny;
+    const double gam = g->gamma;
+#define AT(nn, mm, q) gkr_load(s, u, b, (long)(mm) * N + (nn), q)
+    AT(n, m, st[1]);
+    if (m == M - 1) gkr_in(st[5], gam, st[1]); else AT(n, m + 1, st[5]);
+    if (n == N - 1) gkr_out(st[2], st[1]); else AT(n + 1, m, st[2]);
+    if (n == 0) gkr_mx(st[4], st[1]); else AT(n - 1, m, st[4]);
+    if (m == 0) gkr_my(st[3], st[1]); else AT(n, m - 1, st[3]);
+    /* kernel.cu:2766-2781 */
+    if (n == N - 1) gkr_copy(st[10], st[5]);
+    else if (m == M - 1) gkr_in(st[10], gam, st[2]);
+    else AT(n + 1, m + 1, st[10]);
+    /* kernel.cu:2784-2840 */
+    if (m == 0) {
+        gkr_my(st[12], st[4]);
+        gkr_my(st[11], st[2]);
+    } else {
+        if (n == 0) gkr_mx(st[12], st[3]); else AT(n - 1, m - 1, st[12]);
+        if (n == N - 1) gkr_out(st[11], st[3]); else AT(n + 1, m - 1, st[11]);
+    }
+    /* kernel.cu:2842-2870 */
+    if (n == 0) gkr_mx(st[13], st[5]);
+    else if (m == M - 1) gkr_in(st[13], gam, st[4]);
+    else AT(n - 1, m + 1, st[13]);
+    /* kernel.cu:2872-2957 */
+    if (n >= N - 2) gkr_out(st[6], st[2]); else AT(n + 2, m, st[6]);
+    if (m >= M - 2) gkr_copy(st[9], st[5]); else AT(n, m + 2, st[9]);
+    if (n == 0 || n == 1) gkr_mx(st[8], st[2]); else AT(n - 2, m, st[8]);
+    if (m == 1) gkr_my(st[7], st[5]);
+    else if (m == 0) gkr_my(st[7], st[3]);
+    else AT(n, m - 2, st[7]);
+#undef AT
+}
+
+/* dx * minmod((a - c)/dx, (c - b)/dx) per component, the vy slot taking the
+ * pressure slope (kernel.cu:2962-3038) */
+static inline void gkr_slope(const double* hi, const double* mid, const double* lo, double h, double* D)
+{
+    static const int src[7] = {0, 1, 2, 1, 4, 5, 6};
+    for (int k = 0; k < 7; ++k) {
+        const int c = src[k];
+        D[k] = h * o_minmod((hi[c] - mid[c]) / h, (mid[c] - lo[c]) / h);
+    }
+}
+
+void o_gkr_slopes(const o_grid* g, double st[14][7], double DX[6][7], double DY[6][7])
+{
+    const double dx = g->dx, dy = g->dy;
+    gkr_slope(st[2], st[1], st[4], dx, DX[1]);
+    gkr_slope(st[6], st[2], st[1], dx, DX[2]);
+    gkr_slope(st[1], st[4], st[8], dx, DX[4]);
+    gkr_slope(st[10], st[5], st[13], dx, DX[5]);
+    gkr_slope(st[11], st[3], st[12], dx, DX[3]);
+    gkr_slope(st[5], st[1], st[3], dy, DY[1]);
+    gkr_slope(st[9], st[5], st[1], dy, DY[5]);
+    gkr_slope(st[13], st[4], st[12], dy, DY[4]);
+    gkr_slope(st[10], st[2], st[11], dy, DY[2]);
+    gkr_slope(st[1], st[3], st[7], dy, DY[3]);
+}
+
+int o_kernel_gkr(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, o_d2* s2, o_d2* u2, o_d3* b2,
+                 double* T, const double* T_do, int method)
+{
+    if (method != 2 && method != 3) return -1;
+    if (!(g->y0 > 0.0)) return -1;
+    const double dx = g->dx, dy = g->dy, gam = g->gamma;
+    const double Tdo = *T_do;
+    double tglob = *T;
+#pragma omp parallel for schedule(static) reduction(min : tglob)
+    for (int m = 0; m < g->ny; ++m) {
+        for (int n = 0; n < g->nx; ++n) {
+            const long k = (long)m * g->nx + n;
+            const double x = g->x0 + n * dx, y = g->y0 + m * dy;
+            if (sqrt(x * x + y * y) < g->r_frozen) {
+                s2[k] = s[k]; u2[k] = u[k]; b2[k] = b[k];
+                continue;
+            }
+            double st[14][7], DX[6][7], DY[6][7], QQ[6][7];
+            o_gkr_stencil(g, s, u, b, n, m, st);
+            o_gkr_slopes(g, st, DX, DY);
+            /* kernel.cu:3089-3093: cells 1..5 at y, y, y - dy, y, y + dy */
+            const double yk[6] = {0.0, y, y, y - dy, y, y + dy};
+            for (int c = 1; c <= 5; ++c) o_gkr_predictor(g, st[c], DX[c], DY[c], QQ[c], Tdo / 2.0, yk[c]);
+            double tmin = 1000;
+            double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
+            /* kernel.cu:3105-3238: east, south, west, north faces */
+            const int nb[4] = {2, 3, 4, 5};
+            const double* Din[4] = {DX[1], DY[1], DX[1], DY[1]};
+            const double* Dnb[4] = {DX[2], DY[3], DX[4], DY[5]};
+            const double sgn_in[4] = {1.0, -1.0, -1.0, 1.0};
+            const double nn[4][2] = {{1.0, 0.0}, {0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}};
+            const double rad[4] = {dx, dy, dx, dy};
+            const double len[4] = {dy, dx, dy, dx};
+            for (int f = 0; f < 4; ++f) {
+                const int c = nb[f];
+                double L[7], R[7];
+                for (int q = 0; q < 7; ++q) {
+                    const double m1 = (st[1][q] + QQ[1][q]) / 2.0, m2 = (st[c][q] + QQ[c][q]) / 2.0;
+                    L[q] = sgn_in[f] > 0 ? m1 + Din[f][q] / 2.0 : m1 - Din[f][q] / 2.0;
+                    R[q] = sgn_in[f] > 0 ? m2 - Dnb[f][q] / 2.0 : m2 + Dnb[f][q] / 2.0;
+                    if (q < 2) {
+                        if (L[q] <= 0.0) L[q] = m1;
+                        if (R[q] <= 0.0) R[q] = m2;
+                    }
+                }
+                double P[8];
+                const double t = o_hlld(gam, g->krit, L[0], L[1], L[2], L[3], 0.0, L[4], L[5], L[6], R[0], R[1],
+                                        R[2], R[3], 0.0, R[4], R[5], R[6], P, nn[f][0], nn[f][1], 0.0, rad[f],
+                                        method);
+                tmin = tmin <= t ? tmin : t; /* my_min, kernel.cu:102 */
+                PSx = PSx + P[0] * len[f];
+                PSy = PSy + P[7] * len[f];
+                PUx = PUx + P[1] * len[f];
+                PUy = PUy + P[2] * len[f];
+                PBx = PBx + P[4] * len[f];
+                PBy = PBy + P[5] * len[f];
+                PBz = PBz + P[6] * len[f];
+            }
+            if (tmin < tglob) tglob = tmin;
+            /* kernel.cu:3247-3283: the corrector from the mean of the cell and
+             * its predictor, dt/2, Powell's term zero */
+            const double dV = dx * dy, h = Tdo / 2.0;
+            double Qb[7];
+            for (int q = 0; q < 7; ++q) Qb[q] = (st[1][q] + QQ[1][q]) / 2.0;
+            double r2 = Qb[0] - h * PSx / dV - h * Qb[0] * Qb[3] / y;
+            if (r2 <= 0) {
+                diag_inc(&g_diag.clamp_rho);
+                r2 = Qb[0];
+            }
+            const double vx2 = (Qb[0] * Qb[2] - h * (PUx) / dV - h * (Qb[0] * Qb[2] * Qb[3] - Qb[4] * Qb[5] / CPI4) / y) / r2;
+            const double vy2 = (Qb[0] * Qb[3] - h * (PUy) / dV -
+                                h * (Qb[0] * Qb[3] * Qb[3] + (GKR_KV(Qb[6]) - GKR_KV(Qb[5])) / CPI4) / y) / r2;
+            const double bx2 = (Qb[4] - h * (PBx) / dV - h * (Qb[3] * Qb[4] - Qb[5] * Qb[2]) / y);
+            const double by2 = (Qb[5] - h * (PBy) / dV);
+            const double bz2 = (Qb[6] - h * (PBz) / dV);
+            const double u8 = gkr_u8(gam, Qb[0], Qb[1], Qb[2], Qb[3], 0.0, Qb[4], Qb[5], Qb[6]);
+            double p2 = (u8 - h * (PSy) / dV -
+                         h * (((u8 + Qb[1] + GKR_KVV(Qb[4], Qb[5], Qb[6]) / CPI8) * Qb[3] -
+                               Qb[5] * GKR_SKK(Qb[2], Qb[3], 0.0, Qb[4], Qb[5], Qb[6]) / CPI4) / y) -
+                         0.5 * r2 * GKR_KVV(vx2, vy2, 0.0) - GKR_KVV(bx2, by2, bz2) / CPI8) * (gam - 1.0);
+            if (p2 <= 0) {
+                diag_inc(&g_diag.clamp_p);
+                p2 = 0.000001;
+            }
+            s2[k].x = r2; s2[k].y = p2;
+            u2[k].x = vx2; u2[k].y = vy2;
+            b2[k].x = bx2; b2[k].y = by2; b2[k].z = bz2;
+        }
+    }
+    *T = tglob;
+    return 0;
+}
+
 /* kernel.cu:1722-1733 (the every-5000-steps print is left out) */
 void o_funk_time(double* T, double* T_do, double* TT, int* i)
 {

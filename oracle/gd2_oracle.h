@@ -105,6 +105,21 @@ int o_add2(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b,
 /* MHD minmod MUSCL + Powell source: Cartesian part of Kernel_TVD (kernel.cu:2198-2661). */
 int o_kernel_tvd(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b,
                  o_d2* s2, o_d2* u2, o_d3* b2, double* T, const double* T_do, int i, int method);
+/* Kernel_GKR (kernel.cu:2663): the Godunov-Kolgan-Rodionov predictor-corrector
+ * MHD step with its own hard-coded sides (symmetry west, outflow east, axis
+ * south, inflow north), axisymmetric sources, frozen cells within r_frozen;
+ * method 3 = HLLD (the reference's), 2 = HLLC-MHD. */
+int o_kernel_gkr(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, o_d2* s2, o_d2* u2, o_d3* b2,
+                 double* T, const double* T_do, int method);
+/* its parts: POTOK_Korolkov with Q = 0 (P[8], energy in P[7]), the stencil
+ * st[1..13][7] = (rho, p, vx, vy, Bx, By, Bz), the slopes DX/DY[1..5][7] and
+ * the predictor of one cell (step st, radius y) */
+void o_potok(double gam, double ro_L, double p_L, double v1_L, double v2_L, double v3_L, double Bx_L,
+             double By_L, double Bz_L, double* P, double n1, double n2, double n3);
+void o_gkr_stencil(const o_grid* g, const o_d2* s, const o_d2* u, const o_d3* b, int n, int m, double st[14][7]);
+void o_gkr_slopes(const o_grid* g, double st[14][7], double DX[6][7], double DY[6][7]);
+void o_gkr_predictor(const o_grid* g, const double* Q, const double* DX, const double* DY, double* QQ, double st,
+                     double y);
 /* funk_time (kernel.cu:1722-1733) */
 void o_funk_time(double* T, double* T_do, double* TT, int* i);
 

@@ -89,6 +89,12 @@ def lib():
         for fn in (L.o_add2, L.o_kernel_tvd):
             fn.restype = C.c_int
             fn.argtypes = [gp, vp, vp, vp, vp, vp, vp, dp, dp, C.c_int, C.c_int]
+        L.o_kernel_gkr.restype = C.c_int
+        L.o_kernel_gkr.argtypes = [gp, vp, vp, vp, vp, vp, vp, dp, dp, C.c_int]
+        L.o_potok.restype = None
+        L.o_potok.argtypes = [C.c_double] * 9 + [dp] + [C.c_double] * 3
+        L.o_gkr_predictor.restype = None
+        L.o_gkr_predictor.argtypes = [gp, vp, vp, vp, vp, C.c_double, C.c_double]
         L.o_funk_time.restype = None
         L.o_funk_time.argtypes = [dp, dp, dp, C.POINTER(C.c_int)]
         sp = C.POINTER(Scheme)
@@ -275,6 +281,33 @@ def ref_mhd_step(kind, g, s, u, b, T, T_do, i, method):
     if rc != 0:
         raise ValueError(f"{kind} rejected method {method}")
     return s2, u2, b2, Tc.value
+
+
+def ref_gkr_step(g, s, u, b, T, T_do, method=3):
+    """Kernel_GKR (kernel.cu:2663): one predictor-corrector step. Returns s2, u2, b2, T."""
+    s2, u2, b2 = np.empty_like(s), np.empty_like(u), np.empty_like(b)
+    Tc, Td = C.c_double(T), C.c_double(T_do)
+    rc = lib().o_kernel_gkr(C.byref(g), _ptr(s), _ptr(u), _ptr(b), _ptr(s2), _ptr(u2), _ptr(b2), C.byref(Tc),
+                            C.byref(Td), method)
+    if rc != 0:
+        raise ValueError(f"o_kernel_gkr rejected method {method} or the grid")
+    return s2, u2, b2, Tc.value
+
+
+def potok(gamma, q, n):
+    """POTOK_Korolkov: physical MHD flux of q = (rho, p, vx, vy, vz, Bx, By, Bz)
+    (Gaussian B) along an axis normal n; P[8] with the energy flux last."""
+    P = (C.c_double * 8)()
+    lib().o_potok(gamma, *[float(v) for v in q], P, float(n[0]), float(n[1]), float(n[2]))
+    return np.array(list(P))
+
+
+def gkr_predictor(g, Q, DX, DY, step, y):
+    """predictor (kernel.cu:1611): Q, DX, DY of 7 entries (rho, p, vx, vy, Bx, By, Bz)."""
+    Q, DX, DY = (np.ascontiguousarray(a, dtype=np.float64) for a in (Q, DX, DY))
+    QQ = np.empty(7)
+    lib().o_gkr_predictor(C.byref(g), _ptr(Q), _ptr(DX), _ptr(DY), _ptr(QQ), float(step), float(y))
+    return QQ
 
 
 def funk_time(T, T_do, TT, i):
