@@ -303,16 +303,20 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
 
     dev = torch.device("cuda:0")
     out = {}
-    for kind, n in (("mhd_kernel_tvd", n_mhd), ("euler_kernel_tvd_old", n_euler), ("mhd_wind_kernel_tvd", 0)):
+    for kind, n in (("mhd_kernel_tvd", n_mhd), ("euler_kernel_tvd_old", n_euler), ("mhd_wind_kernel_tvd", 0),
+                    ("mhd_wind_kernel_gkr", 0)):
         mhd = kind.startswith("mhd")
-        wind = kind == "mhd_wind_kernel_tvd"
-        method = 2 if wind else 3
+        wind = kind.startswith("mhd_wind")
+        gkr = kind == "mhd_wind_kernel_gkr"
+        method = 3 if gkr else (2 if wind else 3)
         if wind:
             # the reference's production run: 3226 x 1792, symmetry / outflow /
             # axis / inflow sides, frozen inner sphere, axisymmetric sources,
-            # HLLC-MHD (method 2), stellar-wind initial state
+            # HLLC-MHD (method 2), stellar-wind initial state; Kernel_GKR with
+            # its own frozen radius 2.0 and HLLD
             nx, ny = PB.WIND_NX, PB.WIND_NY
-            g = api.make_grid(nx, ny, **PB.wind_grid_args())
+            wkw = dict(PB.wind_grid_args(), r_frozen=2.0) if gkr else PB.wind_grid_args()
+            g = api.make_grid(nx, ny, **wkw)
             sh0, uh0, bh0 = PB.stellar_wind_prim(g)
             s, u, b = (torch.from_numpy(a).to(dev) for a in (sh0, uh0, bh0))
         else:
@@ -331,7 +335,10 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
 
         def step():
             nonlocal s, u, b, s2, u2, b2
-            if mhd:
+            if gkr:
+                api.kernel_gkr(g, s, u, b, s2, u2, b2, T, T_do, 0, method)
+                b, b2 = b2, b
+            elif mhd:
                 api.kernel_tvd(g, s, u, b, s2, u2, b2, T, T_do, 0, method)
                 b, b2 = b2, b
             else:
@@ -352,7 +359,10 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
         ms = e0.elapsed_time(e1) / steps
         bpc = 112 if mhd else 64
         cells = nx * ny
-        if wind:
+        if gkr:
+            work = (f"gd2_kernel_gkr (GKR predictor-corrector, HLLD) + gd2_funk_time, stellar wind {nx}x{ny}, "
+                    "its own sides, frozen sphere r < 2.0, axisymmetric")
+        elif wind:
             work = (f"gd2_kernel_tvd (MUSCL + HLLC-MHD, method 2) + gd2_funk_time, stellar wind {nx}x{ny}, "
                     f"symmetry/outflow/axis/inflow sides, frozen sphere r < {PB.WIND_DDIST}, axisymmetric")
         elif mhd:
@@ -365,13 +375,15 @@ def reference_layout(n_mhd=2048, n_euler=4096, steps=20, seed=0, with_cpu=True):
         if with_cpu:
             from oracle import oracle as O
             if wind:
-                og = O.make_grid(nx, ny, **PB.wind_grid_args())
+                og = O.make_grid(nx, ny, **wkw)
             else:
                 og = O.make_grid(n, n, gamma=g.gamma, bc=tuple(g.bc))
             sh, uh = s.cpu().numpy().copy(), u.cpu().numpy().copy()
             Tdo = float(T_do.item())
             t0 = time.perf_counter()
-            if mhd:
+            if gkr:
+                O.ref_gkr_step(og, sh, uh, b.cpu().numpy().copy(), 1e7, Tdo, method)
+            elif mhd:
                 O.ref_mhd_step("kernel_tvd", og, sh, uh, b.cpu().numpy().copy(), 1e7, Tdo, 0, method)
             else:
                 O.ref_euler_step("kernel_tvd_old", og, sh, uh, 1e7, Tdo, 1)
