@@ -4,9 +4,18 @@
  *
  * Plain C: pointers, sizes and an opaque stream handle (a hipStream_t passed as
  * void*; NULL = the default stream). Every device pointer is caller-owned
- * device memory; no entry point allocates except gd2_ctx_create. Entry points
- * are stream-ordered and never synchronise the host, except gd2_ctx_time,
- * gd2_ctx_kernel_ms and gd2_get_diagnostics, which say so.
+ * device memory. Entry points are stream-ordered and asynchronous, and the
+ * step, stage, flux, adjoint and layout entry points neither allocate nor
+ * synchronise. The exceptions:
+ *   allocate device memory: gd2_ctx_create, gd2_ensemble_create,
+ *     gd2_ensemble_set_tiling and gd2_ensemble_tune (tile lists);
+ *   allocate host-side HIP objects: gd2_ctx_time_kernels (events),
+ *     gd2_ensemble_advance with use_graph on its first call for a step count
+ *     (a capture stream and a graph);
+ *   synchronise the host: gd2_ctx_time, gd2_ctx_kernel_ms, gd2_ensemble_time,
+ *     gd2_ensemble_tune, gd2_ensemble_create / _set_tiling (copies of the
+ *     tile list), gd2_get_diagnostics, and the destroy calls (device-wide).
+ * None of these belongs inside a stream capture or a hot loop.
  *
  * Return value: GD2_OK (0) or a negative gd2_status.
  *
@@ -224,7 +233,10 @@ int gd2_ctx_destroy(gd2_ctx* ctx);
 int gd2_ctx_start(gd2_ctx* ctx, const double* U, double t0, void* stream);
 /* Advance the device state U in place by nsteps steps. Each step takes
  * dt = min(cfl / wmax(U^n), t_end - t) when t_end > 0, else cfl / wmax(U^n);
- * steps requested after t reached t_end are exact no-ops. No host sync. */
+ * steps requested after t reached t_end are exact no-ops. No host sync. A
+ * one-pass SSP-RK2 step with t_end <= 0 is a single kernel launch that also
+ * updates the device clock; otherwise a one-thread clock kernel precedes
+ * each step's kernels. */
 int gd2_ctx_advance(gd2_ctx* ctx, double* U, int nsteps, double t_end, void* stream);
 /* Synchronises the stream and reads the clock: time, steps taken, last dt. */
 int gd2_ctx_time(gd2_ctx* ctx, double* t, long long* steps, double* dt, void* stream);
@@ -266,7 +278,8 @@ int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, dou
 int gd2_ctx_log_dt(gd2_ctx* ctx, double* dt_log, long long capacity);
 
 /* Kernel timing (no reference counterpart; the reference times whole phases
- * with events around its loop, kernel.cu:4087-4091): gd2_ctx_advance records
+ * with events around its loop, kernel.cu:4087-4091). Not while the stream is
+ * captured: gd2_ctx_advance then records no events. gd2_ctx_advance records
  * a pair of HIP events around the kernel launches of every stride-th step
  * (stride >= 1; the first step after this call included) until `capacity`
  * steps are timed: the one-pass step kernel, or the step's stage kernels, not
