@@ -182,8 +182,8 @@ def batched_vs_serial(seed=0, count=256, steps=50):
         "tiling": {"rows": rows, "blocks": blocks},
         "bitwise_equal_to_serial": same,
         "workload": f"{count} problems, sides log-uniform on [64, 1024] (seed {seed}), C1-style four-quadrant "
-                    "states, HLLC + minmod, SSP-RK2, own CFL dt each; batched = tuned persistent launch per "
-                    "stage replayed from a hipGraph",
+                    "states, HLLC + minmod, SSP-RK2, own CFL dt each; batched = one persistent launch of the "
+                    "one-pass step for all steps (XCD-affine work lists, tuned strips), replayed from a hipGraph",
     }
 
 

@@ -893,13 +893,21 @@ struct gd2_ensemble {
     EnsProblem* dprobs = nullptr;
     Clock* clocks = nullptr;
     double* scratch = nullptr;
-    EnsTile* dtiles = nullptr;
+    EnsTile* dtiles = nullptr;        // stage tiles (an odd step count's first step)
     int ntiles = 0;
-    int rows = 0;       // tile height (0 = per-problem default)
-    int blocks = 0;     // resident waves per stage launch (0 = one per tile)
+    EnsTile* dtiles1 = nullptr;       // one-pass tiles (the persistent launch), by work list
+    int ntiles1 = 0;
+    EnsQueues q_xcd{}, q_one{};       // XCD-affine work lists / one list
+    bool xcd_ok = false;              // the XCD bins balance
+    unsigned* sync = nullptr;         // work-list heads, spin flag, per-problem completions
+    unsigned long long* hist = nullptr;  // per-problem wave-speed maxima of a launch's steps
+    int rows = 0;       // one-pass strip height asked for (0 = default)
+    int rows1 = 0;      // one-pass strip height in use
+    int blocks = 0;     // waves of the persistent launch (0 = all that fit)
+    int resident = 0;   // waves of the persistent kernel that fit on the device
     long long host_steps = 0;
     hipStream_t cap_stream = nullptr;
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, parity) -> graph
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, slot) -> graph
     std::vector<hipGraph_t> raw_graphs;
 };
 
@@ -910,34 +918,107 @@ static void ens_drop_graphs(gd2_ensemble* e) {
     e->raw_graphs.clear();
 }
 
-static int ens_build_tiles(gd2_ensemble* e) {
-    std::vector<EnsTile> t;
-    for (int p = 0; p < e->n; ++p) {
-        EulerArgs& A = e->host[p].args;
-        A.rows = e->rows > 0 ? e->rows : tile_rows_for(A.nx, A.ny);
-        const int ty_n = (A.ny + A.rows - 1) / A.rows;
-        for (int ty = 0; ty < ty_n; ++ty)
-            for (int tx = 0; tx < A.tiles_x; ++tx) t.push_back(EnsTile{p, tx, ty, 0});
+// one-pass strip height: the requested one, else the tallest of 256, 128, ...
+// 16 that still gives the launch two tiles per resident wave
+static int ens_rk2_rows(const gd2_ensemble* e) {
+    if (e->rows > 0) return std::max(2, e->rows);  // strips start at row 0 or at row >= 2
+    int rows = 256;
+    for (; rows > 16; rows /= 2) {
+        long long t = 0;
+        for (int p = 0; p < e->n; ++p) {
+            const EulerArgs& A = e->host[p].args;
+            t += (long long)((A.nx + kRk2W - 1) / kRk2W) * ((A.ny + rows - 1) / rows);
+        }
+        if (t >= 2LL * std::max(e->resident, 1)) break;
     }
-    // large problems first so the tail of a persistent launch holds small tiles
+    return rows;
+}
+
+// large problems first, so the end of each step's share of the list holds
+// small tiles; the tiles of one problem stay together
+static void ens_sort(const gd2_ensemble* e, std::vector<EnsTile>& t) {
     std::stable_sort(t.begin(), t.end(), [&](const EnsTile& a, const EnsTile& b) {
         const long long ca = (long long)e->host[a.problem].args.nx * e->host[a.problem].args.ny;
         const long long cb = (long long)e->host[b.problem].args.nx * e->host[b.problem].args.ny;
         return ca > cb;
     });
+}
+
+static int ens_build_tiles(gd2_ensemble* e) {
+    std::vector<EnsTile> t;
+    const int rows1 = ens_rk2_rows(e);
+    for (int p = 0; p < e->n; ++p) {
+        EnsProblem& P = e->host[p];
+        EulerArgs& A = P.args;
+        A.rows = tile_rows_for(A.nx, A.ny);
+        const int ty_n = (A.ny + A.rows - 1) / A.rows;
+        for (int ty = 0; ty < ty_n; ++ty)
+            for (int tx = 0; tx < A.tiles_x; ++tx) t.push_back(EnsTile{p, tx, ty, 0});
+        P.rk2_rows = rows1;
+        P.rk2_tiles = ((A.nx + kRk2W - 1) / kRk2W) * ((A.ny + rows1 - 1) / rows1);
+    }
+    ens_sort(e, t);
+    // XCD bins: largest work first into the least loaded bin; a problem's work
+    // is its strips' passes (rows plus about five) times its strip columns
+    std::vector<long long> cost(e->n);
+    std::vector<int> order(e->n), bin(e->n);
+    for (int p = 0; p < e->n; ++p) {
+        const EulerArgs& A = e->host[p].args;
+        long long passes = 0;
+        for (int j0 = 0; j0 < A.ny; j0 += rows1) passes += std::min(rows1, A.ny - j0) + 5;
+        cost[p] = passes * ((A.nx + kRk2W - 1) / kRk2W);
+        order[p] = p;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    long long load[kEnsQueues] = {0}, total = 0;
+    for (int p : order) {
+        int b = 0;
+        for (int k = 1; k < kEnsQueues; ++k)
+            if (load[k] < load[b]) b = k;
+        bin[p] = b;
+        load[b] += cost[p];
+        total += cost[p];
+    }
+    long long worst = 0;
+    for (int k = 0; k < kEnsQueues; ++k) worst = std::max(worst, load[k]);
+    e->xcd_ok = worst * kEnsQueues <= total + total / 16;  // within about 6 % of an even split
+    // one-pass tiles by bin, then large problems first (the tiles of one
+    // problem stay together)
+    std::vector<EnsTile> t1;
+    for (int k = 0; k < kEnsQueues; ++k) {
+        e->q_xcd.off[k] = (int)t1.size();
+        std::vector<EnsTile> tb;
+        for (int p = 0; p < e->n; ++p) {
+            if (bin[p] != k) continue;
+            const EulerArgs& A = e->host[p].args;
+            const int tx1 = (A.nx + kRk2W - 1) / kRk2W, ty1 = (A.ny + rows1 - 1) / rows1;
+            for (int ty = 0; ty < ty1; ++ty)
+                for (int tx = 0; tx < tx1; ++tx) tb.push_back(EnsTile{p, tx, ty, 0});
+        }
+        ens_sort(e, tb);
+        t1.insert(t1.end(), tb.begin(), tb.end());
+    }
+    e->q_xcd.off[kEnsQueues] = (int)t1.size();
+    e->q_one.off[0] = 0;
+    for (int k = 1; k <= kEnsQueues; ++k) e->q_one.off[k] = (int)t1.size();
     if (e->dtiles) (void)hipFree(e->dtiles);
-    e->dtiles = nullptr;
+    if (e->dtiles1) (void)hipFree(e->dtiles1);
+    e->dtiles = e->dtiles1 = nullptr;
     GD2_HIP(hipMalloc(&e->dtiles, sizeof(EnsTile) * t.size()));
+    GD2_HIP(hipMalloc(&e->dtiles1, sizeof(EnsTile) * t1.size()));
     GD2_HIP(hipMemcpy(e->dtiles, t.data(), sizeof(EnsTile) * t.size(), hipMemcpyHostToDevice));
+    GD2_HIP(hipMemcpy(e->dtiles1, t1.data(), sizeof(EnsTile) * t1.size(), hipMemcpyHostToDevice));
     GD2_HIP(hipMemcpy(e->dprobs, e->host.data(), sizeof(EnsProblem) * e->n, hipMemcpyHostToDevice));
     e->ntiles = (int)t.size();
+    e->ntiles1 = (int)t1.size();
+    e->rows1 = rows1;
     ens_drop_graphs(e);
     return GD2_OK;
 }
 
 template <int STAGE>
 static int ens_launch_stage(gd2_ensemble* e, int slot, hipStream_t st) {
-    const unsigned grid = (unsigned)(e->blocks > 0 ? std::min(e->blocks, e->ntiles) : e->ntiles);
+    const unsigned grid = (unsigned)e->ntiles;
     const int order = e->sc.order, flux = e->sc.flux;
     if (order == 2 && flux == GD2_FLUX_HLLC)
         hipLaunchKernelGGL((ens_stage_kernel<2, GD2_FLUX_HLLC, STAGE>), dim3(grid), dim3(kWave), 0, st, e->dprobs,
@@ -961,16 +1042,86 @@ static int ens_launch_stage(gd2_ensemble* e, int slot, hipStream_t st) {
     return GD2_OK;
 }
 
-static int ens_enqueue_steps(gd2_ensemble* e, int nsteps, long long first_step, hipStream_t st) {
-    for (int i = 0; i < nsteps; ++i) {
-        const int slot = (int)((first_step + i) % 3);
+static const void* ens_rk2_symbol(int order, int flux) {
+    if (order == 2 && flux == GD2_FLUX_HLLC) return reinterpret_cast<const void*>(&ens_rk2_kernel<2, GD2_FLUX_HLLC, true>);
+    if (order == 1 && flux == GD2_FLUX_HLLC) return reinterpret_cast<const void*>(&ens_rk2_kernel<1, GD2_FLUX_HLLC, true>);
+    if (order == 2 && flux == GD2_FLUX_EXACT)
+        return reinterpret_cast<const void*>(&ens_rk2_kernel<2, GD2_FLUX_EXACT, true>);
+    if (order == 1 && flux == GD2_FLUX_EXACT)
+        return reinterpret_cast<const void*>(&ens_rk2_kernel<1, GD2_FLUX_EXACT, true>);
+    if (order == 2) return reinterpret_cast<const void*>(&ens_rk2_kernel<2, GD2_FLUX_HLL, true>);
+    return reinterpret_cast<const void*>(&ens_rk2_kernel<1, GD2_FLUX_HLL, true>);
+}
+
+template <bool XCD>
+static void ens_rk2_go(gd2_ensemble* e, unsigned grid, const EnsQueues& q, int nsteps, int dry, hipStream_t st) {
+    const int order = e->sc.order, flux = e->sc.flux;
+#define GD2_ENS_GO(O, F)                                                                                           \
+    hipLaunchKernelGGL((ens_rk2_kernel<O, F, XCD>), dim3(grid), dim3(kWave), 0, st, e->dprobs, e->dtiles1, q, nsteps, \
+                       e->hist, e->sync, dry)
+    if (order == 2 && flux == GD2_FLUX_HLLC)
+        GD2_ENS_GO(2, GD2_FLUX_HLLC);
+    else if (order == 1 && flux == GD2_FLUX_HLLC)
+        GD2_ENS_GO(1, GD2_FLUX_HLLC);
+    else if (order == 2 && flux == GD2_FLUX_EXACT)
+        GD2_ENS_GO(2, GD2_FLUX_EXACT);
+    else if (order == 1 && flux == GD2_FLUX_EXACT)
+        GD2_ENS_GO(1, GD2_FLUX_EXACT);
+    else if (order == 2)
+        GD2_ENS_GO(2, GD2_FLUX_HLL);
+    else
+        GD2_ENS_GO(1, GD2_FLUX_HLL);
+#undef GD2_ENS_GO
+}
+
+// persistent launches of nsteps one-pass steps from clock slot `slot`, up to
+// kEnsMaxSteps per launch, each between a reset and (unless dry) a clock
+// finish kernel
+static int ens_launch_rk2(gd2_ensemble* e, int nsteps, int slot, int dry, hipStream_t st) {
+    const int want = e->blocks > 0 ? e->blocks : e->resident;
+    // XCD-affine lists need waves on every XCD: launches of at least 64 waves
+    // (consecutive workgroups go to the XCDs in turn)
+    const bool xcd = e->xcd_ok && want >= 8 * kEnsQueues;
+    const int nr = kEnsSyncHead + e->n;
+    while (nsteps > 0) {
+        const int k = std::min(nsteps, kEnsMaxSteps);
+        hipLaunchKernelGGL(ens_launch_reset_kernel, dim3((nr + 255) / 256), dim3(256), 0, st, e->dprobs, e->n, k, slot,
+                           e->hist, e->sync);
+        GD2_HIP(hipGetLastError());
+        const long long items = (long long)e->ntiles1 * k;
+        const unsigned grid = (unsigned)std::max(1LL, std::min<long long>(want, xcd ? std::max<long long>(items, 64) : items));
+        if (xcd)
+            ens_rk2_go<true>(e, grid, e->q_xcd, k, dry, st);
+        else
+            ens_rk2_go<false>(e, grid, e->q_one, k, dry, st);
+        GD2_HIP(hipGetLastError());
+        if (!dry) {
+            hipLaunchKernelGGL(ens_launch_finish_kernel, dim3((e->n + 63) / 64), dim3(64), 0, st, e->dprobs, e->n, k,
+                               slot, e->hist);
+            GD2_HIP(hipGetLastError());
+            slot = (slot + k) % 3;
+        }
+        nsteps -= k;
+    }
+    return GD2_OK;
+}
+
+// nsteps SSP-RK2 steps from clock slot first_slot: an odd count takes its
+// first step as two stage launches (in place), the rest in one persistent
+// launch that ping-pongs and ends in the caller's buffers
+static int ens_enqueue_steps(gd2_ensemble* e, int nsteps, int first_slot, hipStream_t st) {
+    int slot = first_slot;
+    if (nsteps & 1) {
         hipLaunchKernelGGL(ens_clock_kernel, dim3((e->n + 63) / 64), dim3(64), 0, st, e->dprobs, e->n, slot);
         GD2_HIP(hipGetLastError());
         int rc = ens_launch_stage<1>(e, slot, st);
         if (rc) return rc;
         rc = ens_launch_stage<2>(e, slot, st);
         if (rc) return rc;
+        slot = (slot + 1) % 3;
+        nsteps -= 1;
     }
+    if (nsteps > 0) return ens_launch_rk2(e, nsteps, slot, 0, st);
     return GD2_OK;
 }
 
@@ -980,11 +1131,15 @@ int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, doub
     if (!out) return GD2_ERR_ARG;
     *out = nullptr;
     if (n < 1 || !grids || !sc || !U || !scheme_ok(sc) || sc->physics != GD2_EULER) return GD2_ERR_ARG;
-    // the ensemble's stage kernels are built for the HLLC, exact and HLL fluxes
+    // the ensemble's kernels are built for the HLLC, exact and HLL fluxes
     if (is_hllc_variant(sc->flux)) return GD2_ERR_UNSUPPORTED;
     if (sc->integrator != GD2_SSPRK2 || !(sc->cfl > 0.0)) return GD2_ERR_UNSUPPORTED;
-    for (int p = 0; p < n; ++p)
+    for (int p = 0; p < n; ++p) {
         if (!grid_ok(&grids[p], false) || !U[p]) return GD2_ERR_ARG;
+        // the one-pass step builds the boundary rows and columns of its
+        // intermediate state from the first and last two of them
+        if (grids[p].nx < 2 || grids[p].ny < 2) return GD2_ERR_UNSUPPORTED;
+    }
     if (!diag_ptr()) return GD2_ERR_HIP;
     gd2_ensemble* e = new (std::nothrow) gd2_ensemble();
     if (!e) return GD2_ERR_NOMEM;
@@ -996,11 +1151,23 @@ int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, doub
     hipError_t err = hipMalloc(&e->scratch, sizeof(double) * total);
     if (err == hipSuccess) err = hipMalloc(&e->clocks, sizeof(Clock) * n);
     if (err == hipSuccess) err = hipMalloc(&e->dprobs, sizeof(EnsProblem) * n);
+    // sync words: heads, flag, per-problem completions and per-problem totals
+    if (err == hipSuccess) err = hipMalloc(&e->sync, sizeof(unsigned) * (kEnsSyncHead + 2 * n));
+    if (err == hipSuccess) err = hipMemset(e->sync, 0, sizeof(unsigned) * (kEnsSyncHead + 2 * n));
+    if (err == hipSuccess) err = hipMalloc(&e->hist, sizeof(unsigned long long) * kEnsHist * n);
     if (err != hipSuccess) {
         g_last_hip = (int)err;
         gd2_ensemble_destroy(e);
         return GD2_ERR_NOMEM;
     }
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ens_rk2_symbol(sc->order, sc->flux), kWave, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 8;
+    e->resident = cus * per_cu;
     size_t off = 0;
     for (int p = 0; p < n; ++p) {
         EnsProblem& P = e->host[p];
@@ -1008,6 +1175,7 @@ int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, doub
         P.U = U[p];
         P.S = e->scratch + off;
         P.clk = e->clocks + p;
+        P.clk_cfl = sc->cfl;
         off += (size_t)4 * grids[p].nx * grids[p].ny;
     }
     int rc = ens_build_tiles(e);
@@ -1025,6 +1193,9 @@ int gd2_ensemble_destroy(gd2_ensemble* e) {
     ens_drop_graphs(e);
     if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
     if (e->dtiles) (void)hipFree(e->dtiles);
+    if (e->dtiles1) (void)hipFree(e->dtiles1);
+    if (e->sync) (void)hipFree(e->sync);
+    if (e->hist) (void)hipFree(e->hist);
     if (e->dprobs) (void)hipFree(e->dprobs);
     if (e->clocks) (void)hipFree(e->clocks);
     if (e->scratch) (void)hipFree(e->scratch);
@@ -1041,9 +1212,9 @@ int gd2_ensemble_set_tiling(gd2_ensemble* e, int rows, int blocks) {
 
 int gd2_ensemble_get_tiling(gd2_ensemble* e, int* rows, int* blocks, int* ntiles) {
     if (!e) return GD2_ERR_ARG;
-    if (rows) *rows = e->rows;
-    if (blocks) *blocks = e->blocks;
-    if (ntiles) *ntiles = e->ntiles;
+    if (rows) *rows = e->rows1;
+    if (blocks) *blocks = e->blocks > 0 ? e->blocks : e->resident;
+    if (ntiles) *ntiles = e->ntiles1;
     return GD2_OK;
 }
 
@@ -1068,7 +1239,7 @@ int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* strea
     hipStream_t st = (hipStream_t)stream;
     if (nsteps == 0) return GD2_OK;
     if (!use_graph) {
-        int rc = ens_enqueue_steps(e, nsteps, e->host_steps, st);
+        int rc = ens_enqueue_steps(e, nsteps, (int)(e->host_steps % 3), st);
         if (rc) return rc;
         e->host_steps += nsteps;
         return GD2_OK;
@@ -1100,8 +1271,17 @@ int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* strea
 int gd2_ensemble_time(gd2_ensemble* e, int p, double* t, long long* steps, double* dt, void* stream) {
     if (!e || p < 0 || p >= e->n) return GD2_ERR_ARG;
     Clock h;
+    std::vector<unsigned> sy((size_t)kEnsSyncHead + 2 * e->n);
     GD2_HIP(hipMemcpyAsync(&h, e->clocks + p, sizeof(Clock), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    GD2_HIP(hipMemcpyAsync(sy.data(), e->sync, sizeof(unsigned) * sy.size(), hipMemcpyDeviceToHost,
+                           (hipStream_t)stream));
     GD2_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (sy[kEnsFlag]) {  // a dependency wait reached its spin limit: report it once
+        GD2_HIP(hipMemset(e->sync + kEnsFlag, 0, sizeof(unsigned)));
+        return GD2_ERR_HIP;
+    }
+    for (int q = 0; q < e->n; ++q)  // the last persistent launch took every tile of every problem
+        if (sy[kEnsSyncHead + q] != sy[kEnsSyncHead + e->n + q]) return GD2_ERR_HIP;
     if (t) *t = h.t;
     if (steps) *steps = h.step;
     if (dt) *dt = h.dt;
@@ -1111,28 +1291,33 @@ int gd2_ensemble_time(gd2_ensemble* e, int p, double* t, long long* steps, doubl
 int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows_out, int* blocks_out, double* best_ms) {
     if (!e) return GD2_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    // candidate tile heights and resident-wave counts; each candidate is
-    // timed on stage-1 launches, which write only the scratch states
-    const int rows_c[] = {8, 16, 32, 64};
-    const int blocks_c[] = {0, 2048, 4096, 8192, 16384};
+    // candidate strip heights and persistent-launch sizes, each timed on a
+    // dry launch of kTuneSteps steps (reads the states, writes only the
+    // library's second buffers, leaves the clocks alone); launches of a few
+    // steps favour short strips (their start and end weigh more), so the
+    // dry launch is long enough to time the steady state
+    constexpr int kTuneSteps = 16;
+    const int rows_c[] = {32, 64, 128, 256};
+    const int blocks_c[] = {(3 * e->resident) / 4, e->resident};
+    const int slot = (int)(e->host_steps % 3);
     hipEvent_t a, b;
     GD2_HIP(hipEventCreate(&a));
     GD2_HIP(hipEventCreate(&b));
     double best = 1e300;
     int br = e->rows, bb = e->blocks;
+    int rc = GD2_OK;
     for (int r : rows_c) {
         e->rows = r;
-        int rc = ens_build_tiles(e);
-        if (rc) return rc;
+        rc = ens_build_tiles(e);
+        if (rc) break;
         for (int blk : blocks_c) {
+            if (blk < 1) continue;
             e->blocks = blk;
-            rc = ens_launch_stage<1>(e, 0, st);  // warm-up
-            if (rc) return rc;
+            rc = ens_launch_rk2(e, kTuneSteps, slot, 1, st);  // warm-up
+            if (rc) break;
             GD2_HIP(hipEventRecord(a, st));
-            for (int k = 0; k < 3; ++k) {
-                rc = ens_launch_stage<1>(e, 0, st);
-                if (rc) return rc;
-            }
+            rc = ens_launch_rk2(e, kTuneSteps, slot, 1, st);
+            if (rc) break;
             GD2_HIP(hipEventRecord(b, st));
             GD2_HIP(hipEventSynchronize(b));
             float ms = 0.f;
@@ -1143,16 +1328,18 @@ int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows_out, int* blocks_
                 bb = blk;
             }
         }
+        if (rc) break;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     e->rows = br;
     e->blocks = bb;
-    int rc = ens_build_tiles(e);
+    const int rc2 = ens_build_tiles(e);
     if (rc) return rc;
-    if (rows_out) *rows_out = br;
-    if (blocks_out) *blocks_out = bb;
-    if (best_ms) *best_ms = best / 3.0;
+    if (rc2) return rc2;
+    if (rows_out) *rows_out = e->rows1;
+    if (blocks_out) *blocks_out = e->blocks > 0 ? e->blocks : e->resident;
+    if (best_ms) *best_ms = best / kTuneSteps;
     return GD2_OK;
 }
 

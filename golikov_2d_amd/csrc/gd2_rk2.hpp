@@ -61,8 +61,13 @@ __device__ __forceinline__ PrimE lds_get68(const double (*ring)[4][kRk2Row], int
 
 __device__ __forceinline__ bool prim_pos(const PrimE& q) { return q.r > kPosMin && q.p > kPosMin; }
 
-template <int ORDER, int FLUX, bool KEEP>
-__device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int ty, Rk2Lds& lds) {
+// Returns the wave's maximum of the output wave speed (the same on every
+// lane); the caller folds it into the clock. TURN: the issue-priority table
+// between the two waves of a SIMD (for launches whose waves run one strip
+// each and should finish together; persistent waves take the next strip
+// instead and leave it off).
+template <int ORDER, int FLUX, bool KEEP, bool TURN = true>
+__device__ __forceinline__ double euler_rk2_tile(const EulerArgs& A, int tx, int ty, Rk2Lds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kRk2W;
     const int c = cbase - 2 + lane;
@@ -425,8 +430,10 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
 
     // the wave behind on its SIMD takes the higher issue priority each pass
     SimdTurn turn;
-    turn.init();
-    auto set_turn = [&](int pass) { turn.step(pass); };
+    if constexpr (TURN) turn.init();
+    auto set_turn = [&](int pass) {
+        if constexpr (TURN) turn.step(pass);
+    };
 
     const int s_begin = j0 + 2;            // first pass that updates a row
     const int s_end = min(r_last, j1 + 1);  // last pass whose U1 row the producer forms
@@ -443,10 +450,8 @@ __device__ __forceinline__ void euler_rk2_tile(const EulerArgs& A, int tx, int t
         pass_general(r);
         ++r;
     }
-    turn.finish();
-
-    const double w = wave_max(wmax);
-    if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
+    if constexpr (TURN) turn.finish();
+    return wave_max(wmax);
 }
 
 template <int ORDER, int FLUX, bool KEEP>
@@ -454,7 +459,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2
     __shared__ Rk2Lds lds;
     if (A.clk && blockIdx.x == 0 && threadIdx.x == 0) clock_commit(A.clk, A.clk_slot, A.dt_log, A.dt_cap);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    euler_rk2_tile<ORDER, FLUX, KEEP>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
+    const double w = euler_rk2_tile<ORDER, FLUX, KEEP>(A, tile % A.tiles_x, tile / A.tiles_x, lds);
+    if (threadIdx.x == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
 }
 
 }  // namespace gd2

@@ -243,35 +243,46 @@ int gd2_ctx_time(gd2_ctx* ctx, double* t, long long* steps, double* dt, void* st
 
 /* ----------------------------------------------------------------------
  * Ensembles: n independent Euler problems (SSP-RK2, each with its own grid,
- * boundary kinds and CFL step) advanced together. Each step is three launches
- * for the whole ensemble (per-problem clocks, stage 1, stage 2) over a device
- * work list of (problem, tile) entries; gd2_ensemble_advance can capture the
- * steps in a hipGraph and replay it. No reference counterpart (the reference
- * steps a single grid); results are bitwise those of running each problem
- * alone through gd2_ctx_advance.
+ * boundary kinds and CFL step) advanced together. gd2_ensemble_advance runs
+ * up to 64 steps in one persistent launch for the whole ensemble: resident
+ * waves take (step, tile) entries of device work lists in step order (one
+ * list per XCD when the problems can be spread evenly over the XCDs), and a
+ * tile of a problem's step waits until that problem's previous step is
+ * complete (per-problem clocks and completion counters on the device; a wait
+ * that exceeds its spin limit stops waiting and is reported as GD2_ERR_HIP by
+ * the next gd2_ensemble_time, as is a launch that left tiles untaken). An odd
+ * step count takes its first step as two stage launches. gd2_ensemble_advance can capture its launches in a hipGraph
+ * and replay it. No reference counterpart (the reference steps a single
+ * grid); results are bitwise those of running each problem alone through
+ * gd2_ctx_advance. Launches on one ensemble must not overlap (they share its
+ * work list and second state buffers).
  * ---------------------------------------------------------------------- */
 typedef struct gd2_ensemble gd2_ensemble;
 
-/* U[p] is problem p's caller-owned SoA conserved state (4 planes). Allocates
- * the stage scratch for all problems and the device descriptors. */
+/* U[p] is problem p's caller-owned SoA conserved state (4 planes, nx, ny >=
+ * 2). Allocates a second state for every problem, the device descriptors and
+ * the work lists. */
 int gd2_ensemble_create(int n, const gd2_grid* grids, const gd2_scheme* sc, double* const* U,
                         gd2_ensemble** out);
 int gd2_ensemble_destroy(gd2_ensemble* e);
-/* Tile height (rows per wave strip, 0 = per-problem default) and number of
- * resident waves per stage launch (0 = one wave per tile). */
+/* Strip height of the persistent launch's tiles (rows per wave, 0 = default,
+ * at least 2) and its number of waves (0 = as many as fit on the device). */
 int gd2_ensemble_set_tiling(gd2_ensemble* e, int rows, int blocks);
+/* The strip height and wave count in use, and the tiles per step. */
 int gd2_ensemble_get_tiling(gd2_ensemble* e, int* rows, int* blocks, int* ntiles);
 /* Reset every clock to t = 0 and compute each problem's first dt. */
 int gd2_ensemble_start(gd2_ensemble* e, void* stream);
 /* Advance every problem by nsteps; use_graph != 0 replays a captured hipGraph
  * of the nsteps steps (captured on first use, cached). */
 int gd2_ensemble_advance(gd2_ensemble* e, int nsteps, int use_graph, void* stream);
-/* Synchronises the stream; reads problem p's clock. */
+/* Synchronises the stream; reads problem p's clock. GD2_ERR_HIP if a
+ * persistent launch since the previous call hit its wait limit. */
 int gd2_ensemble_time(gd2_ensemble* e, int p, double* t, long long* steps, double* dt, void* stream);
-/* Times stage launches over candidate tile heights and resident-wave counts
- * (writing only the scratch states) and keeps the fastest. Call after
- * gd2_ensemble_start. Synchronises the stream. */
-int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, double* best_stage_ms);
+/* Times dry persistent launches (they read the states and write only the
+ * library's second buffers; clocks untouched) over candidate strip heights
+ * and wave counts and keeps the fastest. Call after gd2_ensemble_start.
+ * Synchronises the stream; best_step_ms is the best time per step. */
+int gd2_ensemble_tune(gd2_ensemble* e, void* stream, int* rows, int* blocks, double* best_step_ms);
 
 /* Record each taken step's dt into dt_log[step] (device, capacity entries);
  * capacity 0 switches the log off. */

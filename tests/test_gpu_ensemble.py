@@ -1,6 +1,8 @@
 """Batched ensembles on the GPU: bitwise equal to the per-problem path, to
-itself under hipGraph replay and under every tiling the tuner may choose,
-and within the run tolerance of the oracle."""
+itself under hipGraph replay and under every tiling the tuner may choose
+(including a persistent launch of one wave, which takes the whole work list
+in order, and launches whose waves wait on each other's steps), with odd and
+even step counts, and within the run tolerance of the oracle."""
 import numpy as np
 import pytest
 import torch
@@ -27,26 +29,54 @@ def make_problems(count, lo, hi, seed):
     return grids, ogrids, states
 
 
-@pytest.mark.parametrize("flux", [api.FLUX_HLLC, api.FLUX_HLL])
-def test_ensemble_bitwise_equals_individual_runs(flux):
-    say(f"ensemble vs individual flux={flux}")
+@pytest.mark.parametrize("flux,order,steps", [(api.FLUX_HLLC, 2, 10), (api.FLUX_HLL, 2, 10), (api.FLUX_HLLC, 2, 7),
+                                              (api.FLUX_HLLC, 1, 5), (api.FLUX_EXACT, 2, 4), (api.FLUX_EXACT, 1, 3)])
+def test_ensemble_bitwise_equals_individual_runs(flux, order, steps):
+    say(f"ensemble vs individual flux={flux} order={order} steps={steps}")
     grids, _, states = make_problems(12, 16, 200, seed=3)
-    sc = api.make_scheme(api.EULER, flux)
+    sc = api.make_scheme(api.EULER, flux, order)
     ens_U = [to_dev(U) for U in states]
     ind_U = [to_dev(U) for U in states]
     e = api.Ensemble(grids, sc, ens_U)
     e.start()
-    e.advance(10)
+    e.advance(steps)
     for g, U in zip(grids, ind_U):
         s = api.Solver(g, sc)
         s.start(U)
-        s.advance(U, 10)
+        s.advance(U, steps)
         s.close()
     torch.cuda.synchronize()
     for p, (a, b) in enumerate(zip(ens_U, ind_U)):
         assert torch.equal(a, b), p
         t, n, _ = e.time(p)
-        assert n == 10 and t > 0
+        assert n == steps and t > 0
+    e.close()
+
+
+def test_ensemble_waves_waiting_on_each_other():
+    """40 problems, 20 of them single-tile (16..40 cells a side): with 96
+    waves consecutive steps of one problem are taken by different waves, which
+    must wait for each other; 12 steps, bitwise equal to per-problem runs."""
+    say("ensemble dependent waves")
+    g1, _, s1 = make_problems(20, 16, 40, seed=11)
+    g2, _, s2 = make_problems(20, 60, 300, seed=12)
+    grids, states = g1 + g2, s1 + s2
+    sc = api.make_scheme()
+    ens_U = [to_dev(U) for U in states]
+    ind_U = [to_dev(U) for U in states]
+    e = api.Ensemble(grids, sc, ens_U)
+    e.set_tiling(32, 96)
+    e.start()
+    e.advance(12, use_graph=True)
+    for g, U in zip(grids, ind_U):
+        s = api.Solver(g, sc)
+        s.start(U)
+        s.advance(U, 12)
+        s.close()
+    torch.cuda.synchronize()
+    for p, (a, b) in enumerate(zip(ens_U, ind_U)):
+        assert torch.equal(a, b), p
+        assert e.time(p)[1] == 12
     e.close()
 
 
@@ -79,7 +109,7 @@ def test_ensemble_graph_replays_match_eager():
     eb.close()
 
 
-@pytest.mark.parametrize("rows,blocks", [(8, 0), (64, 0), (16, 37), (32, 1)])
+@pytest.mark.parametrize("rows,blocks", [(8, 0), (64, 0), (16, 37), (32, 1), (2, 5), (1, 3), (256, 0), (4, 100000)])
 def test_ensemble_result_independent_of_tiling(rows, blocks):
     say(f"ensemble tiling rows={rows} blocks={blocks}")
     grids, _, states = make_problems(7, 16, 130, seed=5)
@@ -109,7 +139,8 @@ def test_ensemble_tuner_keeps_results():
     ea.start()
     eb.start()
     rows, blocks, ms = eb.tune()
-    assert rows in (8, 16, 32, 64) and blocks >= 0 and ms > 0
+    assert rows in (32, 64, 128, 256) and blocks > 0 and ms > 0
+    assert eb.tiling()[:2] == (rows, blocks)
     for p in range(len(grids)):  # tuning writes only scratch
         assert torch.equal(Ub[p], Ua[p])
     ea.advance(4)
