@@ -760,13 +760,17 @@ __device__ __forceinline__ void atomic_min_nonneg(unsigned long long* dst, doubl
 // neighbours' halo rows and columns then sit in the same L2).
 // Issue priority between the waves sharing a SIMD. The older of two waves
 // wins issue ties, so one would finish far ahead and leave the other to run
-// alone. Each pass a wave publishes its pass count in its SIMD's row of
-// g_simd_passes (slot = hardware wave slot; passes + 1 while it runs, 0 when
-// it has none; 16 slots for each of the 8 x 256 x 4 SIMDs) and reads the row
-// one pass earlier: the wave that is behind takes the higher priority, level
-// waves alternate by pass and slot parity. (The table costs a store and a
-// load per pass, which suits the one-pass step's long passes; the stage VJP's
-// short passes measured slower with it than with a plain parity alternation.)
+// alone. Every kTurnPeriod passes a wave publishes its count of such turns in
+// its SIMD's row of g_simd_passes (slot = hardware wave slot; turns + 1 while
+// it runs, 0 when it has none; 16 slots for each of the 8 x 256 x 4 SIMDs)
+// and reads the row one turn earlier: the wave that is behind takes the
+// higher priority, level waves alternate by turn and slot parity. Measured on
+// the one-pass step at 4096^2 (ms per step): a turn every pass 0.554, none
+// 0.500, every 2 passes 0.479, every 4 0.463-0.469, every 8 0.467-0.471,
+// every 16 0.471, every 32 0.500. Re-deciding priority every pass keeps the
+// two waves in step, so they wait for memory at the same time; every few
+// passes they interleave and still finish together.
+constexpr int kTurnPeriod = 4;
 constexpr int kSimdSlots = 16;
 // one table per translation unit (waves of one kernel share its table)
 static __device__ unsigned g_simd_passes[8 * 256 * 4 * kSimdSlots];

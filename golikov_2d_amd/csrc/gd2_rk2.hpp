@@ -428,11 +428,14 @@ __device__ __forceinline__ double euler_rk2_tile(const EulerArgs& A, int tx, int
         consume_y(r - 2, q1, qpos, true, X2, hj);
     };
 
-    // the wave behind on its SIMD takes the higher issue priority each pass
+    // the wave behind on its SIMD takes the higher issue priority, decided
+    // every kTurnPeriod passes
     SimdTurn turn;
     if constexpr (TURN) turn.init();
     auto set_turn = [&](int pass) {
-        if constexpr (TURN) turn.step(pass);
+        if constexpr (TURN) {
+            if ((pass & (kTurnPeriod - 1)) == 0) turn.step(pass / kTurnPeriod);
+        }
     };
 
     const int s_begin = j0 + 2;            // first pass that updates a row
