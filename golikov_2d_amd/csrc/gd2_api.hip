@@ -316,14 +316,20 @@ static unsigned grid_stride_blocks(long long n) {
 }
 
 template <int ORDER, int METOD>
-static int launch_mhd_vjp(const MhdAdjArgs& J, hipStream_t st) {
+static int launch_mhd_vjp(MhdAdjArgs J, hipStream_t st) {
     hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, true>), dim3(grid_stride_blocks(J.nfx)), dim3(256), 0, st,
                        J);
     GD2_HIP(hipGetLastError());
     hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, false>), dim3(grid_stride_blocks(J.nfy)), dim3(256), 0, st,
                        J);
     GD2_HIP(hipGetLastError());
-    hipLaunchKernelGGL((mhd_vjp_cell_kernel<ORDER>), dim3(grid_stride_blocks(J.plane)), dim3(256), 0, st, J);
+    J.tiles_x = (J.nx + kGatherW - 1) / kGatherW;
+    J.rows = strip_rows(reinterpret_cast<const void*>(&mhd_vjp_gather_kernel<ORDER>), J.tiles_x, J.ny, 4);
+    const unsigned blocks = (unsigned)(J.tiles_x * ((J.ny + J.rows - 1) / J.rows));
+    hipLaunchKernelGGL((mhd_vjp_gather_kernel<ORDER>), dim3(blocks), dim3(kWave), 0, st, J);
+    GD2_HIP(hipGetLastError());
+    // the boundary band: up to 4 columns x ny + 4 rows x nx cells
+    hipLaunchKernelGGL(mhd_vjp_fold_kernel, dim3(grid_stride_blocks(4LL * (J.nx + J.ny))), dim3(256), 0, st, J);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
@@ -348,6 +354,8 @@ static int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* 
     J.nfy = (long long)g->nx * (g->ny + 1);
     J.wx = static_cast<double*>(workspace);
     J.wy = J.wx + 14 * J.nfx;
+    J.gx = J.wy + 14 * J.nfy;
+    J.gy = J.gx + 28LL * g->ny;
     J.dx = g->dx;
     J.dy = g->dy;
     J.dV = g->dx * g->dy;
@@ -640,9 +648,10 @@ int gd2_ctx_log_dt(gd2_ctx* c, double* dt_log, long long capacity) {
 long long gd2_vjp_workspace_bytes(const gd2_grid* g, const gd2_scheme* sc) {
     if (!g || !sc || g->nx < 1 || g->ny < 1) return GD2_ERR_ARG;
     if (sc->physics == GD2_MHD) {
-        // face-state cotangents: 14 planes over the x-faces and the y-faces
+        // face-state cotangents: 14 planes over the x-faces and the y-faces;
+        // the gathers of the 4 ghost columns and the 4 ghost rows (7 each)
         const long long nf = (long long)(g->nx + 1) * g->ny + (long long)g->nx * (g->ny + 1);
-        return (long long)sizeof(double) * 14 * nf;
+        return (long long)sizeof(double) * (14 * nf + 28LL * (g->nx + g->ny));
     }
     const long long len = g->nx > g->ny ? g->nx : g->ny;
     return (long long)sizeof(double) * 4 * 2 * len * 4;

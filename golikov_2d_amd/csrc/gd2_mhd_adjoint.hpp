@@ -13,12 +13,16 @@
 //     (planes 0..6: left state, 7..13: right).
 //     Reflective west / south wall faces are evaluated mirrored, as the
 //     forward stage does.
-//  2. one thread per cell: gathers the face-state cotangents through the
-//     minmod reconstruction of itself and its two neighbours on each line
-//     (the branch the forward takes, 1/2-1/2 on ties, the rho / p <= 0 face
-//     fall-back as a copy), adds the gathers of the boundary ghost cells that
-//     copy it (mirrored components negated), adds the derivatives of the
-//     Powell term, maps primitive to conserved cotangents and forms W.
+//  2. a marching gather (one wave per strip of columns, up its rows): every
+//     cell spreads the cotangents of its two face states on each line over
+//     its minmod stencil once (the branch the forward takes, 1/2-1/2 on ties,
+//     the rho / p <= 0 face fall-back as a copy); a cell's gather is its own
+//     share plus its neighbours' shares (lane shifts along x, a register
+//     march along y); it adds the derivatives of the Powell term, maps
+//     primitive to conserved cotangents and forms W. The gathers of the
+//     boundary ghost cells go to side buffers, and a fold kernel adds them to
+//     the cells they copy (mirrored components negated), with the Powell
+//     terms of the edge cells that read those copies.
 //
 // The forward stage's clamps are not differentiated (they do not fire at the
 // smooth states an adjoint is linearised at).
@@ -689,6 +693,9 @@ struct MhdAdjArgs {
     double a, b, c;   // W = a*A + b*B + c*J^T V
     double* wx;       // x-face state cotangents: 14 planes of (nx+1) x ny
     double* wy;       // y-face state cotangents: 14 planes of nx x (ny+1)
+    double* gx;       // gathers of the ghost columns -2, -1, nx, nx+1: [4][ny][7]
+    double* gy;       // gathers of the ghost rows -2, -1, ny, ny+1: [4][nx][7]
+    int tiles_x, rows;  // strips of the gather march
     int nx, ny;
     long long plane;
     long long nfx, nfy;
@@ -804,56 +811,6 @@ __global__ __launch_bounds__(256) void mhd_vjp_face_kernel(const MhdAdjArgs J) {
     }
 }
 
-// Primitive cotangent of cell e (extended index, -2 .. n+1) of a line (row
-// `line` for x, column `line` for y) gathered from the face-state cotangents
-// through its own reconstruction and those of cells e-1 and e+1.
-template <int ORDER, bool XDIR>
-__device__ __forceinline__ void gather_line_m(const MhdAdjArgs& J, int e, int line, double (&G)[7]) {
-    const int n = XDIR ? J.nx : J.ny;
-    const long long nf = XDIR ? J.nfx : J.nfy;
-    const double* w = XDIR ? J.wx : J.wy;
-    // plane c (left state) or 7 + c (right state) of face f + 1/2; 0 for
-    // faces outside f = -1 .. n-1
-    auto wface = [&](int f, int pl) -> double {
-        if (f < -1 || f > n - 1) return 0.0;
-        const long long id = XDIR ? (long long)line * (n + 1) + (f + 1) : (long long)(f + 1) * J.nx + line;
-        return w[pl * nf + id];
-    };
-    if constexpr (ORDER == 1) {
-#pragma unroll
-        for (int c = 0; c < 7; ++c) G[c] = wface(e, c) + wface(e - 1, 7 + c);
-        return;
-    } else {
-        PrimM q[5];
-#pragma unroll
-        for (int o = 0; o < 5; ++o) q[o] = XDIR ? adj_cell_m(J, e - 2 + o, line) : adj_cell_m(J, line, e - 2 + o);
-#pragma unroll
-        for (int c = 0; c < 7; ++c) {
-            double acc = 0.0;
-            // cell e-1+m (m = 0, 1, 2) spreads the cotangents of its plus value
-            // (left state of face e-1+m+1/2) and minus value (right state of
-            // face e-1+m-1/2) over its stencil; cell e takes its share
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const int cc = e - 1 + m;
-                const double dqp = wface(cc, c), dqm = wface(cc - 1, 7 + c);
-                const double qa = comp_m(q[m], c), qb = comp_m(q[m + 1], c), qd = comp_m(q[m + 2], c);
-                const MMW wt = mm_weights(qb - qa, qd - qb);
-                bool fp = false, fm = false;
-                if (c == 0 || c == 3) {
-                    const double s = minmod(qb - qa, qd - qb);
-                    fp = !(qb + 0.5 * s > 0.0);
-                    fm = !(qb + -0.5 * s > 0.0);
-                }
-                double own, to_prev, to_next;
-                spread(dqp, dqm, wt, fp, fm, own, to_prev, to_next);
-                acc += m == 0 ? to_next : (m == 1 ? own : to_prev);
-            }
-            G[c] = acc;
-        }
-    }
-}
-
 // K(q) . V of the Powell term of a cell (its derivative with respect to the
 // cell's divergence estimate), times -1/dV.
 __device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, long long k) {
@@ -864,109 +821,349 @@ __device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, 
     return -K / J.dV;
 }
 
+// Primitives of a cell from its 7 conserved values (the expressions of
+// cons_to_prim_m, so the gather sees the values the face kernels see).
+__device__ __forceinline__ PrimM prim_of_cons7(const double (&w)[7], double g1) {
+    PrimM q;
+    const double r = w[0], mx = w[1], my = w[2];
+    q.bx = w[3];
+    q.by = w[4];
+    q.bz = w[5];
+    const double E = w[6];
+    const double inv = fast_rcp(r);
+    q.r = r;
+    q.u = mx * inv;
+    q.v = my * inv;
+    q.p = g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
+    return q;
+}
+
+// One cell's shares of the reconstruction adjoint along one line: qa, qb, qd
+// the cell before, the cell itself and the cell after; dqp the cotangent of
+// its plus-face value (left state of the face after it), dqm of its
+// minus-face value (right state of the face before it).
 template <int ORDER>
-__global__ __launch_bounds__(256) void mhd_vjp_cell_kernel(const MhdAdjArgs J) {
-    const long long ncell = J.plane;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % J.nx), j = (int)(t / J.nx);
-        double Pq[7], G[7];
-        gather_line_m<ORDER, true>(J, i, j, Pq);
-        // ghost columns that copy column i
+__device__ __forceinline__ void spread7(const PrimM& qa, const PrimM& qb, const PrimM& qd, const double (&dqp)[7],
+                                        const double (&dqm)[7], double (&own)[7], double (&to_prev)[7],
+                                        double (&to_next)[7]) {
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int gc = g < 2 ? g - 2 : J.nx + g - 2;
-            bool fl;
-            if (map_index(gc, J.nx, J.bc[0], J.bc[1], fl) != i) continue;
-            gather_line_m<ORDER, true>(J, gc, j, G);
-            if (fl) {
-                G[1] = -G[1];
-                G[4] = -G[4];
+    for (int k = 0; k < 7; ++k) {
+        if constexpr (ORDER == 1) {
+            own[k] = dqp[k] + dqm[k];
+            to_prev[k] = to_next[k] = 0.0;
+        } else {
+            const double a = comp_m(qa, k), b = comp_m(qb, k), d = comp_m(qd, k);
+            const MMW wt = mm_weights(b - a, d - b);
+            bool fp = false, fm = false;
+            if (k == 0 || k == 3) {
+                const double sl = minmod(b - a, d - b);
+                fp = !(b + 0.5 * sl > 0.0);
+                fm = !(b + -0.5 * sl > 0.0);
             }
-#pragma unroll
-            for (int c = 0; c < 7; ++c) Pq[c] += G[c];
+            spread(dqp[k], dqm[k], wt, fp, fm, own[k], to_prev[k], to_next[k]);
         }
-        gather_line_m<ORDER, false>(J, j, i, G);
+    }
+}
+
+// primitive -> conserved cotangent at a cell:
+// u = mx/r, v = my/r, p = g1 (E - (mx u + my v)/2 - |B|^2 / 8pi)
+__device__ __forceinline__ void prim_to_cons_cot(const MhdAdjArgs& J, const PrimM& q, const double (&Pq)[7],
+                                                 double (&T)[7]) {
+    const double ir = 1.0 / q.r;
+    const double gp = J.g1 * Pq[3];
+    T[0] = Pq[0] - (q.u * Pq[1] + q.v * Pq[2]) * ir + gp * 0.5 * (q.u * q.u + q.v * q.v);
+    T[1] = Pq[1] * ir - gp * q.u;
+    T[2] = Pq[2] * ir - gp * q.v;
+    T[3] = Pq[4] - gp * q.bx * (1.0 / kCpi4);
+    T[4] = Pq[5] - gp * q.by * (1.0 / kCpi4);
+    T[5] = Pq[6] - gp * q.bz * (1.0 / kCpi4);
+    T[6] = gp;
+}
+
+constexpr int kGatherW = 60;  // output columns per wave of the gather march
+
+// The gather march. Lane l holds column c = cbase - 2 + l (cbase = 60 tx);
+// lanes 2..61 own the strip's cells. Per row every lane loads its cell
+// (through the boundary map) and the two x-face-state cotangents it owns
+// (left state of face c+1/2, right state of face c-1/2), spreads them, and a
+// cell's x-gather is lane l-1's share to its next cell + its own share +
+// lane l+1's share to its previous cell. Along y the shares are carried: row
+// j is finished once the spread of row j+1 exists. The strip at the bottom
+// (top) of the grid also marches the ghost rows -2, -1 (ny, ny+1), and the
+// strips at the sides hold the ghost columns in lanes: their gathers go to
+// the side buffers gx / gy, each written by one wave.
+template <int ORDER>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_gather_kernel(const MhdAdjArgs J) {
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+    const int nx = J.nx, ny = J.ny;
+    const int cbase = tx * kGatherW;
+    const int c = cbase - 2 + lane;
+    const int j0 = ty * J.rows, j1 = min(j0 + J.rows, ny);
+    const int ja = j0 == 0 ? -2 : j0;       // first row whose gather this wave forms
+    const int jb = j1 == ny ? ny + 1 : j1 - 1;  // last
+    const bool col_in = c >= 0 && c < nx;
+    const bool out_lane = col_in && lane >= 2 && lane <= 61;
+    // ghost column of this lane (0..3 for -2, -1, nx, nx+1; -1 otherwise),
+    // written by the one wave that holds the ghost on a lane where its
+    // gather is complete (nx needs lane l+1)
+    int gslot = -1;
+    if (c == -2 || c == -1) gslot = c + 2;
+    else if (c == nx && lane <= 62) gslot = 2;
+    else if (c == nx + 1) gslot = 3;
+    bool fx;
+    const int ci = map_index(c, nx, J.bc[0], J.bc[1], fx);
+    const unsigned oc = (unsigned)ci * 8u;
+    const unsigned ocol = col_in ? (unsigned)c * 8u : kOffNone;  // y-faces and V of this column
+    // x-faces: left state of face c+1/2 (id c+1), right state of face c-1/2 (id c)
+    const unsigned oxl = (c + 1 >= 0 && c + 1 <= nx) ? (unsigned)(c + 1) * 8u : kOffNone;
+    const unsigned oxr = (c >= 0 && c <= nx) ? (unsigned)c * 8u : kOffNone;
+    const long long P = J.plane;
+    const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
+
+    auto load_q = [&](int jl) -> PrimM {
+        bool fy;
+        const int jj = map_index(jl, ny, J.bc[2], J.bc[3], fy);
+        const double* b = J.U + (long long)jj * nx;
+        double w[7];
 #pragma unroll
-        for (int c = 0; c < 7; ++c) Pq[c] += G[c];
+        for (int f = 0; f < 7; ++f) w[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * P)), oc);
+        PrimM q = prim_of_cons7(w, J.g1);
+        if (fx) {
+            q.u = -q.u;
+            q.bx = -q.bx;
+        }
+        if (fy) {
+            q.v = -q.v;
+            q.by = -q.by;
+        }
+        return q;
+    };
+    // y-face-state cotangents of the cell of row r: left state of face r+1/2
+    // (face row r+1), right state of face r-1/2 (face row r); 0 outside
+    auto load_yfaces = [&](int r, double (&dqp)[7], double (&dqm)[7]) {
+        const bool hp = r + 1 >= 0 && r + 1 <= ny, hm = r >= 0 && r <= ny;
+        const double* wp = J.wy + (long long)(hp ? r + 1 : 0) * nx;
+        const double* wm = J.wy + (long long)(hm ? r : 0) * nx;
+        const unsigned op = hp ? ocol : kOffNone, om = hm ? ocol : kOffNone;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int gr = g < 2 ? g - 2 : J.ny + g - 2;
-            bool fl;
-            if (map_index(gr, J.ny, J.bc[2], J.bc[3], fl) != j) continue;
-            gather_line_m<ORDER, false>(J, gr, i, G);
-            if (fl) {
-                G[2] = -G[2];
-                G[5] = -G[5];
+        for (int k = 0; k < 7; ++k) {
+            dqp[k] = ld_buf(buf_rsrc(uniform_ptr(wp + k * J.nfy)), op);
+            dqm[k] = ld_buf(buf_rsrc(uniform_ptr(wm + (7 + k) * J.nfy)), om);
+        }
+    };
+    // K . V of the Powell term of this lane's cell in row r (powell_k), 0
+    // outside the grid; V of the cell in vv (mx, my, bx, by, E components)
+    auto powell_row = [&](int r, const PrimM& q, double (&vv)[5]) -> double {
+        const bool in = r >= 0 && r < ny;
+        const double* b = J.V + (long long)(in ? r : 0) * nx;
+        const unsigned o = in ? ocol : kOffNone;
+        vv[0] = ld_buf(buf_rsrc(uniform_ptr(b + P)), o);
+        vv[1] = ld_buf(buf_rsrc(uniform_ptr(b + 2 * P)), o);
+        vv[2] = ld_buf(buf_rsrc(uniform_ptr(b + 3 * P)), o);
+        vv[3] = ld_buf(buf_rsrc(uniform_ptr(b + 4 * P)), o);
+        vv[4] = ld_buf(buf_rsrc(uniform_ptr(b + 6 * P)), o);
+        const double K = (vv[0] * q.bx + vv[1] * q.by + vv[4] * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) +
+                         vv[2] * q.u + vv[3] * q.v;
+        return (in && col_in) ? -K / J.dV : 0.0;
+    };
+
+    // prologue: rows ja-2 .. ja+1; y-spreads of rows ja-1 (its share to row
+    // ja) and ja
+    PrimM qm = load_q(ja - 1), q0 = load_q(ja), qp = load_q(ja + 1);
+    double tn_prev[7], yown[7], ytn[7];
+    {
+        const PrimM qmm = load_q(ja - 2);
+        double dqp[7], dqm[7], own[7], tp[7];
+        load_yfaces(ja - 1, dqp, dqm);
+        spread7<ORDER>(qmm, qm, q0, dqp, dqm, own, tp, tn_prev);
+        load_yfaces(ja, dqp, dqm);
+        spread7<ORDER>(qm, q0, qp, dqp, dqm, yown, tp, ytn);
+    }
+    double v0[5], vtmp[5];
+    double Km = powell_row(ja - 1, qm, vtmp);
+    double K0 = powell_row(ja, q0, v0);
+
+    for (int j = ja; j <= jb; ++j) {
+        // row j+1's y-spread (rows j .. j+2) and Powell product
+        const PrimM qpp = load_q(j + 2);
+        double vp[5];
+        const double Kp = powell_row(j + 1, qp, vp);
+        double ydqp[7], ydqm[7], own1[7], tp1[7], tn1[7];
+        load_yfaces(j + 1, ydqp, ydqm);
+        spread7<ORDER>(q0, qp, qpp, ydqp, ydqm, own1, tp1, tn1);
+        double Gy[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) Gy[k] = (tn_prev[k] + yown[k]) + tp1[k];
+
+        if (j >= 0 && j < ny) {
+            // x: spreads of the row, gathers by lane shifts (all lanes)
+            double xdqp[7], xdqm[7];
+            {
+                const double* w = J.wx + (long long)j * (nx + 1);
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    xdqp[k] = ld_buf(buf_rsrc(uniform_ptr(w + k * J.nfx)), oxl);
+                    xdqm[k] = ld_buf(buf_rsrc(uniform_ptr(w + (7 + k) * J.nfx)), oxr);
+                }
             }
+            PrimM qw, qe;
+            qw.r = from_prev_lane(q0.r); qe.r = from_next_lane(q0.r);
+            qw.u = from_prev_lane(q0.u); qe.u = from_next_lane(q0.u);
+            qw.v = from_prev_lane(q0.v); qe.v = from_next_lane(q0.v);
+            qw.p = from_prev_lane(q0.p); qe.p = from_next_lane(q0.p);
+            qw.bx = from_prev_lane(q0.bx); qe.bx = from_next_lane(q0.bx);
+            qw.by = from_prev_lane(q0.by); qe.by = from_next_lane(q0.by);
+            qw.bz = from_prev_lane(q0.bz); qe.bz = from_next_lane(q0.bz);
+            double xown[7], xtp[7], xtn[7], Gx[7];
+            spread7<ORDER>(qw, q0, qe, xdqp, xdqm, xown, xtp, xtn);
 #pragma unroll
-            for (int c = 0; c < 7; ++c) Pq[c] += G[c];
+            for (int k = 0; k < 7; ++k) Gx[k] = (from_prev_lane(xtn[k]) + xown[k]) + from_next_lane(xtp[k]);
+            const double Kw = from_prev_lane(K0), Ke = from_next_lane(K0);
+            if (gslot >= 0) {
+                double* g = J.gx + ((long long)gslot * ny + j) * 7;
+#pragma unroll
+                for (int k = 0; k < 7; ++k) g[k] = Gx[k];
+            }
+            if (out_lane) {
+                double Pq[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) Pq[k] = Gx[k] + Gy[k];
+                // Powell term of this cell: -(1/dV) Pdiv K(q) with
+                // K = (0, bx/4pi, by/4pi, u, v, 0, (u bx + v by)/4pi) and
+                // Pdiv = dy (bx_e - bx_w)/2 + dx (by_n - by_s)/2 of the
+                // neighbours (ghosts through the boundary map)
+                const PrimM& q = q0;
+                const double pdiv = J.dy * (qe.bx - qw.bx) * 0.5 + J.dx * (qp.by - qm.by) * 0.5;
+                {
+                    const double vmx = v0[0], vmy = v0[1], vbx = v0[2], vby = v0[3], ve = v0[4];
+                    const double sc = -pdiv / J.dV;
+                    Pq[4] += sc * (vmx + ve * q.u) * (1.0 / kCpi4);
+                    Pq[5] += sc * (vmy + ve * q.v) * (1.0 / kCpi4);
+                    Pq[1] += sc * (vbx + ve * q.bx * (1.0 / kCpi4));
+                    Pq[2] += sc * (vby + ve * q.by * (1.0 / kCpi4));
+                }
+                // ... and the Powell terms of the neighbours in the grid whose
+                // divergence estimate reads this cell's bx (west / east) or
+                // by (south / north): K is 0 outside the grid
+                Pq[4] -= hx * Ke;
+                Pq[4] += hx * Kw;
+                Pq[5] -= hy * Kp;
+                Pq[5] += hy * Km;
+                double T[7];
+                prim_to_cons_cot(J, q, Pq, T);
+                const long long k = (long long)j * nx + c;
+#pragma unroll
+                for (int f = 0; f < 7; ++f) {
+                    double o = J.c * T[f];
+                    if (J.A) o += J.a * J.A[f * P + k];
+                    if (J.B) o += J.b * J.B[f * P + k];
+                    J.W[f * P + k] = o;
+                }
+            }
+        } else if (col_in) {
+            // a ghost row: its y-gather goes to the side buffer
+            const int gr = j == -2 ? 0 : (j == -1 ? 1 : (j == ny ? 2 : 3));
+            double* g = J.gy + ((long long)gr * nx + c) * 7;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) g[k] = Gy[k];
         }
 
-        // Powell term of this cell: -(1/dV) Pdiv K(q) with
-        // K = (0, bx/4pi, by/4pi, u, v, 0, (u bx + v by)/4pi) and
-        // Pdiv = dy (bx_e - bx_w)/2 + dx (by_n - by_s)/2 of the neighbours
-        const PrimM q = adj_cell_m(J, i, j);
-        const PrimM qw = adj_cell_m(J, i - 1, j), qe = adj_cell_m(J, i + 1, j);
-        const PrimM qs = adj_cell_m(J, i, j - 1), qn = adj_cell_m(J, i, j + 1);
-        const double pdiv = J.dy * (qe.bx - qw.bx) * 0.5 + J.dx * (qn.by - qs.by) * 0.5;
-        {
-            const long long P = J.plane;
-            const double vmx = J.V[P + t], vmy = J.V[2 * P + t], vbx = J.V[3 * P + t], vby = J.V[4 * P + t],
-                         ve = J.V[6 * P + t];
-            const double s = -pdiv / J.dV;
-            Pq[4] += s * (vmx + ve * q.u) * (1.0 / kCpi4);
-            Pq[5] += s * (vmy + ve * q.v) * (1.0 / kCpi4);
-            Pq[1] += s * (vbx + ve * q.bx * (1.0 / kCpi4));
-            Pq[2] += s * (vby + ve * q.by * (1.0 / kCpi4));
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            tn_prev[k] = ytn[k];
+            yown[k] = own1[k];
+            ytn[k] = tn1[k];
         }
-        // ... and the Powell terms of the cells whose divergence estimate
-        // reads this cell's bx (as a west / east neighbour) or by (south /
-        // north), ghosts included
-        const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
-        if (i + 1 < J.nx) Pq[4] -= hx * powell_k(J, qe, t + 1);
-        if (i - 1 >= 0) Pq[4] += hx * powell_k(J, qw, t - 1);
-        if (j + 1 < J.ny) Pq[5] -= hy * powell_k(J, qn, t + J.nx);
-        if (j - 1 >= 0) Pq[5] += hy * powell_k(J, qs, t - J.nx);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v0[k] = vp[k];
+        qm = q0;
+        q0 = qp;
+        qp = qpp;
+        Km = K0;
+        K0 = Kp;
+    }
+}
+
+// The gathers of the ghost columns and rows added to the cells they copy
+// (mirrored components negated), with the Powell terms of the edge cells
+// whose divergence estimate reads a ghost that copies the cell: one thread
+// per cell of the boundary band (columns 0, 1, nx-2, nx-1 and rows 0, 1,
+// ny-2, ny-1), W += c * (primitive -> conserved)(those cotangents).
+__global__ __launch_bounds__(256) void mhd_vjp_fold_kernel(const MhdAdjArgs J) {
+    const int nx = J.nx, ny = J.ny;
+    int xs[4], nxs = 0, ys[4], nys = 0;
+    {
+        const int cx[4] = {0, 1, nx - 2, nx - 1}, cy[4] = {0, 1, ny - 2, ny - 1};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            bool dx = cx[a] >= 0 && cx[a] < nx, dy = cy[a] >= 0 && cy[a] < ny;
+            for (int b = 0; b < a; ++b) {
+                dx = dx && cx[b] != cx[a];
+                dy = dy && cy[b] != cy[a];
+            }
+            if (dx) xs[nxs++] = cx[a];
+            if (dy) ys[nys++] = cy[a];
+        }
+    }
+    const long long n1 = (long long)nxs * ny, n2 = (long long)nys * nx;
+    const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i, j;
+        if (t < n1) {
+            i = xs[t % nxs];
+            j = (int)(t / nxs);
+        } else {
+            const long long t2 = t - n1;
+            i = (int)(t2 % nx);
+            j = ys[t2 / nx];
+            bool seen = false;
+            for (int a = 0; a < nxs; ++a) seen = seen || xs[a] == i;
+            if (seen) continue;  // a band column: done by the first part
+        }
+        double Pq[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < 4; ++g) {
+            const int gc = g < 2 ? g - 2 : nx + g - 2;
+            bool fl;
+            if (map_index(gc, nx, J.bc[0], J.bc[1], fl) != i) continue;
+            const double* G = J.gx + ((long long)g * ny + j) * 7;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) Pq[k] += (fl && (k == 1 || k == 4)) ? -G[k] : G[k];
+        }
+        for (int g = 0; g < 4; ++g) {
+            const int gr = g < 2 ? g - 2 : ny + g - 2;
+            bool fl;
+            if (map_index(gr, ny, J.bc[2], J.bc[3], fl) != j) continue;
+            const double* G = J.gy + ((long long)g * nx + i) * 7;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) Pq[k] += (fl && (k == 2 || k == 5)) ? -G[k] : G[k];
+        }
         {
             bool fl;
-            if (map_index(-1, J.nx, J.bc[0], J.bc[1], fl) == i) {
-                const long long k0 = (long long)j * J.nx;
+            if (map_index(-1, nx, J.bc[0], J.bc[1], fl) == i) {
+                const long long k0 = (long long)j * nx;
                 Pq[4] -= (fl ? -hx : hx) * powell_k(J, adj_cell_m(J, 0, j), k0);
             }
-            if (map_index(J.nx, J.nx, J.bc[0], J.bc[1], fl) == i) {
-                const long long k1 = (long long)j * J.nx + J.nx - 1;
-                Pq[4] += (fl ? -hx : hx) * powell_k(J, adj_cell_m(J, J.nx - 1, j), k1);
+            if (map_index(nx, nx, J.bc[0], J.bc[1], fl) == i) {
+                const long long k1 = (long long)j * nx + nx - 1;
+                Pq[4] += (fl ? -hx : hx) * powell_k(J, adj_cell_m(J, nx - 1, j), k1);
             }
-            if (map_index(-1, J.ny, J.bc[2], J.bc[3], fl) == j) {
+            if (map_index(-1, ny, J.bc[2], J.bc[3], fl) == j) {
                 Pq[5] -= (fl ? -hy : hy) * powell_k(J, adj_cell_m(J, i, 0), i);
             }
-            if (map_index(J.ny, J.ny, J.bc[2], J.bc[3], fl) == j) {
-                const long long k1 = (long long)(J.ny - 1) * J.nx + i;
-                Pq[5] += (fl ? -hy : hy) * powell_k(J, adj_cell_m(J, i, J.ny - 1), k1);
+            if (map_index(ny, ny, J.bc[2], J.bc[3], fl) == j) {
+                const long long k1 = (long long)(ny - 1) * nx + i;
+                Pq[5] += (fl ? -hy : hy) * powell_k(J, adj_cell_m(J, i, ny - 1), k1);
             }
         }
-
-        // primitive -> conserved cotangent at this cell:
-        // u = mx/r, v = my/r, p = g1 (E - (mx u + my v)/2 - |B|^2 / 8pi)
-        const double ir = 1.0 / q.r;
-        const double gp = J.g1 * Pq[3];
+        const PrimM q = adj_cell_m(J, i, j);
         double T[7];
-        T[0] = Pq[0] - (q.u * Pq[1] + q.v * Pq[2]) * ir + gp * 0.5 * (q.u * q.u + q.v * q.v);
-        T[1] = Pq[1] * ir - gp * q.u;
-        T[2] = Pq[2] * ir - gp * q.v;
-        T[3] = Pq[4] - gp * q.bx * (1.0 / kCpi4);
-        T[4] = Pq[5] - gp * q.by * (1.0 / kCpi4);
-        T[5] = Pq[6] - gp * q.bz * (1.0 / kCpi4);
-        T[6] = gp;
-        const long long P = J.plane;
+        prim_to_cons_cot(J, q, Pq, T);
+        const long long k = (long long)j * nx + i;
 #pragma unroll
-        for (int c = 0; c < 7; ++c) {
-            double o = J.c * T[c];
-            if (J.A) o += J.a * J.A[c * P + t];
-            if (J.B) o += J.b * J.B[c * P + t];
-            J.W[c * P + t] = o;
-        }
+        for (int f = 0; f < 7; ++f) J.W[f * J.plane + k] += J.c * T[f];
     }
 }
 
