@@ -33,14 +33,15 @@
 
 namespace gd2 {
 
-// Reverse-mode hlld_face: the same forward values as hlld_face_tg (IEEE
-// division and square root, every product rounded on its own), then one
+// Reverse-mode hlld_face: the forward values recomputed with hlld_face's
+// expressions (refined hardware reciprocal and square root, multiply-adds
+// contracted within an expression, the contact speed's numerator products
+// rounded on their own so mirror-image states give SM = 0 exactly), then one
 // reverse sweep taking the cotangent Pb of the face-frame flux to the
 // cotangents ib of the 16 face-frame inputs.
 template <int METOD>
 __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gamma, double inv_g1,
                                               const double (&Pb)[8], double (&ib)[16]) {
-#pragma clang fp contract(off)
     const double ro_L = in[0], p_L = in[1], u1 = in[2], v1 = in[3], w1 = in[4], bn1 = in[5], bt1 = in[6],
                  bm1 = in[7];
     const double ro_R = in[8], p_R = in[9], u2 = in[10], v2 = in[11], w2 = in[12], bn2 = in[13], bt2 = in[14],
@@ -51,12 +52,12 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
     const double uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
     const double uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
     const double gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R;
-    const double iroL = 1.0 / ro_L, iroR = 1.0 / ro_R;
+    const double iroL = fast_rcp(ro_L), iroR = fast_rcp(ro_R);
     const double g4 = 4.0 * gamma;
     const double dL = gpL * gpL - g4 * p_L * (bn1 * bn1), dR = gpR * gpR - g4 * p_R * (bn2 * bn2);
-    const double sdL = sqrt(dL), sdR = sqrt(dR);
+    const double sdL = fast_sqrt(dL), sdR = fast_sqrt(dR);
     const double aL = (gpL + sdL) * (0.5 * iroL), aR = (gpR + sdR) * (0.5 * iroR);
-    const double cfL = sqrt(aL), cfR = sqrt(aR);
+    const double cfL = fast_sqrt(aL), cfR = fast_sqrt(aR);
     const bool cL = cfL >= cfR;  // tmax picks its first argument on ties
     const double cmax = cL ? cfL : cfR;
     const bool uminL = u1 <= u2, umaxL = u1 >= u2;
@@ -69,7 +70,7 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
     const double suR = SR - u2, suL = SL - u1;
     const double mR = suR * ro_R, mL = suL * ro_L;
     const double Dm = mR - mL;
-    const double idm = 1.0 / Dm;
+    const double idm = fast_rcp(Dm);
     const double smR = mR * u2, smLv = mL * u1;
     const double Nn = smR - smLv - pTR + pTL;
     const double SM = Nn * idm;
@@ -99,23 +100,29 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
     UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2;
     UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2;
     const double DS = SR - SL;
-    const double iS = 1.0 / DS;
+    const double iS = fast_rcp(DS);
     const double nbn = SR * bn2 - SL * bn1;
     const double nbt = SR * bt2 - SL * bt1 + FL[6] - FR[6];
     const double nbm = SR * bm2 - SL * bm1 + FL[7] - FR[7];
     const double bn = nbn * iS, bt = nbt * iS, bm = nbm * iS;
     const double bbn = bn * bn;
     const double DLM = SL - SM, DRM = SR - SM;
-    const double iLM = 1.0 / DLM, iRM = 1.0 / DRM;
+    const double iLM = fast_rcp(DLM), iRM = fast_rcp(DRM);
     const double ro_LL = ro_L * suL * iLM;
     const double ro_RR = ro_R * suR * iRM;
 
     // adjoints of every forward quantity
     double b_ro_L = 0.0, b_p_L = 0.0, b_u1 = 0.0, b_v1 = 0.0, b_w1 = 0.0, b_bn1 = 0.0, b_bt1 = 0.0, b_bm1 = 0.0;
     double b_ro_R = 0.0, b_p_R = 0.0, b_u2 = 0.0, b_v2 = 0.0, b_w2 = 0.0, b_bn2 = 0.0, b_bt2 = 0.0, b_bm2 = 0.0;
-    double bFL[8], bFR[8], bUL[8], bUR[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) bFL[i] = bFR[i] = bUL[i] = bUR[i] = 0.0;
+    // The cotangents of the two sides' fluxes F and states U are formed at
+    // the end, from: the side the region formula takes (sel_left), the factor
+    // of its state cotangent (sel_u: -S for HLLC-MHD; -S1, or 0 in the outer
+    // region, for HLLD), the cotangents of the HLL field averages' F terms
+    // (g_bt, g_bm) and of HLLC-MHD's HLL state UZ (gUZ): no 32-entry arrays
+    // live through the sweep.
+    bool sel_left = false;
+    double sel_u = 0.0, g_bt = 0.0, g_bm = 0.0;
+    double gUZ[4] = {0.0, 0.0, 0.0, 0.0};
     double b_SL = 0.0, b_SR = 0.0, b_SM = 0.0, b_bn = 0.0, b_bt = 0.0, b_bm = 0.0, b_ro_LL = 0.0, b_ro_RR = 0.0;
     double b_iS = 0.0, b_iLM = 0.0, b_iRM = 0.0, b_e1 = 0.0, b_e2 = 0.0, b_pTL = 0.0, b_pTR = 0.0;
     double b_suL = 0.0, b_suR = 0.0, b_mL = 0.0, b_mR = 0.0, b_idm = 0.0;
@@ -139,7 +146,7 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         const double suRm = suR * iRM, suLm = suL * iLM;
         const double ptR = pTR + ro_R * suR * (SM - u2), ptL = pTL + ro_L * suL * (SM - u1);
         const double ptz = 0.5 * (ptR + ptL);
-        const double iUZ0 = 1.0 / UZ0;
+        const double iUZ0 = fast_rcp(UZ0);
         const double vv = UZ2 * iUZ0, ww = UZ3 * iUZ0;
         const double nz = bn * UZ1 + bt * UZ2 + bm * UZ3;
         const double sbvz = nz * iUZ0;
@@ -157,10 +164,11 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         for (int i = 0; i < 8; ++i) {
             const double g = i == 5 ? 0.0 : Pb[i];
             const double Ui = left ? UL[i] : UR[i];
-            if (left) { bFL[i] += g; bUL[i] -= g * S; } else { bFR[i] += g; bUR[i] -= g * S; }
             b_S += g * (Us[i] - Ui);
             bUs[i] = g * S;
         }
+        sel_left = left;
+        sel_u = -S;
         if (sS == 1) b_SL += b_S; else if (sS == 2) b_SR += b_S;
         double b_rs = bUs[0] + bUs[1] * SM + bUs[2] * vv + bUs[3] * ww;
         b_SM += bUs[1] * rs;
@@ -239,9 +247,9 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         for (int k = 0; k < 4; ++k) {
             const double g = bUZ[k] * iS;
             b_iS += bUZ[k] * nUZ[k];
-            b_SR += g * UR[k]; bUR[k] += g * SR;
-            b_SL -= g * UL[k]; bUL[k] -= g * SL;
-            bFL[k] += g; bFR[k] -= g;
+            b_SR += g * UR[k];
+            b_SL -= g * UL[k];
+            gUZ[k] = g;
         }
     } else {
         // ---------------------------------------------------------- HLLD forward
@@ -251,7 +259,7 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         const double kL = ro_L * suL, kR = ro_R * suR;
         const double ttL = kL * DLM - bbn, ttR = kR * DRM - bbn;
         const bool gL = fabs(ttL) >= 0.00001, gR = fabs(ttR) >= 0.00001;
-        const double itL = gL ? 1.0 / ttL : 0.0, itR = gR ? 1.0 / ttR : 0.0;
+        const double itL = gL ? fast_rcp(ttL) : 0.0, itR = gR ? fast_rcp(ttR) : 0.0;
         const double dSL = SM - u1, dSR = SM - u2;
         // vLL = v1 - ((bn*bt1)*dSL)*itL
         const double hvL = bn * bt1, hwL = bn * bm1, hvR = bn * bt2, hwR = bn * bm2;
@@ -266,15 +274,15 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         const double nEL = e1 * suL + PTT * SM - pTL * u1 + bn * (sbv1 - vbL);
         const double nER = e2 * suR + PTT * SM - pTR * u2 + bn * (sbv2 - vbR);
         const double eLL = nEL * iLM, eRR = nER * iRM;
-        const double sLL = sqrt(ro_LL), sRR = sqrt(ro_RR);
-        const double isLL = 1.0 / sLL, isRR = 1.0 / sRR;
+        const double sLL = fast_sqrt(ro_LL), sRR = fast_sqrt(ro_RR);
+        const double isLL = fast_rcp(sLL), isRR = fast_rcp(sRR);
         const double abn = bn < 0.0 ? -bn : bn;
         const bool big = fabs(bn) > 0.000001;
         const double idbn = big ? (bn > 0.0 ? 1.0 : -1.0) : 0.0;
         const double SLL = big ? SM - abn * isLL : SM;
         const double SRR = big ? SM + abn * isRR : SM;
         const double ssum = sLL + sRR;
-        const double isum = 1.0 / ssum;
+        const double isum = fast_rcp(ssum);
         const double sLR = sLL * sRR;
         const double n1 = sLL * vLL + sRR * vRR + idbn * (btt2 - btt1);
         const double n2 = sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1);
@@ -303,19 +311,13 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
             // region weights: outer F; single F + S1 (Us - U); double
             // F + S2 Uss - (S2 - S1) Us - S1 U
             const double w1 = outer ? 0.0 : 1.0, w2 = (outer || single) ? 0.0 : 1.0;
-            const double gU = -g * S1 * w1;
-            if (left) {
-                bFL[i] += g;
-                bUL[i] += gU;
-            } else {
-                bFR[i] += g;
-                bUR[i] += gU;
-            }
             b_S1 += g * w1 * (Us[i] - Ui);
             b_S2 += g * w2 * (Uss[i] - Us[i]);
             bUss[i] = g * S2 * w2;
             bUs[i] = w2 > 0.0 ? -g * (S2 - S1) : g * S1 * w1;
         }
+        sel_left = left;
+        sel_u = outer ? 0.0 : -S1;
         // Us = {rs, rs*SM, rs*vs, rs*ws, es, bn, bts, bms}
         double b_rs = bUs[0] + bUs[1] * SM + bUs[2] * vs + bUs[3] * ws;
         b_SM += bUs[1] * rs;
@@ -390,8 +392,8 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
             b_bn += bn < 0.0 ? -b_abn : b_abn;
         }
         // sLL = sqrt(ro_LL)
-        b_ro_LL += sLL > 0.0 ? 0.5 * b_sLL / sLL : 0.0;
-        b_ro_RR += sRR > 0.0 ? 0.5 * b_sRR / sRR : 0.0;
+        b_ro_LL += sLL > 0.0 ? 0.5 * b_sLL * isLL : 0.0;
+        b_ro_RR += sRR > 0.0 ? 0.5 * b_sRR * isRR : 0.0;
         // eLL = nEL*iLM, nEL = e1*suL + PTT*SM - pTL*u1 + bn*(sbv1 - vbL)
         double b_PTT = 0.0;
         {
@@ -521,12 +523,35 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         b_bn2 += gn * SR; b_bn1 -= gn * SL;
         b_bt2 += gt * SR; b_bt1 -= gt * SL;
         b_bm2 += gm * SR; b_bm1 -= gm * SL;
-        bFL[6] += gt; bFR[6] -= gt;
-        bFL[7] += gm; bFR[7] -= gm;
+        g_bt = gt;
+        g_bm = gm;
         const double g = -b_iS * iS * iS;
         b_SR += g; b_SL -= g;
     }
     // U and F of each side
+    double bFL[8], bFR[8], bUL[8], bUR[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double g = i == 5 ? 0.0 : Pb[i];
+        const double gu = g * sel_u;
+        bFL[i] = sel_left ? g : 0.0;
+        bFR[i] = sel_left ? 0.0 : g;
+        bUL[i] = sel_left ? gu : 0.0;
+        bUR[i] = sel_left ? 0.0 : gu;
+    }
+    if constexpr (METOD == 2) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bUR[k] += gUZ[k] * SR;
+            bUL[k] -= gUZ[k] * SL;
+            bFL[k] += gUZ[k];
+            bFR[k] -= gUZ[k];
+        }
+    }
+    bFL[6] += g_bt;
+    bFR[6] -= g_bt;
+    bFL[7] += g_bm;
+    bFR[7] -= g_bm;
     b_ro_L += bUL[0] + bUL[1] * u1 + bUL[2] * v1 + bUL[3] * w1;
     b_u1 += bUL[1] * ro_L; b_v1 += bUL[2] * ro_L; b_w1 += bUL[3] * ro_L;
     b_e1 += bUL[4]; b_bn1 += bUL[5]; b_bt1 += bUL[6]; b_bm1 += bUL[7];
@@ -598,11 +623,11 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
     double b_cfL = cL ? b_cmax : 0.0, b_cfR = cL ? 0.0 : b_cmax;
     // cf = sqrt(a), a = (gp + sqrt(d)) * (0.5*iro), d = gp^2 - (4 gamma p) bn^2
     {
-        const double b_a = cfL > 0.0 ? 0.5 * b_cfL / cfL : 0.0;
+        const double b_a = cfL > 0.0 ? 0.5 * b_cfL * fast_rcp(cfL) : 0.0;
         const double h = 0.5 * iroL;
         const double b_gs = b_a * h;
         const double b_iro = b_a * (gpL + sdL) * 0.5;
-        const double b_d = sdL > 0.0 ? 0.5 * b_gs / sdL : 0.0;
+        const double b_d = sdL > 0.0 ? 0.5 * b_gs * fast_rcp(sdL) : 0.0;
         const double b_gp = b_gs + b_d * 2.0 * gpL;
         // d = gp*gp - (g4*p)*(bn^2)
         b_p_L -= b_d * g4 * (bn1 * bn1);
@@ -612,11 +637,11 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
         b_ro_L -= b_iro * iroL * iroL;
     }
     {
-        const double b_a = cfR > 0.0 ? 0.5 * b_cfR / cfR : 0.0;
+        const double b_a = cfR > 0.0 ? 0.5 * b_cfR * fast_rcp(cfR) : 0.0;
         const double h = 0.5 * iroR;
         const double b_gs = b_a * h;
         const double b_iro = b_a * (gpR + sdR) * 0.5;
-        const double b_d = sdR > 0.0 ? 0.5 * b_gs / sdR : 0.0;
+        const double b_d = sdR > 0.0 ? 0.5 * b_gs * fast_rcp(sdR) : 0.0;
         const double b_gp = b_gs + b_d * 2.0 * gpR;
         b_p_R -= b_d * g4 * (bn2 * bn2);
         b_bn2 -= b_d * (g4 * p_R) * 2.0 * bn2;
