@@ -53,65 +53,52 @@ __device__ __forceinline__ FluxM hlld_face(double ro_L, double p_L, double u1, d
     const double smR = mR * u2;
     const double smL = mL * u1;
     const double SM = (smR - smL - pTR + pTL) * idm;
-    const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
-    const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
 
-    double FL[8], FR[8], UL[8], UR[8];
-    FL[0] = ro_L * u1;
-    FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1;
-    FL[2] = ro_L * u1 * v1 - bn1 * bt1;
-    FL[3] = ro_L * u1 * w1 - bn1 * bm1;
-    FL[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
-    FL[5] = 0.0;
-    FL[6] = u1 * bt1 - v1 * bn1;
-    FL[7] = u1 * bm1 - w1 * bn1;
-    FR[0] = ro_R * u2;
-    FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2;
-    FR[2] = ro_R * u2 * v2 - bn2 * bt2;
-    FR[3] = ro_R * u2 * w2 - bn2 * bm2;
-    FR[4] = (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2);
-    FR[5] = 0.0;
-    FR[6] = u2 * bt2 - v2 * bn2;
-    FR[7] = u2 * bm2 - w2 * bn2;
-    UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1;
-    UL[4] = e1; UL[5] = bn1; UL[6] = bt1; UL[7] = bm1;
-    UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2;
-    UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2;
-
-    // HLL averages of the field (Gas_Dinamic.cu:1127-1130)
+    // HLL averages of the field (Gas_Dinamic.cu:1127-1130); the tangential
+    // field fluxes u bt - v bn, u bm - w bn of both sides
     const double iS = fast_rcp(SR - SL);
     const double bn = (SR * bn2 - SL * bn1) * iS;
-    const double bt = (SR * bt2 - SL * bt1 + FL[6] - FR[6]) * iS;
-    const double bm = (SR * bm2 - SL * bm1 + FL[7] - FR[7]) * iS;
+    const double bt = (SR * bt2 - SL * bt1 + (u1 * bt1 - v1 * bn1) - (u2 * bt2 - v2 * bn2)) * iS;
+    const double bm = (SR * bm2 - SL * bm1 + (u1 * bm1 - w1 * bn1) - (u2 * bm2 - w2 * bn2)) * iS;
     const double bbn = bn * bn;
     const double iLM = fast_rcp(SL - SM), iRM = fast_rcp(SR - SM);
     const double ro_LL = ro_L * suL * iLM;
     const double ro_RR = ro_R * suR * iRM;
 
+    // The flux is F + (star-state terms) of one side K (left when the region
+    // formula takes it); F, U and the star energies are formed for side K
+    // only, from its selected inputs (the same expressions on the same values
+    // as forming both sides and selecting).
     FluxM P;
     if constexpr (METOD == 2) {
         // HLLC-MHD with HLL-averaged tangential velocity (Gas_Dinamic.cu:1137-1301)
-        const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
-        const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
-        const double UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) * iS;
-        const double UZ1 = (SR * UR[1] - SL * UL[1] + FL[1] - FR[1]) * iS;
-        const double UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) * iS;
-        const double UZ3 = (SR * UR[3] - SL * UL[3] + FL[3] - FR[3]) * iS;
-        const double suRm = suR * iRM, suLm = suL * iLM;
+        const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
+        const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
+        const double UZ0 = (SR * ro_R - SL * ro_L + ro_L * u1 - ro_R * u2) * iS;
+        const double UZ1 = (SR * (ro_R * u2) - SL * (ro_L * u1) + (ro_L * u1 * u1 + pTL - bn1 * bn1) -
+                            (ro_R * u2 * u2 + pTR - bn2 * bn2)) * iS;
+        const double UZ2 = (SR * (ro_R * v2) - SL * (ro_L * v1) + (ro_L * u1 * v1 - bn1 * bt1) -
+                            (ro_R * u2 * v2 - bn2 * bt2)) * iS;
+        const double UZ3 = (SR * (ro_R * w2) - SL * (ro_L * w1) + (ro_L * u1 * w1 - bn1 * bm1) -
+                            (ro_R * u2 * w2 - bn2 * bm2)) * iS;
         const double ptz = 0.5 * ((pTR + ro_R * suR * (SM - u2)) + (pTL + ro_L * suL * (SM - u1)));
         const double iUZ0 = fast_rcp(UZ0);
         const double vv = UZ2 * iUZ0, ww = UZ3 * iUZ0;
         const double sbvz = (bn * UZ1 + bt * UZ2 + bm * UZ3) * iUZ0;
-        const double ee2 = e2 * suRm + (ptz * SM - pTR * u2 + bn * (sbv2 - sbvz)) * iRM;
-        const double ee1 = e1 * suLm + (ptz * SM - pTL * u1 + bn * (sbv1 - sbvz)) * iLM;
         // branch order of Gas_Dinamic.cu:1250-1277: SL >= 0, SM >= 0, SR > 0, else
         const bool left = (SL >= 0.0) || (SM >= 0.0);
         const double S = (SL >= 0.0) ? 0.0 : (left ? SL : (SR > 0.0 ? SR : 0.0));
+        const double r = left ? ro_L : ro_R, u = left ? u1 : u2, v = left ? v1 : v2, w = left ? w1 : w2;
+        const double bnK = left ? bn1 : bn2, btK = left ? bt1 : bt2, bmK = left ? bm1 : bm2;
+        const double pT = left ? pTL : pTR, e = left ? e1 : e2;
+        const double su = left ? suL : suR, iKM = left ? iLM : iRM;
+        const double sbv = u * bnK + v * btK + w * bmK;
         const double rs = left ? ro_LL : ro_RR;
-        const double es = left ? ee1 : ee2;
-        const double* F = left ? FL : FR;
-        const double* U = left ? UL : UR;
-        double Us[8] = {rs, rs * SM, rs * vv, rs * ww, es, bn, bt, bm};
+        const double es = e * (su * iKM) + (ptz * SM - pT * u + bn * (sbv - sbvz)) * iKM;
+        const double F[8] = {r * u, r * u * u + pT - bnK * bnK, r * u * v - bnK * btK, r * u * w - bnK * bmK,
+                             (e + pT) * u - bnK * sbv, 0.0, u * btK - v * bnK, u * bmK - w * bnK};
+        const double U[8] = {r, r * u, r * v, r * w, e, bnK, btK, bmK};
+        const double Us[8] = {rs, rs * SM, rs * vv, rs * ww, es, bn, bt, bm};
 #pragma unroll
         for (int i = 0; i < 8; ++i) P.f[i] = F[i] + S * Us[i] - S * U[i];
     } else {
@@ -140,10 +127,6 @@ __device__ __forceinline__ FluxM hlld_face(double ro_L, double p_L, double u1, d
         } else {
             vRR = v2; wRR = w2; btt2 = 0.0; bmm2 = 0.0;
         }
-        const double eLL = (e1 * suL + PTT * SM - pTL * u1 +
-                            bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) * iLM;
-        const double eRR = (e2 * suR + PTT * SM - pTR * u2 +
-                            bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) * iRM;
         const double sLL = fast_sqrt(ro_LL), sRR = fast_sqrt(ro_RR);
         double SLL = SM - fabs(bn) * fast_rcp(sLL);
         double SRR = SM + fabs(bn) * fast_rcp(sRR);
@@ -161,25 +144,33 @@ __device__ __forceinline__ FluxM hlld_face(double ro_L, double p_L, double u1, d
         const double bttt = (sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL)) * isum;
         const double bmmm = (sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL)) * isum;
         const double vb3 = SM * bn + vLLL * bttt + wLLL * bmmm;
-        const double eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - vb3);
-        const double eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - vb3);
         // six regions (Gas_Dinamic.cu:1434-1484): F + S*(U* - U) for the
         // single-star regions, F + S**U** - (S** - S)U* - S U for the double-star
         const bool left = SM >= 0.0;
-        const double* F = left ? FL : FR;
-        const double* U = left ? UL : UR;
+        const double r = left ? ro_L : ro_R, u = left ? u1 : u2, v = left ? v1 : v2, w = left ? w1 : w2;
+        const double pK = left ? p_L : p_R, uuK = left ? uu_L : uu_R, bbK = left ? bb_L : bb_R;
+        const double bnK = left ? bn1 : bn2, btK = left ? bt1 : bt2, bmK = left ? bm1 : bm2;
+        const double pT = left ? pTL : pTR, su = left ? suL : suR, iKM = left ? iLM : iRM;
         const double S1 = left ? SL : SR;
         const double S2 = left ? SLL : SRR;
         const double rs = left ? ro_LL : ro_RR;
         const double vs = left ? vLL : vRR, ws = left ? wLL : wRR;
         const double bts = left ? btt1 : btt2, bms = left ? bmm1 : bmm2;
-        const double es = left ? eLL : eRR;
-        const double ess = left ? eLLL : eRRR;
+        const double sK = left ? sLL : sRR;
+        const double sgn = left ? -idbn : idbn;
+        const double e = pK * inv_g1 + r * uuK + 0.5 * bbK;
+        const double sbv = u * bnK + v * btK + w * bmK;
+        const double vbK = SM * bn + vs * bts + ws * bms;
+        const double es = (e * su + PTT * SM - pT * u + bn * (sbv - vbK)) * iKM;
+        const double ess = es + sgn * sK * (vbK - vb3);
         // region selectors in the reference's branch order: left side
         // SL >= 0 / SLL >= 0 / double star; right side SRR > 0 (double star) /
         // SR > 0 (single star) / outer
         const bool outer = left ? (SL >= 0.0) : !(SRR > 0.0 || SR > 0.0);
         const bool single = left ? (SLL >= 0.0) : !(SRR > 0.0);
+        const double F[8] = {r * u, r * u * u + pT - bnK * bnK, r * u * v - bnK * btK, r * u * w - bnK * bmK,
+                             (e + pT) * u - bnK * sbv, 0.0, u * btK - v * bnK, u * bmK - w * bnK};
+        const double U[8] = {r, r * u, r * v, r * w, e, bnK, btK, bmK};
         const double Us[8] = {rs, rs * SM, rs * vs, rs * ws, es, bn, bts, bms};
         const double Uss[8] = {rs, rs * SM, rs * vLLL, rs * wLLL, ess, bn, bttt, bmmm};
         if (outer) {
