@@ -442,11 +442,100 @@ __device__ __forceinline__ bool frozen_m(const MhdArgs& A, int c, int j) {
     }
 }
 
+// Raw values of one cell as stored: the 7 conserved values (kSoA) or the
+// primitives (rho, p, vx, vy, bx, by, bz) (kAoS).
+struct RawM {
+    double a[7];
+};
+
+// Primitives from the 7 conserved values (the expressions of cons_to_prim_m).
+__device__ __forceinline__ PrimM prim_from_raw_m(const RawM& w, double g1) {
+    PrimM q;
+    const double r = w.a[0], mx = w.a[1], my = w.a[2];
+    q.bx = w.a[3];
+    q.by = w.a[4];
+    q.bz = w.a[5];
+    const double E = w.a[6];
+    const double inv = fast_rcp(r);
+    q.r = r;
+    q.u = mx * inv;
+    q.v = my * inv;
+    q.p = g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
+    return q;
+}
+
+// Loads of cell ci of stored row jj. kSoA: buffer loads from the seven
+// conserved planes at byte offset `off` (kOffNone loads zeros and touches no
+// memory), issued by the whole wave without a branch; kAoS: plain loads of the
+// three arrays where `valid`.
+template <int LAYOUT>
+__device__ __forceinline__ RawM load_raw_m(const MhdArgs& A, int jj, int ci, unsigned off, bool valid) {
+    RawM w;
+    if constexpr (LAYOUT == kSoA) {
+        const double* b = A.Uin + (long long)jj * A.nx;
+#pragma unroll
+        for (int f = 0; f < kMF; ++f) w.a[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * A.plane)), off);
+    } else {
+#pragma unroll
+        for (int f = 0; f < kMF; ++f) w.a[f] = 0.0;
+        if (valid) {
+            const long long k = (long long)jj * A.nx + ci;
+            const double2 sv = A.s[k], uv = A.u[k];
+            const double3 bv = A.b[k];
+            w.a[0] = sv.x;
+            w.a[1] = sv.y;
+            w.a[2] = uv.x;
+            w.a[3] = uv.y;
+            w.a[4] = bv.x;
+            w.a[5] = bv.y;
+            w.a[6] = bv.z;
+        }
+    }
+    return w;
+}
+
+// Primitive state of a loaded cell with its boundary rule: the flips of a
+// reflective side (normal velocity and normal field negated), or with PROD
+// the reference's production side kinds (ghost_m).
+template <int LAYOUT, bool PROD>
+__device__ __forceinline__ PrimM finish_m(const MhdArgs& A, const RawM& w, bool fx, bool fy, int sx, int sy) {
+    PrimM q;
+    if constexpr (LAYOUT == kSoA) {
+        q = prim_from_raw_m(w, A.g1);
+    } else {
+        q.r = w.a[0]; q.p = w.a[1]; q.u = w.a[2]; q.v = w.a[3]; q.bx = w.a[4]; q.by = w.a[5]; q.bz = w.a[6];
+    }
+    if constexpr (PROD) {
+        q = ghost_m(q, sx, sy, A);
+    } else {
+        if (fx) { q.u = -q.u; q.bx = -q.bx; }
+        if (fy) { q.v = -q.v; q.by = -q.by; }
+    }
+    return q;
+}
+
+// Per-wave LDS of the stage kernel: primitive rows (own cells and halo) for
+// the x-faces, row r in slot r & 1, and (kSoA) the own conserved values
+// waiting for their update, row r in slot r % 3 (read after the barrier that
+// hands the primitive slot to row r + 2): 18.4 KB.
+struct MhdLds {
+    double q[2][kMF][kRowBuf];
+    double u[3][kMF][kWave];
+};
+
 // PROD (reference layout only): production side kinds, inner frozen sphere
 // and axisymmetric sources.
+//
+// March of one wave (as in the Euler stage kernel): at pass j the loads of
+// row j+2 are issued first and arrive while the x-face of row j is computed
+// from LDS; after the barrier row j+2 is converted into the free LDS slot,
+// the y-face G(j+1/2) is formed from rows j-1 .. j+2 of the lane's column,
+// the RK base of row j (loaded after the barrier) arrives during it, and row
+// j is updated. Every face is evaluated once; every expression is unchanged
+// by the order of the march.
 template <int ORDER, int METOD, int LAYOUT, bool WANT_W, bool PROD = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == kAoS && ORDER == 2 && METOD == 3) ? 2 : 1))) void mhd_tile_kernel(const MhdArgs A) {
-    __shared__ double ring[3][kMF][kRowBuf];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mhd_tile_kernel(const MhdArgs A) {
+    __shared__ MhdLds lds;
 
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -456,14 +545,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
     const int j1 = min(j0 + A.rows, A.ny);
     const int c = cbase + lane;
 
-    bool fx, hfx;
+    bool fx = false, hfx = false;
     int sxc = -1, sxh = -1;
     const int hcol = lane == 0 ? cbase - 2 : (lane == 1 ? cbase - 1 : cbase + 64);
     int ci, hci;
     if constexpr (PROD) {
         ci = map_index_side(c, A.nx, A.bc[0], A.bc[1], sxc);
         hci = map_index_side(hcol, A.nx, A.bc[0], A.bc[1], sxh);
-        fx = hfx = false;
     } else {
         ci = map_index(c, A.nx, A.bc[0], A.bc[1], fx);
         hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
@@ -471,6 +559,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
     const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
     const bool has_halo = lane < 3;
     const bool owns = (lane < kTileW) && (c < A.nx);
+    const unsigned oc = (unsigned)ci * 8u;                      // own cell within a row (kSoA)
+    const unsigned oh = has_halo ? (unsigned)hci * 8u : kOffNone;  // halo cell within a row (kSoA)
+    const unsigned oo = (unsigned)c * 8u;                       // output cell within a row (kSoA)
+    const unsigned ob_off = owns ? oo : kOffNone;               // RK base cell within a row (kSoA)
 
     const double dt = A.dt_dev ? *A.dt_dev : A.dt;
     const double dx = A.dx, dy = A.dy;
@@ -479,55 +571,116 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
     // add2 / Kernel_TVD (kernel.cu:2070, 2086)
     const double krx = A.krit * dx, kry = A.krit * dy;
 
-    auto load_row = [&](int jl, int slot, bool to_lds) -> PrimM {
+    // logical row jl: raw loads of the own cell and (lanes 0-2) the halo cell
+    struct Pending {
+        RawM own, halo;
         bool fy;
-        int sy = -1;
+        int sy;
+    };
+    auto issue_row = [&](int jl) -> Pending {
+        Pending pr;
+        pr.fy = false;
+        pr.sy = -1;
         int jj;
-        if constexpr (PROD) {
-            jj = map_index_side(jl, A.ny, A.bc[2], A.bc[3], sy);
-            fy = false;
-        } else {
-            jj = map_index(jl, A.ny, A.bc[2], A.bc[3], fy);
-        }
-        const long long rowk = (long long)jj * A.nx;
-        PrimM q = load_cell_m<LAYOUT>(A, rowk + ci, fx, fy);
-        if constexpr (PROD) q = ghost_m(q, sxc, sy, A);
+        if constexpr (PROD)
+            jj = map_index_side(jl, A.ny, A.bc[2], A.bc[3], pr.sy);
+        else
+            jj = map_index(jl, A.ny, A.bc[2], A.bc[3], pr.fy);
+        pr.own = load_raw_m<LAYOUT>(A, jj, ci, oc, true);
+        pr.halo = load_raw_m<LAYOUT>(A, jj, hci, oh, has_halo);
+        return pr;
+    };
+    // convert a pending row; to_lds: publish it (and the halo) in LDS slot
+    // `slot`, with the own conserved values in slot `uslot` (kSoA)
+    auto finish_row = [&](const Pending& pr, int slot, int uslot, bool to_lds) -> PrimM {
+        const PrimM q = finish_m<LAYOUT, PROD>(A, pr.own, fx, pr.fy, sxc, pr.sy);
         if (to_lds) {
-            lds_put_m(ring, slot, lane + 2, q);
-            if (has_halo) {
-                PrimM h = load_cell_m<LAYOUT>(A, rowk + hci, hfx, fy);
-                if constexpr (PROD) h = ghost_m(h, sxh, sy, A);
-                lds_put_m(ring, slot, hslot, h);
+            lds_put_m(lds.q, slot, lane + 2, q);
+            if constexpr (LAYOUT == kSoA) {
+#pragma unroll
+                for (int f = 0; f < kMF; ++f) lds.u[uslot][f][lane] = pr.own.a[f];
             }
+            if (has_halo) lds_put_m(lds.q, slot, hslot, finish_m<LAYOUT, PROD>(A, pr.halo, hfx, pr.fy, sxh, pr.sy));
         }
         return q;
     };
 
     // prologue: rows j0-2, j0-1 (y-stencil only) and j0 (also in LDS); the
     // first y-face comes from the loop's lead-in pass j = j0-1
-    PrimM qm2 = load_row(j0 - 2, 0, false);
-    PrimM q0 = load_row(j0 - 1, 0, false);
-    PrimM q1 = load_row(j0, j0 % 3, true);
-    PrimM s0 = slope3m<ORDER>(qm2, q0, q1);
+    PrimM q0, q1, s0;
+    double by_s;  // cell-centred By of the south neighbour (Powell average)
+    {
+        const Pending pm2 = issue_row(j0 - 2), pm1 = issue_row(j0 - 1), pj0 = issue_row(j0);
+        const PrimM qm2 = finish_row(pm2, 0, 0, false);
+        q0 = finish_row(pm1, 0, 0, false);
+        q1 = finish_row(pj0, j0 & 1, j0 % 3, true);
+        s0 = slope3m<ORDER>(qm2, q0, q1);
+        by_s = qm2.by;
+    }
     double smax;
     LabFluxM Gm;
     Gm.r = Gm.mx = Gm.my = Gm.bx = Gm.by = Gm.bz = Gm.e = 0.0;
     double tGm = 0.0;
-    double by_s = qm2.by;  // cell-centred By of the south neighbour (Powell average)
 
     double wmax = 0.0, tmin = 1e300;
 
     for (int j = j0 - 1; j < j1; ++j) {
+        // loads first: row j+2 arrives while the x-face of row j is computed
+        const Pending pn = issue_row(j + 2);
+
+        // x-face F(c-1/2) from cells c-2, c-1, c, c+1 of row j (LDS; the own
+        // cell is q0); F(c+1/2) from the next lane. Only the differences
+        // F(c+1/2) - F(c-1/2) are carried to the update.
+        LabFluxM dF;
+        double tFl = 0.0, tFr = 0.0, dbx = 0.0;
+        if (j >= j0) {
+            const int sl = j & 1;
+            const PrimM qa = lds_get_m(lds.q, sl, lane);
+            const PrimM qb = lds_get_m(lds.q, sl, lane + 1);
+            const PrimM qd = lds_get_m(lds.q, sl, lane + 3);
+            const PrimM sb = slope3m<ORDER>(qa, qb, q0);
+            const PrimM sc = slope3m<ORDER>(qb, q0, qd);
+            // reflective west wall face from the interior side (see the south wall)
+            const bool wmir = (c == 0) && (A.bc[0] == GD2_BC_REFLECTIVE || (PROD && A.bc[0] == GD2_BC_SYMMETRY));
+            const PrimM xl = face_m(qb, sb, 0.5), xr = face_m(q0, sc, -0.5);
+            LabFluxM Fl = mhd_flux<METOD, true>(wmir ? mirror_x(xr) : xl, wmir ? mirror_x(xl) : xr, A.gamma,
+                                                A.inv_g1, smax);
+            if (wmir) Fl = unmirror_x(Fl);
+            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+            const LabFluxM Fr = shfl_down_m(Fl);
+            tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
+            dF.r = Fr.r - Fl.r;
+            dF.mx = Fr.mx - Fl.mx;
+            dF.my = Fr.my - Fl.my;
+            dF.bx = Fr.bx - Fl.bx;
+            dF.by = Fr.by - Fl.by;
+            dF.bz = Fr.bz - Fl.bz;
+            dF.e = Fr.e - Fl.e;
+            dbx = qd.bx - qb.bx;  // east minus west cell-centred Bx
+        }
+
+        // row j's primitive slot is read: row j+2 may take it
         __syncthreads();
-        PrimM q2 = load_row(j + 2, (j + 2) % 3, true);
-        PrimM s1 = slope3m<ORDER>(q0, q1, q2);
+        const PrimM q2 = finish_row(pn, (j + 2) & 1, (j + 2) % 3, true);
+        // the RK base of row j (kSoA), in flight during the y-face; the
+        // lead-in pass and lanes without a cell load nothing
+        double ub[kMF];
+        if constexpr (LAYOUT == kSoA) {
+            if (!A.plain) {
+                const unsigned off = j >= j0 ? ob_off : kOffNone;
+                const double* b = A.Ubase + (long long)max(j, j0) * A.nx;
+#pragma unroll
+                for (int f = 0; f < kMF; ++f) ub[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * A.plane)), off);
+            }
+        }
+        const PrimM s1 = slope3m<ORDER>(q0, q1, q2);
         // A reflective south wall face is evaluated from the interior cell's
         // side (outward normal), as the reference's cell does; its HLLC-MHD /
         // HLLD flux is discontinuous at SM = 0 when the normal field jumps.
         const bool smir = (j == -1) && (A.bc[2] == GD2_BC_REFLECTIVE || (PROD && A.bc[2] == GD2_BC_AXIS));
         const PrimM yl = face_m(q0, s0, 0.5), yr = face_m(q1, s1, -0.5);
-        LabFluxM Gp = smir ? unmirror_y(mhd_flux<METOD, false>(mirror_y(yr), mirror_y(yl), A.gamma, A.inv_g1, smax))
-                           : mhd_flux<METOD, false>(yl, yr, A.gamma, A.inv_g1, smax);
+        const LabFluxM Gp = smir ? unmirror_y(mhd_flux<METOD, false>(mirror_y(yr), mirror_y(yl), A.gamma, A.inv_g1, smax))
+                                 : mhd_flux<METOD, false>(yl, yr, A.gamma, A.inv_g1, smax);
         const double tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             by_s = q0.by;
@@ -539,24 +692,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
             continue;
         }
 
-        // x-face F(c-1/2) from cells c-2, c-1, c, c+1 of row j (in LDS since two barriers)
-        const int sl = j % 3;
-        const PrimM qa = lds_get_m(ring, sl, lane);
-        const PrimM qb = lds_get_m(ring, sl, lane + 1);
-        const PrimM qd = lds_get_m(ring, sl, lane + 3);
-        const PrimM sb = slope3m<ORDER>(qa, qb, q0);
-        const PrimM sc = slope3m<ORDER>(qb, q0, qd);
-        // reflective west wall face from the interior side (see the south wall)
-        const bool wmir = (c == 0) && (A.bc[0] == GD2_BC_REFLECTIVE || (PROD && A.bc[0] == GD2_BC_SYMMETRY));
-        const PrimM xl = face_m(qb, sb, 0.5), xr = face_m(q0, sc, -0.5);
-        LabFluxM Fl = mhd_flux<METOD, true>(wmir ? mirror_x(xr) : xl, wmir ? mirror_x(xl) : xr, A.gamma, A.inv_g1,
-                                            smax);
-        if (wmir) Fl = unmirror_x(Fl);
-        const double tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
-        const LabFluxM Fr = shfl_down_m(Fl);
-        const double tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
-        const double bx_w = qb.bx, bx_e = qd.bx;
-
         bool sphere = false;
         double yc = 0.0;
         if constexpr (PROD) {
@@ -566,37 +701,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
         }
         if (owns && !frozen_m<LAYOUT>(A, c, j) && !sphere) {
             // face sums differenced per direction (kernel.cu:2559-2604)
-            const double PSr = (Fr.r - Fl.r) * dy + (Gp.r - Gm.r) * dx;
-            const double PSe = (Fr.e - Fl.e) * dy + (Gp.e - Gm.e) * dx;
-            const double PUx = (Fr.mx - Fl.mx) * dy + (Gp.mx - Gm.mx) * dx;
-            const double PUy = (Fr.my - Fl.my) * dy + (Gp.my - Gm.my) * dx;
-            const double PBx = (Fr.bx - Fl.bx) * dy + (Gp.bx - Gm.bx) * dx;
-            const double PBy = (Fr.by - Fl.by) * dy + (Gp.by - Gm.by) * dx;
-            const double PBz = (Fr.bz - Fl.bz) * dy + (Gp.bz - Gm.bz) * dx;
+            const double PSr = dF.r * dy + (Gp.r - Gm.r) * dx;
+            const double PSe = dF.e * dy + (Gp.e - Gm.e) * dx;
+            const double PUx = dF.mx * dy + (Gp.mx - Gm.mx) * dx;
+            const double PUy = dF.my * dy + (Gp.my - Gm.my) * dx;
+            const double PBx = dF.bx * dy + (Gp.bx - Gm.bx) * dx;
+            const double PBy = dF.by * dy + (Gp.by - Gm.by) * dx;
+            const double PBz = dF.bz * dy + (Gp.bz - Gm.bz) * dx;
             // Powell term: face-averaged normal field of the cell-centred
             // neighbours (kernel.cu:2568, 2580, 2592, 2604)
-            const double Pdiv = dy * (bx_e - bx_w) * 0.5 + dx * (q1.by - by_s) * 0.5;
+            const double Pdiv = dy * dbx * 0.5 + dx * (q1.by - by_s) * 0.5;
             const PrimM& q = q0;
-            const long long k = (long long)j * A.nx + c;
             if constexpr (LAYOUT == kSoA) {
-                const long long P = A.plane;
+                double uc[kMF];
+#pragma unroll
+                for (int f = 0; f < kMF; ++f) uc[f] = lds.u[j % 3][f][lane];
                 const double dtV = dt / dV;
                 double o[7];
-                o[0] = A.Uin[k] - dtV * PSr;
-                o[1] = A.Uin[P + k] - dtV * (PUx + (q.bx * (1.0 / kCpi4)) * Pdiv);
-                o[2] = A.Uin[2 * P + k] - dtV * (PUy + (q.by * (1.0 / kCpi4)) * Pdiv);
-                o[3] = A.Uin[3 * P + k] - dtV * (PBx + q.u * Pdiv);
-                o[4] = A.Uin[4 * P + k] - dtV * (PBy + q.v * Pdiv);
-                o[5] = A.Uin[5 * P + k] - dtV * PBz;
-                o[6] = A.Uin[6 * P + k] - dtV * (PSe + ((q.u * q.bx + q.v * q.by) * (1.0 / kCpi4)) * Pdiv);
+                o[0] = uc[0] - dtV * PSr;
+                o[1] = uc[1] - dtV * (PUx + (q.bx * (1.0 / kCpi4)) * Pdiv);
+                o[2] = uc[2] - dtV * (PUy + (q.by * (1.0 / kCpi4)) * Pdiv);
+                o[3] = uc[3] - dtV * (PBx + q.u * Pdiv);
+                o[4] = uc[4] - dtV * (PBy + q.v * Pdiv);
+                o[5] = uc[5] - dtV * PBz;
+                o[6] = uc[6] - dtV * (PSe + ((q.u * q.bx + q.v * q.by) * (1.0 / kCpi4)) * Pdiv);
                 if (!A.plain) {
 #pragma unroll
-                    for (int f = 0; f < 7; ++f) o[f] = A.a * A.Ubase[f * P + k] + A.b_coef * o[f];
+                    for (int f = 0; f < 7; ++f) o[f] = A.a * ub[f] + A.b_coef * o[f];
                 }
                 // clamps (kernel.cu:2144-2148 rho -> old rho; 2192-2195 p -> 1e-6)
                 if (o[0] <= 0.0) {
                     atomicAdd(&A.diag->clamp_rho, 1ull);
-                    o[0] = A.Uin[k];
+                    o[0] = uc[0];
                 }
                 const double inv = fast_rcp(o[0]);
                 const double ux = o[1] * inv, uy = o[2] * inv;
@@ -609,13 +745,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((LAYOUT == k
                     o[6] = 0.000001 * A.inv_g1 + ke + me;
                     p = A.g1 * (o[6] - ke - me);
                 }
+                double* ob = A.Uout + (long long)j * A.nx;
 #pragma unroll
-                for (int f = 0; f < 7; ++f) A.Uout[f * P + k] = o[f];
+                for (int f = 0; f < 7; ++f) st_buf(buf_rsrc(uniform_ptr(ob + f * A.plane)), oo, o[f]);
                 if constexpr (WANT_W) {
                     const double cf = fast_sqrt((A.gamma * p + b2 * (1.0 / kCpi4)) * inv);
                     wmax = fmax(wmax, (fabs(ux) + cf) * A.idx + (fabs(uy) + cf) * A.idy);
                 }
             } else {
+                const long long k = (long long)j * A.nx + c;
                 // primitive-form update of add2 / Kernel_TVD (kernel.cu:2137-2195);
                 // the 1/y terms only with the axisymmetric option
                 const double gm1 = A.gamma - 1.0;
