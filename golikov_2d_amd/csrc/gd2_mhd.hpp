@@ -624,7 +624,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
 
     double wmax = 0.0, tmin = 1e300;
 
+    // the wave behind on its SIMD takes the higher issue priority, decided
+    // every kTurnPeriod passes
+    SimdTurn turn;
+    turn.init();
     for (int j = j0 - 1; j < j1; ++j) {
+        if (((j - j0 + 1) & (kTurnPeriod - 1)) == 0) turn.step((j - j0 + 1) / kTurnPeriod);
         // loads first: row j+2 arrives while the x-face of row j is computed
         const Pending pn = issue_row(j + 2);
 
@@ -820,6 +825,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         tGm = tGp;
     }
 
+    turn.finish();
     if constexpr (LAYOUT == kSoA && WANT_W) {
         const double w = wave_max(wmax);
         if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
