@@ -1118,70 +1118,62 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
 // ny-2, ny-1), W += c * (primitive -> conserved)(those cotangents).
 __global__ __launch_bounds__(256) void mhd_vjp_fold_kernel(const MhdAdjArgs J) {
     const int nx = J.nx, ny = J.ny;
-    int xs[4], nxs = 0, ys[4], nys = 0;
-    {
-        const int cx[4] = {0, 1, nx - 2, nx - 1}, cy[4] = {0, 1, ny - 2, ny - 1};
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            bool dx = cx[a] >= 0 && cx[a] < nx, dy = cy[a] >= 0 && cy[a] < ny;
-            for (int b = 0; b < a; ++b) {
-                dx = dx && cx[b] != cx[a];
-                dy = dy && cy[b] != cy[a];
-            }
-            if (dx) xs[nxs++] = cx[a];
-            if (dy) ys[nys++] = cy[a];
-        }
-    }
-    const long long n1 = (long long)nxs * ny, n2 = (long long)nys * nx;
     const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
+    // thread t < 4 ny: column slot t % 4 (0, 1, nx-2, nx-1) of row t / 4;
+    // then rows 0, 1, ny-2, ny-1 of the columns 2 .. nx-3
+    const long long n1 = 4LL * ny, n2 = 4LL * nx;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2;
          t += (long long)gridDim.x * blockDim.x) {
         int i, j;
         if (t < n1) {
-            i = xs[t % nxs];
-            j = (int)(t / nxs);
+            const int a = (int)(t & 3);
+            j = (int)(t >> 2);
+            i = a < 2 ? a : nx - 4 + a;
+            // slots that repeat an earlier one (narrow grids) or leave the grid
+            // (nx-2 and nx-1 repeat column 0 or 1 exactly when they are below 2)
+            if (i < 0 || i >= nx || (a >= 2 && i < 2)) continue;
         } else {
             const long long t2 = t - n1;
             i = (int)(t2 % nx);
-            j = ys[t2 / nx];
-            bool seen = false;
-            for (int a = 0; a < nxs; ++a) seen = seen || xs[a] == i;
-            if (seen) continue;  // a band column: done by the first part
+            const int b = (int)(t2 / nx);
+            j = b < 2 ? b : ny - 4 + b;
+            if (i < 2 || i >= nx - 2) continue;  // a band column: done above
+            if (j < 0 || j >= ny || (b >= 2 && j < 2)) continue;
         }
+        // which ghosts copy this cell (x: columns -2, -1, nx, nx+1; y: rows
+        // -2, -1, ny, ny+1) and with which flip; every load below is issued
+        // unconditionally from a valid address, then selected
+        bool mx[4], fx4[4], my[4], fy4[4];
+        bool any = false;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            mx[g] = map_index(g < 2 ? g - 2 : nx + g - 2, nx, J.bc[0], J.bc[1], fx4[g]) == i;
+            my[g] = map_index(g < 2 ? g - 2 : ny + g - 2, ny, J.bc[2], J.bc[3], fy4[g]) == j;
+            any = any || mx[g] || my[g];
+        }
+        if (!any) continue;  // no ghost copies this cell
         double Pq[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int g = 0; g < 4; ++g) {
-            const int gc = g < 2 ? g - 2 : nx + g - 2;
-            bool fl;
-            if (map_index(gc, nx, J.bc[0], J.bc[1], fl) != i) continue;
-            const double* G = J.gx + ((long long)g * ny + j) * 7;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) Pq[k] += (fl && (k == 1 || k == 4)) ? -G[k] : G[k];
-        }
         for (int g = 0; g < 4; ++g) {
-            const int gr = g < 2 ? g - 2 : ny + g - 2;
-            bool fl;
-            if (map_index(gr, ny, J.bc[2], J.bc[3], fl) != j) continue;
-            const double* G = J.gy + ((long long)g * nx + i) * 7;
+            const double* Gx = J.gx + ((long long)g * ny + j) * 7;
+            const double* Gy = J.gy + ((long long)g * nx + i) * 7;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) Pq[k] += (fl && (k == 2 || k == 5)) ? -G[k] : G[k];
+            for (int k = 0; k < 7; ++k) {
+                const double ax = Gx[k], ay = Gy[k];
+                Pq[k] += mx[g] ? ((fx4[g] && (k == 1 || k == 4)) ? -ax : ax) : 0.0;
+                Pq[k] += my[g] ? ((fy4[g] && (k == 2 || k == 5)) ? -ay : ay) : 0.0;
+            }
         }
+        // the Powell terms of the edge cells whose west / east / south /
+        // north ghost copies this cell (mx[1], mx[2], my[1], my[2])
         {
-            bool fl;
-            if (map_index(-1, nx, J.bc[0], J.bc[1], fl) == i) {
-                const long long k0 = (long long)j * nx;
-                Pq[4] -= (fl ? -hx : hx) * powell_k(J, adj_cell_m(J, 0, j), k0);
-            }
-            if (map_index(nx, nx, J.bc[0], J.bc[1], fl) == i) {
-                const long long k1 = (long long)j * nx + nx - 1;
-                Pq[4] += (fl ? -hx : hx) * powell_k(J, adj_cell_m(J, nx - 1, j), k1);
-            }
-            if (map_index(-1, ny, J.bc[2], J.bc[3], fl) == j) {
-                Pq[5] -= (fl ? -hy : hy) * powell_k(J, adj_cell_m(J, i, 0), i);
-            }
-            if (map_index(ny, ny, J.bc[2], J.bc[3], fl) == j) {
-                const long long k1 = (long long)(ny - 1) * nx + i;
-                Pq[5] += (fl ? -hy : hy) * powell_k(J, adj_cell_m(J, i, ny - 1), k1);
-            }
+            const long long kw = (long long)j * nx, ke = kw + nx - 1, ks = i, kn = (long long)(ny - 1) * nx + i;
+            const double Kw = powell_k(J, adj_cell_m(J, 0, j), kw), Ke = powell_k(J, adj_cell_m(J, nx - 1, j), ke);
+            const double Ks = powell_k(J, adj_cell_m(J, i, 0), ks), Kn = powell_k(J, adj_cell_m(J, i, ny - 1), kn);
+            if (mx[1]) Pq[4] -= (fx4[1] ? -hx : hx) * Kw;
+            if (mx[2]) Pq[4] += (fx4[2] ? -hx : hx) * Ke;
+            if (my[1]) Pq[5] -= (fy4[1] ? -hy : hy) * Ks;
+            if (my[2]) Pq[5] += (fy4[2] ? -hy : hy) * Kn;
         }
         const PrimM q = adj_cell_m(J, i, j);
         double T[7];
