@@ -33,6 +33,79 @@
 
 namespace gd2 {
 
+// The forward quantities of hlld_face_vjp, from the 16 face-frame inputs IN
+// (hlld_face's expressions); expanded once for the flux-specific part of the
+// reverse sweep and once for the common part.
+#define GD2_HLLD_VJP_FORWARD(IN) \
+    [[maybe_unused]] const double ro_L = IN[0], p_L = IN[1], u1 = IN[2], v1 = IN[3], w1 = IN[4], bn1 = IN[5], bt1 = IN[6], \
+                 bm1 = IN[7]; \
+    [[maybe_unused]] const double ro_R = IN[8], p_R = IN[9], u2 = IN[10], v2 = IN[11], w2 = IN[12], bn2 = IN[13], bt2 = IN[14], \
+                 bm2 = IN[15]; \
+    /* ---------------------------------------------------------------- forward */ \
+    [[maybe_unused]] const double bb_L = bn1 * bn1 + bt1 * bt1 + bm1 * bm1; \
+    [[maybe_unused]] const double bb_R = bn2 * bn2 + bt2 * bt2 + bm2 * bm2; \
+    [[maybe_unused]] const double uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1); \
+    [[maybe_unused]] const double uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2); \
+    [[maybe_unused]] const double gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R; \
+    [[maybe_unused]] const double iroL = fast_rcp(ro_L), iroR = fast_rcp(ro_R); \
+    [[maybe_unused]] const double g4 = 4.0 * gamma; \
+    [[maybe_unused]] const double dL = gpL * gpL - g4 * p_L * (bn1 * bn1), dR = gpR * gpR - g4 * p_R * (bn2 * bn2); \
+    [[maybe_unused]] const double sdL = fast_sqrt(dL), sdR = fast_sqrt(dR); \
+    [[maybe_unused]] const double aL = (gpL + sdL) * (0.5 * iroL), aR = (gpR + sdR) * (0.5 * iroR); \
+    [[maybe_unused]] const double cfL = fast_sqrt(aL), cfR = fast_sqrt(aR); \
+    [[maybe_unused]] const bool cL = cfL >= cfR; /* tmax picks its first argument on ties */ \
+    [[maybe_unused]] const double cmax = cL ? cfL : cfR; \
+    [[maybe_unused]] const bool uminL = u1 <= u2, umaxL = u1 >= u2; \
+    [[maybe_unused]] const double SL = (uminL ? u1 : u2) - cmax; \
+    [[maybe_unused]] const double SR = (umaxL ? u1 : u2) + cmax; \
+    [[maybe_unused]] const double aSL = SL < 0.0 ? -SL : SL, aSR = SR < 0.0 ? -SR : SR; \
+    [[maybe_unused]] const bool smL = aSL >= aSR; \
+    [[maybe_unused]] const double smax = smL ? aSL : aSR; \
+    [[maybe_unused]] const double pTL = p_L + 0.5 * bb_L, pTR = p_R + 0.5 * bb_R; \
+    [[maybe_unused]] const double suR = SR - u2, suL = SL - u1; \
+    [[maybe_unused]] const double mR = suR * ro_R, mL = suL * ro_L; \
+    [[maybe_unused]] const double Dm = mR - mL; \
+    [[maybe_unused]] const double idm = fast_rcp(Dm); \
+    [[maybe_unused]] const double smR = mR * u2, smLv = mL * u1; \
+    [[maybe_unused]] const double Nn = smR - smLv - pTR + pTL; \
+    [[maybe_unused]] const double SM = Nn * idm; \
+    [[maybe_unused]] const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L; \
+    [[maybe_unused]] const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R; \
+    [[maybe_unused]] double FL[8], FR[8], UL[8], UR[8]; \
+    FL[0] = ro_L * u1; \
+    FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1; \
+    FL[2] = ro_L * u1 * v1 - bn1 * bt1; \
+    FL[3] = ro_L * u1 * w1 - bn1 * bm1; \
+    [[maybe_unused]] const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1; \
+    FL[4] = (e1 + pTL) * u1 - bn1 * sbv1; \
+    FL[5] = 0.0; \
+    FL[6] = u1 * bt1 - v1 * bn1; \
+    FL[7] = u1 * bm1 - w1 * bn1; \
+    FR[0] = ro_R * u2; \
+    FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2; \
+    FR[2] = ro_R * u2 * v2 - bn2 * bt2; \
+    FR[3] = ro_R * u2 * w2 - bn2 * bm2; \
+    [[maybe_unused]] const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2; \
+    FR[4] = (e2 + pTR) * u2 - bn2 * sbv2; \
+    FR[5] = 0.0; \
+    FR[6] = u2 * bt2 - v2 * bn2; \
+    FR[7] = u2 * bm2 - w2 * bn2; \
+    UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1; \
+    UL[4] = e1; UL[5] = bn1; UL[6] = bt1; UL[7] = bm1; \
+    UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2; \
+    UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2; \
+    [[maybe_unused]] const double DS = SR - SL; \
+    [[maybe_unused]] const double iS = fast_rcp(DS); \
+    [[maybe_unused]] const double nbn = SR * bn2 - SL * bn1; \
+    [[maybe_unused]] const double nbt = SR * bt2 - SL * bt1 + FL[6] - FR[6]; \
+    [[maybe_unused]] const double nbm = SR * bm2 - SL * bm1 + FL[7] - FR[7]; \
+    [[maybe_unused]] const double bn = nbn * iS, bt = nbt * iS, bm = nbm * iS; \
+    [[maybe_unused]] const double bbn = bn * bn; \
+    [[maybe_unused]] const double DLM = SL - SM, DRM = SR - SM; \
+    [[maybe_unused]] const double iLM = fast_rcp(DLM), iRM = fast_rcp(DRM); \
+    [[maybe_unused]] const double ro_LL = ro_L * suL * iLM; \
+    [[maybe_unused]] const double ro_RR = ro_R * suR * iRM;
+
 // Reverse-mode hlld_face: the forward values recomputed with hlld_face's
 // expressions (refined hardware reciprocal and square root, multiply-adds
 // contracted within an expression, the contact speed's numerator products
@@ -42,75 +115,6 @@ namespace gd2 {
 template <int METOD>
 __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gamma, double inv_g1,
                                               const double (&Pb)[8], double (&ib)[16]) {
-    const double ro_L = in[0], p_L = in[1], u1 = in[2], v1 = in[3], w1 = in[4], bn1 = in[5], bt1 = in[6],
-                 bm1 = in[7];
-    const double ro_R = in[8], p_R = in[9], u2 = in[10], v2 = in[11], w2 = in[12], bn2 = in[13], bt2 = in[14],
-                 bm2 = in[15];
-    // ---------------------------------------------------------------- forward
-    const double bb_L = bn1 * bn1 + bt1 * bt1 + bm1 * bm1;
-    const double bb_R = bn2 * bn2 + bt2 * bt2 + bm2 * bm2;
-    const double uu_L = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
-    const double uu_R = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
-    const double gpL = gamma * p_L + bb_L, gpR = gamma * p_R + bb_R;
-    const double iroL = fast_rcp(ro_L), iroR = fast_rcp(ro_R);
-    const double g4 = 4.0 * gamma;
-    const double dL = gpL * gpL - g4 * p_L * (bn1 * bn1), dR = gpR * gpR - g4 * p_R * (bn2 * bn2);
-    const double sdL = fast_sqrt(dL), sdR = fast_sqrt(dR);
-    const double aL = (gpL + sdL) * (0.5 * iroL), aR = (gpR + sdR) * (0.5 * iroR);
-    const double cfL = fast_sqrt(aL), cfR = fast_sqrt(aR);
-    const bool cL = cfL >= cfR;  // tmax picks its first argument on ties
-    const double cmax = cL ? cfL : cfR;
-    const bool uminL = u1 <= u2, umaxL = u1 >= u2;
-    const double SL = (uminL ? u1 : u2) - cmax;
-    const double SR = (umaxL ? u1 : u2) + cmax;
-    const double aSL = SL < 0.0 ? -SL : SL, aSR = SR < 0.0 ? -SR : SR;
-    const bool smL = aSL >= aSR;
-    const double smax = smL ? aSL : aSR;
-    const double pTL = p_L + 0.5 * bb_L, pTR = p_R + 0.5 * bb_R;
-    const double suR = SR - u2, suL = SL - u1;
-    const double mR = suR * ro_R, mL = suL * ro_L;
-    const double Dm = mR - mL;
-    const double idm = fast_rcp(Dm);
-    const double smR = mR * u2, smLv = mL * u1;
-    const double Nn = smR - smLv - pTR + pTL;
-    const double SM = Nn * idm;
-    const double e1 = p_L * inv_g1 + ro_L * uu_L + 0.5 * bb_L;
-    const double e2 = p_R * inv_g1 + ro_R * uu_R + 0.5 * bb_R;
-    double FL[8], FR[8], UL[8], UR[8];
-    FL[0] = ro_L * u1;
-    FL[1] = ro_L * u1 * u1 + pTL - bn1 * bn1;
-    FL[2] = ro_L * u1 * v1 - bn1 * bt1;
-    FL[3] = ro_L * u1 * w1 - bn1 * bm1;
-    const double sbv1 = u1 * bn1 + v1 * bt1 + w1 * bm1;
-    FL[4] = (e1 + pTL) * u1 - bn1 * sbv1;
-    FL[5] = 0.0;
-    FL[6] = u1 * bt1 - v1 * bn1;
-    FL[7] = u1 * bm1 - w1 * bn1;
-    FR[0] = ro_R * u2;
-    FR[1] = ro_R * u2 * u2 + pTR - bn2 * bn2;
-    FR[2] = ro_R * u2 * v2 - bn2 * bt2;
-    FR[3] = ro_R * u2 * w2 - bn2 * bm2;
-    const double sbv2 = u2 * bn2 + v2 * bt2 + w2 * bm2;
-    FR[4] = (e2 + pTR) * u2 - bn2 * sbv2;
-    FR[5] = 0.0;
-    FR[6] = u2 * bt2 - v2 * bn2;
-    FR[7] = u2 * bm2 - w2 * bn2;
-    UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1;
-    UL[4] = e1; UL[5] = bn1; UL[6] = bt1; UL[7] = bm1;
-    UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2;
-    UR[4] = e2; UR[5] = bn2; UR[6] = bt2; UR[7] = bm2;
-    const double DS = SR - SL;
-    const double iS = fast_rcp(DS);
-    const double nbn = SR * bn2 - SL * bn1;
-    const double nbt = SR * bt2 - SL * bt1 + FL[6] - FR[6];
-    const double nbm = SR * bm2 - SL * bm1 + FL[7] - FR[7];
-    const double bn = nbn * iS, bt = nbt * iS, bm = nbm * iS;
-    const double bbn = bn * bn;
-    const double DLM = SL - SM, DRM = SR - SM;
-    const double iLM = fast_rcp(DLM), iRM = fast_rcp(DRM);
-    const double ro_LL = ro_L * suL * iLM;
-    const double ro_RR = ro_R * suR * iRM;
-
     // adjoints of every forward quantity
     double b_ro_L = 0.0, b_p_L = 0.0, b_u1 = 0.0, b_v1 = 0.0, b_w1 = 0.0, b_bn1 = 0.0, b_bt1 = 0.0, b_bm1 = 0.0;
     double b_ro_R = 0.0, b_p_R = 0.0, b_u2 = 0.0, b_v2 = 0.0, b_w2 = 0.0, b_bn2 = 0.0, b_bt2 = 0.0, b_bm2 = 0.0;
@@ -127,6 +131,8 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
     double b_iS = 0.0, b_iLM = 0.0, b_iRM = 0.0, b_e1 = 0.0, b_e2 = 0.0, b_pTL = 0.0, b_pTR = 0.0;
     double b_suL = 0.0, b_suR = 0.0, b_mL = 0.0, b_mR = 0.0, b_idm = 0.0;
 
+    {
+    GD2_HLLD_VJP_FORWARD(in)
     // P[5] = -(smax * (bn2 - bn1)) replaces the region formula's P[5]
     {
         const double g = Pb[5];
@@ -492,6 +498,19 @@ __device__ __forceinline__ void hlld_face_vjp(const double (&in)[16], double gam
             b_ro_L += b_a1 * ro_R; b_ro_R += b_a1 * ro_L;
         }
     }
+    }
+    // The common part of the sweep reads the forward quantities again,
+    // recomputed from the inputs passed through an empty asm: the compiler
+    // cannot reuse the first copies, so those need not stay live through the
+    // flux-specific part (the recomputation is bitwise the first).
+    double in2[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        double x = in[i];
+        asm volatile("" : "+v"(x));
+        in2[i] = x;
+    }
+    GD2_HLLD_VJP_FORWARD(in2)
     // ---------------------------------------------------------------- common reverse
     // bbn = bn*bn is folded above for HLLD; HLLC-MHD does not use bbn
     // ro_LL = (ro_L*suL)*iLM, ro_RR likewise
@@ -763,7 +782,7 @@ __device__ __forceinline__ double comp_m(const PrimM& q, int f) {
 // One thread per face f + 1/2 of a line (f = -1 .. n-1): the cotangents of
 // its reconstructed left and right states.
 template <int ORDER, int METOD, bool XFACE>
-__global__ __launch_bounds__(256) void mhd_vjp_face_kernel(const MhdAdjArgs J) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_face_kernel(const MhdAdjArgs J) {
     const long long nf = XFACE ? J.nfx : J.nfy;
     double* out = XFACE ? J.wx : J.wy;
     const int n = XFACE ? J.nx : J.ny;
