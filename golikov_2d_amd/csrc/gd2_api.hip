@@ -329,7 +329,8 @@ static int launch_mhd_vjp(MhdAdjArgs J, hipStream_t st) {
     hipLaunchKernelGGL((mhd_vjp_gather_kernel<ORDER>), dim3(blocks), dim3(kWave), 0, st, J);
     GD2_HIP(hipGetLastError());
     // the boundary band: up to 4 columns x ny + 4 rows x nx cells
-    hipLaunchKernelGGL(mhd_vjp_fold_kernel, dim3(grid_stride_blocks(4LL * (J.nx + J.ny))), dim3(256), 0, st, J);
+    hipLaunchKernelGGL(mhd_vjp_fold_kernel, dim3((unsigned)((4LL * (J.nx + J.ny) + kWave - 1) / kWave)), dim3(kWave), 0,
+                       st, J);
     GD2_HIP(hipGetLastError());
     return GD2_OK;
 }
@@ -359,6 +360,7 @@ static int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* 
     J.dx = g->dx;
     J.dy = g->dy;
     J.dV = g->dx * g->dy;
+    J.idV = 1.0 / J.dV;
     J.gamma = g->gamma;
     J.g1 = g->gamma - 1.0;
     J.inv_g1 = 1.0 / (g->gamma - 1.0);

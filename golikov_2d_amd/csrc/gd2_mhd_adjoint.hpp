@@ -743,7 +743,7 @@ struct MhdAdjArgs {
     int nx, ny;
     long long plane;
     long long nfx, nfy;
-    double dx, dy, dV;
+    double dx, dy, dV, idV;  // idV = 1 / dV
     double gamma, g1, inv_g1;
     int bc[4];
 };
@@ -862,7 +862,7 @@ __device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, 
     const double vmx = J.V[P + k], vmy = J.V[2 * P + k], vbx = J.V[3 * P + k], vby = J.V[4 * P + k],
                  ve = J.V[6 * P + k];
     const double K = (vmx * q.bx + vmy * q.by + ve * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) + vbx * q.u + vby * q.v;
-    return -K / J.dV;
+    return -K * J.idV;
 }
 
 // Primitives of a cell from its 7 conserved values (the expressions of
@@ -913,7 +913,7 @@ __device__ __forceinline__ void spread7(const PrimM& qa, const PrimM& qb, const 
 // u = mx/r, v = my/r, p = g1 (E - (mx u + my v)/2 - |B|^2 / 8pi)
 __device__ __forceinline__ void prim_to_cons_cot(const MhdAdjArgs& J, const PrimM& q, const double (&Pq)[7],
                                                  double (&T)[7]) {
-    const double ir = 1.0 / q.r;
+    const double ir = fast_rcp(q.r);
     const double gp = J.g1 * Pq[3];
     T[0] = Pq[0] - (q.u * Pq[1] + q.v * Pq[2]) * ir + gp * 0.5 * (q.u * q.u + q.v * q.v);
     T[1] = Pq[1] * ir - gp * q.u;
@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         vv[4] = ld_buf(buf_rsrc(uniform_ptr(b + 6 * P)), o);
         const double K = (vv[0] * q.bx + vv[1] * q.by + vv[4] * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) +
                          vv[2] * q.u + vv[3] * q.v;
-        return (in && col_in) ? -K / J.dV : 0.0;
+        return (in && col_in) ? -K * J.idV : 0.0;
     };
 
     // prologue: rows ja-2 .. ja+1; y-spreads of rows ja-1 (its share to row
@@ -1082,7 +1082,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
                 const double pdiv = J.dy * (qe.bx - qw.bx) * 0.5 + J.dx * (qp.by - qm.by) * 0.5;
                 {
                     const double vmx = v0[0], vmy = v0[1], vbx = v0[2], vby = v0[3], ve = v0[4];
-                    const double sc = -pdiv / J.dV;
+                    const double sc = -pdiv * J.idV;
                     Pq[4] += sc * (vmx + ve * q.u) * (1.0 / kCpi4);
                     Pq[5] += sc * (vmy + ve * q.v) * (1.0 / kCpi4);
                     Pq[1] += sc * (vbx + ve * q.bx * (1.0 / kCpi4));
@@ -1135,7 +1135,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
 // whose divergence estimate reads a ghost that copies the cell: one thread
 // per cell of the boundary band (columns 0, 1, nx-2, nx-1 and rows 0, 1,
 // ny-2, ny-1), W += c * (primitive -> conserved)(those cotangents).
-__global__ __launch_bounds__(256) void mhd_vjp_fold_kernel(const MhdAdjArgs J) {
+__global__ __launch_bounds__(64) void mhd_vjp_fold_kernel(const MhdAdjArgs J) {
     const int nx = J.nx, ny = J.ny;
     const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
     // thread t < 4 ny: column slot t % 4 (0, 1, nx-2, nx-1) of row t / 4;
