@@ -74,6 +74,13 @@ CASES = [
     (37, 30, (R, R, T, T), 3, 2, "ot"),
     (3, 5, (T, R, P, P), 3, 2, "random"),
     (2, 9, (R, R, T, R), 2, 2, "random"),
+    # wider than one 60-column strip of the gather march: lane shifts across
+    # strips, the ghost columns of the last strip (121: a one-column strip,
+    # whose neighbour holds column nx on lane 63), periodic ghosts in other strips
+    (121, 11, (P, P, T, T), 3, 2, "random"),
+    (125, 13, (R, T, R, T), 2, 2, "random"),
+    (61, 9, (T, T, P, P), 3, 1, "random"),
+    (121, 7, (R, R, P, P), 3, 2, "random"),
 ]
 
 
