@@ -308,21 +308,18 @@ static unsigned ew_blocks(long long n) {
     return (unsigned)b;
 }
 
-static unsigned grid_stride_blocks(long long n) {
-    long long b = (n + 255) / 256;
-    if (b > 8192) b = 8192;
-    if (b < 1) b = 1;
-    return (unsigned)b;
-}
 
 template <int ORDER, int METOD>
 static int launch_mhd_vjp(MhdAdjArgs J, hipStream_t st) {
-    hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, true>), dim3(grid_stride_blocks(J.nfx)), dim3(256), 0, st,
-                       J);
-    GD2_HIP(hipGetLastError());
-    hipLaunchKernelGGL((mhd_vjp_face_kernel<ORDER, METOD, false>), dim3(grid_stride_blocks(J.nfy)), dim3(256), 0, st,
-                       J);
-    GD2_HIP(hipGetLastError());
+    {
+        MhdAdjArgs F = J;
+        F.tiles_x = (F.nx + kFaceW - 1) / kFaceW;
+        F.rows = strip_rows(reinterpret_cast<const void*>(&mhd_vjp_face_march_kernel<ORDER, METOD>), F.tiles_x, F.ny,
+                            1, 64, 1);
+        const unsigned blocks = (unsigned)(F.tiles_x * ((F.ny + F.rows - 1) / F.rows));
+        hipLaunchKernelGGL((mhd_vjp_face_march_kernel<ORDER, METOD>), dim3(blocks), dim3(kWave), 0, st, F);
+        GD2_HIP(hipGetLastError());
+    }
     J.tiles_x = (J.nx + kGatherW - 1) / kGatherW;
     J.rows = strip_rows(reinterpret_cast<const void*>(&mhd_vjp_gather_kernel<ORDER>), J.tiles_x, J.ny, 4);
     const unsigned blocks = (unsigned)(J.tiles_x * ((J.ny + J.rows - 1) / J.rows));
@@ -361,6 +358,8 @@ static int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* 
     J.dy = g->dy;
     J.dV = g->dx * g->dy;
     J.idV = 1.0 / J.dV;
+    J.hfx = g->dy / J.dV;
+    J.hfy = g->dx / J.dV;
     J.gamma = g->gamma;
     J.g1 = g->gamma - 1.0;
     J.inv_g1 = 1.0 / (g->gamma - 1.0);

@@ -5,7 +5,8 @@
 // For L(U) = -(1/dV) (sum of face fluxes x face lengths + Powell term) the
 // kernels compute W = a*A + b*B + c * J_L(U)^T V in two passes:
 //
-//  1. one thread per face (x-faces, then y-faces): the face cotangent
+//  1. a face march (one wave per strip of columns, up its rows; each lane
+//     reverses one y-face and one x-face per row): the face cotangent
 //     phi = (h/dV)(V_right - V_left) (zero on the side of a ghost cell) is
 //     taken through the face flux to its two reconstructed states by one
 //     reverse sweep of hlld_face (hand-reversed, the same branches as the
@@ -744,6 +745,7 @@ struct MhdAdjArgs {
     long long plane;
     long long nfx, nfy;
     double dx, dy, dV, idV;  // idV = 1 / dV
+    double hfx, hfy;         // dy / dV, dx / dV: x- and y-face length over cell area
     double gamma, g1, inv_g1;
     int bc[4];
 };
@@ -779,94 +781,8 @@ __device__ __forceinline__ double comp_m(const PrimM& q, int f) {
     }
 }
 
-// One thread per face f + 1/2 of a line (f = -1 .. n-1): the cotangents of
-// its reconstructed left and right states.
-template <int ORDER, int METOD, bool XFACE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_face_kernel(const MhdAdjArgs J) {
-    const long long nf = XFACE ? J.nfx : J.nfy;
-    double* out = XFACE ? J.wx : J.wy;
-    const int n = XFACE ? J.nx : J.ny;
-    const double h = (XFACE ? J.dy : J.dx) / J.dV;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nf;
-         t += (long long)gridDim.x * blockDim.x) {
-        int i, j, f;  // the face lies between cell f and cell f+1 of its line
-        if constexpr (XFACE) {
-            f = (int)(t % (J.nx + 1)) - 1;
-            j = (int)(t / (J.nx + 1));
-            i = f;
-        } else {
-            i = (int)(t % J.nx);
-            f = (int)(t / J.nx) - 1;
-            j = f;
-        }
-        auto cell = [&](int o) { return XFACE ? adj_cell_m(J, i + o, j) : adj_cell_m(J, i, j + o); };
-        const PrimM q0 = cell(-1), q1 = cell(0), q2 = cell(1), q3 = cell(2);
-        const PrimM s1 = slope3m<ORDER>(q0, q1, q2), s2 = slope3m<ORDER>(q1, q2, q3);
-        const PrimM L = face_m(q1, s1, 0.5), R = face_m(q2, s2, -0.5);
-
-        // face cotangent: the face flux leaves cell f and enters cell f+1
-        double phi[7];
-        const long long kl = (long long)j * J.nx + i;
-        const long long kr = XFACE ? kl + 1 : kl + J.nx;
-#pragma unroll
-        for (int c = 0; c < 7; ++c) {
-            const double vl = f >= 0 ? J.V[c * J.plane + kl] : 0.0;
-            const double vr = f + 1 < n ? J.V[c * J.plane + kr] : 0.0;
-            phi[c] = h * (vr - vl);
-        }
-        // a reflective west / south wall face is evaluated mirrored:
-        // F = unmirror(flux(mirror(R), mirror(L)))
-        const bool wall = f == -1 && J.bc[XFACE ? 0 : 2] == GD2_BC_REFLECTIVE;
-        PrimM X = L, Y = R;
-        if (wall) {
-            X = XFACE ? mirror_x(R) : mirror_y(R);
-            Y = XFACE ? mirror_x(L) : mirror_y(L);
-            // unmirror negates mass, energy and the tangential momentum / field
-            phi[0] = -phi[0];
-            phi[6] = -phi[6];
-            phi[5] = -phi[5];
-            if (XFACE) {
-                phi[2] = -phi[2];
-                phi[4] = -phi[4];
-            } else {
-                phi[1] = -phi[1];
-                phi[3] = -phi[3];
-            }
-        }
-        const int mu = XFACE ? 1 : 2, mb = XFACE ? 4 : 5;  // components a mirror negates
-        double Xv[7], Yv[7], Xb[7], Yb[7];
-#pragma unroll
-        for (int c = 0; c < 7; ++c) {
-            Xv[c] = comp_m(X, c);
-            Yv[c] = comp_m(Y, c);
-        }
-        mhd_flux_vjp<METOD, XFACE>(Xv, Yv, J.gamma, J.inv_g1, phi, Xb, Yb);
-#pragma unroll
-        for (int k = 0; k < 14; ++k) {
-            double s = k < 7 ? Xb[k] : Yb[k - 7];
-            int pl = k;
-            if (wall) {
-                const int comp = k < 7 ? k : k - 7;
-                pl = k < 7 ? comp + 7 : comp;  // X is the mirrored right state
-                if (comp == mu || comp == mb) s = -s;
-            }
-            out[pl * nf + t] = s;
-        }
-    }
-}
-
-// K(q) . V of the Powell term of a cell (its derivative with respect to the
-// cell's divergence estimate), times -1/dV.
-__device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, long long k) {
-    const long long P = J.plane;
-    const double vmx = J.V[P + k], vmy = J.V[2 * P + k], vbx = J.V[3 * P + k], vby = J.V[4 * P + k],
-                 ve = J.V[6 * P + k];
-    const double K = (vmx * q.bx + vmy * q.by + ve * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) + vbx * q.u + vby * q.v;
-    return -K * J.idV;
-}
-
 // Primitives of a cell from its 7 conserved values (the expressions of
-// cons_to_prim_m, so the gather sees the values the face kernels see).
+// cons_to_prim_m); the face march and the gather both convert through it.
 __device__ __forceinline__ PrimM prim_of_cons7(const double (&w)[7], double g1) {
     PrimM q;
     const double r = w[0], mx = w[1], my = w[2];
@@ -880,6 +796,218 @@ __device__ __forceinline__ PrimM prim_of_cons7(const double (&w)[7], double g1) 
     q.v = my * inv;
     q.p = g1 * (E - 0.5 * (mx * q.u + my * q.v) - (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8));
     return q;
+}
+
+// The 14 face-state cotangents of one face (planes 0..6: its left state,
+// 7..13: its right state) from its reconstructed states L, R and its face
+// cotangent phi (conserved order). A reflective west / south wall face is
+// evaluated mirrored, F = unmirror(flux(mirror(R), mirror(L))), as the forward
+// stage does.
+template <int METOD, bool XFACE>
+__device__ __forceinline__ void face_state_cot(const MhdAdjArgs& J, const PrimM& L, const PrimM& R, double (&phi)[7],
+                                               bool wall, double (&s)[14]) {
+    PrimM X = L, Y = R;
+    if (wall) {
+        X = XFACE ? mirror_x(R) : mirror_y(R);
+        Y = XFACE ? mirror_x(L) : mirror_y(L);
+        // unmirror negates mass, energy and the tangential momentum / field
+        phi[0] = -phi[0];
+        phi[6] = -phi[6];
+        phi[5] = -phi[5];
+        if (XFACE) {
+            phi[2] = -phi[2];
+            phi[4] = -phi[4];
+        } else {
+            phi[1] = -phi[1];
+            phi[3] = -phi[3];
+        }
+    }
+    const int mu = XFACE ? 1 : 2, mb = XFACE ? 4 : 5;  // components a mirror negates
+    double Xv[7], Yv[7], Xb[7], Yb[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        Xv[c] = comp_m(X, c);
+        Yv[c] = comp_m(Y, c);
+    }
+    mhd_flux_vjp<METOD, XFACE>(Xv, Yv, J.gamma, J.inv_g1, phi, Xb, Yb);
+    // on a wall face X is the mirrored right state and Y the mirrored left
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        const bool neg = c == mu || c == mb;
+        s[c] = wall ? (neg ? -Yb[c] : Yb[c]) : Xb[c];
+        s[c + 7] = wall ? (neg ? -Xb[c] : Xb[c]) : Yb[c];
+    }
+}
+
+constexpr int kFaceW = 60;  // strip width of the face march (lanes 2..61 own its columns)
+
+// Rows of a strip of the face march in LDS: the primitive rows p-1 and p
+// (slot row & 1; row p+1 replaces row p-1 once the y-face has read it), the
+// cotangent row p-1 and the upper-face value of row p-1 (the left state of
+// its y-face).
+struct MhdFaceLds {
+    double q[2][kMF][kRowBuf];
+    double v[1][kMF][kRowBuf];
+    double up[1][kMF][kRowBuf];
+};
+
+// The face march: one wave per strip of kFaceW columns and rows [j0, j1).
+// Lane l holds column c = cbase - 2 + l. Pass p loads row p+1 (through the
+// boundary map), reverses the flux of the y-face between rows p-1 and p
+// (lanes 2..61 of the grid's columns), then of the x-face c + 1/2 of row p
+// (lanes 1..60 from face -1/2 on; lane 61 for the east boundary face);
+// passes run from j0 to j1 - 1, and to ny on the top strip (its y-face
+// ny - 1/2). Every face is reversed by one lane of one wave; its 14
+// face-state cotangents go to the face buffers. Each cell is loaded and
+// converted once per strip; the face states come from the LDS rows and lane
+// shifts, and nothing stays in registers across a face reversal (the empty
+// asm statements keep the compiler from forwarding LDS values across one).
+template <int ORDER, int METOD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_face_march_kernel(
+    const MhdAdjArgs J) {
+    __shared__ MhdFaceLds lds;
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+    const int nx = J.nx, ny = J.ny;
+    const int c = tx * kFaceW - 2 + lane;
+    const int j0 = ty * J.rows, j1 = min(j0 + J.rows, ny);
+    const int pe = j1 == ny ? ny : j1 - 1;  // last pass
+    const bool col_in = c >= 0 && c < nx;
+    const bool own_y = col_in && lane >= 2 && lane <= 61;
+    const bool own_x = (lane >= 1 && lane <= 60 && c >= -1 && c <= nx - 1) || (lane == 61 && c == nx - 1);
+    const bool wall_x = c == -1 && J.bc[0] == GD2_BC_REFLECTIVE;
+    bool fx;
+    const int ci = map_index(c, nx, J.bc[0], J.bc[1], fx);
+    const unsigned oc = (unsigned)ci * 8u;
+    const unsigned ov = col_in ? (unsigned)c * 8u : kOffNone;       // V of this column
+    const unsigned oy = own_y ? (unsigned)c * 8u : kOffNone;        // its y-face
+    const unsigned ox = own_x ? (unsigned)(c + 1) * 8u : kOffNone;  // its x-face
+    const long long P = J.plane;
+
+    auto load_raw = [&](int jl, double (&w)[7]) {
+        bool fy;
+        const int jj = map_index(jl, ny, J.bc[2], J.bc[3], fy);
+        const double* b = J.U + (long long)jj * nx;
+#pragma unroll
+        for (int f = 0; f < 7; ++f) w[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * P)), oc);
+        return fy;
+    };
+    auto to_prim = [&](const double (&w)[7], bool fy) -> PrimM {
+        PrimM q = prim_of_cons7(w, J.g1);
+        if (fx) {
+            q.u = -q.u;
+            q.bx = -q.bx;
+        }
+        if (fy) {
+            q.v = -q.v;
+            q.by = -q.by;
+        }
+        return q;
+    };
+    auto load_v = [&](int r, double (&v)[7]) {
+        const bool in = r >= 0 && r < ny;
+        const double* b = J.V + (long long)(in ? r : 0) * nx;
+        const unsigned o = in ? ov : kOffNone;
+#pragma unroll
+        for (int f = 0; f < 7; ++f) v[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * P)), o);
+    };
+
+    // prologue: rows j0-1 and j0, the upper-face value of row j0-1 and the
+    // cotangent row j0-1
+    {
+        double wa[7], wb[7], wc[7], v[7];
+        const bool fa = load_raw(j0 - 2, wa), fb = load_raw(j0 - 1, wb), fc = load_raw(j0, wc);
+        load_v(j0 - 1, v);
+        const PrimM qa = to_prim(wa, fa), qb = to_prim(wb, fb), qc = to_prim(wc, fc);
+        lds_put_m(lds.q, (j0 - 1) & 1, lane, qb);
+        lds_put_m(lds.q, j0 & 1, lane, qc);
+        lds_put_m(lds.up, 0, lane, face_m(qb, slope3m<ORDER>(qa, qb, qc), 0.5));
+#pragma unroll
+        for (int f = 0; f < 7; ++f) lds.v[0][f][lane] = v[f];
+    }
+
+    for (int p = j0; p <= pe; ++p) {
+        asm volatile("" ::: "memory");
+        {
+            // y-face between rows p-1 and p; row p+1 then takes row p-1's slot
+            // and the cotangent row p the slot of row p-1
+            double w[7], vp[7];
+            const bool fn = load_raw(p + 1, w);
+            load_v(p, vp);
+            // the lane index taken afresh: LDS addresses are formed per pass,
+            // not held across the face reversals
+            int e = lane;
+            asm volatile("" : "+v"(e));
+            const PrimM qs = lds_get_m(lds.q, (p - 1) & 1, e), q = lds_get_m(lds.q, p & 1, e);
+            const PrimM L = lds_get_m(lds.up, 0, e);
+            double phi[7];
+#pragma unroll
+            for (int f = 0; f < 7; ++f) {
+                const double vs = lds.v[0][f][e];
+                phi[f] = J.hfy * (vp[f] - vs);
+                lds.v[0][f][e] = vp[f];
+            }
+            const PrimM qn = to_prim(w, fn);
+            lds_put_m(lds.q, (p + 1) & 1, e, qn);
+            const PrimM s = slope3m<ORDER>(qs, q, qn);
+            const PrimM R = face_m(q, s, -0.5);
+            lds_put_m(lds.up, 0, e, face_m(q, s, 0.5));
+            const bool wall = p == 0 && J.bc[2] == GD2_BC_REFLECTIVE;
+            double o[14];
+            face_state_cot<METOD, false>(J, L, R, phi, wall, o);
+            double* wy = J.wy + (long long)p * nx;
+#pragma unroll
+            for (int k = 0; k < 14; ++k) st_buf(buf_rsrc(uniform_ptr(wy + k * J.nfy)), oy, o[k]);
+        }
+        asm volatile("" ::: "memory");
+        if (p < j1) {
+            // x-face c + 1/2 of row p: states from the row's lane shifts,
+            // cotangent from this column's and the next column's V
+            int e = lane;
+            asm volatile("" : "+v"(e));
+            const PrimM q = lds_get_m(lds.q, p & 1, e);
+            PrimM qw, qe;
+            qw.r = from_prev_lane(q.r); qe.r = from_next_lane(q.r);
+            qw.u = from_prev_lane(q.u); qe.u = from_next_lane(q.u);
+            qw.v = from_prev_lane(q.v); qe.v = from_next_lane(q.v);
+            qw.p = from_prev_lane(q.p); qe.p = from_next_lane(q.p);
+            qw.bx = from_prev_lane(q.bx); qe.bx = from_next_lane(q.bx);
+            qw.by = from_prev_lane(q.by); qe.by = from_next_lane(q.by);
+            qw.bz = from_prev_lane(q.bz); qe.bz = from_next_lane(q.bz);
+            const PrimM s = slope3m<ORDER>(qw, q, qe);
+            const PrimM L = face_m(q, s, 0.5), Rm = face_m(q, s, -0.5);
+            PrimM R;
+            R.r = from_next_lane(Rm.r);
+            R.u = from_next_lane(Rm.u);
+            R.v = from_next_lane(Rm.v);
+            R.p = from_next_lane(Rm.p);
+            R.bx = from_next_lane(Rm.bx);
+            R.by = from_next_lane(Rm.by);
+            R.bz = from_next_lane(Rm.bz);
+            double phi[7];
+#pragma unroll
+            for (int f = 0; f < 7; ++f) {
+                const double vl = lds.v[0][f][e];
+                phi[f] = J.hfx * (from_next_lane(vl) - vl);
+            }
+            double o[14];
+            face_state_cot<METOD, true>(J, L, R, phi, wall_x, o);
+            double* wx = J.wx + (long long)p * (nx + 1);
+#pragma unroll
+            for (int k = 0; k < 14; ++k) st_buf(buf_rsrc(uniform_ptr(wx + k * J.nfx)), ox, o[k]);
+        }
+    }
+}
+
+// K(q) . V of the Powell term of a cell (its derivative with respect to the
+// cell's divergence estimate), times -1/dV.
+__device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, long long k) {
+    const long long P = J.plane;
+    const double vmx = J.V[P + k], vmy = J.V[2 * P + k], vbx = J.V[3 * P + k], vby = J.V[4 * P + k],
+                 ve = J.V[6 * P + k];
+    const double K = (vmx * q.bx + vmy * q.by + ve * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) + vbx * q.u + vby * q.v;
+    return -K * J.idV;
 }
 
 // One cell's shares of the reconstruction adjoint along one line: qa, qb, qd
