@@ -348,10 +348,9 @@ static int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* 
     J.nx = g->nx;
     J.ny = g->ny;
     J.plane = (long long)g->nx * g->ny;
-    J.nfx = (long long)(g->nx + 1) * g->ny;
     J.nfy = (long long)g->nx * (g->ny + 1);
-    J.wx = static_cast<double*>(workspace);
-    J.wy = J.wx + 14 * J.nfx;
+    J.cx = static_cast<double*>(workspace);
+    J.wy = J.cx + 7 * J.plane;
     J.gx = J.wy + 14 * J.nfy;
     J.gy = J.gx + 28LL * g->ny;
     J.dx = g->dx;
@@ -649,10 +648,11 @@ int gd2_ctx_log_dt(gd2_ctx* c, double* dt_log, long long capacity) {
 long long gd2_vjp_workspace_bytes(const gd2_grid* g, const gd2_scheme* sc) {
     if (!g || !sc || g->nx < 1 || g->ny < 1) return GD2_ERR_ARG;
     if (sc->physics == GD2_MHD) {
-        // face-state cotangents: 14 planes over the x-faces and the y-faces;
-        // the gathers of the 4 ghost columns and the 4 ghost rows (7 each)
-        const long long nf = (long long)(g->nx + 1) * g->ny + (long long)g->nx * (g->ny + 1);
-        return (long long)sizeof(double) * (14 * nf + 28LL * (g->nx + g->ny));
+        // the cells' x-gathers (7 planes), the y-face-state cotangents (14
+        // planes over the y-faces), the gathers of the 4 ghost columns and the
+        // 4 ghost rows (7 each)
+        const long long cells = (long long)g->nx * g->ny, nfy = (long long)g->nx * (g->ny + 1);
+        return (long long)sizeof(double) * (7 * cells + 14 * nfy + 28LL * (g->nx + g->ny));
     }
     const long long len = g->nx > g->ny ? g->nx : g->ny;
     return (long long)sizeof(double) * 4 * 2 * len * 4;

@@ -736,14 +736,14 @@ struct MhdAdjArgs {
     const double* B;  // optional
     double* W;        // output
     double a, b, c;   // W = a*A + b*B + c*J^T V
-    double* wx;       // x-face state cotangents: 14 planes of (nx+1) x ny
+    double* cx;       // x-gathers of the cells: 7 planes of nx x ny
     double* wy;       // y-face state cotangents: 14 planes of nx x (ny+1)
     double* gx;       // gathers of the ghost columns -2, -1, nx, nx+1: [4][ny][7]
     double* gy;       // gathers of the ghost rows -2, -1, ny, ny+1: [4][nx][7]
     int tiles_x, rows;  // strips of the gather march
     int nx, ny;
     long long plane;
-    long long nfx, nfy;
+    long long nfy;    // y-faces: nx x (ny+1)
     double dx, dy, dV, idV;  // idV = 1 / dV
     double hfx, hfy;         // dy / dV, dx / dV: x- and y-face length over cell area
     double gamma, g1, inv_g1;
@@ -798,6 +798,33 @@ __device__ __forceinline__ PrimM prim_of_cons7(const double (&w)[7], double g1) 
     return q;
 }
 
+// One cell's shares of the reconstruction adjoint along one line: qa, qb, qd
+// the cell before, the cell itself and the cell after; dqp the cotangent of
+// its plus-face value (left state of the face after it), dqm of its
+// minus-face value (right state of the face before it).
+template <int ORDER>
+__device__ __forceinline__ void spread7(const PrimM& qa, const PrimM& qb, const PrimM& qd, const double (&dqp)[7],
+                                        const double (&dqm)[7], double (&own)[7], double (&to_prev)[7],
+                                        double (&to_next)[7]) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if constexpr (ORDER == 1) {
+            own[k] = dqp[k] + dqm[k];
+            to_prev[k] = to_next[k] = 0.0;
+        } else {
+            const double a = comp_m(qa, k), b = comp_m(qb, k), d = comp_m(qd, k);
+            const MMW wt = mm_weights(b - a, d - b);
+            bool fp = false, fm = false;
+            if (k == 0 || k == 3) {
+                const double sl = minmod(b - a, d - b);
+                fp = !(b + 0.5 * sl > 0.0);
+                fm = !(b + -0.5 * sl > 0.0);
+            }
+            spread(dqp[k], dqm[k], wt, fp, fm, own[k], to_prev[k], to_next[k]);
+        }
+    }
+}
+
 // The 14 face-state cotangents of one face (planes 0..6: its left state,
 // 7..13: its right state) from its reconstructed states L, R and its face
 // cotangent phi (conserved order). A reflective west / south wall face is
@@ -839,7 +866,7 @@ __device__ __forceinline__ void face_state_cot(const MhdAdjArgs& J, const PrimM&
     }
 }
 
-constexpr int kFaceW = 60;  // strip width of the face march (lanes 2..61 own its columns)
+constexpr int kFaceW = 58;  // strip width of the face march (lanes 3..60 own its columns)
 
 // Rows of a strip of the face march in LDS: the primitive rows p-1 and p
 // (slot row & 1; row p+1 replaces row p-1 once the y-face has read it), the
@@ -852,16 +879,22 @@ struct MhdFaceLds {
 };
 
 // The face march: one wave per strip of kFaceW columns and rows [j0, j1).
-// Lane l holds column c = cbase - 2 + l. Pass p loads row p+1 (through the
-// boundary map), reverses the flux of the y-face between rows p-1 and p
-// (lanes 2..61 of the grid's columns), then of the x-face c + 1/2 of row p
-// (lanes 1..60 from face -1/2 on; lane 61 for the east boundary face);
-// passes run from j0 to j1 - 1, and to ny on the top strip (its y-face
-// ny - 1/2). Every face is reversed by one lane of one wave; its 14
-// face-state cotangents go to the face buffers. Each cell is loaded and
-// converted once per strip; the face states come from the LDS rows and lane
-// shifts, and nothing stays in registers across a face reversal (the empty
-// asm statements keep the compiler from forwarding LDS values across one).
+// Lane l holds column c = cbase - 3 + l (cbase = kFaceW tx); lanes 3..60 own
+// the strip's columns. Pass p loads row p+1 (through the boundary map) and
+// reverses the flux of
+//  * the y-face between rows p-1 and p (owned columns): its 14 face-state
+//    cotangents go to the y-face buffer, for the gather march;
+//  * the x-face c + 1/2 of row p (lanes 1..61; a face outside -1/2 .. nx-1/2
+//    has none): the cotangents of its two states are spread over their
+//    minmod stencils along the row at once, and the x-gather of each owned
+//    cell (its own share and its neighbours' shares, by lane shifts) goes to
+//    the cell's x-gather planes; the last strip writes the gathers of the
+//    ghost columns nx, nx+1 to their side buffer, the first those of -2, -1.
+// Passes run from j0 to j1 - 1, and to ny on the top strip (its y-face
+// ny - 1/2). Each cell is loaded and converted once per strip; the face
+// states come from the LDS rows and lane shifts, and nothing stays in
+// registers across a face reversal (the empty asm statements keep the
+// compiler from forwarding LDS values across one).
 template <int ORDER, int METOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_face_march_kernel(
     const MhdAdjArgs J) {
@@ -870,19 +903,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
     const int nx = J.nx, ny = J.ny;
-    const int c = tx * kFaceW - 2 + lane;
+    const int c = tx * kFaceW - 3 + lane;
     const int j0 = ty * J.rows, j1 = min(j0 + J.rows, ny);
     const int pe = j1 == ny ? ny : j1 - 1;  // last pass
     const bool col_in = c >= 0 && c < nx;
-    const bool own_y = col_in && lane >= 2 && lane <= 61;
-    const bool own_x = (lane >= 1 && lane <= 60 && c >= -1 && c <= nx - 1) || (lane == 61 && c == nx - 1);
+    const bool own = col_in && lane >= 3 && lane <= 60;
+    const bool face_x = c >= -1 && c <= nx - 1;  // face c + 1/2 exists
     const bool wall_x = c == -1 && J.bc[0] == GD2_BC_REFLECTIVE;
+    // ghost column of this lane whose x-gather this wave writes (0..3 for
+    // -2, -1, nx, nx+1; -1 otherwise)
+    int gslot = -1;
+    if (c == -2 || c == -1) gslot = c + 2;
+    else if ((c == nx || c == nx + 1) && tx == J.tiles_x - 1) gslot = c - nx + 2;
     bool fx;
     const int ci = map_index(c, nx, J.bc[0], J.bc[1], fx);
     const unsigned oc = (unsigned)ci * 8u;
-    const unsigned ov = col_in ? (unsigned)c * 8u : kOffNone;       // V of this column
-    const unsigned oy = own_y ? (unsigned)c * 8u : kOffNone;        // its y-face
-    const unsigned ox = own_x ? (unsigned)(c + 1) * 8u : kOffNone;  // its x-face
+    const unsigned ov = col_in ? (unsigned)c * 8u : kOffNone;  // V of this column
+    const unsigned oo = own ? (unsigned)c * 8u : kOffNone;     // this cell's y-face and x-gather
     const long long P = J.plane;
 
     auto load_raw = [&](int jl, double (&w)[7]) {
@@ -912,6 +949,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
 #pragma unroll
         for (int f = 0; f < 7; ++f) v[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * P)), o);
     };
+    // the row's primitives at this lane and its two neighbours
+    auto row_window = [&](int p, PrimM& qw, PrimM& q, PrimM& qe) {
+        // the lane index taken afresh: LDS addresses are formed here, not
+        // held across the face reversals
+        int e = lane;
+        asm volatile("" : "+v"(e));
+        q = lds_get_m(lds.q, p & 1, e);
+        qw.r = from_prev_lane(q.r); qe.r = from_next_lane(q.r);
+        qw.u = from_prev_lane(q.u); qe.u = from_next_lane(q.u);
+        qw.v = from_prev_lane(q.v); qe.v = from_next_lane(q.v);
+        qw.p = from_prev_lane(q.p); qe.p = from_next_lane(q.p);
+        qw.bx = from_prev_lane(q.bx); qe.bx = from_next_lane(q.bx);
+        qw.by = from_prev_lane(q.by); qe.by = from_next_lane(q.by);
+        qw.bz = from_prev_lane(q.bz); qe.bz = from_next_lane(q.bz);
+    };
 
     // prologue: rows j0-1 and j0, the upper-face value of row j0-1 and the
     // cotangent row j0-1
@@ -935,8 +987,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
             double w[7], vp[7];
             const bool fn = load_raw(p + 1, w);
             load_v(p, vp);
-            // the lane index taken afresh: LDS addresses are formed per pass,
-            // not held across the face reversals
             int e = lane;
             asm volatile("" : "+v"(e));
             const PrimM qs = lds_get_m(lds.q, (p - 1) & 1, e), q = lds_get_m(lds.q, p & 1, e);
@@ -958,44 +1008,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
             face_state_cot<METOD, false>(J, L, R, phi, wall, o);
             double* wy = J.wy + (long long)p * nx;
 #pragma unroll
-            for (int k = 0; k < 14; ++k) st_buf(buf_rsrc(uniform_ptr(wy + k * J.nfy)), oy, o[k]);
+            for (int k = 0; k < 14; ++k) st_buf(buf_rsrc(uniform_ptr(wy + k * J.nfy)), oo, o[k]);
         }
         asm volatile("" ::: "memory");
         if (p < j1) {
             // x-face c + 1/2 of row p: states from the row's lane shifts,
             // cotangent from this column's and the next column's V
-            int e = lane;
-            asm volatile("" : "+v"(e));
-            const PrimM q = lds_get_m(lds.q, p & 1, e);
-            PrimM qw, qe;
-            qw.r = from_prev_lane(q.r); qe.r = from_next_lane(q.r);
-            qw.u = from_prev_lane(q.u); qe.u = from_next_lane(q.u);
-            qw.v = from_prev_lane(q.v); qe.v = from_next_lane(q.v);
-            qw.p = from_prev_lane(q.p); qe.p = from_next_lane(q.p);
-            qw.bx = from_prev_lane(q.bx); qe.bx = from_next_lane(q.bx);
-            qw.by = from_prev_lane(q.by); qe.by = from_next_lane(q.by);
-            qw.bz = from_prev_lane(q.bz); qe.bz = from_next_lane(q.bz);
-            const PrimM s = slope3m<ORDER>(qw, q, qe);
-            const PrimM L = face_m(q, s, 0.5), Rm = face_m(q, s, -0.5);
-            PrimM R;
-            R.r = from_next_lane(Rm.r);
-            R.u = from_next_lane(Rm.u);
-            R.v = from_next_lane(Rm.v);
-            R.p = from_next_lane(Rm.p);
-            R.bx = from_next_lane(Rm.bx);
-            R.by = from_next_lane(Rm.by);
-            R.bz = from_next_lane(Rm.bz);
-            double phi[7];
+            double xdqp[7], xdqm[7];
+            {
+                PrimM qw, q, qe;
+                row_window(p, qw, q, qe);
+                const PrimM s = slope3m<ORDER>(qw, q, qe);
+                const PrimM L = face_m(q, s, 0.5), Rm = face_m(q, s, -0.5);
+                PrimM R;
+                R.r = from_next_lane(Rm.r);
+                R.u = from_next_lane(Rm.u);
+                R.v = from_next_lane(Rm.v);
+                R.p = from_next_lane(Rm.p);
+                R.bx = from_next_lane(Rm.bx);
+                R.by = from_next_lane(Rm.by);
+                R.bz = from_next_lane(Rm.bz);
+                int e = lane;
+                asm volatile("" : "+v"(e));
+                double phi[7];
 #pragma unroll
-            for (int f = 0; f < 7; ++f) {
-                const double vl = lds.v[0][f][e];
-                phi[f] = J.hfx * (from_next_lane(vl) - vl);
+                for (int f = 0; f < 7; ++f) {
+                    const double vl = lds.v[0][f][e];
+                    phi[f] = J.hfx * (from_next_lane(vl) - vl);
+                }
+                double o[14];
+                face_state_cot<METOD, true>(J, L, R, phi, wall_x, o);
+                // the left state's cotangent stays with this cell (its plus
+                // face), the right state's goes to the next cell (its minus face)
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    xdqp[k] = face_x ? o[k] : 0.0;
+                    xdqm[k] = from_prev_lane(face_x ? o[7 + k] : 0.0);
+                }
             }
-            double o[14];
-            face_state_cot<METOD, true>(J, L, R, phi, wall_x, o);
-            double* wx = J.wx + (long long)p * (nx + 1);
+            asm volatile("" ::: "memory");
+            PrimM qw, q, qe;
+            row_window(p, qw, q, qe);
+            double xown[7], xtp[7], xtn[7], Gx[7];
+            spread7<ORDER>(qw, q, qe, xdqp, xdqm, xown, xtp, xtn);
 #pragma unroll
-            for (int k = 0; k < 14; ++k) st_buf(buf_rsrc(uniform_ptr(wx + k * J.nfx)), ox, o[k]);
+            for (int k = 0; k < 7; ++k) Gx[k] = (from_prev_lane(xtn[k]) + xown[k]) + from_next_lane(xtp[k]);
+            double* gc = J.cx + (long long)p * nx;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) st_buf(buf_rsrc(uniform_ptr(gc + k * P)), oo, Gx[k]);
+            if (gslot >= 0) {
+                double* g = J.gx + ((long long)gslot * ny + p) * 7;
+#pragma unroll
+                for (int k = 0; k < 7; ++k) g[k] = Gx[k];
+            }
         }
     }
 }
@@ -1008,33 +1073,6 @@ __device__ __forceinline__ double powell_k(const MhdAdjArgs& J, const PrimM& q, 
                  ve = J.V[6 * P + k];
     const double K = (vmx * q.bx + vmy * q.by + ve * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) + vbx * q.u + vby * q.v;
     return -K * J.idV;
-}
-
-// One cell's shares of the reconstruction adjoint along one line: qa, qb, qd
-// the cell before, the cell itself and the cell after; dqp the cotangent of
-// its plus-face value (left state of the face after it), dqm of its
-// minus-face value (right state of the face before it).
-template <int ORDER>
-__device__ __forceinline__ void spread7(const PrimM& qa, const PrimM& qb, const PrimM& qd, const double (&dqp)[7],
-                                        const double (&dqm)[7], double (&own)[7], double (&to_prev)[7],
-                                        double (&to_next)[7]) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        if constexpr (ORDER == 1) {
-            own[k] = dqp[k] + dqm[k];
-            to_prev[k] = to_next[k] = 0.0;
-        } else {
-            const double a = comp_m(qa, k), b = comp_m(qb, k), d = comp_m(qd, k);
-            const MMW wt = mm_weights(b - a, d - b);
-            bool fp = false, fm = false;
-            if (k == 0 || k == 3) {
-                const double sl = minmod(b - a, d - b);
-                fp = !(b + 0.5 * sl > 0.0);
-                fm = !(b + -0.5 * sl > 0.0);
-            }
-            spread(dqp[k], dqm[k], wt, fp, fm, own[k], to_prev[k], to_next[k]);
-        }
-    }
 }
 
 // primitive -> conserved cotangent at a cell:
@@ -1056,14 +1094,13 @@ constexpr int kGatherW = 60;  // output columns per wave of the gather march
 
 // The gather march. Lane l holds column c = cbase - 2 + l (cbase = 60 tx);
 // lanes 2..61 own the strip's cells. Per row every lane loads its cell
-// (through the boundary map) and the two x-face-state cotangents it owns
-// (left state of face c+1/2, right state of face c-1/2), spreads them, and a
-// cell's x-gather is lane l-1's share to its next cell + its own share +
-// lane l+1's share to its previous cell. Along y the shares are carried: row
-// j is finished once the spread of row j+1 exists. The strip at the bottom
-// (top) of the grid also marches the ghost rows -2, -1 (ny, ny+1), and the
-// strips at the sides hold the ghost columns in lanes: their gathers go to
-// the side buffers gx / gy, each written by one wave.
+// (through the boundary map) and the two y-face-state cotangents it owns
+// (left state of face j+1/2, right state of face j-1/2) and spreads them
+// along its column; the shares are carried, so row j is finished once the
+// spread of row j+1 exists. A cell's cotangent is its x-gather (from the face
+// march) + its y-gather + the Powell terms, mapped to conserved variables.
+// The strip at the bottom (top) of the grid also marches the ghost rows -2,
+// -1 (ny, ny+1): their y-gathers go to the side buffer gy.
 template <int ORDER>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mhd_vjp_gather_kernel(const MhdAdjArgs J) {
     const int lane = threadIdx.x;
@@ -1077,20 +1114,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     const int jb = j1 == ny ? ny + 1 : j1 - 1;  // last
     const bool col_in = c >= 0 && c < nx;
     const bool out_lane = col_in && lane >= 2 && lane <= 61;
-    // ghost column of this lane (0..3 for -2, -1, nx, nx+1; -1 otherwise),
-    // written by the one wave that holds the ghost on a lane where its
-    // gather is complete (nx needs lane l+1)
-    int gslot = -1;
-    if (c == -2 || c == -1) gslot = c + 2;
-    else if (c == nx && lane <= 62) gslot = 2;
-    else if (c == nx + 1) gslot = 3;
     bool fx;
     const int ci = map_index(c, nx, J.bc[0], J.bc[1], fx);
     const unsigned oc = (unsigned)ci * 8u;
-    const unsigned ocol = col_in ? (unsigned)c * 8u : kOffNone;  // y-faces and V of this column
-    // x-faces: left state of face c+1/2 (id c+1), right state of face c-1/2 (id c)
-    const unsigned oxl = (c + 1 >= 0 && c + 1 <= nx) ? (unsigned)(c + 1) * 8u : kOffNone;
-    const unsigned oxr = (c >= 0 && c <= nx) ? (unsigned)c * 8u : kOffNone;
+    const unsigned ocol = col_in ? (unsigned)c * 8u : kOffNone;  // y-faces, x-gather and V of this column
     const long long P = J.plane;
     const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
 
@@ -1170,34 +1197,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         for (int k = 0; k < 7; ++k) Gy[k] = (tn_prev[k] + yown[k]) + tp1[k];
 
         if (j >= 0 && j < ny) {
-            // x: spreads of the row, gathers by lane shifts (all lanes)
-            double xdqp[7], xdqm[7];
+            // x: the cell's x-gather (face march) and the neighbours' bx and
+            // Powell products by lane shifts
+            double Gx[7];
             {
-                const double* w = J.wx + (long long)j * (nx + 1);
+                const double* g = J.cx + (long long)j * nx;
 #pragma unroll
-                for (int k = 0; k < 7; ++k) {
-                    xdqp[k] = ld_buf(buf_rsrc(uniform_ptr(w + k * J.nfx)), oxl);
-                    xdqm[k] = ld_buf(buf_rsrc(uniform_ptr(w + (7 + k) * J.nfx)), oxr);
-                }
+                for (int k = 0; k < 7; ++k) Gx[k] = ld_buf(buf_rsrc(uniform_ptr(g + k * P)), ocol);
             }
-            PrimM qw, qe;
-            qw.r = from_prev_lane(q0.r); qe.r = from_next_lane(q0.r);
-            qw.u = from_prev_lane(q0.u); qe.u = from_next_lane(q0.u);
-            qw.v = from_prev_lane(q0.v); qe.v = from_next_lane(q0.v);
-            qw.p = from_prev_lane(q0.p); qe.p = from_next_lane(q0.p);
-            qw.bx = from_prev_lane(q0.bx); qe.bx = from_next_lane(q0.bx);
-            qw.by = from_prev_lane(q0.by); qe.by = from_next_lane(q0.by);
-            qw.bz = from_prev_lane(q0.bz); qe.bz = from_next_lane(q0.bz);
-            double xown[7], xtp[7], xtn[7], Gx[7];
-            spread7<ORDER>(qw, q0, qe, xdqp, xdqm, xown, xtp, xtn);
-#pragma unroll
-            for (int k = 0; k < 7; ++k) Gx[k] = (from_prev_lane(xtn[k]) + xown[k]) + from_next_lane(xtp[k]);
+            const double bxw = from_prev_lane(q0.bx), bxe = from_next_lane(q0.bx);
             const double Kw = from_prev_lane(K0), Ke = from_next_lane(K0);
-            if (gslot >= 0) {
-                double* g = J.gx + ((long long)gslot * ny + j) * 7;
-#pragma unroll
-                for (int k = 0; k < 7; ++k) g[k] = Gx[k];
-            }
             if (out_lane) {
                 double Pq[7];
 #pragma unroll
@@ -1207,7 +1216,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
                 // Pdiv = dy (bx_e - bx_w)/2 + dx (by_n - by_s)/2 of the
                 // neighbours (ghosts through the boundary map)
                 const PrimM& q = q0;
-                const double pdiv = J.dy * (qe.bx - qw.bx) * 0.5 + J.dx * (qp.by - qm.by) * 0.5;
+                const double pdiv = J.dy * (bxe - bxw) * 0.5 + J.dx * (qp.by - qm.by) * 0.5;
                 {
                     const double vmx = v0[0], vmy = v0[1], vbx = v0[2], vby = v0[3], ve = v0[4];
                     const double sc = -pdiv * J.idV;
