@@ -329,7 +329,7 @@ int gd2_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, cons
  * GD2_NOALLOC_SCRATCH_BYTES bytes inside the library (one per device, held
  * until the process ends), looked up on the first call on each device.
  * Returns GD2_ERR_TOO_LARGE when gd2_vjp_workspace_bytes(g, sc) exceeds it:
- * Euler up to max(nx, ny) = 131072, MHD up to about 150 000 cells. Every call
+ * Euler up to max(nx, ny) = 131072, MHD up to about 200 000 cells. Every call
  * on one device shares that device's array, so calls that overlap on one
  * device (concurrent streams or host threads) race; calls in one stream are
  * safe, since each call writes every entry it reads. Results are those of

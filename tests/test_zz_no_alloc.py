@@ -94,7 +94,7 @@ def test_noalloc_vjp_refuses_sizes_beyond_the_scratch_array():
     assert api.vjp_workspace_bytes(big_e, euler) > api.NOALLOC_SCRATCH_BYTES
     assert L.gd2_stage_vjp_noalloc(ctypes.byref(big_e), ctypes.byref(euler), x, x, None, 0.0, None, 0.0, 1.0, y,
                                    None) == api.ERR_TOO_LARGE
-    big_m = api.make_grid(400, 400, gamma=5.0 / 3.0)
+    big_m = api.make_grid(460, 460, gamma=5.0 / 3.0)
     assert api.vjp_workspace_bytes(big_m, mhd) > api.NOALLOC_SCRATCH_BYTES
     assert L.gd2_stage_vjp_noalloc(ctypes.byref(big_m), ctypes.byref(mhd), x, x, None, 0.0, None, 0.0, 1.0, y,
                                    None) == api.ERR_TOO_LARGE
