@@ -1121,13 +1121,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     const long long P = J.plane;
     const double hx = 0.5 * J.dy, hy = 0.5 * J.dx;
 
-    auto load_q = [&](int jl) -> PrimM {
+    auto load_q_raw = [&](int jl, double (&w)[7]) -> bool {
         bool fy;
         const int jj = map_index(jl, ny, J.bc[2], J.bc[3], fy);
         const double* b = J.U + (long long)jj * nx;
-        double w[7];
 #pragma unroll
         for (int f = 0; f < 7; ++f) w[f] = ld_buf(buf_rsrc(uniform_ptr(b + f * P)), oc);
+        return fy;
+    };
+    auto to_prim_q = [&](const double (&w)[7], bool fy) -> PrimM {
         PrimM q = prim_of_cons7(w, J.g1);
         if (fx) {
             q.u = -q.u;
@@ -1138,6 +1140,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
             q.by = -q.by;
         }
         return q;
+    };
+    auto load_q = [&](int jl) -> PrimM {
+        double w[7];
+        const bool fy = load_q_raw(jl, w);
+        return to_prim_q(w, fy);
     };
     // y-face-state cotangents of the cell of row r: left state of face r+1/2
     // (face row r+1), right state of face r-1/2 (face row r); 0 outside
@@ -1154,7 +1161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     };
     // K . V of the Powell term of this lane's cell in row r (powell_k), 0
     // outside the grid; V of the cell in vv (mx, my, bx, by, E components)
-    auto powell_row = [&](int r, const PrimM& q, double (&vv)[5]) -> double {
+    auto load_v_row = [&](int r, double (&vv)[5]) {
         const bool in = r >= 0 && r < ny;
         const double* b = J.V + (long long)(in ? r : 0) * nx;
         const unsigned o = in ? ocol : kOffNone;
@@ -1163,9 +1170,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         vv[2] = ld_buf(buf_rsrc(uniform_ptr(b + 3 * P)), o);
         vv[3] = ld_buf(buf_rsrc(uniform_ptr(b + 4 * P)), o);
         vv[4] = ld_buf(buf_rsrc(uniform_ptr(b + 6 * P)), o);
+    };
+    auto powell_k_row = [&](int r, const PrimM& q, const double (&vv)[5]) -> double {
+        const bool in = r >= 0 && r < ny;
         const double K = (vv[0] * q.bx + vv[1] * q.by + vv[4] * (q.u * q.bx + q.v * q.by)) * (1.0 / kCpi4) +
                          vv[2] * q.u + vv[3] * q.v;
         return (in && col_in) ? -K * J.idV : 0.0;
+    };
+    auto powell_row = [&](int r, const PrimM& q, double (&vv)[5]) -> double {
+        load_v_row(r, vv);
+        return powell_k_row(r, q, vv);
     };
 
     // prologue: rows ja-2 .. ja+1; y-spreads of rows ja-1 (its share to row
@@ -1185,26 +1199,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     double K0 = powell_row(ja, q0, v0);
 
     for (int j = ja; j <= jb; ++j) {
-        // row j+1's y-spread (rows j .. j+2) and Powell product
-        const PrimM qpp = load_q(j + 2);
-        double vp[5];
-        const double Kp = powell_row(j + 1, qp, vp);
-        double ydqp[7], ydqm[7], own1[7], tp1[7], tn1[7];
+        // every load of the pass is issued here, before any of them is
+        // used: row j+2's state, row j+1's cotangent and y-face-state
+        // cotangents (used first), then the cell's x-gather and the affine
+        // terms of row j (0 off the grid's rows and owned cells), still in
+        // flight while row j+1 is spread
+        double w2[7], vp[5], ydqp[7], ydqm[7];
+        const bool fy2 = load_q_raw(j + 2, w2);
+        load_v_row(j + 1, vp);
         load_yfaces(j + 1, ydqp, ydqm);
+        const bool in_row = j >= 0 && j < ny;
+        const unsigned orow = in_row ? ocol : kOffNone;
+        const unsigned oown = (in_row && out_lane) ? (unsigned)c * 8u : kOffNone;
+        const long long kr = (long long)(in_row ? j : 0) * nx;
+        // absent affine terms: the loads are still issued (no branch
+        // between the loads), with an offset that reads nothing
+        const double* Ab = J.A ? J.A : J.cx;
+        const double* Bb = J.B ? J.B : J.cx;
+        const unsigned oa = J.A ? oown : kOffNone, ob = J.B ? oown : kOffNone;
+        double Gx[7], Av[7], Bv[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            Gx[k] = ld_buf(buf_rsrc(uniform_ptr(J.cx + kr + k * P)), orow);
+            Av[k] = ld_buf(buf_rsrc(uniform_ptr(Ab + kr + k * P)), oa);
+            Bv[k] = ld_buf(buf_rsrc(uniform_ptr(Bb + kr + k * P)), ob);
+        }
+        // row j+1's y-spread (rows j .. j+2) and Powell product
+        const PrimM qpp = to_prim_q(w2, fy2);
+        const double Kp = powell_k_row(j + 1, qp, vp);
+        double own1[7], tp1[7], tn1[7];
         spread7<ORDER>(q0, qp, qpp, ydqp, ydqm, own1, tp1, tn1);
         double Gy[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) Gy[k] = (tn_prev[k] + yown[k]) + tp1[k];
 
         if (j >= 0 && j < ny) {
-            // x: the cell's x-gather (face march) and the neighbours' bx and
-            // Powell products by lane shifts
-            double Gx[7];
-            {
-                const double* g = J.cx + (long long)j * nx;
-#pragma unroll
-                for (int k = 0; k < 7; ++k) Gx[k] = ld_buf(buf_rsrc(uniform_ptr(g + k * P)), ocol);
-            }
+            // x: the cell's x-gather (face march, in Gx) and the neighbours'
+            // bx and Powell products by lane shifts
             const double bxw = from_prev_lane(q0.bx), bxe = from_next_lane(q0.bx);
             const double Kw = from_prev_lane(K0), Ke = from_next_lane(K0);
             if (out_lane) {
@@ -1234,13 +1265,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
                 Pq[5] += hy * Km;
                 double T[7];
                 prim_to_cons_cot(J, q, Pq, T);
-                const long long k = (long long)j * nx + c;
 #pragma unroll
                 for (int f = 0; f < 7; ++f) {
                     double o = J.c * T[f];
-                    if (J.A) o += J.a * J.A[f * P + k];
-                    if (J.B) o += J.b * J.B[f * P + k];
-                    J.W[f * P + k] = o;
+                    if (J.A) o += J.a * Av[f];
+                    if (J.B) o += J.b * Bv[f];
+                    st_buf(buf_rsrc(uniform_ptr(J.W + kr + f * P)), oown, o);
                 }
             }
         } else if (col_in) {
