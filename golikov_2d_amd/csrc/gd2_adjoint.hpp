@@ -323,7 +323,7 @@ __device__ __forceinline__ Q4 q4_flip_u(const Q4& x, bool f) {
 }
 
 template <int ORDER>
-__global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void euler_vjp_kernel(const AdjArgs A) {
     __shared__ double ring[3][8][kAdjBuf];
     __shared__ double gyb[2][4][kWave];  // ghost-row cotangents (layer 0 next to the edge)
     const int lane = threadIdx.x;
@@ -512,6 +512,26 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
                 xpass(std::true_type{});
         }
 
+        // the affine terms of row j-1 (finished at the end of this pass),
+        // issued before the y-face (in flight during its reversal); an
+        // offset that reads nothing where this lane stores nothing or the
+        // term is absent
+        double Aw[4], Bw[4];
+        {
+            const int r = j - 1;
+            const bool row_out = r >= ro0 && r < ro1 && r >= 0 && r < ny;  // wave-uniform
+            const bool st = row_out && col_real && interior_lane;
+            const long long rowk = (long long)(row_out ? r : 0) * nx;
+            const unsigned oa = (st && A.A) ? (unsigned)c * 8u : kOffNone;
+            const unsigned ob = (st && A.B) ? (unsigned)c * 8u : kOffNone;
+            const double* Ab = A.A ? A.A : A.U;
+            const double* Bb = A.B ? A.B : A.U;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                Aw[f] = ld_buf(buf_rsrc(uniform_ptr(Ab + f * P + rowk)), oa);
+                Bw[f] = ld_buf(buf_rsrc(uniform_ptr(Bb + f * P + rowk)), ob);
+            }
+        }
         bool pd;
         qd = finish(nr, ((j + 2) % 3 + 3) % 3, true, Vd, pd);  // q_{j+2}
         const double Vn[4] = {Vd[0], Vd[1], Vd[2], Vd[3]};
@@ -565,8 +585,8 @@ __global__ __launch_bounds__(64) void euler_vjp_kernel(const AdjArgs A) {
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
                         double w = A.c * du[f];
-                        if (A.A) w += A.a * ld_buf(buf_rsrc(uniform_ptr(A.A + f * P + rowk)), o);
-                        if (A.B) w += A.b * ld_buf(buf_rsrc(uniform_ptr(A.B + f * P + rowk)), o);
+                        if (A.A) w += A.a * Aw[f];
+                        if (A.B) w += A.b * Bw[f];
                         st_buf(buf_rsrc(uniform_ptr(A.W + f * P + rowk)), o, w);
                     }
                 }
