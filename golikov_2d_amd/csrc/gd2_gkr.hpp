@@ -33,6 +33,7 @@ struct GkrArgs {
     const double* T_do;
     int nx, ny;
     double dx, dy, x0, y0, gamma, inv_g1, krit, r_frozen;
+    double idV;  // 1 / (dx dy)
     DeviceDiag* diag;
 };
 
@@ -128,14 +129,16 @@ struct GkrStencil {
     }
 };
 
-// h * minmod((hi - mid)/h, (mid - lo)/h) per component; the vy slot takes the
-// pressure slope, as the reference writes it (kernel.cu:2965-3038)
-__device__ __forceinline__ Q7 gkr_slope(const Q7& hi, const Q7& mid, const Q7& lo, double h) {
+// h * minmod((hi - mid)/h, (mid - lo)/h) per component (kernel.cu:2965-3038),
+// formed as minmod(hi - mid, mid - lo): the same choice for h > 0, without
+// the two divisions and the product (rounding-level); the vy slot takes the
+// pressure slope, as the reference writes it
+__device__ __forceinline__ Q7 gkr_slope(const Q7& hi, const Q7& mid, const Q7& lo, double) {
     Q7 d;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         const int c = k == 3 ? 1 : k;
-        d.v[k] = h * minmod((hi.v[c] - mid.v[c]) / h, (mid.v[c] - lo.v[c]) / h);
+        d.v[k] = minmod(hi.v[c] - mid.v[c], mid.v[c] - lo.v[c]);
     }
     return d;
 }
@@ -143,10 +146,9 @@ __device__ __forceinline__ Q7 gkr_slope(const Q7& hi, const Q7& mid, const Q7& l
 // POTOK_Korolkov (Gas_Dinamic.cu:70-181) with vz = 0 and an in-plane axis
 // normal (n1, n2): the physical MHD flux in the lab frame, P = (F_rho, F_mx,
 // F_my, F_mz, F_Bx, F_By, F_Bz, F_E).
-__device__ __forceinline__ void gkr_potok(double gamma, double ro, double p, double vx, double vy, double Bx,
+__device__ __forceinline__ void gkr_potok(double inv_g1, double ro, double p, double vx, double vy, double Bx,
                                           double By, double Bz, double n1, double n2, double (&P)[8]) {
-    const double g1 = gamma - 1.0;
-    const double bx = Bx / kSpi4, by = By / kSpi4, bz = Bz / kSpi4;
+    const double bx = Bx * (1.0 / kSpi4), by = By * (1.0 / kSpi4), bz = Bz * (1.0 / kSpi4);
     double t1 = 0, t2 = 0, t3 = 0, m1 = 0, m2 = 0, m3 = 0;
     if (n1 > 0.1) {
         t2 = 1.0;
@@ -170,7 +172,7 @@ __device__ __forceinline__ void gkr_potok(double gamma, double ro, double p, dou
     const double bb = bx * bx + by * by + bz * bz;
     const double uu = (vx * vx + vy * vy) / 2.0;
     const double pT = p + bb / 2.0;
-    const double e = p / g1 + ro * uu + bb / 2.0;
+    const double e = p * inv_g1 + ro * uu + bb / 2.0;
     const double f0 = ro * u1;
     const double f1 = ro * u1 * u1 + pT - bn * bn;
     const double f2 = ro * u1 * v1 - bn * bt;
@@ -188,10 +190,10 @@ __device__ __forceinline__ void gkr_potok(double gamma, double ro, double p, dou
     P[7] = f4;
 }
 
-__device__ __forceinline__ double gkr_u8(double gamma, double ro, double p, double u, double v, double bx, double by,
+__device__ __forceinline__ double gkr_u8(double inv_g1, double ro, double p, double u, double v, double bx, double by,
                                          double bz) {
     // Header.h:16 with w = 0
-    return p / (gamma - 1.0) + 0.5 * ro * (u * u + v * v) + (bx * bx + by * by + bz * bz) / kCpi8;
+    return p * inv_g1 + 0.5 * ro * (u * u + v * v) + (bx * bx + by * by + bz * bz) * (1.0 / kCpi8);
 }
 
 // predictor (kernel.cu:1611-1720): a step of st from the physical fluxes of
@@ -214,7 +216,7 @@ __device__ inline Q7 gkr_predictor(const GkrArgs& A, const Q7& Q, const Q7& DX, 
         if (q[0] <= 0.0) q[0] = Q.v[0];
         if (q[1] <= 0.0) q[1] = Q.v[1];
         double P[8];
-        gkr_potok(A.gamma, q[0], q[1], q[2], q[3], q[4], q[5], q[6], n1, n2, P);
+        gkr_potok(A.inv_g1, q[0], q[1], q[2], q[3], q[4], q[5], q[6], n1, n2, P);
         PSx = PSx + P[0] * len;
         PSy = PSy + P[7] * len;
         PUx = PUx + P[1] * len;
@@ -223,26 +225,28 @@ __device__ inline Q7 gkr_predictor(const GkrArgs& A, const Q7& Q, const Q7& DX, 
         PBy = PBy + P[5] * len;
         PBz = PBz + P[6] * len;
     }
-    const double dV = dx * dy;
+    // divisions by dV, y and r as products with their reciprocals
+    const double sV = st * A.idV, sy = st * fast_rcp(y);
     const double* q = Q.v;
     Q7 QQ;
-    double r = q[0] - st * PSx / dV - st * q[0] * q[3] / y;
+    double r = q[0] - sV * PSx - sy * q[0] * q[3];
     if (r <= 0) {
         if (A.diag) atomicAdd(&A.diag->clamp_rho, 1ull);
         r = q[0];
     }
+    const double ir = fast_rcp(r);
     QQ.v[0] = r;
-    QQ.v[2] = (q[0] * q[2] - st * (PUx) / dV - st * (q[0] * q[2] * q[3] - q[4] * q[5] / kCpi4) / y) / r;
-    QQ.v[3] = (q[0] * q[3] - st * (PUy) / dV - st * (q[0] * q[3] * q[3] + (q[6] * q[6] - q[5] * q[5]) / kCpi4) / y) / r;
-    QQ.v[4] = (q[4] - st * (PBx) / dV - st * (q[3] * q[4] - q[5] * q[2]) / y);
-    QQ.v[5] = (q[5] - st * (PBy) / dV);
-    QQ.v[6] = (q[6] - st * (PBz) / dV);
-    const double u8 = gkr_u8(A.gamma, q[0], q[1], q[2], q[3], q[4], q[5], q[6]);
-    double p = (u8 - st * (PSy) / dV -
-                st * (((u8 + q[1] + (q[4] * q[4] + q[5] * q[5] + q[6] * q[6]) / kCpi8) * q[3] -
-                       q[5] * (q[2] * q[4] + q[3] * q[5]) / kCpi4) / y) -
+    QQ.v[2] = (q[0] * q[2] - sV * (PUx) - sy * (q[0] * q[2] * q[3] - q[4] * q[5] * (1.0 / kCpi4))) * ir;
+    QQ.v[3] = (q[0] * q[3] - sV * (PUy) - sy * (q[0] * q[3] * q[3] + (q[6] * q[6] - q[5] * q[5]) * (1.0 / kCpi4))) * ir;
+    QQ.v[4] = (q[4] - sV * (PBx) - sy * (q[3] * q[4] - q[5] * q[2]));
+    QQ.v[5] = (q[5] - sV * (PBy));
+    QQ.v[6] = (q[6] - sV * (PBz));
+    const double u8 = gkr_u8(A.inv_g1, q[0], q[1], q[2], q[3], q[4], q[5], q[6]);
+    double p = (u8 - sV * (PSy) -
+                sy * (((u8 + q[1] + (q[4] * q[4] + q[5] * q[5] + q[6] * q[6]) * (1.0 / kCpi8)) * q[3] -
+                       q[5] * (q[2] * q[4] + q[3] * q[5]) * (1.0 / kCpi4))) -
                 0.5 * r * (QQ.v[2] * QQ.v[2] + QQ.v[3] * QQ.v[3]) -
-                (QQ.v[4] * QQ.v[4] + QQ.v[5] * QQ.v[5] + QQ.v[6] * QQ.v[6]) / kCpi8) * (A.gamma - 1.0);
+                (QQ.v[4] * QQ.v[4] + QQ.v[5] * QQ.v[5] + QQ.v[6] * QQ.v[6]) * (1.0 / kCpi8)) * (A.gamma - 1.0);
     if (p <= 0) {
         if (A.diag) atomicAdd(&A.diag->clamp_p, 1ull);
         p = 0.000001;
@@ -272,8 +276,8 @@ __device__ __forceinline__ double gkr_face(const GkrArgs& A, const double (&L)[7
         t1 = -1.0;
         m3 = -1.0;
     }
-    const double bxL = L[4] / kSpi4, byL = L[5] / kSpi4, bzL = L[6] / kSpi4;
-    const double bxR = R[4] / kSpi4, byR = R[5] / kSpi4, bzR = R[6] / kSpi4;
+    const double bxL = L[4] * (1.0 / kSpi4), byL = L[5] * (1.0 / kSpi4), bzL = L[6] * (1.0 / kSpi4);
+    const double bxR = R[4] * (1.0 / kSpi4), byR = R[5] * (1.0 / kSpi4), bzR = R[6] * (1.0 / kSpi4);
     double smax;
     const FluxM PO = hlld_face<METOD>(
         L[0], L[1], L[2] * n1 + L[3] * n2, L[2] * t1 + L[3] * t2, L[2] * m1 + L[3] * m2, bxL * n1 + byL * n2,
@@ -371,25 +375,26 @@ __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
             }
             // kernel.cu:3247-3283: from the mean of the cell and its predictor,
             // dt/2, the axisymmetric sources, Powell's term zero
-            const double dV = dx * dy;
-            double r2 = Qb[0] - h * PSx / dV - h * Qb[0] * Qb[3] / y;
+            const double hV = h * A.idV, hy = h * fast_rcp(y);
+            double r2 = Qb[0] - hV * PSx - hy * Qb[0] * Qb[3];
             if (r2 <= 0) {
                 if (A.diag) atomicAdd(&A.diag->clamp_rho, 1ull);
                 r2 = Qb[0];
             }
+            const double ir2 = fast_rcp(r2);
             const double vx2 =
-                (Qb[0] * Qb[2] - h * (PUx) / dV - h * (Qb[0] * Qb[2] * Qb[3] - Qb[4] * Qb[5] / kCpi4) / y) / r2;
-            const double vy2 = (Qb[0] * Qb[3] - h * (PUy) / dV -
-                                h * (Qb[0] * Qb[3] * Qb[3] + (Qb[6] * Qb[6] - Qb[5] * Qb[5]) / kCpi4) / y) /
-                               r2;
-            const double bx2 = (Qb[4] - h * (PBx) / dV - h * (Qb[3] * Qb[4] - Qb[5] * Qb[2]) / y);
-            const double by2 = (Qb[5] - h * (PBy) / dV);
-            const double bz2 = (Qb[6] - h * (PBz) / dV);
-            const double u8 = gkr_u8(A.gamma, Qb[0], Qb[1], Qb[2], Qb[3], Qb[4], Qb[5], Qb[6]);
-            double p2 = (u8 - h * (PSy) / dV -
-                         h * (((u8 + Qb[1] + (Qb[4] * Qb[4] + Qb[5] * Qb[5] + Qb[6] * Qb[6]) / kCpi8) * Qb[3] -
-                               Qb[5] * (Qb[2] * Qb[4] + Qb[3] * Qb[5]) / kCpi4) / y) -
-                         0.5 * r2 * (vx2 * vx2 + vy2 * vy2) - (bx2 * bx2 + by2 * by2 + bz2 * bz2) / kCpi8) *
+                (Qb[0] * Qb[2] - hV * (PUx) - hy * (Qb[0] * Qb[2] * Qb[3] - Qb[4] * Qb[5] * (1.0 / kCpi4))) * ir2;
+            const double vy2 = (Qb[0] * Qb[3] - hV * (PUy) -
+                                hy * (Qb[0] * Qb[3] * Qb[3] + (Qb[6] * Qb[6] - Qb[5] * Qb[5]) * (1.0 / kCpi4))) *
+                               ir2;
+            const double bx2 = (Qb[4] - hV * (PBx) - hy * (Qb[3] * Qb[4] - Qb[5] * Qb[2]));
+            const double by2 = (Qb[5] - hV * (PBy));
+            const double bz2 = (Qb[6] - hV * (PBz));
+            const double u8 = gkr_u8(A.inv_g1, Qb[0], Qb[1], Qb[2], Qb[3], Qb[4], Qb[5], Qb[6]);
+            double p2 = (u8 - hV * (PSy) -
+                         hy * (((u8 + Qb[1] + (Qb[4] * Qb[4] + Qb[5] * Qb[5] + Qb[6] * Qb[6]) * (1.0 / kCpi8)) * Qb[3] -
+                               Qb[5] * (Qb[2] * Qb[4] + Qb[3] * Qb[5]) * (1.0 / kCpi4))) -
+                         0.5 * r2 * (vx2 * vx2 + vy2 * vy2) - (bx2 * bx2 + by2 * by2 + bz2 * bz2) * (1.0 / kCpi8)) *
                         (A.gamma - 1.0);
             if (p2 <= 0) {
                 if (A.diag) atomicAdd(&A.diag->clamp_p, 1ull);

@@ -285,6 +285,7 @@ int gd2_kernel_gkr(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
     A.y0 = g->y0;
     A.gamma = g->gamma;
     A.inv_g1 = 1.0 / (g->gamma - 1.0);
+    A.idV = 1.0 / (g->dx * g->dy);
     A.krit = g->krit;
     A.r_frozen = g->r_frozen;
     A.diag = device_diag();
