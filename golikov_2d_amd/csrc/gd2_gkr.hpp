@@ -312,36 +312,46 @@ __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
             const GkrStencil S{A, n, m};
             const double dx = A.dx, dy = A.dy;
             const double h = *A.T_do / 2.0;
-            const Q7 q1 = S.p1(), q2 = S.p2(), q3 = S.p3(), q4 = S.p4(), q5 = S.p5();
-            const Q7 DX1 = gkr_slope(q2, q1, q4, dx), DY1 = gkr_slope(q5, q1, q3, dy);
-            const Q7 QQ1 = gkr_predictor(A, q1, DX1, DY1, h, y);
+            Q7 DX1, DY1;
             double Qb[7];
+            {
+                const Q7 q1 = S.p1(), q2 = S.p2(), q3 = S.p3(), q4 = S.p4(), q5 = S.p5();
+                DX1 = gkr_slope(q2, q1, q4, dx);
+                DY1 = gkr_slope(q5, q1, q3, dy);
+                const Q7 QQ1 = gkr_predictor(A, q1, DX1, DY1, h, y);
 #pragma unroll
-            for (int k = 0; k < 7; ++k) Qb[k] = (q1.v[k] + QQ1.v[k]) / 2.0;
+                for (int k = 0; k < 7; ++k) Qb[k] = (q1.v[k] + QQ1.v[k]) / 2.0;
+            }
             double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
             // east, south, west, north faces (kernel.cu:3105-3238); the
-            // neighbour's slopes and predictor, its half-slope face state
+            // neighbour's slopes and predictor, its half-slope face state.
+            // The stencil points are read again in each face (cache hits) from
+            // indices passed through an empty asm, so that no point stays in
+            // registers across the faces.
 #pragma unroll 1
             for (int f = 0; f < 4; ++f) {
+                int nn = n, mm = m;
+                asm volatile("" : "+v"(nn), "+v"(mm));
+                const GkrStencil T{A, nn, mm};
                 Q7 qk, DXk, DYk;
                 double yk = y;
                 if (f == 0) {
-                    qk = q2;
-                    DXk = gkr_slope(S.p6(), q2, q1, dx);
-                    DYk = gkr_slope(S.p10(), q2, S.p11(), dy);
+                    qk = T.p2();
+                    DXk = gkr_slope(T.p6(), qk, T.p1(), dx);
+                    DYk = gkr_slope(T.p10(), qk, T.p11(), dy);
                 } else if (f == 1) {
-                    qk = q3;
-                    DXk = gkr_slope(S.p11(), q3, S.p12(), dx);
-                    DYk = gkr_slope(q1, q3, S.p7(), dy);
+                    qk = T.p3();
+                    DXk = gkr_slope(T.p11(), qk, T.p12(), dx);
+                    DYk = gkr_slope(T.p1(), qk, T.p7(), dy);
                     yk = y - dy;
                 } else if (f == 2) {
-                    qk = q4;
-                    DXk = gkr_slope(q1, q4, S.p8(), dx);
-                    DYk = gkr_slope(S.p13(), q4, S.p12(), dy);
+                    qk = T.p4();
+                    DXk = gkr_slope(T.p1(), qk, T.p8(), dx);
+                    DYk = gkr_slope(T.p13(), qk, T.p12(), dy);
                 } else {
-                    qk = q5;
-                    DXk = gkr_slope(S.p10(), q5, S.p13(), dx);
-                    DYk = gkr_slope(S.p9(), q5, q1, dy);
+                    qk = T.p5();
+                    DXk = gkr_slope(T.p10(), qk, T.p13(), dx);
+                    DYk = gkr_slope(T.p9(), qk, T.p1(), dy);
                     yk = y + dy;
                 }
                 const Q7 QQk = gkr_predictor(A, qk, DXk, DYk, h, yk);
