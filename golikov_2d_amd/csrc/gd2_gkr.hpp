@@ -33,7 +33,9 @@ struct GkrArgs {
     const double* T_do;
     int nx, ny;
     double dx, dy, x0, y0, gamma, inv_g1, krit, r_frozen;
-    double idV;  // 1 / (dx dy)
+    double idV;      // 1 / (dx dy)
+    int band_only;   // the per-cell kernel takes the boundary band, the march the interior
+    int tiles_x, rows;  // strips of the interior march
     DeviceDiag* diag;
 };
 
@@ -196,12 +198,18 @@ __device__ __forceinline__ double gkr_u8(double inv_g1, double ro, double p, dou
     return p * inv_g1 + 0.5 * ro * (u * u + v * v) + (bx * bx + by * by + bz * bz) * (1.0 / kCpi8);
 }
 
-// predictor (kernel.cu:1611-1720): a step of st from the physical fluxes of
-// the four half-slope face states (rho, p falling back to the cell value
-// when not positive) and the axisymmetric source terms at radius y.
-__device__ inline Q7 gkr_predictor(const GkrArgs& A, const Q7& Q, const Q7& DX, const Q7& DY, double st, double y) {
+// The seven flux sums of a cell: F_rho, F_E, F_mx, F_my, F_Bx, F_By, F_Bz
+// times the face lengths, summed over its faces.
+struct GkrSums {
+    double PSx, PSy, PUx, PUy, PBx, PBy, PBz;
+};
+
+// predictor (kernel.cu:1611-1720), its flux part: the physical fluxes of the
+// four half-slope face states (rho, p falling back to the cell value when not
+// positive), independent of the radius
+__device__ inline GkrSums gkr_pred_flux(const GkrArgs& A, const Q7& Q, const Q7& DX, const Q7& DY) {
     const double dx = A.dx, dy = A.dy;
-    double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
+    GkrSums S = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         // west (-DX/2, -x), east (+DX/2, +x), north (+DY/2, +y), south (-DY/2, -y)
@@ -217,41 +225,64 @@ __device__ inline Q7 gkr_predictor(const GkrArgs& A, const Q7& Q, const Q7& DX, 
         if (q[1] <= 0.0) q[1] = Q.v[1];
         double P[8];
         gkr_potok(A.inv_g1, q[0], q[1], q[2], q[3], q[4], q[5], q[6], n1, n2, P);
-        PSx = PSx + P[0] * len;
-        PSy = PSy + P[7] * len;
-        PUx = PUx + P[1] * len;
-        PUy = PUy + P[2] * len;
-        PBx = PBx + P[4] * len;
-        PBy = PBy + P[5] * len;
-        PBz = PBz + P[6] * len;
+        S.PSx = S.PSx + P[0] * len;
+        S.PSy = S.PSy + P[7] * len;
+        S.PUx = S.PUx + P[1] * len;
+        S.PUy = S.PUy + P[2] * len;
+        S.PBx = S.PBx + P[4] * len;
+        S.PBy = S.PBy + P[5] * len;
+        S.PBz = S.PBz + P[6] * len;
     }
-    // divisions by dV, y and r as products with their reciprocals
+    return S;
+}
+
+// The conserved update of kernel.cu:1690-1720 (predictor, st = dt/2 of the
+// state Q) and kernel.cu:3247-3283 (corrector, from the mean state): a step of
+// st from the flux sums with the axisymmetric source terms at radius y and
+// Powell's term zero; the density and pressure clamps set bits 0 and 1 of
+// `clamps`. Divisions by dV, y and rho as products with their reciprocals.
+__device__ inline Q7 gkr_update(const GkrArgs& A, const double (&q)[7], const GkrSums& S, double st, double y,
+                                unsigned& clamps) {
     const double sV = st * A.idV, sy = st * fast_rcp(y);
-    const double* q = Q.v;
     Q7 QQ;
-    double r = q[0] - sV * PSx - sy * q[0] * q[3];
+    double r = q[0] - sV * S.PSx - sy * q[0] * q[3];
     if (r <= 0) {
-        if (A.diag) atomicAdd(&A.diag->clamp_rho, 1ull);
+        clamps |= 1u;
         r = q[0];
     }
     const double ir = fast_rcp(r);
     QQ.v[0] = r;
-    QQ.v[2] = (q[0] * q[2] - sV * (PUx) - sy * (q[0] * q[2] * q[3] - q[4] * q[5] * (1.0 / kCpi4))) * ir;
-    QQ.v[3] = (q[0] * q[3] - sV * (PUy) - sy * (q[0] * q[3] * q[3] + (q[6] * q[6] - q[5] * q[5]) * (1.0 / kCpi4))) * ir;
-    QQ.v[4] = (q[4] - sV * (PBx) - sy * (q[3] * q[4] - q[5] * q[2]));
-    QQ.v[5] = (q[5] - sV * (PBy));
-    QQ.v[6] = (q[6] - sV * (PBz));
+    QQ.v[2] = (q[0] * q[2] - sV * (S.PUx) - sy * (q[0] * q[2] * q[3] - q[4] * q[5] * (1.0 / kCpi4))) * ir;
+    QQ.v[3] = (q[0] * q[3] - sV * (S.PUy) - sy * (q[0] * q[3] * q[3] + (q[6] * q[6] - q[5] * q[5]) * (1.0 / kCpi4))) * ir;
+    QQ.v[4] = (q[4] - sV * (S.PBx) - sy * (q[3] * q[4] - q[5] * q[2]));
+    QQ.v[5] = (q[5] - sV * (S.PBy));
+    QQ.v[6] = (q[6] - sV * (S.PBz));
     const double u8 = gkr_u8(A.inv_g1, q[0], q[1], q[2], q[3], q[4], q[5], q[6]);
-    double p = (u8 - sV * (PSy) -
+    double p = (u8 - sV * (S.PSy) -
                 sy * (((u8 + q[1] + (q[4] * q[4] + q[5] * q[5] + q[6] * q[6]) * (1.0 / kCpi8)) * q[3] -
                        q[5] * (q[2] * q[4] + q[3] * q[5]) * (1.0 / kCpi4))) -
                 0.5 * r * (QQ.v[2] * QQ.v[2] + QQ.v[3] * QQ.v[3]) -
                 (QQ.v[4] * QQ.v[4] + QQ.v[5] * QQ.v[5] + QQ.v[6] * QQ.v[6]) * (1.0 / kCpi8)) * (A.gamma - 1.0);
     if (p <= 0) {
-        if (A.diag) atomicAdd(&A.diag->clamp_p, 1ull);
+        clamps |= 2u;
         p = 0.000001;
     }
     QQ.v[1] = p;
+    return QQ;
+}
+
+__device__ __forceinline__ void gkr_count_clamps(const GkrArgs& A, unsigned clamps, unsigned long long users) {
+    if (!A.diag || users == 0) return;
+    if (clamps & 1u) atomicAdd(&A.diag->clamp_rho, users);
+    if (clamps & 2u) atomicAdd(&A.diag->clamp_p, users);
+}
+
+// predictor (kernel.cu:1611-1720): a step of st from the physical fluxes of
+// the four half-slope face states and the axisymmetric source terms at radius y
+__device__ inline Q7 gkr_predictor(const GkrArgs& A, const Q7& Q, const Q7& DX, const Q7& DY, double st, double y) {
+    unsigned clamps = 0;
+    const Q7 QQ = gkr_update(A, Q.v, gkr_pred_flux(A, Q, DX, DY), st, y, clamps);
+    gkr_count_clamps(A, clamps, 1);
     return QQ;
 }
 
@@ -298,10 +329,33 @@ __device__ __forceinline__ double gkr_face(const GkrArgs& A, const double (&L)[7
 template <int METOD>
 __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
     const long long cells = (long long)A.nx * A.ny;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double tmin = 1e300;
-    if (idx < cells) {
-        const int n = (int)(idx % A.nx), m = (int)(idx / A.nx);
+    // with the interior march this kernel takes the two-cell boundary band
+    // only: rows 0, 1, ny-2, ny-1, then columns 0, 1, nx-2, nx-1 of rows
+    // 2 .. ny-3 (nx, ny >= 5)
+    int n, m;
+    long long count;
+    if (A.band_only) {
+        const long long rows4 = 4LL * A.nx;
+        count = rows4 + 4LL * (A.ny - 4);
+        if (t < rows4) {
+            const int rs = (int)(t / A.nx);
+            n = (int)(t % A.nx);
+            m = rs < 2 ? rs : A.ny - 4 + rs;
+        } else {
+            const long long u = t - rows4;
+            const int cs = (int)(u & 3);
+            n = cs < 2 ? cs : A.nx - 4 + cs;
+            m = 2 + (int)(u >> 2);
+        }
+    } else {
+        count = cells;
+        n = (int)(t % A.nx);
+        m = (int)(t / A.nx);
+    }
+    const long long idx = (long long)m * A.nx + n;
+    if (t < count) {
         const double x = A.x0 + n * A.dx, y = A.y0 + m * A.dy;
         if (sqrt(x * x + y * y) < A.r_frozen) {
             // kernel.cu:2688-2695: frozen cells keep their state
@@ -385,31 +439,11 @@ __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
             }
             // kernel.cu:3247-3283: from the mean of the cell and its predictor,
             // dt/2, the axisymmetric sources, Powell's term zero
-            const double hV = h * A.idV, hy = h * fast_rcp(y);
-            double r2 = Qb[0] - hV * PSx - hy * Qb[0] * Qb[3];
-            if (r2 <= 0) {
-                if (A.diag) atomicAdd(&A.diag->clamp_rho, 1ull);
-                r2 = Qb[0];
-            }
-            const double ir2 = fast_rcp(r2);
-            const double vx2 =
-                (Qb[0] * Qb[2] - hV * (PUx) - hy * (Qb[0] * Qb[2] * Qb[3] - Qb[4] * Qb[5] * (1.0 / kCpi4))) * ir2;
-            const double vy2 = (Qb[0] * Qb[3] - hV * (PUy) -
-                                hy * (Qb[0] * Qb[3] * Qb[3] + (Qb[6] * Qb[6] - Qb[5] * Qb[5]) * (1.0 / kCpi4))) *
-                               ir2;
-            const double bx2 = (Qb[4] - hV * (PBx) - hy * (Qb[3] * Qb[4] - Qb[5] * Qb[2]));
-            const double by2 = (Qb[5] - hV * (PBy));
-            const double bz2 = (Qb[6] - hV * (PBz));
-            const double u8 = gkr_u8(A.inv_g1, Qb[0], Qb[1], Qb[2], Qb[3], Qb[4], Qb[5], Qb[6]);
-            double p2 = (u8 - hV * (PSy) -
-                         hy * (((u8 + Qb[1] + (Qb[4] * Qb[4] + Qb[5] * Qb[5] + Qb[6] * Qb[6]) * (1.0 / kCpi8)) * Qb[3] -
-                               Qb[5] * (Qb[2] * Qb[4] + Qb[3] * Qb[5]) * (1.0 / kCpi4))) -
-                         0.5 * r2 * (vx2 * vx2 + vy2 * vy2) - (bx2 * bx2 + by2 * by2 + bz2 * bz2) * (1.0 / kCpi8)) *
-                        (A.gamma - 1.0);
-            if (p2 <= 0) {
-                if (A.diag) atomicAdd(&A.diag->clamp_p, 1ull);
-                p2 = 0.000001;
-            }
+            unsigned clamps = 0;
+            const Q7 Q2 = gkr_update(A, Qb, GkrSums{PSx, PSy, PUx, PUy, PBx, PBy, PBz}, h, y, clamps);
+            gkr_count_clamps(A, clamps, 1);
+            const double r2 = Q2.v[0], p2 = Q2.v[1], vx2 = Q2.v[2], vy2 = Q2.v[3], bx2 = Q2.v[4], by2 = Q2.v[5],
+                         bz2 = Q2.v[6];
             A.s2[idx] = make_double2(r2, p2);
             A.u2[idx] = make_double2(vx2, vy2);
             A.b2[idx] = make_double3(bx2, by2, bz2);
@@ -419,6 +453,224 @@ __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
     // racy *T fold, kernel.cu:3241-3245)
     const double w = wave_min(tmin);
     if ((threadIdx.x & 63) == 0 && w < 1e300) atomic_min_nonneg(A.tmin_bits, w);
+}
+
+// The GKR step's interior (cells 2 <= n <= nx-3, 2 <= m <= ny-3, where no
+// side rule applies) as a march: one wave per strip of kGkrW columns and
+// `rows` rows marching up, lane l holding column kGkrW tx + l (lanes 2..61
+// own the strip's cells). Each cell's slopes, predictor and mean state are
+// formed once, when its row enters the march (the predictor at the cell's own
+// radius), and each face's HLLD flux once (x-faces by the lane west of them,
+// y-faces when the row above exists), given to the cell on the other side
+// negated. The reference evaluates the predictors of a cell's neighbours in
+// the cell's own stencil (at radius y -/+ dy for the south / north one) and
+// every face from both of its cells: against that these differ by rounding
+// (the flux is mirror-covariant) except where the normal field jumps at a face
+// with a contact speed of exactly 0; the band next to the sides, where the
+// reference's side rules make the neighbours' values depend on the cell that
+// reads them, stays with the per-cell kernel. Clamps of a predictor count once
+// for each cell of the strip that would have read it; the face time step
+// enters the minimum for each face of a cell that is updated here.
+constexpr int kGkrW = 60;
+
+template <int METOD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void gkr_march_kernel(const GkrArgs A) {
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+    const int nx = A.nx;
+    const int c = tx * kGkrW + lane;
+    const int cc = min(c, nx - 1);  // lanes past the grid feed no cell of the strip
+    const int r0 = 2 + ty * A.rows, r1 = min(r0 + A.rows, A.ny - 2);
+    const bool own_col = lane >= 2 && lane <= 61 && c <= nx - 3;
+    const double dx = A.dx, dy = A.dy;
+    const double h = *A.T_do / 2.0;
+    const double x = A.x0 + c * dx;
+    auto frozen = [&](int r) {
+        const double y = A.y0 + r * dy;
+        return sqrt(x * x + y * y) < A.r_frozen;
+    };
+    // an updated cell of this strip: owned column, owned row, not frozen
+    auto live = [&](int r) { return own_col && r >= r0 && r < r1 && !frozen(r); };
+    auto lane_next = [](bool b) { return from_next_lane(b ? 1.0 : 0.0) != 0.0; };
+    auto lane_prev = [](bool b) { return from_prev_lane(b ? 1.0 : 0.0) != 0.0; };
+    auto load = [&](int r) { return gkr_cell(A, cc, r); };
+
+    // The row data of row r at this lane: mean state m = (q + predictor) / 2
+    // and its four half-slope side states (rho, p falling back to m)
+    struct RowD {
+        double m[7], E[7], W[7], N[7], S[7];
+    };
+    auto row_data = [&](const Q7& qs, const Q7& q, const Q7& qn, int r, RowD& R) -> unsigned {
+        Q7 qw, qe;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            qw.v[k] = from_prev_lane(q.v[k]);
+            qe.v[k] = from_next_lane(q.v[k]);
+        }
+        const Q7 DX = gkr_slope(qe, q, qw, dx), DY = gkr_slope(qn, q, qs, dy);
+        unsigned clamps = 0;
+        const Q7 QQ = gkr_update(A, q.v, gkr_pred_flux(A, q, DX, DY), h, A.y0 + r * dy, clamps);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const double mk = (q.v[k] + QQ.v[k]) / 2.0;
+            R.m[k] = mk;
+            R.E[k] = mk + DX.v[k] / 2.0;
+            R.W[k] = mk - DX.v[k] / 2.0;
+            R.N[k] = mk + DY.v[k] / 2.0;
+            R.S[k] = mk - DY.v[k] / 2.0;
+            if (k < 2) {
+                if (R.E[k] <= 0.0) R.E[k] = mk;
+                if (R.W[k] <= 0.0) R.W[k] = mk;
+                if (R.N[k] <= 0.0) R.N[k] = mk;
+                if (R.S[k] <= 0.0) R.S[k] = mk;
+            }
+        }
+        return clamps;
+    };
+    // clamps of the predictor of row r at this lane, counted for each live
+    // reader in the strip (itself, its east, west, north and south neighbours)
+    auto count_pred = [&](unsigned clamps, int r) {
+        const bool self = live(r);
+        const bool east = lane_next(self), west = lane_prev(self);
+        if (clamps) {
+            const unsigned long long users =
+                (self ? 1ull : 0ull) + (east ? 1ull : 0ull) + (west ? 1ull : 0ull) + (live(r + 1) ? 1ull : 0ull) +
+                (live(r - 1) ? 1ull : 0ull);
+            gkr_count_clamps(A, clamps, users);
+        }
+    };
+    // the lab-frame flux sums of one face: F_rho, F_E, F_mx, F_my, F_Bx, F_By, F_Bz
+    auto face = [&](const double (&L)[7], const double (&R)[7], bool xface, double (&F)[7]) -> double {
+        double P[8];
+        const double t = gkr_face<METOD>(A, L, R, xface ? 1.0 : 0.0, xface ? 0.0 : 1.0, xface ? dx : dy, P);
+        F[0] = P[0];
+        F[1] = P[7];
+        F[2] = P[1];
+        F[3] = P[2];
+        F[4] = P[4];
+        F[5] = P[5];
+        F[6] = P[6];
+        return t;
+    };
+
+    // LDS: the primitive rows p and p+1 (slot row & 1), and the mean state and
+    // north side state of row p, which wait there across the face fluxes
+    __shared__ double qrow[2][7][kWave];
+    __shared__ double rowp[2][7][kWave];  // [0] m, [1] N
+    auto put_q = [&](int r, const Q7& q) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) qrow[r & 1][k][lane] = q.v[k];
+    };
+    auto get_q = [&](int r, int e) {
+        Q7 q;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) q.v[k] = qrow[r & 1][k][e];
+        return q;
+    };
+
+    double tmin = 1e300;
+    // prologue: rows r0-2 .. r0, the row data of row r0-1
+    {
+        const Q7 qa = load(r0 - 2), qb = load(r0 - 1), qc = load(r0);
+        put_q(r0 - 1, qb);
+        put_q(r0, qc);
+        RowD R;
+        const unsigned cl = row_data(qa, qb, qc, r0 - 1, R);
+        count_pred(cl, r0 - 1);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            rowp[0][k][lane] = R.m[k];
+            rowp[1][k][lane] = R.N[k];
+        }
+    }
+    double PEp[7], PNm[7];  // east flux of row p, north flux of row p-1
+#pragma unroll
+    for (int k = 0; k < 7; ++k) PEp[k] = PNm[k] = 0.0;
+
+    for (int p = r0 - 1; p < r1; ++p) {
+        asm volatile("" ::: "memory");
+        int e = lane;
+        asm volatile("" : "+v"(e));
+        RowD R;  // row p+1
+        {
+            const Q7 q_hi = load(p + 2);
+            const Q7 q_lo = get_q(p, e), q_mid = get_q(p + 1, e);
+            put_q(p + 2, q_hi);  // takes row p's slot, read above
+            const unsigned cl = row_data(q_lo, q_mid, q_hi, p + 1, R);
+            count_pred(cl, p + 1);
+        }
+        const bool lp = live(p), lp1 = live(p + 1);
+        // row p: east, south and west faces (the order the reference sums them)
+        double acc[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const double west = from_prev_lane(PEp[k]);
+            double a = 0.0;
+            a = a + PEp[k] * dy;
+            a = a + (-PNm[k]) * dx;
+            a = a + (-west) * dy;
+            acc[k] = a;
+        }
+        // east face of row p+1 (the west face of the next lane's cell)
+        if (p + 1 < r1) {
+            double Wn[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) Wn[k] = from_next_lane(R.W[k]);
+            const double te = face(R.E, Wn, true, PEp);
+            if (lp1 || lane_next(lp1)) tmin = tmin <= te ? tmin : te;
+        }
+        // north face of row p (the south face of row p+1)
+        double PN[7];
+        {
+            asm volatile("" ::: "memory");
+            int e2 = lane;
+            asm volatile("" : "+v"(e2));
+            double Np[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) Np[k] = rowp[1][k][e2];
+            const double tn = face(Np, R.S, false, PN);
+            if (lp || lp1) tmin = tmin <= tn ? tmin : tn;
+        }
+        asm volatile("" ::: "memory");
+        int e3 = lane;
+        asm volatile("" : "+v"(e3));
+        if (p >= r0 && own_col) {
+            const long long k = (long long)p * nx + c;
+            if (!frozen(p)) {
+                double mp[7];
+#pragma unroll
+                for (int f = 0; f < 7; ++f) mp[f] = rowp[0][f][e3];
+                GkrSums S;
+                S.PSx = acc[0] + PN[0] * dx;
+                S.PSy = acc[1] + PN[1] * dx;
+                S.PUx = acc[2] + PN[2] * dx;
+                S.PUy = acc[3] + PN[3] * dx;
+                S.PBx = acc[4] + PN[4] * dx;
+                S.PBy = acc[5] + PN[5] * dx;
+                S.PBz = acc[6] + PN[6] * dx;
+                unsigned clamps = 0;
+                const Q7 Q2 = gkr_update(A, mp, S, h, A.y0 + p * dy, clamps);
+                gkr_count_clamps(A, clamps, 1);
+                A.s2[k] = make_double2(Q2.v[0], Q2.v[1]);
+                A.u2[k] = make_double2(Q2.v[2], Q2.v[3]);
+                A.b2[k] = make_double3(Q2.v[4], Q2.v[5], Q2.v[6]);
+            } else {
+                // kernel.cu:2688-2695: frozen cells keep their state
+                A.s2[k] = A.s[k];
+                A.u2[k] = A.u[k];
+                A.b2[k] = A.b[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            PNm[k] = PN[k];
+            rowp[0][k][e3] = R.m[k];
+            rowp[1][k][e3] = R.N[k];
+        }
+    }
+    const double w = wave_min(tmin);
+    if (lane == 0 && w < 1e300) atomic_min_nonneg(A.tmin_bits, w);
 }
 
 }  // namespace gd2
