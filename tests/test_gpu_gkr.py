@@ -5,8 +5,10 @@ One step on the perturbed stellar-wind state of the production grid at two
 sizes, both fluxes (HLLD, the reference's, and HLLC-MHD), with the
 reference's frozen radius 2.0 and a small one; tiny grids where the side
 rules and the second-neighbour choices of the first and last two rows and
-columns overlap; a 20-step loop with gd2_funk_time against the oracle's loop;
-clamp counts equal. Bar: relative L1 per field 1e-12 for one step, 1e-10
+columns overlap, and grids whose interior (handled by the march, the
+two-cell band by the per-cell kernel) is one cell, one row or column, or
+several strips wide and high; a 20-step loop with gd2_funk_time against the
+oracle's loop; clamp counts equal. Bar: relative L1 per field 1e-12 for one step, 1e-10
 after 20; the face time step to 1e-14.
 """
 import numpy as np
@@ -65,7 +67,8 @@ def test_gkr_step_matches_oracle(method, nx, ny, r_frozen):
     assert (d["clamp_rho"], d["clamp_p"]) == (dref["clamp_rho"], dref["clamp_p"])
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 4), (4, 1), (2, 2), (3, 5), (5, 3), (6, 6)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 4), (4, 1), (2, 2), (3, 5), (5, 3), (5, 5), (6, 6), (5, 9), (65, 5),
+                                   (125, 70)])
 def test_gkr_tiny_grids_match_oracle(shape):
     nx, ny = shape
     say(f"gkr tiny {nx}x{ny}")
