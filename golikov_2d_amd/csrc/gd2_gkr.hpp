@@ -34,8 +34,7 @@ struct GkrArgs {
     int nx, ny;
     double dx, dy, x0, y0, gamma, inv_g1, krit, r_frozen;
     double idV;      // 1 / (dx dy)
-    int band_only;   // the per-cell kernel takes the boundary band, the march the interior
-    int tiles_x, rows;  // strips of the interior march
+    int tiles_x, rows, march_blocks;  // strips of the interior march, then the band blocks
     DeviceDiag* diag;
 };
 
@@ -326,129 +325,129 @@ __device__ __forceinline__ double gkr_face(const GkrArgs& A, const double (&L)[7
     return A.krit * rad / smax;
 }
 
+// One cell of the GKR step as the reference evaluates it (kernel.cu:2663-3284):
+// its 13-point stencil through the side rules, the slopes and predictors of
+// the cell and its four neighbours, its four faces from its own side.
+template <int METOD>
+__device__ inline void gkr_cell_step(const GkrArgs& A, int n, int m, double& tmin) {
+    const long long idx = (long long)m * A.nx + n;
+    const double x = A.x0 + n * A.dx, y = A.y0 + m * A.dy;
+    if (sqrt(x * x + y * y) < A.r_frozen) {
+        // kernel.cu:2688-2695: frozen cells keep their state
+        A.s2[idx] = A.s[idx];
+        A.u2[idx] = A.u[idx];
+        A.b2[idx] = A.b[idx];
+    } else {
+        const GkrStencil S{A, n, m};
+        const double dx = A.dx, dy = A.dy;
+        const double h = *A.T_do / 2.0;
+        Q7 DX1, DY1;
+        double Qb[7];
+        {
+            const Q7 q1 = S.p1(), q2 = S.p2(), q3 = S.p3(), q4 = S.p4(), q5 = S.p5();
+            DX1 = gkr_slope(q2, q1, q4, dx);
+            DY1 = gkr_slope(q5, q1, q3, dy);
+            const Q7 QQ1 = gkr_predictor(A, q1, DX1, DY1, h, y);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) Qb[k] = (q1.v[k] + QQ1.v[k]) / 2.0;
+        }
+        double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
+        // east, south, west, north faces (kernel.cu:3105-3238); the
+        // neighbour's slopes and predictor, its half-slope face state.
+        // The stencil points are read again in each face (cache hits) from
+        // indices passed through an empty asm, so that no point stays in
+        // registers across the faces.
+#pragma unroll 1
+        for (int f = 0; f < 4; ++f) {
+            int nn = n, mm = m;
+            asm volatile("" : "+v"(nn), "+v"(mm));
+            const GkrStencil T{A, nn, mm};
+            Q7 qk, DXk, DYk;
+            double yk = y;
+            if (f == 0) {
+                qk = T.p2();
+                DXk = gkr_slope(T.p6(), qk, T.p1(), dx);
+                DYk = gkr_slope(T.p10(), qk, T.p11(), dy);
+            } else if (f == 1) {
+                qk = T.p3();
+                DXk = gkr_slope(T.p11(), qk, T.p12(), dx);
+                DYk = gkr_slope(T.p1(), qk, T.p7(), dy);
+                yk = y - dy;
+            } else if (f == 2) {
+                qk = T.p4();
+                DXk = gkr_slope(T.p1(), qk, T.p8(), dx);
+                DYk = gkr_slope(T.p13(), qk, T.p12(), dy);
+            } else {
+                qk = T.p5();
+                DXk = gkr_slope(T.p10(), qk, T.p13(), dx);
+                DYk = gkr_slope(T.p9(), qk, T.p1(), dy);
+                yk = y + dy;
+            }
+            const Q7 QQk = gkr_predictor(A, qk, DXk, DYk, h, yk);
+            const Q7& Din = (f == 0 || f == 2) ? DX1 : DY1;
+            const Q7& Dnb = (f == 0 || f == 2) ? DXk : DYk;
+            const bool plus = f == 0 || f == 3;  // the cell's side of the face is at +D/2
+            double L[7], R[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const double mk = (qk.v[k] + QQk.v[k]) / 2.0;
+                L[k] = plus ? Qb[k] + Din.v[k] / 2.0 : Qb[k] - Din.v[k] / 2.0;
+                R[k] = plus ? mk - Dnb.v[k] / 2.0 : mk + Dnb.v[k] / 2.0;
+                if (k < 2) {
+                    if (L[k] <= 0.0) L[k] = Qb[k];
+                    if (R[k] <= 0.0) R[k] = mk;
+                }
+            }
+            const double n1 = f == 0 ? 1.0 : (f == 2 ? -1.0 : 0.0);
+            const double n2 = f == 3 ? 1.0 : (f == 1 ? -1.0 : 0.0);
+            const double len = (f == 0 || f == 2) ? dy : dx;
+            double P[8];
+            const double t = gkr_face<METOD>(A, L, R, n1, n2, (f == 0 || f == 2) ? dx : dy, P);
+            tmin = tmin <= t ? tmin : t;  // my_min, kernel.cu:102
+            PSx = PSx + P[0] * len;
+            PSy = PSy + P[7] * len;
+            PUx = PUx + P[1] * len;
+            PUy = PUy + P[2] * len;
+            PBx = PBx + P[4] * len;
+            PBy = PBy + P[5] * len;
+            PBz = PBz + P[6] * len;
+        }
+        // kernel.cu:3247-3283: from the mean of the cell and its predictor,
+        // dt/2, the axisymmetric sources, Powell's term zero
+        unsigned clamps = 0;
+        const Q7 Q2 = gkr_update(A, Qb, GkrSums{PSx, PSy, PUx, PUy, PBx, PBy, PBz}, h, y, clamps);
+        gkr_count_clamps(A, clamps, 1);
+        const double r2 = Q2.v[0], p2 = Q2.v[1], vx2 = Q2.v[2], vy2 = Q2.v[3], bx2 = Q2.v[4], by2 = Q2.v[5],
+                     bz2 = Q2.v[6];
+        A.s2[idx] = make_double2(r2, p2);
+        A.u2[idx] = make_double2(vx2, vy2);
+        A.b2[idx] = make_double3(bx2, by2, bz2);
+    }
+}
+
+// The two-cell boundary band in order: rows 0, 1, ny-2, ny-1, then columns 0,
+// 1, nx-2, nx-1 of rows 2 .. ny-3 (nx, ny >= 5); t < gkr_band_cells(A).
+__device__ __forceinline__ long long gkr_band_cells(const GkrArgs& A) { return 4LL * A.nx + 4LL * (A.ny - 4); }
+__device__ __forceinline__ void gkr_band_cell(const GkrArgs& A, long long t, int& n, int& m) {
+    const long long rows4 = 4LL * A.nx;
+    if (t < rows4) {
+        const int rs = (int)(t / A.nx);
+        n = (int)(t % A.nx);
+        m = rs < 2 ? rs : A.ny - 4 + rs;
+    } else {
+        const long long u = t - rows4;
+        const int cs = (int)(u & 3);
+        n = cs < 2 ? cs : A.nx - 4 + cs;
+        m = 2 + (int)(u >> 2);
+    }
+}
+
+// Every cell per thread (grids without an interior, nx or ny < 5).
 template <int METOD>
 __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
-    const long long cells = (long long)A.nx * A.ny;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double tmin = 1e300;
-    // with the interior march this kernel takes the two-cell boundary band
-    // only: rows 0, 1, ny-2, ny-1, then columns 0, 1, nx-2, nx-1 of rows
-    // 2 .. ny-3 (nx, ny >= 5)
-    int n, m;
-    long long count;
-    if (A.band_only) {
-        const long long rows4 = 4LL * A.nx;
-        count = rows4 + 4LL * (A.ny - 4);
-        if (t < rows4) {
-            const int rs = (int)(t / A.nx);
-            n = (int)(t % A.nx);
-            m = rs < 2 ? rs : A.ny - 4 + rs;
-        } else {
-            const long long u = t - rows4;
-            const int cs = (int)(u & 3);
-            n = cs < 2 ? cs : A.nx - 4 + cs;
-            m = 2 + (int)(u >> 2);
-        }
-    } else {
-        count = cells;
-        n = (int)(t % A.nx);
-        m = (int)(t / A.nx);
-    }
-    const long long idx = (long long)m * A.nx + n;
-    if (t < count) {
-        const double x = A.x0 + n * A.dx, y = A.y0 + m * A.dy;
-        if (sqrt(x * x + y * y) < A.r_frozen) {
-            // kernel.cu:2688-2695: frozen cells keep their state
-            A.s2[idx] = A.s[idx];
-            A.u2[idx] = A.u[idx];
-            A.b2[idx] = A.b[idx];
-        } else {
-            const GkrStencil S{A, n, m};
-            const double dx = A.dx, dy = A.dy;
-            const double h = *A.T_do / 2.0;
-            Q7 DX1, DY1;
-            double Qb[7];
-            {
-                const Q7 q1 = S.p1(), q2 = S.p2(), q3 = S.p3(), q4 = S.p4(), q5 = S.p5();
-                DX1 = gkr_slope(q2, q1, q4, dx);
-                DY1 = gkr_slope(q5, q1, q3, dy);
-                const Q7 QQ1 = gkr_predictor(A, q1, DX1, DY1, h, y);
-#pragma unroll
-                for (int k = 0; k < 7; ++k) Qb[k] = (q1.v[k] + QQ1.v[k]) / 2.0;
-            }
-            double PSx = 0, PSy = 0, PUx = 0, PUy = 0, PBx = 0, PBy = 0, PBz = 0;
-            // east, south, west, north faces (kernel.cu:3105-3238); the
-            // neighbour's slopes and predictor, its half-slope face state.
-            // The stencil points are read again in each face (cache hits) from
-            // indices passed through an empty asm, so that no point stays in
-            // registers across the faces.
-#pragma unroll 1
-            for (int f = 0; f < 4; ++f) {
-                int nn = n, mm = m;
-                asm volatile("" : "+v"(nn), "+v"(mm));
-                const GkrStencil T{A, nn, mm};
-                Q7 qk, DXk, DYk;
-                double yk = y;
-                if (f == 0) {
-                    qk = T.p2();
-                    DXk = gkr_slope(T.p6(), qk, T.p1(), dx);
-                    DYk = gkr_slope(T.p10(), qk, T.p11(), dy);
-                } else if (f == 1) {
-                    qk = T.p3();
-                    DXk = gkr_slope(T.p11(), qk, T.p12(), dx);
-                    DYk = gkr_slope(T.p1(), qk, T.p7(), dy);
-                    yk = y - dy;
-                } else if (f == 2) {
-                    qk = T.p4();
-                    DXk = gkr_slope(T.p1(), qk, T.p8(), dx);
-                    DYk = gkr_slope(T.p13(), qk, T.p12(), dy);
-                } else {
-                    qk = T.p5();
-                    DXk = gkr_slope(T.p10(), qk, T.p13(), dx);
-                    DYk = gkr_slope(T.p9(), qk, T.p1(), dy);
-                    yk = y + dy;
-                }
-                const Q7 QQk = gkr_predictor(A, qk, DXk, DYk, h, yk);
-                const Q7& Din = (f == 0 || f == 2) ? DX1 : DY1;
-                const Q7& Dnb = (f == 0 || f == 2) ? DXk : DYk;
-                const bool plus = f == 0 || f == 3;  // the cell's side of the face is at +D/2
-                double L[7], R[7];
-#pragma unroll
-                for (int k = 0; k < 7; ++k) {
-                    const double mk = (qk.v[k] + QQk.v[k]) / 2.0;
-                    L[k] = plus ? Qb[k] + Din.v[k] / 2.0 : Qb[k] - Din.v[k] / 2.0;
-                    R[k] = plus ? mk - Dnb.v[k] / 2.0 : mk + Dnb.v[k] / 2.0;
-                    if (k < 2) {
-                        if (L[k] <= 0.0) L[k] = Qb[k];
-                        if (R[k] <= 0.0) R[k] = mk;
-                    }
-                }
-                const double n1 = f == 0 ? 1.0 : (f == 2 ? -1.0 : 0.0);
-                const double n2 = f == 3 ? 1.0 : (f == 1 ? -1.0 : 0.0);
-                const double len = (f == 0 || f == 2) ? dy : dx;
-                double P[8];
-                const double t = gkr_face<METOD>(A, L, R, n1, n2, (f == 0 || f == 2) ? dx : dy, P);
-                tmin = tmin <= t ? tmin : t;  // my_min, kernel.cu:102
-                PSx = PSx + P[0] * len;
-                PSy = PSy + P[7] * len;
-                PUx = PUx + P[1] * len;
-                PUy = PUy + P[2] * len;
-                PBx = PBx + P[4] * len;
-                PBy = PBy + P[5] * len;
-                PBz = PBz + P[6] * len;
-            }
-            // kernel.cu:3247-3283: from the mean of the cell and its predictor,
-            // dt/2, the axisymmetric sources, Powell's term zero
-            unsigned clamps = 0;
-            const Q7 Q2 = gkr_update(A, Qb, GkrSums{PSx, PSy, PUx, PUy, PBx, PBy, PBz}, h, y, clamps);
-            gkr_count_clamps(A, clamps, 1);
-            const double r2 = Q2.v[0], p2 = Q2.v[1], vx2 = Q2.v[2], vy2 = Q2.v[3], bx2 = Q2.v[4], by2 = Q2.v[5],
-                         bz2 = Q2.v[6];
-            A.s2[idx] = make_double2(r2, p2);
-            A.u2[idx] = make_double2(vx2, vy2);
-            A.b2[idx] = make_double3(bx2, by2, bz2);
-        }
-    }
+    if (t < (long long)A.nx * A.ny) gkr_cell_step<METOD>(A, (int)(t % A.nx), (int)(t / A.nx), tmin);
     // exact device-wide minimum of the face time steps (the reference's
     // racy *T fold, kernel.cu:3241-3245)
     const double w = wave_min(tmin);
@@ -468,7 +467,7 @@ __global__ __launch_bounds__(256) void gkr_kernel(const GkrArgs A) {
 // (the flux is mirror-covariant) except where the normal field jumps at a face
 // with a contact speed of exactly 0; the band next to the sides, where the
 // reference's side rules make the neighbours' values depend on the cell that
-// reads them, stays with the per-cell kernel. Clamps of a predictor count once
+// reads them, is stepped per cell by the blocks after the march's. Clamps of a predictor count once
 // for each cell of the strip that would have read it; the face time step
 // enters the minimum for each face of a cell that is updated here.
 constexpr int kGkrW = 60;
@@ -476,7 +475,20 @@ constexpr int kGkrW = 60;
 template <int METOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void gkr_march_kernel(const GkrArgs A) {
     const int lane = threadIdx.x;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if ((int)blockIdx.x >= A.march_blocks) {
+        // the blocks after the march: the boundary band, one cell per thread
+        const long long t = (long long)(blockIdx.x - A.march_blocks) * kWave + lane;
+        double tb = 1e300;
+        if (t < gkr_band_cells(A)) {
+            int n, m;
+            gkr_band_cell(A, t, n, m);
+            gkr_cell_step<METOD>(A, n, m, tb);
+        }
+        const double w = wave_min(tb);
+        if (lane == 0 && w < 1e300) atomic_min_nonneg(A.tmin_bits, w);
+        return;
+    }
+    const int tile = xcd_tile(blockIdx.x, A.march_blocks);
     const int tx = tile % A.tiles_x, ty = tile / A.tiles_x;
     const int nx = A.nx;
     const int c = tx * kGkrW + lane;

@@ -291,26 +291,28 @@ int gd2_kernel_gkr(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
     A.diag = device_diag();
     if (!A.diag) return GD2_ERR_HIP;
     const hipStream_t st = (hipStream_t)stream;
-    // grids with an interior (nx, ny >= 5): the interior march and the
-    // per-cell kernel over the two-cell boundary band; else per cell
-    A.band_only = (g->nx >= 5 && g->ny >= 5) ? 1 : 0;
-    const long long cells = A.band_only ? 4LL * g->nx + 4LL * (g->ny - 4) : (long long)g->nx * g->ny;
-    const unsigned blocks = (unsigned)((cells + 255) / 256);
-    if (method == 3)
-        hipLaunchKernelGGL(gkr_kernel<3>, dim3(blocks), dim3(256), 0, st, A);
-    else
-        hipLaunchKernelGGL(gkr_kernel<2>, dim3(blocks), dim3(256), 0, st, A);
-    int rc = hip_error_code(hipGetLastError());
-    if (rc != GD2_OK || !A.band_only) return rc;
+    // grids with an interior (nx, ny >= 5): one launch, the interior march's
+    // strips and then one-wave blocks over the two-cell boundary band;
+    // smaller grids per cell
+    if (g->nx < 5 || g->ny < 5) {
+        const unsigned blocks = (unsigned)(((long long)g->nx * g->ny + 255) / 256);
+        if (method == 3)
+            hipLaunchKernelGGL(gkr_kernel<3>, dim3(blocks), dim3(256), 0, st, A);
+        else
+            hipLaunchKernelGGL(gkr_kernel<2>, dim3(blocks), dim3(256), 0, st, A);
+        return hip_error_code(hipGetLastError());
+    }
     A.tiles_x = (g->nx - 4 + kGkrW - 1) / kGkrW;
     const void* k = method == 3 ? reinterpret_cast<const void*>(&gkr_march_kernel<3>)
                                 : reinterpret_cast<const void*>(&gkr_march_kernel<2>);
     A.rows = strip_rows(k, A.tiles_x, g->ny - 4, 2);
-    const unsigned mblocks = (unsigned)(A.tiles_x * ((g->ny - 4 + A.rows - 1) / A.rows));
+    A.march_blocks = A.tiles_x * ((g->ny - 4 + A.rows - 1) / A.rows);
+    const long long band = 4LL * g->nx + 4LL * (g->ny - 4);
+    const unsigned blocks = (unsigned)(A.march_blocks + (band + kWave - 1) / kWave);
     if (method == 3)
-        hipLaunchKernelGGL(gkr_march_kernel<3>, dim3(mblocks), dim3(kWave), 0, st, A);
+        hipLaunchKernelGGL(gkr_march_kernel<3>, dim3(blocks), dim3(kWave), 0, st, A);
     else
-        hipLaunchKernelGGL(gkr_march_kernel<2>, dim3(mblocks), dim3(kWave), 0, st, A);
+        hipLaunchKernelGGL(gkr_march_kernel<2>, dim3(blocks), dim3(kWave), 0, st, A);
     return hip_error_code(hipGetLastError());
 }
 
