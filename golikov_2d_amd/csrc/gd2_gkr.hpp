@@ -2,17 +2,19 @@
 // (Kernel_GKR, kernel.cu:2663-3284, with predictor kernel.cu:1611-1720 and
 // POTOK_Korolkov Gas_Dinamic.cu:70-181) in the reference AoS layout.
 //
-// One thread per cell, as the reference evaluates it: the cell gathers its
+// The reference evaluates the step one thread per cell: the cell gathers its
 // 13-point stencil (radius 2 along the axes plus the four diagonals) through
 // the step's own side rules, forms minmod slopes of itself and its four
 // neighbours, a half-step predictor of each of the five (physical fluxes of
 // the half-slope face states plus the axisymmetric sources), and updates
 // itself from the mean of its state and its predictor with the HLLD (or
 // HLLC-MHD) fluxes of the predictor-averaged face states, over dt/2 with the
-// axisymmetric sources. Each face is evaluated from the cell's own side with
-// its outward normal, as in the reference (the MHD fluxes are not continuous
-// at S_M = 0 when the normal field jumps). Stencil points are read from
-// memory when needed (served by the caches) rather than held in registers.
+// axisymmetric sources; each face from the cell's own side with its outward
+// normal. gkr_cell_step restates that for the two-cell band next to the
+// sides, where the side rules make a neighbour's values depend on the cell
+// that reads them; the interior, where no side rule applies, is a march
+// (gkr_march_kernel) that forms each cell's slopes and predictor once and
+// each face's flux once. Divisions are products with reciprocals.
 //
 // State vector Q7 = (rho, p, vx, vy, Bx, By, Bz), Gaussian B.
 #pragma once
