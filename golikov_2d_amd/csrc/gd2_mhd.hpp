@@ -760,44 +760,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
             } else {
                 const long long k = (long long)j * A.nx + c;
                 // primitive-form update of add2 / Kernel_TVD (kernel.cu:2137-2195);
-                // the 1/y terms only with the axisymmetric option
-                const double gm1 = A.gamma - 1.0;
-                const double U8 = q.p / gm1 + 0.5 * q.r * (q.u * q.u + q.v * q.v) +
-                                  (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / kCpi8;
+                // the 1/y terms only with the axisymmetric option. Divisions by
+                // dV, y, the new density and the constants as products with
+                // reciprocals (dt/dV once per launch, dt/y once per row, 1/rho'
+                // once per cell): rounding-level against the reference's
+                // quotients.
+                const double dtV = dt / dV;
+                const double U8 = q.p * A.inv_g1 + 0.5 * q.r * (q.u * q.u + q.v * q.v) +
+                                  (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8);
                 const double skk = q.u * q.bx + q.v * q.by;
                 double r2, ux, uy, bx, by, bz, p2;
                 if (PROD && A.axisym) {
-                    const double y = yc;
-                    const double Pd = Pdiv + dV * q.by / y;
-                    r2 = q.r - dt * PSr / dV - dt * q.r * q.v / y;
+                    const double iy = 1.0 / yc, dty = dt * iy;
+                    const double Pd = Pdiv + dV * q.by * iy;
+                    r2 = q.r - dtV * PSr - dty * q.r * q.v;
                     if (r2 <= 0.0) {
                         atomicAdd(&A.diag->clamp_rho, 1ull);
                         r2 = q.r;
                     }
-                    ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pd) / dV -
-                          dt * (q.r * q.u * q.v - q.bx * q.by / kCpi4) / y) / r2;
-                    uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pd) / dV -
-                          dt * (q.r * q.v * q.v + (q.bz * q.bz - q.by * q.by) / kCpi4) / y) / r2;
-                    bx = q.bx - dt * (PBx + q.u * Pd) / dV - dt * (q.v * q.bx - q.by * q.u) / y;
-                    by = q.by - dt * (PBy + q.v * Pd) / dV;
-                    bz = q.bz - dt * PBz / dV;
-                    p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pd) / dV -
-                          dt * (((U8 + q.p + (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / kCpi8) * q.v -
-                                 q.by * skk / kCpi4) / y) -
-                          0.5 * r2 * (ux * ux + uy * uy) - (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                    const double ir2 = 1.0 / r2;
+                    ux = (q.r * q.u - dtV * (PUx + (q.bx * (1.0 / kCpi4)) * Pd) -
+                          dty * (q.r * q.u * q.v - q.bx * q.by * (1.0 / kCpi4))) * ir2;
+                    uy = (q.r * q.v - dtV * (PUy + (q.by * (1.0 / kCpi4)) * Pd) -
+                          dty * (q.r * q.v * q.v + (q.bz * q.bz - q.by * q.by) * (1.0 / kCpi4))) * ir2;
+                    bx = q.bx - dtV * (PBx + q.u * Pd) - dty * (q.v * q.bx - q.by * q.u);
+                    by = q.by - dtV * (PBy + q.v * Pd);
+                    bz = q.bz - dtV * PBz;
+                    p2 = (U8 - dtV * (PSe + (skk * (1.0 / kCpi4)) * Pd) -
+                          dty * ((U8 + q.p + (q.bx * q.bx + q.by * q.by + q.bz * q.bz) * (1.0 / kCpi8)) * q.v -
+                                 q.by * skk * (1.0 / kCpi4)) -
+                          0.5 * r2 * (ux * ux + uy * uy) - (bx * bx + by * by + bz * bz) * (1.0 / kCpi8)) *
+                         (A.gamma - 1.0);
                 } else {
-                    r2 = q.r - dt * PSr / dV;
+                    r2 = q.r - dtV * PSr;
                     if (r2 <= 0.0) {
                         atomicAdd(&A.diag->clamp_rho, 1ull);
                         r2 = q.r;
                     }
-                    ux = (q.r * q.u - dt * (PUx + (q.bx / kCpi4) * Pdiv) / dV) / r2;
-                    uy = (q.r * q.v - dt * (PUy + (q.by / kCpi4) * Pdiv) / dV) / r2;
-                    bx = q.bx - dt * (PBx + q.u * Pdiv) / dV;
-                    by = q.by - dt * (PBy + q.v * Pdiv) / dV;
-                    bz = q.bz - dt * PBz / dV;
-                    p2 = (U8 - dt * (PSe + (skk / kCpi4) * Pdiv) / dV - 0.5 * r2 * (ux * ux + uy * uy) -
-                          (bx * bx + by * by + bz * bz) / kCpi8) * gm1;
+                    const double ir2 = 1.0 / r2;
+                    ux = (q.r * q.u - dtV * (PUx + (q.bx * (1.0 / kCpi4)) * Pdiv)) * ir2;
+                    uy = (q.r * q.v - dtV * (PUy + (q.by * (1.0 / kCpi4)) * Pdiv)) * ir2;
+                    bx = q.bx - dtV * (PBx + q.u * Pdiv);
+                    by = q.by - dtV * (PBy + q.v * Pdiv);
+                    bz = q.bz - dtV * PBz;
+                    p2 = (U8 - dtV * (PSe + (skk * (1.0 / kCpi4)) * Pdiv) - 0.5 * r2 * (ux * ux + uy * uy) -
+                          (bx * bx + by * by + bz * bz) * (1.0 / kCpi8)) * (A.gamma - 1.0);
                 }
                 if (p2 <= 0.0) {
                     atomicAdd(&A.diag->clamp_p, 1ull);
