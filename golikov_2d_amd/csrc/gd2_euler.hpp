@@ -511,7 +511,8 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             } else {
                 // primitive-form update of Ker_Dekard (kernel.cu:1859-1874);
                 // Kernel_TVD_old's 1/y terms with the axisymmetric option
-                // (kernel.cu:3651-3662)
+                // (kernel.cu:3651-3662). Divisions by gamma-1, y and the new
+                // density as products with reciprocals (rounding-level).
                 const long long k = (long long)j * A.nx + c;
                 const PrimE& q = v0.q;
                 const double gm1 = A.gamma - 1.0;
@@ -519,16 +520,17 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 const double PSr = P.r, PUx = P.mx, PUy = P.my, PSe = P.e;
                 double r2, ux, uy, p2;
                 if (PROD && ORDER == 2 && A.axisym) {
-                    const double y = A.y0 + j * dy;
-                    r2 = q.r - Tdv * PSr - dt * q.r * q.v / y;
+                    const double dty = dt / (A.y0 + j * dy);
+                    r2 = q.r - Tdv * PSr - dty * q.r * q.v;
                     if (r2 <= 0.0) {
                         atomicAdd(&A.diag->clamp_rho, 1ull);
                         r2 = 0.0001;
                     }
-                    ux = (q.r * q.u - Tdv * PUx - dt * q.r * q.v * q.u / y) / r2;
-                    uy = (q.r * q.v - Tdv * PUy - dt * q.r * q.v * q.v / y) / r2;
-                    p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe -
-                           dt * q.v * (A.gamma * q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) / y) -
+                    const double ir2 = 1.0 / r2;
+                    ux = (q.r * q.u - Tdv * PUx - dty * q.r * q.v * q.u) * ir2;
+                    uy = (q.r * q.v - Tdv * PUy - dty * q.r * q.v * q.v) * ir2;
+                    p2 = (((q.p * A.inv_g1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe -
+                           dty * q.v * (A.gamma * q.p * A.inv_g1 + q.r * (q.u * q.u + q.v * q.v) * 0.5)) -
                           0.5 * r2 * (ux * ux + uy * uy)) * gm1;
                 } else {
                     r2 = q.r - Tdv * PSr;
@@ -536,9 +538,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                         atomicAdd(&A.diag->clamp_rho, 1ull);
                         r2 = 0.0001;
                     }
-                    ux = (q.r * q.u - Tdv * PUx) / r2;
-                    uy = (q.r * q.v - Tdv * PUy) / r2;
-                    p2 = (((q.p / gm1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe) -
+                    const double ir2 = 1.0 / r2;
+                    ux = (q.r * q.u - Tdv * PUx) * ir2;
+                    uy = (q.r * q.v - Tdv * PUy) * ir2;
+                    p2 = (((q.p * A.inv_g1 + q.r * (q.u * q.u + q.v * q.v) * 0.5) - Tdv * PSe) -
                           0.5 * r2 * (ux * ux + uy * uy)) * gm1;
                 }
                 if (p2 <= 0.0) {
