@@ -393,10 +393,15 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     double smax;
     FluxE Gm;  // y-face flux G(j-1/2), frame (n = +y, t = +x)
     Gm.fr = Gm.fn = Gm.ft = Gm.fe = 0.0;
-    double tGm = 0.0;
+    double sGm = 0.0;
 
     double wmax = 0.0;
-    double tmin = 1e300;
+    // kAoS: the largest wave speed over the x-faces and over the y-faces of
+    // the cells this lane updates; the face time step krit rad / S is
+    // monotone in S, so the minimum over the faces is krit rad / max S
+    // (bitwise), formed once at the end
+    double sx = 0.0, sy = 0.0;
+    bool any_upd = false;
 
     for (int j = j0 - 1; j < j1; ++j) {
         // loads first: row j+2 arrives while the x-face of row j is computed
@@ -406,7 +411,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         // (LDS); row j reached LDS two iterations ago
         FluxE Fl, Fr;
         Bal X;
-        double tFl = 0.0, tFr = 0.0;
+        double sFl = 0.0, sFr = 0.0;
         // row j (with its halo) was vetted when it was converted: bit 1 now
         const bool xpos = (pos_hist & 2u) != 0u;
         if (j >= j0) {
@@ -427,13 +432,13 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 Fl = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
             }
             wall_face_fix<FLUX>(Fl, c == 0 && A.bc[0] == GD2_BC_REFLECTIVE);
-            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+            sFl = (LAYOUT == kAoS) ? smax : 0.0;
             // F(c+1/2) from the next lane
             Fr.fr = from_next_lane(Fl.fr);
             Fr.fn = from_next_lane(Fl.fn);
             Fr.ft = from_next_lane(Fl.ft);
             Fr.fe = from_next_lane(Fl.fe);
-            if constexpr (LAYOUT == kAoS) tFr = from_next_lane(tFl);
+            if constexpr (LAYOUT == kAoS) sFr = from_next_lane(sFl);
             X = x_balance(Fl, Fr, dy);
         }
         // the update's inputs: row j's conserved values from LDS (stored when
@@ -464,7 +469,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         // y-face G(j+1/2) from rows j-1 .. j+2
         Slope s1 = slope3<ORDER>(v0.q, v1.q, v2.q);
         FluxE Gp;
-        double tGp;
+        double sGp;
         if (pos_hist == 15u) {
             PrimE L = face_plus<false>(v0.q, s0), R = face_minus<false>(v1.q, s1);
             Gp = face_flux<FLUX, false>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
@@ -473,13 +478,13 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             Gp = face_flux<FLUX>(A, L.r, L.v, L.u, L.p, R.r, R.v, R.u, R.p, smax);
         }
         wall_face_fix<FLUX>(Gp, j == -1 && A.bc[2] == GD2_BC_REFLECTIVE);
-        tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
+        sGp = (LAYOUT == kAoS) ? smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             v0 = v1;
             v1 = v2;
             s0 = s1;
             Gm = Gp;
-            tGm = tGp;
+            sGm = sGp;
             continue;
         }
 
@@ -550,8 +555,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
                 }
                 A.s2[k] = make_double2(r2, p2);
                 A.u2[k] = make_double2(ux, uy);
-                // per-cell tmin starts at 1000 (kernel.cu:1748)
-                tmin = fmin(tmin, fmin(1000.0, fmin(fmin(tFr, tGm), fmin(tFl, tGp))));
+                // per-cell tmin starts at 1000 (kernel.cu:1748): applied at the end
+                sx = fmax(sx, fmax(sFr, sFl));
+                sy = fmax(sy, fmax(sGm, sGp));
+                any_upd = true;
             }
         } else if constexpr (LAYOUT == kAoS) {
             if (owns) {  // frozen edge cell: carried over unchanged
@@ -566,7 +573,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         v1 = v2;
         s0 = s1;
         Gm = Gp;
-        tGm = tGp;
+        sGm = sGp;
     }
 
     if constexpr (LAYOUT == kSoA && WANT_W) {
@@ -574,6 +581,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
         if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
     }
     if constexpr (LAYOUT == kAoS) {
+        const double tmin = any_upd ? fmin(1000.0, fmin(krx / sx, kry / sy)) : 1e300;
         const double t = wave_min(tmin);
         if (lane == 0 && t < 1e300) atomic_min_nonneg(A.tmin_bits, t);
     }

@@ -620,9 +620,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
     double smax;
     LabFluxM Gm;
     Gm.r = Gm.mx = Gm.my = Gm.bx = Gm.by = Gm.bz = Gm.e = 0.0;
-    double tGm = 0.0;
+    double sGm = 0.0;
 
-    double wmax = 0.0, tmin = 1e300;
+    double wmax = 0.0;
+    // kAoS: the largest wave speed over the x-faces and over the y-faces of
+    // the cells this lane updates; the face time step krit rad / S is
+    // monotone in S, so the minimum over the faces is krit rad / max S
+    // (bitwise), formed once at the end
+    double sx = 0.0, sy = 0.0;
+    bool any_upd = false;
 
     // the wave behind on its SIMD takes the higher issue priority, decided
     // every kTurnPeriod passes
@@ -637,7 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         // cell is q0); F(c+1/2) from the next lane. Only the differences
         // F(c+1/2) - F(c-1/2) are carried to the update.
         LabFluxM dF;
-        double tFl = 0.0, tFr = 0.0, dbx = 0.0;
+        double sFl = 0.0, sFr = 0.0, dbx = 0.0;
         if (j >= j0) {
             const int sl = j & 1;
             const PrimM qa = lds_get_m(lds.q, sl, lane);
@@ -651,9 +657,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
             LabFluxM Fl = mhd_flux<METOD, true>(wmir ? mirror_x(xr) : xl, wmir ? mirror_x(xl) : xr, A.gamma,
                                                 A.inv_g1, smax);
             if (wmir) Fl = unmirror_x(Fl);
-            tFl = (LAYOUT == kAoS) ? krx / smax : 0.0;
+            sFl = (LAYOUT == kAoS) ? smax : 0.0;
             const LabFluxM Fr = shfl_down_m(Fl);
-            tFr = (LAYOUT == kAoS) ? __shfl_down(tFl, 1, kWave) : 0.0;
+            sFr = (LAYOUT == kAoS) ? __shfl_down(sFl, 1, kWave) : 0.0;
             dF.r = Fr.r - Fl.r;
             dF.mx = Fr.mx - Fl.mx;
             dF.my = Fr.my - Fl.my;
@@ -686,14 +692,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         const PrimM yl = face_m(q0, s0, 0.5), yr = face_m(q1, s1, -0.5);
         const LabFluxM Gp = smir ? unmirror_y(mhd_flux<METOD, false>(mirror_y(yr), mirror_y(yl), A.gamma, A.inv_g1, smax))
                                  : mhd_flux<METOD, false>(yl, yr, A.gamma, A.inv_g1, smax);
-        const double tGp = (LAYOUT == kAoS) ? kry / smax : 0.0;
+        const double sGp = (LAYOUT == kAoS) ? smax : 0.0;
         if (j < j0) {  // lead-in pass: only G(j0-1/2)
             by_s = q0.by;
             q0 = q1;
             q1 = q2;
             s0 = s1;
             Gm = Gp;
-            tGm = tGp;
+            sGm = sGp;
             continue;
         }
 
@@ -813,7 +819,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
                 A.s2[k] = make_double2(r2, p2);
                 A.u2[k] = make_double2(ux, uy);
                 A.b2[k] = make_double3(bx, by, bz);
-                tmin = fmin(tmin, fmin(1000.0, fmin(fmin(tFr, tGm), fmin(tFl, tGp))));
+                sx = fmax(sx, fmax(sFr, sFl));
+                sy = fmax(sy, fmax(sGm, sGp));
+                any_upd = true;
             }
         } else if constexpr (LAYOUT == kAoS) {
             if (owns) {
@@ -829,7 +837,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         q1 = q2;
         s0 = s1;
         Gm = Gp;
-        tGm = tGp;
+        sGm = sGp;
     }
 
     turn.finish();
@@ -838,6 +846,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mh
         if (lane == 0 && A.wmax_bits) atomic_max_nonneg(A.wmax_bits, w);
     }
     if constexpr (LAYOUT == kAoS) {
+        const double tmin = any_upd ? fmin(1000.0, fmin(krx / sx, kry / sy)) : 1e300;
         const double t = wave_min(tmin);
         if (lane == 0 && t < 1e300) atomic_min_nonneg(A.tmin_bits, t);
     }
