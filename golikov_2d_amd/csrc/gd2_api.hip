@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <atomic>
 #include <mutex>
@@ -100,7 +101,15 @@ static bool grid_ok(const gd2_grid* g, bool allow_frozen, int prod = 0) {
     }
     if (!(prod & kProdOptions) && (g->r_frozen != 0.0 || g->axisym != 0)) return false;
     if (!(g->r_frozen >= 0.0)) return false;
-    if (g->axisym && !(g->y0 > 0.0)) return false;
+    // the 1/y terms take the signed centre y0 + m dy (rows below the axis
+    // included, as Kernel_TVD and Kernel_TVD_old do and as add2's mirrored
+    // evaluation of such rows amounts to); a row centred on the axis would
+    // divide by zero
+    if (g->axisym) {
+        if (!std::isfinite(g->y0)) return false;
+        const double k = std::nearbyint(-g->y0 / g->dy);
+        if (k >= 0.0 && k < (double)g->ny && std::fabs(g->y0 + k * g->dy) <= 1e-9 * g->dy) return false;
+    }
     // a periodic side needs a periodic opposite side
     if ((g->bc[0] == GD2_BC_PERIODIC) != (g->bc[1] == GD2_BC_PERIODIC)) return false;
     if ((g->bc[2] == GD2_BC_PERIODIC) != (g->bc[3] == GD2_BC_PERIODIC)) return false;

@@ -85,8 +85,11 @@ typedef struct gd2_grid {
      * (0 elsewhere): cells with sqrt(x^2 + y^2) < r_frozen keep their state
      * and give no time step (the inner sphere, kernel.cu:1918, 2218, 3303);
      * axisym != 0 adds the axisymmetric (y = r) source terms of
-     * kernel.cu:2137-2190, 2615-2650 (MHD) and 3651-3662 (Euler)
-     * (requires y0 > 0). */
+     * kernel.cu:2137-2190, 2615-2650 (MHD) and 3651-3662 (Euler), taken
+     * with the signed centre y = y0 + m dy: rows below the axis are
+     * accepted (Kernel_TVD and Kernel_TVD_old divide by the signed y; add2
+     * evaluates them in the mirrored frame, kernel.cu:1885-2057, which is
+     * the same step up to rounding); a row centred on y = 0 is GD2_ERR_ARG. */
     double r_frozen;
     int axisym;
 } gd2_grid;

@@ -1224,6 +1224,15 @@ static euler_sum euler_cell_fluxes(const o_grid* g, int method, const faces* F, 
     return r;
 }
 
+/* The axisymmetric steps divide by the signed cell centre y0 + m dy; no row
+ * may be centred on the axis (the reference would divide by zero there). */
+static int axis_rows_ok(const o_grid* g)
+{
+    if (!isfinite(g->y0)) return 0;
+    const double k = nearbyint(-g->y0 / g->dy);
+    return !(k >= 0.0 && k < (double)g->ny && fabs(g->y0 + k * g->dy) <= 1e-9 * g->dy);
+}
+
 static int is_frozen(const o_grid* g, int i, int j)
 {
     return (g->bc[0] == O_BC_FROZEN && i == 0) || (g->bc[1] == O_BC_FROZEN && i == g->nx - 1) ||
@@ -1240,7 +1249,7 @@ static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2*
 {
     if (method < 0 || method > 4) return -1;
     if ((g->axisym || g->r_frozen != 0.0) && order != 2) return -1;
-    if (g->axisym && !(g->y0 > 0.0)) return -1;
+    if (g->axisym && !axis_rows_ok(g)) return -1;
     src_t S = {g, 0, 0, s, u, NULL, NULL, 0};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
@@ -1341,13 +1350,26 @@ static mhd_sum mhd_cell_fluxes(const o_grid* g, int metod, const faces* F, const
     return m;
 }
 
+/* add2's rows below the axis (kernel.cu:1885-1889, 2026-2057): the cell is
+ * evaluated in the mirrored frame, its south and north neighbours swapped
+ * and vy, By, Bz of the five cells negated; the update then takes |y| and
+ * negates vy, By, Bz of the result (kernel.cu:2177-2182). */
+static void mirror_stencil(stencil* st)
+{
+    prim t = st->s1;
+    st->s1 = st->n1;
+    st->n1 = t;
+    prim* q[5] = {&st->c, &st->e1, &st->w1, &st->s1, &st->n1};
+    for (int k = 0; k < 5; ++k) { q[k]->v = -q[k]->v; q[k]->by = -q[k]->by; q[k]->bz = -q[k]->bz; }
+}
+
 /* MHD reference-layout step: Cartesian form of add2 / Kernel_TVD update
  * (kernel.cu:2141-2195, 2619-2660 without the 1/y source terms). */
 static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u, const o_d3* b,
                         o_d2* s2, o_d2* u2, o_d3* b2, double* T, const double* T_do, int method)
 {
     if (method != 2 && method != 3) return -1;
-    if (g->axisym && !(g->y0 > 0.0)) return -1;
+    if (g->axisym && !axis_rows_ok(g)) return -1;
     src_t S = {g, 0, 1, s, u, b, NULL, order == 2};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
@@ -1356,12 +1378,15 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
     for (int j = 0; j < g->ny; ++j) {
         for (int i = 0; i < g->nx; ++i) {
             long k = (long)j * g->nx + i;
-            const double x = g->x0 + i * dx, y = g->y0 + j * dy;
+            const double x = g->x0 + i * dx, y = g->y0 + j * dy, y_signed = y;
             const int sphere = g->r_frozen > 0.0 && sqrt(x * x + y * y) < g->r_frozen;  /* kernel.cu:1919 */
             if (is_frozen(g, i, j) || sphere) { s2[k] = s[k]; u2[k] = u[k]; b2[k] = b[k]; continue; }
             stencil st;
             faces F;
             gather(&S, i, j, order, &st);
+            /* add2 only: Kernel_TVD divides by the signed y (kernel.cu:2203) */
+            const int mirror = order == 1 && g->axisym && y < 0.0;
+            if (mirror) mirror_stencil(&st);
             reconstruct(g, i, j, order, 1, &st, &F);
             mhd_sum m = mhd_cell_fluxes(g, method, &F, &st);
             if (tglob > m.tmin) tglob = m.tmin;
@@ -1373,7 +1398,8 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
             double skk = c->u * c->bx + c->v * c->by + 0.0 * c->bz;
             double r2, ux, uy, bx, by, bz, p2;
             if (g->axisym) {
-                /* axisymmetric sources, kernel.cu:2134-2190 (y = r > 0) */
+                /* axisymmetric sources, kernel.cu:2134-2190 */
+                const double y = mirror ? fabs(y_signed) : y_signed;
                 const double Pdiv = m.Pdiv + dV * c->by / y;
                 r2 = c->r - Tdo * m.PS.x / dV - Tdo * c->r * c->v / y;
                 if (r2 <= 0) { diag_inc(&g_diag.clamp_rho); r2 = c->r; }
@@ -1401,6 +1427,7 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
                       0.5 * r2 * (ux * ux + uy * uy + 0.0 * 0.0) - (bx * bx + by * by + bz * bz) / CPI8) *
                      (gam - 1.0);
             }
+            if (mirror) { uy = -uy; by = -by; bz = -bz; }
             if (p2 <= 0) { diag_inc(&g_diag.clamp_p); p2 = 0.000001; }
             s2[k].x = r2; s2[k].y = p2; u2[k].x = ux; u2[k].y = uy;
             b2[k].x = bx; b2[k].y = by; b2[k].z = bz;

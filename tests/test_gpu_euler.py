@@ -294,3 +294,25 @@ def test_kernel_tvd_old_options_small_grids_match_oracle(shape):
         assert np.max(np.abs(to_host(s2) - s2r) / np.abs(s2r)) <= 1e-12, method
         assert np.max(np.abs(to_host(u2) - u2r)) <= 1e-12, method
         assert T.item() == Tr or abs(T.item() - Tr) <= 1e-14 * Tr, method
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_kernel_tvd_old_rows_below_the_axis_match_oracle(method):
+    """Kernel_TVD_old's 1/y terms take the signed y (kernel.cu:3291,
+    3651-3662): a grid across the axis, one step against the oracle, 1e-12."""
+    say(f"kernel_tvd_old rows below the axis method={method}")
+    bc = (O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_TRANSMISSIVE, O.BC_TRANSMISSIVE)
+    h = 0.01
+    g, og = grids(140, 77, bc, gamma=1.4, krit=0.25, dx=h, dy=h, x0=h / 2, y0=-40.5 * h, axisym=True)
+    U = PB.four_quadrant(og, seed=13)
+    s, u, _ = O.cons_to_prim(og, O.PHYS_EULER, U)
+    T0, Tdo = 1e7, 2e-4
+    s2r, u2r, Tr = O.ref_euler_step("kernel_tvd_old", og, s, u, T0, Tdo, method)
+    sd, ud = to_dev(s), to_dev(u)
+    s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+    T = torch.tensor([T0], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([Tdo], dtype=torch.float64, device=dev())
+    api.kernel_tvd_old(g, sd, ud, s2, u2, T, T_do, method)
+    assert np.max(np.abs(to_host(s2) - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(to_host(u2) - u2r)) <= 1e-12
+    assert abs(T.item() - Tr) <= 1e-14 * Tr

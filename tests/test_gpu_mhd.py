@@ -328,3 +328,27 @@ def test_production_options_small_grids_match_oracle(shape, kind):
         assert np.max(np.abs(u2 - u2r)) <= 1e-12, method
         assert np.max(np.abs(b2 - b2r)) <= 1e-12, method
         assert abs(T - Tr) <= 1e-14 * Tr, method
+
+
+@pytest.mark.parametrize("kind", ["add2", "kernel_tvd"])
+@pytest.mark.parametrize("method", [2, 3])
+def test_rows_below_the_axis_match_oracle(kind, method):
+    """Axisymmetric steps on a grid across the axis (rows at y < 0): add2's
+    mirrored evaluation of those rows (kernel.cu:1885-1889, 2026-2057,
+    2177-2182) and Kernel_TVD's signed 1/y terms, one step against the
+    oracle, 1e-12; a row centred on the axis is rejected."""
+    from tests.test_oracle_wind import smooth_state
+    nx, ny, h = 131, 70, 0.05
+    say(f"rows below the axis {kind} method={method} {nx}x{ny}")
+    g, og = grids(nx, ny, (O.BC_TRANSMISSIVE,) * 4, gamma=G53, dx=h, dy=h, x0=h / 2, y0=-29.5 * h, axisym=True)
+    s, u, b = smooth_state(og, seed=method + 41)
+    T0, Tdo = 1e7, 2e-4
+    s2r, u2r, b2r, Tr = O.ref_mhd_step(kind, og, s, u, b, T0, Tdo, 0, method)
+    s2, u2, b2, T = _ref_step_dev(kind, g, s, u, b, T0, Tdo, method)
+    assert np.max(np.abs(s2 - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(u2 - u2r)) <= 1e-12
+    assert np.max(np.abs(b2 - b2r)) <= 1e-12
+    assert abs(T - Tr) <= 1e-14 * Tr
+    g0 = api.make_grid(nx, ny, gamma=G53, dx=h, dy=h, x0=h / 2, y0=-30 * h, axisym=True)
+    with pytest.raises(api.GD2Error):
+        _ref_step_dev(kind, g0, s, u, b, T0, Tdo, method)

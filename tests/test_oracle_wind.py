@@ -18,6 +18,11 @@ Kernel_TVD_old (axisymmetric source terms, inner sphere; kernel.cu:3303,
 * Frozen sphere: cells inside keep their state, every other cell is the
   step without the sphere bit for bit.
 * The quiescent outer state of the production run is a steady state.
+* Rows below the axis: add2 evaluates them in the mirrored frame
+  (kernel.cu:1885-1889, 2026-2057, 2177-2182), so its step on the mirror
+  image of a grid above the axis is the mirror image of that grid's step,
+  bit for bit (coordinates exact in binary); Kernel_TVD and Kernel_TVD_old
+  divide by the signed y, which keeps the same mirror relation to rounding.
 """
 import numpy as np
 import pytest
@@ -257,3 +262,88 @@ def test_euler_sphere_and_option_scope():
     # Ker_Dekard has neither option (kernel.cu:1736-1874)
     with pytest.raises(ValueError):
         O.ref_euler_step("ker_dekard", g, s, u, 1e7, 1e-3, 1)
+
+
+def _mirror_rows(g, s, u, b=None):
+    """The y-mirror image of a grid and its state: rows reversed, vy, By, Bz
+    negated, the grid reflected about y = 0."""
+    gm = O.make_grid(g.nx, g.ny, dx=g.dx, dy=g.dy, x0=g.x0, y0=-(g.y0 + (g.ny - 1) * g.dy), gamma=g.gamma,
+                     axisym=bool(g.axisym))
+    sm = np.ascontiguousarray(s[::-1])
+    um = np.ascontiguousarray(u[::-1] * (1.0, -1.0))
+    if b is None:
+        return gm, sm, um
+    return gm, sm, um, np.ascontiguousarray(b[::-1] * (1.0, -1.0, -1.0))
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_add2_rows_below_axis_mirror_rows_above(method):
+    h = 1.0 / 64.0
+    g = O.make_grid(11, 9, dx=h, dy=h, x0=h / 2, y0=h / 2 + 3 * h, gamma=G53, axisym=True)
+    s, u, b = smooth_state(g, seed=31 + method)
+    gm, sm, um, bm = _mirror_rows(g, s, u, b)
+    assert gm.y0 + (gm.ny - 1) * gm.dy < 0.0      # every row of the image is below the axis
+    s2, u2, b2, T = O.ref_mhd_step("add2", g, s, u, b, 1e7, 2e-4, 0, method)
+    s2m, u2m, b2m, Tm = O.ref_mhd_step("add2", gm, sm, um, bm, 1e7, 2e-4, 0, method)
+    np.testing.assert_array_equal(s2m, s2[::-1])
+    np.testing.assert_array_equal(u2m, u2[::-1] * (1.0, -1.0))
+    np.testing.assert_array_equal(b2m, b2[::-1] * (1.0, -1.0, -1.0))
+    assert Tm == T
+    # the 1/y sources act on both grids
+    g.axisym = 0
+    s2c, _, _, _ = O.ref_mhd_step("add2", g, s, u, b, 1e7, 2e-4, 0, method)
+    assert np.max(np.abs(s2c - s2)) > 1e-6
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_add2_grid_across_the_axis_is_mirror_symmetric(method):
+    """A state mirror-symmetric about y = 0 on a grid across the axis stays
+    so: the rows below are the mirrored evaluation of the rows above."""
+    h = 1.0 / 64.0
+    nx, half = 10, 6
+    g = O.make_grid(nx, 2 * half, dx=h, dy=h, x0=h / 2, y0=-(half - 0.5) * h, gamma=G53, axisym=True)
+    gu = O.make_grid(nx, half, dx=h, dy=h, x0=h / 2, y0=h / 2, gamma=G53)
+    s_up, u_up, b_up = smooth_state(gu, seed=5 + method)
+    s = np.ascontiguousarray(np.concatenate([s_up[::-1], s_up]))
+    u = np.ascontiguousarray(np.concatenate([u_up[::-1] * (1.0, -1.0), u_up]))
+    b = np.ascontiguousarray(np.concatenate([b_up[::-1] * (1.0, -1.0, -1.0), b_up]))
+    s2, u2, b2, _ = O.ref_mhd_step("add2", g, s, u, b, 1e7, 2e-4, 0, method)
+    np.testing.assert_array_equal(s2[:half], s2[half:][::-1])
+    np.testing.assert_array_equal(u2[:half], u2[half:][::-1] * (1.0, -1.0))
+    np.testing.assert_array_equal(b2[:half], b2[half:][::-1] * (1.0, -1.0, -1.0))
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_kernel_tvd_signed_y_mirror_rows_above(method):
+    h = 1.0 / 64.0
+    g = O.make_grid(12, 10, dx=h, dy=h, x0=h / 2, y0=h / 2 + 2 * h, gamma=G53, axisym=True)
+    s, u, b = smooth_state(g, seed=17 + method)
+    gm, sm, um, bm = _mirror_rows(g, s, u, b)
+    s2, u2, b2, T = O.ref_mhd_step("kernel_tvd", g, s, u, b, 1e7, 2e-4, 0, method)
+    s2m, u2m, b2m, Tm = O.ref_mhd_step("kernel_tvd", gm, sm, um, bm, 1e7, 2e-4, 0, method)
+    assert np.max(np.abs(s2m - s2[::-1])) <= 1e-13
+    assert np.max(np.abs(u2m - u2[::-1] * (1.0, -1.0))) <= 1e-13
+    assert np.max(np.abs(b2m - b2[::-1] * (1.0, -1.0, -1.0))) <= 1e-13
+    assert abs(Tm - T) <= 1e-14 * T
+
+
+def test_kernel_tvd_old_signed_y_mirror_rows_above():
+    h = 1.0 / 64.0
+    g = O.make_grid(12, 10, dx=h, dy=h, x0=h / 2, y0=h / 2 + 2 * h, gamma=1.4, axisym=True)
+    s, u, _ = smooth_state(g, seed=3)
+    gm, sm, um = _mirror_rows(g, s, u)
+    s2, u2, T = O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, 2e-4, 1)
+    s2m, u2m, Tm = O.ref_euler_step("kernel_tvd_old", gm, sm, um, 1e7, 2e-4, 1)
+    assert np.max(np.abs(s2m - s2[::-1])) <= 1e-13
+    assert np.max(np.abs(u2m - u2[::-1] * (1.0, -1.0))) <= 1e-13
+    assert abs(Tm - T) <= 1e-14 * T
+
+
+def test_axisymmetric_row_on_the_axis_is_rejected():
+    h = 0.125
+    g = O.make_grid(6, 5, dx=h, dy=h, y0=-2 * h, gamma=G53, axisym=True)   # row 2 at y = 0
+    s, u, b = smooth_state(g, seed=1)
+    with pytest.raises(ValueError):
+        O.ref_mhd_step("add2", g, s, u, b, 1e7, 1e-4, 0, 3)
+    with pytest.raises(ValueError):
+        O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, 1e-4, 1)
