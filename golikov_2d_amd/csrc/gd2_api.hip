@@ -85,16 +85,21 @@ static DeviceDiag* diag_ptr() {
 
 // prod: which production options of the reference-layout steps an entry
 // point takes: kProdSides (side kinds SYMMETRY / AXIS / OUTFLOW / INFLOW,
-// each on its own side: the MHD steps) and kProdOptions (the frozen sphere
-// and the axisymmetric sources: the MHD steps and Kernel_TVD_old);
-// everywhere else they are rejected rather than ignored.
-enum { kProdSides = 1, kProdOptions = 2, kProdAll = 3 };
+// each on its own side: the MHD steps), kProdOptions (the frozen sphere
+// and the axisymmetric sources: the MHD steps and Kernel_TVD_old) and
+// kProdWestInflow (the WEST_INFLOW side: Kernel_TVD_old); everywhere else
+// they are rejected rather than ignored.
+enum { kProdSides = 1, kProdOptions = 2, kProdAll = 3, kProdWestInflow = 4 };
 static bool grid_ok(const gd2_grid* g, bool allow_frozen, int prod = 0) {
     if (!g || g->nx < 1 || g->ny < 1) return false;
     if (!(g->dx > 0.0) || !(g->dy > 0.0) || !(g->gamma > 1.0)) return false;
     if ((long long)g->nx * g->ny > (1ll << 40)) return false;
     static const int own_side[4] = {GD2_BC_SYMMETRY, GD2_BC_OUTFLOW, GD2_BC_AXIS, GD2_BC_INFLOW};
     for (int s = 0; s < 4; ++s) {
+        if (g->bc[s] == GD2_BC_WEST_INFLOW) {
+            if (s != 0 || !(prod & kProdWestInflow) || !std::isfinite(g->m_inf) || !(g->m_inf > 0.0)) return false;
+            continue;
+        }
         if (g->bc[s] < GD2_BC_TRANSMISSIVE || g->bc[s] > GD2_BC_INFLOW) return false;
         if (g->bc[s] == GD2_BC_FROZEN && !allow_frozen) return false;
         if (g->bc[s] >= GD2_BC_SYMMETRY && (!(prod & kProdSides) || g->bc[s] != own_side[s])) return false;
@@ -216,7 +221,7 @@ static int dispatch_euler(int order, int flux, const EulerArgs& A, hipStream_t s
         if (flux == GD2_FLUX_EXACT) return launch_euler<1, GD2_FLUX_EXACT, LAYOUT, W, BASE>(A, st);
         if (flux == GD2_FLUX_HLL) return launch_euler<1, GD2_FLUX_HLL, LAYOUT, W, BASE>(A, st);
     }
-    if (is_hllc_variant(flux)) return launch_euler_variant(order, flux, LAYOUT, W, BASE, false, A, st);
+    if (is_hllc_variant(flux)) return launch_euler_variant(order, flux, LAYOUT, W, BASE, kRefPlain, A, st);
     return GD2_ERR_UNSUPPORTED;
 }
 
@@ -487,8 +492,10 @@ int gd2_wavespeed_max(const gd2_grid* g, const gd2_scheme* sc, const double* U, 
 
 static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const gd2_double2* u, gd2_double2* s2,
                      gd2_double2* u2, double* T, const double* T_do, int method, void* stream) {
-    // Kernel_TVD_old (order 2) takes the sphere and the 1/y terms, Ker_Dekard none
-    if (!grid_ok(g, true, order == 2 ? kProdOptions : 0) || !s || !u || !s2 || !u2 || !T || !T_do)
+    // Kernel_TVD_old (order 2) takes the sphere, the 1/y terms and its west
+    // ghost, Ker_Dekard none
+    if (!grid_ok(g, true, order == 2 ? kProdOptions | kProdWestInflow : 0) || !s || !u || !s2 || !u2 || !T ||
+        !T_do)
         return GD2_ERR_ARG;
     if ((const void*)s == (const void*)s2 || (const void*)u == (const void*)u2) return GD2_ERR_ARG;
     if (method < 0 || method > 4) return GD2_ERR_UNSUPPORTED;
@@ -506,18 +513,27 @@ static int ref_euler(const gd2_grid* g, int order, const gd2_double2* s, const g
     A.y0 = g->y0;
     A.r_frozen = g->r_frozen;
     A.axisym = g->axisym ? 1 : 0;
+    const bool west_in = g->bc[0] == GD2_BC_WEST_INFLOW;
+    A.p_inflow = west_in ? 1.0 / (g->gamma * g->m_inf * g->m_inf) : 0.0;  // kernel.cu:3326
     if (!A.diag) return GD2_ERR_HIP;
     // methods 3 and 4: the HLLC variants HLLC_Korolkov and HLLC_Korolkov_2D
     // (no reference kernel calls them)
     static const int method_flux[5] = {GD2_FLUX_HLL, GD2_FLUX_HLLC, GD2_FLUX_EXACT, GD2_FLUX_HLLC_KOROLKOV,
                                        GD2_FLUX_HLLC_2D};
     const int flux = method_flux[method];
+    if (order == 2 && west_in) {
+        const hipStream_t st = (hipStream_t)stream;
+        if (is_hllc_variant(flux)) return launch_euler_variant(2, flux, kAoS, false, false, kRefWestInflow, A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, kAoS, false, false, kRefWestInflow>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, kAoS, false, false, kRefWestInflow>(A, st);
+        return launch_euler<2, GD2_FLUX_HLL, kAoS, false, false, kRefWestInflow>(A, st);
+    }
     if (order == 2 && (A.axisym || A.r_frozen > 0.0)) {
         const hipStream_t st = (hipStream_t)stream;
-        if (is_hllc_variant(flux)) return launch_euler_variant(2, flux, kAoS, false, false, true, A, st);
-        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, kAoS, false, false, true>(A, st);
-        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, kAoS, false, false, true>(A, st);
-        return launch_euler<2, GD2_FLUX_HLL, kAoS, false, false, true>(A, st);
+        if (is_hllc_variant(flux)) return launch_euler_variant(2, flux, kAoS, false, false, kRefOptions, A, st);
+        if (flux == GD2_FLUX_HLLC) return launch_euler<2, GD2_FLUX_HLLC, kAoS, false, false, kRefOptions>(A, st);
+        if (flux == GD2_FLUX_EXACT) return launch_euler<2, GD2_FLUX_EXACT, kAoS, false, false, kRefOptions>(A, st);
+        return launch_euler<2, GD2_FLUX_HLL, kAoS, false, false, kRefOptions>(A, st);
     }
     return dispatch_euler<kAoS, false, false>(order, flux, A, (hipStream_t)stream);
 }

@@ -62,7 +62,24 @@ struct EulerArgs {
     // (kernel.cu:3303) and axisymmetric 1/y source terms (kernel.cu:3651-3662)
     double x0, y0, r_frozen;
     int axisym;
+    // pressure 1/(gamma M_inf^2) of Kernel_TVD_old's west ghost (kernel.cu:3326)
+    double p_inflow;
 };
+
+// Which of Kernel_TVD_old's own rules an order-2 kAoS instance applies: none,
+// the options (frozen sphere, 1/y terms), or the options and its west ghost.
+enum { kRefPlain = 0, kRefOptions = 1, kRefWestInflow = 2 };
+
+// Kernel_TVD_old's west ghost from the edge cell q (kernel.cu:3323-3337): its
+// density and vy, the pressure 1/(gamma M_inf^2), and vx = -5 where the edge
+// cell is subsonic with vx > -5. The Mach number is formed without
+// contraction, term by term as written there.
+__device__ __forceinline__ PrimE west_subsonic_rule(const EulerArgs& A, PrimE q) {
+#pragma clang fp contract(off)
+    const double mach = sqrt((q.u * q.u + q.v * q.v) / (A.gamma * q.p / q.r));
+    if ((q.u > -5.0) && (mach < 1.0)) q.u = -5.0;
+    return q;
+}
 
 struct Slope {
     double r, u, v, p;
@@ -294,7 +311,7 @@ __device__ __forceinline__ void clamp_wavespeed(const EulerArgs& A, double& o0, 
     if (count) wmax = fmax(wmax, w);
 }
 
-template <int LAYOUT, bool PROD>
+template <int LAYOUT, int PROD>
 __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
     if constexpr (LAYOUT == kAoS) {
         bool f = (A.bc[0] == GD2_BC_FROZEN && c == 0) || (A.bc[1] == GD2_BC_FROZEN && c == A.nx - 1) ||
@@ -312,8 +329,9 @@ __device__ __forceinline__ bool frozen_cell(const EulerArgs& A, int c, int j) {
 // One wave's strip: columns tx*63 .. tx*63+62, rows ty*rows .. +rows-1.
 // BASE: the output is a*Ubase + b*(stage result) (an SSP-RK combination);
 // otherwise it is the stage result itself. PROD (kAoS, order 2): Kernel_TVD_old's
-// inner sphere and axisymmetric sources.
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, bool PROD = false>
+// inner sphere and axisymmetric sources (kRefOptions), and its west ghost
+// (kRefWestInflow).
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, int PROD = kRefPlain>
 __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, TileLds& lds) {
     const int lane = threadIdx.x;
     const int cbase = tx * kTileW;
@@ -328,6 +346,8 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     const int hci = map_index(hcol, A.nx, A.bc[0], A.bc[1], hfx);
     const int hslot = lane == 0 ? 0 : (lane == 1 ? 1 : 66);
     const bool has_halo = lane < 3;
+    // the first strip of a west-inflow grid: lanes 0, 1 hold the ghost columns
+    const bool west_in = PROD == kRefWestInflow && tx == 0;
     const bool owns = (lane < kTileW) && (c < A.nx);
     constexpr unsigned kCellBytes = LAYOUT == kSoA ? 8u : 16u;
     const unsigned oc = (unsigned)ci * kCellBytes;   // own cell within a row
@@ -369,6 +389,10 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             }
             if (has_halo) {
                 RowVal<LAYOUT> h = finish_cell<LAYOUT>(A, pr.halo, hfx, pr.fy);
+                if (west_in && lane < 2) {
+                    h.q = west_subsonic_rule(A, h.q);
+                    h.q.p = A.p_inflow;
+                }
                 lds_put(lds.q, slot, hslot, h.q);
                 ok = ok && h.q.r > kPosMin && h.q.p > kPosMin;
             }
@@ -422,8 +446,22 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             qb.r = ring[sl][0][lane + 1]; qb.u = ring[sl][1][lane + 1]; qb.v = ring[sl][2][lane + 1]; qb.p = ring[sl][3][lane + 1];
             qd.r = ring[sl][0][lane + 3]; qd.u = ring[sl][1][lane + 3]; qd.v = ring[sl][2][lane + 3]; qd.p = ring[sl][3][lane + 3];
             const PrimE& qc = v0.q;
-            Slope sb = slope3<ORDER>(qa, qb, qc);
             Slope sc = slope3<ORDER>(qb, qc, qd);
+            // west inflow: the reference forms F(1/2) once per cell. The cell
+            // of column 0 sees column -1 as the ghost (lane 1's qa), the cell
+            // of column 1 sees it with column 0's pressure (kernel.cu:3365-3378):
+            // lane 1 forms the first for lane 0, then the second as its own
+            FluxE Fg;
+            double sFg = 0.0;
+            if constexpr (PROD == kRefWestInflow) {
+                if (west_in) {
+                    Slope sg = slope3<ORDER>(qa, qb, qc);
+                    PrimE L = face_plus(qb, sg), R = face_minus(qc, sc);
+                    Fg = face_flux<FLUX>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, sFg);
+                    if (lane == 1) qa.p = qb.p;
+                }
+            }
+            Slope sb = slope3<ORDER>(qa, qb, qc);
             if (xpos) {
                 PrimE L = face_plus<false>(qb, sb), R = face_minus<false>(qc, sc);
                 Fl = face_flux<FLUX, false>(A, L.r, L.u, L.v, L.p, R.r, R.u, R.v, R.p, smax);
@@ -439,6 +477,17 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
             Fr.ft = from_next_lane(Fl.ft);
             Fr.fe = from_next_lane(Fl.fe);
             if constexpr (LAYOUT == kAoS) sFr = from_next_lane(sFl);
+            if constexpr (PROD == kRefWestInflow) {
+                if (west_in) {
+                    const double gr = from_next_lane(Fg.fr), gn = from_next_lane(Fg.fn);
+                    const double gt = from_next_lane(Fg.ft), ge = from_next_lane(Fg.fe);
+                    const double gs = from_next_lane(sFg);
+                    if (lane == 0) {
+                        Fr.fr = gr; Fr.fn = gn; Fr.ft = gt; Fr.fe = ge;
+                        sFr = gs;
+                    }
+                }
+            }
             X = x_balance(Fl, Fr, dy);
         }
         // the update's inputs: row j's conserved values from LDS (stored when
@@ -587,7 +636,7 @@ __device__ __forceinline__ void euler_tile(const EulerArgs& A, int tx, int ty, T
     }
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, bool PROD = false>
+template <int ORDER, int FLUX, int LAYOUT, bool WANT_W, bool BASE, int PROD = kRefPlain>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((FLUX == GD2_FLUX_HLLC || FLUX == GD2_FLUX_HLLC_KOROLKOV || FLUX == GD2_FLUX_HLLC_2D) ? (LAYOUT == kAoS && ORDER == 2 ? 4 : 3) : 1))) void euler_tile_kernel(const EulerArgs A) {
     __shared__ TileLds lds;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);

@@ -16,7 +16,7 @@ inline unsigned euler_blocks(const EulerArgs& A) {
     return (unsigned)(A.tiles_x * ((A.ny + A.rows - 1) / A.rows));
 }
 
-template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE, bool PROD = false>
+template <int ORDER, int FLUX, int LAYOUT, bool W, bool BASE, int PROD = kRefPlain>
 int launch_euler(const EulerArgs& A0, hipStream_t st) {
     EulerArgs A = A0;
     const void* k = reinterpret_cast<const void*>(&euler_tile_kernel<ORDER, FLUX, LAYOUT, W, BASE, PROD>);
@@ -44,9 +44,9 @@ int launch_rk2(const EulerArgs& A0, hipStream_t st) {
 }
 
 // The stage and one-pass launches of the HLLC variants (GD2_FLUX_HLLC_KOROLKOV,
-// GD2_FLUX_HLLC_2D), compiled in gd2_variants.hip. GD2_ERR_UNSUPPORTED for
-// other fluxes or orders.
-int launch_euler_variant(int order, int flux, int layout, bool w, bool base, bool prod, const EulerArgs& A,
+// GD2_FLUX_HLLC_2D), compiled in gd2_variants.hip; prod: kRefPlain /
+// kRefOptions / kRefWestInflow. GD2_ERR_UNSUPPORTED for other fluxes or orders.
+int launch_euler_variant(int order, int flux, int layout, bool w, bool base, int prod, const EulerArgs& A,
                          hipStream_t st);
 int launch_rk2_variant(int order, int flux, bool keep, const EulerArgs& A, hipStream_t st);
 

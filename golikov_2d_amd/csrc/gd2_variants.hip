@@ -21,23 +21,26 @@
 namespace gd2 {
 
 template <int ORDER, int FLUX>
-static int euler_variant(int layout, bool w, bool base, bool prod, const EulerArgs& A, hipStream_t st) {
+static int euler_variant(int layout, bool w, bool base, int prod, const EulerArgs& A, hipStream_t st) {
     if (layout == kAoS) {
         if (w || base) return GD2_ERR_UNSUPPORTED;
-        if (prod) {
-            if constexpr (ORDER == 2) return launch_euler<2, FLUX, kAoS, false, false, true>(A, st);
+        if (prod != kRefPlain) {
+            if constexpr (ORDER == 2) {
+                if (prod == kRefWestInflow) return launch_euler<2, FLUX, kAoS, false, false, kRefWestInflow>(A, st);
+                return launch_euler<2, FLUX, kAoS, false, false, kRefOptions>(A, st);
+            }
             return GD2_ERR_UNSUPPORTED;
         }
         return launch_euler<ORDER, FLUX, kAoS, false, false>(A, st);
     }
-    if (prod) return GD2_ERR_UNSUPPORTED;
+    if (prod != kRefPlain) return GD2_ERR_UNSUPPORTED;
     if (w && base) return launch_euler<ORDER, FLUX, kSoA, true, true>(A, st);
     if (w) return launch_euler<ORDER, FLUX, kSoA, true, false>(A, st);
     if (base) return launch_euler<ORDER, FLUX, kSoA, false, true>(A, st);
     return launch_euler<ORDER, FLUX, kSoA, false, false>(A, st);
 }
 
-int launch_euler_variant(int order, int flux, int layout, bool w, bool base, bool prod, const EulerArgs& A,
+int launch_euler_variant(int order, int flux, int layout, bool w, bool base, int prod, const EulerArgs& A,
                          hipStream_t st) {
     if (flux == GD2_FLUX_HLLC_KOROLKOV) {
         if (order == 2) return euler_variant<2, GD2_FLUX_HLLC_KOROLKOV>(layout, w, base, prod, A, st);

@@ -65,10 +65,17 @@ enum gd2_status {
  *   OUTFLOW (east only): copy of the edge cell with vx raised to 0.01
  *     where it is negative;
  *   INFLOW (north only): rho = 1, p = 1/gamma, v = 0, and B of the edge
- *     cell for gd2_add2, B = 0 for gd2_kernel_tvd. */
+ *     cell for gd2_add2, B = 0 for gd2_kernel_tvd.
+ * WEST_INFLOW is the west ghost of Kernel_TVD_old (kernel.cu:3323-3337,
+ * 3365-3378), accepted by gd2_kernel_tvd_old on the west side only, with
+ * gd2_grid.m_inf > 0: the ghost column takes the edge cell's density and
+ * vy, the pressure 1/(gamma m_inf^2), and vx = -5 where the edge cell is
+ * subsonic with vx > -5 (else its vx). As in the reference, the second cell
+ * of a row sees that column with the edge cell's own pressure instead. */
 enum gd2_bc {
     GD2_BC_TRANSMISSIVE = 0, GD2_BC_PERIODIC = 1, GD2_BC_REFLECTIVE = 2, GD2_BC_FROZEN = 3,
-    GD2_BC_SYMMETRY = 4, GD2_BC_AXIS = 5, GD2_BC_OUTFLOW = 6, GD2_BC_INFLOW = 7
+    GD2_BC_SYMMETRY = 4, GD2_BC_AXIS = 5, GD2_BC_OUTFLOW = 6, GD2_BC_INFLOW = 7,
+    GD2_BC_WEST_INFLOW = 8
 };
 
 /* Grid and physics descriptor. The reference has all of these as
@@ -92,6 +99,9 @@ typedef struct gd2_grid {
      * the same step up to rounding); a row centred on y = 0 is GD2_ERR_ARG. */
     double r_frozen;
     int axisym;
+    /* Free-stream Mach number of the WEST_INFLOW side (M_inf, Header.h:24;
+     * 0 elsewhere). */
+    double m_inf;
 } gd2_grid;
 
 /* ----------------------------------------------------------------------
@@ -118,7 +128,9 @@ int gd2_ker_dekard(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u
 
 /* Euler, minmod MUSCL on primitives. Replaces Kernel_TVD_old
  * (kernel.cu:3286); method as for gd2_ker_dekard. Takes r_frozen and axisym
- * (its inner sphere and 1/y terms); its west inflow ghost is not offered. */
+ * (its inner sphere and 1/y terms) and its west ghost (side kind
+ * WEST_INFLOW with m_inf, above). Its own sides are WEST_INFLOW, FROZEN,
+ * REFLECTIVE, TRANSMISSIVE (west, east, south, north). */
 int gd2_kernel_tvd_old(const gd2_grid* g, const gd2_double2* s, const gd2_double2* u,
                        gd2_double2* s2, gd2_double2* u2, double* T, const double* T_do,
                        int method, void* stream);

@@ -1239,6 +1239,33 @@ static int is_frozen(const o_grid* g, int i, int j)
            (g->bc[2] == O_BC_FROZEN && j == 0) || (g->bc[3] == O_BC_FROZEN && j == g->ny - 1);
 }
 
+/* Kernel_TVD_old's subsonic rule for its west ghost (kernel.cu:3332-3336,
+ * 3372-3376): vx = -5 where the cell is subsonic and vx > -5. */
+static void west_subsonic_rule(const o_grid* g, prim* q)
+{
+    double Max = sqrt((q->u * q->u + q->v * q->v) / (g->gamma * q->p / q->r));
+    if ((q->u > -5.0) && (Max < 1)) q->u = -5.0;
+}
+
+/* Kernel_TVD_old's west side (kernel.cu:3323-3337, 3365-3378): the cell of
+ * column 0 sees both ghost columns as its own density and vy, the pressure
+ * 1/(ggg M_inf^2) and its vx under the subsonic rule; the cell of column 1
+ * sees column -1 as the cell of column 0 under that rule, pressure kept. */
+static void west_inflow(const o_grid* g, int i, stencil* st)
+{
+    if (i == 0) {
+        prim q = st->c;
+        west_subsonic_rule(g, &q);
+        q.p = 1.0 / (g->gamma * g->m_inf * g->m_inf);
+        st->w1 = q;
+        st->w2 = q;
+    } else if (i == 1) {
+        prim q = st->w1;
+        west_subsonic_rule(g, &q);
+        st->w2 = q;
+    }
+}
+
 /* Euler reference-layout step shared by Ker_Dekard (order 1) and
  * Kernel_TVD_old (order 2). Frozen edge cells (the reference's "hard"
  * boundary rows, kernel.cu:1752-1758) are copied. Kernel_TVD_old's inner
@@ -1250,6 +1277,10 @@ static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2*
     if (method < 0 || method > 4) return -1;
     if ((g->axisym || g->r_frozen != 0.0) && order != 2) return -1;
     if (g->axisym && !axis_rows_ok(g)) return -1;
+    const int west_in = g->bc[0] == O_BC_WEST_INFLOW;
+    if (west_in && (order != 2 || !(g->m_inf > 0.0) || !isfinite(g->m_inf))) return -1;
+    for (int k = 1; k < 4; ++k)
+        if (g->bc[k] == O_BC_WEST_INFLOW) return -1;
     src_t S = {g, 0, 0, s, u, NULL, NULL, 0};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;
@@ -1264,6 +1295,7 @@ static int euler_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2*
             stencil st;
             faces F;
             gather(&S, i, j, order, &st);
+            if (west_in) west_inflow(g, i, &st);
             reconstruct(g, i, j, order, 0, &st, &F);
             euler_sum E = euler_cell_fluxes(g, method, &F, 1);
             if (tglob > E.tmin) tglob = E.tmin;   /* kernel.cu:1840-1844 (exact min) */
@@ -1370,6 +1402,8 @@ static int mhd_ref_step(const o_grid* g, int order, const o_d2* s, const o_d2* u
 {
     if (method != 2 && method != 3) return -1;
     if (g->axisym && !axis_rows_ok(g)) return -1;
+    for (int k = 0; k < 4; ++k)
+        if (g->bc[k] == O_BC_WEST_INFLOW) return -1;
     src_t S = {g, 0, 1, s, u, b, NULL, order == 2};
     const double dx = g->dx, dy = g->dy, gam = g->gamma;
     const double Tdo = *T_do;

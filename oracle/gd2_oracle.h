@@ -22,7 +22,9 @@ typedef struct { double x, y, z; } o_d3;
 /* Side boundary kinds (same numbering as the product's gd2_bc). */
 enum { O_BC_TRANSMISSIVE = 0, O_BC_PERIODIC = 1, O_BC_REFLECTIVE = 2, O_BC_FROZEN = 3,
        /* the production ghost rules of add2 / Kernel_TVD (one side each) */
-       O_BC_SYMMETRY = 4, O_BC_AXIS = 5, O_BC_OUTFLOW = 6, O_BC_INFLOW = 7 };
+       O_BC_SYMMETRY = 4, O_BC_AXIS = 5, O_BC_OUTFLOW = 6, O_BC_INFLOW = 7,
+       /* Kernel_TVD_old's west ghost (kernel.cu:3323-3337, 3365-3378) */
+       O_BC_WEST_INFLOW = 8 };
 
 /* Runtime grid/physics descriptor: the reference has these as macros
  * (Header.h:3-16 gamma/krit, kernel.cu:11-20 N, M, dx, dy, x_min, y_min). */
@@ -35,6 +37,7 @@ typedef struct {
     int bc[4];           /* west, east, south, north */
     double r_frozen;     /* MHD reference steps: cells inside this radius keep their state */
     int axisym;          /* MHD reference steps: axisymmetric (y = r) source terms */
+    double m_inf;        /* M_inf (Header.h:24) of the WEST_INFLOW side */
 } o_grid;
 
 /* Scheme selection for the native (SoA conserved) path. */

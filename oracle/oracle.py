@@ -17,6 +17,7 @@ _LIB = None
 
 BC_TRANSMISSIVE, BC_PERIODIC, BC_REFLECTIVE, BC_FROZEN = 0, 1, 2, 3
 BC_SYMMETRY, BC_AXIS, BC_OUTFLOW, BC_INFLOW = 4, 5, 6, 7
+BC_WEST_INFLOW = 8
 PHYS_EULER, PHYS_MHD = 0, 1
 FLUX_HLLC, FLUX_EXACT, FLUX_HLLC_MHD, FLUX_HLLD, FLUX_HLL = 1, 2, 3, 4, 5
 FLUX_HLLC_KOROLKOV, FLUX_HLLC_2D = 6, 7
@@ -33,7 +34,8 @@ class D3(C.Structure):
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("x0", C.c_double), ("y0", C.c_double), ("gamma", C.c_double),
-                ("krit", C.c_double), ("bc", C.c_int * 4), ("r_frozen", C.c_double), ("axisym", C.c_int)]
+                ("krit", C.c_double), ("bc", C.c_int * 4), ("r_frozen", C.c_double), ("axisym", C.c_int),
+                ("m_inf", C.c_double)]
 
 
 class Scheme(C.Structure):
@@ -131,7 +133,7 @@ def lib():
 # ---------------------------------------------------------------- helpers
 
 def make_grid(nx, ny, *, dx=None, dy=None, x0=None, y0=None, gamma=1.4, krit=0.25,
-              bc=(BC_TRANSMISSIVE,) * 4, r_frozen=0.0, axisym=False) -> Grid:
+              bc=(BC_TRANSMISSIVE,) * 4, r_frozen=0.0, axisym=False, m_inf=0.0) -> Grid:
     dx = 1.0 / nx if dx is None else dx
     dy = 1.0 / ny if dy is None else dy
     x0 = dx / 2.0 if x0 is None else x0
@@ -141,6 +143,7 @@ def make_grid(nx, ny, *, dx=None, dy=None, x0=None, y0=None, gamma=1.4, krit=0.2
         g.bc[i] = bc[i]
     g.r_frozen = float(r_frozen)
     g.axisym = 1 if axisym else 0
+    g.m_inf = float(m_inf)
     return g
 
 

@@ -316,3 +316,86 @@ def test_kernel_tvd_old_rows_below_the_axis_match_oracle(method):
     assert np.max(np.abs(to_host(s2) - s2r) / np.abs(s2r)) <= 1e-12
     assert np.max(np.abs(to_host(u2) - u2r)) <= 1e-12
     assert abs(T.item() - Tr) <= 1e-14 * Tr
+
+
+def _west_inflow_rows(ny, nx, seed):
+    """Random state whose west edge cells are subsonic with vx > -5 (the
+    ghost's rule acts), supersonic, subsonic with vx <= -5, or supersonic at
+    their own pressure and subsonic at 1/(gamma M_inf^2) (M_inf 0.7, gamma 1.4)."""
+    rng = np.random.default_rng(seed)
+    s = np.stack([rng.uniform(0.6, 1.4, (ny, nx)), rng.uniform(0.6, 1.4, (ny, nx))], -1)
+    u = rng.uniform(-0.4, 0.4, (ny, nx, 2))
+    u[1::7, 0, 0] = 3.0
+    u[2::7, :2, 0] = -6.0
+    s[2::7, :2, 1] = 80.0
+    s[3::7, 0] = (1.0, 0.6)
+    u[3::7, 0] = (1.3, 0.0)
+    return np.ascontiguousarray(s), np.ascontiguousarray(u)
+
+
+def _tvd_old_dev(g, s, u, T0, Tdo, method):
+    sd, ud = to_dev(s), to_dev(u)
+    s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+    T = torch.tensor([T0], dtype=torch.float64, device=dev())
+    T_do = torch.tensor([Tdo], dtype=torch.float64, device=dev())
+    api.kernel_tvd_old(g, sd, ud, s2, u2, T, T_do, method)
+    return to_host(s2), to_host(u2), T.item()
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("options", [False, True])
+def test_kernel_tvd_old_west_inflow_matches_oracle(method, options):
+    """Kernel_TVD_old with the reference's own sides: the west inflow ghost
+    (kernel.cu:3323-3337, 3365-3378), frozen east column, axis (reflective)
+    south, transmissive north; with and without its sphere and 1/y terms.
+    One step against the oracle, 1e-12, over three wave strips."""
+    say(f"kernel_tvd_old west inflow method={method} options={options}")
+    bc = (O.BC_WEST_INFLOW, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    kw = dict(r_frozen=0.23, axisym=True) if options else {}
+    g, og = grids(157, 90, bc, gamma=1.4, krit=0.25, m_inf=0.7, **kw)
+    s, u = _west_inflow_rows(90, 157, seed=method)
+    T0, Tdo = 1e7, 2e-4
+    s2r, u2r, Tr = O.ref_euler_step("kernel_tvd_old", og, s, u, T0, Tdo, method)
+    s2, u2, T = _tvd_old_dev(g, s, u, T0, Tdo, method)
+    assert np.max(np.abs(s2 - s2r) / np.abs(s2r)) <= 1e-12
+    assert np.max(np.abs(u2 - u2r)) <= 1e-12
+    assert abs(T - Tr) <= 1e-14 * Tr
+    # the west side acts: the step differs from the transmissive one near it
+    gt, ogt = grids(157, 90, (O.BC_TRANSMISSIVE,) + bc[1:], gamma=1.4, krit=0.25, **kw)
+    s2t, _, _ = _tvd_old_dev(gt, s, u, T0, Tdo, method)
+    assert np.any(s2t[:, 0] != s2[:, 0]) and np.any(s2t[:, 1] != s2[:, 1])
+    np.testing.assert_array_equal(s2t[:, 2:], s2[:, 2:])
+
+
+@pytest.mark.parametrize("shape", [(1, 5), (2, 3), (3, 4), (64, 9), (66, 3)])
+def test_kernel_tvd_old_west_inflow_small_grids_match_oracle(shape):
+    """The west inflow ghost on grids one to a few cells wide (column 1 missing
+    or the east ghost), transmissive east."""
+    nx, ny = shape
+    say(f"kernel_tvd_old west inflow small grid {nx}x{ny}")
+    bc = (O.BC_WEST_INFLOW, O.BC_TRANSMISSIVE, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    g, og = grids(nx, ny, bc, gamma=1.4, dx=0.05, dy=0.05, m_inf=0.7)
+    s, u = _west_inflow_rows(ny, nx, seed=nx * 13 + ny)
+    for method in (0, 1, 2):
+        s2r, u2r, Tr = O.ref_euler_step("kernel_tvd_old", og, s, u, 1e7, 1e-4, method)
+        s2, u2, T = _tvd_old_dev(g, s, u, 1e7, 1e-4, method)
+        assert np.max(np.abs(s2 - s2r) / np.abs(s2r)) <= 1e-12, method
+        assert np.max(np.abs(u2 - u2r)) <= 1e-12, method
+        assert abs(T - Tr) <= 1e-14 * Tr, method
+
+
+def test_west_inflow_rejected_elsewhere():
+    """WEST_INFLOW: Kernel_TVD_old only, west side only, M_inf > 0."""
+    say("west inflow rejected elsewhere")
+    s, u = _west_inflow_rows(5, 6, seed=0)
+    ok = (O.BC_WEST_INFLOW, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    _tvd_old_dev(api.make_grid(6, 5, bc=ok, m_inf=0.5), s, u, 1e7, 1e-4, 1)
+    east = (O.BC_TRANSMISSIVE, O.BC_WEST_INFLOW, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    for g in (api.make_grid(6, 5, bc=ok, m_inf=0.0), api.make_grid(6, 5, bc=east, m_inf=0.5)):
+        with pytest.raises(api.GD2Error):
+            _tvd_old_dev(g, s, u, 1e7, 1e-4, 1)
+    sd, ud = to_dev(s), to_dev(u)
+    T = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    with pytest.raises(api.GD2Error):
+        api.ker_dekard(api.make_grid(6, 5, bc=ok, m_inf=0.5), sd, ud, torch.empty_like(sd), torch.empty_like(ud),
+                       T, T.clone(), 1)

@@ -347,3 +347,84 @@ def test_axisymmetric_row_on_the_axis_is_rejected():
         O.ref_mhd_step("add2", g, s, u, b, 1e7, 1e-4, 0, 3)
     with pytest.raises(ValueError):
         O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, 1e-4, 1)
+
+
+def _west_inflow_state(ny, nx, seed):
+    """Rows whose west edge cell is subsonic with vx > -5 (the rule acts),
+    supersonic (it does not), or subsonic with vx <= -5 (it does not)."""
+    rng = np.random.default_rng(seed)
+    s = np.stack([rng.uniform(0.6, 1.4, (ny, nx)), rng.uniform(0.6, 1.4, (ny, nx))], -1)
+    u = rng.uniform(-0.4, 0.4, (ny, nx, 2))
+    u[1, 0, 0] = 3.0                    # Mach > 1
+    u[2, :2, 0] = -6.0                  # vx <= -5 ...
+    s[2, :2, 1] = 80.0                  # ... and subsonic
+    s[3, 0] = (1.0, 0.6)                # Mach 1.42 at its own pressure, 0.91 at
+    u[3, 0] = (1.3, 0.0)                # 1/(gamma M_inf^2) with M_inf = 0.7, gamma = 1.4
+    return np.ascontiguousarray(s), np.ascontiguousarray(u)
+
+
+def _west_ghost(q_s, q_u, gamma, m_inf=None):
+    """The ghost state of Kernel_TVD_old's west side from the edge cells
+    (kernel.cu:3323-3337): vx = -5 where subsonic with vx > -5; with m_inf
+    the pressure 1/(gamma m_inf^2), else the edge cell's own (kernel.cu:3365-3378)."""
+    s, u = q_s.copy(), q_u.copy()
+    mach = np.sqrt((u[:, 0] ** 2 + u[:, 1] ** 2) / (gamma * s[:, 1] / s[:, 0]))
+    push = (u[:, 0] > -5.0) & (mach < 1)
+    u[push, 0] = -5.0
+    if m_inf is not None:
+        s[:, 1] = 1.0 / (gamma * m_inf * m_inf)
+    return s, u, push
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("axisym", [False, True])
+def test_kernel_tvd_old_west_inflow_equals_explicit_ghost_columns(method, axisym):
+    """Kernel_TVD_old's west ghost: column 0 and the columns from 2 on are the
+    interior of the step on a grid two columns wider whose extra columns hold
+    the ghost state; column 1 is that of a wider grid whose extra columns hold
+    the edge cell under the vx rule with its own pressure; bit for bit."""
+    nx, ny, h, gam, minf = 9, 7, 0.125, 1.4, 0.7
+    x0, y0 = 0.0625, 0.0625
+    bc = (O.BC_WEST_INFLOW, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    g = O.make_grid(nx, ny, dx=h, dy=h, x0=x0, y0=y0, gamma=gam, bc=bc, axisym=axisym, m_inf=minf)
+    s, u = _west_inflow_state(ny, nx, seed=method)
+    dt = 1e-3
+    s2, u2, _ = O.ref_euler_step("kernel_tvd_old", g, s, u, 1e7, dt, method)
+    wide_bc = (O.BC_TRANSMISSIVE, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    gw = O.make_grid(nx + 2, ny, dx=h, dy=h, x0=x0 - 2 * h, y0=y0, gamma=gam, bc=wide_bc, axisym=axisym)
+    outs = []
+    for m_inf in (minf, None):
+        gs, gu, push = _west_ghost(s[:, 0], u[:, 0], gam, m_inf)
+        S = np.ascontiguousarray(np.concatenate([gs[:, None], gs[:, None], s], axis=1))
+        U = np.ascontiguousarray(np.concatenate([gu[:, None], gu[:, None], u], axis=1))
+        S2, U2, _ = O.ref_euler_step("kernel_tvd_old", gw, S, U, 1e7, dt, method)
+        outs.append((S2[:, 2:], U2[:, 2:]))
+    assert push.any() and not push.all()
+    (Sa, Ua), (Sb, Ub) = outs
+    cols_a = [0] + list(range(2, nx))
+    np.testing.assert_array_equal(s2[:, cols_a], Sa[:, cols_a])
+    np.testing.assert_array_equal(u2[:, cols_a], Ua[:, cols_a])
+    np.testing.assert_array_equal(s2[:, 1], Sb[:, 1])
+    np.testing.assert_array_equal(u2[:, 1], Ub[:, 1])
+    # the two views of column -1 differ, and the side differs from a copy
+    assert np.any(Sa[:, 1] != Sb[:, 1])
+    gt = O.make_grid(nx, ny, dx=h, dy=h, x0=x0, y0=y0, gamma=gam, bc=wide_bc, axisym=axisym)
+    s2t, _, _ = O.ref_euler_step("kernel_tvd_old", gt, s, u, 1e7, dt, method)
+    assert np.any(s2t[:, 0] != s2[:, 0]) and np.any(s2t[:, 1] != s2[:, 1])
+
+
+def test_west_inflow_scope():
+    """Kernel_TVD_old only, west side only, M_inf > 0."""
+    nx, ny = 6, 5
+    s, u = _west_inflow_state(ny, nx, seed=0)
+    b = np.zeros((ny, nx, 3))
+    ok = (O.BC_WEST_INFLOW, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    O.ref_euler_step("kernel_tvd_old", O.make_grid(nx, ny, bc=ok, m_inf=0.5), s, u, 1e7, 1e-4, 1)
+    for bad in (dict(bc=ok, m_inf=0.0),
+                dict(bc=(O.BC_TRANSMISSIVE, O.BC_WEST_INFLOW, O.BC_TRANSMISSIVE, O.BC_TRANSMISSIVE), m_inf=0.5)):
+        with pytest.raises(ValueError):
+            O.ref_euler_step("kernel_tvd_old", O.make_grid(nx, ny, **bad), s, u, 1e7, 1e-4, 1)
+    with pytest.raises(ValueError):
+        O.ref_euler_step("ker_dekard", O.make_grid(nx, ny, bc=ok, m_inf=0.5), s, u, 1e7, 1e-4, 1)
+    with pytest.raises(ValueError):
+        O.ref_mhd_step("add2", O.make_grid(nx, ny, bc=ok, m_inf=0.5, gamma=G53), s, u, b, 1e7, 1e-4, 0, 3)
