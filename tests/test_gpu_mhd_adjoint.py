@@ -126,6 +126,30 @@ def test_mhd_stage_vjp_affine_combination_and_alias():
     assert np.abs(to_host(Ad) - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("shape", [(30, 22), (70, 41), (3, 5)])
+def test_mhd_stage_vjp_affine_term_that_is_v(shape):
+    """An affine term given as V itself (the rollout's two stage adjoints:
+    A = V, then B = V with W = A): W is bitwise that of the same call with a
+    copy of V."""
+    nx, ny = shape
+    say(f"mhd vjp affine term = V {nx}x{ny}")
+    g = api.make_grid(nx, ny, bc=(T, R, P, P))
+    for metod, order in ((3, 2), (2, 2), (3, 1)):
+        sc = scheme(metod, order)
+        U = random_mhd(g, 7 + metod + order)
+        rng = np.random.default_rng(11)
+        V, A = (rng.normal(size=U.shape) for _ in range(2))
+        Ud, Vd, Vc, Ad = to_dev(U), to_dev(V), to_dev(V), to_dev(A)
+        W1, W2 = torch.empty_like(Ud), torch.empty_like(Ud)
+        api.stage_vjp(g, sc, Ud, Vd, W1, c=0.3, A=Vd, a=0.5)
+        api.stage_vjp(g, sc, Ud, Vd, W2, c=0.3, A=Vc, a=0.5)
+        assert torch.equal(W1.view(torch.int64), W2.view(torch.int64)), (metod, order)
+        A1, A2 = Ad.clone(), Ad.clone()
+        api.stage_vjp(g, sc, Ud, Vd, A1, c=0.7, A=A1, a=0.5, B=Vd, b=1.0)
+        api.stage_vjp(g, sc, Ud, Vd, A2, c=0.7, A=A2, a=0.5, B=Vc, b=1.0)
+        assert torch.equal(A1.view(torch.int64), A2.view(torch.int64)), (metod, order)
+
+
 @pytest.mark.parametrize("bc", [(T, T, T, T), (P, P, P, P), (P, P, T, T)])
 @pytest.mark.parametrize("metod,order", [(3, 2), (2, 2), (3, 1)])
 def test_mhd_stage_vjp_matches_gpu_forward_fd(bc, metod, order):
