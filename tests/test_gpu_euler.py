@@ -399,3 +399,47 @@ def test_west_inflow_rejected_elsewhere():
     with pytest.raises(api.GD2Error):
         api.ker_dekard(api.make_grid(6, 5, bc=ok, m_inf=0.5), sd, ud, torch.empty_like(sd), torch.empty_like(ud),
                        T, T.clone(), 1)
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_kernel_tvd_old_reference_sides_30_steps_match_oracle(method):
+    """Kernel_TVD_old driven as the reference's step loop drives its steps
+    (lagged face time step, funk_time after each, ping-pong buffers) with its
+    own sides: west inflow ghost (M_inf 0.7), frozen east column, axis
+    south, transmissive north, inner sphere and 1/y terms; 30 steps of a
+    smooth subsonic state against the oracle, 1e-10."""
+    nx, ny = 190, 120
+    say(f"kernel_tvd_old reference sides 30 steps method={method}")
+    bc = (O.BC_WEST_INFLOW, O.BC_FROZEN, O.BC_REFLECTIVE, O.BC_TRANSMISSIVE)
+    g, og = grids(nx, ny, bc, gamma=1.4, krit=0.25, dx=1.0 / nx, dy=0.6 / ny, m_inf=0.7, r_frozen=0.12,
+                  axisym=True)
+    x = og.x0 + np.arange(nx) * og.dx
+    y = og.y0 + np.arange(ny) * og.dy
+    X, Y = np.meshgrid(x, y)
+    s = np.empty((ny, nx, 2))
+    u = np.empty((ny, nx, 2))
+    s[..., 0] = 1.0 + 0.2 * np.sin(6 * X) * np.cos(5 * Y)
+    s[..., 1] = 1.0 / (1.4 * 0.49) * (1 + 0.1 * np.cos(4 * X + 3 * Y))
+    u[..., 0] = -0.3 + 0.2 * np.sin(3 * Y)
+    u[..., 1] = 0.1 * np.sin(4 * X) * np.sin(3 * Y)
+    s, u = np.ascontiguousarray(s), np.ascontiguousarray(u)
+    hs, hu = s.copy(), u.copy()
+    T, Tdo, TT, it = 1e7, 1e-8, 0.0, 0
+    for _ in range(30):
+        hs, hu, T = O.ref_euler_step("kernel_tvd_old", og, hs, hu, T, Tdo, method)
+        T, Tdo, TT, it = O.funk_time(T, Tdo, TT, it)
+    sd, ud = to_dev(s), to_dev(u)
+    s2, u2 = torch.empty_like(sd), torch.empty_like(ud)
+    Td = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    Tdod = torch.tensor([1e-8], dtype=torch.float64, device=dev())
+    TTd = torch.zeros(1, dtype=torch.float64, device=dev())
+    itd = torch.zeros(1, dtype=torch.int32, device=dev())
+    for _ in range(30):
+        api.kernel_tvd_old(g, sd, ud, s2, u2, Td, Tdod, method)
+        sd, s2, ud, u2 = s2, sd, u2, ud
+        api.funk_time(Td, Tdod, TTd, itd)
+    gs, gu = to_host(sd), to_host(ud)
+    assert abs(TTd.item() - TT) <= 1e-12 * TT and itd.item() == it == 30
+    assert np.max(np.abs(gs - hs) / np.abs(hs)) <= 1e-10
+    assert np.max(np.abs(gu - hu)) <= 1e-10
+    assert np.all(gs[..., 0] > 0) and np.all(gs[..., 1] > 0)
