@@ -352,3 +352,38 @@ def test_rows_below_the_axis_match_oracle(kind, method):
     g0 = api.make_grid(nx, ny, gamma=G53, dx=h, dy=h, x0=h / 2, y0=-30 * h, axisym=True)
     with pytest.raises(api.GD2Error):
         _ref_step_dev(kind, g0, s, u, b, T0, Tdo, method)
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_rows_below_the_axis_20_steps_match_oracle(method):
+    """add2 then Kernel_TVD steps with funk_time, as the reference's loop
+    drives them, on a grid across the axis (the oracle evaluates add2's rows
+    below it in the mirrored frame, the kernels with the signed y): 20 steps
+    against the oracle, 1e-10."""
+    from tests.test_oracle_wind import smooth_state
+    nx, ny, h = 131, 70, 0.05
+    say(f"rows below the axis 20 steps method={method}")
+    g, og = grids(nx, ny, (O.BC_TRANSMISSIVE,) * 4, gamma=G53, dx=h, dy=h, x0=h / 2, y0=-29.5 * h, axisym=True)
+    s, u, b = smooth_state(og, seed=9)
+    hs, hu, hb = s.copy(), u.copy(), b.copy()
+    T, Tdo, TT, it = 1e7, 1e-8, 0.0, 0
+    for k in range(20):
+        kind = "add2" if k < 10 else "kernel_tvd"
+        hs, hu, hb, T = O.ref_mhd_step(kind, og, hs, hu, hb, T, Tdo, it, method)
+        T, Tdo, TT, it = O.funk_time(T, Tdo, TT, it)
+    sd, ud, bd = to_dev(s), to_dev(u), to_dev(b)
+    s2, u2, b2 = torch.empty_like(sd), torch.empty_like(ud), torch.empty_like(bd)
+    Td = torch.tensor([1e7], dtype=torch.float64, device=dev())
+    Tdod = torch.tensor([1e-8], dtype=torch.float64, device=dev())
+    TTd = torch.zeros(1, dtype=torch.float64, device=dev())
+    itd = torch.zeros(1, dtype=torch.int32, device=dev())
+    for k in range(20):
+        fn = api.add2 if k < 10 else api.kernel_tvd
+        fn(g, sd, ud, bd, s2, u2, b2, Td, Tdod, 0, method)
+        sd, s2, ud, u2, bd, b2 = s2, sd, u2, ud, b2, bd
+        api.funk_time(Td, Tdod, TTd, itd)
+    gs, gu, gb = to_host(sd), to_host(ud), to_host(bd)
+    assert abs(TTd.item() - TT) <= 1e-12 * TT and itd.item() == it
+    assert np.max(np.abs(gs - hs) / np.abs(hs)) <= 1e-10
+    assert np.max(np.abs(gu - hu)) <= 1e-10
+    assert np.max(np.abs(gb - hb)) <= 1e-10
