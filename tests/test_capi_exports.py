@@ -145,3 +145,36 @@ def test_production_driver_missing_restart_file(tmp_path):
     r = subprocess.run([exe, "--nx", "8", "--ny", "8", "--restart", str(tmp_path / "missing.txt"),
                         "--out", str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "gd2_read_state_text" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_ctypes_structs_match_the_headers(tmp_path):
+    """The ctypes mirrors of gd2_grid / gd2_scheme (golikov_2d_amd.api) and of
+    o_grid / o_scheme (the oracle's) have the C headers' sizes and field
+    offsets, so no call reads past or beside what Python passes."""
+    import shutil
+    import subprocess
+    import ctypes as C
+    from oracle import oracle as O
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = [("gd2_grid", api.Grid), ("gd2_scheme", api.Scheme), ("o_grid", O.Grid), ("o_scheme", O.Scheme)]
+    lines = []
+    for cname, py in structs:
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"gd2.h\"\n#include \"gd2_oracle.h\"\n"
+                   "int main(void) {\n" + "\n".join(lines) + "\n    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-I",
+                    os.path.join(ROOT, "oracle"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        *key, val = line.split()
+        got[tuple(key)] = int(val)
+    for cname, py in structs:
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
