@@ -11,7 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 SO = os.path.join(HERE, "libgd2.so")
-SOURCES = ["gd2_api.hip", "gd2_variants.hip", "gd2_io.cpp"]
+SOURCES = ["gd2_api.hip", "gd2_variants.hip", "gd2_mhd_vjp.hip", "gd2_io.cpp"]
+# per-source code-generation options: the MHD stage VJP kernels take the
+# machine scheduler's ILP strategy (measured faster there, slower on the
+# Euler kernels)
+SOURCE_FLAGS = {"gd2_mhd_vjp.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -88,8 +92,8 @@ def build(verbose: bool = False, extra=()) -> str:
     for src in SOURCES:
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmds.append([HIPCC, *compile_flags, *extra, "-I", os.path.join(ROOT, "include"), "-c",
-                     os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *compile_flags, *SOURCE_FLAGS.get(src, []), *extra, "-I", os.path.join(ROOT, "include"),
+                     "-c", os.path.join(CSRC, src), "-o", obj])
 
     def run(cmd):
         if verbose:

@@ -677,7 +677,7 @@ __device__ __forceinline__ bool band_first(int b, int n) {
     return true;
 }
 
-__global__ void adj_fold_kernel(const AdjArgs A) {
+static __global__ void adj_fold_kernel(const AdjArgs A) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = A.nx, ny = A.ny;
     int col, row;

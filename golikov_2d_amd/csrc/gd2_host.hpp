@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/gd2.h"
+
 namespace gd2 {
 
 // Rows per wave strip of the stage launches (GD2_TILE_ROWS overrides).
@@ -13,5 +15,10 @@ int tile_rows_for(int nx, int ny);
 int strip_rows(const void* kernel, int tiles_x, int ny, int overhead, int max_rows = 64, int min_rows = 4);
 // GD2_OK, or GD2_ERR_HIP after recording the code for gd2_last_hip_error.
 int hip_error_code(hipError_t e);
+// The MHD stage VJP W = a A + b B + c J_L(U)^T V (HLLC-MHD or HLLD, order 1
+// or 2) with its workspace (defined in gd2_mhd_vjp.hip); arguments checked
+// by the caller.
+int mhd_stage_vjp(const gd2_grid* g, const gd2_scheme* sc, const double* U, const double* V, const double* A,
+                  double a, const double* B, double b, double c, double* W, void* workspace, hipStream_t st);
 
 }  // namespace gd2
